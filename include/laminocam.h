@@ -1,0 +1,126 @@
+/*
+ * laminocam.h - C ABI of the MI355X (gfx950) LaminoCam reconstruction hot path.
+ *
+ * Plain C: opaque handles, pointers and sizes, int status codes (0 = OK) and a
+ * thread-local message from lc_last_error().  No C++ exception crosses this
+ * boundary.  Arrays are row major float32; complex arrays interleave (re, im).
+ * Unless LC_DEVICE_PTRS is set, array arguments are host pointers and every
+ * call completes before it returns.
+ *
+ * Each entry point names the reference interface it stands in for
+ * (paths relative to the reference repository).
+ */
+#ifndef LAMINOCAM_H
+#define LAMINOCAM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LC_ABI_VERSION 1
+
+/* status codes */
+#define LC_OK 0
+#define LC_EINVAL 1
+#define LC_EHIP 2
+#define LC_ENOMEM 3
+#define LC_EUNSUP 4
+#define LC_EINTERNAL 5
+
+/* flags */
+#define LC_DEVICE_PTRS 0x1u  /* array arguments are device pointers */
+#define LC_GRAPH 0x2u        /* replay solver iterations from a captured hipGraph */
+#define LC_X0_ZERO 0x4u      /* the initial guess is zero (skip A x0) */
+#define LC_TV_ZERO_PAD 0x8u  /* TV differences zero padded (GPU reference) instead
+                                of interior only (CPU reference) */
+#define LC_TV_ANISO 0x10u    /* per-component soft threshold instead of isotropic */
+
+typedef struct lc_ctx_s *lc_ctx;
+typedef struct lc_geom2d_s *lc_geom2d;
+typedef struct lc_batch2d_s *lc_batch2d;
+
+typedef struct {
+    int64_t iters;  /* CG iterations run (last inner solve for Split-Bregman) */
+    int64_t outer;  /* Split-Bregman outer iterations run */
+    double res;     /* ||r|| / (||y|| + 1e-8) after the last CG iteration */
+    double dx;      /* CG: ||alpha p||/(||x||+1e-8); SB: ||x-x_old||/(||x_old||+1e-8) */
+    int32_t stop;   /* 0 iteration cap, 1 tol, 2 xtol, 3 |pAp| < 1e-10 */
+    int32_t pad;
+} lc_report;
+
+int lc_abi_version(void);
+const char *lc_last_error(void);
+
+/* ---- context: one device, one stream ---- */
+int lc_ctx_create(int device, lc_ctx *out);
+int lc_ctx_destroy(lc_ctx ctx);
+int lc_ctx_sync(lc_ctx ctx);
+
+/* ---- 2-D geometry + Toeplitz PSF ----
+ * Replaces cpu::PolarGrid2D(theta, ncols) (include/polar_grid2d.h:50-65) and
+ * cpu::PointSpreadFunction2D(grid, image_size) (include/toeplitz2d.h:81-99).
+ * The PSF is computed in FP64 on the GPU from its closed form and stored as
+ * an FP32 spectrum.  theta in radians. */
+int lc_geom2d_create(lc_ctx ctx, const double *theta, int64_t nang, int64_t nrad,
+                     int64_t image_n, uint32_t flags, lc_geom2d *out);
+int lc_geom2d_destroy(lc_geom2d g);
+/* info[0..4] = nang, nrad, N, psf_size M, M/2+1 */
+int lc_geom2d_info(lc_geom2d g, int64_t *info);
+/* PointSpreadFunction2D::kernel_hat() (include/toeplitz2d.h:63-66):
+ * complex {M, M/2+1} written to out (host) */
+int lc_psf2d_kernel_hat(lc_ctx ctx, lc_geom2d g, float *out);
+
+/* ---- 2-D operators on a stack x of nslices N x N slices ----
+ * sysmat2d(x, PointSpreadFunction2D) (src/gradient.cpp:76-79,
+ * include/toeplitz2d.h:101-125) */
+int lc_sysmat2d_psf(lc_ctx ctx, lc_geom2d g, const float *x, int64_t nslices, float *y,
+                    uint32_t flags);
+
+/* ---- solvers on one stack (global alpha/beta over the stack, as the
+ * reference's cgsolver on an Array{n1,N,N}) ----
+ * opt::cgsolver(A = sysmat2d(., psf), y, x0, max_iter, tol, xtol)
+ * (src/conjgrad.cpp:33-76).  x holds x0 on entry and the solution on exit. */
+int lc_cg2d(lc_ctx ctx, lc_geom2d g, const float *rhs, float *x, int64_t nslices,
+            int64_t max_iter, double tol, double xtol, uint32_t flags, lc_report *rep);
+/* opt::split_bregman (src/bregman.cpp:36-94) with A = sysmat2d(., psf) */
+int lc_split_bregman2d(lc_ctx ctx, lc_geom2d g, const float *rhs, float *x, int64_t nslices,
+                       double lambda, double mu, int64_t outer, int64_t inner, double tol,
+                       double xtol, uint32_t flags, lc_report *rep);
+
+/* ---- finite differences on an n1 x n2 x n3 volume ----
+ * opt::grad_u / divergence / laplacian (src/finitediff.cpp:26-90,
+ * include/bregman.h:34-55).  g holds the 3 components back to back in the
+ * order (slices, rows, cols). */
+int lc_grad(lc_ctx ctx, const float *u, int64_t n1, int64_t n2, int64_t n3, float *g,
+            uint32_t flags);
+int lc_divergence(lc_ctx ctx, const float *g, int64_t n1, int64_t n2, int64_t n3, float *out,
+                  uint32_t flags);
+int lc_laplacian(lc_ctx ctx, const float *u, int64_t n1, int64_t n2, int64_t n3, float *out,
+                 uint32_t flags);
+
+/* ---- batches of independent 2-D problems (per-problem alpha/beta) ----
+ * No reference counterpart: gpu::MBIR2D (include/gpu/tomocam.h:60-75) solves
+ * one stack at a time.  Problem i is a stack of ns[i] slices on geometry
+ * geoms[i]; buffers stay resident on the device between calls. */
+int lc_batch2d_create(lc_ctx ctx, int32_t nprob, const lc_geom2d *geoms, const int32_t *ns,
+                      lc_batch2d *out);
+int lc_batch2d_destroy(lc_batch2d b);
+/* which: 0 = x (solution / initial guess), 1 = y (right-hand side A^T y) */
+int lc_batch2d_set(lc_batch2d b, int32_t prob, int32_t which, const float *src, uint32_t flags);
+int lc_batch2d_get(lc_batch2d b, int32_t prob, int32_t which, float *dst, uint32_t flags);
+/* device address of a problem's slot in buffer `which` */
+int lc_batch2d_device_ptr(lc_batch2d b, int32_t prob, int32_t which, float **out);
+int lc_batch2d_cg(lc_batch2d b, const int32_t *max_iter, double tol, double xtol,
+                  uint32_t flags);
+int lc_batch2d_split_bregman(lc_batch2d b, const float *lambda, const float *mu, int64_t outer,
+                             int64_t inner, double tol, double xtol, uint32_t flags);
+int lc_batch2d_reports(lc_batch2d b, lc_report *reps);
+/* y = A x over every problem of the batch (buffers 0 -> 1 unless stated) */
+int lc_batch2d_sysmat(lc_batch2d b, int32_t src, int32_t dst);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,297 @@
+"""laminocam_amd - MI355X (gfx950) LaminoCam reconstruction hot path.
+
+Python mirror of the reference's C++ API over the C ABI of liblaminocam.so
+(include/laminocam.h).  Names follow the reference:
+
+  PolarGrid2D            include/polar_grid2d.h:35-66
+  PointSpreadFunction2D  include/toeplitz2d.h:40-126
+  sysmat2d               src/gradient.cpp:49-79
+  cgsolver               src/conjgrad.cpp:33-76
+  split_bregman          src/bregman.cpp:36-94
+  grad_u / divergence / laplacian   src/finitediff.cpp:26-90, include/bregman.h:52-55
+
+All compute runs in HIP kernels; nothing here falls back to the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import (LC_DEVICE_PTRS, LC_GRAPH, LC_TV_ANISO, LC_TV_ZERO_PAD, LC_X0_ZERO,
+                   LcError, LcReport)
+
+__all__ = ["Context", "PolarGrid2D", "PointSpreadFunction2D", "sysmat2d", "cgsolver",
+           "split_bregman", "grad_u", "divergence", "laplacian", "Batch2D", "LcError",
+           "default_context"]
+
+_default_ctx = None
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+class Context:
+    """One GPU and one HIP stream."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _lib.load()
+        h = C.c_void_p()
+        _lib.check(self.lib.lc_ctx_create(device, C.byref(h)))
+        self.handle = h
+
+    def sync(self):
+        _lib.check(self.lib.lc_ctx_sync(self.handle))
+
+    def close(self):
+        if self.handle:
+            self.lib.lc_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class PolarGrid2D:
+    """Radial polar grid: x = r cos(theta), y = r sin(theta),
+    r_k = (k + 1/2) 2 pi / ncols - pi (include/polar_grid2d.h:50-65).
+    The GPU generates the points on the fly; only theta and ncols are kept."""
+
+    def __init__(self, theta, ncols: int):
+        self.theta = np.ascontiguousarray(theta, dtype=np.float64)
+        self.ncols = int(ncols)
+
+    def nangles(self):
+        return len(self.theta)
+
+    def nradial(self):
+        return self.ncols
+
+    def dims(self):
+        return (1, len(self.theta), self.ncols)
+
+
+class PointSpreadFunction2D:
+    """Toeplitz kernel of A^T A for a 2-D polar grid, shared by every slice
+    (include/toeplitz2d.h:81-125).  Built in FP64 on the GPU."""
+
+    def __init__(self, grid: PolarGrid2D, image_size: int, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.lib = self.ctx.lib
+        self.grid = grid
+        self.image_size = int(image_size)
+        h = C.c_void_p()
+        th = grid.theta
+        _lib.check(self.lib.lc_geom2d_create(
+            self.ctx.handle, th.ctypes.data_as(C.POINTER(C.c_double)), len(th), grid.ncols,
+            self.image_size, 0, C.byref(h)))
+        self.handle = h
+        info = (C.c_int64 * 5)()
+        _lib.check(self.lib.lc_geom2d_info(h, info))
+        self.info = tuple(info)
+
+    def psf_size(self):
+        return self.info[3]
+
+    def kernel_hat(self):
+        M, H = self.info[3], self.info[4]
+        out = np.empty((1, M, H), np.complex64)
+        _lib.check(self.lib.lc_psf2d_kernel_hat(
+            self.ctx.handle, self.handle, out.view(np.float32).ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def convolve(self, x):
+        x = _f32(x)
+        N = self.image_size
+        if x.shape[-1] != N or x.shape[-2] != N:
+            raise ValueError(f"convolve: slices must be {N}x{N}, got {x.shape}")
+        ns = x.size // (N * N)
+        out = np.empty_like(x)
+        _lib.check(self.lib.lc_sysmat2d_psf(self.ctx.handle, self.handle, _ptr(x), ns,
+                                            _ptr(out), 0))
+        return out
+
+    def close(self):
+        if self.handle:
+            self.lib.lc_geom2d_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def sysmat2d(x, psf: PointSpreadFunction2D):
+    """sysmat2d(x, psf) = psf.convolve(x) (src/gradient.cpp:76-79)"""
+    return psf.convolve(x)
+
+
+def _stack(x, N):
+    x = _f32(x)
+    return x, x.size // (N * N)
+
+
+def cgsolver(psf: PointSpreadFunction2D, y, x0, max_iter: int, tol: float, xtol: float,
+             graph: bool = False):
+    """opt::cgsolver with A = sysmat2d(., psf) on one stack (global alpha/beta
+    over every slice, as the reference).  Returns (x, report dict)."""
+    y, ns = _stack(y, psf.image_size)
+    x = _f32(x0).copy()
+    rep = LcReport()
+    flags = LC_GRAPH if graph else 0
+    if not np.any(x):
+        flags |= LC_X0_ZERO
+    _lib.check(psf.lib.lc_cg2d(psf.ctx.handle, psf.handle, _ptr(y), _ptr(x), ns, int(max_iter),
+                               float(tol), float(xtol), flags, C.byref(rep)))
+    return x, rep.as_dict()
+
+
+def split_bregman(psf: PointSpreadFunction2D, y, x0, lam: float, mu: float, outer_max: int,
+                  inner_max: int, tol: float, xtol: float, zero_pad: bool = False,
+                  aniso: bool = False, graph: bool = False):
+    """opt::split_bregman with A = sysmat2d(., psf).  zero_pad selects the
+    zero-padded finite differences of the reference's GPU path instead of
+    the interior-only ones of its CPU path."""
+    y, ns = _stack(y, psf.image_size)
+    x = _f32(x0).copy()
+    rep = LcReport()
+    flags = (LC_GRAPH if graph else 0) | (LC_TV_ZERO_PAD if zero_pad else 0) | \
+        (LC_TV_ANISO if aniso else 0)
+    _lib.check(psf.lib.lc_split_bregman2d(psf.ctx.handle, psf.handle, _ptr(y), _ptr(x), ns,
+                                          float(lam), float(mu), int(outer_max), int(inner_max),
+                                          float(tol), float(xtol), flags, C.byref(rep)))
+    return x, rep.as_dict()
+
+
+def _vol3(u):
+    u = _f32(u)
+    if u.ndim == 2:
+        return u, (1,) + u.shape
+    if u.ndim != 3:
+        raise ValueError("expected a 2-D or 3-D array")
+    return u, u.shape
+
+
+def grad_u(u, zero_pad: bool = False, ctx: Context | None = None):
+    ctx = ctx or default_context()
+    u, d = _vol3(u)
+    g = np.empty((3,) + d, np.float32)
+    _lib.check(ctx.lib.lc_grad(ctx.handle, _ptr(u), *d, _ptr(g), LC_TV_ZERO_PAD if zero_pad else 0))
+    return g
+
+
+def divergence(g, zero_pad: bool = False, ctx: Context | None = None):
+    ctx = ctx or default_context()
+    g = _f32(g)
+    d = g.shape[1:]
+    if len(d) == 2:
+        d = (1,) + d
+    out = np.empty(d, np.float32)
+    _lib.check(ctx.lib.lc_divergence(ctx.handle, _ptr(g), *d, _ptr(out),
+                                     LC_TV_ZERO_PAD if zero_pad else 0))
+    return out
+
+
+def laplacian(u, zero_pad: bool = False, ctx: Context | None = None):
+    ctx = ctx or default_context()
+    u, d = _vol3(u)
+    out = np.empty(d, np.float32)
+    _lib.check(ctx.lib.lc_laplacian(ctx.handle, _ptr(u), *d, _ptr(out),
+                                    LC_TV_ZERO_PAD if zero_pad else 0))
+    return out
+
+
+class Batch2D:
+    """Many independent 2-D problems solved together with per-problem CG
+    scalars; vectors stay resident on the GPU."""
+
+    X, Y, AX = 0, 1, 2
+
+    def __init__(self, psfs, ns=None, ctx: Context | None = None):
+        self.ctx = ctx or psfs[0].ctx
+        self.lib = self.ctx.lib
+        self.psfs = list(psfs)
+        n = len(self.psfs)
+        self.ns = [1] * n if ns is None else [int(v) for v in ns]
+        geoms = (C.c_void_p * n)(*[p.handle.value for p in self.psfs])
+        nss = (C.c_int32 * n)(*self.ns)
+        h = C.c_void_p()
+        _lib.check(self.lib.lc_batch2d_create(self.ctx.handle, n, geoms, nss, C.byref(h)))
+        self.handle = h
+
+    def __len__(self):
+        return len(self.psfs)
+
+    def shape(self, i):
+        N = self.psfs[i].image_size
+        return (self.ns[i], N, N)
+
+    def set(self, i, which, arr):
+        a = _f32(arr)
+        if a.size != int(np.prod(self.shape(i))):
+            raise ValueError("size mismatch")
+        _lib.check(self.lib.lc_batch2d_set(self.handle, i, which, _ptr(a), 0))
+
+    def get(self, i, which):
+        out = np.empty(self.shape(i), np.float32)
+        _lib.check(self.lib.lc_batch2d_get(self.handle, i, which, _ptr(out), 0))
+        return out
+
+    def sysmat(self, src=0, dst=2):
+        _lib.check(self.lib.lc_batch2d_sysmat(self.handle, src, dst))
+
+    def cg(self, max_iter, tol=1e-5, xtol=1e-5, graph=True, x0_zero=True):
+        n = len(self)
+        mi = [int(max_iter)] * n if np.isscalar(max_iter) else [int(v) for v in max_iter]
+        arr = (C.c_int32 * n)(*mi)
+        flags = (LC_GRAPH if graph else 0) | (LC_X0_ZERO if x0_zero else 0)
+        _lib.check(self.lib.lc_batch2d_cg(self.handle, arr, float(tol), float(xtol), flags))
+
+    def split_bregman(self, lam, mu, outer, inner, tol=1e-5, xtol=1e-5, zero_pad=False,
+                      aniso=False, graph=True):
+        n = len(self)
+        la = np.ascontiguousarray(np.broadcast_to(np.float32(lam), (n,)) if np.isscalar(lam) else lam,
+                                  dtype=np.float32)
+        m = np.ascontiguousarray(np.broadcast_to(np.float32(mu), (n,)) if np.isscalar(mu) else mu,
+                                 dtype=np.float32)
+        flags = (LC_GRAPH if graph else 0) | (LC_TV_ZERO_PAD if zero_pad else 0) | \
+            (LC_TV_ANISO if aniso else 0)
+        _lib.check(self.lib.lc_batch2d_split_bregman(
+            self.handle, la.ctypes.data_as(C.POINTER(C.c_float)),
+            m.ctypes.data_as(C.POINTER(C.c_float)), int(outer), int(inner), float(tol),
+            float(xtol), flags))
+
+    def reports(self):
+        n = len(self)
+        reps = (LcReport * n)()
+        _lib.check(self.lib.lc_batch2d_reports(self.handle, reps))
+        return [r.as_dict() for r in reps]
+
+    def close(self):
+        if self.handle:
+            self.lib.lc_batch2d_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,103 @@
+"""ctypes binding of liblaminocam.so (the gfx950 HIP library).
+
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "liblaminocam.so")
+
+LC_DEVICE_PTRS = 0x1
+LC_GRAPH = 0x2
+LC_X0_ZERO = 0x4
+LC_TV_ZERO_PAD = 0x8
+LC_TV_ANISO = 0x10
+
+# every symbol declared in include/laminocam.h
+EXPORTS = [
+    "lc_abi_version", "lc_last_error", "lc_ctx_create", "lc_ctx_destroy", "lc_ctx_sync",
+    "lc_geom2d_create", "lc_geom2d_destroy", "lc_geom2d_info", "lc_psf2d_kernel_hat",
+    "lc_sysmat2d_psf", "lc_cg2d", "lc_split_bregman2d", "lc_grad", "lc_divergence",
+    "lc_laplacian", "lc_batch2d_create", "lc_batch2d_destroy", "lc_batch2d_set",
+    "lc_batch2d_get", "lc_batch2d_device_ptr", "lc_batch2d_cg", "lc_batch2d_split_bregman",
+    "lc_batch2d_reports", "lc_batch2d_sysmat",
+]
+
+
+class LcReport(C.Structure):
+    _fields_ = [("iters", C.c_int64), ("outer", C.c_int64), ("res", C.c_double),
+                ("dx", C.c_double), ("stop", C.c_int32), ("pad", C.c_int32)]
+
+    def as_dict(self):
+        return {"iters": self.iters, "outer": self.outer, "res": self.res,
+                "dx": self.dx, "stop": self.stop}
+
+
+class LcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"laminocam error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Load liblaminocam.so (build it with laminocam_amd.build.build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build the HIP library first "
+            "(python -c 'import __graft_entry__ as g; g.build()')")
+    L = C.CDLL(LIB_PATH)
+    f32p = C.POINTER(C.c_float)
+    f64p = C.POINTER(C.c_double)
+    vp = C.c_void_p
+    i64, i32, u32 = C.c_int64, C.c_int32, C.c_uint32
+    sig = {
+        "lc_abi_version": (C.c_int, []),
+        "lc_last_error": (C.c_char_p, []),
+        "lc_ctx_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+        "lc_ctx_destroy": (C.c_int, [vp]),
+        "lc_ctx_sync": (C.c_int, [vp]),
+        "lc_geom2d_create": (C.c_int, [vp, f64p, i64, i64, i64, u32, C.POINTER(vp)]),
+        "lc_geom2d_destroy": (C.c_int, [vp]),
+        "lc_geom2d_info": (C.c_int, [vp, C.POINTER(i64)]),
+        "lc_psf2d_kernel_hat": (C.c_int, [vp, vp, f32p]),
+        "lc_sysmat2d_psf": (C.c_int, [vp, vp, vp, i64, vp, u32]),
+        "lc_cg2d": (C.c_int, [vp, vp, vp, vp, i64, i64, C.c_double, C.c_double, u32,
+                              C.POINTER(LcReport)]),
+        "lc_split_bregman2d": (C.c_int, [vp, vp, vp, vp, i64, C.c_double, C.c_double, i64,
+                                         i64, C.c_double, C.c_double, u32,
+                                         C.POINTER(LcReport)]),
+        "lc_grad": (C.c_int, [vp, vp, i64, i64, i64, vp, u32]),
+        "lc_divergence": (C.c_int, [vp, vp, i64, i64, i64, vp, u32]),
+        "lc_laplacian": (C.c_int, [vp, vp, i64, i64, i64, vp, u32]),
+        "lc_batch2d_create": (C.c_int, [vp, i32, C.POINTER(vp), C.POINTER(i32), C.POINTER(vp)]),
+        "lc_batch2d_destroy": (C.c_int, [vp]),
+        "lc_batch2d_set": (C.c_int, [vp, i32, i32, vp, u32]),
+        "lc_batch2d_get": (C.c_int, [vp, i32, i32, vp, u32]),
+        "lc_batch2d_device_ptr": (C.c_int, [vp, i32, i32, C.POINTER(vp)]),
+        "lc_batch2d_cg": (C.c_int, [vp, C.POINTER(i32), C.c_double, C.c_double, u32]),
+        "lc_batch2d_split_bregman": (C.c_int, [vp, f32p, f32p, i64, i64, C.c_double,
+                                               C.c_double, u32]),
+        "lc_batch2d_reports": (C.c_int, [vp, C.POINTER(LcReport)]),
+        "lc_batch2d_sysmat": (C.c_int, [vp, i32, i32]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(status):
+    if status != 0:
+        msg = _lib.lc_last_error().decode(errors="replace") if _lib else ""
+        raise LcError(status, msg)

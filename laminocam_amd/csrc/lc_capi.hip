@@ -1,0 +1,369 @@
+// extern "C" boundary of the library (include/laminocam.h).
+#include <cstring>
+#include <string>
+
+#include "laminocam.h"
+#include "lc_engine.hpp"
+
+using namespace lc;
+
+namespace {
+thread_local std::string g_err;
+
+template <typename F> int guard(F &&f) {
+    try {
+        f();
+        g_err.clear();
+        return LC_OK;
+    } catch (const lc::Error &e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_err = "host allocation failed";
+        return LC_ENOMEM;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return LC_EINTERNAL;
+    } catch (...) {
+        g_err = "unknown error";
+        return LC_EINTERNAL;
+    }
+}
+} // namespace
+
+struct lc_ctx_s {
+    Ctx c;
+    explicit lc_ctx_s(int dev) : c(dev) {}
+};
+
+struct lc_geom2d_s {
+    lc_ctx_s *ctx;
+    Geom2D g;
+};
+
+struct lc_batch2d_s {
+    lc_ctx_s *ctx;
+    std::unique_ptr<Batch2D> b;
+    std::vector<lc_report> reps;
+};
+
+static float *batch_buf(Batch2D &b, int which) {
+    switch (which) {
+    case 0: return b.X();
+    case 1: return b.Y();
+    case 2: return b.AP();
+    default: throw Error(LC_EINVAL, "batch buffer index must be 0 (x), 1 (y) or 2 (Ax)");
+    }
+}
+
+static void fill_report(const CgState &s, lc_report *r) {
+    r->iters = s.iters;
+    r->outer = 0;
+    r->res = s.res;
+    r->dx = s.dx;
+    r->stop = s.stop;
+    r->pad = 0;
+}
+
+extern "C" {
+
+int lc_abi_version(void) { return LC_ABI_VERSION; }
+
+const char *lc_last_error(void) { return g_err.c_str(); }
+
+int lc_ctx_create(int device, lc_ctx *out) {
+    return guard([&] {
+        LC_REQUIRE(out, "lc_ctx_create: out is NULL");
+        *out = new lc_ctx_s(device);
+    });
+}
+
+int lc_ctx_destroy(lc_ctx ctx) {
+    return guard([&] { delete ctx; });
+}
+
+int lc_ctx_sync(lc_ctx ctx) {
+    return guard([&] {
+        LC_REQUIRE(ctx, "NULL context");
+        ctx->c.set_current();
+        ctx->c.sync();
+    });
+}
+
+int lc_geom2d_create(lc_ctx ctx, const double *theta, int64_t nang, int64_t nrad,
+                     int64_t image_n, uint32_t flags, lc_geom2d *out) {
+    (void)flags;
+    return guard([&] {
+        LC_REQUIRE(ctx && theta && out, "lc_geom2d_create: NULL argument");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        ctx->c.set_current();
+        auto *g = new lc_geom2d_s();
+        g->ctx = ctx;
+        try {
+            geom2d_init(ctx->c, g->g, theta, nang, nrad, image_n);
+            geom2d_build_psf(ctx->c, g->g);
+            ctx->c.sync();
+        } catch (...) {
+            delete g;
+            throw;
+        }
+        *out = g;
+    });
+}
+
+int lc_geom2d_destroy(lc_geom2d g) {
+    return guard([&] {
+        if (g) {
+            g->ctx->c.set_current();
+            g->ctx->c.sync();
+        }
+        delete g;
+    });
+}
+
+int lc_geom2d_info(lc_geom2d g, int64_t *info) {
+    return guard([&] {
+        LC_REQUIRE(g && info, "lc_geom2d_info: NULL argument");
+        info[0] = g->g.nang;
+        info[1] = g->g.nrad;
+        info[2] = g->g.N;
+        info[3] = g->g.M;
+        info[4] = g->g.H;
+    });
+}
+
+int lc_psf2d_kernel_hat(lc_ctx ctx, lc_geom2d g, float *out) {
+    return guard([&] {
+        LC_REQUIRE(ctx && g && out, "lc_psf2d_kernel_hat: NULL argument");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        ctx->c.set_current();
+        geom2d_kernel_hat(ctx->c, g->g, out);
+    });
+}
+
+int lc_sysmat2d_psf(lc_ctx ctx, lc_geom2d g, const float *x, int64_t nslices, float *y,
+                    uint32_t flags) {
+    return guard([&] {
+        LC_REQUIRE(ctx && g && x && y, "lc_sysmat2d_psf: NULL argument");
+        LC_REQUIRE(nslices >= 1, "lc_sysmat2d_psf: nslices must be >= 1");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        ctx->c.set_current();
+        Batch2D b(ctx->c, {ProbSpec{&g->g, (int)nslices}});
+        const size_t bytes = sizeof(float) * (size_t)b.total();
+        const bool dev = flags & LC_DEVICE_PTRS;
+        copy_in(ctx->c, b.X(), x, bytes, dev);
+        b.toeplitz(b.X(), b.AP());
+        copy_out(ctx->c, y, b.AP(), bytes, dev);
+        ctx->c.sync();
+    });
+}
+
+int lc_cg2d(lc_ctx ctx, lc_geom2d g, const float *rhs, float *x, int64_t nslices,
+            int64_t max_iter, double tol, double xtol, uint32_t flags, lc_report *rep) {
+    return guard([&] {
+        LC_REQUIRE(ctx && g && rhs && x, "lc_cg2d: NULL argument");
+        LC_REQUIRE(nslices >= 1 && max_iter >= 0, "lc_cg2d: bad sizes");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        ctx->c.set_current();
+        Batch2D b(ctx->c, {ProbSpec{&g->g, (int)nslices}});
+        const size_t bytes = sizeof(float) * (size_t)b.total();
+        const bool dev = flags & LC_DEVICE_PTRS;
+        copy_in(ctx->c, b.Y(), rhs, bytes, dev);
+        copy_in(ctx->c, b.X(), x, bytes, dev);
+        const bool graph = flags & LC_GRAPH;
+        b.cg({(int)max_iter}, tol, xtol, graph, !graph, (flags & LC_X0_ZERO) != 0);
+        copy_out(ctx->c, x, b.X(), bytes, dev);
+        ctx->c.sync();
+        if (rep) fill_report(b.cg_state()[0], rep);
+    });
+}
+
+int lc_split_bregman2d(lc_ctx ctx, lc_geom2d g, const float *rhs, float *x, int64_t nslices,
+                       double lambda, double mu, int64_t outer, int64_t inner, double tol,
+                       double xtol, uint32_t flags, lc_report *rep) {
+    return guard([&] {
+        LC_REQUIRE(ctx && g && rhs && x, "lc_split_bregman2d: NULL argument");
+        LC_REQUIRE(nslices >= 1 && outer >= 0 && inner >= 0, "lc_split_bregman2d: bad sizes");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        ctx->c.set_current();
+        Batch2D b(ctx->c, {ProbSpec{&g->g, (int)nslices}});
+        const size_t bytes = sizeof(float) * (size_t)b.total();
+        const bool dev = flags & LC_DEVICE_PTRS;
+        copy_in(ctx->c, b.Y(), rhs, bytes, dev);
+        copy_in(ctx->c, b.X(), x, bytes, dev);
+        const bool graph = flags & LC_GRAPH;
+        b.split_bregman({(float)lambda}, {(float)mu}, (int)outer, (int)inner, tol, xtol,
+                        (flags & LC_TV_ZERO_PAD) ? 1 : 0, (flags & LC_TV_ANISO) != 0, graph,
+                        !graph);
+        copy_out(ctx->c, x, b.X(), bytes, dev);
+        ctx->c.sync();
+        if (rep) {
+            fill_report(b.cg_state()[0], rep);
+            const SbState s = b.sb_state()[0];
+            rep->outer = s.outer;
+            rep->dx = s.norm_diff;
+        }
+    });
+}
+
+static int fd_call(lc_ctx ctx, int op, const float *in, float *out, int64_t n1, int64_t n2,
+                   int64_t n3, uint32_t flags) {
+    return guard([&] {
+        LC_REQUIRE(ctx && in && out, "finite differences: NULL argument");
+        LC_REQUIRE(n1 >= 1 && n2 >= 1 && n3 >= 1, "finite differences: bad dims");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        ctx->c.set_current();
+        const int64_t n = n1 * n2 * n3;
+        const int64_t nin = (op == 1) ? 3 * n : n, nout = (op == 0) ? 3 * n : n;
+        const bool dev = flags & LC_DEVICE_PTRS;
+        const float *din = in;
+        float *dout = out;
+        if (!dev) {
+            float *buf = (float *)ctx->c.scratch(2, sizeof(float) * (size_t)(nin + nout));
+            copy_in(ctx->c, buf, in, sizeof(float) * nin, false);
+            din = buf;
+            dout = buf + nin;
+        }
+        fd_launch(ctx->c, op, din, dout, (int)n1, (int)n2, (int)n3,
+                  (flags & LC_TV_ZERO_PAD) ? 1 : 0);
+        if (!dev) copy_out(ctx->c, out, dout, sizeof(float) * nout, false);
+        ctx->c.sync();
+    });
+}
+
+int lc_grad(lc_ctx ctx, const float *u, int64_t n1, int64_t n2, int64_t n3, float *g,
+            uint32_t flags) {
+    return fd_call(ctx, 0, u, g, n1, n2, n3, flags);
+}
+
+int lc_divergence(lc_ctx ctx, const float *g, int64_t n1, int64_t n2, int64_t n3, float *out,
+                  uint32_t flags) {
+    return fd_call(ctx, 1, g, out, n1, n2, n3, flags);
+}
+
+int lc_laplacian(lc_ctx ctx, const float *u, int64_t n1, int64_t n2, int64_t n3, float *out,
+                 uint32_t flags) {
+    return fd_call(ctx, 2, u, out, n1, n2, n3, flags);
+}
+
+int lc_batch2d_create(lc_ctx ctx, int32_t nprob, const lc_geom2d *geoms, const int32_t *ns,
+                      lc_batch2d *out) {
+    return guard([&] {
+        LC_REQUIRE(ctx && geoms && ns && out && nprob >= 1, "lc_batch2d_create: bad argument");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        ctx->c.set_current();
+        std::vector<ProbSpec> ps;
+        for (int i = 0; i < nprob; ++i) {
+            LC_REQUIRE(geoms[i] && geoms[i]->ctx == ctx, "lc_batch2d_create: geometry of another context");
+            ps.push_back(ProbSpec{&geoms[i]->g, ns[i]});
+        }
+        auto *b = new lc_batch2d_s();
+        b->ctx = ctx;
+        try {
+            b->b = std::make_unique<Batch2D>(ctx->c, ps);
+            // zero x and y
+            LC_HIP(hipMemsetAsync(b->b->X(), 0, sizeof(float) * b->b->total(), ctx->c.stream()));
+            LC_HIP(hipMemsetAsync(b->b->Y(), 0, sizeof(float) * b->b->total(), ctx->c.stream()));
+            ctx->c.sync();
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        b->reps.resize(nprob);
+        *out = b;
+    });
+}
+
+int lc_batch2d_destroy(lc_batch2d b) {
+    return guard([&] {
+        if (b) {
+            b->ctx->c.set_current();
+            b->ctx->c.sync();
+        }
+        delete b;
+    });
+}
+
+int lc_batch2d_set(lc_batch2d b, int32_t prob, int32_t which, const float *src, uint32_t flags) {
+    return guard([&] {
+        LC_REQUIRE(b && src && prob >= 0 && prob < b->b->nprob(), "lc_batch2d_set: bad argument");
+        std::lock_guard<std::mutex> lk(b->ctx->c.mu);
+        b->ctx->c.set_current();
+        float *dst = batch_buf(*b->b, which) + b->b->offset(prob);
+        copy_in(b->ctx->c, dst, src, sizeof(float) * b->b->size(prob), flags & LC_DEVICE_PTRS);
+        b->ctx->c.sync();
+    });
+}
+
+int lc_batch2d_get(lc_batch2d b, int32_t prob, int32_t which, float *dst, uint32_t flags) {
+    return guard([&] {
+        LC_REQUIRE(b && dst && prob >= 0 && prob < b->b->nprob(), "lc_batch2d_get: bad argument");
+        std::lock_guard<std::mutex> lk(b->ctx->c.mu);
+        b->ctx->c.set_current();
+        const float *src = batch_buf(*b->b, which) + b->b->offset(prob);
+        copy_out(b->ctx->c, dst, src, sizeof(float) * b->b->size(prob), flags & LC_DEVICE_PTRS);
+        b->ctx->c.sync();
+    });
+}
+
+int lc_batch2d_device_ptr(lc_batch2d b, int32_t prob, int32_t which, float **out) {
+    return guard([&] {
+        LC_REQUIRE(b && out && prob >= 0 && prob < b->b->nprob(), "lc_batch2d_device_ptr: bad argument");
+        *out = batch_buf(*b->b, which) + b->b->offset(prob);
+    });
+}
+
+int lc_batch2d_cg(lc_batch2d b, const int32_t *max_iter, double tol, double xtol,
+                  uint32_t flags) {
+    return guard([&] {
+        LC_REQUIRE(b && max_iter, "lc_batch2d_cg: NULL argument");
+        std::lock_guard<std::mutex> lk(b->ctx->c.mu);
+        b->ctx->c.set_current();
+        std::vector<int> mi(max_iter, max_iter + b->b->nprob());
+        const bool graph = flags & LC_GRAPH;
+        b->b->cg(mi, tol, xtol, graph, false, (flags & LC_X0_ZERO) != 0);
+        auto st = b->b->cg_state();
+        for (int i = 0; i < b->b->nprob(); ++i) fill_report(st[i], &b->reps[i]);
+    });
+}
+
+int lc_batch2d_split_bregman(lc_batch2d b, const float *lambda, const float *mu, int64_t outer,
+                             int64_t inner, double tol, double xtol, uint32_t flags) {
+    return guard([&] {
+        LC_REQUIRE(b && lambda && mu, "lc_batch2d_split_bregman: NULL argument");
+        std::lock_guard<std::mutex> lk(b->ctx->c.mu);
+        b->ctx->c.set_current();
+        const int np = b->b->nprob();
+        std::vector<float> la(lambda, lambda + np), m(mu, mu + np);
+        const bool graph = flags & LC_GRAPH;
+        b->b->split_bregman(la, m, (int)outer, (int)inner, tol, xtol,
+                            (flags & LC_TV_ZERO_PAD) ? 1 : 0, (flags & LC_TV_ANISO) != 0, graph,
+                            false);
+        auto st = b->b->cg_state();
+        auto sb = b->b->sb_state();
+        for (int i = 0; i < np; ++i) {
+            fill_report(st[i], &b->reps[i]);
+            b->reps[i].outer = sb[i].outer;
+            b->reps[i].dx = sb[i].norm_diff;
+        }
+    });
+}
+
+int lc_batch2d_reports(lc_batch2d b, lc_report *reps) {
+    return guard([&] {
+        LC_REQUIRE(b && reps, "lc_batch2d_reports: NULL argument");
+        std::memcpy(reps, b->reps.data(), sizeof(lc_report) * b->reps.size());
+    });
+}
+
+int lc_batch2d_sysmat(lc_batch2d b, int32_t src, int32_t dst) {
+    return guard([&] {
+        LC_REQUIRE(b, "lc_batch2d_sysmat: NULL batch");
+        LC_REQUIRE(src != dst, "lc_batch2d_sysmat: src and dst must differ");
+        std::lock_guard<std::mutex> lk(b->ctx->c.mu);
+        b->ctx->c.set_current();
+        b->b->toeplitz(batch_buf(*b->b, src), batch_buf(*b->b, dst));
+    });
+}
+
+} // extern "C"

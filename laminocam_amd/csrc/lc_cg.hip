@@ -1,0 +1,805 @@
+// Batched CG and Split-Bregman engine for 2-D problems (stacks of slices).
+//
+// CG follows src/conjgrad.cpp:33-76 per problem:
+//   r = y - A x0, p = r, rho = <r,r>
+//   loop: Ap = A p; pAp = <p,Ap>; stop if |pAp| < 1e-10; alpha = rho/pAp;
+//         x += alpha p; r -= alpha Ap; rho' = <r,r>; p = r + (rho'/rho) p;
+//         res = ||r||/(||y||+1e-8); dx = ||alpha p||/(||x||+1e-8);
+//         stop if res < tol or dx < xtol
+// All inner products and norms accumulate in FP64 with a fixed reduction
+// order (per-block partials, then one wave per problem), so a problem's
+// iterates do not depend on which other problems share the batch.
+// The p update is fused into the Toeplitz row pass of the next iteration and
+// <p, Ap> into the inverse row pass.
+//
+// Split-Bregman follows src/bregman.cpp:36-94 per problem with the operator
+// A - mu * laplacian, the finite differences of src/finitediff.cpp:26-90
+// (boundary 0) or zero padded ones (boundary 1, src/gpu/finitediff.cu).
+#include <algorithm>
+#include <cstring>
+#include <map>
+
+#include "lc_engine.hpp"
+
+namespace lc {
+
+void toep_configure(ToepClass &k);
+void toep_launch_fwd(Ctx &c, const ToepClass &k, const CgState *st, const float *src,
+                     const float *rsrc, float *pdst, float2 *U, bool fuse);
+void toep_launch_cols(Ctx &c, const ToepClass &k, const ProbDev *pd, const CgState *st,
+                      float2 *U);
+void toep_launch_inv(Ctx &c, const ToepClass &k, const CgState *st, const float2 *U, float *dst,
+                     const float *pvec, double *partial, bool dot);
+
+// ----------------------------------------------------------------------------
+// reductions
+// ----------------------------------------------------------------------------
+template <int NT> __device__ __forceinline__ double block_sum(double v, double *sh) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < NT / 64; ++w) t += sh[w];
+    __syncthreads();
+    return t;   // valid in thread 0
+}
+
+// one wave per problem: fixed-order sum of partial[stride*i + comp] over [b, e)
+__device__ __forceinline__ double wave_sum(const double *partial, int b, int e, int stride,
+                                           int comp) {
+    double v = 0.0;
+    for (int i = b + (int)threadIdx.x; i < e; i += 64) v += partial[(int64_t)i * stride + comp];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return __shfl(v, 0, 64);
+}
+
+__global__ void k_fill(float *p, float v, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// arm CG states for a solve: active = max_iter > 0 (and the SB problem active)
+__global__ void k_cg_arm(CgState *st, const SbState *sb, const int *max_iter, double tol,
+                         double xtol, int nprob) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nprob) return;
+    CgState s = st[p];
+    s.max_iter = max_iter ? max_iter[p] : s.max_iter;
+    s.tol = tol;
+    s.xtol = xtol;
+    s.active = (s.max_iter > 0) && (sb == nullptr || sb[p].active);
+    s.iters = 0;
+    s.stop = 0;
+    s.beta = 0.0;
+    st[p] = s;
+}
+
+// r = y - (x0 == 0 ? 0 : Ap); p = 0; partials <r,r>, <y,y>
+template <int NT>
+__global__ void __launch_bounds__(NT)
+k_cg_init(const ChunkRef *__restrict__ ch, const CgState *__restrict__ st,
+          const float *__restrict__ y, const float *__restrict__ ap, float *__restrict__ r,
+          float *__restrict__ p, double *__restrict__ cpart, int x0_zero) {
+    const ChunkRef c = ch[blockIdx.x];
+    if (!st[c.prob].active) return;
+    double rr = 0.0, yy = 0.0;
+    for (int64_t i = c.begin + threadIdx.x; i < c.end; i += NT) {
+        const float yi = y[i];
+        const float ri = x0_zero ? yi : yi - ap[i];
+        r[i] = ri;
+        p[i] = 0.0f;
+        rr += (double)ri * ri;
+        yy += (double)yi * yi;
+    }
+    __shared__ double sh[NT / 64];
+    const double a = block_sum<NT>(rr, sh);
+    const double b = block_sum<NT>(yy, sh);
+    if (threadIdx.x == 0) {
+        cpart[blockIdx.x * 4 + 0] = a;
+        cpart[blockIdx.x * 4 + 1] = b;
+    }
+}
+
+__global__ void k_cg_fin0(const ProbDev *__restrict__ pd, CgState *st, const double *cpart) {
+    const int p = blockIdx.x;
+    CgState s = st[p];
+    if (!s.active) return;
+    const double rr = wave_sum(cpart, pd[p].chunk0, pd[p].chunk0 + pd[p].nchunk, 4, 0);
+    const double yy = wave_sum(cpart, pd[p].chunk0, pd[p].chunk0 + pd[p].nchunk, 4, 1);
+    if (threadIdx.x == 0) {
+        s.rho = rr;
+        s.ynorm = sqrt(yy);
+        s.beta = 0.0;
+        st[p] = s;
+    }
+}
+
+// pAp from Toeplitz slot partials (use_chunks=0) or chunk partials (1)
+__global__ void k_cg_fin1(const ProbDev *__restrict__ pd, CgState *st, const double *slot_part,
+                          const double *cpart, int use_chunks) {
+    const int p = blockIdx.x;
+    CgState s = st[p];
+    if (!s.active) return;
+    const double pap = use_chunks
+                           ? wave_sum(cpart, pd[p].chunk0, pd[p].chunk0 + pd[p].nchunk, 4, 3)
+                           : wave_sum(slot_part, pd[p].slot0, pd[p].slot0 + pd[p].nslot, 1, 0);
+    if (threadIdx.x == 0) {
+        s.pAp = pap;
+        if (fabs(pap) < 1.0e-10) {
+            s.active = 0;
+            s.stop = 3;
+        } else {
+            s.alpha = s.rho / pap;
+        }
+        st[p] = s;
+    }
+}
+
+// x += alpha p; r -= alpha Ap; partials <r,r>, ||alpha p||^2, ||x||^2
+template <int NT>
+__global__ void __launch_bounds__(NT)
+k_cg_update(const ChunkRef *__restrict__ ch, const CgState *__restrict__ st,
+            float *__restrict__ x, float *__restrict__ r, const float *__restrict__ p,
+            const float *__restrict__ ap, double *__restrict__ cpart) {
+    const ChunkRef c = ch[blockIdx.x];
+    const CgState s = st[c.prob];
+    if (!s.active) return;
+    const float alpha = (float)s.alpha;
+    double rr = 0.0, ss = 0.0, xx = 0.0;
+    for (int64_t i = c.begin + threadIdx.x; i < c.end; i += NT) {
+        const float step = p[i] * alpha;
+        const float xi = x[i] + step;
+        const float ri = r[i] - ap[i] * alpha;
+        x[i] = xi;
+        r[i] = ri;
+        rr += (double)ri * ri;
+        ss += (double)step * step;
+        xx += (double)xi * xi;
+    }
+    __shared__ double sh[NT / 64];
+    const double a = block_sum<NT>(rr, sh);
+    const double b = block_sum<NT>(ss, sh);
+    const double d = block_sum<NT>(xx, sh);
+    if (threadIdx.x == 0) {
+        cpart[blockIdx.x * 4 + 0] = a;
+        cpart[blockIdx.x * 4 + 1] = b;
+        cpart[blockIdx.x * 4 + 2] = d;
+    }
+}
+
+__global__ void k_cg_fin2(const ProbDev *__restrict__ pd, CgState *st, const double *cpart) {
+    const int p = blockIdx.x;
+    CgState s = st[p];
+    if (!s.active) return;
+    const int b = pd[p].chunk0, e = pd[p].chunk0 + pd[p].nchunk;
+    const double rr = wave_sum(cpart, b, e, 4, 0);
+    const double ss = wave_sum(cpart, b, e, 4, 1);
+    const double xx = wave_sum(cpart, b, e, 4, 2);
+    if (threadIdx.x == 0) {
+        s.beta = rr / s.rho;
+        s.rho = rr;
+        s.res = sqrt(rr) / (s.ynorm + 1e-8);
+        s.dx = sqrt(ss) / (sqrt(xx) + 1e-8);
+        s.iters += 1;
+        if (s.res < s.tol) { s.active = 0; s.stop = 1; }
+        else if (s.dx < s.xtol) { s.active = 0; s.stop = 2; }
+        else if (s.iters >= s.max_iter) { s.active = 0; s.stop = 0; }
+        st[p] = s;
+    }
+}
+
+// ----------------------------------------------------------------------------
+// finite differences (per element of a problem's n1 x N x N stack)
+// ----------------------------------------------------------------------------
+struct Idx3 {
+    int s, a, b, n1, n2, n3;
+};
+
+__device__ __forceinline__ float at3(const float *u, const Idx3 &q, int s, int a, int b) {
+    return u[((int64_t)s * q.n2 + a) * q.n3 + b];
+}
+
+__device__ __forceinline__ bool interior(const Idx3 &q, int s, int a, int b) {
+    return s >= 1 && s + 1 < q.n1 && a >= 1 && a + 1 < q.n2 && b >= 1 && b + 1 < q.n3;
+}
+
+__device__ __forceinline__ bool inside(const Idx3 &q, int s, int a, int b) {
+    return s >= 0 && s < q.n1 && a >= 0 && a < q.n2 && b >= 0 && b < q.n3;
+}
+
+// gradient component c (0 slices, 1 rows, 2 cols) at (s,a,b)
+__device__ __forceinline__ float grad_c(const float *u, const Idx3 &q, int c, int s, int a, int b,
+                                        int boundary) {
+    const int ds = c == 0, da = c == 1, db = c == 2;
+    if (boundary == 0) {
+        if (!interior(q, s, a, b)) return 0.0f;
+        return (at3(u, q, s + ds, a + da, b + db) - at3(u, q, s - ds, a - da, b - db)) / 2.0f;
+    }
+    const float up = inside(q, s + ds, a + da, b + db) ? at3(u, q, s + ds, a + da, b + db) : 0.0f;
+    const float um = inside(q, s - ds, a - da, b - db) ? at3(u, q, s - ds, a - da, b - db) : 0.0f;
+    return 0.5f * (up - um);
+}
+
+// divergence of a 3-component field g (component c at g + c*cstride) at (s,a,b)
+__device__ __forceinline__ float div_at(const float *g, int64_t cstride, const Idx3 &q, int s,
+                                        int a, int b, int boundary) {
+    if (boundary == 0) {
+        if (!interior(q, s, a, b)) return 0.0f;
+        return (at3(g, q, s + 1, a, b) - at3(g, q, s - 1, a, b)) / 2.0f +
+               (at3(g + cstride, q, s, a + 1, b) - at3(g + cstride, q, s, a - 1, b)) / 2.0f +
+               (at3(g + 2 * cstride, q, s, a, b + 1) - at3(g + 2 * cstride, q, s, a, b - 1)) / 2.0f;
+    }
+    const float g0p = inside(q, s + 1, a, b) ? at3(g, q, s + 1, a, b) : 0.0f;
+    const float g0m = inside(q, s - 1, a, b) ? at3(g, q, s - 1, a, b) : 0.0f;
+    const float g1p = inside(q, s, a + 1, b) ? at3(g + cstride, q, s, a + 1, b) : 0.0f;
+    const float g1m = inside(q, s, a - 1, b) ? at3(g + cstride, q, s, a - 1, b) : 0.0f;
+    const float g2p = inside(q, s, a, b + 1) ? at3(g + 2 * cstride, q, s, a, b + 1) : 0.0f;
+    const float g2m = inside(q, s, a, b - 1) ? at3(g + 2 * cstride, q, s, a, b - 1) : 0.0f;
+    return 0.5f * (g0p - g0m) + 0.5f * (g1p - g1m) + 0.5f * (g2p - g2m);
+}
+
+// laplacian = div(grad u) evaluated directly from u
+__device__ __forceinline__ float lap_at(const float *u, const Idx3 &q, int s, int a, int b,
+                                        int boundary) {
+    float acc = 0.0f;
+    if (boundary == 0) {
+        if (!interior(q, s, a, b)) return 0.0f;
+        // component c: (G(i+e) - G(i-e))/2 with G zero off the interior
+        for (int c = 0; c < 3; ++c) {
+            const int ds = c == 0, da = c == 1, db = c == 2;
+            const float gp = grad_c(u, q, c, s + ds, a + da, b + db, 0);
+            const float gm = grad_c(u, q, c, s - ds, a - da, b - db, 0);
+            acc += (gp - gm) / 2.0f;
+        }
+        return acc;
+    }
+    for (int c = 0; c < 3; ++c) {
+        const int ds = c == 0, da = c == 1, db = c == 2;
+        const float gp = inside(q, s + ds, a + da, b + db) ? grad_c(u, q, c, s + ds, a + da, b + db, 1) : 0.0f;
+        const float gm = inside(q, s - ds, a - da, b - db) ? grad_c(u, q, c, s - ds, a - da, b - db, 1) : 0.0f;
+        acc += 0.5f * (gp - gm);
+    }
+    return acc;
+}
+
+__device__ __forceinline__ Idx3 idx_of(const ProbDev &d, int64_t i) {
+    const int64_t loc = i - d.off;
+    Idx3 q;
+    q.n2 = d.N;
+    q.n3 = d.N;
+    q.n1 = d.ns;
+    const int64_t nn = (int64_t)d.N * d.N;
+    q.s = (int)(loc / nn);
+    const int64_t rem = loc - (int64_t)q.s * nn;
+    q.a = (int)(rem / d.N);
+    q.b = (int)(rem - (int64_t)q.a * d.N);
+    return q;
+}
+
+// Ap -= mu * lap(p); partial <p, Ap> (Split-Bregman operator, src/bregman.cpp:55)
+template <int NT>
+__global__ void __launch_bounds__(NT)
+k_sb_lapdot(const ChunkRef *__restrict__ ch, const ProbDev *__restrict__ pd,
+            const CgState *__restrict__ st, const float *__restrict__ p, float *__restrict__ ap,
+            double *__restrict__ cpart, int dot) {
+    const ChunkRef c = ch[blockIdx.x];
+    if (st && !st[c.prob].active) return;
+    const ProbDev d = pd[c.prob];
+    const float *pp = p + d.off;
+    double acc = 0.0;
+    for (int64_t i = c.begin + threadIdx.x; i < c.end; i += NT) {
+        const Idx3 q = idx_of(d, i);
+        const float l = lap_at(pp, q, q.s, q.a, q.b, d.boundary);
+        const float v = ap[i] - l * d.mu;
+        ap[i] = v;
+        acc += (double)p[i] * v;
+    }
+    if (dot) {
+        __shared__ double sh[NT / 64];
+        const double t = block_sum<NT>(acc, sh);
+        if (threadIdx.x == 0) cpart[blockIdx.x * 4 + 3] = t;
+    }
+}
+
+// rhs = y0 - mu * div(d - b)   (src/bregman.cpp:60-62)
+template <int NT>
+__global__ void __launch_bounds__(NT)
+k_sb_rhs(const ChunkRef *__restrict__ ch, const ProbDev *__restrict__ pd,
+         const SbState *__restrict__ sb, const float *__restrict__ y0, const float *__restrict__ D,
+         const float *__restrict__ Bv, float *__restrict__ rhs, float *__restrict__ tmp,
+         int64_t total) {
+    const ChunkRef c = ch[blockIdx.x];
+    if (!sb[c.prob].active) return;
+    const ProbDev d = pd[c.prob];
+    // tmp holds d - b for the three components (component stride = total)
+    for (int64_t i = c.begin + threadIdx.x; i < c.end; i += NT) {
+        const Idx3 q = idx_of(d, i);
+        const float *g = tmp + d.off;
+        const float dv = div_at(g, total, q, q.s, q.a, q.b, d.boundary);
+        rhs[i] = y0[i] - dv * d.mu;
+    }
+}
+
+template <int NT>
+__global__ void __launch_bounds__(NT)
+k_sb_dmb(const ChunkRef *__restrict__ ch, const SbState *__restrict__ sb,
+         const float *__restrict__ D, const float *__restrict__ Bv, float *__restrict__ tmp,
+         int64_t total) {
+    const ChunkRef c = ch[blockIdx.x];
+    if (!sb[c.prob].active) return;
+    for (int64_t i = c.begin + threadIdx.x; i < c.end; i += NT)
+        for (int k = 0; k < 3; ++k) tmp[k * total + i] = D[k * total + i] - Bv[k * total + i];
+}
+
+// isotropic (or per-component) shrink and Bregman update
+// (src/bregman.cpp:64-84); partials ||x - x_old||^2, ||x_old||^2; x_old = x
+template <int NT>
+__global__ void __launch_bounds__(NT)
+k_sb_shrink(const ChunkRef *__restrict__ ch, const ProbDev *__restrict__ pd,
+            const SbState *__restrict__ sb, const float *__restrict__ x, float *__restrict__ D,
+            float *__restrict__ Bv, float *__restrict__ xold, double *__restrict__ cpart,
+            int64_t total, int aniso) {
+    const ChunkRef c = ch[blockIdx.x];
+    const SbState S = sb[c.prob];
+    if (!S.active) return;
+    const ProbDev d = pd[c.prob];
+    const float *xp = x + d.off;
+    const float thr = (float)(S.lambda / S.mu);
+    double dn = 0.0, on = 0.0;
+    for (int64_t i = c.begin + threadIdx.x; i < c.end; i += NT) {
+        const Idx3 q = idx_of(d, i);
+        float g[3], v[3];
+        for (int k = 0; k < 3; ++k) {
+            g[k] = grad_c(xp, q, k, q.s, q.a, q.b, d.boundary);
+            v[k] = g[k] + Bv[k * total + i];
+        }
+        if (!aniso) {
+            const float sk = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            const float sh = fmaxf(0.0f, sk - thr);
+            for (int k = 0; k < 3; ++k) {
+                const float dk = sh * (v[k] / (sk + 1e-8f));
+                D[k * total + i] = dk;
+                Bv[k * total + i] += g[k] - dk;
+            }
+        } else {
+            for (int k = 0; k < 3; ++k) {
+                const float sh = fmaxf(0.0f, fabsf(v[k]) - thr);
+                const float dk = copysignf(sh, v[k]);
+                D[k * total + i] = dk;
+                Bv[k * total + i] += g[k] - dk;
+            }
+        }
+        const float xi = x[i], xo = xold[i];
+        dn += (double)(xi - xo) * (double)(xi - xo);
+        on += (double)xo * xo;
+        xold[i] = xi;
+    }
+    __shared__ double sh[NT / 64];
+    const double a = block_sum<NT>(dn, sh);
+    const double b = block_sum<NT>(on, sh);
+    if (threadIdx.x == 0) {
+        cpart[blockIdx.x * 4 + 0] = a;
+        cpart[blockIdx.x * 4 + 1] = b;
+    }
+}
+
+__global__ void k_sb_fin(const ProbDev *__restrict__ pd, SbState *sb, const double *cpart) {
+    const int p = blockIdx.x;
+    SbState s = sb[p];
+    if (!s.active) return;
+    const int b = pd[p].chunk0, e = pd[p].chunk0 + pd[p].nchunk;
+    const double dn = wave_sum(cpart, b, e, 4, 0);
+    const double on = wave_sum(cpart, b, e, 4, 1);
+    if (threadIdx.x == 0) {
+        s.norm_diff = sqrt(dn) / (sqrt(on) + 1e-8);
+        s.outer += 1;
+        if (s.norm_diff < s.xtol) { s.active = 0; s.stop = 2; }
+        else if (s.outer >= s.outer_max) { s.active = 0; s.stop = 0; }
+        sb[p] = s;
+    }
+}
+
+// ----------------------------------------------------------------------------
+// standalone finite differences on one n1 x n2 x n3 volume
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ Idx3 idx_vol(int64_t i, int n1, int n2, int n3) {
+    Idx3 q;
+    q.n1 = n1;
+    q.n2 = n2;
+    q.n3 = n3;
+    q.s = (int)(i / ((int64_t)n2 * n3));
+    const int64_t rem = i - (int64_t)q.s * n2 * n3;
+    q.a = (int)(rem / n3);
+    q.b = (int)(rem - (int64_t)q.a * n3);
+    return q;
+}
+
+__global__ void k_fd_grad(const float *u, float *g, int n1, int n2, int n3, int boundary) {
+    const int64_t n = (int64_t)n1 * n2 * n3;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const Idx3 q = idx_vol(i, n1, n2, n3);
+        for (int c = 0; c < 3; ++c) g[c * n + i] = grad_c(u, q, c, q.s, q.a, q.b, boundary);
+    }
+}
+
+__global__ void k_fd_div(const float *g, float *out, int n1, int n2, int n3, int boundary) {
+    const int64_t n = (int64_t)n1 * n2 * n3;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const Idx3 q = idx_vol(i, n1, n2, n3);
+        out[i] = div_at(g, n, q, q.s, q.a, q.b, boundary);
+    }
+}
+
+__global__ void k_fd_lap(const float *u, float *out, int n1, int n2, int n3, int boundary) {
+    const int64_t n = (int64_t)n1 * n2 * n3;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const Idx3 q = idx_vol(i, n1, n2, n3);
+        out[i] = lap_at(u, q, q.s, q.a, q.b, boundary);
+    }
+}
+
+void fd_launch(Ctx &c, int op, const float *in, float *out, int n1, int n2, int n3,
+               int boundary) {
+    const int64_t n = (int64_t)n1 * n2 * n3;
+    const int blocks = (int)std::min<int64_t>(4096, (n + 255) / 256);
+    if (op == 0)
+        hipLaunchKernelGGL(k_fd_grad, dim3(blocks), dim3(256), 0, c.stream(), in, out, n1, n2, n3,
+                           boundary);
+    else if (op == 1)
+        hipLaunchKernelGGL(k_fd_div, dim3(blocks), dim3(256), 0, c.stream(), in, out, n1, n2, n3,
+                           boundary);
+    else
+        hipLaunchKernelGGL(k_fd_lap, dim3(blocks), dim3(256), 0, c.stream(), in, out, n1, n2, n3,
+                           boundary);
+    LC_HIP(hipGetLastError());
+}
+
+// ----------------------------------------------------------------------------
+// Batch2D
+// ----------------------------------------------------------------------------
+Batch2D::Batch2D(Ctx &c, const std::vector<ProbSpec> &probs) : c_(c), probs_(probs) {
+    LC_REQUIRE(!probs.empty(), "batch: no problems");
+    build();
+}
+
+void Batch2D::build() {
+    const int np = (int)probs_.size();
+    hprob_.resize(np);
+    // group slices by N into classes
+    std::map<int, std::vector<int>> by_n;
+    for (int p = 0; p < np; ++p) {
+        const Geom2D *g = probs_[p].g;
+        LC_REQUIRE(g && g->psf_ready, "batch: geometry without PSF");
+        LC_REQUIRE(probs_[p].ns >= 1, "batch: ns must be >= 1");
+        by_n[(int)g->N].push_back(p);
+    }
+    // vector layout: problems in class order so every class is contiguous
+    int64_t off = 0, uoff = 0;
+    int chunk = 0, slot = 0;
+    std::vector<int> order;
+    for (auto &kv : by_n)
+        for (int p : kv.second) order.push_back(p);
+    // host tables stay alive until the uploads below have completed
+    std::vector<std::vector<SliceRef>> keep;
+    // classes
+    for (auto &kv : by_n) {
+        auto k = std::make_unique<ToepClass>();
+        k->N = kv.first;
+        k->M = (int)next_fast_dim(2 * k->N - 1);
+        k->H = k->M / 2 + 1;
+        LC_REQUIRE(fft_make_plan(k->M, k->plan), "batch: PSF size not 2^p 3^q");
+        k->tw = c_.twiddles<float>(k->M);
+        toep_configure(*k);
+        std::vector<SliceRef> sl;
+        for (int p : kv.second) {
+            const int N = k->N, ns = probs_[p].ns;
+            ProbDev &d = hprob_[p];
+            d.khatT = probs_[p].g->khatT.as<float>();
+            d.N = N;
+            d.ns = ns;
+            d.off = off;
+            d.n = (int64_t)ns * N * N;
+            d.mu = 0.0f;
+            d.boundary = 0;
+            d.slot0 = slot;
+            d.nslot = ns * k->p1_blocks;
+            slot += d.nslot;
+            d.chunk0 = chunk;
+            d.nchunk = ceil_div(d.n, CHUNK);
+            for (int i = 0; i < d.nchunk; ++i) {
+                ChunkRef cr;
+                cr.prob = p;
+                cr.pad = 0;
+                cr.begin = off + (int64_t)i * CHUNK;
+                cr.end = std::min(off + d.n, cr.begin + CHUNK);
+                hchunk_.push_back(cr);
+            }
+            chunk += d.nchunk;
+            for (int s = 0; s < ns; ++s) {
+                SliceRef r;
+                r.prob = p;
+                r.slice = s;
+                r.xoff = off + (int64_t)s * N * N;
+                r.uoff = uoff;
+                uoff += (int64_t)N * k->H;
+                sl.push_back(r);
+            }
+            off += d.n;
+        }
+        k->nslices = (int)sl.size();
+        k->slot_base = hprob_[kv.second.front()].slot0;
+        k->slices.alloc(sizeof(SliceRef) * sl.size());
+        keep.push_back(std::move(sl));
+        LC_HIP(hipMemcpyAsync(k->slices.as(), keep.back().data(),
+                              sizeof(SliceRef) * keep.back().size(), hipMemcpyHostToDevice,
+                              c_.stream()));
+        classes_.push_back(std::move(k));
+    }
+    total_ = off;
+    total_u_ = uoff;
+    nchunk_ = chunk;
+    nslot_ = slot;
+    const size_t fb = sizeof(float) * (size_t)total_;
+    X_.alloc(fb);
+    R_.alloc(fb);
+    P_.alloc(fb);
+    AP_.alloc(fb);
+    Y_.alloc(fb);
+    U_.alloc(sizeof(float2) * (size_t)total_u_);
+    dprob_.alloc(sizeof(ProbDev) * np);
+    dchunk_.alloc(sizeof(ChunkRef) * hchunk_.size());
+    dstate_.alloc(sizeof(CgState) * np);
+    dsb_.alloc(sizeof(SbState) * np);
+    partial_.alloc(sizeof(double) * std::max(1, nslot_));
+    cpartial_.alloc(sizeof(double) * 4 * std::max(1, nchunk_));
+    LC_HIP(hipMemcpyAsync(dprob_.as(), hprob_.data(), sizeof(ProbDev) * np, hipMemcpyHostToDevice,
+                          c_.stream()));
+    LC_HIP(hipMemcpyAsync(dchunk_.as(), hchunk_.data(), sizeof(ChunkRef) * hchunk_.size(),
+                          hipMemcpyHostToDevice, c_.stream()));
+    std::vector<CgState> z(np);
+    for (auto &s : z) {
+        std::memset(&s, 0, sizeof(s));
+        s.active = 1;
+    }
+    LC_HIP(hipMemcpyAsync(dstate_.as(), z.data(), sizeof(CgState) * np, hipMemcpyHostToDevice,
+                          c_.stream()));
+    std::vector<SbState> zs(np);
+    std::memset(zs.data(), 0, sizeof(SbState) * np);
+    LC_HIP(hipMemcpyAsync(dsb_.as(), zs.data(), sizeof(SbState) * np, hipMemcpyHostToDevice,
+                          c_.stream()));
+    c_.sync();
+}
+
+void Batch2D::apply_op(const float *src, float *dst, bool sb, bool dot) {
+    CgState *st = dstate_.as<CgState>();
+    for (auto &k : classes_) {
+        toep_launch_fwd(c_, *k, st, src, nullptr, nullptr, U_.as<float2>(), false);
+        toep_launch_cols(c_, *k, dprob_.as<ProbDev>(), st, U_.as<float2>());
+        toep_launch_inv(c_, *k, st, U_.as<float2>(), dst, src, partial_.as<double>(),
+                        dot && !sb);
+    }
+    if (sb) {
+        hipLaunchKernelGGL((k_sb_lapdot<256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
+                           dchunk_.as<ChunkRef>(), dprob_.as<ProbDev>(), st, src, dst,
+                           cpartial_.as<double>(), dot ? 1 : 0);
+        LC_HIP(hipGetLastError());
+    }
+}
+
+void Batch2D::toeplitz(const float *src, float *dst) {
+    // all problems: arm every state active with no iterations
+    std::vector<int> one(probs_.size(), 1);
+    DBuf mi(sizeof(int) * one.size());
+    LC_HIP(hipMemcpyAsync(mi.as(), one.data(), sizeof(int) * one.size(), hipMemcpyHostToDevice,
+                          c_.stream()));
+    hipLaunchKernelGGL(k_cg_arm, dim3(ceil_div(nprob(), 64)), dim3(64), 0, c_.stream(),
+                       dstate_.as<CgState>(), nullptr, mi.as<int>(), 0.0, 0.0, nprob());
+    LC_HIP(hipGetLastError());
+    apply_op(src, dst, false, false);
+    c_.sync();
+}
+
+void Batch2D::cg_setup(bool x0_zero, bool sb, const std::vector<int> &max_iter, double tol,
+                       double xtol) {
+    (void)max_iter;
+    (void)tol;
+    (void)xtol;
+    CgState *st = dstate_.as<CgState>();
+    if (!x0_zero) apply_op(X(), AP(), sb, false);
+    hipLaunchKernelGGL((k_cg_init<256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
+                       dchunk_.as<ChunkRef>(), st, Y(), AP(), R(), P(), cpartial_.as<double>(),
+                       x0_zero ? 1 : 0);
+    LC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_cg_fin0, dim3(nprob()), dim3(64), 0, c_.stream(), dprob_.as<ProbDev>(),
+                       st, cpartial_.as<double>());
+    LC_HIP(hipGetLastError());
+}
+
+void Batch2D::cg_iteration(bool host_sync, bool sb) {
+    CgState *st = dstate_.as<CgState>();
+    float2 *U = U_.as<float2>();
+    auto hs = [&]() {
+        if (host_sync) {
+            std::vector<CgState> h(nprob());
+            LC_HIP(hipMemcpyAsync(h.data(), st, sizeof(CgState) * nprob(), hipMemcpyDeviceToHost,
+                                  c_.stream()));
+            c_.sync();
+        }
+    };
+    for (auto &k : classes_) {
+        toep_launch_fwd(c_, *k, st, P(), R(), P(), U, true);
+        toep_launch_cols(c_, *k, dprob_.as<ProbDev>(), st, U);
+        toep_launch_inv(c_, *k, st, U, AP(), P(), partial_.as<double>(), !sb);
+    }
+    if (sb) {
+        hipLaunchKernelGGL((k_sb_lapdot<256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
+                           dchunk_.as<ChunkRef>(), dprob_.as<ProbDev>(), st, P(), AP(),
+                           cpartial_.as<double>(), 1);
+        LC_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_cg_fin1, dim3(nprob()), dim3(64), 0, c_.stream(), dprob_.as<ProbDev>(),
+                       st, partial_.as<double>(), cpartial_.as<double>(), sb ? 1 : 0);
+    LC_HIP(hipGetLastError());
+    hs();
+    hipLaunchKernelGGL((k_cg_update<256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
+                       dchunk_.as<ChunkRef>(), st, X(), R(), P(), AP(), cpartial_.as<double>());
+    LC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_cg_fin2, dim3(nprob()), dim3(64), 0, c_.stream(), dprob_.as<ProbDev>(),
+                       st, cpartial_.as<double>());
+    LC_HIP(hipGetLastError());
+    hs();
+}
+
+void Batch2D::cg(const std::vector<int> &max_iter, double tol, double xtol, bool graph,
+                 bool host_sync, bool x0_zero) {
+    LC_REQUIRE((int)max_iter.size() == nprob(), "cg: one max_iter per problem");
+    const int iters = *std::max_element(max_iter.begin(), max_iter.end());
+    // mu = 0: plain CG on A
+    for (auto &d : hprob_) d.mu = 0.0f;
+    LC_HIP(hipMemcpyAsync(dprob_.as(), hprob_.data(), sizeof(ProbDev) * nprob(),
+                          hipMemcpyHostToDevice, c_.stream()));
+    DBuf mi(sizeof(int) * max_iter.size());
+    LC_HIP(hipMemcpyAsync(mi.as(), max_iter.data(), sizeof(int) * max_iter.size(),
+                          hipMemcpyHostToDevice, c_.stream()));
+    hipLaunchKernelGGL(k_cg_arm, dim3(ceil_div(nprob(), 64)), dim3(64), 0, c_.stream(),
+                       dstate_.as<CgState>(), nullptr, mi.as<int>(), tol, xtol, nprob());
+    LC_HIP(hipGetLastError());
+    cg_setup(x0_zero, false, max_iter, tol, xtol);
+    if (graph && !host_sync) {
+        hipGraph_t g;
+        hipGraphExec_t ge;
+        LC_HIP(hipStreamBeginCapture(c_.stream(), hipStreamCaptureModeThreadLocal));
+        cg_iteration(false, false);
+        LC_HIP(hipStreamEndCapture(c_.stream(), &g));
+        LC_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        for (int it = 0; it < iters; ++it) LC_HIP(hipGraphLaunch(ge, c_.stream()));
+        LC_HIP(hipGraphExecDestroy(ge));
+        LC_HIP(hipGraphDestroy(g));
+    } else {
+        for (int it = 0; it < iters; ++it) {
+            cg_iteration(host_sync, false);
+            if (host_sync) {
+                // the plain path stops as soon as its (single) problem stops
+                std::vector<CgState> h = cg_state();
+                bool any = false;
+                for (auto &s : h) any = any || s.active;
+                if (!any) break;
+            }
+        }
+    }
+    c_.sync();
+}
+
+void Batch2D::split_bregman(const std::vector<float> &lambda, const std::vector<float> &mu,
+                            int outer, int inner, double tol, double xtol, int boundary,
+                            bool aniso, bool graph, bool host_sync) {
+    const int np = nprob();
+    LC_REQUIRE((int)lambda.size() == np && (int)mu.size() == np, "sb: one lambda/mu per problem");
+    sb_boundary_ = boundary;
+    sb_aniso_ = aniso;
+    for (int p = 0; p < np; ++p) {
+        hprob_[p].mu = mu[p];
+        hprob_[p].boundary = boundary;
+    }
+    LC_HIP(hipMemcpyAsync(dprob_.as(), hprob_.data(), sizeof(ProbDev) * np, hipMemcpyHostToDevice,
+                          c_.stream()));
+    const size_t fb = sizeof(float) * (size_t)total_;
+    Dv_.reserve(3 * fb);
+    Bv_.reserve(3 * fb);
+    XO_.reserve(fb);
+    Y0_.reserve(fb);
+    T_.reserve(3 * fb);
+    // d = b = 0, x_old = x0, y0 = A^T y
+    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, c_.stream(), Dv_.as<float>(), 0.0f,
+                       3 * total_);
+    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, c_.stream(), Bv_.as<float>(), 0.0f,
+                       3 * total_);
+    LC_HIP(hipMemcpyAsync(XO_.as(), X(), fb, hipMemcpyDeviceToDevice, c_.stream()));
+    LC_HIP(hipMemcpyAsync(Y0_.as(), Y(), fb, hipMemcpyDeviceToDevice, c_.stream()));
+    std::vector<SbState> hs(np);
+    for (int p = 0; p < np; ++p) {
+        std::memset(&hs[p], 0, sizeof(SbState));
+        hs[p].lambda = lambda[p];
+        hs[p].mu = mu[p];
+        hs[p].xtol = xtol;
+        hs[p].outer_max = outer;
+        hs[p].active = outer > 0;
+    }
+    LC_HIP(hipMemcpyAsync(dsb_.as(), hs.data(), sizeof(SbState) * np, hipMemcpyHostToDevice,
+                          c_.stream()));
+    std::vector<int> mi(np, inner);
+    DBuf dmi(sizeof(int) * np);
+    LC_HIP(hipMemcpyAsync(dmi.as(), mi.data(), sizeof(int) * np, hipMemcpyHostToDevice,
+                          c_.stream()));
+    const SbState *sbs = dsb_.as<SbState>();
+    auto outer_iter = [&](bool hsync) {
+        // rhs = y0 - mu div(d - b)
+        hipLaunchKernelGGL((k_sb_dmb<256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
+                           dchunk_.as<ChunkRef>(), sbs, Dv_.as<float>(), Bv_.as<float>(),
+                           T_.as<float>(), total_);
+        hipLaunchKernelGGL((k_sb_rhs<256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
+                           dchunk_.as<ChunkRef>(), dprob_.as<ProbDev>(), sbs, Y0_.as<float>(),
+                           Dv_.as<float>(), Bv_.as<float>(), Y(), T_.as<float>(), total_);
+        LC_HIP(hipGetLastError());
+        // x = cgsolver(A - mu lap, rhs, x, inner)
+        hipLaunchKernelGGL(k_cg_arm, dim3(ceil_div(np, 64)), dim3(64), 0, c_.stream(),
+                           dstate_.as<CgState>(), sbs, dmi.as<int>(), tol, xtol, np);
+        cg_setup(false, true, mi, tol, xtol);
+        for (int it = 0; it < inner; ++it) cg_iteration(hsync, true);
+        // shrink + Bregman update + convergence
+        hipLaunchKernelGGL((k_sb_shrink<256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
+                           dchunk_.as<ChunkRef>(), dprob_.as<ProbDev>(), sbs, X(), Dv_.as<float>(),
+                           Bv_.as<float>(), XO_.as<float>(), cpartial_.as<double>(), total_,
+                           aniso ? 1 : 0);
+        hipLaunchKernelGGL(k_sb_fin, dim3(np), dim3(64), 0, c_.stream(), dprob_.as<ProbDev>(),
+                           dsb_.as<SbState>(), cpartial_.as<double>());
+        LC_HIP(hipGetLastError());
+    };
+    if (graph && !host_sync) {
+        hipGraph_t g;
+        hipGraphExec_t ge;
+        LC_HIP(hipStreamBeginCapture(c_.stream(), hipStreamCaptureModeThreadLocal));
+        outer_iter(false);
+        LC_HIP(hipStreamEndCapture(c_.stream(), &g));
+        LC_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        for (int it = 0; it < outer; ++it) LC_HIP(hipGraphLaunch(ge, c_.stream()));
+        LC_HIP(hipGraphExecDestroy(ge));
+        LC_HIP(hipGraphDestroy(g));
+    } else {
+        for (int it = 0; it < outer; ++it) {
+            outer_iter(host_sync);
+            if (host_sync) {
+                std::vector<SbState> h = sb_state();
+                bool any = false;
+                for (auto &s : h) any = any || s.active;
+                if (!any) break;
+            }
+        }
+    }
+    // restore the CG rhs buffer to A^T y
+    LC_HIP(hipMemcpyAsync(Y(), Y0_.as(), fb, hipMemcpyDeviceToDevice, c_.stream()));
+    c_.sync();
+}
+
+std::vector<CgState> Batch2D::cg_state() {
+    std::vector<CgState> h(nprob());
+    LC_HIP(hipMemcpyAsync(h.data(), dstate_.as(), sizeof(CgState) * nprob(), hipMemcpyDeviceToHost,
+                          c_.stream()));
+    c_.sync();
+    return h;
+}
+
+std::vector<SbState> Batch2D::sb_state() {
+    std::vector<SbState> h(nprob());
+    LC_HIP(hipMemcpyAsync(h.data(), dsb_.as(), sizeof(SbState) * nprob(), hipMemcpyDeviceToHost,
+                          c_.stream()));
+    c_.sync();
+    return h;
+}
+
+} // namespace lc

@@ -1,0 +1,76 @@
+// Common HIP helpers for the laminocam gfx950 library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+#include "laminocam.h"
+
+namespace lc {
+
+// complex storage types
+template <typename T> struct cpx_t;
+template <> struct cpx_t<float> { using type = float2; };
+template <> struct cpx_t<double> { using type = double2; };
+template <typename T> using cpx = typename cpx_t<T>::type;
+
+template <typename C> __host__ __device__ __forceinline__ C cmake(decltype(C{}.x) a, decltype(C{}.x) b) {
+    C z;
+    z.x = a;
+    z.y = b;
+    return z;
+}
+__host__ __device__ __forceinline__ float2 operator+(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__host__ __device__ __forceinline__ float2 operator-(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__host__ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__host__ __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__host__ __device__ __forceinline__ double2 operator+(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__host__ __device__ __forceinline__ double2 operator-(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__host__ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__host__ __device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+
+// error plumbing: library code throws lc::Error; the C ABI converts to status
+struct Error : public std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define LC_HIP(call)                                                                  \
+    do {                                                                              \
+        hipError_t _e = (call);                                                       \
+        if (_e != hipSuccess)                                                         \
+            throw ::lc::Error(LC_EHIP, std::string(#call) + ": " +                    \
+                                           hipGetErrorString(_e) + " at " + __FILE__ + \
+                                           ":" + std::to_string(__LINE__));            \
+    } while (0)
+
+#define LC_REQUIRE(cond, msg)                                                         \
+    do {                                                                              \
+        if (!(cond)) throw ::lc::Error(LC_EINVAL, std::string(msg));                 \
+    } while (0)
+
+inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+// blockIdx remap so that consecutive logical blocks land on the same XCD.
+// Hardware dispatch sends block b to XCD (b mod 8); giving XCD x a contiguous
+// run of logical ids lets its private L2 share the cache lines neighbouring
+// blocks touch.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+    constexpr int NX = 8;
+    if (nb < NX) return b;
+    const int x = b % NX, q = nb / NX, rem = nb % NX;
+    const int before = x * q + (x < rem ? x : rem);
+    return before + b / NX;
+}
+
+} // namespace lc

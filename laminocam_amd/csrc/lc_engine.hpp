@@ -1,0 +1,171 @@
+// 2-D geometry, Toeplitz normal operator and the batched CG / Split-Bregman
+// engine.
+#pragma once
+
+#include <vector>
+
+#include "lc_runtime.hpp"
+
+namespace lc {
+
+// ---------------------------------------------------------------------------
+// Geometry of one 2-D polar grid (include/polar_grid2d.h:50-65) plus its
+// Toeplitz PSF spectrum (include/toeplitz2d.h:81-99).
+// ---------------------------------------------------------------------------
+struct Geom2D {
+    int64_t nang = 0, nrad = 0, N = 0, M = 0, H = 0;
+    std::vector<double> theta;
+    DBuf d_theta;      // double[nang]
+    DBuf khatT;        // float[H][M]: real spectrum of the centred PSF / (M^2 N)
+    bool psf_ready = false;
+};
+
+// next_fast_dim of include/toeplitz2d.h:48-57
+int64_t next_fast_dim(int64_t n);
+
+void geom2d_init(Ctx &c, Geom2D &g, const double *theta, int64_t nang, int64_t nrad,
+                 int64_t image_n);
+// FP64 closed-form PSF + FP64 FFT, stored as FP32 real spectrum
+void geom2d_build_psf(Ctx &c, Geom2D &g);
+// copy out the reference-layout kernel_hat (complex {M, M/2+1}, unscaled)
+void geom2d_kernel_hat(Ctx &c, Geom2D &g, float *out_host);
+
+// ---------------------------------------------------------------------------
+// device-side tables
+// ---------------------------------------------------------------------------
+struct SliceRef {
+    int prob;       // problem index
+    int slice;      // slice index within the problem
+    int64_t xoff;   // float offset of the slice in the vector buffers
+    int64_t uoff;   // complex offset of the slice's N x H intermediate
+};
+
+struct ProbDev {
+    const float *khatT;
+    int64_t off;    // float offset of the problem in the vector buffers
+    int64_t n;      // ns * N * N
+    int N, ns;
+    int chunk0, nchunk;    // element chunks (update kernels)
+    int slot0, nslot;      // Toeplitz partial-dot slots
+    float mu;              // Split-Bregman coupling (0 for plain CG)
+    int boundary;          // 0 interior-only (CPU reference), 1 zero padded
+};
+
+struct ChunkRef {
+    int prob;
+    int pad;
+    int64_t begin, end;    // absolute float offsets
+};
+
+// per-problem CG scalars, FP64 (src/conjgrad.cpp:33-76)
+struct CgState {
+    double rho;      // <r, r>
+    double pAp;
+    double alpha;
+    double beta;
+    double ynorm;
+    double res;
+    double dx;
+    double tol, xtol;
+    int active;
+    int iters;
+    int stop;        // 0 running / max_iter, 1 tol, 2 xtol, 3 pAp guard
+    int max_iter;
+};
+
+// Split-Bregman per-problem scalars (src/bregman.cpp:36-94)
+struct SbState {
+    double lambda, mu, xtol;
+    double norm_diff;
+    int active;
+    int outer;
+    int outer_max;
+    int stop;
+};
+
+constexpr int CHUNK = 4096;   // elements per update-kernel block
+
+// ---------------------------------------------------------------------------
+// One Toeplitz size class: all slices that share N (hence M).
+// ---------------------------------------------------------------------------
+struct ToepClass {
+    int N = 0, M = 0, H = 0;
+    FftPlan plan{};
+    const float2 *tw = nullptr;
+    int nslices = 0;
+    DBuf slices;            // SliceRef[nslices]
+    int ppb = 1, p1_blocks = 0;   // row pairs per block, blocks per slice
+    int B = 1, p2_blocks = 0;     // columns per block, blocks per slice
+    int slot_base = 0;
+};
+
+struct ProbSpec {
+    Geom2D *g;
+    int ns;
+};
+
+// ---------------------------------------------------------------------------
+// A batch of independent 2-D problems (each a stack of ns slices that share
+// one geometry).  Vectors of all problems are concatenated; scalars are kept
+// per problem so that a problem's iterates are the same alone and in a batch.
+// ---------------------------------------------------------------------------
+class Batch2D {
+  public:
+    Batch2D(Ctx &c, const std::vector<ProbSpec> &probs);
+    int nprob() const { return (int)probs_.size(); }
+    int64_t total() const { return total_; }
+    int64_t offset(int p) const { return hprob_[p].off; }
+    int64_t size(int p) const { return hprob_[p].n; }
+    const std::vector<ProbSpec> &specs() const { return probs_; }
+
+    float *X() const { return X_.as<float>(); }
+    float *R() const { return R_.as<float>(); }
+    float *P() const { return P_.as<float>(); }
+    float *AP() const { return AP_.as<float>(); }
+    float *Y() const { return Y_.as<float>(); }
+
+    // y = A x for every problem (Toeplitz only; src/dst are full vectors)
+    void toeplitz(const float *src, float *dst);
+
+    // CG (src/conjgrad.cpp:33-76) on every problem, starting from X(),
+    // right-hand side Y().  graph=true captures one iteration in a hipGraph
+    // and replays it; host_sync=true reads the scalars back after each
+    // kernel (the plain, one-solve-at-a-time mode).
+    void cg(const std::vector<int> &max_iter, double tol, double xtol, bool graph,
+            bool host_sync, bool x0_zero);
+    // Split-Bregman TV (src/bregman.cpp:36-94) on every problem
+    void split_bregman(const std::vector<float> &lambda, const std::vector<float> &mu,
+                       int outer, int inner, double tol, double xtol, int boundary,
+                       bool aniso, bool graph, bool host_sync);
+
+    std::vector<CgState> cg_state();
+    std::vector<SbState> sb_state();
+
+  private:
+    void build();
+    // one CG iteration (all kernels) for the current states
+    void cg_iteration(bool host_sync, bool sb);
+    void cg_setup(bool x0_zero, bool sb, const std::vector<int> &max_iter, double tol,
+                  double xtol);
+    void apply_op(const float *src, float *dst, bool sb, bool dot);
+
+    Ctx &c_;
+    std::vector<ProbSpec> probs_;
+    std::vector<ProbDev> hprob_;
+    std::vector<ChunkRef> hchunk_;
+    std::vector<std::unique_ptr<ToepClass>> classes_;
+    int64_t total_ = 0, total_u_ = 0;
+    int nchunk_ = 0, nslot_ = 0;
+    DBuf X_, R_, P_, AP_, Y_, U_, T_;
+    DBuf dprob_, dchunk_, dstate_, dsb_, partial_, cpartial_;
+    // Split-Bregman buffers (allocated on demand)
+    DBuf Dv_, Bv_, XO_, Y0_;
+    int sb_boundary_ = 0;
+    bool sb_aniso_ = false;
+};
+
+// finite differences on one volume: op 0 grad (3 components out), 1 div
+// (3 components in), 2 laplacian.  boundary 0 interior only, 1 zero padded.
+void fd_launch(Ctx &c, int op, const float *in, float *out, int n1, int n2, int n3, int boundary);
+
+} // namespace lc

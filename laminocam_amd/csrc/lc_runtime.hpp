@@ -1,0 +1,97 @@
+// Device context, buffers and twiddle cache.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "lc_common.hpp"
+#include "lc_fft.hpp"
+
+namespace lc {
+
+// RAII device allocation
+class DBuf {
+  public:
+    DBuf() = default;
+    explicit DBuf(size_t bytes) { alloc(bytes); }
+    ~DBuf() { release(); }
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    DBuf(DBuf &&o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DBuf &operator=(DBuf &&o) noexcept {
+        if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+        return *this;
+    }
+    void alloc(size_t bytes) {
+        release();
+        if (bytes == 0) return;
+        hipError_t e = hipMalloc(&p_, bytes);
+        if (e != hipSuccess) {
+            p_ = nullptr;
+            throw Error(LC_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " +
+                                       hipGetErrorString(e));
+        }
+        n_ = bytes;
+    }
+    // grow-only reallocation (contents not preserved)
+    void reserve(size_t bytes) {
+        if (bytes > n_) alloc(bytes);
+    }
+    void release() {
+        if (p_) (void)hipFree(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    template <typename T = void> T *as() const { return reinterpret_cast<T *>(p_); }
+    size_t bytes() const { return n_; }
+
+  private:
+    void *p_ = nullptr;
+    size_t n_ = 0;
+};
+
+class Ctx {
+  public:
+    explicit Ctx(int device);
+    ~Ctx();
+    int device() const { return device_; }
+    hipStream_t stream() const { return stream_; }
+    void sync();
+    void set_current() const;
+
+    // FP64-generated twiddle table of length L (cached per context)
+    template <typename T> const cpx<T> *twiddles(int L);
+
+    // scratch buffers owned by the context (grow only).  Valid until the next
+    // request for the same slot; callers on one context are serialised.
+    void *scratch(int slot, size_t bytes);
+
+    std::mutex mu;   // serialises API calls that share this context
+
+  private:
+    int device_;
+    hipStream_t stream_ = nullptr;
+    std::map<std::pair<int, int>, DBuf> tw_;
+    std::vector<DBuf> scratch_;
+};
+
+template <typename T> const cpx<T> *Ctx::twiddles(int L) {
+    auto key = std::make_pair(L, (int)sizeof(T));
+    auto it = tw_.find(key);
+    if (it != tw_.end()) return it->second.as<cpx<T>>();
+    DBuf b(sizeof(cpx<T>) * (size_t)L);
+    hipLaunchKernelGGL(k_twiddles<T>, dim3(ceil_div(L, 256)), dim3(256), 0, stream_,
+                       b.as<cpx<T>>(), L);
+    LC_HIP(hipGetLastError());
+    const cpx<T> *p = b.as<cpx<T>>();
+    tw_.emplace(key, std::move(b));
+    return p;
+}
+
+// copy helpers honouring a "pointers are on the device" flag
+void copy_in(Ctx &c, void *dst, const void *src, size_t bytes, bool src_dev);
+void copy_out(Ctx &c, void *dst, const void *src, size_t bytes, bool dst_dev);
+
+} // namespace lc

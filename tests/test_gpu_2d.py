@@ -1,0 +1,159 @@
+"""GPU parity of the 2-D hot path (Toeplitz PSF, CG, Split-Bregman, TV
+differences) against the FP64 oracle, through the C ABI."""
+import numpy as np
+import pytest
+
+from conftest import announce, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def ref_theta(n):
+    # tests/test_adjoint2d.cpp:50-55 and tests/test_sysmat_toeplitz2d.cpp:54-59
+    return (np.arange(n) - n / 2.0) * np.pi / n
+
+
+@pytest.fixture(scope="module")
+def lc():
+    import laminocam_amd
+    return laminocam_amd
+
+
+@pytest.mark.parametrize("N,nang,nrad", [(16, 7, 16), (32, 13, 32), (33, 21, 40), (64, 90, 64),
+                                         (96, 45, 96), (128, 180, 128)])
+def test_kernel_hat_matches_oracle(lc, orc64, N, nang, nrad):
+    announce(f"kernel_hat N={N} nang={nang} nrad={nrad}")
+    th = ref_theta(nang)
+    psf = lc.PointSpreadFunction2D(lc.PolarGrid2D(th, nrad), N)
+    ref = orc64.psf2d(th, nrad, N, method=0)
+    assert psf.psf_size() == ref.dims[1]
+    kh = psf.kernel_hat()
+    kr = ref.kernel_hat()
+    assert kh.shape == kr.shape
+    assert rel_l2(kh, kr) < 2e-6
+
+
+@pytest.mark.parametrize("N,nang,ns", [(16, 7, 5), (32, 13, 8), (31, 17, 2), (64, 90, 3),
+                                       (96, 60, 2), (128, 180, 2), (192, 90, 1), (256, 180, 1)])
+def test_toeplitz_matches_oracle(lc, orc64, N, nang, ns):
+    announce(f"toeplitz N={N} nang={nang} ns={ns}")
+    rng = np.random.default_rng(N * 1000 + nang)
+    th = ref_theta(nang)
+    x = rng.random((ns, N, N))
+    psf = lc.PointSpreadFunction2D(lc.PolarGrid2D(th, N), N)
+    y = psf.convolve(x)
+    yr = orc64.psf2d(th, N, N, method=0).convolve(x)
+    assert rel_l2(y, yr) < 1e-5
+
+
+def test_toeplitz_broadcast_consistency(lc):
+    # tests/test_sysmat_toeplitz2d.cpp:100-137: slice i of a stack == single slice i
+    announce("toeplitz broadcast")
+    rng = np.random.default_rng(7)
+    N = 48
+    th = ref_theta(31)
+    psf = lc.PointSpreadFunction2D(lc.PolarGrid2D(th, N), N)
+    x = rng.random((6, N, N)).astype(np.float32)
+    yb = psf.convolve(x)
+    for i in range(6):
+        assert np.array_equal(psf.convolve(x[i:i + 1]), yb[i:i + 1])
+
+
+def _problem(orc64, N, nang, ns, seed):
+    from laminocam_amd import synth
+    rng = synth.Stream(seed)
+    th = synth.angles(nang)
+    vol = np.stack([synth.ellipse_phantom(N, 10, rng) for _ in range(ns)])
+    sino = orc64.forward2d(vol, th, N)
+    sino = synth.add_gaussian_noise(sino, 0.01, rng).astype(np.float64)
+    rhs = orc64.backproj2d(sino, th, N)
+    return th, vol, rhs
+
+
+@pytest.mark.parametrize("N,nang,ns,iters", [(64, 90, 1, 30), (48, 60, 3, 20), (128, 180, 1, 50)])
+def test_cg_matches_oracle(lc, orc64, N, nang, ns, iters):
+    announce(f"cg N={N} nang={nang} ns={ns} iters={iters}")
+    th, vol, rhs = _problem(orc64, N, nang, ns, 11 + N)
+    psf = lc.PointSpreadFunction2D(lc.PolarGrid2D(th, N), N)
+    x, rep = lc.cgsolver(psf, rhs, np.zeros_like(rhs), iters, 1e-5, 1e-5)
+    P = orc64.psf2d(th, N, N, method=0)
+    xr, rr = orc64.cgsolver(P, rhs, np.zeros_like(rhs), iters, 1e-5, 1e-5)
+    assert rep["iters"] == rr["iters"]
+    assert rel_l2(x, xr) < 1e-3
+
+
+def test_cg_graph_equals_plain(lc, orc64):
+    announce("cg graph vs plain")
+    th, vol, rhs = _problem(orc64, 64, 45, 1, 5)
+    psf = lc.PointSpreadFunction2D(lc.PolarGrid2D(th, 64), 64)
+    x1, r1 = lc.cgsolver(psf, rhs, np.zeros_like(rhs), 25, 1e-5, 1e-5, graph=False)
+    x2, r2 = lc.cgsolver(psf, rhs, np.zeros_like(rhs), 25, 1e-5, 1e-5, graph=True)
+    assert r1["iters"] == r2["iters"]
+    assert np.array_equal(x1, x2)
+
+
+def test_cg_warm_start(lc, orc64):
+    announce("cg warm start")
+    th, vol, rhs = _problem(orc64, 40, 30, 2, 9)
+    psf = lc.PointSpreadFunction2D(lc.PolarGrid2D(th, 40), 40)
+    x0 = (0.1 * vol).astype(np.float32)
+    x, rep = lc.cgsolver(psf, rhs, x0, 10, 1e-5, 1e-5)
+    P = orc64.psf2d(th, 40, 40, method=0)
+    xr, rr = orc64.cgsolver(P, rhs, x0, 10, 1e-5, 1e-5)
+    assert rel_l2(x, xr) < 1e-3
+
+
+@pytest.mark.parametrize("zero_pad,aniso", [(False, False), (True, False), (True, True)])
+def test_split_bregman_matches_oracle(lc, orc64, zero_pad, aniso):
+    announce(f"split bregman zero_pad={zero_pad} aniso={aniso}")
+    N, ns = 32, 5
+    th, vol, rhs = _problem(orc64, N, 45, ns, 21)
+    psf = lc.PointSpreadFunction2D(lc.PolarGrid2D(th, N), N)
+    x, rep = lc.split_bregman(psf, rhs, np.zeros_like(rhs), 0.1, 10.0, 12, 1, 1e-5, 1e-5,
+                              zero_pad=zero_pad, aniso=aniso)
+    P = orc64.psf2d(th, N, N, method=0)
+    xr, rr = orc64.split_bregman(P, rhs, np.zeros_like(rhs), 0.1, 10.0, 12, 1, 1e-5, 1e-5,
+                                 boundary=1 if zero_pad else 0, aniso=aniso)
+    assert rep["outer"] == rr["outer"]
+    assert rel_l2(x, xr) < 1e-3
+
+
+@pytest.mark.parametrize("shape", [(1, 9, 11), (5, 16, 16), (7, 12, 20)])
+@pytest.mark.parametrize("zero_pad", [False, True])
+def test_finite_differences(lc, orc64, shape, zero_pad):
+    announce(f"fd {shape} zero_pad={zero_pad}")
+    rng = np.random.default_rng(3)
+    u = rng.random(shape).astype(np.float32)
+    b = 1 if zero_pad else 0
+    g = lc.grad_u(u, zero_pad=zero_pad)
+    assert np.allclose(g, orc64.grad(u, b), atol=1e-6)
+    gg = rng.random((3,) + shape).astype(np.float32)
+    assert np.allclose(lc.divergence(gg, zero_pad=zero_pad), orc64.div(gg, b), atol=1e-6)
+    assert np.allclose(lc.laplacian(u, zero_pad=zero_pad), orc64.laplacian(u, b), atol=1e-6)
+
+
+def test_batch_same_alone_and_in_mix(lc, orc64):
+    announce("batch alone vs mix")
+    from laminocam_amd import synth
+    rng = synth.Stream(77)
+    sizes = [(32, 30), (64, 45), (48, 90), (64, 20), (96, 60)]
+    psfs, rhss = [], []
+    for N, nang in sizes:
+        th = synth.angles(nang)
+        psfs.append(lc.PointSpreadFunction2D(lc.PolarGrid2D(th, N), N))
+        vol = synth.ellipse_phantom(N, 6, rng)[None]
+        rhss.append(psfs[-1].convolve(vol))
+    iters = [20, 15, 25, 10, 30]
+    batch = lc.Batch2D(psfs)
+    for i, r in enumerate(rhss):
+        batch.set(i, batch.Y, r)
+    batch.cg(iters, 1e-5, 1e-5, graph=True)
+    reps = batch.reports()
+    for i in range(len(sizes)):
+        alone = lc.Batch2D([psfs[i]])
+        alone.set(0, alone.Y, rhss[i])
+        alone.cg([iters[i]], 1e-5, 1e-5, graph=False)
+        assert alone.reports()[0]["iters"] == reps[i]["iters"]
+        assert np.array_equal(alone.get(0, alone.X), batch.get(i, batch.X))
+        xs, _ = lc.cgsolver(psfs[i], rhss[i], np.zeros_like(rhss[i]), iters[i], 1e-5, 1e-5)
+        assert np.array_equal(xs, batch.get(i, batch.X))
