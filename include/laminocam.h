@@ -120,6 +120,16 @@ int lc_batch2d_reports(lc_batch2d b, lc_report *reps);
 /* y = A x over every problem of the batch (buffers 0 -> 1 unless stated) */
 int lc_batch2d_sysmat(lc_batch2d b, int32_t src, int32_t dst);
 
+/* ---- Toeplitz kernel shapes per image size (no reference counterpart) ----
+ * variant[0] = row-pass shape, variant[1] = column-pass shape,
+ * variant[2] = columns per column-pass block.  lc_autotune2d times the
+ * candidate shapes on nslices slices of geometry g, installs the fastest for
+ * g's image size in the context and returns it; batches created afterwards
+ * use it.  lc_set_tuning2d installs a known choice. */
+int lc_autotune2d(lc_ctx ctx, lc_geom2d g, int32_t nslices, int32_t *variant, double *ms);
+int lc_set_tuning2d(lc_ctx ctx, int64_t image_n, const int32_t *variant);
+int lc_get_tuning2d(lc_ctx ctx, int64_t image_n, int32_t *variant);
+
 #ifdef __cplusplus
 }
 #endif

@@ -49,6 +49,38 @@ class Context:
     def sync(self):
         _lib.check(self.lib.lc_ctx_sync(self.handle))
 
+    # ---- Toeplitz kernel shapes per image size ----
+    def autotune(self, psf: "PointSpreadFunction2D", nslices: int = 16):
+        """Time the candidate kernel shapes for psf's image size on `nslices`
+        slices, install the fastest; returns (variant, ms per apply)."""
+        v = (C.c_int32 * 3)()
+        ms = C.c_double()
+        _lib.check(self.lib.lc_autotune2d(self.handle, psf.handle, int(nslices), v, C.byref(ms)))
+        return tuple(v), ms.value
+
+    def set_tuning(self, image_n: int, variant):
+        v = (C.c_int32 * 3)(*[int(a) for a in variant])
+        _lib.check(self.lib.lc_set_tuning2d(self.handle, int(image_n), v))
+
+    def get_tuning(self, image_n: int):
+        v = (C.c_int32 * 3)()
+        _lib.check(self.lib.lc_get_tuning2d(self.handle, int(image_n), v))
+        return tuple(v)
+
+    def load_tuning(self, path=None):
+        """Install a tuning table {"N": [row, col, B], ...} (JSON)."""
+        import json
+        import os
+        path = path or os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                    "tuning_gfx950.json")
+        if not os.path.exists(path):
+            return {}
+        with open(path) as f:
+            table = json.load(f)
+        for n, v in table.get("toeplitz2d", {}).items():
+            self.set_tuning(int(n), v)
+        return table
+
     def close(self):
         if self.handle:
             self.lib.lc_ctx_destroy(self.handle)

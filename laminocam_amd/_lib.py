@@ -23,7 +23,8 @@ EXPORTS = [
     "lc_sysmat2d_psf", "lc_cg2d", "lc_split_bregman2d", "lc_grad", "lc_divergence",
     "lc_laplacian", "lc_batch2d_create", "lc_batch2d_destroy", "lc_batch2d_set",
     "lc_batch2d_get", "lc_batch2d_device_ptr", "lc_batch2d_cg", "lc_batch2d_split_bregman",
-    "lc_batch2d_reports", "lc_batch2d_sysmat",
+    "lc_batch2d_reports", "lc_batch2d_sysmat", "lc_autotune2d", "lc_set_tuning2d",
+    "lc_get_tuning2d",
 ]
 
 
@@ -88,6 +89,9 @@ def load():
                                                C.c_double, u32]),
         "lc_batch2d_reports": (C.c_int, [vp, C.POINTER(LcReport)]),
         "lc_batch2d_sysmat": (C.c_int, [vp, i32, i32]),
+        "lc_autotune2d": (C.c_int, [vp, vp, i32, C.POINTER(i32), C.POINTER(C.c_double)]),
+        "lc_set_tuning2d": (C.c_int, [vp, i64, C.POINTER(i32)]),
+        "lc_get_tuning2d": (C.c_int, [vp, i64, C.POINTER(i32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

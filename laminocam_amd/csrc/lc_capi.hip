@@ -149,7 +149,7 @@ int lc_sysmat2d_psf(lc_ctx ctx, lc_geom2d g, const float *x, int64_t nslices, fl
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         ctx->c.set_current();
         Batch2D b(ctx->c, {ProbSpec{&g->g, (int)nslices}});
-        const size_t bytes = sizeof(float) * (size_t)b.total();
+        const size_t bytes = sizeof(float) * (size_t)b.size(0);   // exactly ns*N*N
         const bool dev = flags & LC_DEVICE_PTRS;
         copy_in(ctx->c, b.X(), x, bytes, dev);
         b.toeplitz(b.X(), b.AP());
@@ -166,7 +166,7 @@ int lc_cg2d(lc_ctx ctx, lc_geom2d g, const float *rhs, float *x, int64_t nslices
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         ctx->c.set_current();
         Batch2D b(ctx->c, {ProbSpec{&g->g, (int)nslices}});
-        const size_t bytes = sizeof(float) * (size_t)b.total();
+        const size_t bytes = sizeof(float) * (size_t)b.size(0);   // exactly ns*N*N
         const bool dev = flags & LC_DEVICE_PTRS;
         copy_in(ctx->c, b.Y(), rhs, bytes, dev);
         copy_in(ctx->c, b.X(), x, bytes, dev);
@@ -187,7 +187,7 @@ int lc_split_bregman2d(lc_ctx ctx, lc_geom2d g, const float *rhs, float *x, int6
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         ctx->c.set_current();
         Batch2D b(ctx->c, {ProbSpec{&g->g, (int)nslices}});
-        const size_t bytes = sizeof(float) * (size_t)b.total();
+        const size_t bytes = sizeof(float) * (size_t)b.size(0);   // exactly ns*N*N
         const bool dev = flags & LC_DEVICE_PTRS;
         copy_in(ctx->c, b.Y(), rhs, bytes, dev);
         copy_in(ctx->c, b.X(), x, bytes, dev);
@@ -353,6 +353,51 @@ int lc_batch2d_reports(lc_batch2d b, lc_report *reps) {
     return guard([&] {
         LC_REQUIRE(b && reps, "lc_batch2d_reports: NULL argument");
         std::memcpy(reps, b->reps.data(), sizeof(lc_report) * b->reps.size());
+    });
+}
+
+int lc_autotune2d(lc_ctx ctx, lc_geom2d g, int32_t nslices, int32_t *variant, double *ms) {
+    return guard([&] {
+        LC_REQUIRE(ctx && g && nslices >= 1, "lc_autotune2d: bad argument");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        ctx->c.set_current();
+        double t = 0.0;
+        const ToepVariant v = toep_autotune(ctx->c, g->g, nslices, &t);
+        if (variant) {
+            variant[0] = v.row;
+            variant[1] = v.col;
+            variant[2] = v.B;
+        }
+        if (ms) *ms = t;
+    });
+}
+
+int lc_set_tuning2d(lc_ctx ctx, int64_t image_n, const int32_t *variant) {
+    return guard([&] {
+        LC_REQUIRE(ctx && variant && image_n >= 2, "lc_set_tuning2d: bad argument");
+        LC_REQUIRE(variant[0] >= 0 && variant[0] < toep_num_row_shapes() && variant[1] >= 0 &&
+                       variant[1] < toep_num_col_shapes() && variant[2] >= 0 && variant[2] <= 16,
+                   "lc_set_tuning2d: variant out of range");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        ToepVariant v;
+        v.row = variant[0];
+        v.col = variant[1];
+        v.B = variant[2];
+        ctx->c.toep_tuning[(int)image_n] = v;
+    });
+}
+
+int lc_get_tuning2d(lc_ctx ctx, int64_t image_n, int32_t *variant) {
+    return guard([&] {
+        LC_REQUIRE(ctx && variant, "lc_get_tuning2d: bad argument");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        auto it = ctx->c.toep_tuning.find((int)image_n);
+        ToepVariant v = it != ctx->c.toep_tuning.end()
+                            ? it->second
+                            : toep_default_variant((int)next_fast_dim(2 * image_n - 1));
+        variant[0] = v.row;
+        variant[1] = v.col;
+        variant[2] = v.B;
     });
 }
 

@@ -23,7 +23,6 @@
 
 namespace lc {
 
-void toep_configure(ToepClass &k);
 void toep_launch_fwd(Ctx &c, const ToepClass &k, const CgState *st, const float *src,
                      const float *rsrc, float *pdst, float2 *U, bool fuse);
 void toep_launch_cols(Ctx &c, const ToepClass &k, const ProbDev *pd, const CgState *st,
@@ -146,12 +145,15 @@ k_cg_update(const ChunkRef *__restrict__ ch, const CgState *__restrict__ st,
     const ChunkRef c = ch[blockIdx.x];
     const CgState s = st[c.prob];
     if (!s.active) return;
-    const float alpha = (float)s.alpha;
+    // alpha stays FP64: each product is rounded once to FP32, as the FP64
+    // oracle's step is (rounding a float alpha first doubles the error that
+    // CG then amplifies)
+    const double alpha = s.alpha;
     double rr = 0.0, ss = 0.0, xx = 0.0;
     for (int64_t i = c.begin + threadIdx.x; i < c.end; i += NT) {
-        const float step = p[i] * alpha;
+        const float step = (float)((double)p[i] * alpha);
         const float xi = x[i] + step;
-        const float ri = r[i] - ap[i] * alpha;
+        const float ri = r[i] - (float)((double)ap[i] * alpha);
         x[i] = xi;
         r[i] = ri;
         rr += (double)ri * ri;
@@ -494,7 +496,10 @@ void Batch2D::build() {
         k->H = k->M / 2 + 1;
         LC_REQUIRE(fft_make_plan(k->M, k->plan), "batch: PSF size not 2^p 3^q");
         k->tw = c_.twiddles<float>(k->M);
-        toep_configure(*k);
+        {
+            auto it = c_.toep_tuning.find(k->N);
+            toep_configure(*k, it != c_.toep_tuning.end() ? it->second : toep_default_variant(k->M));
+        }
         std::vector<SliceRef> sl;
         for (int p : kv.second) {
             const int N = k->N, ns = probs_[p].ns;
@@ -526,10 +531,11 @@ void Batch2D::build() {
                 r.slice = s;
                 r.xoff = off + (int64_t)s * N * N;
                 r.uoff = uoff;
-                uoff += (int64_t)N * k->H;
+                uoff += toep_u_size(*k);
                 sl.push_back(r);
             }
-            off += d.n;
+            // keep every problem 256-byte aligned (vector loads, coalescing)
+            off += (d.n + 63) & ~(int64_t)63;
         }
         k->nslices = (int)sl.size();
         k->slot_base = hprob_[kv.second.front()].slot0;
@@ -602,6 +608,79 @@ void Batch2D::toeplitz(const float *src, float *dst) {
     LC_HIP(hipGetLastError());
     apply_op(src, dst, false, false);
     c_.sync();
+}
+
+float Batch2D::time_apply(int reps) {
+    std::vector<int> one(probs_.size(), 1);
+    DBuf mi(sizeof(int) * one.size());
+    LC_HIP(hipMemcpyAsync(mi.as(), one.data(), sizeof(int) * one.size(), hipMemcpyHostToDevice,
+                          c_.stream()));
+    hipLaunchKernelGGL(k_cg_arm, dim3(ceil_div(nprob(), 64)), dim3(64), 0, c_.stream(),
+                       dstate_.as<CgState>(), nullptr, mi.as<int>(), 0.0, 0.0, nprob());
+    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, c_.stream(), X(), 1.0f, total_);
+    LC_HIP(hipGetLastError());
+    apply_op(X(), AP(), false, false);   // warm-up
+    hipEvent_t e0, e1;
+    LC_HIP(hipEventCreate(&e0));
+    LC_HIP(hipEventCreate(&e1));
+    LC_HIP(hipEventRecord(e0, c_.stream()));
+    for (int r = 0; r < reps; ++r) apply_op(X(), AP(), false, false);
+    LC_HIP(hipEventRecord(e1, c_.stream()));
+    LC_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    LC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c_.sync();
+    return ms / (float)reps;
+}
+
+ToepVariant toep_autotune(Ctx &c, Geom2D &g, int nslices, double *best_ms) {
+    LC_REQUIRE(g.psf_ready, "autotune: geometry without PSF");
+    const int N = (int)g.N, M = (int)g.M;
+    const auto saved = c.toep_tuning;
+    auto time_variant = [&](const ToepVariant &v) -> double {
+        c.toep_tuning[N] = v;
+        Batch2D b(c, {ProbSpec{&g, nslices}});
+        return (double)b.time_apply(3);
+    };
+    ToepVariant best = toep_default_variant(M);
+    double tbest = 1e30;
+    // stage 1: column pass shape and block width (row shape 0)
+    for (int col = 0; col < toep_num_col_shapes(); ++col) {
+        const int cap = toep_col_cap(col);
+        if (M > cap) continue;
+        const int bmax = std::max(1, std::min(cap / M, 16));
+        int last = -1;
+        for (int B : {bmax, bmax / 2, bmax / 4}) {
+            if (B < 1 || B == last) continue;
+            last = B;
+            ToepVariant v;
+            v.row = 0;
+            v.col = col;
+            v.B = B;
+            try {
+                const double t = time_variant(v);
+                if (t < tbest) { tbest = t; best = v; }
+            } catch (const Error &) {
+                // shape not usable for this size
+            }
+        }
+    }
+    // stage 2: row pass shape with the chosen column shape
+    for (int row = 1; row < toep_num_row_shapes(); ++row) {
+        ToepVariant v = best;
+        v.row = row;
+        try {
+            const double t = time_variant(v);
+            if (t < tbest) { tbest = t; best = v; }
+        } catch (const Error &) {
+        }
+    }
+    c.toep_tuning = saved;
+    c.toep_tuning[N] = best;
+    if (best_ms) *best_ms = tbest;
+    return best;
 }
 
 void Batch2D::cg_setup(bool x0_zero, bool sb, const std::vector<int> &max_iter, double tol,

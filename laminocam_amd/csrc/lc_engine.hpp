@@ -88,6 +88,14 @@ constexpr int CHUNK = 4096;   // elements per update-kernel block
 // ---------------------------------------------------------------------------
 // One Toeplitz size class: all slices that share N (hence M).
 // ---------------------------------------------------------------------------
+// shape of one class as the Toeplitz kernels see it (passed by value)
+struct ToepGeom {
+    int N, M, H;
+    int B;      // columns per P2 block (U block width)
+    int nblk;   // ceil(H / B)
+    int ppb;    // row pairs per P1/P3 block
+};
+
 struct ToepClass {
     int N = 0, M = 0, H = 0;
     FftPlan plan{};
@@ -95,9 +103,22 @@ struct ToepClass {
     int nslices = 0;
     DBuf slices;            // SliceRef[nslices]
     int ppb = 1, p1_blocks = 0;   // row pairs per block, blocks per slice
-    int B = 1, p2_blocks = 0;     // columns per block, blocks per slice
+    int B = 1, nblk = 0, p2_blocks = 0;   // columns per block, blocks per slice
     int slot_base = 0;
+    ToepGeom g{};
+    ToepVariant var{};
 };
+
+// block shapes for a class; U elements per slice (independent of the shapes)
+void toep_configure(ToepClass &k, ToepVariant v);
+ToepVariant toep_default_variant(int M);
+int64_t toep_u_size(const ToepClass &k);
+int toep_num_row_shapes();
+int toep_num_col_shapes();
+int toep_col_cap(int col);
+// time candidate shapes for image size g.N on `nslices` slices; installs the
+// fastest in c.toep_tuning and returns it (best_ms: time of one apply)
+ToepVariant toep_autotune(Ctx &c, Geom2D &g, int nslices, double *best_ms);
 
 struct ProbSpec {
     Geom2D *g;
@@ -126,6 +147,8 @@ class Batch2D {
 
     // y = A x for every problem (Toeplitz only; src/dst are full vectors)
     void toeplitz(const float *src, float *dst);
+    // average time (ms) of one Toeplitz apply over all problems (X -> AP)
+    float time_apply(int reps);
 
     // CG (src/conjgrad.cpp:33-76) on every problem, starting from X(),
     // right-hand side Y().  graph=true captures one iteration in a hipGraph

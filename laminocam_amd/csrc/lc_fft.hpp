@@ -1,11 +1,13 @@
 // LDS-resident mixed-radix Stockham FFT core for gfx950.
 //
-// A workgroup transforms `nl` lines of length L held in LDS.  Element e of
-// line l lives at buf[l*ls + e*es]; `lfast` says whether the butterfly index
-// walks lines first (interleaved column layout, es = nl) or elements first
-// (contiguous row layout, es = 1).  Each stage of radix R reads its R inputs
-// into registers, synchronises, then writes the R outputs in autosorted
-// (Stockham) order back into the same buffer, so one LDS buffer suffices.
+// A workgroup transforms `nl` lines of length L held in LDS.  Two layouts:
+//   ROW (LFAST=false): element e of line l at swz(l*ls + e)   (ls >= L)
+//   COL (LFAST=true):  element e of line l at swz(e*nl + l)   (lines interleaved)
+// swz() permutes addresses inside aligned 32-element groups so the strided
+// Stockham writes spread over the 64 LDS banks (LDS buffers are allocated in
+// multiples of 32 elements).  Each stage of radix R reads its R inputs into
+// registers, synchronises, then writes the outputs in autosorted (Stockham)
+// order back into the same buffer, so one LDS buffer suffices.
 //
 // Stage with radix R after Ns = product of the previous radices:
 //   for j < L/R, k = j mod Ns:
@@ -48,6 +50,23 @@ inline bool is_2_3_smooth(int64_t n) {
     while (n % 2 == 0) n /= 2;
     while (n % 3 == 0) n /= 3;
     return n == 1;
+}
+
+// LDS elements to allocate for `n` used elements (swz stays inside 32-groups)
+__host__ __device__ constexpr int lds_round(int n) { return (n + 31) & ~31; }
+
+__device__ __forceinline__ int swz(int a) { return a ^ (((a >> 5) & 7) << 2); }
+__device__ __forceinline__ int lds_row(int line, int e, int ls) { return swz(line * ls + e); }
+__device__ __forceinline__ int lds_col(int line, int e, int nl) { return swz(e * nl + line); }
+
+// quotient of non-negative b < 2^22 by d >= 1 through an FP32 reciprocal,
+// corrected to the exact value
+__device__ __forceinline__ int fdiv(int b, int d, float inv) {
+    int q = (int)((float)b * inv);
+    int r = b - q * d;
+    if (r < 0) { --q; r += d; }
+    if (r >= d) ++q;
+    return q;
 }
 
 // multiply by SIGN * i
@@ -118,7 +137,7 @@ template <int R, int SIGN, typename C> __device__ __forceinline__ void dftR(C *v
 }
 
 template <typename T, int R, int SIGN, int NT, int MAXE, bool LFAST>
-__device__ __forceinline__ void fft_stage(cpx<T> *buf, int nl, int L, int ls, int es, int Ns,
+__device__ __forceinline__ void fft_stage(cpx<T> *buf, int nl, int L, int ls, int Ns,
                                           const cpx<T> *__restrict__ tw) {
     using C = cpx<T>;
     constexpr int MAXB = (MAXE + R - 1) / R;
@@ -126,17 +145,19 @@ __device__ __forceinline__ void fft_stage(cpx<T> *buf, int nl, int L, int ls, in
     const int nbf = nl * LR;
     const int twstep = L / (Ns * R);
     const int tid = threadIdx.x;
+    const float inv_d = LFAST ? 1.0f / (float)nl : 1.0f / (float)LR;
+    const float inv_ns = 1.0f / (float)Ns;
     C v[MAXB][R];
 #pragma unroll
     for (int t = 0; t < MAXB; ++t) {
         const int b = tid + t * NT;
         if (b < nbf) {
             int line, j;
-            if (LFAST) { j = b / nl; line = b - j * nl; }
-            else { line = b / LR; j = b - line * LR; }
-            const C *base = buf + line * ls;
+            if (LFAST) { j = fdiv(b, nl, inv_d); line = b - j * nl; }
+            else { line = fdiv(b, LR, inv_d); j = b - line * LR; }
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[t][r] = base[(j + r * LR) * es];
+            for (int r = 0; r < R; ++r)
+                v[t][r] = buf[LFAST ? lds_col(line, j + r * LR, nl) : lds_row(line, j + r * LR, ls)];
         }
     }
     __syncthreads();
@@ -145,40 +166,50 @@ __device__ __forceinline__ void fft_stage(cpx<T> *buf, int nl, int L, int ls, in
         const int b = tid + t * NT;
         if (b < nbf) {
             int line, j;
-            if (LFAST) { j = b / nl; line = b - j * nl; }
-            else { line = b / LR; j = b - line * LR; }
-            const int k = j % Ns;
+            if (LFAST) { j = fdiv(b, nl, inv_d); line = b - j * nl; }
+            else { line = fdiv(b, LR, inv_d); j = b - line * LR; }
+            const int k = j - fdiv(j, Ns, inv_ns) * Ns;
             if (k != 0) {
-#pragma unroll
-                for (int r = 1; r < R; ++r) {
-                    C w = tw[k * r * twstep];
-                    if (SIGN > 0) w.y = -w.y;
-                    v[t][r] = cmul(v[t][r], w);
+                // powers of w^k from one (radix 8: two) table loads
+                C w[R];
+                w[1] = tw[k * twstep];
+                if (SIGN > 0) w[1].y = -w[1].y;
+                if constexpr (R >= 3) w[2] = cmul(w[1], w[1]);
+                if constexpr (R >= 4) w[3] = cmul(w[2], w[1]);
+                if constexpr (R == 8) {
+                    w[4] = tw[4 * k * twstep];
+                    if (SIGN > 0) w[4].y = -w[4].y;
+                    w[5] = cmul(w[4], w[1]);
+                    w[6] = cmul(w[4], w[2]);
+                    w[7] = cmul(w[4], w[3]);
                 }
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[t][r] = cmul(v[t][r], w[r]);
             }
             dftR<R, SIGN>(v[t]);
-            C *base = buf + line * ls;
             const int o = (j - k) * R + k;   // (j/Ns)*Ns*R + k
 #pragma unroll
-            for (int r = 0; r < R; ++r) base[(o + r * Ns) * es] = v[t][r];
+            for (int r = 0; r < R; ++r)
+                buf[LFAST ? lds_col(line, o + r * Ns, nl) : lds_row(line, o + r * Ns, ls)] = v[t][r];
         }
     }
     __syncthreads();
 }
 
 // Full transform of nl lines (all threads of the block must call this).
+// Requires nl * L <= NT * MAXE.
 template <typename T, int SIGN, int NT, int MAXE, bool LFAST>
-__device__ void fft_lds(cpx<T> *buf, int nl, int ls, int es, const FftPlan &plan,
+__device__ void fft_lds(cpx<T> *buf, int nl, int ls, const FftPlan &plan,
                         const cpx<T> *__restrict__ tw) {
     int Ns = 1;
     const int L = plan.L;
     for (int s = 0; s < plan.nst; ++s) {
         const int R = plan.rad[s];
         switch (R) {
-        case 8: fft_stage<T, 8, SIGN, NT, MAXE, LFAST>(buf, nl, L, ls, es, Ns, tw); break;
-        case 4: fft_stage<T, 4, SIGN, NT, MAXE, LFAST>(buf, nl, L, ls, es, Ns, tw); break;
-        case 3: fft_stage<T, 3, SIGN, NT, MAXE, LFAST>(buf, nl, L, ls, es, Ns, tw); break;
-        default: fft_stage<T, 2, SIGN, NT, MAXE, LFAST>(buf, nl, L, ls, es, Ns, tw); break;
+        case 8: fft_stage<T, 8, SIGN, NT, MAXE, LFAST>(buf, nl, L, ls, Ns, tw); break;
+        case 4: fft_stage<T, 4, SIGN, NT, MAXE, LFAST>(buf, nl, L, ls, Ns, tw); break;
+        case 3: fft_stage<T, 3, SIGN, NT, MAXE, LFAST>(buf, nl, L, ls, Ns, tw); break;
+        default: fft_stage<T, 2, SIGN, NT, MAXE, LFAST>(buf, nl, L, ls, Ns, tw); break;
         }
         Ns *= R;
     }

@@ -52,8 +52,16 @@ class DBuf {
     size_t n_ = 0;
 };
 
+// Block shapes of the three Toeplitz passes for one image size:
+// row = P1/P3 shape index, col = P2 shape index, B = P2 columns per block.
+struct ToepVariant {
+    int row = 0, col = 0, B = 0;
+};
+
 class Ctx {
   public:
+    // per image size N: kernel shapes chosen by the autotuner (or installed)
+    std::map<int, ToepVariant> toep_tuning;
     explicit Ctx(int device);
     ~Ctx();
     int device() const { return device_; }
