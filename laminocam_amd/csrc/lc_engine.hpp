@@ -91,7 +91,8 @@ constexpr int CHUNK = 4096;   // elements per update-kernel block
 // shape of one class as the Toeplitz kernels see it (passed by value)
 struct ToepGeom {
     int N, M, H;
-    int B;      // columns per P2 block (U block width)
+    int B;      // columns per P2 block (U block width, a power of two)
+    int lsh;    // log2(B)
     int nblk;   // ceil(H / B)
     int ppb;    // row pairs per P1/P3 block
 };

@@ -148,6 +148,20 @@ __device__ __forceinline__ void fft_stage(cpx<T> *buf, int nl, int L, int ls, in
     const float inv_d = LFAST ? 1.0f / (float)nl : 1.0f / (float)LR;
     const float inv_ns = 1.0f / (float)Ns;
     C v[MAXB][R];
+    C wa[MAXB], wb[MAXB];
+    // twiddle loads first (unconditional: k = 0 reads w^0 = 1, an exact
+    // multiply), so their latency overlaps the LDS reads and the barrier
+#pragma unroll
+    for (int t = 0; t < MAXB; ++t) {
+        int b = tid + t * NT;
+        b = b < nbf ? b : nbf - 1;
+        int j;
+        if (LFAST) j = fdiv(b, nl, inv_d);
+        else j = b - fdiv(b, LR, inv_d) * LR;
+        const int k = j - fdiv(j, Ns, inv_ns) * Ns;
+        wa[t] = tw[k * twstep];
+        if constexpr (R == 8) wb[t] = tw[4 * k * twstep];
+    }
 #pragma unroll
     for (int t = 0; t < MAXB; ++t) {
         const int b = tid + t * NT;
@@ -169,15 +183,15 @@ __device__ __forceinline__ void fft_stage(cpx<T> *buf, int nl, int L, int ls, in
             if (LFAST) { j = fdiv(b, nl, inv_d); line = b - j * nl; }
             else { line = fdiv(b, LR, inv_d); j = b - line * LR; }
             const int k = j - fdiv(j, Ns, inv_ns) * Ns;
-            if (k != 0) {
-                // powers of w^k from one (radix 8: two) table loads
+            {
+                // powers of w^k from one (radix 8: two) table entries
                 C w[R];
-                w[1] = tw[k * twstep];
+                w[1] = wa[t];
                 if (SIGN > 0) w[1].y = -w[1].y;
                 if constexpr (R >= 3) w[2] = cmul(w[1], w[1]);
                 if constexpr (R >= 4) w[3] = cmul(w[2], w[1]);
                 if constexpr (R == 8) {
-                    w[4] = tw[4 * k * twstep];
+                    w[4] = wb[t];
                     if (SIGN > 0) w[4].y = -w[4].y;
                     w[5] = cmul(w[4], w[1]);
                     w[6] = cmul(w[4], w[2]);
@@ -213,6 +227,112 @@ __device__ void fft_lds(cpx<T> *buf, int nl, int ls, const FftPlan &plan,
         }
         Ns *= R;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Compile-time-length variant: the radix plan, strides and twiddle steps are
+// constants, so the index arithmetic folds to shifts and constant multiplies
+// and the stage loop unrolls.  Column layout (LFAST) requires the number of
+// lines to be a power of two, passed as its log2.
+// ---------------------------------------------------------------------------
+constexpr int ct_twos(int L) { int t = 0; while (L % 2 == 0) { L /= 2; ++t; } return t; }
+constexpr int ct_threes(int L) { int t = 0; while (L % 3 == 0) { L /= 3; ++t; } return t; }
+constexpr int ct_nst(int L) {
+    return ct_twos(L) / 3 + (ct_twos(L) % 3 ? 1 : 0) + ct_threes(L);
+}
+constexpr int ct_radix(int L, int s) {
+    const int n8 = ct_twos(L) / 3, rem = ct_twos(L) % 3;
+    if (s < n8) return 8;
+    if (rem && s == n8) return rem == 2 ? 4 : 2;
+    return 3;
+}
+
+template <typename T, int L, int R, int NS, int SIGN, int NT, int MAXE, bool LFAST>
+__device__ __forceinline__ void fft_stage_ct(cpx<T> *buf, int nl, int lsh,
+                                             const cpx<T> *__restrict__ tw) {
+    using C = cpx<T>;
+    constexpr int MAXB = (MAXE + R - 1) / R;
+    constexpr int LR = L / R;
+    constexpr int TWS = L / (NS * R);
+    const int nbf = nl * LR;
+    const int tid = threadIdx.x;
+    C v[MAXB][R];
+    C wa[MAXB], wb[MAXB];
+    // twiddle loads first (k = 0 reads w^0 = 1, an exact multiply) so their
+    // latency overlaps the LDS reads and the barrier
+    if constexpr (NS > 1) {
+#pragma unroll
+        for (int t = 0; t < MAXB; ++t) {
+            int b = tid + t * NT;
+            b = b < nbf ? b : nbf - 1;
+            const int j = LFAST ? (b >> lsh) : (b % LR);
+            const int k = j % NS;
+            wa[t] = tw[k * TWS];
+            if constexpr (R == 8) wb[t] = tw[4 * k * TWS];
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < MAXB; ++t) {
+        const int b = tid + t * NT;
+        if (b < nbf) {
+            int j, line;
+            if (LFAST) { j = b >> lsh; line = b & ((1 << lsh) - 1); }
+            else { line = b / LR; j = b - line * LR; }
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                v[t][r] = buf[LFAST ? swz(((j + r * LR) << lsh) + line) : swz(line * L + j + r * LR)];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < MAXB; ++t) {
+        const int b = tid + t * NT;
+        if (b < nbf) {
+            int j, line;
+            if (LFAST) { j = b >> lsh; line = b & ((1 << lsh) - 1); }
+            else { line = b / LR; j = b - line * LR; }
+            const int k = j % NS;
+            if constexpr (NS > 1) {
+                C w[R];
+                w[1] = wa[t];
+                if (SIGN > 0) w[1].y = -w[1].y;
+                if constexpr (R >= 3) w[2] = cmul(w[1], w[1]);
+                if constexpr (R >= 4) w[3] = cmul(w[2], w[1]);
+                if constexpr (R == 8) {
+                    w[4] = wb[t];
+                    if (SIGN > 0) w[4].y = -w[4].y;
+                    w[5] = cmul(w[4], w[1]);
+                    w[6] = cmul(w[4], w[2]);
+                    w[7] = cmul(w[4], w[3]);
+                }
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[t][r] = cmul(v[t][r], w[r]);
+            }
+            dftR<R, SIGN>(v[t]);
+            const int o = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                buf[LFAST ? swz(((o + r * NS) << lsh) + line) : swz(line * L + o + r * NS)] = v[t][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int L, int S, int NS, int SIGN, int NT, int MAXE, bool LFAST>
+__device__ __forceinline__ void fft_stages_ct(cpx<T> *buf, int nl, int lsh,
+                                              const cpx<T> *__restrict__ tw) {
+    constexpr int R = ct_radix(L, S);
+    fft_stage_ct<T, L, R, NS, SIGN, NT, MAXE, LFAST>(buf, nl, lsh, tw);
+    if constexpr (S + 1 < ct_nst(L))
+        fft_stages_ct<T, L, S + 1, NS * R, SIGN, NT, MAXE, LFAST>(buf, nl, lsh, tw);
+}
+
+// Full transform of nl lines of compile-time length L (row layout: line
+// stride L; column layout: nl = 1 << lsh lines interleaved).
+template <typename T, int L, int SIGN, int NT, int MAXE, bool LFAST>
+__device__ __forceinline__ void fft_lds_ct(cpx<T> *buf, int nl, int lsh,
+                                           const cpx<T> *__restrict__ tw) {
+    fft_stages_ct<T, L, 0, 1, SIGN, NT, MAXE, LFAST>(buf, nl, lsh, tw);
 }
 
 // FP64 twiddle generation: tw[k] = exp(-2 pi i k / L)

@@ -237,343 +237,50 @@ void geom2d_kernel_hat(Ctx &c, Geom2D &g, float *out_host) {
 }
 
 // ----------------------------------------------------------------------------
-// Toeplitz passes
-// ----------------------------------------------------------------------------
-
-__device__ __forceinline__ bool prob_active(const CgState *st, int prob) {
-    return st == nullptr || st[prob].active != 0;
-}
-
-// P1: U from pairs of real rows.  FUSE: the row source is the CG direction
-// p = r + beta p (written back) instead of a plain input.
-template <int NT, int MAXE, bool FUSE>
-__global__ void __launch_bounds__(NT)
-k_toep_rows_fwd(const SliceRef *__restrict__ sl, const CgState *__restrict__ st,
-                const float *src, const float *__restrict__ rsrc, float *pdst,
-                float2 *__restrict__ U, ToepGeom g, FftPlan plan, const float2 *__restrict__ tw) {
-    const SliceRef s = sl[blockIdx.y];
-    if (!prob_active(st, s.prob)) return;
-    float2 *buf = reinterpret_cast<float2 *>(lc_smem);
-    const int N = g.N, M = g.M, H = g.H, B = g.B, ppb = g.ppb;
-    const int a0 = 2 * blockIdx.x * ppb;
-    const int nr = min(2 * ppb, N - a0);
-    double beta = 0.0;
-    if (FUSE) beta = st[s.prob].beta;
-    const float *x = src + s.xoff + (int64_t)a0 * N;
-    const float *rr = FUSE ? rsrc + s.xoff + (int64_t)a0 * N : nullptr;
-    float *pw = FUSE ? pdst + s.xoff + (int64_t)a0 * N : nullptr;
-    const int half = M / 2;
-    if (((N | M) & 1) == 0) {
-        // items: (line l, column pair q) over ppb x M/2; loads batched first
-        constexpr int LPT = MAXE / 2;
-        const int nitems = ppb * half;
-        const float inv = 1.0f / (float)half;
-        for (int base = 0; base < nitems; base += NT * LPT) {
-            float2 va[LPT], vb[LPT];
-#pragma unroll
-            for (int t = 0; t < LPT; ++t) {
-                const int it = base + threadIdx.x + t * NT;
-                va[t] = make_float2(0.f, 0.f);
-                vb[t] = make_float2(0.f, 0.f);
-                if (it < nitems) {
-                    const int l = fdiv(it, half, inv), q = it - l * half;
-                    const int n = 2 * q;
-                    if (n < N) {
-                        const int r0 = 2 * l;
-                        if (r0 < nr) va[t] = *reinterpret_cast<const float2 *>(x + (int64_t)r0 * N + n);
-                        if (r0 + 1 < nr) vb[t] = *reinterpret_cast<const float2 *>(x + (int64_t)(r0 + 1) * N + n);
-                        if (FUSE) {
-                            float2 ra = make_float2(0.f, 0.f), rb = ra;
-                            if (r0 < nr) ra = *reinterpret_cast<const float2 *>(rr + (int64_t)r0 * N + n);
-                            if (r0 + 1 < nr) rb = *reinterpret_cast<const float2 *>(rr + (int64_t)(r0 + 1) * N + n);
-                            va[t] = make_float2((float)(ra.x + beta * va[t].x), (float)(ra.y + beta * va[t].y));
-                            vb[t] = make_float2((float)(rb.x + beta * vb[t].x), (float)(rb.y + beta * vb[t].y));
-                        }
-                    }
-                }
-            }
-#pragma unroll
-            for (int t = 0; t < LPT; ++t) {
-                const int it = base + threadIdx.x + t * NT;
-                if (it < nitems) {
-                    const int l = fdiv(it, half, inv), q = it - l * half;
-                    const int n = 2 * q;
-                    if (FUSE && n < N) {
-                        const int r0 = 2 * l;
-                        if (r0 < nr) *reinterpret_cast<float2 *>(pw + (int64_t)r0 * N + n) = va[t];
-                        if (r0 + 1 < nr) *reinterpret_cast<float2 *>(pw + (int64_t)(r0 + 1) * N + n) = vb[t];
-                    }
-                    buf[lds_row(l, n, M)] = make_float2(va[t].x, vb[t].x);
-                    buf[lds_row(l, n + 1, M)] = make_float2(va[t].y, vb[t].y);
-                }
-            }
-        }
-    } else {
-        for (int it = threadIdx.x; it < ppb * M; it += NT) {
-            const int l = it / M, n = it - l * M;
-            float re = 0.f, im = 0.f;
-            if (n < N) {
-                const int r0 = 2 * l;
-                if (r0 < nr) {
-                    re = x[(int64_t)r0 * N + n];
-                    if (FUSE) { re = (float)(rr[(int64_t)r0 * N + n] + beta * re); pw[(int64_t)r0 * N + n] = re; }
-                }
-                if (r0 + 1 < nr) {
-                    im = x[(int64_t)(r0 + 1) * N + n];
-                    if (FUSE) { im = (float)(rr[(int64_t)(r0 + 1) * N + n] + beta * im); pw[(int64_t)(r0 + 1) * N + n] = im; }
-                }
-            }
-            buf[lds_row(l, n, M)] = make_float2(re, im);
-        }
-    }
-    __syncthreads();
-    fft_lds<float, -1, NT, MAXE, false>(buf, ppb, M, plan, tw);
-    // split into half spectra and store column-blocked: items (kb, row, c)
-    float2 *u = U + s.uoff;
-    const int per_blk = nr * B;
-    const int nitems = g.nblk * per_blk;
-    const float inv_pb = 1.0f / (float)per_blk, inv_b = 1.0f / (float)B;
-    for (int it = threadIdx.x; it < nitems; it += NT) {
-        const int kb = fdiv(it, per_blk, inv_pb);
-        const int rem = it - kb * per_blk;
-        const int row = fdiv(rem, B, inv_b), c = rem - row * B;
-        const int k = kb * B + c;
-        float2 v = make_float2(0.f, 0.f);
-        if (k < H) {
-            const int l = row >> 1;
-            const float2 z = buf[lds_row(l, k, M)];
-            const float2 zc = cconj(buf[lds_row(l, k == 0 ? 0 : M - k, M)]);
-            // X_a = (Z + conj Z(-k))/2,  X_b = (Z - conj Z(-k))/(2i)
-            v = (row & 1) ? make_float2(0.5f * (z.y - zc.y), -0.5f * (z.x - zc.x))
-                          : make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
-        }
-        u[((int64_t)kb * N + a0 + row) * B + c] = v;
-    }
-}
-
-// P2: column FFT, multiply by the real spectrum, inverse FFT, crop to N rows.
-// The spectrum columns are fetched into registers before the first FFT so
-// their latency hides behind it.
-template <int NT, int MAXE, bool PF>
-__global__ void __launch_bounds__(NT)
-k_toep_cols(const SliceRef *__restrict__ sl, const ProbDev *__restrict__ pd,
-            const CgState *__restrict__ st, float2 *__restrict__ U, ToepGeom g, FftPlan plan,
-            const float2 *__restrict__ tw) {
-    const int nbx = g.nblk;
-    const int lin = blockIdx.x + blockIdx.y * gridDim.x;
-    const int logical = xcd_remap(lin, gridDim.x * gridDim.y);
-    const int sidx = logical / nbx, kb = logical - sidx * nbx;
-    const SliceRef s = sl[sidx];
-    if (!prob_active(st, s.prob)) return;
-    const int N = g.N, M = g.M, H = g.H, B = g.B;
-    const float *__restrict__ kh = pd[s.prob].khatT;
-    float2 *buf = reinterpret_cast<float2 *>(lc_smem);
-    float4 *blk4 = reinterpret_cast<float4 *>(U + s.uoff + (int64_t)kb * N * B);
-    const int nval = N * B;            // complex values in the block (kept even)
-    const int tot = M * B;             // <= NT * MAXE
-    const bool vec_k = (M & 3) == 0;
-    // spectrum columns: B*M/4 float4 <= NT*MAXE/4 (fetched before the first
-    // FFT when PF, after it otherwise)
-    constexpr int KPT = MAXE / 4;
-    float4 kv[KPT];
-    const int m4 = M >> 2;
-    const float inv_m4 = 1.0f / (float)(m4 > 0 ? m4 : 1);
-    auto fetch_k = [&]() {
-#pragma unroll
-        for (int t = 0; t < KPT; ++t) {
-            const int it = threadIdx.x + t * NT;
-            kv[t] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (it < B * m4) {
-                const int c = fdiv(it, m4, inv_m4), i4 = it - c * m4;
-                if (kb * B + c < H)
-                    kv[t] = *reinterpret_cast<const float4 *>(kh + (int64_t)(kb * B + c) * M + 4 * i4);
-            }
-        }
-    };
-    if (PF && vec_k) fetch_k();
-    // block load: (tot+1)/2 float4 <= NT*MAXE/2; rows >= N are zero
-    {
-        constexpr int LPT = MAXE / 4;     // two passes of MAXE/4 float4
-        const int n4 = nval >> 1;
-        const int t2 = (tot + 1) >> 1;
-        for (int base = 0; base < t2; base += NT * LPT) {
-            float4 v[LPT];
-#pragma unroll
-            for (int t = 0; t < LPT; ++t) {
-                const int it = base + threadIdx.x + t * NT;
-                v[t] = (it < n4) ? blk4[it] : make_float4(0.f, 0.f, 0.f, 0.f);
-            }
-#pragma unroll
-            for (int t = 0; t < LPT; ++t) {
-                const int it = base + threadIdx.x + t * NT;
-                if (it < t2) {
-                    buf[swz(2 * it)] = make_float2(v[t].x, v[t].y);
-                    if (2 * it + 1 < tot) buf[swz(2 * it + 1)] = make_float2(v[t].z, v[t].w);
-                }
-            }
-        }
-    }
-    __syncthreads();
-    fft_lds<float, -1, NT, MAXE, true>(buf, B, 0, plan, tw);
-    if (vec_k) {
-        if (!PF) fetch_k();
-#pragma unroll
-        for (int t = 0; t < KPT; ++t) {
-            const int it = threadIdx.x + t * NT;
-            if (it < B * m4) {
-                const int c = fdiv(it, m4, inv_m4), i = 4 * (it - c * m4);
-                float2 *p0 = buf + lds_col(c, i, B);
-                *p0 = cscale(*p0, kv[t].x);
-                float2 *p1 = buf + lds_col(c, i + 1, B);
-                *p1 = cscale(*p1, kv[t].y);
-                float2 *p2 = buf + lds_col(c, i + 2, B);
-                *p2 = cscale(*p2, kv[t].z);
-                float2 *p3 = buf + lds_col(c, i + 3, B);
-                *p3 = cscale(*p3, kv[t].w);
-            }
-        }
-    } else {
-        for (int it = threadIdx.x; it < B * M; it += NT) {
-            const int c = it / M, i = it - c * M;
-            const float k = (kb * B + c < H) ? kh[(int64_t)(kb * B + c) * M + i] : 0.0f;
-            float2 *p0 = buf + lds_col(c, i, B);
-            *p0 = cscale(*p0, k);
-        }
-    }
-    __syncthreads();
-    fft_lds<float, 1, NT, MAXE, true>(buf, B, 0, plan, tw);
-    {
-        const int n4 = nval >> 1;
-        for (int it = threadIdx.x; it < n4; it += NT) {
-            const float2 a = buf[swz(2 * it)], b = buf[swz(2 * it + 1)];
-            blk4[it] = make_float4(a.x, a.y, b.x, b.y);
-        }
-    }
-}
-
-// P3: rebuild full spectra of two rows, inverse FFT, write real rows.
-// DOT: also the block's partial of <p, Ap> into partial[slot].
-template <int NT, int MAXE, bool DOT>
-__global__ void __launch_bounds__(NT)
-k_toep_rows_inv(const SliceRef *__restrict__ sl, const CgState *__restrict__ st,
-                const float2 *__restrict__ U, float *__restrict__ dst,
-                const float *__restrict__ pvec, double *__restrict__ partial, int slot_base,
-                ToepGeom g, FftPlan plan, const float2 *__restrict__ tw) {
-    const SliceRef s = sl[blockIdx.y];
-    if (!prob_active(st, s.prob)) return;
-    float2 *buf = reinterpret_cast<float2 *>(lc_smem);
-    const int N = g.N, M = g.M, H = g.H, B = g.B, ppb = g.ppb;
-    const int a0 = 2 * blockIdx.x * ppb;
-    const int nr = min(2 * ppb, N - a0);
-    const int np = (nr + 1) >> 1;
-    const float2 *u = U + s.uoff;
-    const int half = M / 2;
-    // gather: items (kb, l, c); each reads rows 2l and 2l+1 at kx = kb*B + c
-    {
-        constexpr int LPT = MAXE / 2;
-        const int per_blk = np * B;
-        const int nitems = g.nblk * per_blk;
-        const float inv_pb = 1.0f / (float)per_blk, inv_b = 1.0f / (float)B;
-        for (int base = 0; base < nitems; base += NT * LPT) {
-            float2 ya[LPT], yb[LPT];
-#pragma unroll
-            for (int t = 0; t < LPT; ++t) {
-                const int it = base + threadIdx.x + t * NT;
-                ya[t] = make_float2(0.f, 0.f);
-                yb[t] = ya[t];
-                if (it < nitems) {
-                    const int kb = fdiv(it, per_blk, inv_pb);
-                    const int rem = it - kb * per_blk;
-                    const int l = fdiv(rem, B, inv_b), c = rem - l * B;
-                    const int k = kb * B + c;
-                    if (k < H) {
-                        const int64_t bse = ((int64_t)kb * N + a0 + 2 * l) * B + c;
-                        ya[t] = u[bse];
-                        if (2 * l + 1 < nr) yb[t] = u[bse + B];
-                    }
-                }
-            }
-#pragma unroll
-            for (int t = 0; t < LPT; ++t) {
-                const int it = base + threadIdx.x + t * NT;
-                if (it < nitems) {
-                    const int kb = fdiv(it, per_blk, inv_pb);
-                    const int rem = it - kb * per_blk;
-                    const int l = fdiv(rem, B, inv_b), c = rem - l * B;
-                    const int k = kb * B + c;
-                    if (k < H) {
-                        float2 a = ya[t], b = yb[t];
-                        if (k == 0 || ((M & 1) == 0 && k == half)) { a.y = 0.f; b.y = 0.f; }
-                        buf[lds_row(l, k, M)] = make_float2(a.x - b.y, a.y + b.x);
-                        if (k >= 1 && k <= M - H) buf[lds_row(l, M - k, M)] = make_float2(a.x + b.y, b.x - a.y);
-                    }
-                }
-            }
-        }
-    }
-    __syncthreads();
-    fft_lds<float, 1, NT, MAXE, false>(buf, ppb, M, plan, tw);
-    float *out = dst + s.xoff + (int64_t)a0 * N;
-    const float *pp = pvec + s.xoff + (int64_t)a0 * N;
-    double acc = 0.0;
-    if ((N & 1) == 0) {
-        const int hn = N >> 1;
-        const float inv = 1.0f / (float)hn;
-        for (int it = threadIdx.x; it < nr * hn; it += NT) {
-            const int row = fdiv(it, hn, inv), q = it - row * hn;
-            const int n = 2 * q, l = row >> 1;
-            const float2 z0 = buf[lds_row(l, n, M)], z1 = buf[lds_row(l, n + 1, M)];
-            const float2 v = (row & 1) ? make_float2(z0.y, z1.y) : make_float2(z0.x, z1.x);
-            *reinterpret_cast<float2 *>(out + (int64_t)row * N + n) = v;
-            if (DOT) {
-                const float2 pv = *reinterpret_cast<const float2 *>(pp + (int64_t)row * N + n);
-                acc += (double)pv.x * (double)v.x + (double)pv.y * (double)v.y;
-            }
-        }
-    } else {
-        for (int it = threadIdx.x; it < nr * N; it += NT) {
-            const int row = it / N, n = it - row * N;
-            const float2 z = buf[lds_row(row >> 1, n, M)];
-            const float v = (row & 1) ? z.y : z.x;
-            out[(int64_t)row * N + n] = v;
-            if (DOT) acc += (double)pp[(int64_t)row * N + n] * (double)v;
-        }
-    }
-    if (DOT) {
-        // block reduction in a fixed order
-        for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-        __shared__ double wsum[NT / 64];
-        if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = acc;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double t = 0.0;
-            for (int w = 0; w < NT / 64; ++w) t += wsum[w];
-            partial[slot_base + blockIdx.y * gridDim.x + blockIdx.x] = t;
-        }
-    }
-}
-
-// ----------------------------------------------------------------------------
 // launch configuration: a few compiled block shapes per pass; a variant picks
-// one per size class (defaults below, or the autotuner's choice)
+// one per size class (defaults below, or the autotuner's choice).  Lengths
+// with a compile-time instantiation use it; others take the runtime plan.
 // ----------------------------------------------------------------------------
 namespace {
 struct RowShape { int nt, maxe; };
-struct ColShape { int nt, maxe; bool pf; };
-constexpr RowShape ROW_SHAPES[] = {{256, 8}, {256, 16}, {512, 8}};
-constexpr ColShape COL_SHAPES[] = {{512, 16, false}, {512, 16, true}, {256, 16, false}, {256, 8, false}};
-constexpr int N_ROW_SHAPES = 3, N_COL_SHAPES = 4;
+struct ColShape { int nt, maxe; };
+constexpr RowShape ROW_SHAPES[] = {{256, 8}, {512, 8}};
+constexpr ColShape COL_SHAPES[] = {{256, 16}, {256, 8}, {512, 16}};
+constexpr int N_ROW_SHAPES = 2, N_COL_SHAPES = 3;
 } // namespace
+
+#define LC_DECL_L(L)                                                                              \
+    void toep_fwd_L##L(Ctx &, const ToepClass &, const CgState *, const float *, const float *,   \
+                       float *, float2 *, bool);                                                  \
+    void toep_cols_L##L(Ctx &, const ToepClass &, const ProbDev *, const CgState *, float2 *);    \
+    void toep_inv_L##L(Ctx &, const ToepClass &, const CgState *, const float2 *, float *,        \
+                       const float *, double *, bool);
+LC_DECL_L(0)
+LC_DECL_L(256)
+LC_DECL_L(384)
+LC_DECL_L(512)
+LC_DECL_L(768)
+LC_DECL_L(1024)
+LC_DECL_L(1536)
+LC_DECL_L(2048)
 
 int toep_num_row_shapes() { return N_ROW_SHAPES; }
 int toep_num_col_shapes() { return N_COL_SHAPES; }
 int toep_col_cap(int col) { return COL_SHAPES[col].nt * COL_SHAPES[col].maxe; }
 
+static int pow2_floor(int v) {
+    int p = 1;
+    while (2 * p <= v) p *= 2;
+    return p;
+}
+
 ToepVariant toep_default_variant(int M) {
     ToepVariant v;
     v.row = 0;
-    v.col = 0;
-    v.B = std::max(1, std::min(toep_col_cap(0) / M, 16));
+    v.col = 1;
+    v.B = 0;
+    if (M > toep_col_cap(1)) v.col = 0;
+    if (M > ROW_SHAPES[0].nt * ROW_SHAPES[0].maxe) v.row = 1;
     return v;
 }
 
@@ -588,10 +295,14 @@ void toep_configure(ToepClass &k, ToepVariant v) {
     k.p1_blocks = ceil_div(npairs, k.ppb);
     const int cap_col = toep_col_cap(v.col);
     LC_REQUIRE(k.M <= cap_col, "toeplitz: column shape too small for this size");
-    int B = v.B <= 0 ? std::max(1, std::min(cap_col / k.M, 16)) : v.B;
-    B = std::max(1, std::min(B, std::min(cap_col / k.M, 16)));
-    if ((k.N & 1) && (B & 1)) B = (B + 1 <= std::min(cap_col / k.M, 16)) ? B + 1 : B - 1;
-    LC_REQUIRE(B >= 1 && !((k.N & 1) && (B & 1)), "toeplitz: no even block width fits");
+    // columns per block: a power of two (interleaved LDS lines), even when N
+    // is odd so that N*B stays even for the float4 block copies
+    const int bmax = pow2_floor(std::min(cap_col / k.M, 16));
+    int B = v.B <= 0 ? bmax : pow2_floor(std::min(v.B, bmax));
+    if ((k.N & 1) && B < 2) B = 2;
+    LC_REQUIRE(B <= cap_col / k.M, "toeplitz: no block width fits this size");
+    int lsh = 0;
+    while ((1 << lsh) < B) ++lsh;
     v.B = B;
     k.var = v;
     k.B = B;
@@ -601,6 +312,7 @@ void toep_configure(ToepClass &k, ToepVariant v) {
     k.g.M = k.M;
     k.g.H = k.H;
     k.g.B = B;
+    k.g.lsh = lsh;
     k.g.nblk = k.nblk;
     k.g.ppb = k.ppb;
 }
@@ -608,53 +320,37 @@ void toep_configure(ToepClass &k, ToepVariant v) {
 // U elements per slice, enough for any block width <= 16
 int64_t toep_u_size(const ToepClass &k) { return ((int64_t)(k.H + 16) * k.N + 1) & ~(int64_t)1; }
 
-#define LC_ROW_DISPATCH(KERNEL, FLAG, ...)                                                 \
-    switch (k.var.row) {                                                                  \
-    case 0: hipLaunchKernelGGL((KERNEL<256, 8, FLAG>), grid, dim3(256), sh, c.stream(), __VA_ARGS__); break; \
-    case 1: hipLaunchKernelGGL((KERNEL<256, 16, FLAG>), grid, dim3(256), sh, c.stream(), __VA_ARGS__); break; \
-    default: hipLaunchKernelGGL((KERNEL<512, 8, FLAG>), grid, dim3(512), sh, c.stream(), __VA_ARGS__); break; \
+#define LC_DISPATCH_M(CALL)                                                                       \
+    switch (k.M) {                                                                                \
+    case 256: CALL(256); break;                                                                   \
+    case 384: CALL(384); break;                                                                   \
+    case 512: CALL(512); break;                                                                   \
+    case 768: CALL(768); break;                                                                   \
+    case 1024: CALL(1024); break;                                                                 \
+    case 1536: CALL(1536); break;                                                                 \
+    case 2048: CALL(2048); break;                                                                 \
+    default: CALL(0); break;                                                                      \
     }
 
 void toep_launch_fwd(Ctx &c, const ToepClass &k, const CgState *st, const float *src,
                      const float *rsrc, float *pdst, float2 *U, bool fuse) {
-    dim3 grid(k.p1_blocks, k.nslices);
-    size_t sh = sizeof(float2) * lds_round(k.ppb * k.M);
-    if (fuse) {
-        LC_ROW_DISPATCH(k_toep_rows_fwd, true, k.slices.as<SliceRef>(), st, src, rsrc, pdst, U,
-                        k.g, k.plan, k.tw);
-    } else {
-        LC_ROW_DISPATCH(k_toep_rows_fwd, false, k.slices.as<SliceRef>(), st, src, rsrc, pdst, U,
-                        k.g, k.plan, k.tw);
-    }
-    LC_HIP(hipGetLastError());
+#define LC_F(L) toep_fwd_L##L(c, k, st, src, rsrc, pdst, U, fuse)
+    LC_DISPATCH_M(LC_F)
+#undef LC_F
 }
 
 void toep_launch_cols(Ctx &c, const ToepClass &k, const ProbDev *pd, const CgState *st,
                       float2 *U) {
-    dim3 grid(k.p2_blocks, k.nslices);
-    size_t sh = sizeof(float2) * lds_round(k.B * k.M);
-    const SliceRef *sl = k.slices.as<SliceRef>();
-    switch (k.var.col) {
-    case 0: hipLaunchKernelGGL((k_toep_cols<512, 16, false>), grid, dim3(512), sh, c.stream(), sl, pd, st, U, k.g, k.plan, k.tw); break;
-    case 1: hipLaunchKernelGGL((k_toep_cols<512, 16, true>), grid, dim3(512), sh, c.stream(), sl, pd, st, U, k.g, k.plan, k.tw); break;
-    case 2: hipLaunchKernelGGL((k_toep_cols<256, 16, false>), grid, dim3(256), sh, c.stream(), sl, pd, st, U, k.g, k.plan, k.tw); break;
-    default: hipLaunchKernelGGL((k_toep_cols<256, 8, false>), grid, dim3(256), sh, c.stream(), sl, pd, st, U, k.g, k.plan, k.tw); break;
-    }
-    LC_HIP(hipGetLastError());
+#define LC_F(L) toep_cols_L##L(c, k, pd, st, U)
+    LC_DISPATCH_M(LC_F)
+#undef LC_F
 }
 
 void toep_launch_inv(Ctx &c, const ToepClass &k, const CgState *st, const float2 *U, float *dst,
                      const float *pvec, double *partial, bool dot) {
-    dim3 grid(k.p1_blocks, k.nslices);
-    size_t sh = sizeof(float2) * lds_round(k.ppb * k.M);
-    if (dot) {
-        LC_ROW_DISPATCH(k_toep_rows_inv, true, k.slices.as<SliceRef>(), st, U, dst, pvec, partial,
-                        k.slot_base, k.g, k.plan, k.tw);
-    } else {
-        LC_ROW_DISPATCH(k_toep_rows_inv, false, k.slices.as<SliceRef>(), st, U, dst, pvec,
-                        partial, k.slot_base, k.g, k.plan, k.tw);
-    }
-    LC_HIP(hipGetLastError());
+#define LC_F(L) toep_inv_L##L(c, k, st, U, dst, pvec, partial, dot)
+    LC_DISPATCH_M(LC_F)
+#undef LC_F
 }
 
 } // namespace lc
