@@ -1,0 +1,354 @@
+#!/usr/bin/env python3
+"""Batched 2-D CG benchmark on one MI355X.
+
+Workload (the batched 2-D CG configuration): 512 independent slices of uneven
+size, N drawn uniformly from {128, 192, 256, 384, 512, 768, 1024}, angle counts
+from {90, 180, 360, 720} over [0, 180), detector width N; float32
+random-ellipse phantoms (10 ellipses), sinograms from the NUFFT forward
+projector with Gaussian noise of 1% of the maximum, normalised by the maximum
+(include/mbir2d_prep.h:45-65); right-hand side = backproj2d; CG with the
+Toeplitz normal operator, 50 iterations, tol 1e-5, xtol 1e-5 (the recon_tomo
+defaults, include/config.h:365-385).  Seeded synthetic data.
+
+One step = one batched solve of all slices (every slice its own problem with
+its own CG scalars), replayed from a captured hipGraph, inputs resident in
+HBM.  Reported beside it:
+  * the plain path: the same slices solved one at a time through lc_cg2d
+    (host arrays in and out, host-synchronised iterations), on a sample;
+  * relative L2 error against the FP64 oracle on a sample of slices;
+  * roofline of the dominant Toeplitz pass from HIP events on the library's
+    stream, right after the timed steps, on the same batch;
+  * the FP32 CPU restatement of the reference path (oracle, OpenMP) timed on
+    the host on a bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+N_SET = (128, 192, 256, 384, 512, 768, 1024)
+ANG_SET = (90, 180, 360, 720)
+HBM_PEAK_GBS = 8000.0
+PASS_NAMES = ("rows_fwd", "cols", "rows_inv")
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def workload(nslices, seed_stream=4 << 16):
+    from laminocam_amd import synth
+    rng = synth.Stream(seed_stream)
+    Ns = rng.choice(N_SET, nslices).astype(int)
+    As = rng.choice(ANG_SET, nslices).astype(int)
+    return rng, Ns, As
+
+
+def build_problem(lc, synth, rng, Ns, As):
+    """geometries, phantoms, noisy normalised sinograms and right-hand sides"""
+    geoms = {}
+    for N, a in zip(Ns, As):
+        key = (int(N), int(a))
+        if key not in geoms:
+            grid = lc.PolarGrid2D(synth.angles(int(a)), int(N))
+            geoms[key] = (grid, lc.PointSpreadFunction2D(grid, int(N)))
+    groups = {}
+    for i, (N, a) in enumerate(zip(Ns, As)):
+        groups.setdefault((int(N), int(a)), []).append(i)
+    rhs = [None] * len(Ns)
+    for key, idx in sorted(groups.items()):
+        grid, _ = geoms[key]
+        N = key[0]
+        vol = np.stack([synth.ellipse_phantom(N, 10, synth.Stream((4 << 16) + 1 + i)) for i in idx])
+        sino = lc.forward2d(vol, grid)
+        for j, i in enumerate(idx):
+            s = synth.add_gaussian_noise(sino[j], 0.01, synth.Stream((4 << 16) + 100000 + i))
+            sino[j] = s / np.float32(np.max(np.abs(s)))
+        y = lc.backproj2d(sino, grid, (len(idx), N, N))
+        for j, i in enumerate(idx):
+            rhs[i] = y[j:j + 1]
+    return geoms, rhs
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def cpu_baseline(args, geoms_keys, iters_by_n, Ns, rhs_by_n):
+    """FP32 CPU restatement of the reference path (Toeplitz CG), one slice per
+    image size for a few iterations; the time of the whole workload follows
+    from the per-iteration time of each size and the iteration counts the
+    batched solve ran."""
+    from oracle.oracle import Oracle
+    o32 = Oracle(32)
+    threads = int(o32.lib.or_num_threads())
+    t_iter = {}
+    spent = 0.0
+    for N in sorted(rhs_by_n):
+        y, theta = rhs_by_n[N]
+        psf = o32.psf2d(theta, N, N, method=0)
+        k = args.cpu_iters
+        t0 = time.perf_counter()
+        o32.cgsolver(psf, y, np.zeros_like(y), k, 0.0, 0.0)
+        dt = time.perf_counter() - t0
+        t_iter[N] = dt / k
+        spent += dt
+    total = sum(t_iter[int(N)] * iters_by_n[i] for i, N in enumerate(Ns))
+    return {
+        "value": len(Ns) / total,
+        "unit": "slices/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"FP32 C restatement of the reference CPU path (Toeplitz CG), OpenMP "
+                   f"{threads} threads: one slice per image size N in {sorted(t_iter)}, "
+                   f"{args.cpu_iters} CG iterations each ({spent:.1f} s); workload time = "
+                   f"sum over the {len(Ns)} slices of (seconds per iteration at its N) x "
+                   f"(iterations the batched solve ran)"),
+        "sec_per_iter": {str(k): v for k, v in t_iter.items()},
+    }
+
+
+def pmc_probe(args):
+    """Short run for rocprofv3 --pmc: a device copy of known size (calibration)
+    and a few batched Toeplitz applies of the benchmark workload."""
+    import ctypes as C
+    import laminocam_amd as lc
+    from laminocam_amd import synth
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = 256 << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0   # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    log(f"copy probe: {nbytes} bytes x3")
+    ctx = lc.default_context()
+    rng, Ns, As = workload(args.slices)
+    geoms, rhs = build_problem(lc, synth, rng, Ns, As)
+    batch = lc.Batch2D([geoms[(int(N), int(a))][1] for N, a in zip(Ns, As)], ctx=ctx)
+    for i in range(len(Ns)):
+        batch.set(i, batch.Y, rhs[i])
+    prof = batch.profile_passes(args.kernel_reps)
+    ctx.sync()
+    for r in prof:
+        log(f"pass N={r['N']} {PASS_NAMES[r['pass_']]} ms={r['ms']:.4f} bytes={r['bytes']:.0f}")
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--slices", type=int, default=512)
+    ap.add_argument("--iters", type=int, default=50)
+    ap.add_argument("--tol", type=float, default=1e-5)
+    ap.add_argument("--xtol", type=float, default=1e-5)
+    ap.add_argument("--plain-slices", type=int, default=64,
+                    help="slices of the workload solved one at a time for the plain-path rate")
+    ap.add_argument("--err-slices", type=int, default=3,
+                    help="smallest slices compared with the FP64 oracle")
+    ap.add_argument("--short-iters", type=int, default=5,
+                    help="CG iterations of the short-horizon error check")
+    ap.add_argument("--cpu-iters", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--kernel-reps", type=int, default=10)
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="only run a calibration copy and a few Toeplitz applies (for rocprofv3 --pmc)")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus > 1 or world > 1:
+        print(f"bench.py: this build measures one GPU; --gpus {max(args.gpus, world)} is not "
+              f"supported: the sharded multi-GPU path is not built", file=sys.stderr)
+        sys.exit(2)
+
+    if args.pmc_probe:
+        pmc_probe(args)
+        return
+
+    import laminocam_amd as lc
+    from laminocam_amd import synth
+
+    t_setup = time.perf_counter()
+    ctx = lc.default_context()
+    rng, Ns, As = workload(args.slices)
+    geoms, rhs = build_problem(lc, synth, rng, Ns, As)
+    psfs = [geoms[(int(N), int(a))][1] for N, a in zip(Ns, As)]
+    batch = lc.Batch2D(psfs, ctx=ctx)
+    for i in range(len(Ns)):
+        batch.set(i, batch.Y, rhs[i])
+    ctx.sync()
+    log(f"setup {time.perf_counter() - t_setup:.1f} s: {len(Ns)} slices, {len(geoms)} geometries")
+
+    # ---- timed batched solves ----
+    for _ in range(args.warmup):
+        batch.cg(args.iters, args.tol, args.xtol, graph=True)
+    ctx.sync()
+    step_s = []
+    for _ in range(args.steps):
+        ctx.sync()
+        t0 = time.perf_counter()
+        batch.cg(args.iters, args.tol, args.xtol, graph=True)
+        ctx.sync()
+        step_s.append(time.perf_counter() - t0)
+    reps = batch.reports()
+    iters = np.array([r["iters"] for r in reps])
+    ms_step = 1e3 * float(np.mean(step_s))
+    value = len(Ns) / float(np.mean(step_s))
+    log(f"batched: {ms_step:.1f} ms/step, {value:.1f} slices/s, iterations mean {iters.mean():.1f}")
+    xb = [batch.get(i, batch.X) for i in range(len(Ns))]
+
+    # ---- dominant kernel: per-launch HIP-event timing on the library stream ----
+    prof = batch.profile_passes(args.kernel_reps)
+    dom = max(prof, key=lambda r: r["ms"])
+    achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_toeplitz_batched_cg.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            t = json.load(f)
+        key = f"N{dom['N']}_{PASS_NAMES[dom['pass_']]}"
+        traffic = t.get("bytes_per_launch", {}).get(key)
+    roofline = {
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+        "kernel": f"toeplitz {PASS_NAMES[dom['pass_']]} pass, N={dom['N']} class "
+                  f"({dom['nslices']} slices per launch)",
+        "launch_ms": round(dom["ms"], 4), "algorithmic_bytes": dom["bytes"],
+    }
+    log(f"dominant: {roofline['kernel']}: {dom['ms']:.3f} ms/launch, {achieved:.0f} GB/s")
+
+    # ---- plain path: one solve at a time through the C ABI, host arrays ----
+    order = np.arange(len(Ns))
+    plain_idx = order[:min(args.plain_slices, len(Ns))]
+    lat = []
+    max_diff = 0.0
+    for i in plain_idx:
+        t0 = time.perf_counter()
+        x, _ = lc.cgsolver(psfs[i], rhs[i], np.zeros_like(rhs[i]), args.iters, args.tol, args.xtol)
+        lat.append(time.perf_counter() - t0)
+        max_diff = max(max_diff, float(np.max(np.abs(x - xb[i]))))
+    lat = np.array(lat)
+    # the plain rate for the whole workload: per-slice time by image size
+    t_by_n = {}
+    for i, t in zip(plain_idx, lat):
+        t_by_n.setdefault(int(Ns[i]), []).append(t)
+    t_mean = {n: float(np.mean(v)) for n, v in t_by_n.items()}
+    if len(t_mean) == len(set(int(n) for n in Ns)):
+        plain_total = sum(t_mean[int(n)] for n in Ns)
+    else:
+        plain_total = float(np.mean(lat)) * len(Ns)
+    plain_rate = len(Ns) / plain_total
+    log(f"plain: {plain_rate:.1f} slices/s over {len(plain_idx)} slices, max |x_plain - x_batched| "
+        f"= {max_diff:.3e}")
+
+    # ---- relative error vs the FP64 oracle on the smallest slices ----
+    # Long CG solves on these under-sampled systems amplify rounding: two FP64
+    # runs whose right-hand sides differ by 1e-15 relative drift apart too.
+    # That drift (fp64_floor) and the FP32 restatement of the reference path
+    # are reported beside the GPU error; at a short horizon the solve is
+    # well conditioned and the GPU error is the arithmetic error alone.
+    from oracle.oracle import Oracle
+    o64, o32 = Oracle(64), Oracle(32)
+    small = sorted(range(len(Ns)), key=lambda i: (Ns[i], As[i], i))[:args.err_slices]
+    errs, errs32, floor64, errs_short = [], [], [], []
+    for i in small:
+        N, a = int(Ns[i]), int(As[i])
+        th = synth.angles(a)
+        y = rhs[i]
+        y64 = y.astype(np.float64)
+        p64 = o64.psf2d(th, N, N, method=0)
+        x64, _ = o64.cgsolver(p64, y64, np.zeros(y.shape), args.iters, args.tol, args.xtol)
+        yp = y64 * (1.0 + 1e-15 * synth.Stream((4 << 16) + 200000 + i).normal(y.size).reshape(y.shape))
+        x64p, _ = o64.cgsolver(p64, yp, np.zeros(y.shape), args.iters, args.tol, args.xtol)
+        x32, _ = o32.cgsolver(o32.psf2d(th, N, N, method=0), y, np.zeros_like(y), args.iters,
+                              args.tol, args.xtol)
+        errs.append(rel_l2(xb[i], x64))
+        floor64.append(rel_l2(x64p, x64))
+        errs32.append(rel_l2(x32, x64))
+        xs, _ = lc.cgsolver(psfs[i], y, np.zeros_like(y), args.short_iters, args.tol, args.xtol)
+        x64s, _ = o64.cgsolver(p64, y64, np.zeros(y.shape), args.short_iters, args.tol, args.xtol)
+        errs_short.append(rel_l2(xs, x64s))
+    log(f"rel err vs FP64 oracle ({args.iters} it): GPU {['%.2e' % e for e in errs]}; FP32 CPU "
+        f"port {['%.2e' % e for e in errs32]}; FP64 rerun floor {['%.2e' % e for e in floor64]}; "
+        f"GPU at {args.short_iters} it {['%.2e' % e for e in errs_short]}")
+
+    # ---- CPU baseline ----
+    cpu = None
+    if not args.no_cpu:
+        rhs_by_n = {}
+        for i in range(len(Ns)):
+            N = int(Ns[i])
+            if N not in rhs_by_n or As[i] < rhs_by_n[N][2]:
+                rhs_by_n[N] = (rhs[i], synth.angles(int(As[i])), int(As[i]))
+        rhs_by_n = {k: v[:2] for k, v in rhs_by_n.items()}
+        cpu = cpu_baseline(args, None, iters, Ns, rhs_by_n)
+        log(f"cpu baseline: {cpu['value']:.3f} slices/s on {cpu['cores']} threads")
+
+    out = {
+        "metric": "slices/sec + p50/p99 latency, batched/served vs plain GPU CG at <=1e-3 rel err",
+        "value": round(value, 2),
+        "unit": "slices/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 dots/norms/PSF/twiddles)",
+        "data": "synthetic (seeded random-ellipse phantoms, NUFFT sinograms, 1% Gaussian noise)",
+        "config": {
+            "workload": f"batched 2-D CG: {len(Ns)} slices, N in {list(N_SET)}, angles in "
+                        f"{list(ANG_SET)} over [0,180), nrad = N, {args.iters} iterations, "
+                        f"tol {args.tol:g}, xtol {args.xtol:g}, one hipGraph replay per iteration",
+            "slices": len(Ns), "iters": args.iters,
+        },
+        "latency_ms": {"p50": round(1e3 * float(np.median(step_s)), 3),
+                       "p99": round(1e3 * float(np.percentile(step_s, 99)), 3),
+                       "note": "batched: every slice is submitted at the start of a step and "
+                               "completes with it"},
+        "plain": {"slices_per_s": round(plain_rate, 2), "sample_slices": int(len(plain_idx)),
+                  "latency_ms": {"p50": round(1e3 * float(np.median(lat)), 3),
+                                 "p99": round(1e3 * float(np.percentile(lat, 99)), 3)},
+                  "max_abs_diff_vs_batched": max_diff},
+        "speedup_vs_plain": round(value / plain_rate, 2),
+        "cg_iterations": {"mean": round(float(iters.mean()), 2), "min": int(iters.min()),
+                          "max": int(iters.max())},
+        "rel_err_vs_fp64": {"slices": [f"N={int(Ns[i])},nang={int(As[i])}" for i in small],
+                            "iters": args.iters,
+                            "gpu": [float("%.3e" % e) for e in errs],
+                            "fp32_cpu_port": [float("%.3e" % e) for e in errs32],
+                            "fp64_floor": [float("%.3e" % e) for e in floor64],
+                            "short_iters": args.short_iters,
+                            "gpu_short": [float("%.3e" % e) for e in errs_short],
+                            "note": "fp64_floor: FP64 oracle rerun on a right-hand side perturbed "
+                                    "by 1e-15 relative; long CG solves of these under-sampled "
+                                    "systems drift by this much from rounding alone"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernel_passes": [{"N": r["N"], "pass": PASS_NAMES[r["pass_"]], "ms": round(r["ms"], 4),
+                           "GBps": round(r["bytes"] / (r["ms"] * 1e-3) / 1e9, 1)} for r in prof],
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

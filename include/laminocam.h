@@ -32,10 +32,12 @@ extern "C" {
 /* flags */
 #define LC_DEVICE_PTRS 0x1u  /* array arguments are device pointers */
 #define LC_GRAPH 0x2u        /* replay solver iterations from a captured hipGraph */
-#define LC_X0_ZERO 0x4u      /* the initial guess is zero (skip A x0) */
+#define LC_X0_ZERO 0x4u      /* start from x0 = 0: x is cleared, not read (skips A x0) */
 #define LC_TV_ZERO_PAD 0x8u  /* TV differences zero padded (GPU reference) instead
                                 of interior only (CPU reference) */
 #define LC_TV_ANISO 0x10u    /* per-component soft threshold instead of isotropic */
+#define LC_GEOM_LAZY_PSF 0x20u /* lc_geom2d_create: build the PSF on first use
+                                 (geometries used only for projection) */
 
 typedef struct lc_ctx_s *lc_ctx;
 typedef struct lc_geom2d_s *lc_geom2d;
@@ -77,6 +79,19 @@ int lc_psf2d_kernel_hat(lc_ctx ctx, lc_geom2d g, float *out);
  * include/toeplitz2d.h:101-125) */
 int lc_sysmat2d_psf(lc_ctx ctx, lc_geom2d g, const float *x, int64_t nslices, float *y,
                     uint32_t flags);
+/* forward2d (src/projection.cpp:146-160, include/projection.h:56-58):
+ * sino {nslices, nang, nrad} = Re(fftshift1(ifft1(ifftshift1(T2 vol))))/nrad
+ * from vol {nslices, N, N}; T2 the type-2 NUFFT on the polar grid. */
+int lc_forward2d(lc_ctx ctx, lc_geom2d g, const float *vol, int64_t nslices, float *sino,
+                 uint32_t flags);
+/* backproj2d (src/projection.cpp:168-182, include/projection.h:59-61):
+ * vol = Re(T1(ifftshift1(fft1(fftshift1(sino)))))/nrad */
+int lc_backproj2d(lc_ctx ctx, lc_geom2d g, const float *sino, int64_t nslices, float *vol,
+                  uint32_t flags);
+/* sysmat2d(x, PolarGrid2D) (src/gradient.cpp:59-69, include/projection.h:64-65):
+ * Re(T1(T2 x))/nrad, the NUFFT form of the normal operator (no detector FFT) */
+int lc_sysmat2d_nufft(lc_ctx ctx, lc_geom2d g, const float *x, int64_t nslices, float *y,
+                      uint32_t flags);
 
 /* ---- solvers on one stack (global alpha/beta over the stack, as the
  * reference's cgsolver on an Array{n1,N,N}) ----
@@ -119,6 +134,13 @@ int lc_batch2d_split_bregman(lc_batch2d b, const float *lambda, const float *mu,
 int lc_batch2d_reports(lc_batch2d b, lc_report *reps);
 /* y = A x over every problem of the batch (buffers 0 -> 1 unless stated) */
 int lc_batch2d_sysmat(lc_batch2d b, int32_t src, int32_t dst);
+/* Per-launch HIP-event timing of the three Toeplitz passes as one CG iteration
+ * issues them (no reference counterpart; measurement only).  Writes one row of
+ * 5 doubles per (image size class, pass): N, pass (0 rows+FFT, 1 columns,
+ * 2 inverse rows), slices, ms per launch, algorithmic HBM bytes per launch.
+ * Leaves x and y untouched; clobbers the other solver buffers. */
+int lc_batch2d_profile(lc_batch2d b, int32_t reps, int32_t max_rows, double *rows,
+                       int32_t *nrows);
 
 /* ---- Toeplitz kernel shapes per image size (no reference counterpart) ----
  * variant[0] = row-pass shape, variant[1] = column-pass shape,

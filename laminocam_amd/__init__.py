@@ -6,6 +6,7 @@ Python mirror of the reference's C++ API over the C ABI of liblaminocam.so
   PolarGrid2D            include/polar_grid2d.h:35-66
   PointSpreadFunction2D  include/toeplitz2d.h:40-126
   sysmat2d               src/gradient.cpp:49-79
+  forward2d / backproj2d src/projection.cpp:146-182
   cgsolver               src/conjgrad.cpp:33-76
   split_bregman          src/bregman.cpp:36-94
   grad_u / divergence / laplacian   src/finitediff.cpp:26-90, include/bregman.h:52-55
@@ -22,7 +23,8 @@ from . import _lib
 from ._lib import (LC_DEVICE_PTRS, LC_GRAPH, LC_TV_ANISO, LC_TV_ZERO_PAD, LC_X0_ZERO,
                    LcError, LcReport)
 
-__all__ = ["Context", "PolarGrid2D", "PointSpreadFunction2D", "sysmat2d", "cgsolver",
+__all__ = ["Context", "PolarGrid2D", "PointSpreadFunction2D", "sysmat2d", "forward2d",
+           "backproj2d", "cgsolver",
            "split_bregman", "grad_u", "divergence", "laplacian", "Batch2D", "LcError",
            "default_context"]
 
@@ -94,9 +96,11 @@ class Context:
 
 
 def default_context() -> Context:
+    """Context on device 0 with the shipped gfx950 kernel-shape table installed."""
     global _default_ctx
     if _default_ctx is None:
         _default_ctx = Context(0)
+        _default_ctx.load_tuning()
     return _default_ctx
 
 
@@ -117,6 +121,68 @@ class PolarGrid2D:
 
     def dims(self):
         return (1, len(self.theta), self.ncols)
+
+    def _geometry(self, image_n: int, ctx: "Context | None" = None):
+        """device geometry for N x N images on this grid (PSF built on demand)"""
+        ctx = ctx or default_context()
+        key = (id(ctx), int(image_n))
+        cache = self.__dict__.setdefault("_geoms", {})
+        if key not in cache:
+            h = C.c_void_p()
+            _lib.check(ctx.lib.lc_geom2d_create(
+                ctx.handle, self.theta.ctypes.data_as(C.POINTER(C.c_double)), len(self.theta),
+                self.ncols, int(image_n), _lib.LC_GEOM_LAZY_PSF, C.byref(h)))
+            cache[key] = _Geometry(ctx, h)
+        return cache[key]
+
+
+class _Geometry:
+    """owning wrapper of an lc_geom2d handle"""
+
+    def __init__(self, ctx, handle):
+        self.ctx, self.handle = ctx, handle
+
+    def __del__(self):
+        try:
+            if self.handle:
+                self.ctx.lib.lc_geom2d_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def forward2d(volume, grid: PolarGrid2D, ctx: "Context | None" = None):
+    """forward2d(volume, grid) (src/projection.cpp:146-160): sinogram
+    {nslices, nangles, nradial} = Re(fftshift1(ifft1(ifftshift1(T2 volume))))/nradial
+    for square slices {nslices, N, N}."""
+    v = _f32(volume)
+    if v.ndim == 2:
+        v = v[None]
+    if v.ndim != 3 or v.shape[1] != v.shape[2]:
+        raise ValueError(f"forward2d: volume must be {{nslices, N, N}}, got {v.shape}")
+    ns, N = v.shape[0], v.shape[1]
+    g = grid._geometry(N, ctx)
+    out = np.empty((ns, grid.nangles(), grid.nradial()), np.float32)
+    _lib.check(g.ctx.lib.lc_forward2d(g.ctx.handle, g.handle, _ptr(v), ns, _ptr(out), 0))
+    return out
+
+
+def backproj2d(sinogram, grid: PolarGrid2D, recon_dims, ctx: "Context | None" = None):
+    """backproj2d(sinogram, grid, recon_dims) (src/projection.cpp:168-182):
+    Re(T1(ifftshift1(fft1(fftshift1(sinogram)))))/nradial on {nslices, N, N}."""
+    s = _f32(sinogram)
+    if s.ndim == 2:
+        s = s[None]
+    ns, na, nr = s.shape
+    if (na, nr) != (grid.nangles(), grid.nradial()):
+        raise ValueError(f"backproj2d: sinogram {s.shape} does not match the grid")
+    d = tuple(int(x) for x in recon_dims)
+    if len(d) != 3 or d[0] != ns or d[1] != d[2]:
+        raise ValueError(f"backproj2d: recon_dims must be ({ns}, N, N), got {d}")
+    g = grid._geometry(d[1], ctx)
+    out = np.empty(d, np.float32)
+    _lib.check(g.ctx.lib.lc_backproj2d(g.ctx.handle, g.handle, _ptr(s), ns, _ptr(out), 0))
+    return out
 
 
 class PointSpreadFunction2D:
@@ -171,9 +237,23 @@ class PointSpreadFunction2D:
             pass
 
 
-def sysmat2d(x, psf: PointSpreadFunction2D):
-    """sysmat2d(x, psf) = psf.convolve(x) (src/gradient.cpp:76-79)"""
-    return psf.convolve(x)
+def sysmat2d(x, op, ctx: "Context | None" = None):
+    """sysmat2d(x, psf) = psf.convolve(x) (src/gradient.cpp:76-79);
+    sysmat2d(x, grid) = Re(T1(T2 x))/nradial (src/gradient.cpp:59-69)."""
+    if isinstance(op, PointSpreadFunction2D):
+        return op.convolve(x)
+    if not isinstance(op, PolarGrid2D):
+        raise TypeError("sysmat2d: second argument must be a PolarGrid2D or PointSpreadFunction2D")
+    v = _f32(x)
+    if v.ndim == 2:
+        v = v[None]
+    if v.ndim != 3 or v.shape[1] != v.shape[2]:
+        raise ValueError(f"sysmat2d: x must be {{nslices, N, N}}, got {v.shape}")
+    g = op._geometry(v.shape[1], ctx)
+    out = np.empty_like(v)
+    _lib.check(g.ctx.lib.lc_sysmat2d_nufft(g.ctx.handle, g.handle, _ptr(v), v.shape[0],
+                                           _ptr(out), 0))
+    return out
 
 
 def _stack(x, N):
@@ -316,6 +396,25 @@ class Batch2D:
         reps = (LcReport * n)()
         _lib.check(self.lib.lc_batch2d_reports(self.handle, reps))
         return [r.as_dict() for r in reps]
+
+    def profile_passes(self, reps=5):
+        """HIP-event duration of every Toeplitz pass launch as a CG iteration
+        issues it: list of dicts (N, pass, nslices, ms, bytes).  Clobbers the
+        solver state and Ax, keeps x and y."""
+        cap = 3 * 64
+        rows = (C.c_double * (5 * cap))()
+        n = C.c_int32()
+        _lib.check(self.lib.lc_batch2d_profile(self.handle, int(reps), cap, rows, C.byref(n)))
+        out = []
+        for i in range(n.value):
+            r = rows[5 * i:5 * i + 5]
+            out.append(dict(N=int(r[0]), pass_=int(r[1]), nslices=int(r[2]), ms=r[3], bytes=r[4]))
+        return out
+
+    def device_ptr(self, i, which):
+        p = C.c_void_p()
+        _lib.check(self.lib.lc_batch2d_device_ptr(self.handle, i, which, C.byref(p)))
+        return p.value
 
     def close(self):
         if self.handle:

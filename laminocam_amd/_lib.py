@@ -15,15 +15,16 @@ LC_GRAPH = 0x2
 LC_X0_ZERO = 0x4
 LC_TV_ZERO_PAD = 0x8
 LC_TV_ANISO = 0x10
+LC_GEOM_LAZY_PSF = 0x20
 
 # every symbol declared in include/laminocam.h
 EXPORTS = [
     "lc_abi_version", "lc_last_error", "lc_ctx_create", "lc_ctx_destroy", "lc_ctx_sync",
     "lc_geom2d_create", "lc_geom2d_destroy", "lc_geom2d_info", "lc_psf2d_kernel_hat",
-    "lc_sysmat2d_psf", "lc_cg2d", "lc_split_bregman2d", "lc_grad", "lc_divergence",
+    "lc_sysmat2d_psf", "lc_forward2d", "lc_backproj2d", "lc_sysmat2d_nufft", "lc_cg2d", "lc_split_bregman2d", "lc_grad", "lc_divergence",
     "lc_laplacian", "lc_batch2d_create", "lc_batch2d_destroy", "lc_batch2d_set",
     "lc_batch2d_get", "lc_batch2d_device_ptr", "lc_batch2d_cg", "lc_batch2d_split_bregman",
-    "lc_batch2d_reports", "lc_batch2d_sysmat", "lc_autotune2d", "lc_set_tuning2d",
+    "lc_batch2d_reports", "lc_batch2d_sysmat", "lc_batch2d_profile", "lc_autotune2d", "lc_set_tuning2d",
     "lc_get_tuning2d",
 ]
 
@@ -71,6 +72,9 @@ def load():
         "lc_geom2d_info": (C.c_int, [vp, C.POINTER(i64)]),
         "lc_psf2d_kernel_hat": (C.c_int, [vp, vp, f32p]),
         "lc_sysmat2d_psf": (C.c_int, [vp, vp, vp, i64, vp, u32]),
+        "lc_forward2d": (C.c_int, [vp, vp, vp, i64, vp, u32]),
+        "lc_backproj2d": (C.c_int, [vp, vp, vp, i64, vp, u32]),
+        "lc_sysmat2d_nufft": (C.c_int, [vp, vp, vp, i64, vp, u32]),
         "lc_cg2d": (C.c_int, [vp, vp, vp, vp, i64, i64, C.c_double, C.c_double, u32,
                               C.POINTER(LcReport)]),
         "lc_split_bregman2d": (C.c_int, [vp, vp, vp, vp, i64, C.c_double, C.c_double, i64,
@@ -89,6 +93,7 @@ def load():
                                                C.c_double, u32]),
         "lc_batch2d_reports": (C.c_int, [vp, C.POINTER(LcReport)]),
         "lc_batch2d_sysmat": (C.c_int, [vp, i32, i32]),
+        "lc_batch2d_profile": (C.c_int, [vp, i32, i32, f64p, C.POINTER(i32)]),
         "lc_autotune2d": (C.c_int, [vp, vp, i32, C.POINTER(i32), C.POINTER(C.c_double)]),
         "lc_set_tuning2d": (C.c_int, [vp, i64, C.POINTER(i32)]),
         "lc_get_tuning2d": (C.c_int, [vp, i64, C.POINTER(i32)]),

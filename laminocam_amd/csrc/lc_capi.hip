@@ -92,7 +92,6 @@ int lc_ctx_sync(lc_ctx ctx) {
 
 int lc_geom2d_create(lc_ctx ctx, const double *theta, int64_t nang, int64_t nrad,
                      int64_t image_n, uint32_t flags, lc_geom2d *out) {
-    (void)flags;
     return guard([&] {
         LC_REQUIRE(ctx && theta && out, "lc_geom2d_create: NULL argument");
         std::lock_guard<std::mutex> lk(ctx->c.mu);
@@ -101,7 +100,7 @@ int lc_geom2d_create(lc_ctx ctx, const double *theta, int64_t nang, int64_t nrad
         g->ctx = ctx;
         try {
             geom2d_init(ctx->c, g->g, theta, nang, nrad, image_n);
-            geom2d_build_psf(ctx->c, g->g);
+            if (!(flags & LC_GEOM_LAZY_PSF)) geom2d_build_psf(ctx->c, g->g);
             ctx->c.sync();
         } catch (...) {
             delete g;
@@ -156,6 +155,51 @@ int lc_sysmat2d_psf(lc_ctx ctx, lc_geom2d g, const float *x, int64_t nslices, fl
         copy_out(ctx->c, y, b.AP(), bytes, dev);
         ctx->c.sync();
     });
+}
+
+// forward2d (mode 0) / backproj2d (1) / sysmat2d NUFFT form (2); host arrays
+// are staged through context scratch
+static int project2d(lc_ctx ctx, lc_geom2d g, const float *in, int64_t nslices, float *out,
+                     uint32_t flags, int mode) {
+    return guard([&] {
+        LC_REQUIRE(ctx && g && in && out, "projector: NULL argument");
+        LC_REQUIRE(nslices >= 1 && nslices <= (1 << 20), "projector: bad nslices");
+        LC_REQUIRE(g->ctx == ctx, "projector: geometry of another context");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        ctx->c.set_current();
+        const Geom2D &gg = g->g;
+        const int64_t nv = nslices * gg.N * gg.N, ns_ = nslices * gg.nang * gg.nrad;
+        const int64_t nin = mode == 1 ? ns_ : nv, nout = mode == 0 ? ns_ : nv;
+        const bool dev = flags & LC_DEVICE_PTRS;
+        const float *din = in;
+        float *dout = out;
+        if (!dev) {
+            float *buf = (float *)ctx->c.scratch(2, sizeof(float) * (size_t)(nin + nout));
+            copy_in(ctx->c, buf, in, sizeof(float) * nin, false);
+            din = buf;
+            dout = buf + nin;
+        }
+        if (mode == 0) forward2d_dev(ctx->c, g->g, din, dout, (int)nslices);
+        else if (mode == 1) backproj2d_dev(ctx->c, g->g, din, dout, (int)nslices);
+        else sysmat2d_nufft_dev(ctx->c, g->g, din, dout, (int)nslices);
+        if (!dev) copy_out(ctx->c, out, dout, sizeof(float) * nout, false);
+        ctx->c.sync();
+    });
+}
+
+int lc_forward2d(lc_ctx ctx, lc_geom2d g, const float *vol, int64_t nslices, float *sino,
+                 uint32_t flags) {
+    return project2d(ctx, g, vol, nslices, sino, flags, 0);
+}
+
+int lc_backproj2d(lc_ctx ctx, lc_geom2d g, const float *sino, int64_t nslices, float *vol,
+                  uint32_t flags) {
+    return project2d(ctx, g, sino, nslices, vol, flags, 1);
+}
+
+int lc_sysmat2d_nufft(lc_ctx ctx, lc_geom2d g, const float *x, int64_t nslices, float *y,
+                      uint32_t flags) {
+    return project2d(ctx, g, x, nslices, y, flags, 2);
 }
 
 int lc_cg2d(lc_ctx ctx, lc_geom2d g, const float *rhs, float *x, int64_t nslices,
@@ -398,6 +442,26 @@ int lc_get_tuning2d(lc_ctx ctx, int64_t image_n, int32_t *variant) {
         variant[0] = v.row;
         variant[1] = v.col;
         variant[2] = v.B;
+    });
+}
+
+int lc_batch2d_profile(lc_batch2d b, int32_t reps, int32_t max_rows, double *rows,
+                       int32_t *nrows) {
+    return guard([&] {
+        LC_REQUIRE(b && rows && nrows && reps >= 1, "lc_batch2d_profile: bad argument");
+        std::lock_guard<std::mutex> lk(b->ctx->c.mu);
+        b->ctx->c.set_current();
+        const auto t = b->b->profile_passes(reps);
+        LC_REQUIRE((int)t.size() <= max_rows, "lc_batch2d_profile: max_rows too small");
+        for (size_t i = 0; i < t.size(); ++i) {
+            double *r = rows + 5 * i;
+            r[0] = t[i].N;
+            r[1] = t[i].pass;
+            r[2] = t[i].nslices;
+            r[3] = t[i].ms;
+            r[4] = t[i].bytes;
+        }
+        *nrows = (int32_t)t.size();
     });
 }
 

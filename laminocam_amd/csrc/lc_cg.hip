@@ -80,12 +80,13 @@ template <int NT>
 __global__ void __launch_bounds__(NT)
 k_cg_init(const ChunkRef *__restrict__ ch, const CgState *__restrict__ st,
           const float *__restrict__ y, const float *__restrict__ ap, float *__restrict__ r,
-          float *__restrict__ p, double *__restrict__ cpart, int x0_zero) {
+          float *__restrict__ p, float *__restrict__ x, double *__restrict__ cpart, int x0_zero) {
     const ChunkRef c = ch[blockIdx.x];
     if (!st[c.prob].active) return;
     double rr = 0.0, yy = 0.0;
     for (int64_t i = c.begin + threadIdx.x; i < c.end; i += NT) {
         const float yi = y[i];
+        if (x0_zero) x[i] = 0.0f;   // x0 = 0: x is not read, only cleared
         const float ri = x0_zero ? yi : yi - ap[i];
         r[i] = ri;
         p[i] = 0.0f;
@@ -475,8 +476,9 @@ void Batch2D::build() {
     // group slices by N into classes
     std::map<int, std::vector<int>> by_n;
     for (int p = 0; p < np; ++p) {
-        const Geom2D *g = probs_[p].g;
-        LC_REQUIRE(g && g->psf_ready, "batch: geometry without PSF");
+        Geom2D *g = probs_[p].g;
+        LC_REQUIRE(g, "batch: NULL geometry");
+        if (!g->psf_ready) geom2d_build_psf(c_, *g);
         LC_REQUIRE(probs_[p].ns >= 1, "batch: ns must be >= 1");
         by_n[(int)g->N].push_back(p);
     }
@@ -635,8 +637,57 @@ float Batch2D::time_apply(int reps) {
     return ms / (float)reps;
 }
 
+std::vector<Batch2D::PassTiming> Batch2D::profile_passes(int reps) {
+    LC_REQUIRE(reps >= 1, "profile: reps must be >= 1");
+    std::vector<int> one(probs_.size(), 1);
+    DBuf mi(sizeof(int) * one.size());
+    LC_HIP(hipMemcpyAsync(mi.as(), one.data(), sizeof(int) * one.size(), hipMemcpyHostToDevice,
+                          c_.stream()));
+    hipLaunchKernelGGL(k_cg_arm, dim3(ceil_div(nprob(), 64)), dim3(64), 0, c_.stream(),
+                       dstate_.as<CgState>(), nullptr, mi.as<int>(), 0.0, 0.0, nprob());
+    LC_HIP(hipGetLastError());
+    CgState *st = dstate_.as<CgState>();
+    float2 *U = U_.as<float2>();
+    const int nk = (int)classes_.size();
+    std::vector<hipEvent_t> ev(4 * nk);
+    for (auto &e : ev) LC_HIP(hipEventCreate(&e));
+    std::vector<double> acc(3 * nk, 0.0);
+    for (int r = 0; r <= reps; ++r) {   // r = 0 warms up
+        for (int i = 0; i < nk; ++i) {
+            const ToepClass &k = *classes_[i];
+            LC_HIP(hipEventRecord(ev[4 * i], c_.stream()));
+            toep_launch_fwd(c_, k, st, P(), R(), P(), U, true);
+            LC_HIP(hipEventRecord(ev[4 * i + 1], c_.stream()));
+            toep_launch_cols(c_, k, dprob_.as<ProbDev>(), st, U);
+            LC_HIP(hipEventRecord(ev[4 * i + 2], c_.stream()));
+            toep_launch_inv(c_, k, st, U, AP(), P(), partial_.as<double>(), true);
+            LC_HIP(hipEventRecord(ev[4 * i + 3], c_.stream()));
+        }
+        LC_HIP(hipEventSynchronize(ev[4 * nk - 1]));
+        if (r == 0) continue;
+        for (int i = 0; i < nk; ++i)
+            for (int q = 0; q < 3; ++q) {
+                float ms = 0.0f;
+                LC_HIP(hipEventElapsedTime(&ms, ev[4 * i + q], ev[4 * i + q + 1]));
+                acc[3 * i + q] += ms;
+            }
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    std::vector<PassTiming> out;
+    for (int i = 0; i < nk; ++i) {
+        const ToepClass &k = *classes_[i];
+        const double N = k.N, H = k.H, M = k.M, ns = k.nslices;
+        const double bytes[3] = {ns * (12.0 * N * N + 8.0 * N * H),
+                                 ns * 16.0 * N * H + 4.0 * M * H,
+                                 ns * (8.0 * N * H + 8.0 * N * N)};
+        for (int q = 0; q < 3; ++q)
+            out.push_back(PassTiming{k.N, q, k.nslices, acc[3 * i + q] / reps, bytes[q]});
+    }
+    return out;
+}
+
 ToepVariant toep_autotune(Ctx &c, Geom2D &g, int nslices, double *best_ms) {
-    LC_REQUIRE(g.psf_ready, "autotune: geometry without PSF");
+    if (!g.psf_ready) geom2d_build_psf(c, g);
     const int N = (int)g.N, M = (int)g.M;
     const auto saved = c.toep_tuning;
     auto time_variant = [&](const ToepVariant &v) -> double {
@@ -691,7 +742,7 @@ void Batch2D::cg_setup(bool x0_zero, bool sb, const std::vector<int> &max_iter, 
     CgState *st = dstate_.as<CgState>();
     if (!x0_zero) apply_op(X(), AP(), sb, false);
     hipLaunchKernelGGL((k_cg_init<256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
-                       dchunk_.as<ChunkRef>(), st, Y(), AP(), R(), P(), cpartial_.as<double>(),
+                       dchunk_.as<ChunkRef>(), st, Y(), AP(), R(), P(), X(), cpartial_.as<double>(),
                        x0_zero ? 1 : 0);
     LC_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_cg_fin0, dim3(nprob()), dim3(64), 0, c_.stream(), dprob_.as<ProbDev>(),

@@ -9,6 +9,21 @@
 namespace lc {
 
 // ---------------------------------------------------------------------------
+// NUFFT state of one 2-D polar grid: fine grid size, kernel parameters and
+// the deconvolution factors (-1)^m / psi_hat(2 pi m / nf) for m in [-N/2, N/2)
+// ---------------------------------------------------------------------------
+struct Nufft2D {
+    int nf = 0;           // fine grid edge (2N rounded up to 2^p 3^q)
+    int w = 0;            // kernel width in cells
+    double beta = 0.0;    // kernel shape
+    DBuf corr;            // float[N]
+    FftPlan plan_nf{};    // fine-grid FFT
+    FftPlan plan_det{};   // detector FFT (valid if det_fast)
+    bool det_fast = false;
+    bool ready = false;
+};
+
+// ---------------------------------------------------------------------------
 // Geometry of one 2-D polar grid (include/polar_grid2d.h:50-65) plus its
 // Toeplitz PSF spectrum (include/toeplitz2d.h:81-99).
 // ---------------------------------------------------------------------------
@@ -18,7 +33,16 @@ struct Geom2D {
     DBuf d_theta;      // double[nang]
     DBuf khatT;        // float[H][M]: real spectrum of the centred PSF / (M^2 N)
     bool psf_ready = false;
+    Nufft2D nu;
 };
+
+// NUFFT projectors on device buffers (src/projection.cpp:146-182):
+// forward2d: vol (ns x N x N) -> sino (ns x nang x nrad); backproj2d: reverse
+void nufft2d_init(Ctx &c, Geom2D &g);
+void forward2d_dev(Ctx &c, Geom2D &g, const float *vol, float *sino, int ns);
+void backproj2d_dev(Ctx &c, Geom2D &g, const float *sino, float *vol, int ns);
+// sysmat2d(x, grid) = Re(T1(T2 x)) / nrad (src/gradient.cpp:59-69)
+void sysmat2d_nufft_dev(Ctx &c, Geom2D &g, const float *x, float *y, int ns);
 
 // next_fast_dim of include/toeplitz2d.h:48-57
 int64_t next_fast_dim(int64_t n);
@@ -150,6 +174,16 @@ class Batch2D {
     void toeplitz(const float *src, float *dst);
     // average time (ms) of one Toeplitz apply over all problems (X -> AP)
     float time_apply(int reps);
+    // HIP-event time of every launch of the three Toeplitz passes in the form a
+    // CG iteration issues them (P1 fused with p = r + beta p, P3 with the
+    // <p, Ap> partials), averaged over reps.  Overwrites P, AP and the solver
+    // state, not X or Y.
+    struct PassTiming {
+        int N, pass, nslices;
+        double ms;      // average duration of one launch
+        double bytes;   // algorithmic HBM bytes of one launch
+    };
+    std::vector<PassTiming> profile_passes(int reps);
 
     // CG (src/conjgrad.cpp:33-76) on every problem, starting from X(),
     // right-hand side Y().  graph=true captures one iteration in a hipGraph

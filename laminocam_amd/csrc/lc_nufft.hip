@@ -1,0 +1,587 @@
+// 2-D polar-grid NUFFT projectors on gfx950 (forward2d / backproj2d,
+// src/projection.cpp:146-182).
+//
+// Type 1 (backproj2d):  f_m = sum_j c_j exp(+i m.w_j),   w_j = r_k (cos th_i, sin th_i)
+// Type 2 (forward2d):   c_j = sum_m f_m exp(-i m.w_j)
+// with modes m in [-N/2, N/2) (CMCL order; include/nufft.h:64-95).  The
+// method is the exponential-of-semicircle NUFFT (Barnett, Magland and
+// af Klinteberg, SIAM J. Sci. Comput. 41(5), 2019): spread / interpolate
+// with psi(u) = exp(beta (sqrt(1 - (2u/w)^2) - 1)) on a fine grid of nf = 2N
+// (rounded to 2^p 3^q) cells, FFT the fine grid, divide by the kernel's
+// Fourier transform.  The fine grid is indexed with its centre at nf/2 so a
+// radial line of points is a straight segment of cells; the resulting
+// (-1)^m phase is folded into the correction factors.
+//
+// Spreading is deterministic: each workgroup owns a T x T tile of the fine
+// grid, gathers the points whose footprint reaches the tile (ray / box
+// intersection per angle, angles tested 64 at a time), and each of its waves
+// accumulates its own share of the rays into a private LDS copy of the tile,
+// one point at a time with the 8 x 8 footprint spread over the 64 lanes.  The
+// copies are summed in a fixed order at the end.
+#include <cmath>
+
+#include "lc_engine.hpp"
+
+
+namespace lc {
+
+extern __shared__ __align__(16) char lc_smem[];
+
+constexpr int NU_W = 8;        // kernel width (cells); 64 lanes cover the footprint
+constexpr int NU_T = 32;       // spread tile edge (cells)
+constexpr int NU_WAVES = 4;    // waves per spread workgroup
+
+__device__ __forceinline__ float es_kernel(float z, float beta) {
+    const float s = 1.0f - z * z;
+    return s > 0.0f ? __expf(beta * (sqrtf(s) - 1.0f)) : 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// detector transforms along each sinogram row (length n = nrad)
+// backproj: G = ifftshift1(fft1(fftshift1(y)))      (complex out)
+// forward:  y = Re(fftshift1(ifft1(ifftshift1(C)))) / n
+// fftshift: out[k] = in[(k + n/2) % n]; ifftshift: d = n/2 + (n odd)
+// ---------------------------------------------------------------------------
+template <int SIGN, int NT, int MAXE>
+__global__ void __launch_bounds__(NT)
+k_det_fft(const float *__restrict__ real_in, const float2 *__restrict__ cplx_in,
+          float2 *__restrict__ cplx_out, float *__restrict__ real_out, int nrows, int n, int nl,
+          FftPlan plan, const float2 *__restrict__ tw, float scale) {
+    float2 *buf = reinterpret_cast<float2 *>(lc_smem);
+    const int row0 = blockIdx.x * nl;
+    const int nloc = min(nl, nrows - row0);
+    const int dfw = n / 2, dinv = n / 2 + (n & 1);
+    // input shift: fftshift for backproj (SIGN -1), ifftshift for forward (+1)
+    const int din = SIGN < 0 ? dfw : dinv;
+    const int dout = SIGN < 0 ? dinv : dfw;
+    for (int idx = threadIdx.x; idx < nl * n; idx += NT) {
+        const int l = idx / n, k = idx - l * n;
+        float2 v = make_float2(0.f, 0.f);
+        if (l < nloc) {
+            int src = k + din;
+            if (src >= n) src -= n;
+            const int64_t q = (int64_t)(row0 + l) * n + src;
+            v = SIGN < 0 ? make_float2(real_in[q], 0.f) : cplx_in[q];
+        }
+        buf[lds_row(l, k, n)] = v;
+    }
+    __syncthreads();
+    fft_lds<float, SIGN, NT, MAXE, false>(buf, nl, n, plan, tw);
+    for (int idx = threadIdx.x; idx < nloc * n; idx += NT) {
+        const int l = idx / n, k = idx - l * n;
+        int src = k + dout;
+        if (src >= n) src -= n;
+        const float2 v = buf[lds_row(l, src, n)];
+        const int64_t q = (int64_t)(row0 + l) * n + k;
+        if (SIGN < 0) cplx_out[q] = v;
+        else real_out[q] = v.x * scale;
+    }
+}
+
+// direct DFT fallback for detector widths that are not 2^p 3^q
+template <int SIGN>
+__global__ void k_det_dft(const float *__restrict__ real_in, const float2 *__restrict__ cplx_in,
+                          float2 *__restrict__ cplx_out, float *__restrict__ real_out, int nrows,
+                          int n, const float2 *__restrict__ tw, float scale) {
+    const int row = blockIdx.x;   // grid: one block per row
+    const int dfw = n / 2, dinv = n / 2 + (n & 1);
+    const int din = SIGN < 0 ? dfw : dinv, dout = SIGN < 0 ? dinv : dfw;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        // output index k reads transform bin kk
+        int kk = k + dout;
+        if (kk >= n) kk -= n;
+        double re = 0.0, im = 0.0;
+        for (int j = 0; j < n; ++j) {
+            int src = j + din;
+            if (src >= n) src -= n;
+            const int64_t q = (int64_t)row * n + src;
+            const float2 v = SIGN < 0 ? make_float2(real_in[q], 0.f) : cplx_in[q];
+            float2 w = tw[(int)(((int64_t)j * kk) % n)];
+            if (SIGN > 0) w.y = -w.y;
+            re += (double)v.x * w.x - (double)v.y * w.y;
+            im += (double)v.x * w.y + (double)v.y * w.x;
+        }
+        const int64_t q = (int64_t)row * n + k;
+        if (SIGN < 0) cplx_out[q] = make_float2((float)re, (float)im);
+        else real_out[q] = (float)re * scale;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Type-1 spreading (deterministic tile gather)
+// ---------------------------------------------------------------------------
+struct SpreadGeom {
+    int nang, nrad, nf, ntx;   // ntx tiles per fine-grid edge
+    double dr;                 // 2 pi / nrad
+    double sc;                 // nf / (2 pi): radians -> cells
+    float beta;
+};
+
+// intersect the ray of angle (c, s) with box [x0,x1]x[y0,y1] (cells, shifted
+// grid) -> radial index range [k0, k1] (empty if k0 > k1)
+__device__ __forceinline__ void ray_box(double c, double s, const SpreadGeom &g, double x0,
+                                        double x1, double y0, double y1, int &k0, int &k1) {
+    // g(k) = nf/2 + sc * r_k * (c, s),  r_k = (k + 1/2) dr - pi
+    double rlo = -1e300, rhi = 1e300;
+    const double h = 0.5 * g.nf;
+    const double ax = g.sc * c, ay = g.sc * s;
+    if (fabs(ax) > 1e-12) {
+        double a = (x0 - h) / ax, b = (x1 - h) / ax;
+        if (a > b) { double t = a; a = b; b = t; }
+        rlo = fmax(rlo, a);
+        rhi = fmin(rhi, b);
+    } else if (h < x0 || h > x1) {
+        k0 = 1;
+        k1 = 0;
+        return;
+    }
+    if (fabs(ay) > 1e-12) {
+        double a = (y0 - h) / ay, b = (y1 - h) / ay;
+        if (a > b) { double t = a; a = b; b = t; }
+        rlo = fmax(rlo, a);
+        rhi = fmin(rhi, b);
+    } else if (h < y0 || h > y1) {
+        k0 = 1;
+        k1 = 0;
+        return;
+    }
+    if (rlo > rhi) {
+        k0 = 1;
+        k1 = 0;
+        return;
+    }
+    // r = (k + 1/2) dr - pi  ->  k = (r + pi)/dr - 1/2 ; widen by one
+    const double klo = (rlo + M_PI) / g.dr - 0.5, khi = (rhi + M_PI) / g.dr - 0.5;
+    k0 = max(0, (int)floor(klo) - 1);
+    k1 = min(g.nrad - 1, (int)ceil(khi) + 1);
+}
+
+// grid: (ntx * ntx, nslices).  C: per slice nang x nrad complex (detector-
+// transformed sinogram).  fine: per slice nf x nf complex.
+__global__ void __launch_bounds__(64 * NU_WAVES)
+k_spread2d(const float2 *__restrict__ C, float2 *__restrict__ fine, const double *__restrict__ theta,
+           SpreadGeom g) {
+    float2 *acc = reinterpret_cast<float2 *>(lc_smem);   // [NU_WAVES][T*T]
+    const int slice = blockIdx.y;
+    const int tile = blockIdx.x;
+    const int ty = tile / g.ntx, tx = tile - ty * g.ntx;
+    const int x0 = tx * NU_T, y0 = ty * NU_T;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    float2 *my = acc + wave * NU_T * NU_T;
+    for (int i = lane; i < NU_T * NU_T; i += 64) my[i] = make_float2(0.f, 0.f);
+    __syncthreads();
+    const float2 *Cs = C + (int64_t)slice * g.nang * g.nrad;
+    const int dx = lane & 7, dy = lane >> 3;
+    const double hw = 0.5 * NU_W;
+    const float inv_hw = 1.0f / (float)hw;
+    // periodic images of the tile box that can receive wrapped contributions
+    for (int oy = -1; oy <= 1; ++oy) {
+        for (int ox = -1; ox <= 1; ++ox) {
+            const double bx0 = x0 + ox * g.nf - hw - 1.0, bx1 = x0 + NU_T - 1 + ox * g.nf + hw + 1.0;
+            const double by0 = y0 + oy * g.nf - hw - 1.0, by1 = y0 + NU_T - 1 + oy * g.nf + hw + 1.0;
+            if (bx1 < 0.0 || bx0 > g.nf || by1 < 0.0 || by0 > g.nf) continue;
+            // waves take 64-angle chunks round robin; lanes test one angle each
+            for (int a0 = wave * 64; a0 < g.nang; a0 += 64 * NU_WAVES) {
+                const int ai = a0 + lane;
+                int k0 = 1, k1 = 0;
+                double cth = 0.0, sth = 0.0;
+                if (ai < g.nang) {
+                    sincos(theta[ai], &sth, &cth);
+                    ray_box(cth, sth, g, bx0, bx1, by0, by1, k0, k1);
+                }
+                unsigned long long hits = __ballot(k0 <= k1);
+                while (hits) {
+                    const int src = __ffsll((long long)hits) - 1;
+                    hits &= hits - 1;
+                    const int rk0 = __shfl(k0, src, 64), rk1 = __shfl(k1, src, 64);
+                    const double rc = __shfl(cth, src, 64), rs = __shfl(sth, src, 64);
+                    const int ang = a0 + src;
+                    for (int k = rk0; k <= rk1; ++k) {
+                        const double r = ((double)k + 0.5) * g.dr - M_PI;
+                        const double gx = 0.5 * g.nf + g.sc * r * rc - ox * g.nf;
+                        const double gy = 0.5 * g.nf + g.sc * r * rs - oy * g.nf;
+                        const int lx0 = (int)ceil(gx - hw), ly0 = (int)ceil(gy - hw);
+                        const int cx = lx0 + dx - x0, cy = ly0 + dy - y0;
+                        if (cx >= 0 && cx < NU_T && cy >= 0 && cy < NU_T) {
+                            const float zx = (float)((double)(lx0 + dx) - gx) * inv_hw;
+                            const float zy = (float)((double)(ly0 + dy) - gy) * inv_hw;
+                            const float wgt = es_kernel(zx, g.beta) * es_kernel(zy, g.beta);
+                            const float2 cv = Cs[(int64_t)ang * g.nrad + k];
+                            float2 &a = my[cy * NU_T + cx];
+                            a.x += wgt * cv.x;
+                            a.y += wgt * cv.y;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // fixed-order sum of the wave copies; each tile cell has one writer
+    float2 *fs = fine + (int64_t)slice * g.nf * g.nf;
+    for (int i = threadIdx.x; i < NU_T * NU_T; i += 64 * NU_WAVES) {
+        float2 s = acc[i];
+        for (int w = 1; w < NU_WAVES; ++w) {
+            s.x += acc[w * NU_T * NU_T + i].x;
+            s.y += acc[w * NU_T * NU_T + i].y;
+        }
+        const int cy = i / NU_T, cx = i - cy * NU_T;
+        const int gxx = x0 + cx, gyy = y0 + cy;
+        if (gxx < g.nf && gyy < g.nf) fs[(int64_t)gyy * g.nf + gxx] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Type-2 interpolation: one thread per point
+// ---------------------------------------------------------------------------
+__global__ void k_interp2d(const float2 *__restrict__ fine, float2 *__restrict__ C,
+                           const double *__restrict__ theta, SpreadGeom g, int npts) {
+    const int slice = blockIdx.y;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npts) return;
+    const int ang = p / g.nrad, k = p - ang * g.nrad;
+    double s, c;
+    sincos(theta[ang], &s, &c);
+    const double r = ((double)k + 0.5) * g.dr - M_PI;
+    const double gx = 0.5 * g.nf + g.sc * r * c, gy = 0.5 * g.nf + g.sc * r * s;
+    const double hw = 0.5 * NU_W;
+    const int lx0 = (int)ceil(gx - hw), ly0 = (int)ceil(gy - hw);
+    const float inv_hw = 1.0f / (float)hw;
+    float wx[NU_W], wy[NU_W];
+#pragma unroll
+    for (int d = 0; d < NU_W; ++d) {
+        wx[d] = es_kernel((float)((double)(lx0 + d) - gx) * inv_hw, g.beta);
+        wy[d] = es_kernel((float)((double)(ly0 + d) - gy) * inv_hw, g.beta);
+    }
+    const float2 *fs = fine + (int64_t)slice * g.nf * g.nf;
+    float re = 0.f, im = 0.f;
+#pragma unroll
+    for (int a = 0; a < NU_W; ++a) {
+        int yy = ly0 + a;
+        yy = yy < 0 ? yy + g.nf : (yy >= g.nf ? yy - g.nf : yy);
+        const float2 *row = fs + (int64_t)yy * g.nf;
+        float rr = 0.f, ri = 0.f;
+#pragma unroll
+        for (int b = 0; b < NU_W; ++b) {
+            int xx = lx0 + b;
+            xx = xx < 0 ? xx + g.nf : (xx >= g.nf ? xx - g.nf : xx);
+            const float2 v = row[xx];
+            rr += wx[b] * v.x;
+            ri += wx[b] * v.y;
+        }
+        re += wy[a] * rr;
+        im += wy[a] * ri;
+    }
+    C[(int64_t)slice * npts + p] = make_float2(re, im);
+}
+
+// ---------------------------------------------------------------------------
+// fine-grid transforms (pruned to the N x N modes)
+// fine row r, column l <-> cell (x = l, y = r); mode m <-> fine index m mod nf
+// V: nf rows x N mode columns (row major), per slice
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int mode_to_fine(int a, int N, int nf) {
+    const int m = a - N / 2;
+    return m < 0 ? m + nf : m;
+}
+
+// type 1, pass A: FFT+ of every fine row, keep the N mode columns -> V
+template <int NT, int MAXE>
+__global__ void __launch_bounds__(NT)
+k_fine_rows_t1(const float2 *__restrict__ fine, float2 *__restrict__ V, int nf, int N, int nl,
+               FftPlan plan, const float2 *__restrict__ tw) {
+    float2 *buf = reinterpret_cast<float2 *>(lc_smem);
+    const int slice = blockIdx.y;
+    const int row0 = blockIdx.x * nl;
+    const int nloc = min(nl, nf - row0);
+    const float2 *fs = fine + (int64_t)slice * nf * nf;
+    for (int idx = threadIdx.x; idx < nl * nf; idx += NT) {
+        const int l = idx / nf, e = idx - l * nf;
+        buf[lds_row(l, e, nf)] = l < nloc ? fs[(int64_t)(row0 + l) * nf + e] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    fft_lds<float, 1, NT, MAXE, false>(buf, nl, nf, plan, tw);
+    float2 *vs = V + (int64_t)slice * nf * N;
+    for (int idx = threadIdx.x; idx < nloc * N; idx += NT) {
+        const int l = idx / N, b = idx - l * N;
+        vs[(int64_t)(row0 + l) * N + b] = buf[lds_row(l, mode_to_fine(b, N, nf), nf)];
+    }
+}
+
+// type 1, pass B: FFT+ down each mode column, keep the N mode rows,
+// deconvolve, real part / nrad -> image (N x N real)
+template <int NT, int MAXE>
+__global__ void __launch_bounds__(NT)
+k_fine_cols_t1(const float2 *__restrict__ V, float *__restrict__ img, const float *__restrict__ corr,
+               int nf, int N, int B, float scale, FftPlan plan, const float2 *__restrict__ tw) {
+    float2 *buf = reinterpret_cast<float2 *>(lc_smem);
+    const int slice = blockIdx.y;
+    const int col0 = blockIdx.x * B;
+    const float2 *vs = V + (int64_t)slice * nf * N;
+    for (int idx = threadIdx.x; idx < nf * B; idx += NT) {
+        const int i = idx / B, c = idx - i * B;
+        buf[lds_col(c, i, B)] = col0 + c < N ? vs[(int64_t)i * N + col0 + c] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    fft_lds<float, 1, NT, MAXE, true>(buf, B, 0, plan, tw);
+    float *is = img + (int64_t)slice * N * N;
+    for (int idx = threadIdx.x; idx < N * B; idx += NT) {
+        const int a = idx / B, c = idx - a * B;
+        const int b = col0 + c;
+        if (b < N) is[(int64_t)a * N + b] = buf[lds_col(c, mode_to_fine(a, N, nf), B)].x * corr[a] * corr[b] * scale;
+    }
+}
+
+// type 2, pass B: deconvolve the image, place the N mode rows on the fine
+// column, FFT- -> V (nf x N)
+template <int NT, int MAXE>
+__global__ void __launch_bounds__(NT)
+k_fine_cols_t2(const float *__restrict__ img, float2 *__restrict__ V, const float *__restrict__ corr,
+               int nf, int N, int B, FftPlan plan, const float2 *__restrict__ tw) {
+    float2 *buf = reinterpret_cast<float2 *>(lc_smem);
+    const int slice = blockIdx.y;
+    const int col0 = blockIdx.x * B;
+    const float *is = img + (int64_t)slice * N * N;
+    for (int idx = threadIdx.x; idx < nf * B; idx += NT) buf[swz(idx)] = make_float2(0.f, 0.f);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < N * B; idx += NT) {
+        const int a = idx / B, c = idx - a * B;
+        const int b = col0 + c;
+        if (b < N) buf[lds_col(c, mode_to_fine(a, N, nf), B)] = make_float2(is[(int64_t)a * N + b] * corr[a] * corr[b], 0.f);
+    }
+    __syncthreads();
+    fft_lds<float, -1, NT, MAXE, true>(buf, B, 0, plan, tw);
+    float2 *vs = V + (int64_t)slice * nf * N;
+    for (int idx = threadIdx.x; idx < nf * B; idx += NT) {
+        const int i = idx / B, c = idx - i * B;
+        if (col0 + c < N) vs[(int64_t)i * N + col0 + c] = buf[lds_col(c, i, B)];
+    }
+}
+
+// type 2, pass A: place the N mode columns of each V row on the fine row,
+// FFT- -> fine grid
+template <int NT, int MAXE>
+__global__ void __launch_bounds__(NT)
+k_fine_rows_t2(const float2 *__restrict__ V, float2 *__restrict__ fine, int nf, int N, int nl,
+               FftPlan plan, const float2 *__restrict__ tw) {
+    float2 *buf = reinterpret_cast<float2 *>(lc_smem);
+    const int slice = blockIdx.y;
+    const int row0 = blockIdx.x * nl;
+    const int nloc = min(nl, nf - row0);
+    for (int idx = threadIdx.x; idx < nl * nf; idx += NT) buf[swz(idx)] = make_float2(0.f, 0.f);
+    __syncthreads();
+    const float2 *vs = V + (int64_t)slice * nf * N;
+    for (int idx = threadIdx.x; idx < nloc * N; idx += NT) {
+        const int l = idx / N, b = idx - l * N;
+        buf[lds_row(l, mode_to_fine(b, N, nf), nf)] = vs[(int64_t)(row0 + l) * N + b];
+    }
+    __syncthreads();
+    fft_lds<float, -1, NT, MAXE, false>(buf, nl, nf, plan, tw);
+    float2 *fs = fine + (int64_t)slice * nf * nf;
+    for (int idx = threadIdx.x; idx < nloc * nf; idx += NT) {
+        const int l = idx / nf, e = idx - l * nf;
+        fs[(int64_t)(row0 + l) * nf + e] = buf[lds_row(l, e, nf)];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+// Gauss-Legendre quadrature of the kernel's Fourier transform (FP64):
+// Psi(xi) = (w/2) int_{-pi/2}^{pi/2} e^{beta (cos t - 1)} cos(xi (w/2) sin t) cos t dt
+static double es_phihat(double beta, int w, double xi) {
+    static double gx[96], gw[96];
+    static bool ready = false;
+    const int n = 96;
+    if (!ready) {
+        for (int i = 0; i < n; ++i) {
+            double x = std::cos(M_PI * (i + 0.75) / (n + 0.5));
+            for (int it = 0; it < 100; ++it) {
+                double p0 = 1.0, p1 = x;
+                for (int k = 2; k <= n; ++k) {
+                    double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+                    p0 = p1;
+                    p1 = p2;
+                }
+                double dp = n * (x * p1 - p0) / (x * x - 1.0);
+                double dx = p1 / dp;
+                x -= dx;
+                if (std::fabs(dx) < 1e-16) break;
+            }
+            double p0 = 1.0, p1 = x;
+            for (int k = 2; k <= n; ++k) {
+                double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            double dp = n * (x * p1 - p0) / (x * x - 1.0);
+            gx[i] = x;
+            gw[i] = 2.0 / ((1.0 - x * x) * dp * dp);
+        }
+        ready = true;
+    }
+    const double hw = 0.5 * w;
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double t = 0.5 * M_PI * gx[i];
+        const double ct = std::cos(t);
+        s += gw[i] * std::exp(beta * (ct - 1.0)) * std::cos(xi * hw * std::sin(t)) * ct;
+    }
+    return hw * 0.5 * M_PI * s;
+}
+
+void nufft2d_init(Ctx &c, Geom2D &g) {
+    Nufft2D &u = g.nu;
+    const int N = (int)g.N;
+    int64_t nf = 2 * N;
+    while (!is_2_3_smooth(nf) || nf < 2 * NU_W) ++nf;
+    u.nf = (int)nf;
+    u.w = NU_W;
+    u.beta = 2.30 * NU_W;
+    std::vector<float> corr(N);
+    for (int a = 0; a < N; ++a) {
+        const int m = a - N / 2;
+        const double ph = es_phihat(u.beta, NU_W, (double)m * 2.0 * M_PI / (double)nf);
+        const double sgn = (m & 1) ? -1.0 : 1.0;   // (-1)^m from the centred fine grid
+        corr[a] = (float)(sgn / ph);
+    }
+    u.corr.alloc(sizeof(float) * N);
+    LC_HIP(hipMemcpy(u.corr.as(), corr.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+    LC_REQUIRE(fft_make_plan(u.nf, u.plan_nf), "nufft: fine grid size not 2^p 3^q");
+    u.det_fast = fft_make_plan((int)g.nrad, u.plan_det);
+    u.ready = true;
+}
+
+namespace {
+constexpr int NU_NT = 256, NU_ME = 16, NU_MEC = 32;   // rows / columns: NT*ME elements in LDS
+
+SpreadGeom spread_geom(const Geom2D &g) {
+    SpreadGeom s;
+    s.nang = (int)g.nang;
+    s.nrad = (int)g.nrad;
+    s.nf = g.nu.nf;
+    s.ntx = ceil_div(g.nu.nf, NU_T);
+    s.dr = 2.0 * M_PI / (double)g.nrad;
+    s.sc = (double)g.nu.nf / (2.0 * M_PI);
+    s.beta = (float)g.nu.beta;
+    return s;
+}
+
+void det_transform(Ctx &c, const Geom2D &g, int sign, const float *real_in, const float2 *cplx_in,
+                   float2 *cplx_out, float *real_out, int nrows) {
+    const int n = (int)g.nrad;
+    const float2 *tw = c.twiddles<float>(n);
+    const float scale = 1.0f / (float)n;
+    if (g.nu.det_fast && n <= NU_NT * NU_ME) {
+        const int nl = std::max(1, std::min(NU_NT * NU_ME / n, 64));
+        const size_t sh = sizeof(float2) * lds_round(nl * n);
+        if (sign < 0)
+            hipLaunchKernelGGL((k_det_fft<-1, NU_NT, NU_ME>), dim3(ceil_div(nrows, nl)), dim3(NU_NT), sh,
+                               c.stream(), real_in, cplx_in, cplx_out, real_out, nrows, n, nl,
+                               g.nu.plan_det, tw, scale);
+        else
+            hipLaunchKernelGGL((k_det_fft<1, NU_NT, NU_ME>), dim3(ceil_div(nrows, nl)), dim3(NU_NT), sh,
+                               c.stream(), real_in, cplx_in, cplx_out, real_out, nrows, n, nl,
+                               g.nu.plan_det, tw, scale);
+    } else {
+        dim3 grid(nrows);
+        if (sign < 0)
+            hipLaunchKernelGGL(k_det_dft<-1>, grid, dim3(128), 0, c.stream(), real_in, cplx_in,
+                               cplx_out, real_out, nrows, n, tw, scale);
+        else
+            hipLaunchKernelGGL(k_det_dft<1>, grid, dim3(128), 0, c.stream(), real_in, cplx_in,
+                               cplx_out, real_out, nrows, n, tw, scale);
+    }
+    LC_HIP(hipGetLastError());
+}
+} // namespace
+
+namespace {
+struct NuWork {
+    float2 *C, *F, *V;
+};
+
+NuWork nu_work(Ctx &c, const Geom2D &g, int ns) {
+    const int N = (int)g.N, nf = g.nu.nf;
+    const size_t bC = sizeof(float2) * (size_t)ns * g.nang * g.nrad;
+    const size_t bF = sizeof(float2) * (size_t)ns * nf * nf;
+    const size_t bV = sizeof(float2) * (size_t)ns * nf * N;
+    char *ws = reinterpret_cast<char *>(c.scratch(3, bC + bF + bV));
+    return NuWork{reinterpret_cast<float2 *>(ws), reinterpret_cast<float2 *>(ws + bC),
+                  reinterpret_cast<float2 *>(ws + bC + bF)};
+}
+
+// type 1: C (ns x nang x nrad complex) -> vol = Re(T1 C) / nrad
+void type1(Ctx &c, const Geom2D &g, const NuWork &w, float *vol, int ns) {
+    const Nufft2D &u = g.nu;
+    const int N = (int)g.N, nf = u.nf;
+    const SpreadGeom sg = spread_geom(g);
+    const size_t shs = sizeof(float2) * NU_WAVES * NU_T * NU_T;
+    hipLaunchKernelGGL(k_spread2d, dim3(sg.ntx * sg.ntx, ns), dim3(64 * NU_WAVES), shs, c.stream(),
+                       w.C, w.F, g.d_theta.as<double>(), sg);
+    LC_HIP(hipGetLastError());
+    const float2 *tw = c.twiddles<float>(nf);
+    const int nl = std::min(NU_NT * NU_ME / nf, nf);
+    hipLaunchKernelGGL((k_fine_rows_t1<NU_NT, NU_ME>), dim3(ceil_div(nf, nl), ns), dim3(NU_NT),
+                       sizeof(float2) * lds_round(nl * nf), c.stream(), w.F, w.V, nf, N, nl, u.plan_nf, tw);
+    LC_HIP(hipGetLastError());
+    const int B = std::min(NU_NT * NU_MEC / nf, N);
+    hipLaunchKernelGGL((k_fine_cols_t1<NU_NT, NU_MEC>), dim3(ceil_div(N, B), ns), dim3(NU_NT),
+                       sizeof(float2) * lds_round(B * nf), c.stream(), w.V, vol, u.corr.as<float>(), nf, N,
+                       B, 1.0f / (float)g.nrad, u.plan_nf, tw);
+    LC_HIP(hipGetLastError());
+}
+
+// type 2: vol (ns x N x N real) -> C = T2 vol
+void type2(Ctx &c, const Geom2D &g, const NuWork &w, const float *vol, int ns) {
+    const Nufft2D &u = g.nu;
+    const int N = (int)g.N, nf = u.nf;
+    const int64_t npts = g.nang * g.nrad;
+    const float2 *tw = c.twiddles<float>(nf);
+    const int B = std::min(NU_NT * NU_MEC / nf, N);
+    hipLaunchKernelGGL((k_fine_cols_t2<NU_NT, NU_MEC>), dim3(ceil_div(N, B), ns), dim3(NU_NT),
+                       sizeof(float2) * lds_round(B * nf), c.stream(), vol, w.V, u.corr.as<float>(), nf,
+                       N, B, u.plan_nf, tw);
+    LC_HIP(hipGetLastError());
+    const int nl = std::min(NU_NT * NU_ME / nf, nf);
+    hipLaunchKernelGGL((k_fine_rows_t2<NU_NT, NU_ME>), dim3(ceil_div(nf, nl), ns), dim3(NU_NT),
+                       sizeof(float2) * lds_round(nl * nf), c.stream(), w.V, w.F, nf, N, nl, u.plan_nf, tw);
+    LC_HIP(hipGetLastError());
+    const SpreadGeom sg = spread_geom(g);
+    hipLaunchKernelGGL(k_interp2d, dim3(ceil_div(npts, 256), ns), dim3(256), 0, c.stream(), w.F, w.C,
+                       g.d_theta.as<double>(), sg, (int)npts);
+    LC_HIP(hipGetLastError());
+}
+
+void nu_check(Ctx &c, Geom2D &g, int ns) {
+    if (!g.nu.ready) nufft2d_init(c, g);
+    LC_REQUIRE(g.nu.nf <= NU_NT * NU_ME, "nufft: fine grid too large (image size above 2048)");
+    LC_REQUIRE(ns >= 1 && ns <= 65535, "nufft: 1 to 65535 slices per call");
+}
+} // namespace
+
+// backproj2d: sino (ns x nang x nrad real, device) -> vol (ns x N x N real, device)
+void backproj2d_dev(Ctx &c, Geom2D &g, const float *sino, float *vol, int ns) {
+    nu_check(c, g, ns);
+    const NuWork w = nu_work(c, g, ns);
+    det_transform(c, g, -1, sino, nullptr, w.C, nullptr, (int)(ns * g.nang));
+    type1(c, g, w, vol, ns);
+}
+
+// forward2d: vol (ns x N x N real, device) -> sino (ns x nang x nrad real, device)
+void forward2d_dev(Ctx &c, Geom2D &g, const float *vol, float *sino, int ns) {
+    nu_check(c, g, ns);
+    const NuWork w = nu_work(c, g, ns);
+    type2(c, g, w, vol, ns);
+    det_transform(c, g, 1, nullptr, w.C, nullptr, sino, (int)(ns * g.nang));
+}
+
+// sysmat2d(x, grid) = Re(T1(T2 x)) / nrad (src/gradient.cpp:59-69)
+void sysmat2d_nufft_dev(Ctx &c, Geom2D &g, const float *x, float *y, int ns) {
+    nu_check(c, g, ns);
+    const NuWork w = nu_work(c, g, ns);
+    type2(c, g, w, x, ns);
+    type1(c, g, w, y, ns);
+}
+
+} // namespace lc

@@ -225,6 +225,7 @@ void geom2d_build_psf(Ctx &c, Geom2D &g) {
 }
 
 void geom2d_kernel_hat(Ctx &c, Geom2D &g, float *out_host) {
+    if (!g.psf_ready) geom2d_build_psf(c, g);
     const int M = (int)g.M, H = (int)g.H;
     float2 *tmp = reinterpret_cast<float2 *>(c.scratch(1, sizeof(float2) * (size_t)M * H));
     const float unscale = (float)((double)M * (double)M * (double)g.N);

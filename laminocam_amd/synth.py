@@ -61,20 +61,27 @@ def ellipse_phantom(N: int, n_ellipses: int, rng: Stream, piecewise: bool = Fals
     centres U[-0.5, 0.5], semi-axes U[0.05, 0.4], rotation U[0, pi),
     intensity U[0, 1]; piecewise=True paints instead of adding."""
     c = (np.arange(N) + 0.5) * (2.0 / N) - 1.0
-    yy, xx = np.meshgrid(c, c, indexing="ij")
     img = np.zeros((N, N), np.float64)
     for _ in range(n_ellipses):
         cx, cy = rng.uniform(2, -0.5, 0.5)
         a, b = rng.uniform(2, 0.05, 0.4)
         phi = rng.uniform(1, 0.0, np.pi)[0]
         val = rng.uniform(1)[0]
+        # only the bounding square of the circumscribed circle can be inside
+        rmax = max(a, b)
+        ix = np.nonzero(np.abs(c - cx) <= rmax)[0]
+        iy = np.nonzero(np.abs(c - cy) <= rmax)[0]
+        if ix.size == 0 or iy.size == 0:
+            continue
+        yy, xx = np.meshgrid(c[iy], c[ix], indexing="ij")
         xr = (xx - cx) * np.cos(phi) + (yy - cy) * np.sin(phi)
         yr = -(xx - cx) * np.sin(phi) + (yy - cy) * np.cos(phi)
         inside = (xr / a) ** 2 + (yr / b) ** 2 <= 1.0
+        sub = img[iy[0]:iy[-1] + 1, ix[0]:ix[-1] + 1]
         if piecewise:
-            img[inside] = val
+            sub[inside] = val
         else:
-            img[inside] += val
+            sub[inside] += val
     return img.astype(np.float32)
 
 
