@@ -87,11 +87,13 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def cpu_baseline(args, geoms_keys, iters_by_n, Ns, rhs_by_n):
-    """FP32 CPU restatement of the reference path (Toeplitz CG), one slice per
-    image size for a few iterations; the time of the whole workload follows
-    from the per-iteration time of each size and the iteration counts the
-    batched solve ran."""
+def cpu_baseline(args, iters_by_slice, Ns, rhs_by_n):
+    """FP32 CPU restatement of the reference path (Toeplitz CG): per image
+    size N one slice of the workload is solved cpu_reps times with cpu_iters
+    CG iterations (the median is kept).  A Toeplitz CG iteration costs the same
+    whatever the angle count, so the time of the whole workload is the sum
+    over its slices of (seconds per iteration at that N) x (iterations the
+    batched solve ran)."""
     from oracle.oracle import Oracle
     o32 = Oracle(32)
     threads = int(o32.lib.or_num_threads())
@@ -101,22 +103,25 @@ def cpu_baseline(args, geoms_keys, iters_by_n, Ns, rhs_by_n):
         y, theta = rhs_by_n[N]
         psf = o32.psf2d(theta, N, N, method=0)
         k = args.cpu_iters
-        t0 = time.perf_counter()
-        o32.cgsolver(psf, y, np.zeros_like(y), k, 0.0, 0.0)
-        dt = time.perf_counter() - t0
-        t_iter[N] = dt / k
-        spent += dt
-    total = sum(t_iter[int(N)] * iters_by_n[i] for i, N in enumerate(Ns))
+        ts = []
+        for _ in range(args.cpu_reps):
+            t0 = time.perf_counter()
+            o32.cgsolver(psf, y, np.zeros_like(y), k, 0.0, 0.0)
+            ts.append(time.perf_counter() - t0)
+        spent += sum(ts)
+        t_iter[N] = float(np.median(ts)) / k
+    total = sum(t_iter[int(N)] * iters_by_slice[i] for i, N in enumerate(Ns))
     return {
         "value": len(Ns) / total,
         "unit": "slices/s",
         "cores": threads,
         "kind": "port",
         "sample": (f"FP32 C restatement of the reference CPU path (Toeplitz CG), OpenMP "
-                   f"{threads} threads: one slice per image size N in {sorted(t_iter)}, "
-                   f"{args.cpu_iters} CG iterations each ({spent:.1f} s); workload time = "
-                   f"sum over the {len(Ns)} slices of (seconds per iteration at its N) x "
-                   f"(iterations the batched solve ran)"),
+                   f"{threads} threads: one workload slice per image size N in {sorted(t_iter)}, "
+                   f"{args.cpu_iters} CG iterations, median of {args.cpu_reps} solves "
+                   f"({spent:.1f} s of CPU solves); workload time = sum over the {len(Ns)} "
+                   f"slices of (seconds per iteration at its N) x (iterations the batched "
+                   f"solve ran)"),
         "sec_per_iter": {str(k): v for k, v in t_iter.items()},
     }
 
@@ -166,7 +171,8 @@ def main():
                     help="smallest slices compared with the FP64 oracle")
     ap.add_argument("--short-iters", type=int, default=5,
                     help="CG iterations of the short-horizon error check")
-    ap.add_argument("--cpu-iters", type=int, default=5)
+    ap.add_argument("--cpu-iters", type=int, default=50)
+    ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--kernel-reps", type=int, default=10)
     ap.add_argument("--pmc-probe", action="store_true",
@@ -299,7 +305,7 @@ def main():
             if N not in rhs_by_n or As[i] < rhs_by_n[N][2]:
                 rhs_by_n[N] = (rhs[i], synth.angles(int(As[i])), int(As[i]))
         rhs_by_n = {k: v[:2] for k, v in rhs_by_n.items()}
-        cpu = cpu_baseline(args, None, iters, Ns, rhs_by_n)
+        cpu = cpu_baseline(args, iters, Ns, rhs_by_n)
         log(f"cpu baseline: {cpu['value']:.3f} slices/s on {cpu['cores']} threads")
 
     out = {

@@ -41,6 +41,7 @@ extern "C" {
 
 typedef struct lc_ctx_s *lc_ctx;
 typedef struct lc_geom2d_s *lc_geom2d;
+typedef struct lc_geom3d_s *lc_geom3d;
 typedef struct lc_batch2d_s *lc_batch2d;
 
 typedef struct {
@@ -104,6 +105,44 @@ int lc_split_bregman2d(lc_ctx ctx, lc_geom2d g, const float *rhs, float *x, int6
                        double lambda, double mu, int64_t outer, int64_t inner, double tol,
                        double xtol, uint32_t flags, lc_report *rep);
 
+/* ---- 3-D laminography geometry + Toeplitz PSF ----
+ * Replaces cpu::PolarGrid(theta, gamma, beta, nrows, ncols)
+ * (include/polar_grid.h:52-94, rotation include/rotation.h:28-56) and
+ * cpu::PointSpreadFunction(grid, recon_dims) (include/toeplitz.h:61-99) for a
+ * volume recon_dims = {n1, n2, n3}.  The projectors take any dims; the
+ * Toeplitz operator and the solvers need n2 == n3 (LC_EUNSUP otherwise).  Angles in radians, one (theta, gamma, beta) per projection.
+ * The PSF is computed in FP64 on the GPU from its closed form
+ * P(m) = sum_i D_ncols(m.u_i) D_nrows(m.v_i) and stored as an FP32 spectrum. */
+int lc_geom3d_create(lc_ctx ctx, const double *theta, const double *gamma, const double *beta,
+                     int64_t nproj, int64_t nrows, int64_t ncols, const int64_t *recon_dims,
+                     uint32_t flags, lc_geom3d *out);
+int lc_geom3d_destroy(lc_geom3d g);
+/* info[0..8] = nproj, nrows, ncols, n1, n2, n3, PSF dims (3) */
+int lc_geom3d_info(lc_geom3d g, int64_t *info);
+/* PointSpreadFunction::kernel_hat() (include/toeplitz.h:55-57): complex
+ * {d1, d2, d3/2+1} written to out (host) */
+int lc_psf3d_kernel_hat(lc_ctx ctx, lc_geom3d g, float *out);
+/* sysmat(x, PointSpreadFunction) (src/gradient.cpp:49-52, include/toeplitz.h:101-129)
+ * on the {n1, n2, n3} volume x */
+int lc_sysmat3d_psf(lc_ctx ctx, lc_geom3d g, const float *x, float *y, uint32_t flags);
+/* forward(volume, grid) (src/projection.cpp:42-64, include/projection.h:38-39):
+ * proj {nproj, nrows, ncols} = Re(ifftshift2(ifft2(fftshift2(w . T2 vol))))/(nrows ncols) */
+int lc_forward3d(lc_ctx ctx, lc_geom3d g, const float *vol, float *proj, uint32_t flags);
+/* backproj(proj, grid, recon_dims[, shifts]) (src/projection.cpp:71-131,
+ * include/projection.h:41-47): Re(T1(w . phase . ifftshift2(fft2(fftshift2 proj))))/(nrows ncols).
+ * shifts: NULL or nproj (dx, dy) pairs, phase = exp(-i (qx dx + qy dy)) (include/fft.h:53-74) */
+int lc_backproj3d(lc_ctx ctx, lc_geom3d g, const float *proj, const double *shifts, float *vol,
+                  uint32_t flags);
+/* sysmat(x, PolarGrid) (src/gradient.cpp:32-41): Re(T1(T2 x))/(nrows ncols), no mask */
+int lc_sysmat3d_nufft(lc_ctx ctx, lc_geom3d g, const float *x, float *y, uint32_t flags);
+/* opt::cgsolver / opt::split_bregman with A = sysmat(., psf) on one volume
+ * (src/conjgrad.cpp:33-76, src/bregman.cpp:36-94; the solve of src/mbir.cpp:122-157) */
+int lc_cg3d(lc_ctx ctx, lc_geom3d g, const float *rhs, float *x, int64_t max_iter, double tol,
+            double xtol, uint32_t flags, lc_report *rep);
+int lc_split_bregman3d(lc_ctx ctx, lc_geom3d g, const float *rhs, float *x, double lambda,
+                       double mu, int64_t outer, int64_t inner, double tol, double xtol,
+                       uint32_t flags, lc_report *rep);
+
 /* ---- finite differences on an n1 x n2 x n3 volume ----
  * opt::grad_u / divergence / laplacian (src/finitediff.cpp:26-90,
  * include/bregman.h:34-55).  g holds the 3 components back to back in the
@@ -121,6 +160,10 @@ int lc_laplacian(lc_ctx ctx, const float *u, int64_t n1, int64_t n2, int64_t n3,
  * geoms[i]; buffers stay resident on the device between calls. */
 int lc_batch2d_create(lc_ctx ctx, int32_t nprob, const lc_geom2d *geoms, const int32_t *ns,
                       lc_batch2d *out);
+/* mixed batch: problem i is a 2-D stack (g2[i], ns[i]) or a 3-D volume g3[i];
+ * exactly one of g2[i], g3[i] is non-NULL (either array may be NULL) */
+int lc_batch_create(lc_ctx ctx, int32_t nprob, const lc_geom2d *g2, const lc_geom3d *g3,
+                    const int32_t *ns, lc_batch2d *out);
 int lc_batch2d_destroy(lc_batch2d b);
 /* which: 0 = x (solution / initial guess), 1 = y (right-hand side A^T y) */
 int lc_batch2d_set(lc_batch2d b, int32_t prob, int32_t which, const float *src, uint32_t flags);

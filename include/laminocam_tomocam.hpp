@@ -1,0 +1,518 @@
+/*
+ * laminocam_tomocam.hpp - the reference's C++ hot-path API re-implemented over
+ * the C ABI of liblaminocam.so (laminocam.h), header only.
+ *
+ * Names and signatures follow the reference headers they replace:
+ *   include/polar_grid2d.h:35-66     cpu::PolarGrid2D
+ *   include/polar_grid.h:35-94       cpu::PolarGrid
+ *   include/toeplitz2d.h:40-126      cpu::PointSpreadFunction2D
+ *   include/toeplitz.h:35-130        cpu::PointSpreadFunction
+ *   include/projection.h:38-68       forward, backproj (2 overloads), sysmat (2),
+ *                                    forward2d, backproj2d, sysmat2d (2)
+ *   include/optimize.h:31-91         opt::Function, opt::cgsolver, opt::split_bregman
+ *   include/bregman.h:34-55          opt::grad_u, opt::divergence, opt::laplacian
+ * Errors: a nonzero status becomes std::invalid_argument (bad arguments,
+ * unsupported shapes) or std::runtime_error (device failures), as the
+ * reference throws (src/projection.cpp:105-108, include/finufft_plan.h:106).
+ *
+ * Array types.  The functions use only dims() / data() / size() and the
+ * Array(dims_t) constructor of tomocam::Array<T> (include/array.h:61-110), so
+ * they compile against the reference's array.h when it is included first.
+ * Defining LAMINOCAM_STANDALONE_ARRAY instead provides a minimal
+ * tomocam::dims_t / tomocam::Array<T> with that same interface.
+ *
+ * Precision: the device computes in FP32 (FP64 reductions, PSF and twiddles);
+ * double arrays are converted at the boundary.
+ *
+ * Solvers: opt::Function<T> is constructible from any callable, like the
+ * reference's std::function alias.  cgsolver / split_bregman run on the device
+ * when the operator is a Toeplitz PSF (opt::Function<T>(psf) or
+ * opt::toeplitz(psf)); an operator known only as an opaque callable cannot be
+ * moved to the device and is rejected with std::invalid_argument.
+ */
+#ifndef LAMINOCAM_TOMOCAM_HPP
+#define LAMINOCAM_TOMOCAM_HPP
+
+#include <array>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "laminocam.h"
+
+#ifdef LAMINOCAM_STANDALONE_ARRAY
+namespace tomocam {
+struct dims_t {
+    size_t n1 = 0, n2 = 0, n3 = 0;
+    dims_t() = default;
+    dims_t(size_t a, size_t b, size_t c) : n1(a), n2(b), n3(c) {}
+    size_t size() const { return n1 * n2 * n3; }
+    bool operator==(const dims_t &o) const { return n1 == o.n1 && n2 == o.n2 && n3 == o.n3; }
+};
+
+template <typename T> class Array {
+  public:
+    Array() = default;
+    explicit Array(dims_t d) : dims_(d), v_(d.size()) {}
+    static Array zeros(dims_t d) { return Array(d); }
+    dims_t dims() const { return dims_; }
+    size_t size() const { return v_.size(); }
+    size_t nslices() const { return dims_.n1; }
+    size_t nrows() const { return dims_.n2; }
+    size_t ncols() const { return dims_.n3; }
+    T *data() { return v_.data(); }
+    const T *data() const { return v_.data(); }
+    T *begin() { return v_.data(); }
+    const T *begin() const { return v_.data(); }
+    T *end() { return v_.data() + v_.size(); }
+    const T *end() const { return v_.data() + v_.size(); }
+    T &operator[](size_t i) { return v_[i]; }
+    const T &operator[](size_t i) const { return v_[i]; }
+
+  private:
+    dims_t dims_;
+    std::vector<T> v_;
+};
+} // namespace tomocam
+#endif
+
+namespace laminocam {
+
+inline void check(int status) {
+    if (status == LC_OK) return;
+    const std::string msg = lc_last_error();
+    if (status == LC_EINVAL || status == LC_EUNSUP) throw std::invalid_argument(msg);
+    throw std::runtime_error(msg);
+}
+
+// one device context per process (device 0), created on first use
+inline lc_ctx context() {
+    static lc_ctx ctx = [] {
+        lc_ctx c = nullptr;
+        check(lc_ctx_create(0, &c));
+        return c;
+    }();
+    return ctx;
+}
+
+// FP32 staging of a T array (no copy for float)
+template <typename T> struct F32In {
+    std::vector<float> buf;
+    const float *p;
+    F32In(const T *src, size_t n) {
+        if constexpr (std::is_same<T, float>::value) {
+            p = src;
+        } else {
+            buf.assign(src, src + n);
+            p = buf.data();
+        }
+    }
+};
+
+template <typename T> struct F32Out {
+    std::vector<float> buf;
+    T *dst;
+    float *p;
+    size_t n;
+    F32Out(T *d, size_t n_) : dst(d), n(n_) {
+        if constexpr (std::is_same<T, float>::value) {
+            p = d;
+        } else {
+            buf.resize(n);
+            p = buf.data();
+        }
+    }
+    void commit() {
+        if constexpr (!std::is_same<T, float>::value)
+            for (size_t i = 0; i < n; ++i) dst[i] = (T)buf[i];
+    }
+};
+
+struct Geom2 {
+    lc_geom2d h = nullptr;
+    ~Geom2() { if (h) lc_geom2d_destroy(h); }
+};
+struct Geom3 {
+    lc_geom3d h = nullptr;
+    ~Geom3() { if (h) lc_geom3d_destroy(h); }
+};
+
+} // namespace laminocam
+
+namespace tomocam::cpu {
+
+// include/polar_grid2d.h:35-66.  The points x = r cos(theta), y = r sin(theta)
+// are generated on the device; the angles and the radial count are kept.
+template <typename T> struct PolarGrid2D {
+    std::vector<double> theta;
+    size_t ncols_ = 0;
+    size_t nangles() const { return theta.size(); }
+    size_t nradial() const { return ncols_; }
+    dims_t dims() const { return dims_t{1, theta.size(), ncols_}; }
+    size_t size() const { return theta.size() * ncols_; }
+    PolarGrid2D() = default;
+    PolarGrid2D(const std::vector<T> &th, size_t ncols) : theta(th.begin(), th.end()), ncols_(ncols) {}
+
+    // device geometry for N x N images (PSF built on demand)
+    lc_geom2d geometry(size_t N) const {
+        auto &g = cache_[N];
+        if (!g) {
+            g = std::make_shared<laminocam::Geom2>();
+            laminocam::check(lc_geom2d_create(laminocam::context(), theta.data(), (int64_t)theta.size(),
+                                              (int64_t)ncols_, (int64_t)N, LC_GEOM_LAZY_PSF, &g->h));
+        }
+        return g->h;
+    }
+
+  private:
+    mutable std::map<size_t, std::shared_ptr<laminocam::Geom2>> cache_;
+};
+
+// include/polar_grid.h:35-94: points R_i (qX, qY, 0), R_i = RotationTranspose(theta, gamma, beta)
+template <typename T> struct PolarGrid {
+    std::vector<double> theta, gamma, beta;
+    size_t nrows_ = 0, ncols_ = 0;
+    dims_t dims() const { return dims_t{theta.size(), nrows_, ncols_}; }
+    size_t size() const { return theta.size() * nrows_ * ncols_; }
+    PolarGrid() = default;
+    PolarGrid(const std::vector<T> &th, const std::vector<T> &ga, const std::vector<T> &be, size_t nrows,
+              size_t ncols)
+        : theta(th.begin(), th.end()), gamma(ga.begin(), ga.end()), beta(be.begin(), be.end()),
+          nrows_(nrows), ncols_(ncols) {
+        if (ga.size() != th.size() || be.size() != th.size())
+            throw std::invalid_argument("PolarGrid: theta, gamma and beta sizes differ");
+    }
+
+    lc_geom3d geometry(dims_t rd, bool lazy = true) const {
+        const std::array<size_t, 3> key{rd.n1, rd.n2, rd.n3};
+        auto &g = cache_[key];
+        if (!g) {
+            g = std::make_shared<laminocam::Geom3>();
+            const int64_t d[3] = {(int64_t)rd.n1, (int64_t)rd.n2, (int64_t)rd.n3};
+            laminocam::check(lc_geom3d_create(laminocam::context(), theta.data(), gamma.data(), beta.data(),
+                                              (int64_t)theta.size(), (int64_t)nrows_, (int64_t)ncols_, d,
+                                              lazy ? LC_GEOM_LAZY_PSF : 0, &g->h));
+        }
+        return g->h;
+    }
+
+  private:
+    mutable std::map<std::array<size_t, 3>, std::shared_ptr<laminocam::Geom3>> cache_;
+};
+
+// include/toeplitz2d.h:40-126
+template <typename T> class PointSpreadFunction2D {
+  public:
+    using complex_t = std::complex<T>;
+    PointSpreadFunction2D() = default;
+    PointSpreadFunction2D(const PolarGrid2D<T> &grid, size_t image_size) : N_(image_size) {
+        g_ = std::make_shared<laminocam::Geom2>();
+        laminocam::check(lc_geom2d_create(laminocam::context(), grid.theta.data(), (int64_t)grid.theta.size(),
+                                          (int64_t)grid.nradial(), (int64_t)image_size, 0, &g_->h));
+        int64_t info[5];
+        laminocam::check(lc_geom2d_info(g_->h, info));
+        M_ = (size_t)info[3];
+    }
+    size_t psf_size() const { return M_; }
+    size_t image_size() const { return N_; }
+    lc_geom2d handle() const { return g_->h; }
+
+    Array<complex_t> kernel_hat() const {
+        const size_t H = M_ / 2 + 1;
+        std::vector<float> buf(2 * M_ * H);
+        laminocam::check(lc_psf2d_kernel_hat(laminocam::context(), g_->h, buf.data()));
+        Array<complex_t> out(dims_t{1, M_, H});
+        for (size_t i = 0; i < M_ * H; ++i) out.data()[i] = complex_t((T)buf[2 * i], (T)buf[2 * i + 1]);
+        return out;
+    }
+
+    Array<T> convolve(const Array<T> &x) const {
+        const dims_t d = x.dims();
+        if (d.n2 != N_ || d.n3 != N_) throw std::invalid_argument("convolve: slice size differs from the PSF's");
+        Array<T> y(d);
+        laminocam::F32In<T> in(x.data(), x.size());
+        laminocam::F32Out<T> out(y.data(), y.size());
+        laminocam::check(lc_sysmat2d_psf(laminocam::context(), g_->h, in.p, (int64_t)d.n1, out.p, 0));
+        out.commit();
+        return y;
+    }
+
+  private:
+    size_t N_ = 0, M_ = 0;
+    std::shared_ptr<laminocam::Geom2> g_;
+};
+
+// include/toeplitz.h:35-130
+template <typename T> class PointSpreadFunction {
+  public:
+    using complex_t = std::complex<T>;
+    PointSpreadFunction() = default;
+    PointSpreadFunction(const PolarGrid<T> &grid, dims_t recon_dims) : rd_(recon_dims) {
+        g_ = std::make_shared<laminocam::Geom3>();
+        const int64_t d[3] = {(int64_t)recon_dims.n1, (int64_t)recon_dims.n2, (int64_t)recon_dims.n3};
+        laminocam::check(lc_geom3d_create(laminocam::context(), grid.theta.data(), grid.gamma.data(),
+                                          grid.beta.data(), (int64_t)grid.theta.size(), (int64_t)grid.dims().n2,
+                                          (int64_t)grid.dims().n3, d, 0, &g_->h));
+        int64_t info[9];
+        laminocam::check(lc_geom3d_info(g_->h, info));
+        dims_ = dims_t{(size_t)info[6], (size_t)info[7], (size_t)info[8]};
+    }
+    dims_t dims() const { return dims_; }
+    lc_geom3d handle() const { return g_->h; }
+
+    Array<complex_t> kernel_hat() const {
+        const size_t H = dims_.n3 / 2 + 1, n = dims_.n1 * dims_.n2 * H;
+        std::vector<float> buf(2 * n);
+        laminocam::check(lc_psf3d_kernel_hat(laminocam::context(), g_->h, buf.data()));
+        Array<complex_t> out(dims_t{dims_.n1, dims_.n2, H});
+        for (size_t i = 0; i < n; ++i) out.data()[i] = complex_t((T)buf[2 * i], (T)buf[2 * i + 1]);
+        return out;
+    }
+
+    Array<T> convolve(const Array<T> &x) const {
+        if (!(x.dims() == rd_)) throw std::invalid_argument("convolve: volume dims differ from the PSF's");
+        Array<T> y(x.dims());
+        laminocam::F32In<T> in(x.data(), x.size());
+        laminocam::F32Out<T> out(y.data(), y.size());
+        laminocam::check(lc_sysmat3d_psf(laminocam::context(), g_->h, in.p, out.p, 0));
+        out.commit();
+        return y;
+    }
+
+  private:
+    dims_t rd_, dims_;
+    std::shared_ptr<laminocam::Geom3> g_;
+};
+
+} // namespace tomocam::cpu
+
+namespace tomocam {
+
+// include/projection.h:56-58, src/projection.cpp:146-160
+template <typename T> Array<T> forward2d(const Array<T> &volume, const cpu::PolarGrid2D<T> &grid) {
+    const dims_t d = volume.dims();
+    if (d.n2 != d.n3) throw std::invalid_argument("forward2d: slices must be square");
+    Array<T> out(dims_t{d.n1, grid.nangles(), grid.nradial()});
+    laminocam::F32In<T> in(volume.data(), volume.size());
+    laminocam::F32Out<T> o(out.data(), out.size());
+    laminocam::check(lc_forward2d(laminocam::context(), grid.geometry(d.n2), in.p, (int64_t)d.n1, o.p, 0));
+    o.commit();
+    return out;
+}
+
+// include/projection.h:59-61, src/projection.cpp:168-182
+template <typename T>
+Array<T> backproj2d(const Array<T> &sino, const cpu::PolarGrid2D<T> &grid, const dims_t &recon_dims) {
+    const dims_t d = sino.dims();
+    if (d.n2 != grid.nangles() || d.n3 != grid.nradial())
+        throw std::invalid_argument("backproj2d: sinogram does not match the grid");
+    if (recon_dims.n1 != d.n1 || recon_dims.n2 != recon_dims.n3)
+        throw std::invalid_argument("backproj2d: recon dims must be {nslices, N, N}");
+    Array<T> out(recon_dims);
+    laminocam::F32In<T> in(sino.data(), sino.size());
+    laminocam::F32Out<T> o(out.data(), out.size());
+    laminocam::check(
+        lc_backproj2d(laminocam::context(), grid.geometry(recon_dims.n2), in.p, (int64_t)d.n1, o.p, 0));
+    o.commit();
+    return out;
+}
+
+// src/gradient.cpp:59-69 (NUFFT form) and :76-79 (Toeplitz form)
+template <typename T> Array<T> sysmat2d(const Array<T> &x, const cpu::PolarGrid2D<T> &grid) {
+    const dims_t d = x.dims();
+    if (d.n2 != d.n3) throw std::invalid_argument("sysmat2d: slices must be square");
+    Array<T> out(d);
+    laminocam::F32In<T> in(x.data(), x.size());
+    laminocam::F32Out<T> o(out.data(), out.size());
+    laminocam::check(lc_sysmat2d_nufft(laminocam::context(), grid.geometry(d.n2), in.p, (int64_t)d.n1, o.p, 0));
+    o.commit();
+    return out;
+}
+
+template <typename T> Array<T> sysmat2d(const Array<T> &x, const cpu::PointSpreadFunction2D<T> &psf) {
+    return psf.convolve(x);
+}
+
+// include/projection.h:38-39, src/projection.cpp:42-64
+template <typename T> Array<T> forward(const Array<T> &volume, const cpu::PolarGrid<T> &grid) {
+    Array<T> out(grid.dims());
+    laminocam::F32In<T> in(volume.data(), volume.size());
+    laminocam::F32Out<T> o(out.data(), out.size());
+    laminocam::check(lc_forward3d(laminocam::context(), grid.geometry(volume.dims()), in.p, o.p, 0));
+    o.commit();
+    return out;
+}
+
+// include/projection.h:41-47, src/projection.cpp:71-131
+template <typename T>
+Array<T> backproj(const Array<T> &proj, const cpu::PolarGrid<T> &grid, const dims_t &recon_dims) {
+    if (!(proj.dims() == grid.dims())) throw std::invalid_argument("backproj: projections do not match the grid");
+    Array<T> out(recon_dims);
+    laminocam::F32In<T> in(proj.data(), proj.size());
+    laminocam::F32Out<T> o(out.data(), out.size());
+    laminocam::check(lc_backproj3d(laminocam::context(), grid.geometry(recon_dims), in.p, nullptr, o.p, 0));
+    o.commit();
+    return out;
+}
+
+template <typename T>
+Array<T> backproj(const Array<T> &proj, const cpu::PolarGrid<T> &grid, const dims_t &recon_dims,
+                  const std::vector<std::array<T, 2>> &shifts) {
+    if (shifts.size() != proj.nslices())
+        throw std::invalid_argument("shifts size must match the number of slices in the projection");
+    if (!(proj.dims() == grid.dims())) throw std::invalid_argument("backproj: projections do not match the grid");
+    std::vector<double> sh(2 * shifts.size());
+    for (size_t i = 0; i < shifts.size(); ++i) {
+        sh[2 * i] = (double)shifts[i][0];
+        sh[2 * i + 1] = (double)shifts[i][1];
+    }
+    Array<T> out(recon_dims);
+    laminocam::F32In<T> in(proj.data(), proj.size());
+    laminocam::F32Out<T> o(out.data(), out.size());
+    laminocam::check(lc_backproj3d(laminocam::context(), grid.geometry(recon_dims), in.p, sh.data(), o.p, 0));
+    o.commit();
+    return out;
+}
+
+// src/gradient.cpp:32-41 (NUFFT form, no mask) and :49-52 (Toeplitz form)
+template <typename T> Array<T> sysmat(const Array<T> &x, const cpu::PolarGrid<T> &grid) {
+    Array<T> out(x.dims());
+    laminocam::F32In<T> in(x.data(), x.size());
+    laminocam::F32Out<T> o(out.data(), out.size());
+    laminocam::check(lc_sysmat3d_nufft(laminocam::context(), grid.geometry(x.dims()), in.p, o.p, 0));
+    o.commit();
+    return out;
+}
+
+template <typename T> Array<T> sysmat(const Array<T> &x, const cpu::PointSpreadFunction<T> &psf) {
+    return psf.convolve(x);
+}
+
+} // namespace tomocam
+
+namespace tomocam::opt {
+
+// include/optimize.h:31-32: a callable Array -> Array.  It also records the
+// Toeplitz PSF it applies when built from one, which lets the solvers run the
+// whole iteration on the device.
+template <typename T> class Function {
+  public:
+    Function() = default;
+    template <typename F,
+              typename = typename std::enable_if<!std::is_same<typename std::decay<F>::type, Function>::value>::type>
+    Function(F &&f) : fn_(std::forward<F>(f)) {}
+    Function(const cpu::PointSpreadFunction2D<T> &psf)
+        : fn_([psf](const Array<T> &x) { return psf.convolve(x); }), psf2_(psf), kind_(2) {}
+    Function(const cpu::PointSpreadFunction<T> &psf)
+        : fn_([psf](const Array<T> &x) { return psf.convolve(x); }), psf3_(psf), kind_(3) {}
+    Array<T> operator()(const Array<T> &x) const { return fn_(x); }
+    int kind() const { return kind_; }
+    const cpu::PointSpreadFunction2D<T> &psf2() const { return psf2_; }
+    const cpu::PointSpreadFunction<T> &psf3() const { return psf3_; }
+
+  private:
+    std::function<Array<T>(const Array<T> &)> fn_;
+    cpu::PointSpreadFunction2D<T> psf2_;
+    cpu::PointSpreadFunction<T> psf3_;
+    int kind_ = 0;   // 0 opaque callable, 2 / 3 Toeplitz PSF
+};
+
+template <typename T> Function<T> toeplitz(const cpu::PointSpreadFunction2D<T> &psf) { return Function<T>(psf); }
+template <typename T> Function<T> toeplitz(const cpu::PointSpreadFunction<T> &psf) { return Function<T>(psf); }
+
+namespace detail {
+template <typename T> void need_psf(const Function<T> &A, const char *who) {
+    if (A.kind() == 0)
+        throw std::invalid_argument(std::string(who) +
+                                    ": the operator is an opaque callable; build it from the "
+                                    "PointSpreadFunction (opt::toeplitz(psf)) to solve on the device");
+}
+} // namespace detail
+
+// include/optimize.h:89-91, src/conjgrad.cpp:33-76
+template <typename T>
+Array<T> cgsolver(const Function<T> &A, const Array<T> &y, const Array<T> &x, size_t max_iter, T tol, T xtol) {
+    detail::need_psf(A, "cgsolver");
+    Array<T> out(x.dims());
+    laminocam::F32In<T> yin(y.data(), y.size());
+    laminocam::F32Out<T> o(out.data(), out.size());
+    std::memcpy(o.p, laminocam::F32In<T>(x.data(), x.size()).p, sizeof(float) * x.size());
+    lc_report rep;
+    if (A.kind() == 2)
+        laminocam::check(lc_cg2d(laminocam::context(), A.psf2().handle(), yin.p, o.p, (int64_t)y.dims().n1,
+                                 (int64_t)max_iter, (double)tol, (double)xtol, 0, &rep));
+    else
+        laminocam::check(lc_cg3d(laminocam::context(), A.psf3().handle(), yin.p, o.p, (int64_t)max_iter,
+                                 (double)tol, (double)xtol, 0, &rep));
+    o.commit();
+    return out;
+}
+
+// include/optimize.h:74-77, src/bregman.cpp:36-94 (isotropic shrink,
+// interior-only differences as the reference CPU path)
+template <typename T>
+Array<T> split_bregman(const Function<T> &A, const Array<T> &y, const Array<T> &x0, T lambda, T mu,
+                       size_t outer_max, size_t inner_max, T tol, T xtol) {
+    detail::need_psf(A, "split_bregman");
+    Array<T> out(x0.dims());
+    laminocam::F32In<T> yin(y.data(), y.size());
+    laminocam::F32Out<T> o(out.data(), out.size());
+    std::memcpy(o.p, laminocam::F32In<T>(x0.data(), x0.size()).p, sizeof(float) * x0.size());
+    lc_report rep;
+    if (A.kind() == 2)
+        laminocam::check(lc_split_bregman2d(laminocam::context(), A.psf2().handle(), yin.p, o.p,
+                                            (int64_t)y.dims().n1, (double)lambda, (double)mu,
+                                            (int64_t)outer_max, (int64_t)inner_max, (double)tol,
+                                            (double)xtol, 0, &rep));
+    else
+        laminocam::check(lc_split_bregman3d(laminocam::context(), A.psf3().handle(), yin.p, o.p,
+                                            (double)lambda, (double)mu, (int64_t)outer_max,
+                                            (int64_t)inner_max, (double)tol, (double)xtol, 0, &rep));
+    o.commit();
+    return out;
+}
+
+// include/bregman.h:34-55, src/finitediff.cpp:26-90 (interior-only differences)
+template <typename T> std::array<Array<T>, 3> grad_u(const Array<T> &u) {
+    const dims_t d = u.dims();
+    std::vector<float> g(3 * u.size());
+    laminocam::F32In<T> in(u.data(), u.size());
+    laminocam::check(lc_grad(laminocam::context(), in.p, (int64_t)d.n1, (int64_t)d.n2, (int64_t)d.n3, g.data(), 0));
+    std::array<Array<T>, 3> out{Array<T>(d), Array<T>(d), Array<T>(d)};
+    for (int c = 0; c < 3; ++c)
+        for (size_t i = 0; i < u.size(); ++i) out[c].data()[i] = (T)g[c * u.size() + i];
+    return out;
+}
+
+template <typename T> Array<T> divergence(const std::array<Array<T>, 3> &g) {
+    const dims_t d = g[0].dims();
+    const size_t n = g[0].size();
+    std::vector<float> in(3 * n);
+    for (int c = 0; c < 3; ++c)
+        for (size_t i = 0; i < n; ++i) in[c * n + i] = (float)g[c].data()[i];
+    Array<T> out(d);
+    laminocam::F32Out<T> o(out.data(), n);
+    laminocam::check(lc_divergence(laminocam::context(), in.data(), (int64_t)d.n1, (int64_t)d.n2, (int64_t)d.n3, o.p, 0));
+    o.commit();
+    return out;
+}
+
+template <typename T> Array<T> laplacian(const Array<T> &u) {
+    const dims_t d = u.dims();
+    Array<T> out(d);
+    laminocam::F32In<T> in(u.data(), u.size());
+    laminocam::F32Out<T> o(out.data(), out.size());
+    laminocam::check(lc_laplacian(laminocam::context(), in.p, (int64_t)d.n1, (int64_t)d.n2, (int64_t)d.n3, o.p, 0));
+    o.commit();
+    return out;
+}
+
+} // namespace tomocam::opt
+
+#endif // LAMINOCAM_TOMOCAM_HPP

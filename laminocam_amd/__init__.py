@@ -5,7 +5,9 @@ Python mirror of the reference's C++ API over the C ABI of liblaminocam.so
 
   PolarGrid2D            include/polar_grid2d.h:35-66
   PointSpreadFunction2D  include/toeplitz2d.h:40-126
-  sysmat2d               src/gradient.cpp:49-79
+  PolarGrid              include/polar_grid.h:35-94
+  PointSpreadFunction    include/toeplitz.h:35-130
+  sysmat2d / sysmat      src/gradient.cpp:32-79
   forward2d / backproj2d src/projection.cpp:146-182
   cgsolver               src/conjgrad.cpp:33-76
   split_bregman          src/bregman.cpp:36-94
@@ -23,7 +25,8 @@ from . import _lib
 from ._lib import (LC_DEVICE_PTRS, LC_GRAPH, LC_TV_ANISO, LC_TV_ZERO_PAD, LC_X0_ZERO,
                    LcError, LcReport)
 
-__all__ = ["Context", "PolarGrid2D", "PointSpreadFunction2D", "sysmat2d", "forward2d",
+__all__ = ["Context", "PolarGrid2D", "PointSpreadFunction2D", "PolarGrid", "PointSpreadFunction",
+           "sysmat", "sysmat2d", "forward", "backproj", "forward2d",
            "backproj2d", "cgsolver",
            "split_bregman", "grad_u", "divergence", "laplacian", "Batch2D", "LcError",
            "default_context"]
@@ -237,6 +240,132 @@ class PointSpreadFunction2D:
             pass
 
 
+class PolarGrid:
+    """3-D laminography grid: points R_i (qX_k, qY_j, 0), R_i =
+    RotationTranspose(theta_i, gamma_i, beta_i), qX_k = (k + 1/2) 2 pi / ncols - pi,
+    qY_j = (j + 1/2) 2 pi / nrows - pi (include/polar_grid.h:52-94,
+    include/rotation.h:28-56).  Points are generated on the GPU; only the angles
+    and detector dims are kept."""
+
+    def __init__(self, theta, gamma, beta, nrows: int, ncols: int):
+        self.theta = np.ascontiguousarray(theta, dtype=np.float64)
+        n = len(self.theta)
+        self.gamma = np.ascontiguousarray(np.broadcast_to(gamma, (n,)), dtype=np.float64)
+        self.beta = np.ascontiguousarray(np.broadcast_to(beta, (n,)), dtype=np.float64)
+        self.nrows, self.ncols = int(nrows), int(ncols)
+
+    def dims(self):
+        return (len(self.theta), self.nrows, self.ncols)
+
+    def size(self):
+        return int(np.prod(self.dims()))
+
+
+class PointSpreadFunction:
+    """3-D Toeplitz kernel of A^T A for a {n1, n2, n3} volume (n2 == n3)
+    (include/toeplitz.h:35-130).  Built in FP64 on the GPU."""
+
+    def __init__(self, grid: PolarGrid, recon_dims, ctx: Context | None = None, lazy: bool = False):
+        self.ctx = ctx or default_context()
+        self.lib = self.ctx.lib
+        self.grid = grid
+        self.recon_dims = tuple(int(d) for d in recon_dims)
+        rd = (C.c_int64 * 3)(*self.recon_dims)
+        h = C.c_void_p()
+        f64 = C.POINTER(C.c_double)
+        _lib.check(self.lib.lc_geom3d_create(
+            self.ctx.handle, grid.theta.ctypes.data_as(f64), grid.gamma.ctypes.data_as(f64),
+            grid.beta.ctypes.data_as(f64), len(grid.theta), grid.nrows, grid.ncols, rd,
+            _lib.LC_GEOM_LAZY_PSF if lazy else 0, C.byref(h)))
+        self.handle = h
+        info = (C.c_int64 * 9)()
+        _lib.check(self.lib.lc_geom3d_info(h, info))
+        self.info = tuple(info)
+
+    def dims(self):
+        return self.info[6:9]
+
+    def kernel_hat(self):
+        d1, d2, d3 = self.dims()
+        out = np.empty((d1, d2, d3 // 2 + 1), np.complex64)
+        _lib.check(self.lib.lc_psf3d_kernel_hat(
+            self.ctx.handle, self.handle, out.view(np.float32).ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def convolve(self, x):
+        x = _f32(x)
+        if x.shape != self.recon_dims:
+            raise ValueError(f"convolve: volume must be {self.recon_dims}, got {x.shape}")
+        out = np.empty_like(x)
+        _lib.check(self.lib.lc_sysmat3d_psf(self.ctx.handle, self.handle, _ptr(x), _ptr(out), 0))
+        return out
+
+    def close(self):
+        if self.handle:
+            self.lib.lc_geom3d_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _geom3(grid: PolarGrid, recon_dims, ctx):
+    """device geometry of a 3-D grid for recon_dims (PSF built on demand), cached on the grid"""
+    ctx = ctx or default_context()
+    key = (id(ctx), tuple(int(d) for d in recon_dims))
+    cache = grid.__dict__.setdefault("_geoms", {})
+    if key not in cache:
+        cache[key] = PointSpreadFunction(grid, key[1], ctx=ctx, lazy=True)
+    return cache[key]
+
+
+def forward(volume, grid: PolarGrid, ctx: "Context | None" = None):
+    """forward(volume, grid) (src/projection.cpp:42-64): projections
+    {nproj, nrows, ncols} = Re(ifftshift2(ifft2(fftshift2(w . T2 volume))))/(nrows ncols)."""
+    v = _f32(volume)
+    if v.ndim != 3:
+        raise ValueError("forward: volume must be {n1, n2, n3}")
+    g = _geom3(grid, v.shape, ctx)
+    out = np.empty(grid.dims(), np.float32)
+    _lib.check(g.lib.lc_forward3d(g.ctx.handle, g.handle, _ptr(v), _ptr(out), 0))
+    return out
+
+
+def backproj(projections, grid: PolarGrid, recon_dims, shifts=None, ctx: "Context | None" = None):
+    """backproj(projections, grid, recon_dims[, shifts]) (src/projection.cpp:71-131)."""
+    p = _f32(projections)
+    if p.shape != grid.dims():
+        raise ValueError(f"backproj: projections {p.shape} do not match the grid {grid.dims()}")
+    sh = None
+    if shifts is not None:
+        sh = np.ascontiguousarray(shifts, dtype=np.float64)
+        if sh.shape != (p.shape[0], 2):
+            # include/projection.h / src/projection.cpp:105-108 throw std::invalid_argument
+            raise ValueError("shifts size must match the number of slices in the projection")
+    g = _geom3(grid, recon_dims, ctx)
+    out = np.empty(tuple(int(d) for d in recon_dims), np.float32)
+    _lib.check(g.lib.lc_backproj3d(g.ctx.handle, g.handle, _ptr(p),
+                                   None if sh is None else _ptr(sh), _ptr(out), 0))
+    return out
+
+
+def sysmat(x, op, ctx: "Context | None" = None):
+    """sysmat(x, psf) = psf.convolve(x) (src/gradient.cpp:49-52);
+    sysmat(x, grid) = Re(T1(T2 x))/(nrows ncols) (src/gradient.cpp:32-41)."""
+    if isinstance(op, PointSpreadFunction):
+        return op.convolve(x)
+    if not isinstance(op, PolarGrid):
+        raise TypeError("sysmat: second argument must be a PolarGrid or PointSpreadFunction")
+    v = _f32(x)
+    g = _geom3(op, v.shape, ctx)
+    out = np.empty_like(v)
+    _lib.check(g.lib.lc_sysmat3d_nufft(g.ctx.handle, g.handle, _ptr(v), _ptr(out), 0))
+    return out
+
+
 def sysmat2d(x, op, ctx: "Context | None" = None):
     """sysmat2d(x, psf) = psf.convolve(x) (src/gradient.cpp:76-79);
     sysmat2d(x, grid) = Re(T1(T2 x))/nradial (src/gradient.cpp:59-69)."""
@@ -261,14 +390,22 @@ def _stack(x, N):
     return x, x.size // (N * N)
 
 
-def cgsolver(psf: PointSpreadFunction2D, y, x0, max_iter: int, tol: float, xtol: float,
-             graph: bool = False):
+def cgsolver(psf, y, x0, max_iter: int, tol: float, xtol: float, graph: bool = False):
     """opt::cgsolver with A = sysmat2d(., psf) on one stack (global alpha/beta
-    over every slice, as the reference).  Returns (x, report dict)."""
-    y, ns = _stack(y, psf.image_size)
-    x = _f32(x0).copy()
+    over every slice, as the reference) or A = sysmat(., psf) on one 3-D
+    volume.  Returns (x, report dict)."""
     rep = LcReport()
     flags = LC_GRAPH if graph else 0
+    if isinstance(psf, PointSpreadFunction):
+        y = _f32(y)
+        x = _f32(x0).reshape(y.shape).copy()
+        if not np.any(x):
+            flags |= LC_X0_ZERO
+        _lib.check(psf.lib.lc_cg3d(psf.ctx.handle, psf.handle, _ptr(y), _ptr(x), int(max_iter),
+                                   float(tol), float(xtol), flags, C.byref(rep)))
+        return x, rep.as_dict()
+    y, ns = _stack(y, psf.image_size)
+    x = _f32(x0).copy()
     if not np.any(x):
         flags |= LC_X0_ZERO
     _lib.check(psf.lib.lc_cg2d(psf.ctx.handle, psf.handle, _ptr(y), _ptr(x), ns, int(max_iter),
@@ -276,17 +413,25 @@ def cgsolver(psf: PointSpreadFunction2D, y, x0, max_iter: int, tol: float, xtol:
     return x, rep.as_dict()
 
 
-def split_bregman(psf: PointSpreadFunction2D, y, x0, lam: float, mu: float, outer_max: int,
+def split_bregman(psf, y, x0, lam: float, mu: float, outer_max: int,
                   inner_max: int, tol: float, xtol: float, zero_pad: bool = False,
                   aniso: bool = False, graph: bool = False):
     """opt::split_bregman with A = sysmat2d(., psf).  zero_pad selects the
     zero-padded finite differences of the reference's GPU path instead of
     the interior-only ones of its CPU path."""
-    y, ns = _stack(y, psf.image_size)
-    x = _f32(x0).copy()
     rep = LcReport()
     flags = (LC_GRAPH if graph else 0) | (LC_TV_ZERO_PAD if zero_pad else 0) | \
         (LC_TV_ANISO if aniso else 0)
+    if isinstance(psf, PointSpreadFunction):
+        y = _f32(y)
+        x = _f32(x0).reshape(y.shape).copy()
+        _lib.check(psf.lib.lc_split_bregman3d(psf.ctx.handle, psf.handle, _ptr(y), _ptr(x),
+                                              float(lam), float(mu), int(outer_max),
+                                              int(inner_max), float(tol), float(xtol), flags,
+                                              C.byref(rep)))
+        return x, rep.as_dict()
+    y, ns = _stack(y, psf.image_size)
+    x = _f32(x0).copy()
     _lib.check(psf.lib.lc_split_bregman2d(psf.ctx.handle, psf.handle, _ptr(y), _ptr(x), ns,
                                           float(lam), float(mu), int(outer_max), int(inner_max),
                                           float(tol), float(xtol), flags, C.byref(rep)))
@@ -338,23 +483,32 @@ class Batch2D:
     X, Y, AX = 0, 1, 2
 
     def __init__(self, psfs, ns=None, ctx: Context | None = None):
+        """psfs: PointSpreadFunction2D (a stack of ns[i] slices) or
+        PointSpreadFunction (one 3-D volume) per problem."""
         self.ctx = ctx or psfs[0].ctx
         self.lib = self.ctx.lib
         self.psfs = list(psfs)
         n = len(self.psfs)
         self.ns = [1] * n if ns is None else [int(v) for v in ns]
-        geoms = (C.c_void_p * n)(*[p.handle.value for p in self.psfs])
+        is3 = [isinstance(p, PointSpreadFunction) for p in self.psfs]
+        for i, p in enumerate(self.psfs):
+            if is3[i]:
+                self.ns[i] = p.recon_dims[0]
+        g2 = (C.c_void_p * n)(*[None if is3[i] else p.handle.value for i, p in enumerate(self.psfs)])
+        g3 = (C.c_void_p * n)(*[p.handle.value if is3[i] else None for i, p in enumerate(self.psfs)])
         nss = (C.c_int32 * n)(*self.ns)
         h = C.c_void_p()
-        _lib.check(self.lib.lc_batch2d_create(self.ctx.handle, n, geoms, nss, C.byref(h)))
+        _lib.check(self.lib.lc_batch_create(self.ctx.handle, n, g2, g3, nss, C.byref(h)))
         self.handle = h
 
     def __len__(self):
         return len(self.psfs)
 
     def shape(self, i):
-        N = self.psfs[i].image_size
-        return (self.ns[i], N, N)
+        p = self.psfs[i]
+        if isinstance(p, PointSpreadFunction):
+            return p.recon_dims
+        return (self.ns[i], p.image_size, p.image_size)
 
     def set(self, i, which, arr):
         a = _f32(arr)

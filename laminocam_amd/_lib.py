@@ -24,7 +24,9 @@ EXPORTS = [
     "lc_sysmat2d_psf", "lc_forward2d", "lc_backproj2d", "lc_sysmat2d_nufft", "lc_cg2d", "lc_split_bregman2d", "lc_grad", "lc_divergence",
     "lc_laplacian", "lc_batch2d_create", "lc_batch2d_destroy", "lc_batch2d_set",
     "lc_batch2d_get", "lc_batch2d_device_ptr", "lc_batch2d_cg", "lc_batch2d_split_bregman",
-    "lc_batch2d_reports", "lc_batch2d_sysmat", "lc_batch2d_profile", "lc_autotune2d", "lc_set_tuning2d",
+    "lc_batch2d_reports", "lc_batch2d_sysmat", "lc_batch2d_profile",
+    "lc_geom3d_create", "lc_geom3d_destroy", "lc_geom3d_info", "lc_psf3d_kernel_hat",
+    "lc_sysmat3d_psf", "lc_forward3d", "lc_backproj3d", "lc_sysmat3d_nufft", "lc_cg3d", "lc_split_bregman3d", "lc_batch_create", "lc_autotune2d", "lc_set_tuning2d",
     "lc_get_tuning2d",
 ]
 
@@ -94,6 +96,21 @@ def load():
         "lc_batch2d_reports": (C.c_int, [vp, C.POINTER(LcReport)]),
         "lc_batch2d_sysmat": (C.c_int, [vp, i32, i32]),
         "lc_batch2d_profile": (C.c_int, [vp, i32, i32, f64p, C.POINTER(i32)]),
+        "lc_geom3d_create": (C.c_int, [vp, f64p, f64p, f64p, i64, i64, i64, C.POINTER(i64), u32,
+                                       C.POINTER(vp)]),
+        "lc_geom3d_destroy": (C.c_int, [vp]),
+        "lc_geom3d_info": (C.c_int, [vp, C.POINTER(i64)]),
+        "lc_psf3d_kernel_hat": (C.c_int, [vp, vp, f32p]),
+        "lc_sysmat3d_psf": (C.c_int, [vp, vp, vp, vp, u32]),
+        "lc_forward3d": (C.c_int, [vp, vp, vp, vp, u32]),
+        "lc_backproj3d": (C.c_int, [vp, vp, vp, vp, vp, u32]),
+        "lc_sysmat3d_nufft": (C.c_int, [vp, vp, vp, vp, u32]),
+        "lc_cg3d": (C.c_int, [vp, vp, vp, vp, i64, C.c_double, C.c_double, u32,
+                              C.POINTER(LcReport)]),
+        "lc_split_bregman3d": (C.c_int, [vp, vp, vp, vp, C.c_double, C.c_double, i64, i64,
+                                         C.c_double, C.c_double, u32, C.POINTER(LcReport)]),
+        "lc_batch_create": (C.c_int, [vp, i32, C.POINTER(vp), C.POINTER(vp), C.POINTER(i32),
+                                      C.POINTER(vp)]),
         "lc_autotune2d": (C.c_int, [vp, vp, i32, C.POINTER(i32), C.POINTER(C.c_double)]),
         "lc_set_tuning2d": (C.c_int, [vp, i64, C.POINTER(i32)]),
         "lc_get_tuning2d": (C.c_int, [vp, i64, C.POINTER(i32)]),

@@ -68,5 +68,23 @@ def build(verbose=False, jobs=None):
     return LIB
 
 
+CXX_TEST_SRC = os.path.join(HERE, "..", "tests", "cxx", "test_shim.cpp")
+CXX_TEST_BIN = os.path.join(HERE, "..", "tests", "cxx", "_bin", "test_shim")
+
+
+def build_cxx_test():
+    """g++ build of the C++ API test program against the in-tree library."""
+    os.makedirs(os.path.dirname(CXX_TEST_BIN), exist_ok=True)
+    cxx = os.environ.get("CXX") or shutil.which("g++") or "g++"
+    cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-I", os.path.join(HERE, "..", "include"),
+           CXX_TEST_SRC, "-o", CXX_TEST_BIN, "-L", LIBDIR, "-llaminocam",
+           "-Wl,-rpath,$ORIGIN/../../../laminocam_amd/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ test build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return CXX_TEST_BIN
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_cxx_test())

@@ -41,6 +41,11 @@ struct lc_geom2d_s {
     Geom2D g;
 };
 
+struct lc_geom3d_s {
+    lc_ctx_s *ctx;
+    Geom3D g;
+};
+
 struct lc_batch2d_s {
     lc_ctx_s *ctx;
     std::unique_ptr<Batch2D> b;
@@ -140,20 +145,64 @@ int lc_psf2d_kernel_hat(lc_ctx ctx, lc_geom2d g, float *out) {
     });
 }
 
+// one-problem solves and applies, shared by the 2-D and 3-D entry points
+static void sysmat_psf(lc_ctx ctx, ProbSpec ps, const float *x, float *y, uint32_t flags) {
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    ctx->c.set_current();
+    Batch2D b(ctx->c, {ps});
+    const size_t bytes = sizeof(float) * (size_t)b.size(0);   // exactly the problem's size
+    const bool dev = flags & LC_DEVICE_PTRS;
+    copy_in(ctx->c, b.X(), x, bytes, dev);
+    b.toeplitz(b.X(), b.AP());
+    copy_out(ctx->c, y, b.AP(), bytes, dev);
+    ctx->c.sync();
+}
+
+static void cg_one(lc_ctx ctx, ProbSpec ps, const float *rhs, float *x, int64_t max_iter,
+                   double tol, double xtol, uint32_t flags, lc_report *rep) {
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    ctx->c.set_current();
+    Batch2D b(ctx->c, {ps});
+    const size_t bytes = sizeof(float) * (size_t)b.size(0);
+    const bool dev = flags & LC_DEVICE_PTRS;
+    copy_in(ctx->c, b.Y(), rhs, bytes, dev);
+    if (!(flags & LC_X0_ZERO)) copy_in(ctx->c, b.X(), x, bytes, dev);
+    const bool graph = flags & LC_GRAPH;
+    b.cg({(int)max_iter}, tol, xtol, graph, !graph, (flags & LC_X0_ZERO) != 0);
+    copy_out(ctx->c, x, b.X(), bytes, dev);
+    ctx->c.sync();
+    if (rep) fill_report(b.cg_state()[0], rep);
+}
+
+static void sb_one(lc_ctx ctx, ProbSpec ps, const float *rhs, float *x, double lambda, double mu,
+                   int64_t outer, int64_t inner, double tol, double xtol, uint32_t flags,
+                   lc_report *rep) {
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    ctx->c.set_current();
+    Batch2D b(ctx->c, {ps});
+    const size_t bytes = sizeof(float) * (size_t)b.size(0);
+    const bool dev = flags & LC_DEVICE_PTRS;
+    copy_in(ctx->c, b.Y(), rhs, bytes, dev);
+    copy_in(ctx->c, b.X(), x, bytes, dev);
+    const bool graph = flags & LC_GRAPH;
+    b.split_bregman({(float)lambda}, {(float)mu}, (int)outer, (int)inner, tol, xtol,
+                    (flags & LC_TV_ZERO_PAD) ? 1 : 0, (flags & LC_TV_ANISO) != 0, graph, !graph);
+    copy_out(ctx->c, x, b.X(), bytes, dev);
+    ctx->c.sync();
+    if (rep) {
+        fill_report(b.cg_state()[0], rep);
+        const SbState s = b.sb_state()[0];
+        rep->outer = s.outer;
+        rep->dx = s.norm_diff;
+    }
+}
+
 int lc_sysmat2d_psf(lc_ctx ctx, lc_geom2d g, const float *x, int64_t nslices, float *y,
                     uint32_t flags) {
     return guard([&] {
         LC_REQUIRE(ctx && g && x && y, "lc_sysmat2d_psf: NULL argument");
         LC_REQUIRE(nslices >= 1, "lc_sysmat2d_psf: nslices must be >= 1");
-        std::lock_guard<std::mutex> lk(ctx->c.mu);
-        ctx->c.set_current();
-        Batch2D b(ctx->c, {ProbSpec{&g->g, (int)nslices}});
-        const size_t bytes = sizeof(float) * (size_t)b.size(0);   // exactly ns*N*N
-        const bool dev = flags & LC_DEVICE_PTRS;
-        copy_in(ctx->c, b.X(), x, bytes, dev);
-        b.toeplitz(b.X(), b.AP());
-        copy_out(ctx->c, y, b.AP(), bytes, dev);
-        ctx->c.sync();
+        sysmat_psf(ctx, ProbSpec{&g->g, (int)nslices}, x, y, flags);
     });
 }
 
@@ -207,18 +256,7 @@ int lc_cg2d(lc_ctx ctx, lc_geom2d g, const float *rhs, float *x, int64_t nslices
     return guard([&] {
         LC_REQUIRE(ctx && g && rhs && x, "lc_cg2d: NULL argument");
         LC_REQUIRE(nslices >= 1 && max_iter >= 0, "lc_cg2d: bad sizes");
-        std::lock_guard<std::mutex> lk(ctx->c.mu);
-        ctx->c.set_current();
-        Batch2D b(ctx->c, {ProbSpec{&g->g, (int)nslices}});
-        const size_t bytes = sizeof(float) * (size_t)b.size(0);   // exactly ns*N*N
-        const bool dev = flags & LC_DEVICE_PTRS;
-        copy_in(ctx->c, b.Y(), rhs, bytes, dev);
-        copy_in(ctx->c, b.X(), x, bytes, dev);
-        const bool graph = flags & LC_GRAPH;
-        b.cg({(int)max_iter}, tol, xtol, graph, !graph, (flags & LC_X0_ZERO) != 0);
-        copy_out(ctx->c, x, b.X(), bytes, dev);
-        ctx->c.sync();
-        if (rep) fill_report(b.cg_state()[0], rep);
+        cg_one(ctx, ProbSpec{&g->g, (int)nslices}, rhs, x, max_iter, tol, xtol, flags, rep);
     });
 }
 
@@ -228,25 +266,132 @@ int lc_split_bregman2d(lc_ctx ctx, lc_geom2d g, const float *rhs, float *x, int6
     return guard([&] {
         LC_REQUIRE(ctx && g && rhs && x, "lc_split_bregman2d: NULL argument");
         LC_REQUIRE(nslices >= 1 && outer >= 0 && inner >= 0, "lc_split_bregman2d: bad sizes");
+        sb_one(ctx, ProbSpec{&g->g, (int)nslices}, rhs, x, lambda, mu, outer, inner, tol, xtol,
+               flags, rep);
+    });
+}
+
+// ---- 3-D ----
+int lc_geom3d_create(lc_ctx ctx, const double *theta, const double *gamma, const double *beta,
+                     int64_t nproj, int64_t nrows, int64_t ncols, const int64_t *recon_dims,
+                     uint32_t flags, lc_geom3d *out) {
+    return guard([&] {
+        LC_REQUIRE(ctx && theta && gamma && beta && recon_dims && out, "lc_geom3d_create: NULL argument");
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         ctx->c.set_current();
-        Batch2D b(ctx->c, {ProbSpec{&g->g, (int)nslices}});
-        const size_t bytes = sizeof(float) * (size_t)b.size(0);   // exactly ns*N*N
-        const bool dev = flags & LC_DEVICE_PTRS;
-        copy_in(ctx->c, b.Y(), rhs, bytes, dev);
-        copy_in(ctx->c, b.X(), x, bytes, dev);
-        const bool graph = flags & LC_GRAPH;
-        b.split_bregman({(float)lambda}, {(float)mu}, (int)outer, (int)inner, tol, xtol,
-                        (flags & LC_TV_ZERO_PAD) ? 1 : 0, (flags & LC_TV_ANISO) != 0, graph,
-                        !graph);
-        copy_out(ctx->c, x, b.X(), bytes, dev);
-        ctx->c.sync();
-        if (rep) {
-            fill_report(b.cg_state()[0], rep);
-            const SbState s = b.sb_state()[0];
-            rep->outer = s.outer;
-            rep->dx = s.norm_diff;
+        auto *g = new lc_geom3d_s();
+        g->ctx = ctx;
+        try {
+            geom3d_init(ctx->c, g->g, theta, gamma, beta, nproj, nrows, ncols, recon_dims);
+            if (!(flags & LC_GEOM_LAZY_PSF)) geom3d_build_psf(ctx->c, g->g);
+            ctx->c.sync();
+        } catch (...) {
+            delete g;
+            throw;
         }
+        *out = g;
+    });
+}
+
+int lc_geom3d_destroy(lc_geom3d g) {
+    return guard([&] {
+        if (g) {
+            g->ctx->c.set_current();
+            g->ctx->c.sync();
+        }
+        delete g;
+    });
+}
+
+int lc_geom3d_info(lc_geom3d g, int64_t *info) {
+    return guard([&] {
+        LC_REQUIRE(g && info, "lc_geom3d_info: NULL argument");
+        const Geom3D &q = g->g;
+        const int64_t v[9] = {q.nproj, q.nrows, q.ncols, q.n1, q.n2, q.n3, q.M1, q.M, q.M};
+        for (int i = 0; i < 9; ++i) info[i] = v[i];
+    });
+}
+
+int lc_psf3d_kernel_hat(lc_ctx ctx, lc_geom3d g, float *out) {
+    return guard([&] {
+        LC_REQUIRE(ctx && g && out, "lc_psf3d_kernel_hat: NULL argument");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        ctx->c.set_current();
+        geom3d_kernel_hat(ctx->c, g->g, out);
+    });
+}
+
+int lc_sysmat3d_psf(lc_ctx ctx, lc_geom3d g, const float *x, float *y, uint32_t flags) {
+    return guard([&] {
+        LC_REQUIRE(ctx && g && x && y, "lc_sysmat3d_psf: NULL argument");
+        ProbSpec ps{nullptr, (int)g->g.n1, &g->g};
+        sysmat_psf(ctx, ps, x, y, flags);
+    });
+}
+
+// forward (mode 0) / backproj (1) / NUFFT sysmat (2) on one volume
+static int project3d(lc_ctx ctx, lc_geom3d g, const float *in, float *out, const double *shifts,
+                     uint32_t flags, int mode) {
+    return guard([&] {
+        LC_REQUIRE(ctx && g && in && out, "3-D projector: NULL argument");
+        LC_REQUIRE(g->ctx == ctx, "3-D projector: geometry of another context");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        ctx->c.set_current();
+        const Geom3D &q = g->g;
+        const int64_t nv = q.n1 * q.n2 * q.n3, np = q.nproj * q.nrows * q.ncols;
+        const int64_t nin = mode == 1 ? np : nv, nout = mode == 0 ? np : nv;
+        const bool dev = flags & LC_DEVICE_PTRS;
+        const float *din = in;
+        float *dout = out;
+        if (!dev) {
+            float *buf = (float *)ctx->c.scratch(2, sizeof(float) * (size_t)(nin + nout));
+            copy_in(ctx->c, buf, in, sizeof(float) * nin, false);
+            din = buf;
+            dout = buf + nin;
+        }
+        DBuf dsh;
+        if (mode == 1 && shifts) {
+            dsh.alloc(sizeof(double) * 2 * q.nproj);
+            copy_in(ctx->c, dsh.as(), shifts, sizeof(double) * 2 * q.nproj, dev);
+        }
+        if (mode == 0) forward3d_dev(ctx->c, g->g, din, dout);
+        else if (mode == 1) backproj3d_dev(ctx->c, g->g, din, dout, shifts ? dsh.as<double>() : nullptr);
+        else sysmat3d_nufft_dev(ctx->c, g->g, din, dout);
+        if (!dev) copy_out(ctx->c, out, dout, sizeof(float) * nout, false);
+        ctx->c.sync();
+    });
+}
+
+int lc_forward3d(lc_ctx ctx, lc_geom3d g, const float *vol, float *proj, uint32_t flags) {
+    return project3d(ctx, g, vol, proj, nullptr, flags, 0);
+}
+
+int lc_backproj3d(lc_ctx ctx, lc_geom3d g, const float *proj, const double *shifts, float *vol,
+                  uint32_t flags) {
+    return project3d(ctx, g, proj, vol, shifts, flags, 1);
+}
+
+int lc_sysmat3d_nufft(lc_ctx ctx, lc_geom3d g, const float *x, float *y, uint32_t flags) {
+    return project3d(ctx, g, x, y, nullptr, flags, 2);
+}
+
+int lc_cg3d(lc_ctx ctx, lc_geom3d g, const float *rhs, float *x, int64_t max_iter, double tol,
+            double xtol, uint32_t flags, lc_report *rep) {
+    return guard([&] {
+        LC_REQUIRE(ctx && g && rhs && x, "lc_cg3d: NULL argument");
+        LC_REQUIRE(max_iter >= 0, "lc_cg3d: bad max_iter");
+        cg_one(ctx, ProbSpec{nullptr, (int)g->g.n1, &g->g}, rhs, x, max_iter, tol, xtol, flags, rep);
+    });
+}
+
+int lc_split_bregman3d(lc_ctx ctx, lc_geom3d g, const float *rhs, float *x, double lambda,
+                       double mu, int64_t outer, int64_t inner, double tol, double xtol,
+                       uint32_t flags, lc_report *rep) {
+    return guard([&] {
+        LC_REQUIRE(ctx && g && rhs && x, "lc_split_bregman3d: NULL argument");
+        LC_REQUIRE(outer >= 0 && inner >= 0, "lc_split_bregman3d: bad sizes");
+        sb_one(ctx, ProbSpec{nullptr, (int)g->g.n1, &g->g}, rhs, x, lambda, mu, outer, inner, tol,
+               xtol, flags, rep);
     });
 }
 
@@ -292,14 +437,29 @@ int lc_laplacian(lc_ctx ctx, const float *u, int64_t n1, int64_t n2, int64_t n3,
 
 int lc_batch2d_create(lc_ctx ctx, int32_t nprob, const lc_geom2d *geoms, const int32_t *ns,
                       lc_batch2d *out) {
+    return lc_batch_create(ctx, nprob, geoms, nullptr, ns, out);
+}
+
+int lc_batch_create(lc_ctx ctx, int32_t nprob, const lc_geom2d *g2, const lc_geom3d *g3,
+                    const int32_t *ns, lc_batch2d *out) {
     return guard([&] {
-        LC_REQUIRE(ctx && geoms && ns && out && nprob >= 1, "lc_batch2d_create: bad argument");
+        LC_REQUIRE(ctx && out && nprob >= 1 && (g2 || g3), "lc_batch_create: bad argument");
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         ctx->c.set_current();
         std::vector<ProbSpec> ps;
         for (int i = 0; i < nprob; ++i) {
-            LC_REQUIRE(geoms[i] && geoms[i]->ctx == ctx, "lc_batch2d_create: geometry of another context");
-            ps.push_back(ProbSpec{&geoms[i]->g, ns[i]});
+            const lc_geom2d a = g2 ? g2[i] : nullptr;
+            const lc_geom3d v = g3 ? g3[i] : nullptr;
+            LC_REQUIRE((a != nullptr) != (v != nullptr),
+                       "lc_batch_create: each problem needs exactly one geometry");
+            if (a) {
+                LC_REQUIRE(a->ctx == ctx, "lc_batch_create: geometry of another context");
+                LC_REQUIRE(ns && ns[i] >= 1, "lc_batch_create: ns[i] must be >= 1 for a 2-D problem");
+                ps.push_back(ProbSpec{&a->g, ns[i]});
+            } else {
+                LC_REQUIRE(v->ctx == ctx, "lc_batch_create: geometry of another context");
+                ps.push_back(ProbSpec{nullptr, (int)v->g.n1, &v->g});
+            }
         }
         auto *b = new lc_batch2d_s();
         b->ctx = ctx;

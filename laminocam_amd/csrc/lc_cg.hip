@@ -470,35 +470,66 @@ Batch2D::Batch2D(Ctx &c, const std::vector<ProbSpec> &probs) : c_(c), probs_(pro
     build();
 }
 
+void Batch2D::toep_mid(const ToepClass &k, const CgState *st, float2 *U) {
+    if (k.g3) toep3d_mid(c_, k, st, U);
+    else toep_launch_cols(c_, k, dprob_.as<ProbDev>(), st, U);
+}
+
 void Batch2D::build() {
     const int np = (int)probs_.size();
     hprob_.resize(np);
-    // group slices by N into classes
+    // 2-D problems: classes by N (slices of every problem of that size share
+    // each launch); 3-D volumes: one class each
     std::map<int, std::vector<int>> by_n;
+    std::vector<int> vols;
     for (int p = 0; p < np; ++p) {
+        LC_REQUIRE(probs_[p].ns >= 1, "batch: ns must be >= 1");
+        if (Geom3D *g3 = probs_[p].g3) {
+            LC_REQUIRE(probs_[p].g == nullptr, "batch: problem with both a 2-D and a 3-D geometry");
+            if (!g3->psf_ready) geom3d_build_psf(c_, *g3);
+            probs_[p].ns = (int)g3->n1;
+            vols.push_back(p);
+            continue;
+        }
         Geom2D *g = probs_[p].g;
         LC_REQUIRE(g, "batch: NULL geometry");
         if (!g->psf_ready) geom2d_build_psf(c_, *g);
-        LC_REQUIRE(probs_[p].ns >= 1, "batch: ns must be >= 1");
         by_n[(int)g->N].push_back(p);
     }
+    std::vector<std::pair<int, std::vector<int>>> groups;
+    for (auto &kv : by_n) groups.push_back(kv);
+    for (int p : vols) groups.push_back({(int)probs_[p].g3->N, {p}});
     // vector layout: problems in class order so every class is contiguous
     int64_t off = 0, uoff = 0;
     int chunk = 0, slot = 0;
-    std::vector<int> order;
-    for (auto &kv : by_n)
-        for (int p : kv.second) order.push_back(p);
     // host tables stay alive until the uploads below have completed
     std::vector<std::vector<SliceRef>> keep;
     // classes
-    for (auto &kv : by_n) {
+    for (auto &kv : groups) {
         auto k = std::make_unique<ToepClass>();
+        const Geom3D *g3 = probs_[kv.second.front()].g3;
         k->N = kv.first;
-        k->M = (int)next_fast_dim(2 * k->N - 1);
+        k->M = g3 ? (int)g3->M : (int)next_fast_dim(2 * k->N - 1);
         k->H = k->M / 2 + 1;
         LC_REQUIRE(fft_make_plan(k->M, k->plan), "batch: PSF size not 2^p 3^q");
         k->tw = c_.twiddles<float>(k->M);
-        {
+        if (g3) {
+            // runtime-plan shapes: row shape by M, column blocks B * M <= the 3-D cap
+            ToepVariant v;
+            v.row = k->M > 2048 ? 1 : 0;
+            v.col = 2;
+            v.B = 0;
+            toep_configure(*k, v);
+            LC_REQUIRE(k->B * k->M <= toep3d_col_cap(), "batch: 3-D column block too large");
+            k->g3 = g3;
+            k->n1 = (int)g3->n1;
+            k->M1 = (int)g3->M1;
+            LC_REQUIRE(fft_make_plan(k->M1, k->plan1), "batch: 3-D slice-axis size not 2^p 3^q");
+            k->tw1 = c_.twiddles<float>(k->M1);
+            LC_REQUIRE(k->B * k->M1 <= toep3d_slice_cap(), "batch: 3-D slice axis too long");
+            k->kb2 = std::max(1, toep3d_slice_cap() / (k->B * k->M1));
+            k->W.alloc(sizeof(float2) * (size_t)k->nblk * k->M * k->n1 * k->B);
+        } else {
             auto it = c_.toep_tuning.find(k->N);
             toep_configure(*k, it != c_.toep_tuning.end() ? it->second : toep_default_variant(k->M));
         }
@@ -506,7 +537,7 @@ void Batch2D::build() {
         for (int p : kv.second) {
             const int N = k->N, ns = probs_[p].ns;
             ProbDev &d = hprob_[p];
-            d.khatT = probs_[p].g->khatT.as<float>();
+            d.khatT = g3 ? nullptr : probs_[p].g->khatT.as<float>();
             d.N = N;
             d.ns = ns;
             d.off = off;
@@ -587,7 +618,7 @@ void Batch2D::apply_op(const float *src, float *dst, bool sb, bool dot) {
     CgState *st = dstate_.as<CgState>();
     for (auto &k : classes_) {
         toep_launch_fwd(c_, *k, st, src, nullptr, nullptr, U_.as<float2>(), false);
-        toep_launch_cols(c_, *k, dprob_.as<ProbDev>(), st, U_.as<float2>());
+        toep_mid(*k, st, U_.as<float2>());
         toep_launch_inv(c_, *k, st, U_.as<float2>(), dst, src, partial_.as<double>(),
                         dot && !sb);
     }
@@ -658,7 +689,7 @@ std::vector<Batch2D::PassTiming> Batch2D::profile_passes(int reps) {
             LC_HIP(hipEventRecord(ev[4 * i], c_.stream()));
             toep_launch_fwd(c_, k, st, P(), R(), P(), U, true);
             LC_HIP(hipEventRecord(ev[4 * i + 1], c_.stream()));
-            toep_launch_cols(c_, k, dprob_.as<ProbDev>(), st, U);
+            toep_mid(k, st, U);
             LC_HIP(hipEventRecord(ev[4 * i + 2], c_.stream()));
             toep_launch_inv(c_, k, st, U, AP(), P(), partial_.as<double>(), true);
             LC_HIP(hipEventRecord(ev[4 * i + 3], c_.stream()));
@@ -677,9 +708,10 @@ std::vector<Batch2D::PassTiming> Batch2D::profile_passes(int reps) {
     for (int i = 0; i < nk; ++i) {
         const ToepClass &k = *classes_[i];
         const double N = k.N, H = k.H, M = k.M, ns = k.nslices;
-        const double bytes[3] = {ns * (12.0 * N * N + 8.0 * N * H),
-                                 ns * 16.0 * N * H + 4.0 * M * H,
-                                 ns * (8.0 * N * H + 8.0 * N * N)};
+        double bytes[3] = {ns * (12.0 * N * N + 8.0 * N * H), ns * 16.0 * N * H + 4.0 * M * H,
+                           ns * (8.0 * N * H + 8.0 * N * N)};
+        if (k.g3)   // column FFTs to and from W, slice-axis pass over W, 3-D spectrum
+            bytes[1] = ns * (16.0 * N * H + 16.0 * M * H) + 16.0 * M * H * ns + 4.0 * M * H * k.M1;
         for (int q = 0; q < 3; ++q)
             out.push_back(PassTiming{k.N, q, k.nslices, acc[3 * i + q] / reps, bytes[q]});
     }
@@ -763,7 +795,7 @@ void Batch2D::cg_iteration(bool host_sync, bool sb) {
     };
     for (auto &k : classes_) {
         toep_launch_fwd(c_, *k, st, P(), R(), P(), U, true);
-        toep_launch_cols(c_, *k, dprob_.as<ProbDev>(), st, U);
+        toep_mid(*k, st, U);
         toep_launch_inv(c_, *k, st, U, AP(), P(), partial_.as<double>(), !sb);
     }
     if (sb) {

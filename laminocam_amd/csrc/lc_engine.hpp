@@ -47,6 +47,73 @@ void sysmat2d_nufft_dev(Ctx &c, Geom2D &g, const float *x, float *y, int ns);
 // next_fast_dim of include/toeplitz2d.h:48-57
 int64_t next_fast_dim(int64_t n);
 
+// D_n(t) = sin(pi t) / sin(pi t / n) evaluated at t = k n + d as
+// (-1)^{k(n-1)} sin(pi d) / sin(pi d / n), exact at the removable zeros
+__device__ __forceinline__ double dirichlet_d(double t, int n, double inv_n) {
+    const double kq = rint(t * inv_n);
+    const double d = fma(-kq, (double)n, t);
+    const long long k = (long long)kq;
+    const double sg = ((k * (long long)(n - 1)) & 1LL) ? -1.0 : 1.0;
+    if (d == 0.0) return sg * (double)n;
+    return sg * sinpi(d) / sinpi(d * inv_n);
+}
+
+// ---------------------------------------------------------------------------
+// Geometry of one 3-D laminography grid (include/polar_grid.h:52-94): per
+// projection i the points are R_i (qX_k, qY_j, 0) with R_i =
+// RotationTranspose(theta_i, gamma_i, beta_i) (include/rotation.h:28-56),
+// qX_k = (k + 1/2) 2 pi / ncols - pi, qY_j = (j + 1/2) 2 pi / nrows - pi.
+// Only the first two columns u_i, v_i of R_i are kept; points are generated
+// on the fly.  Plus the Toeplitz PSF spectrum of include/toeplitz.h:61-129 for
+// a {n1, N, N} volume.
+// ---------------------------------------------------------------------------
+// one detector axis of length n: Stockham plan, or Bluestein (plan of the
+// padded length, chirp e^{i pi k^2/n}, chirp spectra for both signs / L)
+struct DetLine {
+    int n = 0;
+    bool blue = false;
+    FftPlan plan{};
+    DBuf chirp, bhat;
+};
+void det_line_init(DetLine &d, int n);
+
+// NUFFT state of one 3-D grid: fine grid (z, y, x), per-axis corrections
+struct Nufft3D {
+    int nf[3] = {0, 0, 0};
+    int coff[3] = {0, 0, 0};   // offsets of the per-axis factors in corr
+    double beta = 0.0;
+    DBuf corr;                 // float[n1 + n2 + n3]
+    FftPlan plan[3]{};
+    DetLine det_x, det_y;      // detector columns (ncols) and rows (nrows)
+    bool ready = false;
+};
+
+struct Geom3D {
+    int64_t nproj = 0, nrows = 0, ncols = 0;
+    int64_t n1 = 0, n2 = 0, n3 = 0;   // recon dims
+    int64_t N = 0;                    // n2 (== n3 for the Toeplitz operator)
+    int64_t M1 = 0, M = 0, H = 0;     // PSF dims {M1, M, M}; H = M/2 + 1
+    std::vector<double> rot;          // per projection: u (3) then v (3)
+    DBuf d_rot;                       // double[nproj][6]
+    DBuf khat3;                       // float[M][H][M1]: real spectrum of the centred PSF / (M1 M^2 N^2)
+    bool psf_ready = false;
+    Nufft3D nu;
+};
+
+// 3-D projectors on device buffers (src/projection.cpp:42-131, src/gradient.cpp:32-41)
+void nufft3d_init(Ctx &c, Geom3D &g);
+void forward3d_dev(Ctx &c, Geom3D &g, const float *vol, float *proj);
+void backproj3d_dev(Ctx &c, Geom3D &g, const float *proj, float *vol, const double *shifts);
+void sysmat3d_nufft_dev(Ctx &c, Geom3D &g, const float *x, float *y);
+
+void geom3d_init(Ctx &c, Geom3D &g, const double *theta, const double *gamma, const double *beta,
+                 int64_t nproj, int64_t nrows, int64_t ncols, const int64_t *recon_dims);
+// FP64 closed-form PSF P(m) = sum_i D_ncols(m.u_i) D_nrows(m.v_i), FP64 3-D FFT,
+// stored as the FP32 real spectrum
+void geom3d_build_psf(Ctx &c, Geom3D &g);
+// reference-layout kernel_hat (complex {M1, M, M/2+1}, unscaled) to host
+void geom3d_kernel_hat(Ctx &c, Geom3D &g, float *out_host);
+
 void geom2d_init(Ctx &c, Geom2D &g, const double *theta, int64_t nang, int64_t nrad,
                  int64_t image_n);
 // FP64 closed-form PSF + FP64 FFT, stored as FP32 real spectrum
@@ -123,6 +190,13 @@ struct ToepGeom {
 
 struct ToepClass {
     int N = 0, M = 0, H = 0;
+    // 3-D volume class (one problem): the middle pass becomes column FFT,
+    // slice-axis convolution, inverse column FFT through W
+    const Geom3D *g3 = nullptr;
+    int n1 = 0, M1 = 0, kb2 = 1;      // slices, slice-axis FFT length, k2 rows per block
+    FftPlan plan1{};
+    const float2 *tw1 = nullptr;
+    DBuf W;                           // [nblk][M][n1][B] complex
     FftPlan plan{};
     const float2 *tw = nullptr;
     int nslices = 0;
@@ -146,9 +220,16 @@ int toep_col_cap(int col);
 ToepVariant toep_autotune(Ctx &c, Geom2D &g, int nslices, double *best_ms);
 
 struct ProbSpec {
-    Geom2D *g;
+    Geom2D *g;             // 2-D: a stack of ns slices on one polar grid
     int ns;
+    Geom3D *g3 = nullptr;  // 3-D: one {n1, N, N} volume (g == nullptr, ns == n1)
 };
+
+// middle Toeplitz pass of a 3-D volume class (three launches) and its
+// block-size limits (B * M for the column FFTs, B * M1 for the slice axis)
+void toep3d_mid(Ctx &c, const ToepClass &k, const CgState *st, float2 *U);
+int toep3d_col_cap();
+int toep3d_slice_cap();
 
 // ---------------------------------------------------------------------------
 // A batch of independent 2-D problems (each a stack of ns slices that share
@@ -206,6 +287,8 @@ class Batch2D {
     void cg_setup(bool x0_zero, bool sb, const std::vector<int> &max_iter, double tol,
                   double xtol);
     void apply_op(const float *src, float *dst, bool sb, bool dot);
+    // middle pass of a class: 2-D column pass or the 3-D slice-coupled pass
+    void toep_mid(const ToepClass &k, const CgState *st, float2 *U);
 
     Ctx &c_;
     std::vector<ProbSpec> probs_;

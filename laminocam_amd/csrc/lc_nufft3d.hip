@@ -1,0 +1,754 @@
+// 3-D laminography NUFFT projectors on gfx950: forward, backproj (with
+// optional centre-of-rotation shifts) and the NUFFT normal operator
+// (src/projection.cpp:42-131, src/gradient.cpp:32-41).
+//
+//   forward(x)  = Re(ifftshift2(ifft2(fftshift2(w . T2 x)))) / (nrows ncols)
+//   backproj(y) = Re(T1(w . e^{-i(qx dx + qy dy)} . ifftshift2(fft2(fftshift2 y)))) / (nrows ncols)
+//   sysmat(x)   = Re(T1(T2 x)) / (nrows ncols)                (no mask)
+// T1/T2: type-1/type-2 NUFFT on the grid R_i (qX_k, qY_j, 0), modes {n1,n2,n3}
+// with m_x <-> n3, m_y <-> n2, m_z <-> n1 (include/nufft.h:37-58); w = 0 where
+// any coordinate exceeds pi (include/polar_grid.h:80-86).
+//
+// Method as in 2-D: exponential-of-semicircle kernel, width 8, fine grid 2n
+// per axis (2^p 3^q, at least 16), centred so points map to cells without a
+// shift; the (-1)^m phase and 1/psi_hat sit in per-axis correction factors.
+//   type 1: deterministic tile-gather spreading -> fine grid, then one pruned
+//           FFT pass per axis (x, y, z), deconvolution in each pass;
+//   type 2: the reverse passes, then 8x8x8 interpolation per point.
+// Detector transforms are line FFTs along x then y with the reference's shift
+// index maps; widths that are not 2^p 3^q use Bluestein's algorithm with an
+// FP64-built chirp and chirp spectrum.
+#include <cmath>
+#include <complex>
+
+#include "lc_engine.hpp"
+
+namespace lc {
+
+extern __shared__ __align__(16) char lc_smem[];
+
+namespace {
+
+constexpr int W3 = 8;                       // kernel width (cells)
+constexpr int TZ = 8, TY = 16, TX = 16;     // spread tile (cells)
+constexpr int S3_WAVES = 4;
+constexpr int LN_NT = 256, LN_ME = 32;      // line passes: NT * ME complex in LDS
+
+__device__ __forceinline__ float es3(float z, float beta) {
+    const float s = 1.0f - z * z;
+    return s > 0.0f ? __expf(beta * (sqrtf(s) - 1.0f)) : 0.0f;
+}
+
+__device__ __forceinline__ int wrapi(int a, int n) {
+    a %= n;
+    return a < 0 ? a + n : a;
+}
+
+struct Grid3 {
+    int nproj, nrows, ncols;
+    int nf[3];          // fine grid (z, y, x)
+    double dX, dY;      // lattice steps 2 pi / ncols, 2 pi / nrows
+    float beta;
+    int mask;           // 1: drop points with any |coordinate| > pi
+};
+
+__device__ __forceinline__ void point3(const double *rot, int p, int j, int k, const Grid3 &g,
+                                       double &x, double &y, double &z) {
+    const double *r = rot + 6 * (int64_t)p;
+    const double qx = ((double)k + 0.5) * g.dX - M_PI;
+    const double qy = ((double)j + 0.5) * g.dY - M_PI;
+    x = qx * r[0] + qy * r[3];
+    y = qx * r[1] + qy * r[4];
+    z = qx * r[2] + qy * r[5];
+}
+
+__device__ __forceinline__ bool inside_pi(double x, double y, double z) {
+    return fabs(x) <= M_PI && fabs(y) <= M_PI && fabs(z) <= M_PI;
+}
+
+// ---------------------------------------------------------------------------
+// type-2 interpolation: one thread per point
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_interp3d(const float2 *__restrict__ F, float2 *__restrict__ C, const double *__restrict__ rot,
+           Grid3 g) {
+    const int64_t npts = (int64_t)g.nproj * g.nrows * g.ncols;
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= npts) return;
+    const int k = (int)(q % g.ncols);
+    const int64_t t = q / g.ncols;
+    const int j = (int)(t % g.nrows), p = (int)(t / g.nrows);
+    double x, y, z;
+    point3(rot, p, j, k, g, x, y, z);
+    if (g.mask && !inside_pi(x, y, z)) {
+        C[q] = make_float2(0.f, 0.f);
+        return;
+    }
+    const double hw = 0.5 * W3;
+    const float ihw = 1.0f / (float)hw;
+    const double gz = 0.5 * g.nf[0] + z * (g.nf[0] / (2.0 * M_PI));
+    const double gy = 0.5 * g.nf[1] + y * (g.nf[1] / (2.0 * M_PI));
+    const double gx = 0.5 * g.nf[2] + x * (g.nf[2] / (2.0 * M_PI));
+    const int z0 = (int)ceil(gz - hw), y0 = (int)ceil(gy - hw), x0 = (int)ceil(gx - hw);
+    float wz[W3], wy[W3], wx[W3];
+    int xi[W3];
+#pragma unroll
+    for (int d = 0; d < W3; ++d) {
+        wz[d] = es3((float)((double)(z0 + d) - gz) * ihw, g.beta);
+        wy[d] = es3((float)((double)(y0 + d) - gy) * ihw, g.beta);
+        wx[d] = es3((float)((double)(x0 + d) - gx) * ihw, g.beta);
+        xi[d] = wrapi(x0 + d, g.nf[2]);
+    }
+    float re = 0.f, im = 0.f;
+    for (int a = 0; a < W3; ++a) {
+        const int zz = wrapi(z0 + a, g.nf[0]);
+        float pr = 0.f, pi = 0.f;
+        for (int b = 0; b < W3; ++b) {
+            const int yy = wrapi(y0 + b, g.nf[1]);
+            const float2 *row = F + ((int64_t)zz * g.nf[1] + yy) * g.nf[2];
+            float rr = 0.f, ri = 0.f;
+#pragma unroll
+            for (int c = 0; c < W3; ++c) {
+                const float2 v = row[xi[c]];
+                rr += wx[c] * v.x;
+                ri += wx[c] * v.y;
+            }
+            pr += wy[b] * rr;
+            pi += wy[b] * ri;
+        }
+        re += wz[a] * pr;
+        im += wz[a] * pi;
+    }
+    C[q] = make_float2(re, im);
+}
+
+// ---------------------------------------------------------------------------
+// type-1 spreading: each workgroup owns a TZ x TY x TX tile of the fine grid.
+// For every periodic image of the tile, lanes test 64 projections at a time:
+// the plane of projection p meets the (kernel-widened) box iff the box corners
+// lie on both sides of it, and the lattice indices that can land in the box
+// are bounded by the corners' coordinates along u_p and v_p.  Candidates of a
+// hit plane are tested 64 at a time; each accepted point spreads its 8x8x8
+// footprint over the 64 lanes (8x8 in x, y; 8 steps in z) into the wave's
+// private LDS copy of the tile.  The copies are summed in a fixed order.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(64 * S3_WAVES)
+k_spread3d(const float2 *__restrict__ C, float2 *__restrict__ F, const double *__restrict__ rot,
+           const double *__restrict__ shifts, Grid3 g, int ntz, int nty, int ntx) {
+    constexpr int TC = TZ * TY * TX;
+    float2 *acc = reinterpret_cast<float2 *>(lc_smem);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    float2 *my = acc + wave * TC;
+    for (int i = lane; i < TC; i += 64) my[i] = make_float2(0.f, 0.f);
+    __syncthreads();
+    const int tile = blockIdx.x;
+    const int tz = tile / (nty * ntx), ty = (tile / ntx) % nty, tx = tile % ntx;
+    const int c0[3] = {tz * TZ, ty * TY, tx * TX};
+    const int tdim[3] = {TZ, TY, TX};
+    const double hw = 0.5 * W3;
+    const float ihw = 1.0f / (float)hw;
+    const double sc[3] = {g.nf[0] / (2.0 * M_PI), g.nf[1] / (2.0 * M_PI), g.nf[2] / (2.0 * M_PI)};
+    const int dxl = lane & 7, dyl = lane >> 3;
+    for (int oz = -1; oz <= 1; ++oz)
+        for (int oy = -1; oy <= 1; ++oy)
+            for (int ox = -1; ox <= 1; ++ox) {
+                const int o[3] = {oz, oy, ox};
+                // box of fine coordinates (unwrapped) whose footprint reaches the tile
+                double blo[3], bhi[3];
+                bool skip = false;
+                for (int a = 0; a < 3; ++a) {
+                    blo[a] = c0[a] + o[a] * g.nf[a] - hw - 1.0;
+                    bhi[a] = c0[a] + tdim[a] - 1 + o[a] * g.nf[a] + hw + 1.0;
+                    // points lie in [0, nf] (masked) or within 0.75 nf of the centre
+                    const double lo = g.mask ? 0.0 : -0.25 * g.nf[a], hi = g.mask ? g.nf[a] : 1.25 * g.nf[a];
+                    if (bhi[a] < lo || blo[a] > hi) skip = true;
+                }
+                if (skip) continue;
+                // the box in coordinate space (radians)
+                double wlo[3], whi[3];
+                for (int a = 0; a < 3; ++a) {
+                    wlo[a] = (blo[a] - 0.5 * g.nf[a]) / sc[a];
+                    whi[a] = (bhi[a] - 0.5 * g.nf[a]) / sc[a];
+                }
+                for (int p0 = wave * 64; p0 < g.nproj; p0 += 64 * S3_WAVES) {
+                    const int p = p0 + lane;
+                    int k0 = 1, k1 = 0, j0 = 1, j1 = 0;
+                    if (p < g.nproj) {
+                        const double *r = rot + 6 * (int64_t)p;
+                        // r[0..2] = u (x, y, z), r[3..5] = v; corners in (z, y, x) order
+                        const double nx = r[1] * r[5] - r[2] * r[4];
+                        const double ny = r[2] * r[3] - r[0] * r[5];
+                        const double nz = r[0] * r[4] - r[1] * r[3];
+                        double nmin = 1e300, nmax = -1e300, umin = 1e300, umax = -1e300,
+                               vmin = 1e300, vmax = -1e300;
+                        for (int c = 0; c < 8; ++c) {
+                            const double cz = (c & 4) ? whi[0] : wlo[0];
+                            const double cy = (c & 2) ? whi[1] : wlo[1];
+                            const double cx = (c & 1) ? whi[2] : wlo[2];
+                            const double dn = cx * nx + cy * ny + cz * nz;
+                            const double du = cx * r[0] + cy * r[1] + cz * r[2];
+                            const double dv = cx * r[3] + cy * r[4] + cz * r[5];
+                            nmin = fmin(nmin, dn); nmax = fmax(nmax, dn);
+                            umin = fmin(umin, du); umax = fmax(umax, du);
+                            vmin = fmin(vmin, dv); vmax = fmax(vmax, dv);
+                        }
+                        if (nmin <= 0.0 && nmax >= 0.0) {
+                            // qX_k = (k + 1/2) dX - pi in [umin, umax]
+                            k0 = max(0, (int)ceil((umin + M_PI) / g.dX - 0.5) - 1);
+                            k1 = min(g.ncols - 1, (int)floor((umax + M_PI) / g.dX - 0.5) + 1);
+                            j0 = max(0, (int)ceil((vmin + M_PI) / g.dY - 0.5) - 1);
+                            j1 = min(g.nrows - 1, (int)floor((vmax + M_PI) / g.dY - 0.5) + 1);
+                        }
+                    }
+                    unsigned long long hits = __ballot(k0 <= k1 && j0 <= j1);
+                    while (hits) {
+                        const int src = __ffsll((long long)hits) - 1;
+                        hits &= hits - 1;
+                        const int pp = p0 + src;
+                        const int hk0 = __shfl(k0, src, 64), hk1 = __shfl(k1, src, 64);
+                        const int hj0 = __shfl(j0, src, 64), hj1 = __shfl(j1, src, 64);
+                        const int nk = hk1 - hk0 + 1;
+                        const int ncand = nk * (hj1 - hj0 + 1);
+                        double sdx = 0.0, sdy = 0.0;
+                        if (shifts) {
+                            sdx = shifts[2 * pp];
+                            sdy = shifts[2 * pp + 1];
+                        }
+                        for (int cb = 0; cb < ncand; cb += 64) {
+                            const int ci = cb + lane;
+                            bool ok = false;
+                            double gz = 0, gy = 0, gx = 0;
+                            int jj = 0, kk = 0;
+                            if (ci < ncand) {
+                                jj = hj0 + ci / nk;
+                                kk = hk0 + ci % nk;
+                                double x, y, z;
+                                point3(rot, pp, jj, kk, g, x, y, z);
+                                ok = !g.mask || inside_pi(x, y, z);
+                                gz = 0.5 * g.nf[0] + z * sc[0] - o[0] * g.nf[0];
+                                gy = 0.5 * g.nf[1] + y * sc[1] - o[1] * g.nf[1];
+                                gx = 0.5 * g.nf[2] + x * sc[2] - o[2] * g.nf[2];
+                                ok = ok && gz >= c0[0] - hw && gz <= c0[0] + TZ - 1 + hw &&
+                                     gy >= c0[1] - hw && gy <= c0[1] + TY - 1 + hw &&
+                                     gx >= c0[2] - hw && gx <= c0[2] + TX - 1 + hw;
+                            }
+                            float2 cv = make_float2(0.f, 0.f);
+                            if (ok) {
+                                cv = C[((int64_t)pp * g.nrows + jj) * g.ncols + kk];
+                                if (shifts) {
+                                    // include/fft.h:53-74: C *= exp(-i (qx dx + qy dy))
+                                    const double qx = ((double)kk + 0.5) * g.dX - M_PI;
+                                    const double qy = ((double)jj + 0.5) * g.dY - M_PI;
+                                    double s, c;
+                                    sincos(qx * sdx + qy * sdy, &s, &c);
+                                    const float cr = (float)c, si = (float)-s;
+                                    cv = make_float2(cv.x * cr - cv.y * si, cv.x * si + cv.y * cr);
+                                }
+                            }
+                            unsigned long long acc_pts = __ballot(ok);
+                            while (acc_pts) {
+                                const int s2 = __ffsll((long long)acc_pts) - 1;
+                                acc_pts &= acc_pts - 1;
+                                const double pz = __shfl(gz, s2, 64), py = __shfl(gy, s2, 64),
+                                             px = __shfl(gx, s2, 64);
+                                const float vr = __shfl(cv.x, s2, 64), vi = __shfl(cv.y, s2, 64);
+                                const int lz0 = (int)ceil(pz - hw), ly0 = (int)ceil(py - hw),
+                                          lx0 = (int)ceil(px - hw);
+                                const float wxy = es3((float)((double)(lx0 + dxl) - px) * ihw, g.beta) *
+                                                  es3((float)((double)(ly0 + dyl) - py) * ihw, g.beta);
+                                const float wzl = es3((float)((double)(lz0 + dxl) - pz) * ihw, g.beta);
+                                const int cx = lx0 + dxl - c0[2], cy = ly0 + dyl - c0[1];
+                                const bool inxy = cx >= 0 && cx < TX && cy >= 0 && cy < TY;
+#pragma unroll
+                                for (int dz = 0; dz < W3; ++dz) {
+                                    const float wz = __shfl(wzl, dz, 64);
+                                    const int cz = lz0 + dz - c0[0];
+                                    if (inxy && cz >= 0 && cz < TZ) {
+                                        const float wgt = wxy * wz;
+                                        float2 &a = my[(cz * TY + cy) * TX + cx];
+                                        a.x += wgt * vr;
+                                        a.y += wgt * vi;
+                                    }
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+    __syncthreads();
+    for (int i = threadIdx.x; i < TC; i += 64 * S3_WAVES) {
+        float2 s = acc[i];
+        for (int w = 1; w < S3_WAVES; ++w) {
+            s.x += acc[w * TC + i].x;
+            s.y += acc[w * TC + i].y;
+        }
+        const int cz = i / (TY * TX), cy = (i / TX) % TY, cx = i % TX;
+        const int zz = c0[0] + cz, yy = c0[1] + cy, xx = c0[2] + cx;
+        if (zz < g.nf[0] && yy < g.nf[1] && xx < g.nf[2])
+            F[((int64_t)zz * g.nf[1] + yy) * g.nf[2] + xx] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// pruned line passes between the fine grid and the modes along one axis.
+// Array (outer, L, inner); row mode when inner == 1 (nl lines per block),
+// otherwise column mode with B consecutive inner indices per block.
+// DIR 1 (type 1): in (outer, nf, inner) -> FFT+ -> out (outer, nm, inner),
+//                 out[a] = X[m_a mod nf] * corr[a] (* scale, real part if ROUT)
+// DIR 2 (type 2): in (outer, nm, inner) (* corr, real if RIN) placed at
+//                 m_a mod nf, zeros elsewhere -> FFT- -> out (outer, nf, inner)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int m2f(int a, int nm, int nf) {
+    const int m = a - nm / 2;
+    return m < 0 ? m + nf : m;
+}
+
+template <int DIR, bool RIN, bool ROUT, int NT, int MAXE>
+__global__ void __launch_bounds__(NT)
+k_nu_axis(const void *__restrict__ vin, void *__restrict__ vout, int64_t outer, int64_t inner,
+          int nf, int nm, int nl, const float *__restrict__ corr, float scale, FftPlan plan,
+          const float2 *__restrict__ tw) {
+    float2 *buf = reinterpret_cast<float2 *>(lc_smem);
+    const int Lin = DIR == 1 ? nf : nm, Lout = DIR == 1 ? nm : nf;
+    const bool rowm = inner == 1;
+    int64_t o0, i0;
+    int nloc;
+    if (rowm) {
+        o0 = (int64_t)blockIdx.x * nl;
+        i0 = 0;
+        nloc = (int)min((int64_t)nl, outer - o0);
+    } else {
+        o0 = blockIdx.y;
+        i0 = (int64_t)blockIdx.x * nl;
+        nloc = (int)min((int64_t)nl, inner - i0);
+    }
+    // element e of local line l: address of (o, e, i)
+    auto addr = [&](int l, int e, int L) -> int64_t {
+        return rowm ? (o0 + l) * L + e : (o0 * L + e) * inner + i0 + l;
+    };
+    auto lds = [&](int l, int e) -> int { return rowm ? lds_row(l, e, nf) : lds_col(l, e, nl); };
+    if (DIR == 2) {
+        for (int idx = threadIdx.x; idx < nl * nf; idx += NT)
+            buf[rowm ? swz(idx) : swz(idx)] = make_float2(0.f, 0.f);
+        __syncthreads();
+    }
+    for (int idx = threadIdx.x; idx < nl * Lin; idx += NT) {
+        int l, e;
+        if (rowm) { l = idx / Lin; e = idx - l * Lin; }
+        else { e = idx / nl; l = idx - e * nl; }
+        if (l >= nloc) continue;
+        const int64_t q = addr(l, e, Lin);
+        float2 v = RIN ? make_float2(reinterpret_cast<const float *>(vin)[q], 0.f)
+                       : reinterpret_cast<const float2 *>(vin)[q];
+        if (DIR == 1) {
+            buf[lds(l, e)] = v;
+        } else {
+            const float c = corr[e];
+            buf[lds(l, m2f(e, nm, nf))] = make_float2(v.x * c, v.y * c);
+        }
+    }
+    if (DIR == 1) {
+        // lines beyond nloc hold stale LDS: clear them so the FFT reads zeros
+        for (int idx = threadIdx.x; idx < nl * nf; idx += NT) {
+            int l, e;
+            if (rowm) { l = idx / nf; e = idx - l * nf; }
+            else { e = idx / nl; l = idx - e * nl; }
+            if (l >= nloc) buf[lds(l, e)] = make_float2(0.f, 0.f);
+        }
+    }
+    __syncthreads();
+    if (rowm) fft_lds<float, DIR == 1 ? 1 : -1, NT, MAXE, false>(buf, nl, nf, plan, tw);
+    else fft_lds<float, DIR == 1 ? 1 : -1, NT, MAXE, true>(buf, nl, 0, plan, tw);
+    for (int idx = threadIdx.x; idx < nl * Lout; idx += NT) {
+        int l, e;
+        if (rowm) { l = idx / Lout; e = idx - l * Lout; }
+        else { e = idx / nl; l = idx - e * nl; }
+        if (l >= nloc) continue;
+        const int64_t q = addr(l, e, Lout);
+        if (DIR == 1) {
+            const float2 v = buf[lds(l, m2f(e, nm, nf))];
+            const float c = corr[e] * scale;
+            if (ROUT) reinterpret_cast<float *>(vout)[q] = v.x * c;
+            else reinterpret_cast<float2 *>(vout)[q] = make_float2(v.x * c, v.y * c);
+        } else {
+            reinterpret_cast<float2 *>(vout)[q] = buf[lds(l, e)];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// detector line transform along one axis with the reference shift maps:
+// buf[k] = in[(k + din) mod n]; X = DFT_sign(buf); out[k] = X[(k + dout) mod n]
+// Stockham when n is 2^p 3^q, else Bluestein of length Lb:
+// X_m = a_m sum_k (x_k a_k) b_{m-k},  a_k = e^{s i pi k^2/n},  b_k = conj(a_k)
+// computed as a_m * FFT+(FFT-(x a) * bhat) with bhat = FFT-(b) / Lb.
+// ---------------------------------------------------------------------------
+template <bool RIN, bool ROUT, int NT, int MAXE>
+__global__ void __launch_bounds__(NT)
+k_det_axis(const void *__restrict__ vin, void *__restrict__ vout, int64_t outer, int64_t inner,
+           int n, int nl, int din, int dout, int sign, float scale, int blue, FftPlan plan,
+           const float2 *__restrict__ tw, const float2 *__restrict__ chirp,
+           const float2 *__restrict__ bhat) {
+    float2 *buf = reinterpret_cast<float2 *>(lc_smem);
+    const int L = blue ? plan.L : n;
+    const bool rowm = inner == 1;
+    int64_t o0, i0;
+    int nloc;
+    if (rowm) {
+        o0 = (int64_t)blockIdx.x * nl;
+        i0 = 0;
+        nloc = (int)min((int64_t)nl, outer - o0);
+    } else {
+        o0 = blockIdx.y;
+        i0 = (int64_t)blockIdx.x * nl;
+        nloc = (int)min((int64_t)nl, inner - i0);
+    }
+    auto addr = [&](int l, int e) -> int64_t {
+        return rowm ? (o0 + l) * n + e : (o0 * n + e) * inner + i0 + l;
+    };
+    auto lds = [&](int l, int e) -> int { return rowm ? lds_row(l, e, L) : lds_col(l, e, nl); };
+    for (int idx = threadIdx.x; idx < nl * L; idx += NT) {
+        int l, e;
+        if (rowm) { l = idx / L; e = idx - l * L; }
+        else { e = idx / nl; l = idx - e * nl; }
+        float2 v = make_float2(0.f, 0.f);
+        if (l < nloc && e < n) {
+            int src = e + din;
+            if (src >= n) src -= n;
+            const int64_t q = addr(l, src);
+            v = RIN ? make_float2(reinterpret_cast<const float *>(vin)[q], 0.f)
+                    : reinterpret_cast<const float2 *>(vin)[q];
+            if (blue) {
+                float2 a = chirp[e];
+                if (sign < 0) a.y = -a.y;
+                v = cmul(v, a);
+            }
+        }
+        buf[lds(l, e)] = v;
+    }
+    __syncthreads();
+    if (!blue) {
+        if (sign < 0) {
+            if (rowm) fft_lds<float, -1, NT, MAXE, false>(buf, nl, L, plan, tw);
+            else fft_lds<float, -1, NT, MAXE, true>(buf, nl, 0, plan, tw);
+        } else {
+            if (rowm) fft_lds<float, 1, NT, MAXE, false>(buf, nl, L, plan, tw);
+            else fft_lds<float, 1, NT, MAXE, true>(buf, nl, 0, plan, tw);
+        }
+    } else {
+        if (rowm) fft_lds<float, -1, NT, MAXE, false>(buf, nl, L, plan, tw);
+        else fft_lds<float, -1, NT, MAXE, true>(buf, nl, 0, plan, tw);
+        const float2 *bh = bhat + (sign < 0 ? 0 : L);
+        for (int idx = threadIdx.x; idx < nl * L; idx += NT) {
+            int l, e;
+            if (rowm) { l = idx / L; e = idx - l * L; }
+            else { e = idx / nl; l = idx - e * nl; }
+            float2 *p = buf + lds(l, e);
+            *p = cmul(*p, bh[e]);
+        }
+        __syncthreads();
+        if (rowm) fft_lds<float, 1, NT, MAXE, false>(buf, nl, L, plan, tw);
+        else fft_lds<float, 1, NT, MAXE, true>(buf, nl, 0, plan, tw);
+    }
+    for (int idx = threadIdx.x; idx < nl * n; idx += NT) {
+        int l, e;
+        if (rowm) { l = idx / n; e = idx - l * n; }
+        else { e = idx / nl; l = idx - e * nl; }
+        if (l >= nloc) continue;
+        int m = e + dout;
+        if (m >= n) m -= n;
+        float2 v = buf[lds(l, m)];
+        if (blue) {
+            float2 a = chirp[m];
+            if (sign < 0) a.y = -a.y;
+            v = cmul(v, a);
+        }
+        const int64_t q = addr(l, e);
+        if (ROUT) reinterpret_cast<float *>(vout)[q] = v.x * scale;
+        else reinterpret_cast<float2 *>(vout)[q] = make_float2(v.x * scale, v.y * scale);
+    }
+}
+
+bool smooth23(int64_t n) { return is_2_3_smooth(n); }
+
+int fine_size(int64_t n) {
+    int64_t f = std::max<int64_t>(2 * n, 2 * W3);
+    while (!smooth23(f)) ++f;
+    return (int)f;
+}
+
+// psi_hat of the width-8 kernel by 96-point Gauss-Legendre quadrature (FP64)
+double es_phihat3(double beta, double xi) {
+    static double gx[96], gw[96];
+    static bool ready = false;
+    const int n = 96;
+    if (!ready) {
+        for (int i = 0; i < n; ++i) {
+            double x = std::cos(M_PI * (i + 0.75) / (n + 0.5));
+            for (int it = 0; it < 100; ++it) {
+                double p0 = 1.0, p1 = x;
+                for (int k = 2; k <= n; ++k) {
+                    double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+                    p0 = p1;
+                    p1 = p2;
+                }
+                double dp = n * (x * p1 - p0) / (x * x - 1.0);
+                double dx = p1 / dp;
+                x -= dx;
+                if (std::fabs(dx) < 1e-16) break;
+            }
+            double p0 = 1.0, p1 = x;
+            for (int k = 2; k <= n; ++k) {
+                double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            double dp = n * (x * p1 - p0) / (x * x - 1.0);
+            gx[i] = x;
+            gw[i] = 2.0 / ((1.0 - x * x) * dp * dp);
+        }
+        ready = true;
+    }
+    const double hw = 0.5 * W3;
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double t = 0.5 * M_PI * gx[i];
+        const double ct = std::cos(t);
+        s += gw[i] * std::exp(beta * (ct - 1.0)) * std::cos(xi * hw * std::sin(t)) * ct;
+    }
+    return hw * 0.5 * M_PI * s;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+void det_line_init(DetLine &d, int n) {
+    d.n = n;
+    if (n <= LN_NT * LN_ME && fft_make_plan(n, d.plan)) {
+        d.blue = false;
+        return;
+    }
+    int64_t L = 2 * (int64_t)n - 1;
+    while (!is_2_3_smooth(L)) ++L;
+    LC_REQUIRE(L <= LN_NT * LN_ME, "detector transform: width too large");
+    LC_REQUIRE(fft_make_plan((int)L, d.plan), "detector transform: no Bluestein length");
+    d.blue = true;
+    // chirp a_k = e^{+i pi k^2 / n} (the kernel conjugates it for sign -1)
+    std::vector<float2> ch(n);
+    std::vector<std::complex<double>> a(n);
+    for (int k = 0; k < n; ++k) {
+        const int64_t q = ((int64_t)k * k) % (2 * (int64_t)n);
+        const double ang = M_PI * (double)q / (double)n;
+        a[k] = std::complex<double>(std::cos(ang), std::sin(ang));
+        ch[k] = make_float2((float)a[k].real(), (float)a[k].imag());
+    }
+    // bhat for sign s: b_k = conj(a_k^s) = e^{-s i pi k^2/n} on k and L-k;
+    // bhat = FFT-(b) / L in FP64 (direct DFT), [sign -1 | sign +1]
+    std::vector<float2> bh(2 * L);
+    for (int si = 0; si < 2; ++si) {
+        const double s = si == 0 ? -1.0 : 1.0;
+        std::vector<std::complex<double>> b(L, 0.0);
+        for (int k = 0; k < n; ++k) {
+            const std::complex<double> v = s < 0 ? a[k] : std::conj(a[k]);
+            b[k] = v;
+            if (k > 0) b[L - k] = v;
+        }
+        for (int64_t m = 0; m < L; ++m) {
+            std::complex<double> acc = 0.0;
+            for (int64_t k = 0; k < L; ++k) {
+                const int64_t q = (k * m) % L;
+                const double ang = -2.0 * M_PI * (double)q / (double)L;
+                acc += b[k] * std::complex<double>(std::cos(ang), std::sin(ang));
+            }
+            acc /= (double)L;
+            bh[si * L + m] = make_float2((float)acc.real(), (float)acc.imag());
+        }
+    }
+    d.chirp.alloc(sizeof(float2) * n);
+    d.bhat.alloc(sizeof(float2) * 2 * L);
+    LC_HIP(hipMemcpy(d.chirp.as(), ch.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+    LC_HIP(hipMemcpy(d.bhat.as(), bh.data(), sizeof(float2) * 2 * L, hipMemcpyHostToDevice));
+}
+
+void nufft3d_init(Ctx &c, Geom3D &g) {
+    Nufft3D &u = g.nu;
+    const int64_t nm[3] = {g.n1, g.n2, g.n3};
+    const double beta = 2.30 * W3;
+    u.beta = beta;
+    std::vector<float> corr;
+    for (int a = 0; a < 3; ++a) {
+        u.nf[a] = fine_size(nm[a]);
+        LC_REQUIRE(u.nf[a] <= LN_NT * LN_ME, "3-D NUFFT: fine grid too large");
+        LC_REQUIRE(fft_make_plan(u.nf[a], u.plan[a]), "3-D NUFFT: fine size not 2^p 3^q");
+        u.coff[a] = (int)corr.size();
+        for (int64_t i = 0; i < nm[a]; ++i) {
+            const int64_t m = i - nm[a] / 2;
+            const double ph = es_phihat3(beta, (double)m * 2.0 * M_PI / (double)u.nf[a]);
+            corr.push_back((float)(((m & 1) ? -1.0 : 1.0) / ph));
+        }
+    }
+    u.corr.alloc(sizeof(float) * corr.size());
+    LC_HIP(hipMemcpy(u.corr.as(), corr.data(), sizeof(float) * corr.size(), hipMemcpyHostToDevice));
+    det_line_init(u.det_x, (int)g.ncols);
+    det_line_init(u.det_y, (int)g.nrows);
+    u.ready = true;
+}
+
+namespace {
+
+Grid3 grid3(const Geom3D &g, bool mask) {
+    Grid3 r;
+    r.nproj = (int)g.nproj;
+    r.nrows = (int)g.nrows;
+    r.ncols = (int)g.ncols;
+    for (int a = 0; a < 3; ++a) r.nf[a] = g.nu.nf[a];
+    r.dX = 2.0 * M_PI / (double)g.ncols;
+    r.dY = 2.0 * M_PI / (double)g.nrows;
+    r.beta = (float)g.nu.beta;
+    r.mask = mask ? 1 : 0;
+    return r;
+}
+
+struct Work3 {
+    float2 *C, *F, *V1, *V2;
+};
+
+Work3 work3(Ctx &c, const Geom3D &g) {
+    const Nufft3D &u = g.nu;
+    const size_t bC = sizeof(float2) * (size_t)g.nproj * g.nrows * g.ncols;
+    const size_t bF = sizeof(float2) * (size_t)u.nf[0] * u.nf[1] * u.nf[2];
+    const size_t bV1 = sizeof(float2) * (size_t)u.nf[0] * u.nf[1] * g.n3;
+    const size_t bV2 = sizeof(float2) * (size_t)u.nf[0] * g.n2 * g.n3;
+    char *ws = reinterpret_cast<char *>(c.scratch(4, bC + bF + bV1 + bV2));
+    return Work3{reinterpret_cast<float2 *>(ws), reinterpret_cast<float2 *>(ws + bC),
+                 reinterpret_cast<float2 *>(ws + bC + bF),
+                 reinterpret_cast<float2 *>(ws + bC + bF + bV1)};
+}
+
+// one axis pass of the fine-grid transform
+template <int DIR, bool RIN, bool ROUT>
+void nu_axis(Ctx &c, const void *in, void *out, int64_t outer, int64_t inner, int nf, int nm,
+             const float *corr, float scale, const FftPlan &plan) {
+    const float2 *tw = c.twiddles<float>(nf);
+    const int cap = LN_NT * LN_ME;
+    LC_REQUIRE(nf <= cap, "3-D NUFFT: line too long");
+    int nl;
+    dim3 grid;
+    if (inner == 1) {
+        nl = std::max(1, std::min(cap / nf, 64));
+        grid = dim3(ceil_div(outer, nl));
+    } else {
+        nl = (int)std::max<int64_t>(1, std::min<int64_t>(std::min(cap / nf, 64), inner));
+        LC_REQUIRE(outer <= 65535, "3-D NUFFT: too many planes");
+        grid = dim3(ceil_div(inner, nl), (unsigned)outer);
+    }
+    const size_t sh = sizeof(float2) * lds_round(nl * nf);
+    hipLaunchKernelGGL((k_nu_axis<DIR, RIN, ROUT, LN_NT, LN_ME>), grid, dim3(LN_NT), sh, c.stream(),
+                       in, out, outer, inner, nf, nm, nl, corr, scale, plan, tw);
+    LC_HIP(hipGetLastError());
+}
+
+// detector transform along one axis
+template <bool RIN, bool ROUT>
+void det_axis(Ctx &c, const DetLine &d, const void *in, void *out, int64_t outer, int64_t inner,
+              int din, int dout, int sign, float scale) {
+    const int L = d.blue ? d.plan.L : d.n;
+    const float2 *tw = c.twiddles<float>(d.plan.L);
+    const int cap = LN_NT * LN_ME;
+    int nl;
+    dim3 grid;
+    if (inner == 1) {
+        nl = std::max(1, std::min(cap / L, 64));
+        grid = dim3(ceil_div(outer, nl));
+    } else {
+        nl = (int)std::max<int64_t>(1, std::min<int64_t>(std::min(cap / L, 64), inner));
+        LC_REQUIRE(outer <= 65535, "detector transform: too many projections");
+        grid = dim3(ceil_div(inner, nl), (unsigned)outer);
+    }
+    const size_t sh = sizeof(float2) * lds_round(nl * L);
+    hipLaunchKernelGGL((k_det_axis<RIN, ROUT, LN_NT, LN_ME>), grid, dim3(LN_NT), sh, c.stream(), in,
+                       out, outer, inner, d.n, nl, din, dout, sign, scale, d.blue ? 1 : 0, d.plan, tw,
+                       d.blue ? d.chirp.as<float2>() : nullptr, d.blue ? d.bhat.as<float2>() : nullptr);
+    LC_HIP(hipGetLastError());
+}
+
+void type2_3d(Ctx &c, const Geom3D &g, const Work3 &w, const float *vol, bool mask) {
+    const Nufft3D &u = g.nu;
+    const int n1 = (int)g.n1, n2 = (int)g.n2, n3 = (int)g.n3;
+    const float *corr = g.nu.corr.as<float>();
+    // z: (n1, n2*n3) real -> (nf0, n2*n3)
+    nu_axis<2, true, false>(c, vol, w.V2, 1, (int64_t)n2 * n3, u.nf[0], n1, corr + u.coff[0], 1.0f, u.plan[0]);
+    // y: (nf0, n2, n3) -> (nf0, nf1, n3)
+    nu_axis<2, false, false>(c, w.V2, w.V1, u.nf[0], n3, u.nf[1], n2, corr + u.coff[1], 1.0f, u.plan[1]);
+    // x: rows (nf0*nf1, n3) -> (nf0*nf1, nf2)
+    nu_axis<2, false, false>(c, w.V1, w.F, (int64_t)u.nf[0] * u.nf[1], 1, u.nf[2], n3,
+                             corr + u.coff[2], 1.0f, u.plan[2]);
+    const Grid3 gg = grid3(g, mask);
+    const int64_t npts = g.nproj * g.nrows * g.ncols;
+    hipLaunchKernelGGL(k_interp3d, dim3(ceil_div(npts, 256)), dim3(256), 0, c.stream(), w.F, w.C,
+                       g.d_rot.as<double>(), gg);
+    LC_HIP(hipGetLastError());
+}
+
+void type1_3d(Ctx &c, const Geom3D &g, const Work3 &w, float *vol, bool mask, const double *shifts,
+              float scale) {
+    const Nufft3D &u = g.nu;
+    const int n1 = (int)g.n1, n2 = (int)g.n2, n3 = (int)g.n3;
+    const Grid3 gg = grid3(g, mask);
+    const int ntz = ceil_div(u.nf[0], TZ), nty = ceil_div(u.nf[1], TY), ntx = ceil_div(u.nf[2], TX);
+    const size_t shs = sizeof(float2) * S3_WAVES * TZ * TY * TX;
+    hipLaunchKernelGGL(k_spread3d, dim3(ntz * nty * ntx), dim3(64 * S3_WAVES), shs, c.stream(), w.C,
+                       w.F, g.d_rot.as<double>(), shifts, gg, ntz, nty, ntx);
+    LC_HIP(hipGetLastError());
+    const float *corr = g.nu.corr.as<float>();
+    // x: rows (nf0*nf1, nf2) -> (nf0*nf1, n3)
+    nu_axis<1, false, false>(c, w.F, w.V1, (int64_t)u.nf[0] * u.nf[1], 1, u.nf[2], n3,
+                             corr + u.coff[2], 1.0f, u.plan[2]);
+    // y: (nf0, nf1, n3) -> (nf0, n2, n3)
+    nu_axis<1, false, false>(c, w.V1, w.V2, u.nf[0], n3, u.nf[1], n2, corr + u.coff[1], 1.0f, u.plan[1]);
+    // z: (nf0, n2*n3) -> (n1, n2*n3) real, scaled
+    nu_axis<1, false, true>(c, w.V2, vol, 1, (int64_t)n2 * n3, u.nf[0], n1, corr + u.coff[0], scale,
+                            u.plan[0]);
+}
+
+void nu3_check(Ctx &c, Geom3D &g) {
+    if (!g.nu.ready) nufft3d_init(c, g);
+}
+
+} // namespace
+
+// forward (src/projection.cpp:42-64): vol {n1, N, N} -> proj {nproj, nrows, ncols}
+void forward3d_dev(Ctx &c, Geom3D &g, const float *vol, float *proj) {
+    nu3_check(c, g);
+    const Work3 w = work3(c, g);
+    type2_3d(c, g, w, vol, true);
+    const int nr = (int)g.nrows, nc = (int)g.ncols;
+    // fftshift2 -> ifft2 -> ifftshift2 along x then y; real part / (nrows ncols)
+    det_axis<false, false>(c, g.nu.det_x, w.C, w.C, g.nproj * nr, 1, nc / 2, nc / 2 + (nc & 1), 1, 1.0f);
+    det_axis<false, true>(c, g.nu.det_y, w.C, proj, g.nproj, nc, nr / 2, nr / 2 + (nr & 1), 1,
+                          1.0f / ((float)nr * (float)nc));
+}
+
+// backproj (src/projection.cpp:71-131): proj -> vol; shifts (dx, dy) per
+// projection or nullptr
+void backproj3d_dev(Ctx &c, Geom3D &g, const float *proj, float *vol, const double *shifts) {
+    nu3_check(c, g);
+    const Work3 w = work3(c, g);
+    const int nr = (int)g.nrows, nc = (int)g.ncols;
+    // fftshift2 -> fft2 -> ifftshift2 along x then y
+    det_axis<true, false>(c, g.nu.det_x, proj, w.C, g.nproj * nr, 1, nc / 2, nc / 2 + (nc & 1), -1, 1.0f);
+    det_axis<false, false>(c, g.nu.det_y, w.C, w.C, g.nproj, nc, nr / 2, nr / 2 + (nr & 1), -1, 1.0f);
+    type1_3d(c, g, w, vol, true, shifts, 1.0f / ((float)nr * (float)nc));
+}
+
+// sysmat(x, grid) (src/gradient.cpp:32-41): Re(T1(T2 x)) / (nrows ncols), no mask
+void sysmat3d_nufft_dev(Ctx &c, Geom3D &g, const float *x, float *y) {
+    nu3_check(c, g);
+    const Work3 w = work3(c, g);
+    type2_3d(c, g, w, x, false);
+    type1_3d(c, g, w, y, false, nullptr, 1.0f / ((float)g.nrows * (float)g.ncols));
+}
+
+} // namespace lc

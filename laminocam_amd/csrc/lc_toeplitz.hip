@@ -37,18 +37,9 @@ int64_t next_fast_dim(int64_t n) {
 // ----------------------------------------------------------------------------
 // FP64 PSF: P(m) = sum_i D_nrad(m_x cos th_i + m_y sin th_i),
 // D_n(t) = sin(pi t)/sin(pi t/n) (the radial sum of exp(i r_k t) over the
-// polar grid's r_k = (k+1/2) 2pi/n - pi), evaluated at t = k n + d as
-// (-1)^{k(n-1)} sin(pi d)/sin(pi d/n) so it never divides 0 by 0.
+// polar grid's r_k = (k+1/2) 2pi/n - pi); dirichlet_d evaluates it without
+// dividing 0 by 0.
 // ----------------------------------------------------------------------------
-__device__ __forceinline__ double dirichlet_d(double t, int n, double inv_n) {
-    const double kq = rint(t * inv_n);
-    const double d = fma(-kq, (double)n, t);
-    const long long k = (long long)kq;
-    const double sg = ((k * (long long)(n - 1)) & 1LL) ? -1.0 : 1.0;
-    if (d == 0.0) return sg * (double)n;
-    return sg * sinpi(d) / sinpi(d * inv_n);
-}
-
 constexpr int PSF_RUN = 8;          // consecutive kx per thread (rotation recurrence)
 constexpr int PSF_ANG_CHUNK = 512;  // angles staged in LDS per pass
 
@@ -58,7 +49,7 @@ constexpr int PSF_ANG_CHUNK = 512;  // angles staged in LDS per pass
 // fixed rotation per step; the ratio of their imaginary parts is D_n(t).
 // Where sin(pi t / n) is small the direct reduced form is used instead.
 __global__ void __launch_bounds__(256)
-k_psf2d_gen(double2 *P, const double *theta, int nang, int nrad, int M) {
+k_psf2d_gen(double2 *P, const double *theta, int nang, int nrad, int M, int full) {
     __shared__ double4 ang[PSF_ANG_CHUNK];    // (cos, sin, cos pi c, sin pi c)
     __shared__ double2 ang2[PSF_ANG_CHUNK];   // (cos pi c/n, sin pi c/n)
     const double inv_n = 1.0 / (double)nrad;
@@ -66,7 +57,8 @@ k_psf2d_gen(double2 *P, const double *theta, int nang, int nrad, int M) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const int iy = gid / runs;
     const int run = gid - iy * runs;
-    const bool valid = iy <= M / 2;      // no early return: the block synchronises below
+    // full: every row computed (no mirror), as the exported kernel_hat needs
+    const bool valid = full ? iy < M : iy <= M / 2;   // no early return: the block synchronises below
     const int ix0 = run * PSF_RUN;
     const int half = M / 2;
     const double ky = (double)(iy < (M + 1) / 2 ? iy : iy - M);
@@ -113,7 +105,7 @@ k_psf2d_gen(double2 *P, const double *theta, int nang, int nrad, int M) {
     // rows 0 and M/2 (even M) are computed in full and own themselves; rows
     // strictly inside also fill their mirror row, so every entry has exactly
     // one writer
-    const bool mirror = iy >= 1 && 2 * iy < M;
+    const bool mirror = !full && iy >= 1 && 2 * iy < M;
     (void)half;
 #pragma unroll
     for (int j = 0; j < PSF_RUN; ++j) {
@@ -164,19 +156,33 @@ __global__ void __launch_bounds__(NT) k_psf_cols(const double2 *P, float *khatT,
     }
 }
 
-// reference-layout kernel_hat from the stored real spectrum.  The reference
-// transforms the PSF stored with its centre at index h = floor(M/2), which
-// multiplies the centred spectrum by exp(-2 pi i (kx + ky) h / M):
-// khat_ref[ky][kx] = khatT[kx][ky] * exp(-2 pi i (kx+ky) h / M) * (M^2 N)
-__global__ void k_khat_ref(const float *khatT, float2 *out, int M, int H, float unscale) {
-    const int kx = blockIdx.x * blockDim.x + threadIdx.x;
-    const int ky = blockIdx.y;
-    if (kx >= H) return;
-    const float v = khatT[(size_t)kx * M + ky] * unscale;
-    const long long ph = ((long long)(kx + ky) * (M / 2)) % M;
-    double s, c;
-    sincospi(-2.0 * (double)ph / (double)M, &s, &c);
-    out[(size_t)ky * H + kx] = make_float2((float)(v * c), (float)(v * s));
+// reference-layout kernel_hat: FP64 column FFT of the row-transformed full
+// PSF (centred layout) for kx < H, times the phase of the reference's storage
+// with the centre at index h = floor(M/2): exp(-2 pi i (kx + ky) h / M).
+// For even M the row and column m = -M/2 have no +M/2 partner, so this
+// spectrum is complex; those entries never reach the cropped convolution.
+template <int NT, int MAXE>
+__global__ void __launch_bounds__(NT) k_khat_export(const double2 *P, float2 *out, int M, int H,
+                                                    int B, FftPlan plan,
+                                                    const double2 *__restrict__ tw) {
+    double2 *buf = reinterpret_cast<double2 *>(lc_smem);
+    const int col0 = blockIdx.x * B;
+    for (int idx = threadIdx.x; idx < M * B; idx += NT) {
+        const int i = idx / B, c = idx - i * B;
+        buf[swz(idx)] = (col0 + c < H) ? P[(size_t)i * M + col0 + c] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    fft_lds<double, -1, NT, MAXE, true>(buf, B, 0, plan, tw);
+    for (int idx = threadIdx.x; idx < M * B; idx += NT) {
+        const int ky = idx / B, c = idx - ky * B;
+        const int kx = col0 + c;
+        if (kx >= H) continue;
+        const double2 v = buf[lds_col(c, ky, B)];
+        const long long ph = ((long long)(kx + ky) * (M / 2)) % M;
+        double s, co;
+        sincospi(-2.0 * (double)ph / (double)M, &s, &co);
+        out[(size_t)ky * H + kx] = make_float2((float)(v.x * co - v.y * s), (float)(v.x * s + v.y * co));
+    }
 }
 
 void geom2d_init(Ctx &c, Geom2D &g, const double *theta, int64_t nang, int64_t nrad,
@@ -205,7 +211,7 @@ void geom2d_build_psf(Ctx &c, Geom2D &g) {
     const int runs = (M + PSF_RUN - 1) / PSF_RUN;
     const int threads = (M / 2 + 1) * runs;
     hipLaunchKernelGGL(k_psf2d_gen, dim3(ceil_div(threads, 256)), dim3(256), 0, c.stream(), P,
-                       g.d_theta.as<double>(), (int)g.nang, (int)g.nrad, M);
+                       g.d_theta.as<double>(), (int)g.nang, (int)g.nrad, M, 0);
     LC_HIP(hipGetLastError());
     const double2 *tw = c.twiddles<double>(M);
     constexpr int NT = 256, MAXE = 16;
@@ -225,12 +231,24 @@ void geom2d_build_psf(Ctx &c, Geom2D &g) {
 }
 
 void geom2d_kernel_hat(Ctx &c, Geom2D &g, float *out_host) {
-    if (!g.psf_ready) geom2d_build_psf(c, g);
     const int M = (int)g.M, H = (int)g.H;
+    FftPlan plan;
+    LC_REQUIRE(fft_make_plan(M, plan), "PSF size is not 2^p 3^q");
+    double2 *P = reinterpret_cast<double2 *>(c.scratch(0, sizeof(double2) * (size_t)M * M));
+    const int runs = (M + PSF_RUN - 1) / PSF_RUN;
+    hipLaunchKernelGGL(k_psf2d_gen, dim3(ceil_div((int64_t)M * runs, 256)), dim3(256), 0, c.stream(),
+                       P, g.d_theta.as<double>(), (int)g.nang, (int)g.nrad, M, 1);
+    LC_HIP(hipGetLastError());
+    const double2 *tw = c.twiddles<double>(M);
+    constexpr int NT = 256, MAXE = 16;
+    const int nl = std::max(1, NT * MAXE / M);
+    hipLaunchKernelGGL((k_rows_c2c<double, -1, NT, MAXE>), dim3(ceil_div(M, nl)), dim3(NT),
+                       sizeof(double2) * lds_round(nl * M), c.stream(), P, M, nl, plan, tw);
+    LC_HIP(hipGetLastError());
     float2 *tmp = reinterpret_cast<float2 *>(c.scratch(1, sizeof(float2) * (size_t)M * H));
-    const float unscale = (float)((double)M * (double)M * (double)g.N);
-    hipLaunchKernelGGL(k_khat_ref, dim3(ceil_div(H, 256), M), dim3(256), 0, c.stream(),
-                       g.khatT.as<float>(), tmp, M, H, unscale);
+    const int B = std::max(1, NT * MAXE / M);
+    hipLaunchKernelGGL((k_khat_export<NT, MAXE>), dim3(ceil_div(H, B)), dim3(NT),
+                       sizeof(double2) * lds_round(B * M), c.stream(), P, tmp, M, H, B, plan, tw);
     LC_HIP(hipGetLastError());
     LC_HIP(hipMemcpyAsync(out_host, tmp, sizeof(float2) * (size_t)M * H, hipMemcpyDeviceToHost,
                           c.stream()));

@@ -1,0 +1,156 @@
+// C++ API over the C ABI (include/laminocam_tomocam.hpp), exercised the way
+// the reference's own tests exercise its API:
+//   tests/test_adjoint2d.cpp        <forward2d y, z> == <y... , backproj2d z>
+//   tests/test_sysmat_toeplitz2d.cpp sysmat2d(x, grid) == sysmat2d(x, psf)
+//   tests/test_sysmat_toeplitz.cpp   sysmat(x, grid) == sysmat(x, psf)
+//   tests/test_cg_normal.cpp         CG on A x = A x_true reduces the residual
+//   src/projection.cpp:105-108       shift count mismatch -> std::invalid_argument
+// Seeded inputs from a SplitMix64 stream; exits nonzero on the first failure.
+#define LAMINOCAM_STANDALONE_ARRAY
+#include "laminocam_tomocam.hpp"
+
+#include <cmath>
+#include <cstdio>
+
+using namespace tomocam;
+
+static uint64_t g_state = 0x1A41C0DEull;
+static double urand() {
+    uint64_t z = (g_state += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+template <typename T> static Array<T> random_array(dims_t d, double lo = 0.0, double hi = 1.0) {
+    Array<T> a(d);
+    for (size_t i = 0; i < a.size(); ++i) a[i] = (T)(lo + (hi - lo) * urand());
+    return a;
+}
+
+template <typename T> static double dot(const Array<T> &a, const Array<T> &b) {
+    double s = 0.0;
+    for (size_t i = 0; i < a.size(); ++i) s += (double)a[i] * (double)b[i];
+    return s;
+}
+
+template <typename T> static double rel_l2(const Array<T> &a, const Array<T> &b) {
+    double num = 0.0, den = 0.0;
+    for (size_t i = 0; i < a.size(); ++i) {
+        const double d = (double)a[i] - (double)b[i];
+        num += d * d;
+        den += (double)b[i] * (double)b[i];
+    }
+    return std::sqrt(num / den);
+}
+
+static int g_fail = 0;
+#define CHECK(name, cond)                                                     \
+    do {                                                                      \
+        const bool ok_ = (cond);                                              \
+        std::printf("%s %s\n", ok_ ? "PASS" : "FAIL", name);                  \
+        std::fflush(stdout);                                                  \
+        if (!ok_) ++g_fail;                                                   \
+    } while (0)
+
+int main() {
+    std::printf("[case] C++ API over the C ABI\n");
+    std::fflush(stdout);
+    // ---- 2-D ----
+    {
+        const size_t N = 64, nang = 90;
+        std::vector<float> th(nang);
+        for (size_t i = 0; i < nang; ++i) th[i] = (float)(((double)i - nang / 2.0) * M_PI / nang);
+        cpu::PolarGrid2D<float> grid(th, N);
+        Array<float> x = random_array<float>(dims_t{2, N, N});
+        Array<float> z = random_array<float>(dims_t{2, nang, N}, -1.0, 1.0);
+        Array<float> fx = forward2d(x, grid);
+        Array<float> bz = backproj2d(z, grid, dims_t{2, N, N});
+        const double l = dot(fx, z), r = dot(x, bz);
+        CHECK("adjoint2d", std::fabs(l - r) <= 1e-5 * (std::fabs(l) + std::fabs(r)));
+        cpu::PointSpreadFunction2D<float> psf(grid, N);
+        CHECK("psf_size", psf.psf_size() == 128);
+        Array<float> t = sysmat2d(x, psf);
+        Array<float> s = sysmat2d(x, grid);
+        CHECK("sysmat2d toeplitz == nufft", rel_l2(t, s) < 1e-4);
+        Array<std::complex<float>> kh = psf.kernel_hat();
+        CHECK("kernel_hat dims", kh.dims() == (dims_t{1, 128, 65}));
+        // CG on the normal equations (tests/test_cg_normal.cpp)
+        Array<float> xt = random_array<float>(dims_t{1, N, N});
+        Array<float> y = sysmat2d(xt, psf);
+        opt::Function<float> A = opt::toeplitz(psf);
+        Array<float> x0(dims_t{1, N, N});
+        Array<float> x5 = opt::cgsolver(A, y, x0, 5, 1e-12f, 0.0f);
+        Array<float> x40 = opt::cgsolver(A, y, x0, 40, 1e-12f, 0.0f);
+        auto resid = [&](const Array<float> &xx) {
+            Array<float> ax = sysmat2d(xx, psf);
+            double n = 0.0, d = 0.0;
+            for (size_t i = 0; i < ax.size(); ++i) {
+                n += ((double)ax[i] - y[i]) * ((double)ax[i] - y[i]);
+                d += (double)y[i] * y[i];
+            }
+            return std::sqrt(n / d);
+        };
+        const double r5 = resid(x5), r40 = resid(x40);
+        std::printf("  cg residual 5 it %.3e, 40 it %.3e\n", r5, r40);
+        CHECK("cgsolver residual decreases", r40 < r5 && r40 < 1e-2);
+        // an opaque callable cannot be moved to the device
+        opt::Function<float> opaque = [&psf](const Array<float> &v) { return sysmat2d(v, psf); };
+        bool threw = false;
+        try {
+            opt::cgsolver(opaque, y, x0, 3, 1e-5f, 1e-5f);
+        } catch (const std::invalid_argument &) {
+            threw = true;
+        }
+        CHECK("opaque operator rejected", threw);
+        Array<float> sb = opt::split_bregman(A, y, x0, 0.1f, 10.0f, 3, 1, 1e-12f, 0.0f);
+        CHECK("split_bregman runs", std::isfinite(dot(sb, sb)));
+        // interior-only differences (src/finitediff.cpp:26-90) need n1 >= 3
+        Array<float> v = random_array<float>(dims_t{5, 32, 32});
+        auto g = opt::grad_u(v);
+        Array<float> lap = opt::laplacian(v);
+        Array<float> dv = opt::divergence(g);
+        CHECK("laplacian == divergence(grad)", dot(lap, lap) > 0.0 && rel_l2(lap, dv) < 1e-6);
+        // double arrays are converted at the boundary
+        std::vector<double> thd(th.begin(), th.end());
+        cpu::PolarGrid2D<double> gridd(thd, N);
+        Array<double> xd(dims_t{1, N, N});
+        for (size_t i = 0; i < xd.size(); ++i) xd[i] = x[i];
+        Array<double> fd = forward2d(xd, gridd);
+        double md = 0.0;
+        for (size_t i = 0; i < fd.size(); ++i) md = std::fmax(md, std::fabs(fd[i] - fx[i]));
+        CHECK("double arrays", md < 1e-5);
+    }
+    // ---- 3-D ----
+    {
+        const size_t nproj = 9, nrows = 16, ncols = 16;
+        std::vector<float> th(nproj), ga(nproj, (float)(-30.0 * M_PI / 180.0)), be(nproj, 0.0f);
+        for (size_t i = 0; i < nproj; ++i) th[i] = (float)(2.0 * M_PI * i / nproj);
+        cpu::PolarGrid<float> grid(th, ga, be, nrows, ncols);
+        const dims_t rd{5, 16, 16};
+        Array<float> x = random_array<float>(rd);
+        cpu::PointSpreadFunction<float> psf(grid, rd);
+        Array<float> t = sysmat(x, psf);
+        Array<float> s = sysmat(x, grid);
+        CHECK("sysmat3d toeplitz == nufft", rel_l2(t, s) < 1e-4);
+        Array<float> p = forward(x, grid);
+        CHECK("forward3d dims", p.dims() == grid.dims());
+        Array<float> z = random_array<float>(grid.dims(), -1.0, 1.0);
+        const double l = dot(p, z), r = dot(x, backproj(z, grid, rd));
+        CHECK("adjoint3d", std::fabs(l - r) <= 1e-5 * (std::fabs(l) + std::fabs(r)));
+        std::vector<std::array<float, 2>> shifts(nproj - 1);
+        bool threw = false;
+        try {
+            backproj(z, grid, rd, shifts);
+        } catch (const std::invalid_argument &) {
+            threw = true;
+        }
+        CHECK("backproj shift count mismatch", threw);
+        Array<float> y = sysmat(x, psf);
+        Array<float> x3 = opt::cgsolver(opt::toeplitz(psf), y, Array<float>(rd), 4, 1e-12f, 0.0f);
+        CHECK("cgsolver 3-D runs", std::isfinite(dot(x3, x3)));
+    }
+    std::printf("%s: %d failure(s)\n", g_fail ? "FAILED" : "OK", g_fail);
+    return g_fail ? 1 : 0;
+}

@@ -33,6 +33,16 @@ def test_kernel_hat_matches_oracle(lc, orc64, N, nang, nrad):
     assert rel_l2(kh, kr) < 2e-6
 
 
+@pytest.mark.parametrize("N,nang", [(16, 7), (33, 17), (64, 45)])
+def test_kernel_hat_asymmetric_angles(lc, orc64, N, nang):
+    # angles uniform on [0, pi): the PSF has no m_y -> -m_y symmetry, so for
+    # even M the unpaired m = -M/2 row and column make kernel_hat complex
+    announce(f"kernel_hat asymmetric N={N} nang={nang}")
+    th = np.arange(nang) * np.pi / nang + 0.05
+    psf = lc.PointSpreadFunction2D(lc.PolarGrid2D(th, N), N)
+    assert rel_l2(psf.kernel_hat(), orc64.psf2d(th, N, N, method=0).kernel_hat()) < 2e-6
+
+
 @pytest.mark.parametrize("N,nang,ns", [(16, 7, 5), (32, 13, 8), (31, 17, 2), (64, 90, 3),
                                        (96, 60, 2), (128, 180, 2), (192, 90, 1), (256, 180, 1)])
 def test_toeplitz_matches_oracle(lc, orc64, N, nang, ns):
