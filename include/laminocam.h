@@ -185,6 +185,44 @@ int lc_batch2d_sysmat(lc_batch2d b, int32_t src, int32_t dst);
 int lc_batch2d_profile(lc_batch2d b, int32_t reps, int32_t max_rows, double *rows,
                        int32_t *nrows);
 
+/* ---- reconstruction server (no reference counterpart) ----
+ * Continuous batching with HBM-budget admission.  Requests queue on the host;
+ * each of `lanes` device lanes (own stream, own thread) takes every queued
+ * request that fits the free budget and shares the head request's solver
+ * settings, runs them as one batch from captured graphs, and releases their
+ * memory.  Lanes overlap on the device.  Each problem keeps its own solver
+ * scalars: its result is the same alone and in any mix. */
+typedef struct lc_server_s *lc_server;
+typedef struct {
+    lc_geom2d g2;          /* a 2-D stack of nslices slices on g2, or */
+    lc_geom3d g3;          /* one 3-D volume on g3 (exactly one is non-NULL) */
+    int32_t nslices;
+    int32_t solver;        /* 0 CG (src/conjgrad.cpp:33-76), 1 Split-Bregman (src/bregman.cpp:36-94) */
+    int32_t max_iter;      /* CG iterations */
+    int32_t outer, inner;  /* Split-Bregman iterations */
+    double tol, xtol;
+    double lambda, mu;     /* Split-Bregman */
+    uint32_t flags;        /* LC_TV_ZERO_PAD, LC_TV_ANISO */
+    const float *sino;     /* sinogram {nslices, nang, nrad} / projections {nproj, nrows, ncols},
+                              back-projected on the device; or NULL and: */
+    const float *rhs;      /* the right-hand side A^T y */
+    const float *x0;       /* initial guess, NULL for zero */
+} lc_request;
+int lc_server_create(lc_ctx ctx, uint64_t hbm_budget, int32_t lanes, int32_t max_batch,
+                     lc_server *out);
+/* copies the inputs; returns a ticket */
+int lc_server_submit(lc_server s, const lc_request *r, uint64_t *ticket);
+int lc_server_poll(lc_server s, uint64_t ticket, int32_t *done);
+/* timeout_s < 0 waits without limit; *done = 0 on timeout */
+int lc_server_wait(lc_server s, uint64_t ticket, double timeout_s, int32_t *done);
+/* copies the solution into x (problem size) and releases the ticket;
+ * times[3] = submit, start, done (seconds, steady clock) */
+int lc_server_result(lc_server s, uint64_t ticket, float *x, lc_report *rep, double *times);
+/* stats[6] = in-flight bytes, queued, running, finished, batches, budget */
+int lc_server_stats(lc_server s, double *stats);
+/* finishes every queued request, then stops the lanes */
+int lc_server_destroy(lc_server s);
+
 /* ---- Toeplitz kernel shapes per image size (no reference counterpart) ----
  * variant[0] = row-pass shape, variant[1] = column-pass shape,
  * variant[2] = columns per column-pass block.  lc_autotune2d times the

@@ -29,7 +29,7 @@ __all__ = ["Context", "PolarGrid2D", "PointSpreadFunction2D", "PolarGrid", "Poin
            "sysmat", "sysmat2d", "forward", "backproj", "forward2d",
            "backproj2d", "cgsolver",
            "split_bregman", "grad_u", "divergence", "laplacian", "Batch2D", "LcError",
-           "default_context"]
+           "default_context", "Server"]
 
 _default_ctx = None
 
@@ -574,6 +574,106 @@ class Batch2D:
         if self.handle:
             self.lib.lc_batch2d_destroy(self.handle)
             self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Reconstruction server: continuous batching with HBM-budget admission
+    over `lanes` device lanes (lc_server_* in laminocam.h).  submit() copies
+    the inputs and returns a ticket; result() waits and returns
+    (x, report, (t_submit, t_start, t_done))."""
+
+    def __init__(self, hbm_budget: int = 64 << 30, lanes: int = 2, max_batch: int = 512,
+                 ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.lib = self.ctx.lib
+        h = C.c_void_p()
+        _lib.check(self.lib.lc_server_create(self.ctx.handle, int(hbm_budget), int(lanes),
+                                             int(max_batch), C.byref(h)))
+        self.handle = h
+        self._shapes = {}
+
+    def submit(self, psf, *, sino=None, rhs=None, x0=None, solver="cg", max_iter=50,
+               tol=1e-5, xtol=1e-5, lam=0.1, mu=10.0, outer=1, inner=1, zero_pad=False,
+               aniso=False):
+        """psf: PointSpreadFunction2D (a stack) or PointSpreadFunction (a volume).
+        sino: sinogram {ns, nang, nrad} / projections {nproj, nrows, ncols}
+        (back-projected on the device), or rhs: the right-hand side."""
+        r = _lib.LcRequest()
+        keep = []
+        if isinstance(psf, PointSpreadFunction):
+            r.g3 = psf.handle.value
+            shape = psf.recon_dims
+            ns = shape[0]
+        else:
+            r.g2 = psf.handle.value
+            a = sino if sino is not None else rhs
+            ns = int(np.asarray(a).shape[0]) if np.asarray(a).ndim == 3 else 1
+            shape = (ns, psf.image_size, psf.image_size)
+        r.nslices = ns
+        r.solver = 0 if solver == "cg" else 1
+        r.max_iter, r.outer, r.inner = int(max_iter), int(outer), int(inner)
+        r.tol, r.xtol, r.lam, r.mu = float(tol), float(xtol), float(lam), float(mu)
+        r.flags = (LC_TV_ZERO_PAD if zero_pad else 0) | (LC_TV_ANISO if aniso else 0)
+        if sino is not None:
+            s = _f32(sino)
+            keep.append(s)
+            r.sino = s.ctypes.data
+        else:
+            y = _f32(rhs)
+            if y.size != int(np.prod(shape)):
+                raise ValueError(f"rhs size {y.size} does not match {shape}")
+            keep.append(y)
+            r.rhs = y.ctypes.data
+        if x0 is not None:
+            x = _f32(x0)
+            keep.append(x)
+            r.x0 = x.ctypes.data
+        t = C.c_uint64()
+        _lib.check(self.lib.lc_server_submit(self.handle, C.byref(r), C.byref(t)))
+        self._shapes[t.value] = shape
+        return t.value
+
+    def done(self, ticket) -> bool:
+        d = C.c_int32()
+        _lib.check(self.lib.lc_server_poll(self.handle, C.c_uint64(ticket), C.byref(d)))
+        return bool(d.value)
+
+    def result(self, ticket, timeout: float = -1.0):
+        d = C.c_int32()
+        _lib.check(self.lib.lc_server_wait(self.handle, C.c_uint64(ticket), float(timeout),
+                                           C.byref(d)))
+        if not d.value:
+            raise TimeoutError(f"request {ticket} not finished after {timeout} s")
+        shape = self._shapes.pop(ticket)
+        x = np.empty(shape, np.float32)
+        rep = LcReport()
+        times = (C.c_double * 3)()
+        _lib.check(self.lib.lc_server_result(self.handle, C.c_uint64(ticket), _ptr(x),
+                                             C.byref(rep), times))
+        return x, rep.as_dict(), tuple(times)
+
+    def stats(self):
+        s = (C.c_double * 6)()
+        _lib.check(self.lib.lc_server_stats(self.handle, s))
+        return dict(inflight_bytes=s[0], queued=s[1], running=s[2], finished=s[3],
+                    batches=s[4], budget=s[5])
+
+    def close(self):
+        if self.handle:
+            self.lib.lc_server_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def __del__(self):
         try:

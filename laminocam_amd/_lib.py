@@ -26,6 +26,8 @@ EXPORTS = [
     "lc_batch2d_get", "lc_batch2d_device_ptr", "lc_batch2d_cg", "lc_batch2d_split_bregman",
     "lc_batch2d_reports", "lc_batch2d_sysmat", "lc_batch2d_profile",
     "lc_geom3d_create", "lc_geom3d_destroy", "lc_geom3d_info", "lc_psf3d_kernel_hat",
+    "lc_server_create", "lc_server_submit", "lc_server_poll", "lc_server_wait",
+    "lc_server_result", "lc_server_stats", "lc_server_destroy",
     "lc_sysmat3d_psf", "lc_forward3d", "lc_backproj3d", "lc_sysmat3d_nufft", "lc_cg3d", "lc_split_bregman3d", "lc_batch_create", "lc_autotune2d", "lc_set_tuning2d",
     "lc_get_tuning2d",
 ]
@@ -38,6 +40,15 @@ class LcReport(C.Structure):
     def as_dict(self):
         return {"iters": self.iters, "outer": self.outer, "res": self.res,
                 "dx": self.dx, "stop": self.stop}
+
+
+class LcRequest(C.Structure):
+    """lc_request of laminocam.h"""
+    _fields_ = [("g2", C.c_void_p), ("g3", C.c_void_p), ("nslices", C.c_int32),
+                ("solver", C.c_int32), ("max_iter", C.c_int32), ("outer", C.c_int32),
+                ("inner", C.c_int32), ("tol", C.c_double), ("xtol", C.c_double),
+                ("lam", C.c_double), ("mu", C.c_double), ("flags", C.c_uint32),
+                ("sino", C.c_void_p), ("rhs", C.c_void_p), ("x0", C.c_void_p)]
 
 
 class LcError(RuntimeError):
@@ -102,6 +113,13 @@ def load():
         "lc_geom3d_info": (C.c_int, [vp, C.POINTER(i64)]),
         "lc_psf3d_kernel_hat": (C.c_int, [vp, vp, f32p]),
         "lc_sysmat3d_psf": (C.c_int, [vp, vp, vp, vp, u32]),
+        "lc_server_create": (C.c_int, [vp, C.c_uint64, i32, i32, C.POINTER(vp)]),
+        "lc_server_submit": (C.c_int, [vp, C.POINTER(LcRequest), C.POINTER(C.c_uint64)]),
+        "lc_server_poll": (C.c_int, [vp, C.c_uint64, C.POINTER(i32)]),
+        "lc_server_wait": (C.c_int, [vp, C.c_uint64, C.c_double, C.POINTER(i32)]),
+        "lc_server_result": (C.c_int, [vp, C.c_uint64, vp, C.POINTER(LcReport), f64p]),
+        "lc_server_stats": (C.c_int, [vp, f64p]),
+        "lc_server_destroy": (C.c_int, [vp]),
         "lc_forward3d": (C.c_int, [vp, vp, vp, vp, u32]),
         "lc_backproj3d": (C.c_int, [vp, vp, vp, vp, vp, u32]),
         "lc_sysmat3d_nufft": (C.c_int, [vp, vp, vp, vp, u32]),

@@ -4,6 +4,7 @@
 
 #include "laminocam.h"
 #include "lc_engine.hpp"
+#include "lc_server.hpp"
 
 using namespace lc;
 
@@ -633,6 +634,114 @@ int lc_batch2d_sysmat(lc_batch2d b, int32_t src, int32_t dst) {
         b->ctx->c.set_current();
         b->b->toeplitz(batch_buf(*b->b, src), batch_buf(*b->b, dst));
     });
+}
+
+// ---- server ----
+struct lc_server_s {
+    lc_ctx_s *ctx;
+    std::unique_ptr<Server> s;
+};
+
+int lc_server_create(lc_ctx ctx, uint64_t hbm_budget, int32_t lanes, int32_t max_batch,
+                     lc_server *out) {
+    return guard([&] {
+        LC_REQUIRE(ctx && out && hbm_budget > 0, "lc_server_create: bad argument");
+        auto *sv = new lc_server_s();
+        sv->ctx = ctx;
+        try {
+            sv->s = std::make_unique<Server>(ctx->c, (size_t)hbm_budget, lanes, max_batch);
+        } catch (...) {
+            delete sv;
+            throw;
+        }
+        *out = sv;
+    });
+}
+
+int lc_server_submit(lc_server s, const lc_request *r, uint64_t *ticket) {
+    return guard([&] {
+        LC_REQUIRE(s && r && ticket, "lc_server_submit: NULL argument");
+        LC_REQUIRE((r->g2 != nullptr) != (r->g3 != nullptr), "lc_server_submit: need exactly one geometry");
+        LC_REQUIRE(r->sino || r->rhs, "lc_server_submit: need a sinogram or a right-hand side");
+        LC_REQUIRE(r->solver == 0 || r->solver == 1, "lc_server_submit: solver must be 0 (CG) or 1 (SB)");
+        auto q = std::make_unique<ServerRequest>();
+        int64_t nvol, nsino;
+        if (r->g3) {
+            LC_REQUIRE(r->g3->ctx == s->ctx, "lc_server_submit: geometry of another context");
+            const Geom3D &g = r->g3->g;
+            q->g3 = &r->g3->g;
+            q->ns = (int)g.n1;
+            nvol = g.n1 * g.n2 * g.n3;
+            nsino = g.nproj * g.nrows * g.ncols;
+        } else {
+            LC_REQUIRE(r->g2->ctx == s->ctx, "lc_server_submit: geometry of another context");
+            LC_REQUIRE(r->nslices >= 1, "lc_server_submit: nslices must be >= 1");
+            const Geom2D &g = r->g2->g;
+            q->g2 = &r->g2->g;
+            q->ns = r->nslices;
+            nvol = (int64_t)r->nslices * g.N * g.N;
+            nsino = (int64_t)r->nslices * g.nang * g.nrad;
+        }
+        q->solver = r->solver;
+        q->max_iter = r->max_iter;
+        q->outer = r->outer;
+        q->inner = r->inner;
+        LC_REQUIRE(q->max_iter >= 0 && q->outer >= 0 && q->inner >= 0, "lc_server_submit: negative iterations");
+        q->tol = r->tol;
+        q->xtol = r->xtol;
+        q->lambda = (float)r->lambda;
+        q->mu = (float)r->mu;
+        q->flags = r->flags;
+        if (r->sino) {
+            q->has_sino = true;
+            q->sino.assign(r->sino, r->sino + nsino);
+        } else {
+            q->rhs.assign(r->rhs, r->rhs + nvol);
+        }
+        if (r->x0) q->x0.assign(r->x0, r->x0 + nvol);
+        *ticket = s->s->submit(std::move(q));
+    });
+}
+
+int lc_server_poll(lc_server s, uint64_t ticket, int32_t *done) {
+    return guard([&] {
+        LC_REQUIRE(s && done, "lc_server_poll: NULL argument");
+        *done = s->s->poll(ticket) ? 1 : 0;
+    });
+}
+
+int lc_server_wait(lc_server s, uint64_t ticket, double timeout_s, int32_t *done) {
+    return guard([&] {
+        LC_REQUIRE(s, "lc_server_wait: NULL server");
+        const bool d = s->s->wait(ticket, timeout_s);
+        if (done) *done = d ? 1 : 0;
+    });
+}
+
+int lc_server_result(lc_server s, uint64_t ticket, float *x, lc_report *rep, double *times) {
+    return guard([&] {
+        LC_REQUIRE(s && x, "lc_server_result: NULL argument");
+        auto r = s->s->take(ticket);
+        if (r->state == 3) throw Error(LC_EINTERNAL, "server: request failed: " + r->err);
+        std::memcpy(x, r->x.data(), sizeof(float) * r->x.size());
+        if (rep) *rep = r->rep;
+        if (times) {
+            times[0] = r->t_submit;
+            times[1] = r->t_start;
+            times[2] = r->t_done;
+        }
+    });
+}
+
+int lc_server_stats(lc_server s, double *stats) {
+    return guard([&] {
+        LC_REQUIRE(s && stats, "lc_server_stats: NULL argument");
+        s->s->stats(stats);
+    });
+}
+
+int lc_server_destroy(lc_server s) {
+    return guard([&] { delete s; });
 }
 
 } // extern "C"

@@ -1,0 +1,284 @@
+// Reconstruction request server: HBM-budget admission and continuous
+// batching over several device lanes.
+//
+// A request is one reconstruction: a 2-D stack or a 3-D volume on a
+// geometry, its sinogram (back-projected on the device) or right-hand side,
+// and a CG or Split-Bregman solve.  Requests queue on the host.  Each lane is
+// a context of its own (one stream, its own scratch) driven by one thread:
+// whenever a lane is idle it takes every queued request that fits the free
+// HBM budget and shares its solver settings, builds one batch, and runs the
+// solve from captured graphs.  Requests arriving while other lanes are busy
+// start on the next idle lane, so their work overlaps on the device with the
+// batches already running.  Each problem keeps its own CG / Bregman scalars,
+// so its result does not depend on the batch it ran in.
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <thread>
+
+#include "lc_engine.hpp"
+#include "lc_server.hpp"
+
+namespace lc {
+
+namespace {
+double now_s() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+} // namespace
+
+// device bytes one request holds while it runs (batch vectors, Toeplitz
+// intermediates, Split-Bregman fields, back-projection scratch)
+size_t request_bytes(const ServerRequest &r) {
+    int64_t N, ns, M, H, extra = 0, bp = 0;
+    if (r.g3) {
+        const Geom3D &g = *r.g3;
+        N = g.N;
+        ns = g.n1;
+        M = g.M;
+        H = g.H;
+        extra = 8 * (H + 16) * M * ns;   // W: column blocks padded to B <= 16
+        if (r.has_sino) {
+            const Nufft3D &u = g.nu;
+            bp = 8 * (g.nproj * g.nrows * g.ncols) * 2 +
+                 8 * ((int64_t)u.nf[0] * u.nf[1] * u.nf[2] + (int64_t)u.nf[0] * u.nf[1] * g.n3 +
+                      (int64_t)u.nf[0] * g.n2 * g.n3);
+        }
+    } else {
+        const Geom2D &g = *r.g2;
+        N = g.N;
+        ns = r.ns;
+        M = g.M;
+        H = g.H;
+        if (r.has_sino) {
+            const int64_t nf = g.nu.nf;
+            bp = 4 * ns * g.nang * g.nrad + 8 * ns * (g.nang * g.nrad + nf * nf + nf * N);
+        }
+    }
+    const int64_t n = ns * N * N;
+    int64_t b = 5 * 4 * n + 8 * ns * (H + 16) * N + extra + bp;
+    if (r.solver == 1) b += 4 * 8 * n;   // d, b (3 components each), x_old, y0
+    return (size_t)b + (1 << 20);
+}
+
+Server::Server(Ctx &main, size_t budget, int nlanes, int max_batch)
+    : main_(main), budget_(budget), max_batch_(max_batch) {
+    LC_REQUIRE(nlanes >= 1 && nlanes <= 16, "server: 1 to 16 lanes");
+    LC_REQUIRE(max_batch >= 1, "server: max_batch must be >= 1");
+    for (int i = 0; i < nlanes; ++i) {
+        auto l = std::make_unique<Lane>();
+        l->ctx = std::make_unique<Ctx>(main.device());
+        l->ctx->toep_tuning = main.toep_tuning;
+        lanes_.push_back(std::move(l));
+    }
+    for (auto &l : lanes_) l->th = std::thread([this, lp = l.get()] { lane_loop(*lp); });
+}
+
+Server::~Server() {
+    {
+        std::lock_guard<std::mutex> lk(m_);
+        stop_ = true;
+    }
+    cv_work_.notify_all();
+    for (auto &l : lanes_)
+        if (l->th.joinable()) l->th.join();
+}
+
+uint64_t Server::submit(std::unique_ptr<ServerRequest> r) {
+    // geometry state the lanes only read: PSF and NUFFT tables, built here
+    {
+        std::lock_guard<std::mutex> lk(main_.mu);
+        main_.set_current();
+        if (r->g3) {
+            if (!r->g3->psf_ready) geom3d_build_psf(main_, *r->g3);
+            if (r->has_sino && !r->g3->nu.ready) nufft3d_init(main_, *r->g3);
+        } else {
+            if (!r->g2->psf_ready) geom2d_build_psf(main_, *r->g2);
+            if (r->has_sino && !r->g2->nu.ready) nufft2d_init(main_, *r->g2);
+        }
+        main_.sync();
+    }
+    r->bytes = request_bytes(*r);
+    LC_REQUIRE(r->bytes <= budget_, "server: request needs " + std::to_string(r->bytes) +
+                                        " bytes, above the HBM budget of " + std::to_string(budget_));
+    r->t_submit = now_s();
+    std::lock_guard<std::mutex> lk(m_);
+    const uint64_t id = ++next_id_;
+    r->id = id;
+    queue_.push_back(id);
+    reqs_[id] = std::move(r);
+    ++queued_;
+    cv_work_.notify_one();
+    return id;
+}
+
+bool Server::wait(uint64_t id, double timeout_s) {
+    std::unique_lock<std::mutex> lk(m_);
+    auto it = reqs_.find(id);
+    LC_REQUIRE(it != reqs_.end(), "server: unknown ticket");
+    ServerRequest *r = it->second.get();
+    auto done = [r] { return r->state >= 2; };
+    if (timeout_s < 0) cv_done_.wait(lk, done);
+    else cv_done_.wait_for(lk, std::chrono::duration<double>(timeout_s), done);
+    return r->state >= 2;
+}
+
+bool Server::poll(uint64_t id) {
+    std::lock_guard<std::mutex> lk(m_);
+    auto it = reqs_.find(id);
+    LC_REQUIRE(it != reqs_.end(), "server: unknown ticket");
+    return it->second->state >= 2;
+}
+
+std::unique_ptr<ServerRequest> Server::take(uint64_t id) {
+    std::lock_guard<std::mutex> lk(m_);
+    auto it = reqs_.find(id);
+    LC_REQUIRE(it != reqs_.end(), "server: unknown ticket");
+    LC_REQUIRE(it->second->state >= 2, "server: request not finished");
+    auto r = std::move(it->second);
+    reqs_.erase(it);
+    return r;
+}
+
+void Server::stats(double *out) {
+    std::lock_guard<std::mutex> lk(m_);
+    out[0] = (double)inflight_;
+    out[1] = (double)queued_;
+    out[2] = (double)running_;
+    out[3] = (double)done_;
+    out[4] = (double)batches_;
+    out[5] = (double)budget_;
+}
+
+// requests that can share one batch solve
+static bool same_group(const ServerRequest &a, const ServerRequest &b) {
+    if (a.solver != b.solver || a.tol != b.tol || a.xtol != b.xtol) return false;
+    if (a.solver == 1)
+        return a.outer == b.outer && a.inner == b.inner &&
+               (a.flags & (LC_TV_ZERO_PAD | LC_TV_ANISO)) == (b.flags & (LC_TV_ZERO_PAD | LC_TV_ANISO));
+    return true;
+}
+
+void Server::lane_loop(Lane &lane) {
+    Ctx &c = *lane.ctx;
+    for (;;) {
+        std::vector<ServerRequest *> batch;
+        {
+            std::unique_lock<std::mutex> lk(m_);
+            for (;;) {
+                if (stop_ && queue_.empty()) return;
+                if (!queue_.empty()) {
+                    // first-come requests of the head's group that fit the free budget
+                    ServerRequest *head = reqs_[queue_.front()].get();
+                    size_t room = budget_ - inflight_;
+                    if (head->bytes <= room) {
+                        for (auto it = queue_.begin(); it != queue_.end() && (int)batch.size() < max_batch_;) {
+                            ServerRequest *r = reqs_[*it].get();
+                            if (same_group(*head, *r) && r->bytes <= room) {
+                                room -= r->bytes;
+                                batch.push_back(r);
+                                it = queue_.erase(it);
+                            } else {
+                                ++it;
+                            }
+                        }
+                        break;
+                    }
+                }
+                cv_work_.wait(lk);
+            }
+            for (ServerRequest *r : batch) {
+                inflight_ += r->bytes;
+                r->state = 1;
+                r->t_start = now_s();
+            }
+            queued_ -= batch.size();
+            running_ += batch.size();
+            ++batches_;
+        }
+        std::string err;
+        try {
+            run_batch(c, batch);
+        } catch (const std::exception &e) {
+            err = e.what();
+        }
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            const double t = now_s();
+            for (ServerRequest *r : batch) {
+                inflight_ -= r->bytes;
+                r->t_done = t;
+                r->state = err.empty() ? 2 : 3;
+                r->err = err;
+            }
+            running_ -= batch.size();
+            done_ += batch.size();
+        }
+        cv_done_.notify_all();
+        cv_work_.notify_all();   // memory freed: other lanes may admit now
+    }
+}
+
+void Server::run_batch(Ctx &c, std::vector<ServerRequest *> &batch) {
+    std::lock_guard<std::mutex> lk(c.mu);
+    c.set_current();
+    std::vector<ProbSpec> ps;
+    for (ServerRequest *r : batch)
+        ps.push_back(r->g3 ? ProbSpec{nullptr, (int)r->g3->n1, r->g3} : ProbSpec{r->g2, r->ns});
+    Batch2D b(c, ps);
+    // inputs: right-hand sides (or sinograms back-projected into y), initial guesses
+    DBuf sino;
+    bool any_x0 = false;
+    for (size_t i = 0; i < batch.size(); ++i) {
+        ServerRequest *r = batch[i];
+        float *y = b.Y() + b.offset((int)i);
+        if (r->has_sino) {
+            sino.reserve(sizeof(float) * r->sino.size());
+            copy_in(c, sino.as(), r->sino.data(), sizeof(float) * r->sino.size(), false);
+            if (r->g3) backproj3d_dev(c, *r->g3, sino.as<float>(), y, nullptr);
+            else backproj2d_dev(c, *r->g2, sino.as<float>(), y, r->ns);
+            c.sync();   // the staging buffer is reused by the next request
+        } else {
+            copy_in(c, y, r->rhs.data(), sizeof(float) * r->rhs.size(), false);
+        }
+        if (!r->x0.empty()) any_x0 = true;
+    }
+    for (size_t i = 0; i < batch.size(); ++i) {
+        ServerRequest *r = batch[i];
+        float *x = b.X() + b.offset((int)i);
+        if (!r->x0.empty()) copy_in(c, x, r->x0.data(), sizeof(float) * r->x0.size(), false);
+        else LC_HIP(hipMemsetAsync(x, 0, sizeof(float) * (size_t)b.size((int)i), c.stream()));
+    }
+    const ServerRequest &h = *batch.front();
+    if (h.solver == 0) {
+        std::vector<int> mi;
+        for (ServerRequest *r : batch) mi.push_back(r->max_iter);
+        b.cg(mi, h.tol, h.xtol, true, false, !any_x0);
+    } else {
+        std::vector<float> la, mu;
+        for (ServerRequest *r : batch) {
+            la.push_back(r->lambda);
+            mu.push_back(r->mu);
+        }
+        b.split_bregman(la, mu, h.outer, h.inner, h.tol, h.xtol, (h.flags & LC_TV_ZERO_PAD) ? 1 : 0,
+                        (h.flags & LC_TV_ANISO) != 0, true, false);
+    }
+    const auto cs = b.cg_state();
+    const auto ss = h.solver == 1 ? b.sb_state() : std::vector<SbState>();
+    for (size_t i = 0; i < batch.size(); ++i) {
+        ServerRequest *r = batch[i];
+        r->x.resize((size_t)b.size((int)i));
+        copy_out(c, r->x.data(), b.X() + b.offset((int)i), sizeof(float) * r->x.size(), false);
+        r->rep.iters = cs[i].iters;
+        r->rep.outer = h.solver == 1 ? ss[i].outer : 0;
+        r->rep.res = cs[i].res;
+        r->rep.dx = h.solver == 1 ? ss[i].norm_diff : cs[i].dx;
+        r->rep.stop = cs[i].stop;
+        r->rep.pad = 0;
+    }
+    c.sync();
+}
+
+} // namespace lc

@@ -1,0 +1,75 @@
+// Reconstruction request server (continuous batching, HBM-budget admission).
+#pragma once
+
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "lc_engine.hpp"
+
+namespace lc {
+
+struct ServerRequest {
+    uint64_t id = 0;
+    Geom2D *g2 = nullptr;      // 2-D stack of ns slices, or
+    Geom3D *g3 = nullptr;      // one 3-D volume
+    int ns = 1;
+    int solver = 0;            // 0 CG, 1 Split-Bregman
+    int max_iter = 50;         // CG iterations
+    int outer = 1, inner = 1;  // Split-Bregman iterations
+    double tol = 1e-5, xtol = 1e-5;
+    float lambda = 0.1f, mu = 10.0f;
+    uint32_t flags = 0;        // LC_TV_ZERO_PAD, LC_TV_ANISO
+    bool has_sino = false;
+    std::vector<float> sino;   // sinogram / projections (back-projected on the device)
+    std::vector<float> rhs;    // right-hand side when no sinogram
+    std::vector<float> x0;     // empty: start from zero
+    // results
+    std::vector<float> x;
+    lc_report rep{};
+    size_t bytes = 0;          // device bytes held while running
+    double t_submit = 0, t_start = 0, t_done = 0;
+    int state = 0;             // 0 queued, 1 running, 2 done, 3 failed
+    std::string err;
+};
+
+size_t request_bytes(const ServerRequest &r);
+
+class Server {
+  public:
+    Server(Ctx &main, size_t budget, int lanes, int max_batch);
+    ~Server();
+    uint64_t submit(std::unique_ptr<ServerRequest> r);
+    bool poll(uint64_t id);
+    bool wait(uint64_t id, double timeout_s);   // < 0: no timeout
+    std::unique_ptr<ServerRequest> take(uint64_t id);
+    // in-flight bytes, queued, running, finished, batches, budget
+    void stats(double *out);
+
+  private:
+    struct Lane {
+        std::unique_ptr<Ctx> ctx;
+        std::thread th;
+    };
+    void lane_loop(Lane &lane);
+    void run_batch(Ctx &c, std::vector<ServerRequest *> &batch);
+
+    Ctx &main_;
+    size_t budget_;
+    int max_batch_;
+    std::vector<std::unique_ptr<Lane>> lanes_;
+    std::mutex m_;
+    std::condition_variable cv_work_, cv_done_;
+    std::deque<uint64_t> queue_;
+    std::map<uint64_t, std::unique_ptr<ServerRequest>> reqs_;
+    uint64_t next_id_ = 0;
+    size_t inflight_ = 0, queued_ = 0, running_ = 0, done_ = 0, batches_ = 0;
+    bool stop_ = false;
+};
+
+} // namespace lc

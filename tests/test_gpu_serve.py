@@ -1,0 +1,119 @@
+"""Reconstruction server (lc_server_*): continuous batching with HBM-budget
+admission.  Every result must be bitwise the result of solving that request
+alone through the single-problem entry points, whatever else ran with it."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from conftest import announce
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lc():
+    import laminocam_amd
+    return laminocam_amd
+
+
+@pytest.fixture(scope="module")
+def problems(lc):
+    rng = np.random.default_rng(17)
+    g64 = lc.PolarGrid2D(np.arange(90) * np.pi / 90, 64)
+    g96 = lc.PolarGrid2D(np.arange(60) * np.pi / 60, 96)
+    p64 = lc.PointSpreadFunction2D(g64, 64)
+    p96 = lc.PointSpreadFunction2D(g96, 96)
+    th = np.arange(13) * 2 * np.pi / 13
+    g3 = lc.PolarGrid(th, np.full(13, -np.pi / 6), np.zeros(13), 10, 32)
+    p3 = lc.PointSpreadFunction(g3, (6, 32, 32))
+    reqs = []
+    for i in range(10):
+        if i % 4 == 3:
+            y = p3.convolve(rng.random((6, 32, 32)))
+            reqs.append(dict(psf=p3, rhs=y, solver="cg", max_iter=8))
+        elif i % 4 == 2:
+            y = p96.convolve(rng.random((2, 96, 96)))
+            reqs.append(dict(psf=p96, rhs=y, solver="sb", lam=0.05 + 0.01 * i, mu=10.0, outer=4,
+                             inner=1))
+        else:
+            g, p, N = (g64, p64, 64) if i % 2 == 0 else (g96, p96, 96)
+            sino = lc.forward2d(rng.random((1 + i % 3, N, N)), g)
+            reqs.append(dict(psf=p, grid=g, sino=sino, solver="cg", max_iter=10))
+    return reqs
+
+
+def solve_alone(lc, r):
+    p = r["psf"]
+    if "sino" in r:
+        N = p.image_size
+        rhs = lc.backproj2d(r["sino"], r["grid"], (r["sino"].shape[0], N, N))
+    else:
+        rhs = r["rhs"]
+    x0 = np.zeros(rhs.shape, np.float32)
+    if r["solver"] == "cg":
+        return lc.cgsolver(p, rhs, x0, r["max_iter"], 1e-5, 1e-5)[0]
+    return lc.split_bregman(p, rhs, x0, r["lam"], r["mu"], r["outer"], r["inner"], 1e-5, 1e-5)[0]
+
+
+def submit(srv, r):
+    kw = {k: v for k, v in r.items() if k not in ("psf", "grid")}
+    return srv.submit(r["psf"], **kw)
+
+
+def test_server_same_alone_and_in_mix(lc, problems):
+    announce("server mix vs alone")
+    ref = [solve_alone(lc, r) for r in problems]
+    with lc.Server(hbm_budget=8 << 30, lanes=3) as srv:
+        tickets = []
+        for i, r in enumerate(problems):
+            tickets.append(submit(srv, r))
+            if i % 3 == 2:
+                time.sleep(0.002)   # later requests arrive while earlier ones run
+        out = [srv.result(t, timeout=120.0) for t in tickets]
+        st = srv.stats()
+    for i, ((x, rep, times), xr) in enumerate(zip(out, ref)):
+        assert x.shape == xr.shape, i
+        assert np.array_equal(x, xr), f"request {i} differs from its solo solve"
+        assert times[0] <= times[1] <= times[2]
+    assert st["finished"] == len(problems) and st["inflight_bytes"] == 0
+
+
+def test_server_budget_admission(lc, problems):
+    # a budget that holds only one or two of these requests at a time still
+    # completes every request, over several batches, with the same results
+    announce("server budget admission")
+    reqs = [r for r in problems if r["solver"] == "cg"][:4]
+    ref = [solve_alone(lc, r) for r in reqs]
+    with lc.Server(hbm_budget=5 << 20, lanes=2) as srv:
+        tickets = [submit(srv, r) for r in reqs]
+        out = [srv.result(t, timeout=120.0)[0] for t in tickets]
+        st = srv.stats()
+    for x, xr in zip(out, ref):
+        assert np.array_equal(x, xr)
+    assert st["batches"] >= 2
+
+
+def test_server_rejects_request_above_budget(lc, problems):
+    announce("server oversized request")
+    with lc.Server(hbm_budget=1 << 20, lanes=1) as srv:
+        with pytest.raises(lc.LcError):
+            submit(srv, problems[0])
+
+
+def test_server_threads_submit_concurrently(lc, problems):
+    announce("server concurrent submitters")
+    ref = [solve_alone(lc, r) for r in problems[:6]]
+    res = [None] * 6
+    with lc.Server(hbm_budget=8 << 30, lanes=2) as srv:
+        def worker(i):
+            t = submit(srv, problems[i])
+            res[i] = srv.result(t, timeout=120.0)[0]
+        ths = [threading.Thread(target=worker, args=(i,)) for i in range(6)]
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join(timeout=180.0)
+    for x, xr in zip(res, ref):
+        assert x is not None and np.array_equal(x, xr)
