@@ -126,6 +126,96 @@ def cpu_baseline(args, iters_by_slice, Ns, rhs_by_n):
     }
 
 
+def served_stream(args, lc, synth, ctx):
+    """Served mixed stream: Poisson arrivals at a fixed rate of 2-D slices
+    (N 128-1024, 180-720 angles) and 3-D laminography volumes (8-32 slices x
+    256-511^2, 180-720 projections, 30 degree tilt), 70% CG / 30% Split-Bregman
+    TV (lambda U[0.05, 0.2], mu 10), submitted as sinograms / projections to the
+    reconstruction server (HBM-budget admission, continuous batching).  Inputs
+    come from a pool of seeded phantoms per geometry built before the clock."""
+    import threading
+    rng = synth.Stream((5 << 16) + 1)
+    geo2, pool2 = [], []
+    for N in N_SET:
+        for a in (180, 360, 720):
+            grid = lc.PolarGrid2D(synth.angles(a), N)
+            psf = lc.PointSpreadFunction2D(grid, N, ctx=ctx)
+            ph = synth.ellipse_phantom(N, 10, synth.Stream((5 << 16) + 100 + len(geo2)))[None]
+            s = lc.forward2d(ph, grid, ctx=ctx)
+            s = synth.add_gaussian_noise(s, 0.01, synth.Stream((5 << 16) + 200 + len(geo2)))
+            geo2.append(psf)
+            pool2.append(s / np.float32(np.max(np.abs(s))))
+    vols = [((8, 256, 256), 180, 64), ((16, 384, 384), 360, 96), ((32, 511, 511), 720, 128)]
+    geo3, pool3 = [], []
+    for i, (dims, nproj, nrows) in enumerate(vols):
+        th = np.arange(nproj) * (2 * np.pi / nproj)
+        grid = lc.PolarGrid(th, np.full(nproj, -np.pi / 6), np.zeros(nproj), nrows, dims[2])
+        psf = lc.PointSpreadFunction(grid, dims, ctx=ctx)
+        ph = synth.ellipsoid_phantom(dims, 12, synth.Stream((5 << 16) + 300 + i))
+        p = lc.forward(ph, grid, ctx=ctx)
+        p = synth.add_gaussian_noise(p, 0.01, synth.Stream((5 << 16) + 400 + i))
+        geo3.append(psf)
+        pool3.append(p / np.float32(np.max(np.abs(p))))
+    n = max(1, int(round(args.serve_rate * args.serve_seconds)))
+    gaps = -np.log(1.0 - rng.uniform(n)) / args.serve_rate
+    kinds = rng.uniform(n)
+    picks2 = rng.choice(list(range(len(geo2))), n)
+    picks3 = rng.choice(list(range(len(geo3))), n)
+    solv = rng.uniform(n)
+    lams = rng.uniform(n, 0.05, 0.2)
+    reqs = []
+    for i in range(n):
+        is3 = kinds[i] < args.serve_3d_frac
+        psf = geo3[picks3[i]] if is3 else geo2[picks2[i]]
+        sino = pool3[picks3[i]] if is3 else pool2[picks2[i]]
+        sb = solv[i] < 0.3
+        slices = psf.recon_dims[0] if is3 else 1
+        kw = dict(solver="sb", lam=float(lams[i]), mu=10.0, outer=args.sb_outer, inner=1) if sb \
+            else dict(solver="cg", max_iter=args.iters)
+        reqs.append((psf, sino, kw, slices, is3, sb))
+    with lc.Server(hbm_budget=int(args.serve_budget_gb * (1 << 30)), lanes=args.serve_lanes,
+                   ctx=ctx) as srv:
+        # warm the lanes' kernels and scratch on one request of each kind
+        for psf, sino in ((geo2[0], pool2[0]), (geo3[0], pool3[0])):
+            srv.result(srv.submit(psf, sino=sino, max_iter=2), timeout=120.0)
+        tickets = [None] * n
+        t0 = time.perf_counter()
+        arrive = t0 + np.cumsum(gaps)
+
+        def submitter():
+            for i, (psf, sino, kw, _, _, _) in enumerate(reqs):
+                d = arrive[i] - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                tickets[i] = srv.submit(psf, sino=sino, tol=args.tol, xtol=args.xtol, **kw)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        out = []
+        for i in range(n):
+            x, rep, tm = srv.result(tickets[i], timeout=600.0)
+            out.append(tm)
+        st = srv.stats()
+    lat = np.array([tm[2] - tm[0] for tm in out])
+    t_first = min(tm[0] for tm in out)
+    t_last = max(tm[2] for tm in out)
+    slices = sum(r[3] for r in reqs)
+    return {
+        "requests": n, "rate_req_per_s": args.serve_rate, "seconds_of_arrivals": args.serve_seconds,
+        "slices": int(slices), "slices_per_s": round(slices / (t_last - t_first), 2),
+        "requests_per_s": round(n / (t_last - t_first), 2),
+        "latency_ms": {"p50": round(1e3 * float(np.median(lat)), 2),
+                       "p99": round(1e3 * float(np.percentile(lat, 99)), 2)},
+        "mix": {"3d": int(sum(r[4] for r in reqs)), "split_bregman": int(sum(r[5] for r in reqs))},
+        "lanes": args.serve_lanes, "budget_gb": args.serve_budget_gb, "batches": int(st["batches"]),
+        "workload": "Poisson arrivals; 2-D slices N 128-1024 x 180/360/720 angles and 3-D volumes "
+                    "8x256^2 / 16x384^2 / 32x511^2 (180/360/720 projections, 30 deg tilt); 70% CG "
+                    f"({args.iters} it), 30% Split-Bregman ({args.sb_outer} outer x 1 inner, "
+                    "lambda U[0.05,0.2], mu 10); sinograms back-projected on the device",
+    }
+
+
 def pmc_probe(args):
     """Short run for rocprofv3 --pmc: a device copy of known size (calibration)
     and a few batched Toeplitz applies of the benchmark workload."""
@@ -175,6 +265,13 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--kernel-reps", type=int, default=10)
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="served stream: requests/s")
+    ap.add_argument("--serve-seconds", type=float, default=5.0, help="served stream: seconds of arrivals")
+    ap.add_argument("--serve-3d-frac", type=float, default=0.05)
+    ap.add_argument("--serve-lanes", type=int, default=3)
+    ap.add_argument("--serve-budget-gb", type=float, default=96.0)
+    ap.add_argument("--sb-outer", type=int, default=20)
+    ap.add_argument("--no-serve", action="store_true", help="skip the served stream")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only run a calibration copy and a few Toeplitz applies (for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -296,6 +393,14 @@ def main():
         f"port {['%.2e' % e for e in errs32]}; FP64 rerun floor {['%.2e' % e for e in floor64]}; "
         f"GPU at {args.short_iters} it {['%.2e' % e for e in errs_short]}")
 
+    # ---- served mixed stream ----
+    served = None
+    if not args.no_serve:
+        served = served_stream(args, lc, synth, ctx)
+        log(f"served: {served['requests']} requests, {served['slices_per_s']} slices/s, latency "
+            f"p50 {served['latency_ms']['p50']} ms p99 {served['latency_ms']['p99']} ms, "
+            f"{served['batches']} batches")
+
     # ---- CPU baseline ----
     cpu = None
     if not args.no_cpu:
@@ -348,6 +453,7 @@ def main():
                             "note": "fp64_floor: FP64 oracle rerun on a right-hand side perturbed "
                                     "by 1e-15 relative; long CG solves of these under-sampled "
                                     "systems drift by this much from rounding alone"},
+        "served": served,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernel_passes": [{"N": r["N"], "pass": PASS_NAMES[r["pass_"]], "ms": round(r["ms"], 4),

@@ -91,20 +91,28 @@ def ellipsoid_phantom(dims, n_ellipsoids: int, rng: Stream, piecewise: bool = Fa
     z = (np.arange(n1) + 0.5) * (2.0 / n1) - 1.0
     y = (np.arange(n2) + 0.5) * (2.0 / n2) - 1.0
     x = (np.arange(n3) + 0.5) * (2.0 / n3) - 1.0
-    zz, yy, xx = np.meshgrid(z, y, x, indexing="ij")
     vol = np.zeros(dims, np.float64)
     for _ in range(n_ellipsoids):
         cz, cy, cx = rng.uniform(3, -0.5, 0.5)
         az, ay, ax = rng.uniform(3, 0.05, 0.4)
         phi = rng.uniform(1, 0.0, np.pi)[0]
         val = rng.uniform(1)[0]
+        # only the bounding box of the circumscribed cylinder can be inside
+        rxy = max(ax, ay)
+        iz = np.nonzero(np.abs(z - cz) <= az)[0]
+        iy = np.nonzero(np.abs(y - cy) <= rxy)[0]
+        ix = np.nonzero(np.abs(x - cx) <= rxy)[0]
+        if iz.size == 0 or iy.size == 0 or ix.size == 0:
+            continue
+        zz, yy, xx = np.meshgrid(z[iz], y[iy], x[ix], indexing="ij")
         xr = (xx - cx) * np.cos(phi) + (yy - cy) * np.sin(phi)
         yr = -(xx - cx) * np.sin(phi) + (yy - cy) * np.cos(phi)
         inside = (xr / ax) ** 2 + (yr / ay) ** 2 + ((zz - cz) / az) ** 2 <= 1.0
+        sub = vol[iz[0]:iz[-1] + 1, iy[0]:iy[-1] + 1, ix[0]:ix[-1] + 1]
         if piecewise:
-            vol[inside] = val
+            sub[inside] = val
         else:
-            vol[inside] += val
+            sub[inside] += val
     return vol.astype(np.float32)
 
 
