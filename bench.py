@@ -201,6 +201,24 @@ def served_stream(args, lc, synth, ctx):
     t_first = min(tm[0] for tm in out)
     t_last = max(tm[2] for tm in out)
     slices = sum(r[3] for r in reqs)
+    # the plain path on the same mix: the first requests of the stream solved
+    # one at a time through the single-problem entry points (back-projection,
+    # then cgsolver / split_bregman, host arrays in and out)
+    k = min(n, args.serve_plain_sample)
+    t0 = time.perf_counter()
+    for psf, sino, kw, _, is3, sb in reqs[:k]:
+        if is3:
+            rhs = lc.backproj(sino, psf.grid, psf.recon_dims, ctx=ctx)
+        else:
+            N = psf.image_size
+            rhs = lc.backproj2d(sino, psf.grid, (sino.shape[0], N, N), ctx=ctx)
+        x0 = np.zeros(rhs.shape, np.float32)
+        if sb:
+            lc.split_bregman(psf, rhs, x0, kw["lam"], kw["mu"], kw["outer"], kw["inner"], args.tol,
+                             args.xtol)
+        else:
+            lc.cgsolver(psf, rhs, x0, kw["max_iter"], args.tol, args.xtol)
+    plain_rate = sum(r[3] for r in reqs[:k]) / (time.perf_counter() - t0)
     return {
         "requests": n, "rate_req_per_s": args.serve_rate, "seconds_of_arrivals": args.serve_seconds,
         "slices": int(slices), "slices_per_s": round(slices / (t_last - t_first), 2),
@@ -208,12 +226,82 @@ def served_stream(args, lc, synth, ctx):
         "latency_ms": {"p50": round(1e3 * float(np.median(lat)), 2),
                        "p99": round(1e3 * float(np.percentile(lat, 99)), 2)},
         "mix": {"3d": int(sum(r[4] for r in reqs)), "split_bregman": int(sum(r[5] for r in reqs))},
+        "plain_same_mix": {"slices_per_s": round(plain_rate, 2), "sample_requests": k},
+        "speedup_vs_plain_same_mix": round(slices / (t_last - t_first) / plain_rate, 2),
         "lanes": args.serve_lanes, "budget_gb": args.serve_budget_gb, "batches": int(st["batches"]),
+        "lane_seconds": {k[7:]: round(v, 3) for k, v in st.items() if k.startswith("lane_s_")},
         "workload": "Poisson arrivals; 2-D slices N 128-1024 x 180/360/720 angles and 3-D volumes "
                     "8x256^2 / 16x384^2 / 32x511^2 (180/360/720 projections, 30 deg tilt); 70% CG "
                     f"({args.iters} it), 30% Split-Bregman ({args.sb_outer} outer x 1 inner, "
                     "lambda U[0.05,0.2], mu 10); sinograms back-projected on the device",
     }
+
+
+def lamino_leg(args, lc, synth, ctx):
+    """3-D laminography solves at the padded shapes of the laminography
+    configurations (src/mbir.cpp:34-161 pads recon [21, 511, 511] to
+    [29, 721, 721] and projections 360 x 64 x 511 to 360 x 89 x 721):
+    360 angles over [0, 360) at 30 degree tilt, PSF (64, 1458, 1458) in FP64.
+      CG:  12-ellipsoid phantom, 1% Gaussian noise, 100 iterations.
+      SB:  piecewise-constant phantom, Poisson counts at 1e4 photons / ray
+           (I = Poisson(1e4 exp(-s p)), s = 2 / max p, y = -ln(max(I, 1) / 1e4) / s),
+           lambda 0.1, mu 10, inner 1, 100 outer iterations, isotropic shrink,
+           interior-only differences.
+    Projections come from the GPU forward projector; timing covers the solve
+    only (inputs resident)."""
+    dims = (29, 721, 721)
+    nproj = 360
+    th = np.arange(nproj) * (2 * np.pi / nproj)
+    grid = lc.PolarGrid(th, np.full(nproj, -np.pi / 6), np.zeros(nproj), 89, 721)
+    t0 = time.perf_counter()
+    psf = lc.PointSpreadFunction(grid, dims, ctx=ctx)
+    ctx.sync()
+    t_psf = time.perf_counter() - t0
+    out = {"dims": list(dims), "projections": [nproj, 89, 721], "psf_dims": list(psf.dims()),
+           "psf_build_s": round(t_psf, 3)}
+    # CG (configuration 2)
+    ph = synth.ellipsoid_phantom(dims, 12, synth.Stream((2 << 16) + 1))
+    p = lc.forward(ph, grid, ctx=ctx)
+    p = synth.add_gaussian_noise(p, 0.01, synth.Stream((2 << 16) + 2))
+    p /= np.float32(np.max(np.abs(p)))
+    t0 = time.perf_counter()
+    y = lc.backproj(p, grid, dims, ctx=ctx)
+    ctx.sync()
+    t_bp = time.perf_counter() - t0
+    b = lc.Batch2D([psf], ctx=ctx)
+    b.set(0, b.Y, y)
+    b.cg(2, args.tol, args.xtol, graph=True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    b.cg(args.lamino_iters, 0.0, 0.0, graph=True)
+    ctx.sync()
+    t_cg = time.perf_counter() - t0
+    rep = b.reports()[0]
+    out["cg"] = {"iters": int(rep["iters"]), "seconds": round(t_cg, 4),
+                 "ms_per_iter": round(1e3 * t_cg / max(1, rep["iters"]), 3),
+                 "backproj_s": round(t_bp, 4), "res": rep["res"]}
+    # Split-Bregman (configuration 3)
+    ph = synth.ellipsoid_phantom(dims, 12, synth.Stream((3 << 16) + 1), piecewise=True)
+    pr = lc.forward(ph, grid, ctx=ctx).astype(np.float64)
+    s = 2.0 / max(float(np.max(pr)), 1e-12)
+    counts = np.random.default_rng((3 << 16) + 2).poisson(1e4 * np.exp(-s * np.maximum(pr, 0.0)))
+    yp = (-np.log(np.maximum(counts, 1) / 1e4) / s).astype(np.float32)
+    y = lc.backproj(yp, grid, dims, ctx=ctx)
+    b.set(0, b.Y, y)
+    b.split_bregman(0.1, 10.0, 2, 1, 0.0, 0.0)
+    ctx.sync()
+    t0 = time.perf_counter()
+    b.split_bregman(0.1, 10.0, args.lamino_iters, 1, 0.0, 0.0)
+    ctx.sync()
+    t_sb = time.perf_counter() - t0
+    rep = b.reports()[0]
+    out["split_bregman"] = {"outer": int(rep["outer"]), "seconds": round(t_sb, 4),
+                            "ms_per_outer": round(1e3 * t_sb / max(1, rep["outer"]), 3)}
+    out["workload"] = ("3-D laminography at the padded configuration shapes; CG 100 iterations "
+                       "(1% Gaussian noise) and Split-Bregman TV 100 outer x 1 inner (Poisson "
+                       "counts, lambda 0.1, mu 10, isotropic, interior-only differences); "
+                       "synthetic phantoms, inputs resident")
+    return out
 
 
 def pmc_probe(args):
@@ -271,7 +359,13 @@ def main():
     ap.add_argument("--serve-lanes", type=int, default=3)
     ap.add_argument("--serve-budget-gb", type=float, default=96.0)
     ap.add_argument("--sb-outer", type=int, default=20)
+    ap.add_argument("--serve-plain-sample", type=int, default=60,
+                    help="served stream: requests re-solved one at a time for the plain rate")
     ap.add_argument("--no-serve", action="store_true", help="skip the served stream")
+    ap.add_argument("--no-lamino", action="store_true", help="skip the 3-D laminography solves")
+    ap.add_argument("--lamino-iters", type=int, default=100)
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run only the served stream and print its JSON (profiling)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only run a calibration copy and a few Toeplitz applies (for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -288,6 +382,11 @@ def main():
 
     import laminocam_amd as lc
     from laminocam_amd import synth
+
+    if args.serve_only:
+        served = served_stream(args, lc, synth, lc.default_context())
+        print(json.dumps({"served": served}), flush=True)
+        return
 
     t_setup = time.perf_counter()
     ctx = lc.default_context()
@@ -401,6 +500,14 @@ def main():
             f"p50 {served['latency_ms']['p50']} ms p99 {served['latency_ms']['p99']} ms, "
             f"{served['batches']} batches")
 
+    # ---- 3-D laminography solves ----
+    lamino = None
+    if not args.no_lamino:
+        lamino = lamino_leg(args, lc, synth, ctx)
+        log(f"lamino 3-D: CG {lamino['cg']['iters']} it {lamino['cg']['seconds']} s, SB "
+            f"{lamino['split_bregman']['outer']} outer {lamino['split_bregman']['seconds']} s, "
+            f"PSF {lamino['psf_build_s']} s")
+
     # ---- CPU baseline ----
     cpu = None
     if not args.no_cpu:
@@ -454,6 +561,7 @@ def main():
                                     "by 1e-15 relative; long CG solves of these under-sampled "
                                     "systems drift by this much from rounding alone"},
         "served": served,
+        "lamino3d": lamino,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernel_passes": [{"N": r["N"], "pass": PASS_NAMES[r["pass_"]], "ms": round(r["ms"], 4),

@@ -218,7 +218,8 @@ int lc_server_wait(lc_server s, uint64_t ticket, double timeout_s, int32_t *done
 /* copies the solution into x (problem size) and releases the ticket;
  * times[3] = submit, start, done (seconds, steady clock) */
 int lc_server_result(lc_server s, uint64_t ticket, float *x, lc_report *rep, double *times);
-/* stats[6] = in-flight bytes, queued, running, finished, batches, budget */
+/* stats[10] = in-flight bytes, queued, running, finished, batches, budget, and
+ * summed lane seconds building batches, loading inputs, solving, reading back */
 int lc_server_stats(lc_server s, double *stats);
 /* finishes every queued request, then stops the lanes */
 int lc_server_destroy(lc_server s);

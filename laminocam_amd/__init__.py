@@ -659,10 +659,11 @@ class Server:
         return x, rep.as_dict(), tuple(times)
 
     def stats(self):
-        s = (C.c_double * 6)()
+        s = (C.c_double * 10)()
         _lib.check(self.lib.lc_server_stats(self.handle, s))
         return dict(inflight_bytes=s[0], queued=s[1], running=s[2], finished=s[3],
-                    batches=s[4], budget=s[5])
+                    batches=s[4], budget=s[5], lane_s_build=s[6], lane_s_input=s[7],
+                    lane_s_solve=s[8], lane_s_output=s[9])
 
     def close(self):
         if self.handle:

@@ -75,7 +75,7 @@ struct DetLine {
     FftPlan plan{};
     DBuf chirp, bhat;
 };
-void det_line_init(DetLine &d, int n);
+void det_line_init(Ctx &c, DetLine &d, int n);
 
 // NUFFT state of one 3-D grid: fine grid (z, y, x), per-axis corrections
 struct Nufft3D {

@@ -446,7 +446,10 @@ void nufft2d_init(Ctx &c, Geom2D &g) {
         corr[a] = (float)(sgn / ph);
     }
     u.corr.alloc(sizeof(float) * N);
-    LC_HIP(hipMemcpy(u.corr.as(), corr.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+    // stream-ordered copy: a legacy-stream copy would have to wait on other
+    // threads' streams, some of which may be capturing graphs
+    LC_HIP(hipMemcpyAsync(u.corr.as(), corr.data(), sizeof(float) * N, hipMemcpyHostToDevice, c.stream()));
+    c.sync();
     LC_REQUIRE(fft_make_plan(u.nf, u.plan_nf), "nufft: fine grid size not 2^p 3^q");
     u.det_fast = fft_make_plan((int)g.nrad, u.plan_det);
     u.ready = true;

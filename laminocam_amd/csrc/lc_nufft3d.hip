@@ -524,7 +524,7 @@ double es_phihat3(double beta, double xi) {
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-void det_line_init(DetLine &d, int n) {
+void det_line_init(Ctx &c, DetLine &d, int n) {
     d.n = n;
     if (n <= LN_NT * LN_ME && fft_make_plan(n, d.plan)) {
         d.blue = false;
@@ -568,8 +568,9 @@ void det_line_init(DetLine &d, int n) {
     }
     d.chirp.alloc(sizeof(float2) * n);
     d.bhat.alloc(sizeof(float2) * 2 * L);
-    LC_HIP(hipMemcpy(d.chirp.as(), ch.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
-    LC_HIP(hipMemcpy(d.bhat.as(), bh.data(), sizeof(float2) * 2 * L, hipMemcpyHostToDevice));
+    LC_HIP(hipMemcpyAsync(d.chirp.as(), ch.data(), sizeof(float2) * n, hipMemcpyHostToDevice, c.stream()));
+    LC_HIP(hipMemcpyAsync(d.bhat.as(), bh.data(), sizeof(float2) * 2 * L, hipMemcpyHostToDevice, c.stream()));
+    c.sync();
 }
 
 void nufft3d_init(Ctx &c, Geom3D &g) {
@@ -590,9 +591,11 @@ void nufft3d_init(Ctx &c, Geom3D &g) {
         }
     }
     u.corr.alloc(sizeof(float) * corr.size());
-    LC_HIP(hipMemcpy(u.corr.as(), corr.data(), sizeof(float) * corr.size(), hipMemcpyHostToDevice));
-    det_line_init(u.det_x, (int)g.ncols);
-    det_line_init(u.det_y, (int)g.nrows);
+    LC_HIP(hipMemcpyAsync(u.corr.as(), corr.data(), sizeof(float) * corr.size(), hipMemcpyHostToDevice,
+                          c.stream()));
+    c.sync();
+    det_line_init(c, u.det_x, (int)g.ncols);
+    det_line_init(c, u.det_y, (int)g.nrows);
     u.ready = true;
 }
 

@@ -11,6 +11,57 @@
 
 namespace lc {
 
+// Cache of device blocks for one worker thread.  hipFree synchronises the
+// whole device, so a thread whose buffers come and go (a serving lane, one
+// batch after another) keeps them here and reuses them; blocks are sized in
+// bins of 1, 1.25, 1.5, 1.75 x 2^k (at least 64 KiB) and freed only when the
+// pool is destroyed.
+class MemPool {
+  public:
+    MemPool() = default;
+    MemPool(const MemPool &) = delete;
+    MemPool &operator=(const MemPool &) = delete;
+    ~MemPool() {
+        for (auto &kv : free_) (void)hipFree(kv.second);
+    }
+    static size_t bin(size_t bytes) {
+        size_t b = 64 << 10;
+        while (b < bytes) b <<= 1;
+        if (b <= (64 << 10)) return b;
+        const size_t q = b >> 3;   // candidates 4q, 5q, 6q, 7q, 8q
+        for (size_t m = 4; m <= 8; ++m)
+            if (m * q >= bytes) return m * q;
+        return b;
+    }
+    void *get(size_t bytes, size_t &got) {
+        got = bin(bytes);
+        std::lock_guard<std::mutex> lk(m_);
+        auto it = free_.find(got);
+        if (it != free_.end()) {
+            void *p = it->second;
+            free_.erase(it);
+            return p;
+        }
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, got);
+        if (e != hipSuccess)
+            throw Error(LC_ENOMEM, "hipMalloc of " + std::to_string(got) + " bytes failed: " +
+                                       hipGetErrorString(e));
+        return p;
+    }
+    void put(void *p, size_t got) {
+        std::lock_guard<std::mutex> lk(m_);
+        free_.emplace(got, p);
+    }
+
+  private:
+    std::mutex m_;
+    std::multimap<size_t, void *> free_;
+};
+
+// allocations made by this thread go through this pool when set
+inline thread_local MemPool *tl_pool = nullptr;
+
 // RAII device allocation
 class DBuf {
   public:
@@ -19,37 +70,54 @@ class DBuf {
     ~DBuf() { release(); }
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
-    DBuf(DBuf &&o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DBuf(DBuf &&o) noexcept : p_(o.p_), n_(o.n_), cap_(o.cap_), pool_(o.pool_) {
+        o.p_ = nullptr; o.n_ = 0; o.cap_ = 0; o.pool_ = nullptr;
+    }
     DBuf &operator=(DBuf &&o) noexcept {
-        if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+        if (this != &o) {
+            release();
+            p_ = o.p_; n_ = o.n_; cap_ = o.cap_; pool_ = o.pool_;
+            o.p_ = nullptr; o.n_ = 0; o.cap_ = 0; o.pool_ = nullptr;
+        }
         return *this;
     }
     void alloc(size_t bytes) {
         release();
         if (bytes == 0) return;
+        if (tl_pool) {
+            p_ = tl_pool->get(bytes, cap_);
+            pool_ = tl_pool;
+            n_ = bytes;
+            return;
+        }
         hipError_t e = hipMalloc(&p_, bytes);
         if (e != hipSuccess) {
             p_ = nullptr;
             throw Error(LC_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " +
                                        hipGetErrorString(e));
         }
-        n_ = bytes;
+        n_ = cap_ = bytes;
     }
     // grow-only reallocation (contents not preserved)
     void reserve(size_t bytes) {
         if (bytes > n_) alloc(bytes);
     }
     void release() {
-        if (p_) (void)hipFree(p_);
+        if (p_) {
+            if (pool_) pool_->put(p_, cap_);
+            else (void)hipFree(p_);
+        }
         p_ = nullptr;
-        n_ = 0;
+        n_ = cap_ = 0;
+        pool_ = nullptr;
     }
     template <typename T = void> T *as() const { return reinterpret_cast<T *>(p_); }
     size_t bytes() const { return n_; }
 
   private:
     void *p_ = nullptr;
-    size_t n_ = 0;
+    size_t n_ = 0, cap_ = 0;
+    MemPool *pool_ = nullptr;
 };
 
 // Block shapes of the three Toeplitz passes for one image size:

@@ -150,6 +150,10 @@ void Server::stats(double *out) {
     out[3] = (double)done_;
     out[4] = (double)batches_;
     out[5] = (double)budget_;
+    out[6] = t_build_;
+    out[7] = t_input_;
+    out[8] = t_solve_;
+    out[9] = t_output_;
 }
 
 // requests that can share one batch solve
@@ -163,6 +167,7 @@ static bool same_group(const ServerRequest &a, const ServerRequest &b) {
 
 void Server::lane_loop(Lane &lane) {
     Ctx &c = *lane.ctx;
+    tl_pool = &lane.pool;   // this lane's batches reuse device blocks instead of hipFree
     for (;;) {
         std::vector<ServerRequest *> batch;
         {
@@ -224,22 +229,31 @@ void Server::lane_loop(Lane &lane) {
 void Server::run_batch(Ctx &c, std::vector<ServerRequest *> &batch) {
     std::lock_guard<std::mutex> lk(c.mu);
     c.set_current();
+    const double t0 = now_s();
     std::vector<ProbSpec> ps;
     for (ServerRequest *r : batch)
         ps.push_back(r->g3 ? ProbSpec{nullptr, (int)r->g3->n1, r->g3} : ProbSpec{r->g2, r->ns});
     Batch2D b(c, ps);
-    // inputs: right-hand sides (or sinograms back-projected into y), initial guesses
+    const double t1 = now_s();
+    // inputs: right-hand sides (or sinograms back-projected into y), initial
+    // guesses.  All sinograms of the batch share one staging buffer, so the
+    // uploads and back-projections queue on the stream without host waits.
+    size_t sino_total = 0;
+    for (ServerRequest *r : batch)
+        if (r->has_sino) sino_total += (r->sino.size() + 63) & ~(size_t)63;
     DBuf sino;
+    if (sino_total) sino.alloc(sizeof(float) * sino_total);
+    size_t soff = 0;
     bool any_x0 = false;
     for (size_t i = 0; i < batch.size(); ++i) {
         ServerRequest *r = batch[i];
         float *y = b.Y() + b.offset((int)i);
         if (r->has_sino) {
-            sino.reserve(sizeof(float) * r->sino.size());
-            copy_in(c, sino.as(), r->sino.data(), sizeof(float) * r->sino.size(), false);
-            if (r->g3) backproj3d_dev(c, *r->g3, sino.as<float>(), y, nullptr);
-            else backproj2d_dev(c, *r->g2, sino.as<float>(), y, r->ns);
-            c.sync();   // the staging buffer is reused by the next request
+            float *s = sino.as<float>() + soff;
+            soff += (r->sino.size() + 63) & ~(size_t)63;
+            copy_in(c, s, r->sino.data(), sizeof(float) * r->sino.size(), false);
+            if (r->g3) backproj3d_dev(c, *r->g3, s, y, nullptr);
+            else backproj2d_dev(c, *r->g2, s, y, r->ns);
         } else {
             copy_in(c, y, r->rhs.data(), sizeof(float) * r->rhs.size(), false);
         }
@@ -251,6 +265,8 @@ void Server::run_batch(Ctx &c, std::vector<ServerRequest *> &batch) {
         if (!r->x0.empty()) copy_in(c, x, r->x0.data(), sizeof(float) * r->x0.size(), false);
         else LC_HIP(hipMemsetAsync(x, 0, sizeof(float) * (size_t)b.size((int)i), c.stream()));
     }
+    c.sync();
+    const double t2 = now_s();
     const ServerRequest &h = *batch.front();
     if (h.solver == 0) {
         std::vector<int> mi;
@@ -265,6 +281,8 @@ void Server::run_batch(Ctx &c, std::vector<ServerRequest *> &batch) {
         b.split_bregman(la, mu, h.outer, h.inner, h.tol, h.xtol, (h.flags & LC_TV_ZERO_PAD) ? 1 : 0,
                         (h.flags & LC_TV_ANISO) != 0, true, false);
     }
+    c.sync();
+    const double t3 = now_s();
     const auto cs = b.cg_state();
     const auto ss = h.solver == 1 ? b.sb_state() : std::vector<SbState>();
     for (size_t i = 0; i < batch.size(); ++i) {
@@ -279,6 +297,12 @@ void Server::run_batch(Ctx &c, std::vector<ServerRequest *> &batch) {
         r->rep.pad = 0;
     }
     c.sync();
+    const double t4 = now_s();
+    std::lock_guard<std::mutex> lk2(m_);
+    t_build_ += t1 - t0;
+    t_input_ += t2 - t1;
+    t_solve_ += t3 - t2;
+    t_output_ += t4 - t3;
 }
 
 } // namespace lc

@@ -48,11 +48,14 @@ class Server {
     bool poll(uint64_t id);
     bool wait(uint64_t id, double timeout_s);   // < 0: no timeout
     std::unique_ptr<ServerRequest> take(uint64_t id);
-    // in-flight bytes, queued, running, finished, batches, budget
+    // in-flight bytes, queued, running, finished, batches, budget, then the
+    // summed lane seconds spent building batches, loading inputs (incl.
+    // back-projection), solving, and reading results back
     void stats(double *out);
 
   private:
     struct Lane {
+        MemPool pool;                // outlives the context's buffers
         std::unique_ptr<Ctx> ctx;
         std::thread th;
     };
@@ -69,6 +72,7 @@ class Server {
     std::map<uint64_t, std::unique_ptr<ServerRequest>> reqs_;
     uint64_t next_id_ = 0;
     size_t inflight_ = 0, queued_ = 0, running_ = 0, done_ = 0, batches_ = 0;
+    double t_build_ = 0, t_input_ = 0, t_solve_ = 0, t_output_ = 0;
     bool stop_ = false;
 };
 
