@@ -4,6 +4,9 @@
  * Plain C: opaque handles, pointers and sizes, int status codes (0 = OK) and a
  * thread-local message from lc_last_error().  No C++ exception crosses this
  * boundary.  Arrays are row major float32; complex arrays interleave (re, im).
+ * Entry points ending in _f64 take float64 arrays and run the whole operator
+ * or solve in FP64 (vectors, PSF spectrum, FFTs and twiddles), standing in for
+ * the reference's Array<double> instantiations.
  * Unless LC_DEVICE_PTRS is set, array arguments are host pointers and every
  * call completes before it returns.
  *
@@ -38,6 +41,7 @@ extern "C" {
 #define LC_TV_ANISO 0x10u    /* per-component soft threshold instead of isotropic */
 #define LC_GEOM_LAZY_PSF 0x20u /* lc_geom2d_create: build the PSF on first use
                                  (geometries used only for projection) */
+#define LC_FP64 0x40u        /* lc_batch_create_ex: FP64 vectors, spectra and FFTs */
 
 typedef struct lc_ctx_s *lc_ctx;
 typedef struct lc_geom2d_s *lc_geom2d;
@@ -104,6 +108,16 @@ int lc_cg2d(lc_ctx ctx, lc_geom2d g, const float *rhs, float *x, int64_t nslices
 int lc_split_bregman2d(lc_ctx ctx, lc_geom2d g, const float *rhs, float *x, int64_t nslices,
                        double lambda, double mu, int64_t outer, int64_t inner, double tol,
                        double xtol, uint32_t flags, lc_report *rep);
+/* FP64 forms: sysmat2d<double>(x, PointSpreadFunction2D<double>) (src/gradient.cpp:82-83),
+ * cgsolver<double> (src/conjgrad.cpp:82-85), split_bregman<double> (src/bregman.cpp:101-104) */
+int lc_sysmat2d_psf_f64(lc_ctx ctx, lc_geom2d g, const double *x, int64_t nslices, double *y,
+                        uint32_t flags);
+int lc_cg2d_f64(lc_ctx ctx, lc_geom2d g, const double *rhs, double *x, int64_t nslices,
+                int64_t max_iter, double tol, double xtol, uint32_t flags, lc_report *rep);
+int lc_split_bregman2d_f64(lc_ctx ctx, lc_geom2d g, const double *rhs, double *x,
+                           int64_t nslices, double lambda, double mu, int64_t outer,
+                           int64_t inner, double tol, double xtol, uint32_t flags,
+                           lc_report *rep);
 
 /* ---- 3-D laminography geometry + Toeplitz PSF ----
  * Replaces cpu::PolarGrid(theta, gamma, beta, nrows, ncols)
@@ -142,6 +156,14 @@ int lc_cg3d(lc_ctx ctx, lc_geom3d g, const float *rhs, float *x, int64_t max_ite
 int lc_split_bregman3d(lc_ctx ctx, lc_geom3d g, const float *rhs, float *x, double lambda,
                        double mu, int64_t outer, int64_t inner, double tol, double xtol,
                        uint32_t flags, lc_report *rep);
+/* FP64 forms: sysmat<double>(x, PointSpreadFunction<double>) (src/gradient.cpp:55-56),
+ * cgsolver<double>, split_bregman<double> */
+int lc_sysmat3d_psf_f64(lc_ctx ctx, lc_geom3d g, const double *x, double *y, uint32_t flags);
+int lc_cg3d_f64(lc_ctx ctx, lc_geom3d g, const double *rhs, double *x, int64_t max_iter,
+                double tol, double xtol, uint32_t flags, lc_report *rep);
+int lc_split_bregman3d_f64(lc_ctx ctx, lc_geom3d g, const double *rhs, double *x, double lambda,
+                           double mu, int64_t outer, int64_t inner, double tol, double xtol,
+                           uint32_t flags, lc_report *rep);
 
 /* ---- finite differences on an n1 x n2 x n3 volume ----
  * opt::grad_u / divergence / laplacian (src/finitediff.cpp:26-90,
@@ -153,6 +175,13 @@ int lc_divergence(lc_ctx ctx, const float *g, int64_t n1, int64_t n2, int64_t n3
                   uint32_t flags);
 int lc_laplacian(lc_ctx ctx, const float *u, int64_t n1, int64_t n2, int64_t n3, float *out,
                  uint32_t flags);
+/* grad_u<double> / divergence<double> (src/finitediff.cpp:69, 93) and the laplacian */
+int lc_grad_f64(lc_ctx ctx, const double *u, int64_t n1, int64_t n2, int64_t n3, double *g,
+                uint32_t flags);
+int lc_divergence_f64(lc_ctx ctx, const double *g, int64_t n1, int64_t n2, int64_t n3,
+                      double *out, uint32_t flags);
+int lc_laplacian_f64(lc_ctx ctx, const double *u, int64_t n1, int64_t n2, int64_t n3, double *out,
+                     uint32_t flags);
 
 /* ---- batches of independent 2-D problems (per-problem alpha/beta) ----
  * No reference counterpart: gpu::MBIR2D (include/gpu/tomocam.h:60-75) solves
@@ -164,12 +193,20 @@ int lc_batch2d_create(lc_ctx ctx, int32_t nprob, const lc_geom2d *geoms, const i
  * exactly one of g2[i], g3[i] is non-NULL (either array may be NULL) */
 int lc_batch_create(lc_ctx ctx, int32_t nprob, const lc_geom2d *g2, const lc_geom3d *g3,
                     const int32_t *ns, lc_batch2d *out);
+/* flags: LC_FP64 makes an FP64 batch, whose buffers are read and written with
+ * the _f64 accessors below (the float accessors refuse it, and vice versa) */
+int lc_batch_create_ex(lc_ctx ctx, int32_t nprob, const lc_geom2d *g2, const lc_geom3d *g3,
+                       const int32_t *ns, uint32_t flags, lc_batch2d *out);
 int lc_batch2d_destroy(lc_batch2d b);
 /* which: 0 = x (solution / initial guess), 1 = y (right-hand side A^T y) */
 int lc_batch2d_set(lc_batch2d b, int32_t prob, int32_t which, const float *src, uint32_t flags);
 int lc_batch2d_get(lc_batch2d b, int32_t prob, int32_t which, float *dst, uint32_t flags);
 /* device address of a problem's slot in buffer `which` */
 int lc_batch2d_device_ptr(lc_batch2d b, int32_t prob, int32_t which, float **out);
+int lc_batch2d_set_f64(lc_batch2d b, int32_t prob, int32_t which, const double *src,
+                       uint32_t flags);
+int lc_batch2d_get_f64(lc_batch2d b, int32_t prob, int32_t which, double *dst, uint32_t flags);
+int lc_batch2d_device_ptr_f64(lc_batch2d b, int32_t prob, int32_t which, double **out);
 int lc_batch2d_cg(lc_batch2d b, const int32_t *max_iter, double tol, double xtol,
                   uint32_t flags);
 int lc_batch2d_split_bregman(lc_batch2d b, const float *lambda, const float *mu, int64_t outer,

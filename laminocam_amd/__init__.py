@@ -22,8 +22,8 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import (LC_DEVICE_PTRS, LC_GRAPH, LC_TV_ANISO, LC_TV_ZERO_PAD, LC_X0_ZERO,
-                   LcError, LcReport)
+from ._lib import (LC_DEVICE_PTRS, LC_FP64, LC_GRAPH, LC_TV_ANISO, LC_TV_ZERO_PAD,
+                   LC_X0_ZERO, LcError, LcReport)
 
 __all__ = ["Context", "PolarGrid2D", "PointSpreadFunction2D", "PolarGrid", "PointSpreadFunction",
            "sysmat", "sysmat2d", "forward", "backproj", "forward2d",
@@ -36,6 +36,21 @@ _default_ctx = None
 
 def _f32(a):
     return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _real(a, dtype=None):
+    """Contiguous float array for an operator or solver call: float64 input
+    (or dtype=float64) selects the FP64 path, like the reference's
+    Array<double> instantiations; every other input runs in FP32."""
+    a = np.asarray(a)
+    if dtype is None:
+        dtype = np.float64 if a.dtype == np.float64 else np.float32
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def _entry(lib, name, a):
+    """the C entry point for the element type of a (name or name_f64)"""
+    return getattr(lib, name + "_f64" if a.dtype == np.float64 else name)
 
 
 def _ptr(a):
@@ -218,14 +233,15 @@ class PointSpreadFunction2D:
         return out
 
     def convolve(self, x):
-        x = _f32(x)
+        """Toeplitz apply; float64 x runs the FP64 operator"""
+        x = _real(x)
         N = self.image_size
         if x.shape[-1] != N or x.shape[-2] != N:
             raise ValueError(f"convolve: slices must be {N}x{N}, got {x.shape}")
         ns = x.size // (N * N)
         out = np.empty_like(x)
-        _lib.check(self.lib.lc_sysmat2d_psf(self.ctx.handle, self.handle, _ptr(x), ns,
-                                            _ptr(out), 0))
+        _lib.check(_entry(self.lib, "lc_sysmat2d_psf", x)(self.ctx.handle, self.handle, _ptr(x),
+                                                           ns, _ptr(out), 0))
         return out
 
     def close(self):
@@ -293,11 +309,13 @@ class PointSpreadFunction:
         return out
 
     def convolve(self, x):
-        x = _f32(x)
+        """Toeplitz apply; float64 x runs the FP64 operator"""
+        x = _real(x)
         if x.shape != self.recon_dims:
             raise ValueError(f"convolve: volume must be {self.recon_dims}, got {x.shape}")
         out = np.empty_like(x)
-        _lib.check(self.lib.lc_sysmat3d_psf(self.ctx.handle, self.handle, _ptr(x), _ptr(out), 0))
+        _lib.check(_entry(self.lib, "lc_sysmat3d_psf", x)(self.ctx.handle, self.handle, _ptr(x),
+                                                           _ptr(out), 0))
         return out
 
     def close(self):
@@ -386,30 +404,33 @@ def sysmat2d(x, op, ctx: "Context | None" = None):
 
 
 def _stack(x, N):
-    x = _f32(x)
+    x = _real(x)
     return x, x.size // (N * N)
 
 
 def cgsolver(psf, y, x0, max_iter: int, tol: float, xtol: float, graph: bool = False):
     """opt::cgsolver with A = sysmat2d(., psf) on one stack (global alpha/beta
     over every slice, as the reference) or A = sysmat(., psf) on one 3-D
-    volume.  Returns (x, report dict)."""
+    volume.  float64 y runs cgsolver<double> (FP64 vectors, spectrum and
+    FFTs).  Returns (x, report dict)."""
     rep = LcReport()
     flags = LC_GRAPH if graph else 0
     if isinstance(psf, PointSpreadFunction):
-        y = _f32(y)
-        x = _f32(x0).reshape(y.shape).copy()
+        y = _real(y)
+        x = _real(x0, y.dtype).reshape(y.shape).copy()
         if not np.any(x):
             flags |= LC_X0_ZERO
-        _lib.check(psf.lib.lc_cg3d(psf.ctx.handle, psf.handle, _ptr(y), _ptr(x), int(max_iter),
-                                   float(tol), float(xtol), flags, C.byref(rep)))
+        _lib.check(_entry(psf.lib, "lc_cg3d", y)(psf.ctx.handle, psf.handle, _ptr(y), _ptr(x),
+                                                 int(max_iter), float(tol), float(xtol), flags,
+                                                 C.byref(rep)))
         return x, rep.as_dict()
     y, ns = _stack(y, psf.image_size)
-    x = _f32(x0).copy()
+    x = _real(x0, y.dtype).copy()
     if not np.any(x):
         flags |= LC_X0_ZERO
-    _lib.check(psf.lib.lc_cg2d(psf.ctx.handle, psf.handle, _ptr(y), _ptr(x), ns, int(max_iter),
-                               float(tol), float(xtol), flags, C.byref(rep)))
+    _lib.check(_entry(psf.lib, "lc_cg2d", y)(psf.ctx.handle, psf.handle, _ptr(y), _ptr(x), ns,
+                                             int(max_iter), float(tol), float(xtol), flags,
+                                             C.byref(rep)))
     return x, rep.as_dict()
 
 
@@ -418,28 +439,28 @@ def split_bregman(psf, y, x0, lam: float, mu: float, outer_max: int,
                   aniso: bool = False, graph: bool = False):
     """opt::split_bregman with A = sysmat2d(., psf).  zero_pad selects the
     zero-padded finite differences of the reference's GPU path instead of
-    the interior-only ones of its CPU path."""
+    the interior-only ones of its CPU path.  float64 y runs
+    split_bregman<double>."""
     rep = LcReport()
     flags = (LC_GRAPH if graph else 0) | (LC_TV_ZERO_PAD if zero_pad else 0) | \
         (LC_TV_ANISO if aniso else 0)
     if isinstance(psf, PointSpreadFunction):
-        y = _f32(y)
-        x = _f32(x0).reshape(y.shape).copy()
-        _lib.check(psf.lib.lc_split_bregman3d(psf.ctx.handle, psf.handle, _ptr(y), _ptr(x),
-                                              float(lam), float(mu), int(outer_max),
-                                              int(inner_max), float(tol), float(xtol), flags,
-                                              C.byref(rep)))
+        y = _real(y)
+        x = _real(x0, y.dtype).reshape(y.shape).copy()
+        _lib.check(_entry(psf.lib, "lc_split_bregman3d", y)(
+            psf.ctx.handle, psf.handle, _ptr(y), _ptr(x), float(lam), float(mu), int(outer_max),
+            int(inner_max), float(tol), float(xtol), flags, C.byref(rep)))
         return x, rep.as_dict()
     y, ns = _stack(y, psf.image_size)
-    x = _f32(x0).copy()
-    _lib.check(psf.lib.lc_split_bregman2d(psf.ctx.handle, psf.handle, _ptr(y), _ptr(x), ns,
-                                          float(lam), float(mu), int(outer_max), int(inner_max),
-                                          float(tol), float(xtol), flags, C.byref(rep)))
+    x = _real(x0, y.dtype).copy()
+    _lib.check(_entry(psf.lib, "lc_split_bregman2d", y)(
+        psf.ctx.handle, psf.handle, _ptr(y), _ptr(x), ns, float(lam), float(mu), int(outer_max),
+        int(inner_max), float(tol), float(xtol), flags, C.byref(rep)))
     return x, rep.as_dict()
 
 
 def _vol3(u):
-    u = _f32(u)
+    u = _real(u)
     if u.ndim == 2:
         return u, (1,) + u.shape
     if u.ndim != 3:
@@ -450,29 +471,30 @@ def _vol3(u):
 def grad_u(u, zero_pad: bool = False, ctx: Context | None = None):
     ctx = ctx or default_context()
     u, d = _vol3(u)
-    g = np.empty((3,) + d, np.float32)
-    _lib.check(ctx.lib.lc_grad(ctx.handle, _ptr(u), *d, _ptr(g), LC_TV_ZERO_PAD if zero_pad else 0))
+    g = np.empty((3,) + d, u.dtype)
+    _lib.check(_entry(ctx.lib, "lc_grad", u)(ctx.handle, _ptr(u), *d, _ptr(g),
+                                             LC_TV_ZERO_PAD if zero_pad else 0))
     return g
 
 
 def divergence(g, zero_pad: bool = False, ctx: Context | None = None):
     ctx = ctx or default_context()
-    g = _f32(g)
+    g = _real(g)
     d = g.shape[1:]
     if len(d) == 2:
         d = (1,) + d
-    out = np.empty(d, np.float32)
-    _lib.check(ctx.lib.lc_divergence(ctx.handle, _ptr(g), *d, _ptr(out),
-                                     LC_TV_ZERO_PAD if zero_pad else 0))
+    out = np.empty(d, g.dtype)
+    _lib.check(_entry(ctx.lib, "lc_divergence", g)(ctx.handle, _ptr(g), *d, _ptr(out),
+                                                   LC_TV_ZERO_PAD if zero_pad else 0))
     return out
 
 
 def laplacian(u, zero_pad: bool = False, ctx: Context | None = None):
     ctx = ctx or default_context()
     u, d = _vol3(u)
-    out = np.empty(d, np.float32)
-    _lib.check(ctx.lib.lc_laplacian(ctx.handle, _ptr(u), *d, _ptr(out),
-                                    LC_TV_ZERO_PAD if zero_pad else 0))
+    out = np.empty(d, u.dtype)
+    _lib.check(_entry(ctx.lib, "lc_laplacian", u)(ctx.handle, _ptr(u), *d, _ptr(out),
+                                                  LC_TV_ZERO_PAD if zero_pad else 0))
     return out
 
 
@@ -482,9 +504,10 @@ class Batch2D:
 
     X, Y, AX = 0, 1, 2
 
-    def __init__(self, psfs, ns=None, ctx: Context | None = None):
+    def __init__(self, psfs, ns=None, ctx: Context | None = None, fp64: bool = False):
         """psfs: PointSpreadFunction2D (a stack of ns[i] slices) or
-        PointSpreadFunction (one 3-D volume) per problem."""
+        PointSpreadFunction (one 3-D volume) per problem.  fp64: FP64
+        vectors, spectra and FFTs (set/get then take float64 arrays)."""
         self.ctx = ctx or psfs[0].ctx
         self.lib = self.ctx.lib
         self.psfs = list(psfs)
@@ -498,7 +521,10 @@ class Batch2D:
         g3 = (C.c_void_p * n)(*[p.handle.value if is3[i] else None for i, p in enumerate(self.psfs)])
         nss = (C.c_int32 * n)(*self.ns)
         h = C.c_void_p()
-        _lib.check(self.lib.lc_batch_create(self.ctx.handle, n, g2, g3, nss, C.byref(h)))
+        self.fp64 = bool(fp64)
+        self.dtype = np.float64 if self.fp64 else np.float32
+        _lib.check(self.lib.lc_batch_create_ex(self.ctx.handle, n, g2, g3, nss,
+                                               LC_FP64 if self.fp64 else 0, C.byref(h)))
         self.handle = h
 
     def __len__(self):
@@ -511,14 +537,14 @@ class Batch2D:
         return (self.ns[i], p.image_size, p.image_size)
 
     def set(self, i, which, arr):
-        a = _f32(arr)
+        a = _real(arr, self.dtype)
         if a.size != int(np.prod(self.shape(i))):
             raise ValueError("size mismatch")
-        _lib.check(self.lib.lc_batch2d_set(self.handle, i, which, _ptr(a), 0))
+        _lib.check(_entry(self.lib, "lc_batch2d_set", a)(self.handle, i, which, _ptr(a), 0))
 
     def get(self, i, which):
-        out = np.empty(self.shape(i), np.float32)
-        _lib.check(self.lib.lc_batch2d_get(self.handle, i, which, _ptr(out), 0))
+        out = np.empty(self.shape(i), self.dtype)
+        _lib.check(_entry(self.lib, "lc_batch2d_get", out)(self.handle, i, which, _ptr(out), 0))
         return out
 
     def sysmat(self, src=0, dst=2):
@@ -567,7 +593,8 @@ class Batch2D:
 
     def device_ptr(self, i, which):
         p = C.c_void_p()
-        _lib.check(self.lib.lc_batch2d_device_ptr(self.handle, i, which, C.byref(p)))
+        fn = self.lib.lc_batch2d_device_ptr_f64 if self.fp64 else self.lib.lc_batch2d_device_ptr
+        _lib.check(fn(self.handle, i, which, C.byref(p)))
         return p.value
 
     def close(self):

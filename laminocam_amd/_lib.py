@@ -16,6 +16,7 @@ LC_X0_ZERO = 0x4
 LC_TV_ZERO_PAD = 0x8
 LC_TV_ANISO = 0x10
 LC_GEOM_LAZY_PSF = 0x20
+LC_FP64 = 0x40
 
 # every symbol declared in include/laminocam.h
 EXPORTS = [
@@ -30,6 +31,10 @@ EXPORTS = [
     "lc_server_result", "lc_server_stats", "lc_server_destroy",
     "lc_sysmat3d_psf", "lc_forward3d", "lc_backproj3d", "lc_sysmat3d_nufft", "lc_cg3d", "lc_split_bregman3d", "lc_batch_create", "lc_autotune2d", "lc_set_tuning2d",
     "lc_get_tuning2d",
+    "lc_sysmat2d_psf_f64", "lc_cg2d_f64", "lc_split_bregman2d_f64", "lc_sysmat3d_psf_f64",
+    "lc_cg3d_f64", "lc_split_bregman3d_f64", "lc_grad_f64", "lc_divergence_f64",
+    "lc_laplacian_f64", "lc_batch_create_ex", "lc_batch2d_set_f64", "lc_batch2d_get_f64",
+    "lc_batch2d_device_ptr_f64",
 ]
 
 
@@ -132,6 +137,25 @@ def load():
         "lc_autotune2d": (C.c_int, [vp, vp, i32, C.POINTER(i32), C.POINTER(C.c_double)]),
         "lc_set_tuning2d": (C.c_int, [vp, i64, C.POINTER(i32)]),
         "lc_get_tuning2d": (C.c_int, [vp, i64, C.POINTER(i32)]),
+        "lc_sysmat2d_psf_f64": (C.c_int, [vp, vp, vp, i64, vp, u32]),
+        "lc_cg2d_f64": (C.c_int, [vp, vp, vp, vp, i64, i64, C.c_double, C.c_double, u32,
+                                  C.POINTER(LcReport)]),
+        "lc_split_bregman2d_f64": (C.c_int, [vp, vp, vp, vp, i64, C.c_double, C.c_double, i64,
+                                             i64, C.c_double, C.c_double, u32,
+                                             C.POINTER(LcReport)]),
+        "lc_sysmat3d_psf_f64": (C.c_int, [vp, vp, vp, vp, u32]),
+        "lc_cg3d_f64": (C.c_int, [vp, vp, vp, vp, i64, C.c_double, C.c_double, u32,
+                                  C.POINTER(LcReport)]),
+        "lc_split_bregman3d_f64": (C.c_int, [vp, vp, vp, vp, C.c_double, C.c_double, i64, i64,
+                                             C.c_double, C.c_double, u32, C.POINTER(LcReport)]),
+        "lc_grad_f64": (C.c_int, [vp, vp, i64, i64, i64, vp, u32]),
+        "lc_divergence_f64": (C.c_int, [vp, vp, i64, i64, i64, vp, u32]),
+        "lc_laplacian_f64": (C.c_int, [vp, vp, i64, i64, i64, vp, u32]),
+        "lc_batch_create_ex": (C.c_int, [vp, i32, C.POINTER(vp), C.POINTER(vp), C.POINTER(i32),
+                                         u32, C.POINTER(vp)]),
+        "lc_batch2d_set_f64": (C.c_int, [vp, i32, i32, vp, u32]),
+        "lc_batch2d_get_f64": (C.c_int, [vp, i32, i32, vp, u32]),
+        "lc_batch2d_device_ptr_f64": (C.c_int, [vp, i32, i32, C.POINTER(vp)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

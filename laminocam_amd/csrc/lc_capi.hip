@@ -53,13 +53,21 @@ struct lc_batch2d_s {
     std::vector<lc_report> reps;
 };
 
-static float *batch_buf(Batch2D &b, int which) {
+// device address of buffer `which` (element type of the batch)
+static void *batch_buf(Batch2D &b, int which) {
     switch (which) {
-    case 0: return b.X();
-    case 1: return b.Y();
-    case 2: return b.AP();
+    case 0: return b.Xv();
+    case 1: return b.Yv();
+    case 2: return b.APv();
     default: throw Error(LC_EINVAL, "batch buffer index must be 0 (x), 1 (y) or 2 (Ax)");
     }
+}
+
+// element type T of the caller's arrays must match the batch's
+template <typename T> static void batch_check_type(const Batch2D &b, const char *fn) {
+    if (b.f64() != (sizeof(T) == 8))
+        throw Error(LC_EINVAL, std::string(fn) + (b.f64() ? ": FP64 batch, use the _f64 entry point"
+                                                           : ": FP32 batch, use the float entry point"));
 }
 
 static void fill_report(const CgState &s, lc_report *r) {
@@ -146,49 +154,63 @@ int lc_psf2d_kernel_hat(lc_ctx ctx, lc_geom2d g, float *out) {
     });
 }
 
-// one-problem solves and applies, shared by the 2-D and 3-D entry points
-static void sysmat_psf(lc_ctx ctx, ProbSpec ps, const float *x, float *y, uint32_t flags) {
+// one-problem solves and applies, shared by the 2-D and 3-D entry points;
+// T = float runs the FP32 path, T = double the FP64 path
+} // extern "C"
+
+template <typename T>
+static void sysmat_psf(lc_ctx ctx, ProbSpec ps, const T *x, T *y, uint32_t flags) {
     std::lock_guard<std::mutex> lk(ctx->c.mu);
     ctx->c.set_current();
-    Batch2D b(ctx->c, {ps});
-    const size_t bytes = sizeof(float) * (size_t)b.size(0);   // exactly the problem's size
+    Batch2D b(ctx->c, {ps}, sizeof(T) == 8);
+    const size_t bytes = sizeof(T) * (size_t)b.size(0);   // exactly the problem's size
     const bool dev = flags & LC_DEVICE_PTRS;
-    copy_in(ctx->c, b.X(), x, bytes, dev);
-    b.toeplitz(b.X(), b.AP());
-    copy_out(ctx->c, y, b.AP(), bytes, dev);
+    copy_in(ctx->c, b.Xv(), x, bytes, dev);
+    b.toeplitz(b.Xv(), b.APv());
+    copy_out(ctx->c, y, b.APv(), bytes, dev);
     ctx->c.sync();
 }
 
-static void cg_one(lc_ctx ctx, ProbSpec ps, const float *rhs, float *x, int64_t max_iter,
-                   double tol, double xtol, uint32_t flags, lc_report *rep) {
+extern "C" {
+
+} // extern "C"
+
+template <typename T>
+static void cg_one(lc_ctx ctx, ProbSpec ps, const T *rhs, T *x, int64_t max_iter, double tol,
+                   double xtol, uint32_t flags, lc_report *rep) {
     std::lock_guard<std::mutex> lk(ctx->c.mu);
     ctx->c.set_current();
-    Batch2D b(ctx->c, {ps});
-    const size_t bytes = sizeof(float) * (size_t)b.size(0);
+    Batch2D b(ctx->c, {ps}, sizeof(T) == 8);
+    const size_t bytes = sizeof(T) * (size_t)b.size(0);
     const bool dev = flags & LC_DEVICE_PTRS;
-    copy_in(ctx->c, b.Y(), rhs, bytes, dev);
-    if (!(flags & LC_X0_ZERO)) copy_in(ctx->c, b.X(), x, bytes, dev);
+    copy_in(ctx->c, b.Yv(), rhs, bytes, dev);
+    if (!(flags & LC_X0_ZERO)) copy_in(ctx->c, b.Xv(), x, bytes, dev);
     const bool graph = flags & LC_GRAPH;
     b.cg({(int)max_iter}, tol, xtol, graph, !graph, (flags & LC_X0_ZERO) != 0);
-    copy_out(ctx->c, x, b.X(), bytes, dev);
+    copy_out(ctx->c, x, b.Xv(), bytes, dev);
     ctx->c.sync();
     if (rep) fill_report(b.cg_state()[0], rep);
 }
 
-static void sb_one(lc_ctx ctx, ProbSpec ps, const float *rhs, float *x, double lambda, double mu,
+extern "C" {
+
+} // extern "C"
+
+template <typename T>
+static void sb_one(lc_ctx ctx, ProbSpec ps, const T *rhs, T *x, double lambda, double mu,
                    int64_t outer, int64_t inner, double tol, double xtol, uint32_t flags,
                    lc_report *rep) {
     std::lock_guard<std::mutex> lk(ctx->c.mu);
     ctx->c.set_current();
-    Batch2D b(ctx->c, {ps});
-    const size_t bytes = sizeof(float) * (size_t)b.size(0);
+    Batch2D b(ctx->c, {ps}, sizeof(T) == 8);
+    const size_t bytes = sizeof(T) * (size_t)b.size(0);
     const bool dev = flags & LC_DEVICE_PTRS;
-    copy_in(ctx->c, b.Y(), rhs, bytes, dev);
-    copy_in(ctx->c, b.X(), x, bytes, dev);
+    copy_in(ctx->c, b.Yv(), rhs, bytes, dev);
+    copy_in(ctx->c, b.Xv(), x, bytes, dev);
     const bool graph = flags & LC_GRAPH;
     b.split_bregman({(float)lambda}, {(float)mu}, (int)outer, (int)inner, tol, xtol,
                     (flags & LC_TV_ZERO_PAD) ? 1 : 0, (flags & LC_TV_ANISO) != 0, graph, !graph);
-    copy_out(ctx->c, x, b.X(), bytes, dev);
+    copy_out(ctx->c, x, b.Xv(), bytes, dev);
     ctx->c.sync();
     if (rep) {
         fill_report(b.cg_state()[0], rep);
@@ -198,13 +220,30 @@ static void sb_one(lc_ctx ctx, ProbSpec ps, const float *rhs, float *x, double l
     }
 }
 
-int lc_sysmat2d_psf(lc_ctx ctx, lc_geom2d g, const float *x, int64_t nslices, float *y,
-                    uint32_t flags) {
+extern "C" {
+
+} // extern "C"
+
+template <typename T>
+static int sysmat2d_psf_t(lc_ctx ctx, lc_geom2d g, const T *x, int64_t nslices, T *y,
+                          uint32_t flags) {
     return guard([&] {
         LC_REQUIRE(ctx && g && x && y, "lc_sysmat2d_psf: NULL argument");
         LC_REQUIRE(nslices >= 1, "lc_sysmat2d_psf: nslices must be >= 1");
-        sysmat_psf(ctx, ProbSpec{&g->g, (int)nslices}, x, y, flags);
+        sysmat_psf<T>(ctx, ProbSpec{&g->g, (int)nslices}, x, y, flags);
     });
+}
+
+extern "C" {
+
+int lc_sysmat2d_psf(lc_ctx ctx, lc_geom2d g, const float *x, int64_t nslices, float *y,
+                    uint32_t flags) {
+    return sysmat2d_psf_t(ctx, g, x, nslices, y, flags);
+}
+
+int lc_sysmat2d_psf_f64(lc_ctx ctx, lc_geom2d g, const double *x, int64_t nslices, double *y,
+                        uint32_t flags) {
+    return sysmat2d_psf_t(ctx, g, x, nslices, y, flags);
 }
 
 // forward2d (mode 0) / backproj2d (1) / sysmat2d NUFFT form (2); host arrays
@@ -252,24 +291,57 @@ int lc_sysmat2d_nufft(lc_ctx ctx, lc_geom2d g, const float *x, int64_t nslices, 
     return project2d(ctx, g, x, nslices, y, flags, 2);
 }
 
-int lc_cg2d(lc_ctx ctx, lc_geom2d g, const float *rhs, float *x, int64_t nslices,
-            int64_t max_iter, double tol, double xtol, uint32_t flags, lc_report *rep) {
+} // extern "C"
+
+template <typename T>
+static int cg2d_t(lc_ctx ctx, lc_geom2d g, const T *rhs, T *x, int64_t nslices, int64_t max_iter,
+                  double tol, double xtol, uint32_t flags, lc_report *rep) {
     return guard([&] {
         LC_REQUIRE(ctx && g && rhs && x, "lc_cg2d: NULL argument");
         LC_REQUIRE(nslices >= 1 && max_iter >= 0, "lc_cg2d: bad sizes");
-        cg_one(ctx, ProbSpec{&g->g, (int)nslices}, rhs, x, max_iter, tol, xtol, flags, rep);
+        cg_one<T>(ctx, ProbSpec{&g->g, (int)nslices}, rhs, x, max_iter, tol, xtol, flags, rep);
     });
 }
+
+extern "C" {
+
+int lc_cg2d(lc_ctx ctx, lc_geom2d g, const float *rhs, float *x, int64_t nslices,
+            int64_t max_iter, double tol, double xtol, uint32_t flags, lc_report *rep) {
+    return cg2d_t(ctx, g, rhs, x, nslices, max_iter, tol, xtol, flags, rep);
+}
+
+int lc_cg2d_f64(lc_ctx ctx, lc_geom2d g, const double *rhs, double *x, int64_t nslices,
+                int64_t max_iter, double tol, double xtol, uint32_t flags, lc_report *rep) {
+    return cg2d_t(ctx, g, rhs, x, nslices, max_iter, tol, xtol, flags, rep);
+}
+
+} // extern "C"
+
+template <typename T>
+static int sb2d_t(lc_ctx ctx, lc_geom2d g, const T *rhs, T *x, int64_t nslices, double lambda,
+                  double mu, int64_t outer, int64_t inner, double tol, double xtol, uint32_t flags,
+                  lc_report *rep) {
+    return guard([&] {
+        LC_REQUIRE(ctx && g && rhs && x, "lc_split_bregman2d: NULL argument");
+        LC_REQUIRE(nslices >= 1 && outer >= 0 && inner >= 0, "lc_split_bregman2d: bad sizes");
+        sb_one<T>(ctx, ProbSpec{&g->g, (int)nslices}, rhs, x, lambda, mu, outer, inner, tol, xtol,
+                  flags, rep);
+    });
+}
+
+extern "C" {
 
 int lc_split_bregman2d(lc_ctx ctx, lc_geom2d g, const float *rhs, float *x, int64_t nslices,
                        double lambda, double mu, int64_t outer, int64_t inner, double tol,
                        double xtol, uint32_t flags, lc_report *rep) {
-    return guard([&] {
-        LC_REQUIRE(ctx && g && rhs && x, "lc_split_bregman2d: NULL argument");
-        LC_REQUIRE(nslices >= 1 && outer >= 0 && inner >= 0, "lc_split_bregman2d: bad sizes");
-        sb_one(ctx, ProbSpec{&g->g, (int)nslices}, rhs, x, lambda, mu, outer, inner, tol, xtol,
-               flags, rep);
-    });
+    return sb2d_t(ctx, g, rhs, x, nslices, lambda, mu, outer, inner, tol, xtol, flags, rep);
+}
+
+int lc_split_bregman2d_f64(lc_ctx ctx, lc_geom2d g, const double *rhs, double *x,
+                           int64_t nslices, double lambda, double mu, int64_t outer,
+                           int64_t inner, double tol, double xtol, uint32_t flags,
+                           lc_report *rep) {
+    return sb2d_t(ctx, g, rhs, x, nslices, lambda, mu, outer, inner, tol, xtol, flags, rep);
 }
 
 // ---- 3-D ----
@@ -322,12 +394,25 @@ int lc_psf3d_kernel_hat(lc_ctx ctx, lc_geom3d g, float *out) {
     });
 }
 
-int lc_sysmat3d_psf(lc_ctx ctx, lc_geom3d g, const float *x, float *y, uint32_t flags) {
+} // extern "C"
+
+template <typename T>
+static int sysmat3d_psf_t(lc_ctx ctx, lc_geom3d g, const T *x, T *y, uint32_t flags) {
     return guard([&] {
         LC_REQUIRE(ctx && g && x && y, "lc_sysmat3d_psf: NULL argument");
         ProbSpec ps{nullptr, (int)g->g.n1, &g->g};
-        sysmat_psf(ctx, ps, x, y, flags);
+        sysmat_psf<T>(ctx, ps, x, y, flags);
     });
+}
+
+extern "C" {
+
+int lc_sysmat3d_psf(lc_ctx ctx, lc_geom3d g, const float *x, float *y, uint32_t flags) {
+    return sysmat3d_psf_t(ctx, g, x, y, flags);
+}
+
+int lc_sysmat3d_psf_f64(lc_ctx ctx, lc_geom3d g, const double *x, double *y, uint32_t flags) {
+    return sysmat3d_psf_t(ctx, g, x, y, flags);
 }
 
 // forward (mode 0) / backproj (1) / NUFFT sysmat (2) on one volume
@@ -376,28 +461,64 @@ int lc_sysmat3d_nufft(lc_ctx ctx, lc_geom3d g, const float *x, float *y, uint32_
     return project3d(ctx, g, x, y, nullptr, flags, 2);
 }
 
-int lc_cg3d(lc_ctx ctx, lc_geom3d g, const float *rhs, float *x, int64_t max_iter, double tol,
-            double xtol, uint32_t flags, lc_report *rep) {
+} // extern "C"
+
+template <typename T>
+static int cg3d_t(lc_ctx ctx, lc_geom3d g, const T *rhs, T *x, int64_t max_iter, double tol,
+                  double xtol, uint32_t flags, lc_report *rep) {
     return guard([&] {
         LC_REQUIRE(ctx && g && rhs && x, "lc_cg3d: NULL argument");
         LC_REQUIRE(max_iter >= 0, "lc_cg3d: bad max_iter");
-        cg_one(ctx, ProbSpec{nullptr, (int)g->g.n1, &g->g}, rhs, x, max_iter, tol, xtol, flags, rep);
+        cg_one<T>(ctx, ProbSpec{nullptr, (int)g->g.n1, &g->g}, rhs, x, max_iter, tol, xtol, flags,
+                  rep);
     });
 }
+
+extern "C" {
+
+int lc_cg3d(lc_ctx ctx, lc_geom3d g, const float *rhs, float *x, int64_t max_iter, double tol,
+            double xtol, uint32_t flags, lc_report *rep) {
+    return cg3d_t(ctx, g, rhs, x, max_iter, tol, xtol, flags, rep);
+}
+
+int lc_cg3d_f64(lc_ctx ctx, lc_geom3d g, const double *rhs, double *x, int64_t max_iter,
+                double tol, double xtol, uint32_t flags, lc_report *rep) {
+    return cg3d_t(ctx, g, rhs, x, max_iter, tol, xtol, flags, rep);
+}
+
+} // extern "C"
+
+template <typename T>
+static int sb3d_t(lc_ctx ctx, lc_geom3d g, const T *rhs, T *x, double lambda, double mu,
+                  int64_t outer, int64_t inner, double tol, double xtol, uint32_t flags,
+                  lc_report *rep) {
+    return guard([&] {
+        LC_REQUIRE(ctx && g && rhs && x, "lc_split_bregman3d: NULL argument");
+        LC_REQUIRE(outer >= 0 && inner >= 0, "lc_split_bregman3d: bad sizes");
+        sb_one<T>(ctx, ProbSpec{nullptr, (int)g->g.n1, &g->g}, rhs, x, lambda, mu, outer, inner,
+                  tol, xtol, flags, rep);
+    });
+}
+
+extern "C" {
 
 int lc_split_bregman3d(lc_ctx ctx, lc_geom3d g, const float *rhs, float *x, double lambda,
                        double mu, int64_t outer, int64_t inner, double tol, double xtol,
                        uint32_t flags, lc_report *rep) {
-    return guard([&] {
-        LC_REQUIRE(ctx && g && rhs && x, "lc_split_bregman3d: NULL argument");
-        LC_REQUIRE(outer >= 0 && inner >= 0, "lc_split_bregman3d: bad sizes");
-        sb_one(ctx, ProbSpec{nullptr, (int)g->g.n1, &g->g}, rhs, x, lambda, mu, outer, inner, tol,
-               xtol, flags, rep);
-    });
+    return sb3d_t(ctx, g, rhs, x, lambda, mu, outer, inner, tol, xtol, flags, rep);
 }
 
-static int fd_call(lc_ctx ctx, int op, const float *in, float *out, int64_t n1, int64_t n2,
-                   int64_t n3, uint32_t flags) {
+int lc_split_bregman3d_f64(lc_ctx ctx, lc_geom3d g, const double *rhs, double *x, double lambda,
+                           double mu, int64_t outer, int64_t inner, double tol, double xtol,
+                           uint32_t flags, lc_report *rep) {
+    return sb3d_t(ctx, g, rhs, x, lambda, mu, outer, inner, tol, xtol, flags, rep);
+}
+
+} // extern "C"
+
+template <typename T>
+static int fd_call(lc_ctx ctx, int op, const T *in, T *out, int64_t n1, int64_t n2, int64_t n3,
+                   uint32_t flags) {
     return guard([&] {
         LC_REQUIRE(ctx && in && out, "finite differences: NULL argument");
         LC_REQUIRE(n1 >= 1 && n2 >= 1 && n3 >= 1, "finite differences: bad dims");
@@ -406,20 +527,22 @@ static int fd_call(lc_ctx ctx, int op, const float *in, float *out, int64_t n1, 
         const int64_t n = n1 * n2 * n3;
         const int64_t nin = (op == 1) ? 3 * n : n, nout = (op == 0) ? 3 * n : n;
         const bool dev = flags & LC_DEVICE_PTRS;
-        const float *din = in;
-        float *dout = out;
+        const T *din = in;
+        T *dout = out;
         if (!dev) {
-            float *buf = (float *)ctx->c.scratch(2, sizeof(float) * (size_t)(nin + nout));
-            copy_in(ctx->c, buf, in, sizeof(float) * nin, false);
+            T *buf = (T *)ctx->c.scratch(2, sizeof(T) * (size_t)(nin + nout));
+            copy_in(ctx->c, buf, in, sizeof(T) * nin, false);
             din = buf;
             dout = buf + nin;
         }
         fd_launch(ctx->c, op, din, dout, (int)n1, (int)n2, (int)n3,
                   (flags & LC_TV_ZERO_PAD) ? 1 : 0);
-        if (!dev) copy_out(ctx->c, out, dout, sizeof(float) * nout, false);
+        if (!dev) copy_out(ctx->c, out, dout, sizeof(T) * nout, false);
         ctx->c.sync();
     });
 }
+
+extern "C" {
 
 int lc_grad(lc_ctx ctx, const float *u, int64_t n1, int64_t n2, int64_t n3, float *g,
             uint32_t flags) {
@@ -436,13 +559,33 @@ int lc_laplacian(lc_ctx ctx, const float *u, int64_t n1, int64_t n2, int64_t n3,
     return fd_call(ctx, 2, u, out, n1, n2, n3, flags);
 }
 
+int lc_grad_f64(lc_ctx ctx, const double *u, int64_t n1, int64_t n2, int64_t n3, double *g,
+                uint32_t flags) {
+    return fd_call(ctx, 0, u, g, n1, n2, n3, flags);
+}
+
+int lc_divergence_f64(lc_ctx ctx, const double *g, int64_t n1, int64_t n2, int64_t n3,
+                      double *out, uint32_t flags) {
+    return fd_call(ctx, 1, g, out, n1, n2, n3, flags);
+}
+
+int lc_laplacian_f64(lc_ctx ctx, const double *u, int64_t n1, int64_t n2, int64_t n3, double *out,
+                     uint32_t flags) {
+    return fd_call(ctx, 2, u, out, n1, n2, n3, flags);
+}
+
 int lc_batch2d_create(lc_ctx ctx, int32_t nprob, const lc_geom2d *geoms, const int32_t *ns,
                       lc_batch2d *out) {
-    return lc_batch_create(ctx, nprob, geoms, nullptr, ns, out);
+    return lc_batch_create_ex(ctx, nprob, geoms, nullptr, ns, 0, out);
 }
 
 int lc_batch_create(lc_ctx ctx, int32_t nprob, const lc_geom2d *g2, const lc_geom3d *g3,
                     const int32_t *ns, lc_batch2d *out) {
+    return lc_batch_create_ex(ctx, nprob, g2, g3, ns, 0, out);
+}
+
+int lc_batch_create_ex(lc_ctx ctx, int32_t nprob, const lc_geom2d *g2, const lc_geom3d *g3,
+                       const int32_t *ns, uint32_t flags, lc_batch2d *out) {
     return guard([&] {
         LC_REQUIRE(ctx && out && nprob >= 1 && (g2 || g3), "lc_batch_create: bad argument");
         std::lock_guard<std::mutex> lk(ctx->c.mu);
@@ -465,10 +608,11 @@ int lc_batch_create(lc_ctx ctx, int32_t nprob, const lc_geom2d *g2, const lc_geo
         auto *b = new lc_batch2d_s();
         b->ctx = ctx;
         try {
-            b->b = std::make_unique<Batch2D>(ctx->c, ps);
+            b->b = std::make_unique<Batch2D>(ctx->c, ps, (flags & LC_FP64) != 0);
             // zero x and y
-            LC_HIP(hipMemsetAsync(b->b->X(), 0, sizeof(float) * b->b->total(), ctx->c.stream()));
-            LC_HIP(hipMemsetAsync(b->b->Y(), 0, sizeof(float) * b->b->total(), ctx->c.stream()));
+            const size_t bytes = b->b->esize() * b->b->total();
+            LC_HIP(hipMemsetAsync(b->b->Xv(), 0, bytes, ctx->c.stream()));
+            LC_HIP(hipMemsetAsync(b->b->Yv(), 0, bytes, ctx->c.stream()));
             ctx->c.sync();
         } catch (...) {
             delete b;
@@ -489,33 +633,71 @@ int lc_batch2d_destroy(lc_batch2d b) {
     });
 }
 
-int lc_batch2d_set(lc_batch2d b, int32_t prob, int32_t which, const float *src, uint32_t flags) {
+} // extern "C"
+
+template <typename T>
+static int batch_set_t(lc_batch2d b, int32_t prob, int32_t which, const T *src, uint32_t flags) {
     return guard([&] {
         LC_REQUIRE(b && src && prob >= 0 && prob < b->b->nprob(), "lc_batch2d_set: bad argument");
+        batch_check_type<T>(*b->b, "lc_batch2d_set");
         std::lock_guard<std::mutex> lk(b->ctx->c.mu);
         b->ctx->c.set_current();
-        float *dst = batch_buf(*b->b, which) + b->b->offset(prob);
-        copy_in(b->ctx->c, dst, src, sizeof(float) * b->b->size(prob), flags & LC_DEVICE_PTRS);
+        void *dst = b->b->elem(batch_buf(*b->b, which), b->b->offset(prob));
+        copy_in(b->ctx->c, dst, src, sizeof(T) * b->b->size(prob), flags & LC_DEVICE_PTRS);
         b->ctx->c.sync();
     });
 }
 
-int lc_batch2d_get(lc_batch2d b, int32_t prob, int32_t which, float *dst, uint32_t flags) {
+extern "C" {
+
+} // extern "C"
+
+template <typename T>
+static int batch_get_t(lc_batch2d b, int32_t prob, int32_t which, T *dst, uint32_t flags) {
     return guard([&] {
         LC_REQUIRE(b && dst && prob >= 0 && prob < b->b->nprob(), "lc_batch2d_get: bad argument");
+        batch_check_type<T>(*b->b, "lc_batch2d_get");
         std::lock_guard<std::mutex> lk(b->ctx->c.mu);
         b->ctx->c.set_current();
-        const float *src = batch_buf(*b->b, which) + b->b->offset(prob);
-        copy_out(b->ctx->c, dst, src, sizeof(float) * b->b->size(prob), flags & LC_DEVICE_PTRS);
+        const void *src = b->b->elem(batch_buf(*b->b, which), b->b->offset(prob));
+        copy_out(b->ctx->c, dst, src, sizeof(T) * b->b->size(prob), flags & LC_DEVICE_PTRS);
         b->ctx->c.sync();
     });
 }
 
-int lc_batch2d_device_ptr(lc_batch2d b, int32_t prob, int32_t which, float **out) {
+extern "C" {
+
+} // extern "C"
+
+template <typename T>
+static int batch_ptr_t(lc_batch2d b, int32_t prob, int32_t which, T **out) {
     return guard([&] {
         LC_REQUIRE(b && out && prob >= 0 && prob < b->b->nprob(), "lc_batch2d_device_ptr: bad argument");
-        *out = batch_buf(*b->b, which) + b->b->offset(prob);
+        batch_check_type<T>(*b->b, "lc_batch2d_device_ptr");
+        *out = static_cast<T *>(b->b->elem(batch_buf(*b->b, which), b->b->offset(prob)));
     });
+}
+
+extern "C" {
+
+int lc_batch2d_set(lc_batch2d b, int32_t prob, int32_t which, const float *src, uint32_t flags) {
+    return batch_set_t(b, prob, which, src, flags);
+}
+int lc_batch2d_get(lc_batch2d b, int32_t prob, int32_t which, float *dst, uint32_t flags) {
+    return batch_get_t(b, prob, which, dst, flags);
+}
+int lc_batch2d_device_ptr(lc_batch2d b, int32_t prob, int32_t which, float **out) {
+    return batch_ptr_t(b, prob, which, out);
+}
+int lc_batch2d_set_f64(lc_batch2d b, int32_t prob, int32_t which, const double *src,
+                       uint32_t flags) {
+    return batch_set_t(b, prob, which, src, flags);
+}
+int lc_batch2d_get_f64(lc_batch2d b, int32_t prob, int32_t which, double *dst, uint32_t flags) {
+    return batch_get_t(b, prob, which, dst, flags);
+}
+int lc_batch2d_device_ptr_f64(lc_batch2d b, int32_t prob, int32_t which, double **out) {
+    return batch_ptr_t(b, prob, which, out);
 }
 
 int lc_batch2d_cg(lc_batch2d b, const int32_t *max_iter, double tol, double xtol,

@@ -15,6 +15,10 @@
 // Split-Bregman follows src/bregman.cpp:36-94 per problem with the operator
 // A - mu * laplacian, the finite differences of src/finitediff.cpp:26-90
 // (boundary 0) or zero padded ones (boundary 1, src/gpu/finitediff.cu).
+//
+// Every element kernel is a template on the element type R: float (FP32 bulk
+// arithmetic) or double (the FP64 path of the reference's Array<double>
+// instantiations, src/conjgrad.cpp:82-85, src/bregman.cpp:101-104).
 #include <algorithm>
 #include <cstring>
 #include <map>
@@ -22,13 +26,6 @@
 #include "lc_engine.hpp"
 
 namespace lc {
-
-void toep_launch_fwd(Ctx &c, const ToepClass &k, const CgState *st, const float *src,
-                     const float *rsrc, float *pdst, float2 *U, bool fuse);
-void toep_launch_cols(Ctx &c, const ToepClass &k, const ProbDev *pd, const CgState *st,
-                      float2 *U);
-void toep_launch_inv(Ctx &c, const ToepClass &k, const CgState *st, const float2 *U, float *dst,
-                     const float *pvec, double *partial, bool dot);
 
 // ----------------------------------------------------------------------------
 // reductions
@@ -53,7 +50,7 @@ __device__ __forceinline__ double wave_sum(const double *partial, int b, int e, 
     return __shfl(v, 0, 64);
 }
 
-__global__ void k_fill(float *p, float v, int64_t n) {
+template <typename R> __global__ void k_fill(R *p, R v, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v;
@@ -76,20 +73,20 @@ __global__ void k_cg_arm(CgState *st, const SbState *sb, const int *max_iter, do
 }
 
 // r = y - (x0 == 0 ? 0 : Ap); p = 0; partials <r,r>, <y,y>
-template <int NT>
+template <typename R, int NT>
 __global__ void __launch_bounds__(NT)
 k_cg_init(const ChunkRef *__restrict__ ch, const CgState *__restrict__ st,
-          const float *__restrict__ y, const float *__restrict__ ap, float *__restrict__ r,
-          float *__restrict__ p, float *__restrict__ x, double *__restrict__ cpart, int x0_zero) {
+          const R *__restrict__ y, const R *__restrict__ ap, R *__restrict__ r,
+          R *__restrict__ p, R *__restrict__ x, double *__restrict__ cpart, int x0_zero) {
     const ChunkRef c = ch[blockIdx.x];
     if (!st[c.prob].active) return;
     double rr = 0.0, yy = 0.0;
     for (int64_t i = c.begin + threadIdx.x; i < c.end; i += NT) {
-        const float yi = y[i];
-        if (x0_zero) x[i] = 0.0f;   // x0 = 0: x is not read, only cleared
-        const float ri = x0_zero ? yi : yi - ap[i];
+        const R yi = y[i];
+        if (x0_zero) x[i] = (R)0;   // x0 = 0: x is not read, only cleared
+        const R ri = x0_zero ? yi : yi - ap[i];
         r[i] = ri;
-        p[i] = 0.0f;
+        p[i] = (R)0;
         rr += (double)ri * ri;
         yy += (double)yi * yi;
     }
@@ -138,11 +135,11 @@ __global__ void k_cg_fin1(const ProbDev *__restrict__ pd, CgState *st, const dou
 }
 
 // x += alpha p; r -= alpha Ap; partials <r,r>, ||alpha p||^2, ||x||^2
-template <int NT>
+template <typename R, int NT>
 __global__ void __launch_bounds__(NT)
 k_cg_update(const ChunkRef *__restrict__ ch, const CgState *__restrict__ st,
-            float *__restrict__ x, float *__restrict__ r, const float *__restrict__ p,
-            const float *__restrict__ ap, double *__restrict__ cpart) {
+            R *__restrict__ x, R *__restrict__ r, const R *__restrict__ p,
+            const R *__restrict__ ap, double *__restrict__ cpart) {
     const ChunkRef c = ch[blockIdx.x];
     const CgState s = st[c.prob];
     if (!s.active) return;
@@ -152,9 +149,9 @@ k_cg_update(const ChunkRef *__restrict__ ch, const CgState *__restrict__ st,
     const double alpha = s.alpha;
     double rr = 0.0, ss = 0.0, xx = 0.0;
     for (int64_t i = c.begin + threadIdx.x; i < c.end; i += NT) {
-        const float step = (float)((double)p[i] * alpha);
-        const float xi = x[i] + step;
-        const float ri = r[i] - (float)((double)ap[i] * alpha);
+        const R step = (R)((double)p[i] * alpha);
+        const R xi = x[i] + step;
+        const R ri = r[i] - (R)((double)ap[i] * alpha);
         x[i] = xi;
         r[i] = ri;
         rr += (double)ri * ri;
@@ -200,7 +197,8 @@ struct Idx3 {
     int s, a, b, n1, n2, n3;
 };
 
-__device__ __forceinline__ float at3(const float *u, const Idx3 &q, int s, int a, int b) {
+template <typename R>
+__device__ __forceinline__ R at3(const R *u, const Idx3 &q, int s, int a, int b) {
     return u[((int64_t)s * q.n2 + a) * q.n3 + b];
 }
 
@@ -213,56 +211,59 @@ __device__ __forceinline__ bool inside(const Idx3 &q, int s, int a, int b) {
 }
 
 // gradient component c (0 slices, 1 rows, 2 cols) at (s,a,b)
-__device__ __forceinline__ float grad_c(const float *u, const Idx3 &q, int c, int s, int a, int b,
-                                        int boundary) {
+template <typename R>
+__device__ __forceinline__ R grad_c(const R *u, const Idx3 &q, int c, int s, int a, int b,
+                                    int boundary) {
     const int ds = c == 0, da = c == 1, db = c == 2;
     if (boundary == 0) {
-        if (!interior(q, s, a, b)) return 0.0f;
-        return (at3(u, q, s + ds, a + da, b + db) - at3(u, q, s - ds, a - da, b - db)) / 2.0f;
+        if (!interior(q, s, a, b)) return (R)0;
+        return (at3(u, q, s + ds, a + da, b + db) - at3(u, q, s - ds, a - da, b - db)) / (R)2;
     }
-    const float up = inside(q, s + ds, a + da, b + db) ? at3(u, q, s + ds, a + da, b + db) : 0.0f;
-    const float um = inside(q, s - ds, a - da, b - db) ? at3(u, q, s - ds, a - da, b - db) : 0.0f;
-    return 0.5f * (up - um);
+    const R up = inside(q, s + ds, a + da, b + db) ? at3(u, q, s + ds, a + da, b + db) : (R)0;
+    const R um = inside(q, s - ds, a - da, b - db) ? at3(u, q, s - ds, a - da, b - db) : (R)0;
+    return (R)0.5 * (up - um);
 }
 
 // divergence of a 3-component field g (component c at g + c*cstride) at (s,a,b)
-__device__ __forceinline__ float div_at(const float *g, int64_t cstride, const Idx3 &q, int s,
-                                        int a, int b, int boundary) {
+template <typename R>
+__device__ __forceinline__ R div_at(const R *g, int64_t cstride, const Idx3 &q, int s, int a, int b,
+                                    int boundary) {
     if (boundary == 0) {
-        if (!interior(q, s, a, b)) return 0.0f;
-        return (at3(g, q, s + 1, a, b) - at3(g, q, s - 1, a, b)) / 2.0f +
-               (at3(g + cstride, q, s, a + 1, b) - at3(g + cstride, q, s, a - 1, b)) / 2.0f +
-               (at3(g + 2 * cstride, q, s, a, b + 1) - at3(g + 2 * cstride, q, s, a, b - 1)) / 2.0f;
+        if (!interior(q, s, a, b)) return (R)0;
+        return (at3(g, q, s + 1, a, b) - at3(g, q, s - 1, a, b)) / (R)2 +
+               (at3(g + cstride, q, s, a + 1, b) - at3(g + cstride, q, s, a - 1, b)) / (R)2 +
+               (at3(g + 2 * cstride, q, s, a, b + 1) - at3(g + 2 * cstride, q, s, a, b - 1)) / (R)2;
     }
-    const float g0p = inside(q, s + 1, a, b) ? at3(g, q, s + 1, a, b) : 0.0f;
-    const float g0m = inside(q, s - 1, a, b) ? at3(g, q, s - 1, a, b) : 0.0f;
-    const float g1p = inside(q, s, a + 1, b) ? at3(g + cstride, q, s, a + 1, b) : 0.0f;
-    const float g1m = inside(q, s, a - 1, b) ? at3(g + cstride, q, s, a - 1, b) : 0.0f;
-    const float g2p = inside(q, s, a, b + 1) ? at3(g + 2 * cstride, q, s, a, b + 1) : 0.0f;
-    const float g2m = inside(q, s, a, b - 1) ? at3(g + 2 * cstride, q, s, a, b - 1) : 0.0f;
-    return 0.5f * (g0p - g0m) + 0.5f * (g1p - g1m) + 0.5f * (g2p - g2m);
+    const R g0p = inside(q, s + 1, a, b) ? at3(g, q, s + 1, a, b) : (R)0;
+    const R g0m = inside(q, s - 1, a, b) ? at3(g, q, s - 1, a, b) : (R)0;
+    const R g1p = inside(q, s, a + 1, b) ? at3(g + cstride, q, s, a + 1, b) : (R)0;
+    const R g1m = inside(q, s, a - 1, b) ? at3(g + cstride, q, s, a - 1, b) : (R)0;
+    const R g2p = inside(q, s, a, b + 1) ? at3(g + 2 * cstride, q, s, a, b + 1) : (R)0;
+    const R g2m = inside(q, s, a, b - 1) ? at3(g + 2 * cstride, q, s, a, b - 1) : (R)0;
+    const R h = (R)0.5;
+    return h * (g0p - g0m) + h * (g1p - g1m) + h * (g2p - g2m);
 }
 
 // laplacian = div(grad u) evaluated directly from u
-__device__ __forceinline__ float lap_at(const float *u, const Idx3 &q, int s, int a, int b,
-                                        int boundary) {
-    float acc = 0.0f;
+template <typename R>
+__device__ __forceinline__ R lap_at(const R *u, const Idx3 &q, int s, int a, int b, int boundary) {
+    R acc = (R)0;
     if (boundary == 0) {
-        if (!interior(q, s, a, b)) return 0.0f;
+        if (!interior(q, s, a, b)) return (R)0;
         // component c: (G(i+e) - G(i-e))/2 with G zero off the interior
         for (int c = 0; c < 3; ++c) {
             const int ds = c == 0, da = c == 1, db = c == 2;
-            const float gp = grad_c(u, q, c, s + ds, a + da, b + db, 0);
-            const float gm = grad_c(u, q, c, s - ds, a - da, b - db, 0);
-            acc += (gp - gm) / 2.0f;
+            const R gp = grad_c(u, q, c, s + ds, a + da, b + db, 0);
+            const R gm = grad_c(u, q, c, s - ds, a - da, b - db, 0);
+            acc += (gp - gm) / (R)2;
         }
         return acc;
     }
     for (int c = 0; c < 3; ++c) {
         const int ds = c == 0, da = c == 1, db = c == 2;
-        const float gp = inside(q, s + ds, a + da, b + db) ? grad_c(u, q, c, s + ds, a + da, b + db, 1) : 0.0f;
-        const float gm = inside(q, s - ds, a - da, b - db) ? grad_c(u, q, c, s - ds, a - da, b - db, 1) : 0.0f;
-        acc += 0.5f * (gp - gm);
+        const R gp = inside(q, s + ds, a + da, b + db) ? grad_c(u, q, c, s + ds, a + da, b + db, 1) : (R)0;
+        const R gm = inside(q, s - ds, a - da, b - db) ? grad_c(u, q, c, s - ds, a - da, b - db, 1) : (R)0;
+        acc += (R)0.5 * (gp - gm);
     }
     return acc;
 }
@@ -282,20 +283,20 @@ __device__ __forceinline__ Idx3 idx_of(const ProbDev &d, int64_t i) {
 }
 
 // Ap -= mu * lap(p); partial <p, Ap> (Split-Bregman operator, src/bregman.cpp:55)
-template <int NT>
+template <typename R, int NT>
 __global__ void __launch_bounds__(NT)
 k_sb_lapdot(const ChunkRef *__restrict__ ch, const ProbDev *__restrict__ pd,
-            const CgState *__restrict__ st, const float *__restrict__ p, float *__restrict__ ap,
+            const CgState *__restrict__ st, const R *__restrict__ p, R *__restrict__ ap,
             double *__restrict__ cpart, int dot) {
     const ChunkRef c = ch[blockIdx.x];
     if (st && !st[c.prob].active) return;
     const ProbDev d = pd[c.prob];
-    const float *pp = p + d.off;
+    const R *pp = p + d.off;
     double acc = 0.0;
     for (int64_t i = c.begin + threadIdx.x; i < c.end; i += NT) {
         const Idx3 q = idx_of(d, i);
-        const float l = lap_at(pp, q, q.s, q.a, q.b, d.boundary);
-        const float v = ap[i] - l * d.mu;
+        const R l = lap_at(pp, q, q.s, q.a, q.b, d.boundary);
+        const R v = ap[i] - l * (R)d.mu;
         ap[i] = v;
         acc += (double)p[i] * v;
     }
@@ -307,28 +308,27 @@ k_sb_lapdot(const ChunkRef *__restrict__ ch, const ProbDev *__restrict__ pd,
 }
 
 // rhs = y0 - mu * div(d - b)   (src/bregman.cpp:60-62)
-template <int NT>
+template <typename R, int NT>
 __global__ void __launch_bounds__(NT)
 k_sb_rhs(const ChunkRef *__restrict__ ch, const ProbDev *__restrict__ pd,
-         const SbState *__restrict__ sb, const float *__restrict__ y0, const float *__restrict__ D,
-         const float *__restrict__ Bv, float *__restrict__ rhs, float *__restrict__ tmp,
-         int64_t total) {
+         const SbState *__restrict__ sb, const R *__restrict__ y0, R *__restrict__ rhs,
+         const R *__restrict__ tmp, int64_t total) {
     const ChunkRef c = ch[blockIdx.x];
     if (!sb[c.prob].active) return;
     const ProbDev d = pd[c.prob];
     // tmp holds d - b for the three components (component stride = total)
     for (int64_t i = c.begin + threadIdx.x; i < c.end; i += NT) {
         const Idx3 q = idx_of(d, i);
-        const float *g = tmp + d.off;
-        const float dv = div_at(g, total, q, q.s, q.a, q.b, d.boundary);
-        rhs[i] = y0[i] - dv * d.mu;
+        const R *g = tmp + d.off;
+        const R dv = div_at(g, total, q, q.s, q.a, q.b, d.boundary);
+        rhs[i] = y0[i] - dv * (R)d.mu;
     }
 }
 
-template <int NT>
+template <typename R, int NT>
 __global__ void __launch_bounds__(NT)
 k_sb_dmb(const ChunkRef *__restrict__ ch, const SbState *__restrict__ sb,
-         const float *__restrict__ D, const float *__restrict__ Bv, float *__restrict__ tmp,
+         const R *__restrict__ D, const R *__restrict__ Bv, R *__restrict__ tmp,
          int64_t total) {
     const ChunkRef c = ch[blockIdx.x];
     if (!sb[c.prob].active) return;
@@ -338,43 +338,43 @@ k_sb_dmb(const ChunkRef *__restrict__ ch, const SbState *__restrict__ sb,
 
 // isotropic (or per-component) shrink and Bregman update
 // (src/bregman.cpp:64-84); partials ||x - x_old||^2, ||x_old||^2; x_old = x
-template <int NT>
+template <typename R, int NT>
 __global__ void __launch_bounds__(NT)
 k_sb_shrink(const ChunkRef *__restrict__ ch, const ProbDev *__restrict__ pd,
-            const SbState *__restrict__ sb, const float *__restrict__ x, float *__restrict__ D,
-            float *__restrict__ Bv, float *__restrict__ xold, double *__restrict__ cpart,
+            const SbState *__restrict__ sb, const R *__restrict__ x, R *__restrict__ D,
+            R *__restrict__ Bv, R *__restrict__ xold, double *__restrict__ cpart,
             int64_t total, int aniso) {
     const ChunkRef c = ch[blockIdx.x];
     const SbState S = sb[c.prob];
     if (!S.active) return;
     const ProbDev d = pd[c.prob];
-    const float *xp = x + d.off;
-    const float thr = (float)(S.lambda / S.mu);
+    const R *xp = x + d.off;
+    const R thr = (R)(S.lambda / S.mu);
     double dn = 0.0, on = 0.0;
     for (int64_t i = c.begin + threadIdx.x; i < c.end; i += NT) {
         const Idx3 q = idx_of(d, i);
-        float g[3], v[3];
+        R g[3], v[3];
         for (int k = 0; k < 3; ++k) {
             g[k] = grad_c(xp, q, k, q.s, q.a, q.b, d.boundary);
             v[k] = g[k] + Bv[k * total + i];
         }
         if (!aniso) {
-            const float sk = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-            const float sh = fmaxf(0.0f, sk - thr);
+            const R sk = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            const R sh = fmax((R)0, sk - thr);
             for (int k = 0; k < 3; ++k) {
-                const float dk = sh * (v[k] / (sk + 1e-8f));
+                const R dk = sh * (v[k] / (sk + (R)1e-8));
                 D[k * total + i] = dk;
                 Bv[k * total + i] += g[k] - dk;
             }
         } else {
             for (int k = 0; k < 3; ++k) {
-                const float sh = fmaxf(0.0f, fabsf(v[k]) - thr);
-                const float dk = copysignf(sh, v[k]);
+                const R sh = fmax((R)0, fabs(v[k]) - thr);
+                const R dk = copysign(sh, v[k]);
                 D[k * total + i] = dk;
                 Bv[k * total + i] += g[k] - dk;
             }
         }
-        const float xi = x[i], xo = xold[i];
+        const R xi = x[i], xo = xold[i];
         dn += (double)(xi - xo) * (double)(xi - xo);
         on += (double)xo * xo;
         xold[i] = xi;
@@ -419,7 +419,8 @@ __device__ __forceinline__ Idx3 idx_vol(int64_t i, int n1, int n2, int n3) {
     return q;
 }
 
-__global__ void k_fd_grad(const float *u, float *g, int n1, int n2, int n3, int boundary) {
+template <typename R>
+__global__ void k_fd_grad(const R *u, R *g, int n1, int n2, int n3, int boundary) {
     const int64_t n = (int64_t)n1 * n2 * n3;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
@@ -428,7 +429,8 @@ __global__ void k_fd_grad(const float *u, float *g, int n1, int n2, int n3, int 
     }
 }
 
-__global__ void k_fd_div(const float *g, float *out, int n1, int n2, int n3, int boundary) {
+template <typename R>
+__global__ void k_fd_div(const R *g, R *out, int n1, int n2, int n3, int boundary) {
     const int64_t n = (int64_t)n1 * n2 * n3;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
@@ -437,7 +439,8 @@ __global__ void k_fd_div(const float *g, float *out, int n1, int n2, int n3, int
     }
 }
 
-__global__ void k_fd_lap(const float *u, float *out, int n1, int n2, int n3, int boundary) {
+template <typename R>
+__global__ void k_fd_lap(const R *u, R *out, int n1, int n2, int n3, int boundary) {
     const int64_t n = (int64_t)n1 * n2 * n3;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
@@ -446,33 +449,45 @@ __global__ void k_fd_lap(const float *u, float *out, int n1, int n2, int n3, int
     }
 }
 
-void fd_launch(Ctx &c, int op, const float *in, float *out, int n1, int n2, int n3,
-               int boundary) {
+template <typename R>
+static void fd_launch_t(Ctx &c, int op, const R *in, R *out, int n1, int n2, int n3, int boundary) {
     const int64_t n = (int64_t)n1 * n2 * n3;
     const int blocks = (int)std::min<int64_t>(4096, (n + 255) / 256);
     if (op == 0)
-        hipLaunchKernelGGL(k_fd_grad, dim3(blocks), dim3(256), 0, c.stream(), in, out, n1, n2, n3,
-                           boundary);
+        hipLaunchKernelGGL(k_fd_grad<R>, dim3(blocks), dim3(256), 0, c.stream(), in, out, n1, n2,
+                           n3, boundary);
     else if (op == 1)
-        hipLaunchKernelGGL(k_fd_div, dim3(blocks), dim3(256), 0, c.stream(), in, out, n1, n2, n3,
-                           boundary);
+        hipLaunchKernelGGL(k_fd_div<R>, dim3(blocks), dim3(256), 0, c.stream(), in, out, n1, n2,
+                           n3, boundary);
     else
-        hipLaunchKernelGGL(k_fd_lap, dim3(blocks), dim3(256), 0, c.stream(), in, out, n1, n2, n3,
-                           boundary);
+        hipLaunchKernelGGL(k_fd_lap<R>, dim3(blocks), dim3(256), 0, c.stream(), in, out, n1, n2,
+                           n3, boundary);
     LC_HIP(hipGetLastError());
+}
+
+void fd_launch(Ctx &c, int op, const float *in, float *out, int n1, int n2, int n3, int boundary) {
+    fd_launch_t(c, op, in, out, n1, n2, n3, boundary);
+}
+void fd_launch(Ctx &c, int op, const double *in, double *out, int n1, int n2, int n3,
+               int boundary) {
+    fd_launch_t(c, op, in, out, n1, n2, n3, boundary);
 }
 
 // ----------------------------------------------------------------------------
 // Batch2D
 // ----------------------------------------------------------------------------
-Batch2D::Batch2D(Ctx &c, const std::vector<ProbSpec> &probs) : c_(c), probs_(probs) {
+Batch2D::Batch2D(Ctx &c, const std::vector<ProbSpec> &probs, bool f64)
+    : c_(c), f64_(f64), probs_(probs) {
     LC_REQUIRE(!probs.empty(), "batch: no problems");
     build();
 }
 
-void Batch2D::toep_mid(const ToepClass &k, const CgState *st, float2 *U) {
-    if (k.g3) toep3d_mid(c_, k, st, U);
-    else toep_launch_cols(c_, k, dprob_.as<ProbDev>(), st, U);
+// call f(R{}) with R the batch's element type
+#define LC_ETYPE(f) (f64_ ? (f)(double{}) : (f)(float{}))
+
+template <typename R> void Batch2D::toep_mid(const ToepClass &k, const CgState *st, cpx<R> *U) {
+    if (k.g3) toep3d_mid<R>(c_, k, st, U);
+    else toep_launch_cols<R>(c_, k, dprob_.as<ProbDev>(), st, U);
 }
 
 void Batch2D::build() {
@@ -486,19 +501,20 @@ void Batch2D::build() {
         LC_REQUIRE(probs_[p].ns >= 1, "batch: ns must be >= 1");
         if (Geom3D *g3 = probs_[p].g3) {
             LC_REQUIRE(probs_[p].g == nullptr, "batch: problem with both a 2-D and a 3-D geometry");
-            if (!g3->psf_ready) geom3d_build_psf(c_, *g3);
+            if (f64_ ? !g3->psf64_ready : !g3->psf_ready) geom3d_build_psf(c_, *g3, f64_);
             probs_[p].ns = (int)g3->n1;
             vols.push_back(p);
             continue;
         }
         Geom2D *g = probs_[p].g;
         LC_REQUIRE(g, "batch: NULL geometry");
-        if (!g->psf_ready) geom2d_build_psf(c_, *g);
+        if (f64_ ? !g->psf64_ready : !g->psf_ready) geom2d_build_psf(c_, *g, f64_);
         by_n[(int)g->N].push_back(p);
     }
     std::vector<std::pair<int, std::vector<int>>> groups;
     for (auto &kv : by_n) groups.push_back(kv);
     for (int p : vols) groups.push_back({(int)probs_[p].g3->N, {p}});
+    const size_t cb = 2 * esize();   // bytes of one complex element
     // vector layout: problems in class order so every class is contiguous
     int64_t off = 0, uoff = 0;
     int chunk = 0, slot = 0;
@@ -512,32 +528,39 @@ void Batch2D::build() {
         k->M = g3 ? (int)g3->M : (int)next_fast_dim(2 * k->N - 1);
         k->H = k->M / 2 + 1;
         LC_REQUIRE(fft_make_plan(k->M, k->plan), "batch: PSF size not 2^p 3^q");
-        k->tw = c_.twiddles<float>(k->M);
+        if (f64_) k->tw64 = c_.twiddles<double>(k->M);
+        else k->tw = c_.twiddles<float>(k->M);
         if (g3) {
             // runtime-plan shapes: row shape by M, column blocks B * M <= the 3-D cap
             ToepVariant v;
             v.row = k->M > 2048 ? 1 : 0;
-            v.col = 2;
+            v.col = f64_ ? 0 : 2;   // block-width bound only: 4096 (FP64) or 8192 elements
             v.B = 0;
             toep_configure(*k, v);
-            LC_REQUIRE(k->B * k->M <= toep3d_col_cap(), "batch: 3-D column block too large");
+            LC_REQUIRE(k->B * k->M <= toep3d_col_cap(f64_), "batch: 3-D column block too large");
             k->g3 = g3;
             k->n1 = (int)g3->n1;
             k->M1 = (int)g3->M1;
             LC_REQUIRE(fft_make_plan(k->M1, k->plan1), "batch: 3-D slice-axis size not 2^p 3^q");
-            k->tw1 = c_.twiddles<float>(k->M1);
-            LC_REQUIRE(k->B * k->M1 <= toep3d_slice_cap(), "batch: 3-D slice axis too long");
-            k->kb2 = std::max(1, toep3d_slice_cap() / (k->B * k->M1));
-            k->W.alloc(sizeof(float2) * (size_t)k->nblk * k->M * k->n1 * k->B);
+            if (f64_) k->tw1_64 = c_.twiddles<double>(k->M1);
+            else k->tw1 = c_.twiddles<float>(k->M1);
+            LC_REQUIRE(k->B * k->M1 <= toep3d_slice_cap(f64_), "batch: 3-D slice axis too long");
+            k->kb2 = std::max(1, toep3d_slice_cap(f64_) / (k->B * k->M1));
+            k->W.alloc(cb * (size_t)k->nblk * k->M * k->n1 * k->B);
         } else {
             auto it = c_.toep_tuning.find(k->N);
-            toep_configure(*k, it != c_.toep_tuning.end() ? it->second : toep_default_variant(k->M));
+            ToepVariant v = it != c_.toep_tuning.end() ? it->second : toep_default_variant(k->M);
+            // a tuned FP32 column shape may not fit 16-byte elements
+            if (v.col >= toep_num_col_shapes_for(f64_)) v = toep_default_variant(k->M);
+            toep_configure(*k, v);
         }
         std::vector<SliceRef> sl;
         for (int p : kv.second) {
             const int N = k->N, ns = probs_[p].ns;
             ProbDev &d = hprob_[p];
-            d.khatT = g3 ? nullptr : probs_[p].g->khatT.as<float>();
+            d.khat = g3 ? nullptr
+                        : (f64_ ? (const void *)probs_[p].g->khatT64.as()
+                                : (const void *)probs_[p].g->khatT.as());
             d.N = N;
             d.ns = ns;
             d.off = off;
@@ -583,13 +606,13 @@ void Batch2D::build() {
     total_u_ = uoff;
     nchunk_ = chunk;
     nslot_ = slot;
-    const size_t fb = sizeof(float) * (size_t)total_;
+    const size_t fb = esize() * (size_t)total_;
     X_.alloc(fb);
     R_.alloc(fb);
     P_.alloc(fb);
     AP_.alloc(fb);
     Y_.alloc(fb);
-    U_.alloc(sizeof(float2) * (size_t)total_u_);
+    U_.alloc(cb * (size_t)total_u_);
     dprob_.alloc(sizeof(ProbDev) * np);
     dchunk_.alloc(sizeof(ChunkRef) * hchunk_.size());
     dstate_.alloc(sizeof(CgState) * np);
@@ -614,23 +637,30 @@ void Batch2D::build() {
     c_.sync();
 }
 
-void Batch2D::apply_op(const float *src, float *dst, bool sb, bool dot) {
+template <typename R> void Batch2D::apply_op_t(const R *src, R *dst, bool sb, bool dot) {
     CgState *st = dstate_.as<CgState>();
+    cpx<R> *U = U_.as<cpx<R>>();
     for (auto &k : classes_) {
-        toep_launch_fwd(c_, *k, st, src, nullptr, nullptr, U_.as<float2>(), false);
-        toep_mid(*k, st, U_.as<float2>());
-        toep_launch_inv(c_, *k, st, U_.as<float2>(), dst, src, partial_.as<double>(),
-                        dot && !sb);
+        toep_launch_fwd<R>(c_, *k, st, src, nullptr, nullptr, U, false);
+        toep_mid<R>(*k, st, U);
+        toep_launch_inv<R>(c_, *k, st, U, dst, src, partial_.as<double>(), dot && !sb);
     }
     if (sb) {
-        hipLaunchKernelGGL((k_sb_lapdot<256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
+        hipLaunchKernelGGL((k_sb_lapdot<R, 256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
                            dchunk_.as<ChunkRef>(), dprob_.as<ProbDev>(), st, src, dst,
                            cpartial_.as<double>(), dot ? 1 : 0);
         LC_HIP(hipGetLastError());
     }
 }
 
-void Batch2D::toeplitz(const float *src, float *dst) {
+void Batch2D::apply_op(const void *src, void *dst, bool sb, bool dot) {
+    LC_ETYPE([&](auto z) {
+        using R = decltype(z);
+        apply_op_t<R>(static_cast<const R *>(src), static_cast<R *>(dst), sb, dot);
+    });
+}
+
+void Batch2D::toeplitz(const void *src, void *dst) {
     // all problems: arm every state active with no iterations
     std::vector<int> one(probs_.size(), 1);
     DBuf mi(sizeof(int) * one.size());
@@ -650,14 +680,18 @@ float Batch2D::time_apply(int reps) {
                           c_.stream()));
     hipLaunchKernelGGL(k_cg_arm, dim3(ceil_div(nprob(), 64)), dim3(64), 0, c_.stream(),
                        dstate_.as<CgState>(), nullptr, mi.as<int>(), 0.0, 0.0, nprob());
-    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, c_.stream(), X(), 1.0f, total_);
+    LC_ETYPE([&](auto z) {
+        using R = decltype(z);
+        hipLaunchKernelGGL(k_fill<R>, dim3(1024), dim3(256), 0, c_.stream(), X_.as<R>(), (R)1,
+                           total_);
+    });
     LC_HIP(hipGetLastError());
-    apply_op(X(), AP(), false, false);   // warm-up
+    apply_op(X_.as(), AP_.as(), false, false);   // warm-up
     hipEvent_t e0, e1;
     LC_HIP(hipEventCreate(&e0));
     LC_HIP(hipEventCreate(&e1));
     LC_HIP(hipEventRecord(e0, c_.stream()));
-    for (int r = 0; r < reps; ++r) apply_op(X(), AP(), false, false);
+    for (int r = 0; r < reps; ++r) apply_op(X_.as(), AP_.as(), false, false);
     LC_HIP(hipEventRecord(e1, c_.stream()));
     LC_HIP(hipEventSynchronize(e1));
     float ms = 0.0f;
@@ -678,7 +712,6 @@ std::vector<Batch2D::PassTiming> Batch2D::profile_passes(int reps) {
                        dstate_.as<CgState>(), nullptr, mi.as<int>(), 0.0, 0.0, nprob());
     LC_HIP(hipGetLastError());
     CgState *st = dstate_.as<CgState>();
-    float2 *U = U_.as<float2>();
     const int nk = (int)classes_.size();
     std::vector<hipEvent_t> ev(4 * nk);
     for (auto &e : ev) LC_HIP(hipEventCreate(&e));
@@ -687,11 +720,22 @@ std::vector<Batch2D::PassTiming> Batch2D::profile_passes(int reps) {
         for (int i = 0; i < nk; ++i) {
             const ToepClass &k = *classes_[i];
             LC_HIP(hipEventRecord(ev[4 * i], c_.stream()));
-            toep_launch_fwd(c_, k, st, P(), R(), P(), U, true);
+            LC_ETYPE([&](auto z) {
+                using R = decltype(z);
+                toep_launch_fwd<R>(c_, k, st, P_.as<R>(), R_.as<R>(), P_.as<R>(), U_.as<cpx<R>>(),
+                                   true);
+            });
             LC_HIP(hipEventRecord(ev[4 * i + 1], c_.stream()));
-            toep_mid(k, st, U);
+            LC_ETYPE([&](auto z) {
+                using R = decltype(z);
+                toep_mid<R>(k, st, U_.as<cpx<R>>());
+            });
             LC_HIP(hipEventRecord(ev[4 * i + 2], c_.stream()));
-            toep_launch_inv(c_, k, st, U, AP(), P(), partial_.as<double>(), true);
+            LC_ETYPE([&](auto z) {
+                using R = decltype(z);
+                toep_launch_inv<R>(c_, k, st, U_.as<cpx<R>>(), AP_.as<R>(), P_.as<R>(),
+                                   partial_.as<double>(), true);
+            });
             LC_HIP(hipEventRecord(ev[4 * i + 3], c_.stream()));
         }
         LC_HIP(hipEventSynchronize(ev[4 * nk - 1]));
@@ -705,13 +749,15 @@ std::vector<Batch2D::PassTiming> Batch2D::profile_passes(int reps) {
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
     std::vector<PassTiming> out;
+    // element bytes e (4 or 8): real vectors e, complex 2e
+    const double e = (double)esize();
     for (int i = 0; i < nk; ++i) {
         const ToepClass &k = *classes_[i];
         const double N = k.N, H = k.H, M = k.M, ns = k.nslices;
-        double bytes[3] = {ns * (12.0 * N * N + 8.0 * N * H), ns * 16.0 * N * H + 4.0 * M * H,
-                           ns * (8.0 * N * H + 8.0 * N * N)};
+        double bytes[3] = {ns * (3 * e * N * N + 2 * e * N * H), ns * 4 * e * N * H + e * M * H,
+                           ns * (2 * e * N * H + 2 * e * N * N)};
         if (k.g3)   // column FFTs to and from W, slice-axis pass over W, 3-D spectrum
-            bytes[1] = ns * (16.0 * N * H + 16.0 * M * H) + 16.0 * M * H * ns + 4.0 * M * H * k.M1;
+            bytes[1] = ns * (4 * e * N * H + 4 * e * M * H) + 4 * e * M * H * ns + e * M * H * k.M1;
         for (int q = 0; q < 3; ++q)
             out.push_back(PassTiming{k.N, q, k.nslices, acc[3 * i + q] / reps, bytes[q]});
     }
@@ -772,19 +818,23 @@ void Batch2D::cg_setup(bool x0_zero, bool sb, const std::vector<int> &max_iter, 
     (void)tol;
     (void)xtol;
     CgState *st = dstate_.as<CgState>();
-    if (!x0_zero) apply_op(X(), AP(), sb, false);
-    hipLaunchKernelGGL((k_cg_init<256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
-                       dchunk_.as<ChunkRef>(), st, Y(), AP(), R(), P(), X(), cpartial_.as<double>(),
-                       x0_zero ? 1 : 0);
+    if (!x0_zero) apply_op(X_.as(), AP_.as(), sb, false);
+    LC_ETYPE([&](auto z) {
+        using R = decltype(z);
+        hipLaunchKernelGGL((k_cg_init<R, 256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
+                           dchunk_.as<ChunkRef>(), st, Y_.as<R>(), AP_.as<R>(), R_.as<R>(),
+                           P_.as<R>(), X_.as<R>(), cpartial_.as<double>(), x0_zero ? 1 : 0);
+    });
     LC_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_cg_fin0, dim3(nprob()), dim3(64), 0, c_.stream(), dprob_.as<ProbDev>(),
                        st, cpartial_.as<double>());
     LC_HIP(hipGetLastError());
 }
 
-void Batch2D::cg_iteration(bool host_sync, bool sb) {
+template <typename R> void Batch2D::cg_iteration_t(bool host_sync, bool sb) {
     CgState *st = dstate_.as<CgState>();
-    float2 *U = U_.as<float2>();
+    cpx<R> *U = U_.as<cpx<R>>();
+    R *X = X_.as<R>(), *Rv = R_.as<R>(), *P = P_.as<R>(), *AP = AP_.as<R>();
     auto hs = [&]() {
         if (host_sync) {
             std::vector<CgState> h(nprob());
@@ -794,13 +844,13 @@ void Batch2D::cg_iteration(bool host_sync, bool sb) {
         }
     };
     for (auto &k : classes_) {
-        toep_launch_fwd(c_, *k, st, P(), R(), P(), U, true);
-        toep_mid(*k, st, U);
-        toep_launch_inv(c_, *k, st, U, AP(), P(), partial_.as<double>(), !sb);
+        toep_launch_fwd<R>(c_, *k, st, P, Rv, P, U, true);
+        toep_mid<R>(*k, st, U);
+        toep_launch_inv<R>(c_, *k, st, U, AP, P, partial_.as<double>(), !sb);
     }
     if (sb) {
-        hipLaunchKernelGGL((k_sb_lapdot<256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
-                           dchunk_.as<ChunkRef>(), dprob_.as<ProbDev>(), st, P(), AP(),
+        hipLaunchKernelGGL((k_sb_lapdot<R, 256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
+                           dchunk_.as<ChunkRef>(), dprob_.as<ProbDev>(), st, P, AP,
                            cpartial_.as<double>(), 1);
         LC_HIP(hipGetLastError());
     }
@@ -808,13 +858,17 @@ void Batch2D::cg_iteration(bool host_sync, bool sb) {
                        st, partial_.as<double>(), cpartial_.as<double>(), sb ? 1 : 0);
     LC_HIP(hipGetLastError());
     hs();
-    hipLaunchKernelGGL((k_cg_update<256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
-                       dchunk_.as<ChunkRef>(), st, X(), R(), P(), AP(), cpartial_.as<double>());
+    hipLaunchKernelGGL((k_cg_update<R, 256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
+                       dchunk_.as<ChunkRef>(), st, X, Rv, P, AP, cpartial_.as<double>());
     LC_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_cg_fin2, dim3(nprob()), dim3(64), 0, c_.stream(), dprob_.as<ProbDev>(),
                        st, cpartial_.as<double>());
     LC_HIP(hipGetLastError());
     hs();
+}
+
+void Batch2D::cg_iteration(bool host_sync, bool sb) {
+    LC_ETYPE([&](auto z) { cg_iteration_t<decltype(z)>(host_sync, sb); });
 }
 
 void Batch2D::cg(const std::vector<int> &max_iter, double tol, double xtol, bool graph,
@@ -857,6 +911,34 @@ void Batch2D::cg(const std::vector<int> &max_iter, double tol, double xtol, bool
     c_.sync();
 }
 
+// one Split-Bregman outer iteration (src/bregman.cpp:58-88) for every active problem
+template <typename R>
+void Batch2D::sb_outer_t(bool hsync, double tol, double xtol, int inner, const int *dmi,
+                         const std::vector<int> &mi) {
+    const int np = nprob();
+    const SbState *sbs = dsb_.as<SbState>();
+    // rhs = y0 - mu div(d - b)
+    hipLaunchKernelGGL((k_sb_dmb<R, 256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
+                       dchunk_.as<ChunkRef>(), sbs, Dv_.as<R>(), Bv_.as<R>(), T_.as<R>(), total_);
+    hipLaunchKernelGGL((k_sb_rhs<R, 256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
+                       dchunk_.as<ChunkRef>(), dprob_.as<ProbDev>(), sbs, Y0_.as<R>(), Y_.as<R>(),
+                       T_.as<R>(), total_);
+    LC_HIP(hipGetLastError());
+    // x = cgsolver(A - mu lap, rhs, x, inner)
+    hipLaunchKernelGGL(k_cg_arm, dim3(ceil_div(np, 64)), dim3(64), 0, c_.stream(),
+                       dstate_.as<CgState>(), sbs, dmi, tol, xtol, np);
+    cg_setup(false, true, mi, tol, xtol);
+    for (int it = 0; it < inner; ++it) cg_iteration_t<R>(hsync, true);
+    // shrink + Bregman update + convergence
+    hipLaunchKernelGGL((k_sb_shrink<R, 256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
+                       dchunk_.as<ChunkRef>(), dprob_.as<ProbDev>(), sbs, X_.as<R>(), Dv_.as<R>(),
+                       Bv_.as<R>(), XO_.as<R>(), cpartial_.as<double>(), total_,
+                       sb_aniso_ ? 1 : 0);
+    hipLaunchKernelGGL(k_sb_fin, dim3(np), dim3(64), 0, c_.stream(), dprob_.as<ProbDev>(),
+                       dsb_.as<SbState>(), cpartial_.as<double>());
+    LC_HIP(hipGetLastError());
+}
+
 void Batch2D::split_bregman(const std::vector<float> &lambda, const std::vector<float> &mu,
                             int outer, int inner, double tol, double xtol, int boundary,
                             bool aniso, bool graph, bool host_sync) {
@@ -870,19 +952,23 @@ void Batch2D::split_bregman(const std::vector<float> &lambda, const std::vector<
     }
     LC_HIP(hipMemcpyAsync(dprob_.as(), hprob_.data(), sizeof(ProbDev) * np, hipMemcpyHostToDevice,
                           c_.stream()));
-    const size_t fb = sizeof(float) * (size_t)total_;
+    const size_t fb = esize() * (size_t)total_;
     Dv_.reserve(3 * fb);
     Bv_.reserve(3 * fb);
     XO_.reserve(fb);
     Y0_.reserve(fb);
     T_.reserve(3 * fb);
     // d = b = 0, x_old = x0, y0 = A^T y
-    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, c_.stream(), Dv_.as<float>(), 0.0f,
-                       3 * total_);
-    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, c_.stream(), Bv_.as<float>(), 0.0f,
-                       3 * total_);
-    LC_HIP(hipMemcpyAsync(XO_.as(), X(), fb, hipMemcpyDeviceToDevice, c_.stream()));
-    LC_HIP(hipMemcpyAsync(Y0_.as(), Y(), fb, hipMemcpyDeviceToDevice, c_.stream()));
+    LC_ETYPE([&](auto z) {
+        using R = decltype(z);
+        hipLaunchKernelGGL(k_fill<R>, dim3(1024), dim3(256), 0, c_.stream(), Dv_.as<R>(), (R)0,
+                           3 * total_);
+        hipLaunchKernelGGL(k_fill<R>, dim3(1024), dim3(256), 0, c_.stream(), Bv_.as<R>(), (R)0,
+                           3 * total_);
+    });
+    LC_HIP(hipGetLastError());
+    LC_HIP(hipMemcpyAsync(XO_.as(), X_.as(), fb, hipMemcpyDeviceToDevice, c_.stream()));
+    LC_HIP(hipMemcpyAsync(Y0_.as(), Y_.as(), fb, hipMemcpyDeviceToDevice, c_.stream()));
     std::vector<SbState> hs(np);
     for (int p = 0; p < np; ++p) {
         std::memset(&hs[p], 0, sizeof(SbState));
@@ -898,29 +984,10 @@ void Batch2D::split_bregman(const std::vector<float> &lambda, const std::vector<
     DBuf dmi(sizeof(int) * np);
     LC_HIP(hipMemcpyAsync(dmi.as(), mi.data(), sizeof(int) * np, hipMemcpyHostToDevice,
                           c_.stream()));
-    const SbState *sbs = dsb_.as<SbState>();
     auto outer_iter = [&](bool hsync) {
-        // rhs = y0 - mu div(d - b)
-        hipLaunchKernelGGL((k_sb_dmb<256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
-                           dchunk_.as<ChunkRef>(), sbs, Dv_.as<float>(), Bv_.as<float>(),
-                           T_.as<float>(), total_);
-        hipLaunchKernelGGL((k_sb_rhs<256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
-                           dchunk_.as<ChunkRef>(), dprob_.as<ProbDev>(), sbs, Y0_.as<float>(),
-                           Dv_.as<float>(), Bv_.as<float>(), Y(), T_.as<float>(), total_);
-        LC_HIP(hipGetLastError());
-        // x = cgsolver(A - mu lap, rhs, x, inner)
-        hipLaunchKernelGGL(k_cg_arm, dim3(ceil_div(np, 64)), dim3(64), 0, c_.stream(),
-                           dstate_.as<CgState>(), sbs, dmi.as<int>(), tol, xtol, np);
-        cg_setup(false, true, mi, tol, xtol);
-        for (int it = 0; it < inner; ++it) cg_iteration(hsync, true);
-        // shrink + Bregman update + convergence
-        hipLaunchKernelGGL((k_sb_shrink<256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
-                           dchunk_.as<ChunkRef>(), dprob_.as<ProbDev>(), sbs, X(), Dv_.as<float>(),
-                           Bv_.as<float>(), XO_.as<float>(), cpartial_.as<double>(), total_,
-                           aniso ? 1 : 0);
-        hipLaunchKernelGGL(k_sb_fin, dim3(np), dim3(64), 0, c_.stream(), dprob_.as<ProbDev>(),
-                           dsb_.as<SbState>(), cpartial_.as<double>());
-        LC_HIP(hipGetLastError());
+        LC_ETYPE([&](auto z) {
+            sb_outer_t<decltype(z)>(hsync, tol, xtol, inner, dmi.as<int>(), mi);
+        });
     };
     if (graph && !host_sync) {
         hipGraph_t g;
@@ -944,7 +1011,7 @@ void Batch2D::split_bregman(const std::vector<float> &lambda, const std::vector<
         }
     }
     // restore the CG rhs buffer to A^T y
-    LC_HIP(hipMemcpyAsync(Y(), Y0_.as(), fb, hipMemcpyDeviceToDevice, c_.stream()));
+    LC_HIP(hipMemcpyAsync(Y_.as(), Y0_.as(), fb, hipMemcpyDeviceToDevice, c_.stream()));
     c_.sync();
 }
 
@@ -963,5 +1030,7 @@ std::vector<SbState> Batch2D::sb_state() {
     c_.sync();
     return h;
 }
+
+#undef LC_ETYPE
 
 } // namespace lc

@@ -24,6 +24,22 @@ template <typename C> __host__ __device__ __forceinline__ C cmake(decltype(C{}.x
     z.y = b;
     return z;
 }
+
+// four-wide vector of the element type (block copies of two complex values)
+template <typename T> struct vec4_t;
+template <> struct vec4_t<float> { using type = float4; };
+template <> struct vec4_t<double> { using type = double4; };
+template <typename T> using vec4 = typename vec4_t<T>::type;
+template <typename V>
+__host__ __device__ __forceinline__ V vmake4(decltype(V{}.x) a, decltype(V{}.x) b,
+                                             decltype(V{}.x) c, decltype(V{}.x) d) {
+    V v;
+    v.x = a;
+    v.y = b;
+    v.z = c;
+    v.w = d;
+    return v;
+}
 __host__ __device__ __forceinline__ float2 operator+(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __host__ __device__ __forceinline__ float2 operator-(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __host__ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {

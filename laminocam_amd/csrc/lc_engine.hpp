@@ -32,7 +32,8 @@ struct Geom2D {
     std::vector<double> theta;
     DBuf d_theta;      // double[nang]
     DBuf khatT;        // float[H][M]: real spectrum of the centred PSF / (M^2 N)
-    bool psf_ready = false;
+    DBuf khatT64;      // double[H][M]: the same spectrum for FP64 solves (built on demand)
+    bool psf_ready = false, psf64_ready = false;
     Nufft2D nu;
 };
 
@@ -96,7 +97,8 @@ struct Geom3D {
     std::vector<double> rot;          // per projection: u (3) then v (3)
     DBuf d_rot;                       // double[nproj][6]
     DBuf khat3;                       // float[M][H][M1]: real spectrum of the centred PSF / (M1 M^2 N^2)
-    bool psf_ready = false;
+    DBuf khat3_64;                    // double[M][H][M1]: the same for FP64 solves (on demand)
+    bool psf_ready = false, psf64_ready = false;
     Nufft3D nu;
 };
 
@@ -109,15 +111,16 @@ void sysmat3d_nufft_dev(Ctx &c, Geom3D &g, const float *x, float *y);
 void geom3d_init(Ctx &c, Geom3D &g, const double *theta, const double *gamma, const double *beta,
                  int64_t nproj, int64_t nrows, int64_t ncols, const int64_t *recon_dims);
 // FP64 closed-form PSF P(m) = sum_i D_ncols(m.u_i) D_nrows(m.v_i), FP64 3-D FFT,
-// stored as the FP32 real spectrum
-void geom3d_build_psf(Ctx &c, Geom3D &g);
+// stored as the FP32 real spectrum (f64: as the FP64 spectrum khat3_64)
+void geom3d_build_psf(Ctx &c, Geom3D &g, bool f64 = false);
 // reference-layout kernel_hat (complex {M1, M, M/2+1}, unscaled) to host
 void geom3d_kernel_hat(Ctx &c, Geom3D &g, float *out_host);
 
 void geom2d_init(Ctx &c, Geom2D &g, const double *theta, int64_t nang, int64_t nrad,
                  int64_t image_n);
-// FP64 closed-form PSF + FP64 FFT, stored as FP32 real spectrum
-void geom2d_build_psf(Ctx &c, Geom2D &g);
+// FP64 closed-form PSF + FP64 FFT, stored as FP32 real spectrum (f64: as the
+// FP64 spectrum khatT64)
+void geom2d_build_psf(Ctx &c, Geom2D &g, bool f64 = false);
 // copy out the reference-layout kernel_hat (complex {M, M/2+1}, unscaled)
 void geom2d_kernel_hat(Ctx &c, Geom2D &g, float *out_host);
 
@@ -132,7 +135,7 @@ struct SliceRef {
 };
 
 struct ProbDev {
-    const float *khatT;
+    const void *khat;      // 2-D: spectrum [H][M] of the batch's element type
     int64_t off;    // float offset of the problem in the vector buffers
     int64_t n;      // ns * N * N
     int N, ns;
@@ -196,9 +199,11 @@ struct ToepClass {
     int n1 = 0, M1 = 0, kb2 = 1;      // slices, slice-axis FFT length, k2 rows per block
     FftPlan plan1{};
     const float2 *tw1 = nullptr;
+    const double2 *tw1_64 = nullptr;
     DBuf W;                           // [nblk][M][n1][B] complex
     FftPlan plan{};
-    const float2 *tw = nullptr;
+    const float2 *tw = nullptr;       // FFT_M twiddles, FP32 batches
+    const double2 *tw64 = nullptr;    // FFT_M twiddles, FP64 batches
     int nslices = 0;
     DBuf slices;            // SliceRef[nslices]
     int ppb = 1, p1_blocks = 0;   // row pairs per block, blocks per slice
@@ -208,6 +213,10 @@ struct ToepClass {
     ToepVariant var{};
 };
 
+template <typename R> inline const cpx<R> *toep_tw(const ToepClass &k);
+template <> inline const float2 *toep_tw<float>(const ToepClass &k) { return k.tw; }
+template <> inline const double2 *toep_tw<double>(const ToepClass &k) { return k.tw64; }
+
 // block shapes for a class; U elements per slice (independent of the shapes)
 void toep_configure(ToepClass &k, ToepVariant v);
 ToepVariant toep_default_variant(int M);
@@ -215,6 +224,8 @@ int64_t toep_u_size(const ToepClass &k);
 int toep_num_row_shapes();
 int toep_num_col_shapes();
 int toep_col_cap(int col);
+// largest column shape index usable with 8-byte (FP32) or 16-byte (FP64) complex
+int toep_num_col_shapes_for(bool f64);
 // time candidate shapes for image size g.N on `nslices` slices; installs the
 // fastest in c.toep_tuning and returns it (best_ms: time of one apply)
 ToepVariant toep_autotune(Ctx &c, Geom2D &g, int nslices, double *best_ms);
@@ -227,18 +238,32 @@ struct ProbSpec {
 
 // middle Toeplitz pass of a 3-D volume class (three launches) and its
 // block-size limits (B * M for the column FFTs, B * M1 for the slice axis)
-void toep3d_mid(Ctx &c, const ToepClass &k, const CgState *st, float2 *U);
-int toep3d_col_cap();
-int toep3d_slice_cap();
+template <typename R> void toep3d_mid(Ctx &c, const ToepClass &k, const CgState *st, cpx<R> *U);
+int toep3d_col_cap(bool f64 = false);
+int toep3d_slice_cap(bool f64 = false);
+
+// the three Toeplitz passes of one class, element type R (float or double)
+template <typename R>
+void toep_launch_fwd(Ctx &c, const ToepClass &k, const CgState *st, const R *src, const R *rsrc,
+                     R *pdst, cpx<R> *U, bool fuse);
+template <typename R>
+void toep_launch_cols(Ctx &c, const ToepClass &k, const ProbDev *pd, const CgState *st, cpx<R> *U);
+template <typename R>
+void toep_launch_inv(Ctx &c, const ToepClass &k, const CgState *st, const cpx<R> *U, R *dst,
+                     const R *pvec, double *partial, bool dot);
 
 // ---------------------------------------------------------------------------
 // A batch of independent 2-D problems (each a stack of ns slices that share
 // one geometry).  Vectors of all problems are concatenated; scalars are kept
 // per problem so that a problem's iterates are the same alone and in a batch.
+// f64 = true: vectors, spectra and FFTs are FP64 (the X() ... Y() pointers
+// then address doubles; use the *v() accessors).
 // ---------------------------------------------------------------------------
 class Batch2D {
   public:
-    Batch2D(Ctx &c, const std::vector<ProbSpec> &probs);
+    Batch2D(Ctx &c, const std::vector<ProbSpec> &probs, bool f64 = false);
+    bool f64() const { return f64_; }
+    size_t esize() const { return f64_ ? sizeof(double) : sizeof(float); }
     int nprob() const { return (int)probs_.size(); }
     int64_t total() const { return total_; }
     int64_t offset(int p) const { return hprob_[p].off; }
@@ -250,9 +275,14 @@ class Batch2D {
     float *P() const { return P_.as<float>(); }
     float *AP() const { return AP_.as<float>(); }
     float *Y() const { return Y_.as<float>(); }
+    // element-type-neutral views; elem(p, off) = address of element off of p
+    void *Xv() const { return X_.as(); }
+    void *Yv() const { return Y_.as(); }
+    void *APv() const { return AP_.as(); }
+    void *elem(void *p, int64_t off) const { return static_cast<char *>(p) + off * (int64_t)esize(); }
 
     // y = A x for every problem (Toeplitz only; src/dst are full vectors)
-    void toeplitz(const float *src, float *dst);
+    void toeplitz(const void *src, void *dst);
     // average time (ms) of one Toeplitz apply over all problems (X -> AP)
     float time_apply(int reps);
     // HIP-event time of every launch of the three Toeplitz passes in the form a
@@ -284,13 +314,18 @@ class Batch2D {
     void build();
     // one CG iteration (all kernels) for the current states
     void cg_iteration(bool host_sync, bool sb);
+    template <typename R> void cg_iteration_t(bool host_sync, bool sb);
     void cg_setup(bool x0_zero, bool sb, const std::vector<int> &max_iter, double tol,
                   double xtol);
-    void apply_op(const float *src, float *dst, bool sb, bool dot);
+    void apply_op(const void *src, void *dst, bool sb, bool dot);
+    template <typename R> void apply_op_t(const R *src, R *dst, bool sb, bool dot);
     // middle pass of a class: 2-D column pass or the 3-D slice-coupled pass
-    void toep_mid(const ToepClass &k, const CgState *st, float2 *U);
+    template <typename R> void toep_mid(const ToepClass &k, const CgState *st, cpx<R> *U);
+    template <typename R> void sb_outer_t(bool hsync, double tol, double xtol, int inner,
+                                          const int *dmi, const std::vector<int> &mi);
 
     Ctx &c_;
+    bool f64_ = false;
     std::vector<ProbSpec> probs_;
     std::vector<ProbDev> hprob_;
     std::vector<ChunkRef> hchunk_;
@@ -308,5 +343,6 @@ class Batch2D {
 // finite differences on one volume: op 0 grad (3 components out), 1 div
 // (3 components in), 2 laplacian.  boundary 0 interior only, 1 zero padded.
 void fd_launch(Ctx &c, int op, const float *in, float *out, int n1, int n2, int n3, int boundary);
+void fd_launch(Ctx &c, int op, const double *in, double *out, int n1, int n2, int n3, int boundary);
 
 } // namespace lc

@@ -68,6 +68,7 @@ void geom3d_init(Ctx &c, Geom3D &g, const double *theta, const double *gamma, co
                           hipMemcpyHostToDevice, c.stream()));
     c.sync();
     g.psf_ready = false;
+    g.psf64_ready = false;
 }
 
 // ---------------------------------------------------------------------------
@@ -199,10 +200,10 @@ k_psf3_cols(double2 *P, int M, int ncol, int B, FftPlan plan, const double2 *__r
 }
 
 // FP64 FFT along z of columns (k2, k3 < H), real part x scale ->
-// khat3[k2][k3][k1] (float).  grid (ceil(H / B), M)
-template <int NT, int MAXE>
+// khat3[k2][k3][k1] (OT: float or double).  grid (ceil(H / B), M)
+template <typename OT, int NT, int MAXE>
 __global__ void __launch_bounds__(NT)
-k_psf3_slices(const double2 *P, float *khat3, int M1, int M, int H, int B, FftPlan plan,
+k_psf3_slices(const double2 *P, OT *khat3, int M1, int M, int H, int B, FftPlan plan,
               const double2 *__restrict__ tw, double scale) {
     double2 *buf = reinterpret_cast<double2 *>(lc_smem);
     const int k30 = blockIdx.x * B;
@@ -216,11 +217,11 @@ k_psf3_slices(const double2 *P, float *khat3, int M1, int M, int H, int B, FftPl
     for (int idx = threadIdx.x; idx < M1 * B; idx += NT) {
         const int c = idx / M1, z = idx - c * M1;
         if (k30 + c < H)
-            khat3[((size_t)k2 * H + k30 + c) * M1 + z] = (float)(buf[lds_col(c, z, B)].x * scale);
+            khat3[((size_t)k2 * H + k30 + c) * M1 + z] = (OT)(buf[lds_col(c, z, B)].x * scale);
     }
 }
 
-void geom3d_build_psf(Ctx &c, Geom3D &g) {
+void geom3d_build_psf(Ctx &c, Geom3D &g, bool f64) {
     if (g.n2 != g.n3)
         throw Error(LC_EUNSUP, "3-D Toeplitz operator: recon dims n2 != n3 are not supported");
     const int M1 = (int)g.M1, M = (int)g.M, H = (int)g.H;
@@ -252,17 +253,25 @@ void geom3d_build_psf(Ctx &c, Geom3D &g) {
         LC_HIP(hipGetLastError());
     }
     // z + extraction
-    g.khat3.alloc(sizeof(float) * (size_t)M * H * M1);
     {
         const int B = std::max(1, std::min(NT * MAXE / M1, 64));
         const double scale = 1.0 / ((double)M1 * M * M * (double)g.N * (double)g.N);
-        hipLaunchKernelGGL((k_psf3_slices<NT, MAXE>), dim3(ceil_div(H, B), M), dim3(NT),
-                           sizeof(double2) * lds_round(B * M1), c.stream(), P, g.khat3.as<float>(),
-                           M1, M, H, B, plan1, tw1, scale);
+        const size_t sh = sizeof(double2) * lds_round(B * M1);
+        const dim3 grid(ceil_div(H, B), M);
+        if (f64) {
+            g.khat3_64.alloc(sizeof(double) * (size_t)M * H * M1);
+            hipLaunchKernelGGL((k_psf3_slices<double, NT, MAXE>), grid, dim3(NT), sh, c.stream(), P,
+                               g.khat3_64.as<double>(), M1, M, H, B, plan1, tw1, scale);
+        } else {
+            g.khat3.alloc(sizeof(float) * (size_t)M * H * M1);
+            hipLaunchKernelGGL((k_psf3_slices<float, NT, MAXE>), grid, dim3(NT), sh, c.stream(), P,
+                               g.khat3.as<float>(), M1, M, H, B, plan1, tw1, scale);
+        }
         LC_HIP(hipGetLastError());
     }
     c.sync();
-    g.psf_ready = true;
+    if (f64) g.psf64_ready = true;
+    else g.psf_ready = true;
 }
 
 // reference-layout kernel_hat: FP64 FFT along z of the (x, y)-transformed full
@@ -340,34 +349,38 @@ void geom3d_kernel_hat(Ctx &c, Geom3D &g, float *out_host) {
 // ---------------------------------------------------------------------------
 // 3-D Toeplitz middle pass
 // ---------------------------------------------------------------------------
-constexpr int T3_NT = 256, T3_ME = 32;    // column FFTs: B * M <= 8192
-constexpr int T3B_NT = 256, T3B_ME = 16;  // slice-axis FFTs: kb2 * B * M1 <= 4096
+// FP32: column FFTs B * M <= 8192, slice-axis FFTs kb2 * B * M1 <= 4096.
+// FP64 (16-byte complex) keeps every block within 64 KiB of LDS: 4096 / 4096.
+constexpr int T3_NT = 256, T3_ME = 32;
+constexpr int T3B_NT = 256, T3B_ME = 16;
+template <typename R> constexpr int t3_me() { return sizeof(R) == 4 ? T3_ME : T3_ME / 2; }
 
 __device__ __forceinline__ bool prob_active3(const CgState *st, int prob) {
     return st == nullptr || st[prob].active != 0;
 }
 
 // P2f: U block (N x B, slice i1) -> FFT_M down columns -> W[kb][k2][i1][c]
-template <int NT, int MAXE>
+template <typename R, int NT, int MAXE>
 __global__ void __launch_bounds__(NT)
 k_t3_cols_fwd(const SliceRef *__restrict__ sl, const CgState *__restrict__ st,
-              const float2 *__restrict__ U, float2 *__restrict__ W, ToepGeom g, int n1, FftPlan plan,
-              const float2 *__restrict__ tw) {
+              const cpx<R> *__restrict__ U, cpx<R> *__restrict__ W, ToepGeom g, int n1, FftPlan plan,
+              const cpx<R> *__restrict__ tw) {
+    using C = cpx<R>;
     const int lin = blockIdx.x + blockIdx.y * gridDim.x;
     const int logical = xcd_remap(lin, gridDim.x * gridDim.y);
     const int kb = logical / n1, i1 = logical - kb * n1;
     const SliceRef s = sl[i1];
     if (!prob_active3(st, s.prob)) return;
     const int N = g.N, M = g.M, B = g.B;
-    float2 *buf = reinterpret_cast<float2 *>(lc_smem);
-    const float2 *blk = U + s.uoff + (int64_t)kb * N * B;
+    C *buf = reinterpret_cast<C *>(lc_smem);
+    const C *blk = U + s.uoff + (int64_t)kb * N * B;
     for (int idx = threadIdx.x; idx < M * B; idx += NT) {
         const int i = idx / B, c = idx - i * B;
-        buf[lds_col(c, i, B)] = i < N ? blk[idx] : make_float2(0.f, 0.f);
+        buf[lds_col(c, i, B)] = i < N ? blk[idx] : cmake<C>(0, 0);
     }
     __syncthreads();
-    fft_lds<float, -1, NT, MAXE, true>(buf, B, 0, plan, tw);
-    float2 *w = W + (int64_t)kb * M * n1 * B;
+    fft_lds<R, -1, NT, MAXE, true>(buf, B, 0, plan, tw);
+    C *w = W + (int64_t)kb * M * n1 * B;
     for (int idx = threadIdx.x; idx < M * B; idx += NT) {
         const int k2 = idx / B, c = idx - k2 * B;
         w[((int64_t)k2 * n1 + i1) * B + c] = buf[lds_col(c, k2, B)];
@@ -377,37 +390,38 @@ k_t3_cols_fwd(const SliceRef *__restrict__ sl, const CgState *__restrict__ st,
 // P2b: for kb2 consecutive k2 of column block kb: FFT_M1 along the slices
 // (n1 values, zero padded), x khat3[k2][k3][k1], inverse, keep n1.
 // Lines (j, c) -> line j * B + c.  grid (nblk, ceil(M / kb2))
-template <int NT, int MAXE>
+template <typename R, int NT, int MAXE>
 __global__ void __launch_bounds__(NT)
-k_t3_slices(const SliceRef *__restrict__ sl, const CgState *__restrict__ st, float2 *__restrict__ W,
-            const float *__restrict__ khat3, ToepGeom g, int n1, int M1, int kb2, FftPlan plan1,
-            const float2 *__restrict__ tw1) {
+k_t3_slices(const SliceRef *__restrict__ sl, const CgState *__restrict__ st, cpx<R> *__restrict__ W,
+            const R *__restrict__ khat3, ToepGeom g, int n1, int M1, int kb2, FftPlan plan1,
+            const cpx<R> *__restrict__ tw1) {
+    using C = cpx<R>;
     if (!prob_active3(st, sl[0].prob)) return;
     const int M = g.M, H = g.H, B = g.B;
     const int kb = blockIdx.x, k20 = blockIdx.y * kb2;
     const int nk2 = min(kb2, M - k20);
     const int nl = kb2 * B;
-    float2 *buf = reinterpret_cast<float2 *>(lc_smem);
-    float2 *w = W + ((int64_t)kb * M + k20) * n1 * B;   // tile [j][i1][c], contiguous
+    C *buf = reinterpret_cast<C *>(lc_smem);
+    C *w = W + ((int64_t)kb * M + k20) * n1 * B;   // tile [j][i1][c], contiguous
     for (int idx = threadIdx.x; idx < nl * M1; idx += NT) {
         const int e = idx / nl, line = idx - e * nl;     // e: slice index, line = j*B + c
         const int j = line / B, c = line - j * B;
-        float2 v = make_float2(0.f, 0.f);
+        C v = cmake<C>(0, 0);
         if (j < nk2 && e < n1) v = w[((int64_t)j * n1 + e) * B + c];
         buf[lds_col(line, e, nl)] = v;
     }
     __syncthreads();
-    fft_lds<float, -1, NT, MAXE, true>(buf, nl, 0, plan1, tw1);
+    fft_lds<R, -1, NT, MAXE, true>(buf, nl, 0, plan1, tw1);
     for (int idx = threadIdx.x; idx < nl * M1; idx += NT) {
         const int line = idx / M1, e = idx - line * M1;   // e fastest: contiguous khat reads
         const int j = line / B, c = line - j * B;
         const int k2 = k20 + j, k3 = kb * B + c;
-        const float kv = (j < nk2 && k3 < H) ? khat3[((int64_t)k2 * H + k3) * M1 + e] : 0.f;
-        float2 *p = buf + lds_col(line, e, nl);
+        const R kv = (j < nk2 && k3 < H) ? khat3[((int64_t)k2 * H + k3) * M1 + e] : (R)0;
+        C *p = buf + lds_col(line, e, nl);
         *p = cscale(*p, kv);
     }
     __syncthreads();
-    fft_lds<float, 1, NT, MAXE, true>(buf, nl, 0, plan1, tw1);
+    fft_lds<R, 1, NT, MAXE, true>(buf, nl, 0, plan1, tw1);
     for (int idx = threadIdx.x; idx < nl * n1; idx += NT) {
         const int e = idx / nl, line = idx - e * nl;
         const int j = line / B, c = line - j * B;
@@ -416,51 +430,70 @@ k_t3_slices(const SliceRef *__restrict__ sl, const CgState *__restrict__ st, flo
 }
 
 // P2i: W[kb][:][i1][:] -> inverse FFT_M down columns -> keep N rows -> U block
-template <int NT, int MAXE>
+template <typename R, int NT, int MAXE>
 __global__ void __launch_bounds__(NT)
 k_t3_cols_inv(const SliceRef *__restrict__ sl, const CgState *__restrict__ st,
-              const float2 *__restrict__ W, float2 *__restrict__ U, ToepGeom g, int n1, FftPlan plan,
-              const float2 *__restrict__ tw) {
+              const cpx<R> *__restrict__ W, cpx<R> *__restrict__ U, ToepGeom g, int n1, FftPlan plan,
+              const cpx<R> *__restrict__ tw) {
+    using C = cpx<R>;
     const int lin = blockIdx.x + blockIdx.y * gridDim.x;
     const int logical = xcd_remap(lin, gridDim.x * gridDim.y);
     const int kb = logical / n1, i1 = logical - kb * n1;
     const SliceRef s = sl[i1];
     if (!prob_active3(st, s.prob)) return;
     const int N = g.N, M = g.M, B = g.B;
-    float2 *buf = reinterpret_cast<float2 *>(lc_smem);
-    const float2 *w = W + (int64_t)kb * M * n1 * B;
+    C *buf = reinterpret_cast<C *>(lc_smem);
+    const C *w = W + (int64_t)kb * M * n1 * B;
     for (int idx = threadIdx.x; idx < M * B; idx += NT) {
         const int k2 = idx / B, c = idx - k2 * B;
         buf[lds_col(c, k2, B)] = w[((int64_t)k2 * n1 + i1) * B + c];
     }
     __syncthreads();
-    fft_lds<float, 1, NT, MAXE, true>(buf, B, 0, plan, tw);
-    float2 *blk = U + s.uoff + (int64_t)kb * N * B;
+    fft_lds<R, 1, NT, MAXE, true>(buf, B, 0, plan, tw);
+    C *blk = U + s.uoff + (int64_t)kb * N * B;
     for (int idx = threadIdx.x; idx < N * B; idx += NT) {
         const int i = idx / B, c = idx - i * B;
         blk[idx] = buf[lds_col(c, i, B)];
     }
 }
 
-void toep3d_mid(Ctx &c, const ToepClass &k, const CgState *st, float2 *U) {
+template <typename R> static const cpx<R> *t3_tw1(const ToepClass &k);
+template <> const float2 *t3_tw1<float>(const ToepClass &k) { return k.tw1; }
+template <> const double2 *t3_tw1<double>(const ToepClass &k) { return k.tw1_64; }
+template <typename R> static const R *t3_khat(const Geom3D &g);
+template <> const float *t3_khat<float>(const Geom3D &g) { return g.khat3.as<float>(); }
+template <> const double *t3_khat<double>(const Geom3D &g) { return g.khat3_64.as<double>(); }
+
+template <typename R> void toep3d_mid(Ctx &c, const ToepClass &k, const CgState *st, cpx<R> *U) {
+    using C = cpx<R>;
+    constexpr int ME = t3_me<R>();
+    const bool f64 = sizeof(R) == 8;
+    LC_REQUIRE(k.B * k.M <= toep3d_col_cap(f64) && k.kb2 * k.B * k.M1 <= toep3d_slice_cap(f64),
+               "toeplitz 3-D: block exceeds the LDS shape");
     const SliceRef *sl = k.slices.as<SliceRef>();
-    float2 *W = k.W.as<float2>();
-    const size_t shc = sizeof(float2) * lds_round(k.B * k.M);
-    hipLaunchKernelGGL((k_t3_cols_fwd<T3_NT, T3_ME>), dim3(k.nblk, k.n1), dim3(T3_NT), shc,
-                       c.stream(), sl, st, U, W, k.g, k.n1, k.plan, k.tw);
+    C *W = k.W.as<C>();
+    const C *tw = toep_tw<R>(k);
+    const size_t shc = sizeof(C) * lds_round(k.B * k.M);
+    hipLaunchKernelGGL((k_t3_cols_fwd<R, T3_NT, ME>), dim3(k.nblk, k.n1), dim3(T3_NT), shc,
+                       c.stream(), sl, st, U, W, k.g, k.n1, k.plan, tw);
     LC_HIP(hipGetLastError());
-    const size_t shb = sizeof(float2) * lds_round(k.kb2 * k.B * k.M1);
-    hipLaunchKernelGGL((k_t3_slices<T3B_NT, T3B_ME>), dim3(k.nblk, ceil_div(k.M, k.kb2)), dim3(T3B_NT),
-                       shb, c.stream(), sl, st, W, k.g3->khat3.as<float>(), k.g, k.n1, k.M1, k.kb2,
-                       k.plan1, k.tw1);
+    const size_t shb = sizeof(C) * lds_round(k.kb2 * k.B * k.M1);
+    hipLaunchKernelGGL((k_t3_slices<R, T3B_NT, T3B_ME>), dim3(k.nblk, ceil_div(k.M, k.kb2)),
+                       dim3(T3B_NT), shb, c.stream(), sl, st, W, t3_khat<R>(*k.g3), k.g, k.n1,
+                       k.M1, k.kb2, k.plan1, t3_tw1<R>(k));
     LC_HIP(hipGetLastError());
-    hipLaunchKernelGGL((k_t3_cols_inv<T3_NT, T3_ME>), dim3(k.nblk, k.n1), dim3(T3_NT), shc,
-                       c.stream(), sl, st, W, U, k.g, k.n1, k.plan, k.tw);
+    hipLaunchKernelGGL((k_t3_cols_inv<R, T3_NT, ME>), dim3(k.nblk, k.n1), dim3(T3_NT), shc,
+                       c.stream(), sl, st, W, U, k.g, k.n1, k.plan, tw);
     LC_HIP(hipGetLastError());
 }
+template void toep3d_mid<float>(Ctx &, const ToepClass &, const CgState *, float2 *);
+template void toep3d_mid<double>(Ctx &, const ToepClass &, const CgState *, double2 *);
 
 // shape limits of the 3-D middle pass for the class set-up
-int toep3d_col_cap() { return T3_NT * T3_ME; }
-int toep3d_slice_cap() { return T3B_NT * T3B_ME; }
+int toep3d_col_cap(bool f64) { return T3_NT * (f64 ? T3_ME / 2 : T3_ME); }
+int toep3d_slice_cap(bool f64) {
+    (void)f64;   // 4096 complex: 32 KiB (FP32) or 64 KiB (FP64) of LDS
+    return T3B_NT * T3B_ME;
+}
 
 } // namespace lc

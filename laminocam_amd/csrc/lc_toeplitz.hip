@@ -137,9 +137,9 @@ __global__ void __launch_bounds__(NT) k_rows_c2c(cpx<T> *data, int nrows, int nl
 }
 
 // FP64 column FFT of the row-transformed PSF, keeping columns kx < H:
-// khatT[kx][ky] = Re(FFT)[ky][kx] * scale (float)
-template <int NT, int MAXE>
-__global__ void __launch_bounds__(NT) k_psf_cols(const double2 *P, float *khatT, int M, int H, int B,
+// khatT[kx][ky] = Re(FFT)[ky][kx] * scale (stored as OT: float or double)
+template <typename OT, int NT, int MAXE>
+__global__ void __launch_bounds__(NT) k_psf_cols(const double2 *P, OT *khatT, int M, int H, int B,
                                                  FftPlan plan, const double2 *__restrict__ tw,
                                                  double scale) {
     double2 *buf = reinterpret_cast<double2 *>(lc_smem);
@@ -152,7 +152,7 @@ __global__ void __launch_bounds__(NT) k_psf_cols(const double2 *P, float *khatT,
     fft_lds<double, -1, NT, MAXE, true>(buf, B, 0, plan, tw);
     for (int idx = threadIdx.x; idx < M * B; idx += NT) {
         const int c = idx / M, i = idx - c * M;
-        if (col0 + c < H) khatT[(size_t)(col0 + c) * M + i] = (float)(buf[lds_col(c, i, B)].x * scale);
+        if (col0 + c < H) khatT[(size_t)(col0 + c) * M + i] = (OT)(buf[lds_col(c, i, B)].x * scale);
     }
 }
 
@@ -201,9 +201,10 @@ void geom2d_init(Ctx &c, Geom2D &g, const double *theta, int64_t nang, int64_t n
                           c.stream()));
     c.sync();
     g.psf_ready = false;
+    g.psf64_ready = false;
 }
 
-void geom2d_build_psf(Ctx &c, Geom2D &g) {
+void geom2d_build_psf(Ctx &c, Geom2D &g, bool f64) {
     const int M = (int)g.M, H = (int)g.H;
     FftPlan plan;
     LC_REQUIRE(fft_make_plan(M, plan), "PSF size is not 2^p 3^q");
@@ -220,14 +221,24 @@ void geom2d_build_psf(Ctx &c, Geom2D &g) {
     hipLaunchKernelGGL((k_rows_c2c<double, -1, NT, MAXE>), dim3(ceil_div(M, nl)), dim3(NT),
                        sizeof(double2) * lds_round(nl * M), c.stream(), P, M, nl, plan, tw);
     LC_HIP(hipGetLastError());
-    g.khatT.alloc(sizeof(float) * (size_t)H * M);
     const int B = std::max(1, NT * MAXE / M);
     const double scale = 1.0 / ((double)M * (double)M * (double)g.N);
-    hipLaunchKernelGGL((k_psf_cols<NT, MAXE>), dim3(ceil_div(H, B)), dim3(NT),
-                       sizeof(double2) * lds_round(B * M), c.stream(), P, g.khatT.as<float>(), M,
-                       H, B, plan, tw, scale);
+    if (f64) {
+        g.khatT64.alloc(sizeof(double) * (size_t)H * M);
+        hipLaunchKernelGGL((k_psf_cols<double, NT, MAXE>), dim3(ceil_div(H, B)), dim3(NT),
+                           sizeof(double2) * lds_round(B * M), c.stream(), P,
+                           g.khatT64.as<double>(), M, H, B, plan, tw, scale);
+    } else {
+        g.khatT.alloc(sizeof(float) * (size_t)H * M);
+        hipLaunchKernelGGL((k_psf_cols<float, NT, MAXE>), dim3(ceil_div(H, B)), dim3(NT),
+                           sizeof(double2) * lds_round(B * M), c.stream(), P, g.khatT.as<float>(),
+                           M, H, B, plan, tw, scale);
+    }
     LC_HIP(hipGetLastError());
-    g.psf_ready = true;
+    // P lives in a context scratch slot that the next call may reuse
+    c.sync();
+    if (f64) g.psf64_ready = true;
+    else g.psf_ready = true;
 }
 
 void geom2d_kernel_hat(Ctx &c, Geom2D &g, float *out_host) {
@@ -269,11 +280,14 @@ constexpr int N_ROW_SHAPES = 2, N_COL_SHAPES = 3;
 } // namespace
 
 #define LC_DECL_L(L)                                                                              \
-    void toep_fwd_L##L(Ctx &, const ToepClass &, const CgState *, const float *, const float *,   \
-                       float *, float2 *, bool);                                                  \
-    void toep_cols_L##L(Ctx &, const ToepClass &, const ProbDev *, const CgState *, float2 *);    \
-    void toep_inv_L##L(Ctx &, const ToepClass &, const CgState *, const float2 *, float *,        \
-                       const float *, double *, bool);
+    template <typename R>                                                                         \
+    void toep_fwd_L##L(Ctx &, const ToepClass &, const CgState *, const R *, const R *, R *,      \
+                       cpx<R> *, bool);                                                           \
+    template <typename R>                                                                         \
+    void toep_cols_L##L(Ctx &, const ToepClass &, const ProbDev *, const CgState *, cpx<R> *);    \
+    template <typename R>                                                                         \
+    void toep_inv_L##L(Ctx &, const ToepClass &, const CgState *, const cpx<R> *, R *, const R *, \
+                       double *, bool);
 LC_DECL_L(0)
 LC_DECL_L(256)
 LC_DECL_L(384)
@@ -286,6 +300,8 @@ LC_DECL_L(2048)
 int toep_num_row_shapes() { return N_ROW_SHAPES; }
 int toep_num_col_shapes() { return N_COL_SHAPES; }
 int toep_col_cap(int col) { return COL_SHAPES[col].nt * COL_SHAPES[col].maxe; }
+// 16-byte complex: shapes whose LDS block stays within 64 KiB (shape 2 does not)
+int toep_num_col_shapes_for(bool f64) { return f64 ? 2 : N_COL_SHAPES; }
 
 static int pow2_floor(int v) {
     int p = 1;
@@ -351,25 +367,38 @@ int64_t toep_u_size(const ToepClass &k) { return ((int64_t)(k.H + 16) * k.N + 1)
     default: CALL(0); break;                                                                      \
     }
 
-void toep_launch_fwd(Ctx &c, const ToepClass &k, const CgState *st, const float *src,
-                     const float *rsrc, float *pdst, float2 *U, bool fuse) {
-#define LC_F(L) toep_fwd_L##L(c, k, st, src, rsrc, pdst, U, fuse)
+template <typename R>
+void toep_launch_fwd(Ctx &c, const ToepClass &k, const CgState *st, const R *src, const R *rsrc,
+                     R *pdst, cpx<R> *U, bool fuse) {
+#define LC_F(L) toep_fwd_L##L<R>(c, k, st, src, rsrc, pdst, U, fuse)
     LC_DISPATCH_M(LC_F)
 #undef LC_F
 }
 
-void toep_launch_cols(Ctx &c, const ToepClass &k, const ProbDev *pd, const CgState *st,
-                      float2 *U) {
-#define LC_F(L) toep_cols_L##L(c, k, pd, st, U)
+template <typename R>
+void toep_launch_cols(Ctx &c, const ToepClass &k, const ProbDev *pd, const CgState *st, cpx<R> *U) {
+#define LC_F(L) toep_cols_L##L<R>(c, k, pd, st, U)
     LC_DISPATCH_M(LC_F)
 #undef LC_F
 }
 
-void toep_launch_inv(Ctx &c, const ToepClass &k, const CgState *st, const float2 *U, float *dst,
-                     const float *pvec, double *partial, bool dot) {
-#define LC_F(L) toep_inv_L##L(c, k, st, U, dst, pvec, partial, dot)
+template <typename R>
+void toep_launch_inv(Ctx &c, const ToepClass &k, const CgState *st, const cpx<R> *U, R *dst,
+                     const R *pvec, double *partial, bool dot) {
+#define LC_F(L) toep_inv_L##L<R>(c, k, st, U, dst, pvec, partial, dot)
     LC_DISPATCH_M(LC_F)
 #undef LC_F
 }
+
+#define LC_LAUNCH_INST(R)                                                                         \
+    template void toep_launch_fwd<R>(Ctx &, const ToepClass &, const CgState *, const R *,        \
+                                     const R *, R *, cpx<R> *, bool);                             \
+    template void toep_launch_cols<R>(Ctx &, const ToepClass &, const ProbDev *, const CgState *, \
+                                      cpx<R> *);                                                  \
+    template void toep_launch_inv<R>(Ctx &, const ToepClass &, const CgState *, const cpx<R> *,   \
+                                     R *, const R *, double *, bool);
+LC_LAUNCH_INST(float)
+LC_LAUNCH_INST(double)
+#undef LC_LAUNCH_INST
 
 } // namespace lc

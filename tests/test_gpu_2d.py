@@ -49,7 +49,7 @@ def test_toeplitz_matches_oracle(lc, orc64, N, nang, ns):
     announce(f"toeplitz N={N} nang={nang} ns={ns}")
     rng = np.random.default_rng(N * 1000 + nang)
     th = ref_theta(nang)
-    x = rng.random((ns, N, N))
+    x = rng.random((ns, N, N)).astype(np.float32)   # float32: the FP32 operator
     psf = lc.PointSpreadFunction2D(lc.PolarGrid2D(th, N), N)
     y = psf.convolve(x)
     yr = orc64.psf2d(th, N, N, method=0).convolve(x)
@@ -77,7 +77,9 @@ def _problem(orc64, N, nang, ns, seed):
     sino = orc64.forward2d(vol, th, N)
     sino = synth.add_gaussian_noise(sino, 0.01, rng).astype(np.float64)
     rhs = orc64.backproj2d(sino, th, N)
-    return th, vol, rhs
+    # float32 right-hand side: the solvers run their FP32 path (float64 input
+    # would select the FP64 one)
+    return th, vol, rhs.astype(np.float32)
 
 
 @pytest.mark.parametrize("N,nang,ns,iters", [(64, 90, 1, 5), (64, 90, 1, 8), (48, 60, 3, 6),
@@ -178,7 +180,7 @@ def test_batch_same_alone_and_in_mix(lc, orc64):
     for N, nang in sizes:
         th = synth.angles(nang)
         psfs.append(lc.PointSpreadFunction2D(lc.PolarGrid2D(th, N), N))
-        vol = synth.ellipse_phantom(N, 6, rng)[None]
+        vol = synth.ellipse_phantom(N, 6, rng)[None].astype(np.float32)
         rhss.append(psfs[-1].convolve(vol))
     iters = [20, 15, 25, 10, 30]
     batch = lc.Batch2D(psfs)

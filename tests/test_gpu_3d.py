@@ -49,7 +49,7 @@ def test_toeplitz3d_matches_oracle(lc, orc64, dims, nproj, nrows, ncols, tilt):
     rng = np.random.default_rng(sum(dims) + nproj)
     th, ga, be = lamino_angles(nproj, tilt)
     psf = lc.PointSpreadFunction(lc.PolarGrid(th, ga, be, nrows, ncols), dims)
-    x = rng.random(dims)
+    x = rng.random(dims).astype(np.float32)   # float32: the FP32 operator
     y = psf.convolve(x)
     yr = orc64.psf3d(th, ga, be, nrows, ncols, dims, method=0).convolve(x)
     err = rel_l2(y, yr)
@@ -120,9 +120,10 @@ def test_batch_mixed_2d_3d_same_alone_and_in_mix(lc):
     p3 = lc.PointSpreadFunction(lc.PolarGrid(th, ga, be, 10, 32), dims)
     p2a = lc.PointSpreadFunction2D(lc.PolarGrid2D(np.arange(30) * np.pi / 30, 48), 48)
     p2b = lc.PointSpreadFunction2D(lc.PolarGrid2D(np.arange(20) * np.pi / 20, 32), 32)
-    y3 = p3.convolve(rng.random(dims))
-    y2a = p2a.convolve(rng.random((2, 48, 48)))
-    y2b = p2b.convolve(rng.random((1, 32, 32)))
+    f32 = np.float32
+    y3 = p3.convolve(rng.random(dims).astype(f32))
+    y2a = p2a.convolve(rng.random((2, 48, 48)).astype(f32))
+    y2b = p2b.convolve(rng.random((1, 32, 32)).astype(f32))
     mix = lc.Batch2D([p2a, p3, p2b], ns=[2, 0, 1])
     for i, y in enumerate([y2a, y3, y2b]):
         mix.set(i, mix.Y, y)

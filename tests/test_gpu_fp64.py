@@ -1,0 +1,269 @@
+"""GPU parity of the FP64 path (float64 arrays: the reference's Array<double>
+instantiations of sysmat / sysmat2d, cgsolver, split_bregman, grad_u and
+divergence; src/gradient.cpp:45-83, src/conjgrad.cpp:82-85,
+src/bregman.cpp:101-104, src/finitediff.cpp:69-93) against the FP64 oracle,
+through the C ABI's _f64 entry points."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import announce, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+# the Toeplitz apply is one FP64 FFT convolution: only rounding separates it
+# from the oracle's
+TOEP_TOL = 1e-11
+
+
+@pytest.fixture(scope="module")
+def lc():
+    import laminocam_amd
+    return laminocam_amd
+
+
+def ref_theta(n):
+    # tests/test_adjoint2d.cpp:50-55
+    return (np.arange(n) - n / 2.0) * np.pi / n
+
+
+def lamino_angles(n, tilt_deg=-30.0):
+    th = np.arange(n) * 2 * np.pi / n
+    return th, np.full(n, np.deg2rad(tilt_deg)), np.zeros(n)
+
+
+def _problem(orc64, N, nang, ns, seed):
+    from laminocam_amd import synth
+    rng = synth.Stream(seed)
+    th = synth.angles(nang)
+    vol = np.stack([synth.ellipse_phantom(N, 10, rng) for _ in range(ns)])
+    sino = orc64.forward2d(vol, th, N)
+    sino = synth.add_gaussian_noise(sino, 0.01, rng).astype(np.float64)
+    return th, orc64.backproj2d(sino, th, N).astype(np.float64)
+
+
+# sizes with compiled FFT lengths (M = 256, 384, 512) and runtime plans
+# (M = 32, 64, 432, 1458), odd N, one and several slices
+@pytest.mark.parametrize("N,nang,ns", [(16, 7, 5), (31, 17, 2), (64, 90, 3), (128, 180, 2),
+                                       (192, 90, 1), (256, 180, 1), (200, 90, 2), (700, 60, 1)])
+def test_toeplitz2d_fp64(lc, orc64, N, nang, ns):
+    announce(f"fp64 toeplitz2d N={N} nang={nang} ns={ns}")
+    rng = np.random.default_rng(N * 31 + nang)
+    th = ref_theta(nang)
+    x = rng.random((ns, N, N))
+    psf = lc.PointSpreadFunction2D(lc.PolarGrid2D(th, N), N)
+    y = psf.convolve(x)
+    assert y.dtype == np.float64
+    err = rel_l2(y, orc64.psf2d(th, N, N, method=0).convolve(x))
+    print(f"rel_l2 = {err:.3e}")
+    assert err < TOEP_TOL
+    # the FP32 operator on the same input stays within its own bar
+    y32 = psf.convolve(x.astype(np.float32))
+    assert y32.dtype == np.float32 and rel_l2(y32, y) < 1e-5
+
+
+SHAPES3 = [((5, 16, 16), 7, 9, 16, -30.0), ((3, 33, 33), 11, 12, 30, -30.0),
+           ((8, 40, 40), 30, 16, 40, 20.0), ((1, 24, 24), 9, 6, 24, -30.0)]
+
+
+@pytest.mark.parametrize("dims,nproj,nrows,ncols,tilt", SHAPES3)
+def test_toeplitz3d_fp64(lc, orc64, dims, nproj, nrows, ncols, tilt):
+    announce(f"fp64 toeplitz3d dims={dims}")
+    rng = np.random.default_rng(sum(dims) + nproj)
+    th, ga, be = lamino_angles(nproj, tilt)
+    psf = lc.PointSpreadFunction(lc.PolarGrid(th, ga, be, nrows, ncols), dims)
+    x = rng.random(dims)
+    y = psf.convolve(x)
+    err = rel_l2(y, orc64.psf3d(th, ga, be, nrows, ncols, dims, method=0).convolve(x))
+    print(f"rel_l2 = {err:.3e}")
+    assert err < TOEP_TOL
+
+
+@pytest.mark.parametrize("N,nang,ns,iters", [(64, 90, 1, 8), (48, 60, 3, 10), (96, 45, 2, 12)])
+def test_cg_fp64_short(lc, orc64, N, nang, ns, iters):
+    # Up to ~12 iterations CG amplifies rounding by at most ~1e8 on these
+    # systems (an FP64 oracle rerun on a right-hand side perturbed by 1e-15
+    # relative ends 1e-7 away at 12 iterations), so 1e-6 is the FP64 bar.
+    announce(f"fp64 cg N={N} nang={nang} ns={ns} iters={iters}")
+    th, rhs = _problem(orc64, N, nang, ns, 3 + N)
+    psf = lc.PointSpreadFunction2D(lc.PolarGrid2D(th, N), N)
+    x, rep = lc.cgsolver(psf, rhs, np.zeros_like(rhs), iters, 1e-12, 0.0)
+    xr, rr = orc64.cgsolver(orc64.psf2d(th, N, N, method=0), rhs, np.zeros_like(rhs), iters,
+                            1e-12, 0.0)
+    assert x.dtype == np.float64 and rep["iters"] == rr["iters"]
+    err = rel_l2(x, xr)
+    print(f"rel_l2 = {err:.3e}")
+    assert err < 1e-6
+
+
+def _cg_np(A, y, iters):
+    """plain CG (src/conjgrad.cpp:33-76, fixed iteration count) with numpy's
+    summation order: an FP64 realisation independent of the oracle's"""
+    x = np.zeros_like(y)
+    r = y.copy()
+    p = r.copy()
+    rho = np.vdot(r, r)
+    for _ in range(iters):
+        ap = A(p)
+        alpha = rho / np.vdot(p, ap)
+        x = x + alpha * p
+        r = r - alpha * ap
+        rn = np.vdot(r, r)
+        p = r + (rn / rho) * p
+        rho = rn
+    return x
+
+
+def _fft_operator(P, N):
+    """the oracle PSF's Toeplitz operator applied with numpy's FFT: the same
+    operator to 1e-15, rounded differently"""
+    kh = np.squeeze(P.kernel_hat())
+    M = kh.shape[0]
+    k = np.arange(M)
+    # kernel_hat stores the PSF centred at M/2: undo that phase to get the
+    # real spectrum of the centred PSF
+    khc = (kh * np.exp(2j * np.pi * np.add.outer(k, k[:kh.shape[1]]) * (M // 2) / M)).real
+    khc = khc / N
+
+    def A(x):
+        X = np.zeros((x.shape[0], M, M))
+        X[:, :N, :N] = x
+        return np.fft.irfft2(np.fft.rfft2(X) * khc, s=(M, M))[:, :N, :N]
+    return A
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_cg_fp64_long_at_rounding_floor(lc, orc64, seed):
+    # 50 CG iterations on the 128 x 128, 90-angle slices of the batched
+    # workload.  Here plain CG is chaotic in FP64 itself: two FP64 solves that
+    # differ only in summation order or FFT rounding end 1e-3 .. 1e-2 apart.
+    # The bar: the GPU's FP64 result is no farther from the oracle than those
+    # independent FP64 realisations are, its residual is as close to the
+    # oracle's as theirs, and its distance to the true object matches the
+    # oracle's to 1e-3 relative.
+    N, nang, iters = 128, 90, 50
+    announce(f"fp64 cg long N={N} nang={nang} iters={iters} seed={seed}")
+    from laminocam_amd import synth
+    rng = synth.Stream(100 + seed)
+    th = synth.angles(nang)
+    vol = synth.ellipse_phantom(N, 10, rng)[None].astype(np.float64)
+    sino = synth.add_gaussian_noise(orc64.forward2d(vol, th, N), 0.01, rng).astype(np.float64)
+    rhs = orc64.backproj2d(sino, th, N).astype(np.float64)
+    psf = lc.PointSpreadFunction2D(lc.PolarGrid2D(th, N), N)
+    x, rep = lc.cgsolver(psf, rhs, np.zeros_like(rhs), iters, 0.0, 0.0, graph=True)
+    P = orc64.psf2d(th, N, N, method=0)
+    xr, rr = orc64.cgsolver(P, rhs, np.zeros_like(rhs), iters, 0.0, 0.0)
+    assert rep["iters"] == rr["iters"] == iters
+    xa = _cg_np(P.convolve, rhs, iters)
+    xb = _cg_np(_fft_operator(P, N), rhs, iters)
+    floor = max(rel_l2(xa, xr), rel_l2(xb, xr))
+    err = rel_l2(x, xr)
+    ynorm = np.linalg.norm(rhs)
+    res = [np.linalg.norm(rhs - P.convolve(v)) / ynorm for v in (x, xr, xa, xb)]
+    d_obj = [rel_l2(v, vol) for v in (x, xr, xa, xb)]
+    print(f"rel_l2 vs oracle: gpu {err:.3e}, independent FP64 {rel_l2(xa, xr):.3e} / "
+          f"{rel_l2(xb, xr):.3e}; true residual gpu {res[0]:.4e} oracle {res[1]:.4e} "
+          f"independent {res[2]:.4e} / {res[3]:.4e}; distance to object gpu {d_obj[0]:.6f} "
+          f"oracle {d_obj[1]:.6f} independent {d_obj[2]:.6f} / {d_obj[3]:.6f}")
+    assert err <= max(1e-3, 2.0 * floor)
+    spread = max(abs(np.log(res[2] / res[1])), abs(np.log(res[3] / res[1])))
+    assert abs(np.log(res[0] / res[1])) <= max(np.log(1.05), 2.0 * spread)
+    dspread = max(abs(d_obj[2] - d_obj[1]), abs(d_obj[3] - d_obj[1]))
+    assert abs(d_obj[0] - d_obj[1]) <= max(1e-3 * d_obj[1], 2.0 * dspread)
+
+
+@pytest.mark.parametrize("zero_pad,aniso", [(False, False), (True, False), (True, True)])
+def test_split_bregman_fp64(lc, orc64, zero_pad, aniso):
+    announce(f"fp64 split bregman zero_pad={zero_pad} aniso={aniso}")
+    N, ns = 32, 4
+    th, rhs = _problem(orc64, N, 45, ns, 21)
+    psf = lc.PointSpreadFunction2D(lc.PolarGrid2D(th, N), N)
+    x, rep = lc.split_bregman(psf, rhs, np.zeros_like(rhs), 0.1, 10.0, 12, 2, 1e-12, 1e-12,
+                              zero_pad=zero_pad, aniso=aniso)
+    xr, rr = orc64.split_bregman(orc64.psf2d(th, N, N, method=0), rhs, np.zeros_like(rhs), 0.1,
+                                 10.0, 12, 2, 1e-12, 1e-12, boundary=1 if zero_pad else 0,
+                                 aniso=aniso)
+    assert rep["outer"] == rr["outer"]
+    err = rel_l2(x, xr)
+    print(f"rel_l2 = {err:.3e}")
+    assert err < 1e-9
+
+
+def test_solvers3d_fp64(lc, orc64):
+    announce("fp64 cg3d / split_bregman3d")
+    dims = (5, 16, 16)
+    rng = np.random.default_rng(5)
+    th, ga, be = lamino_angles(9)
+    psf = lc.PointSpreadFunction(lc.PolarGrid(th, ga, be, 9, 16), dims)
+    P = orc64.psf3d(th, ga, be, 9, 16, dims, method=0)
+    y = P.convolve(rng.random(dims))
+    x, rep = lc.cgsolver(psf, y, np.zeros(dims), 8, 1e-12, 0.0)
+    xr, rr = orc64.cgsolver(P, y, np.zeros(dims), 8, 1e-12, 0.0)
+    assert rep["iters"] == rr["iters"]
+    assert rel_l2(x, xr) < 1e-9
+    x, rep = lc.split_bregman(psf, y, np.zeros(dims), 0.1, 10.0, 6, 2, 1e-12, 0.0, zero_pad=True)
+    xr, rr = orc64.split_bregman(P, y, np.zeros(dims), 0.1, 10.0, 6, 2, 1e-12, 0.0, boundary=1)
+    assert rel_l2(x, xr) < 1e-9
+
+
+@pytest.mark.parametrize("shape", [(1, 9, 11), (5, 16, 16)])
+@pytest.mark.parametrize("zero_pad", [False, True])
+def test_finite_differences_fp64(lc, orc64, shape, zero_pad):
+    announce(f"fp64 fd {shape} zero_pad={zero_pad}")
+    rng = np.random.default_rng(3)
+    u = rng.random(shape)
+    b = 1 if zero_pad else 0
+    g = lc.grad_u(u, zero_pad=zero_pad)
+    assert g.dtype == np.float64
+    assert np.allclose(g, orc64.grad(u, b), rtol=0, atol=1e-14)
+    gg = rng.random((3,) + shape)
+    assert np.allclose(lc.divergence(gg, zero_pad=zero_pad), orc64.div(gg, b), rtol=0, atol=1e-14)
+    assert np.allclose(lc.laplacian(u, zero_pad=zero_pad), orc64.laplacian(u, b), rtol=0,
+                       atol=1e-14)
+
+
+def test_batch_fp64_same_alone_and_in_mix(lc, orc64):
+    # mixed FP64 batch (2-D stacks of several sizes and a 3-D volume): each
+    # problem's result is bitwise its result alone, graph or not
+    announce("fp64 batch alone vs mix")
+    rng = np.random.default_rng(9)
+    p2 = [lc.PointSpreadFunction2D(lc.PolarGrid2D(ref_theta(a), N), N)
+          for N, a in [(32, 30), (64, 45), (48, 90)]]
+    th, ga, be = lamino_angles(13)
+    p3 = lc.PointSpreadFunction(lc.PolarGrid(th, ga, be, 10, 32), (6, 32, 32))
+    psfs = p2 + [p3]
+    ns = [2, 1, 3, 0]
+    ys = [p.convolve(rng.random((n, p.image_size, p.image_size))) for p, n in zip(p2, ns)]
+    ys.append(p3.convolve(rng.random((6, 32, 32))))
+    iters = [12, 7, 20, 9]
+    mix = lc.Batch2D(psfs, ns=ns, fp64=True)
+    for i, y in enumerate(ys):
+        mix.set(i, mix.Y, y)
+    mix.cg(iters, 1e-12, 0.0, graph=True)
+    reps = mix.reports()
+    for i in range(len(psfs)):
+        alone = lc.Batch2D([psfs[i]], ns=[ns[i]], fp64=True)
+        alone.set(0, alone.Y, ys[i])
+        alone.cg([iters[i]], 1e-12, 0.0, graph=False)
+        xm = mix.get(i, mix.X)
+        assert xm.dtype == np.float64
+        assert alone.reports()[0]["iters"] == reps[i]["iters"]
+        assert np.array_equal(alone.get(0, alone.X), xm)
+        xs, _ = lc.cgsolver(psfs[i], ys[i], np.zeros_like(ys[i]), iters[i], 1e-12, 0.0)
+        assert np.array_equal(xs.reshape(xm.shape), xm)
+
+
+def test_batch_fp64_rejects_float_accessors(lc):
+    announce("fp64 batch type check")
+    from laminocam_amd import _lib
+    p = lc.PointSpreadFunction2D(lc.PolarGrid2D(ref_theta(20), 32), 32)
+    b = lc.Batch2D([p], fp64=True)
+    a = np.zeros((1, 32, 32), np.float32)
+    with pytest.raises(_lib.LcError) as ei:
+        _lib.check(b.lib.lc_batch2d_set(b.handle, 0, 0, C.c_void_p(a.ctypes.data), 0))
+    assert ei.value.code == 1
+    b32 = lc.Batch2D([p])
+    d = np.zeros((1, 32, 32), np.float64)
+    with pytest.raises(_lib.LcError):
+        _lib.check(b32.lib.lc_batch2d_get_f64(b32.handle, 0, 0, C.c_void_p(d.ctypes.data), 0))

@@ -31,10 +31,10 @@ def problems(lc):
     reqs = []
     for i in range(10):
         if i % 4 == 3:
-            y = p3.convolve(rng.random((6, 32, 32)))
+            y = p3.convolve(rng.random((6, 32, 32)).astype(np.float32))
             reqs.append(dict(psf=p3, rhs=y, solver="cg", max_iter=8))
         elif i % 4 == 2:
-            y = p96.convolve(rng.random((2, 96, 96)))
+            y = p96.convolve(rng.random((2, 96, 96)).astype(np.float32))
             reqs.append(dict(psf=p96, rhs=y, solver="sb", lam=0.05 + 0.01 * i, mu=10.0, outer=4,
                              inner=1))
         else:
