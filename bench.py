@@ -15,7 +15,9 @@ its own CG scalars), replayed from a captured hipGraph, inputs resident in
 HBM.  Reported beside it:
   * the plain path: the same slices solved one at a time through lc_cg2d
     (host arrays in and out, host-synchronised iterations), on a sample;
-  * relative L2 error against the FP64 oracle on a sample of slices;
+  * the same batched solve in the FP64 path (float64 vectors, spectra, FFTs);
+  * relative L2 error against the FP64 oracle on a sample of slices, for the
+    FP32 and the FP64 path, beside independent FP64 realisations;
   * roofline of the dominant Toeplitz pass from HIP events on the library's
     stream, right after the timed steps, on the same batch;
   * the FP32 CPU restatement of the reference path (oracle, OpenMP) timed on
@@ -79,6 +81,37 @@ def build_problem(lc, synth, rng, Ns, As):
         for j, i in enumerate(idx):
             rhs[i] = y[j:j + 1]
     return geoms, rhs
+
+
+def fp64_cg_np(A, y, iters):
+    """plain CG (src/conjgrad.cpp:33-76, fixed iterations) with numpy's
+    summation order: an FP64 realisation independent of the oracle's rounding"""
+    x, r = np.zeros(y.shape), y.copy()
+    p, rho = r.copy(), float(np.vdot(y, y))
+    for _ in range(iters):
+        ap = A(p)
+        alpha = rho / float(np.vdot(p, ap))
+        x = x + alpha * p
+        r = r - alpha * ap
+        rn = float(np.vdot(r, r))
+        p = r + (rn / rho) * p
+        rho = rn
+    return x
+
+
+def fft_operator(P, N):
+    """the Toeplitz operator of the oracle PSF P applied with numpy's FFT (the
+    same operator to 1e-15, rounded differently)"""
+    kh = np.squeeze(P.kernel_hat())
+    M = kh.shape[0]
+    k = np.arange(M)
+    khc = (kh * np.exp(2j * np.pi * np.add.outer(k, k[:kh.shape[1]]) * (M // 2) / M)).real / N
+
+    def A(x):
+        X = np.zeros((x.shape[0], M, M))
+        X[:, :N, :N] = x
+        return np.fft.irfft2(np.fft.rfft2(X) * khc, s=(M, M))[:, :N, :N]
+    return A
 
 
 def rel_l2(a, b):
@@ -353,6 +386,8 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--kernel-reps", type=int, default=10)
+    ap.add_argument("--fp64-steps", type=int, default=2, help="timed FP64 batched solves")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 batched solve")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="served stream: requests/s")
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="served stream: seconds of arrivals")
     ap.add_argument("--serve-3d-frac", type=float, default=0.05)
@@ -437,6 +472,34 @@ def main():
     }
     log(f"dominant: {roofline['kernel']}: {dom['ms']:.3f} ms/launch, {achieved:.0f} GB/s")
 
+    # ---- FP64 path: the same batched solve in float64 ----
+    fp64 = None
+    if not args.no_fp64:
+        b64 = lc.Batch2D(psfs, ctx=ctx, fp64=True)
+        for i in range(len(Ns)):
+            b64.set(i, b64.Y, rhs[i].astype(np.float64))
+        b64.cg(args.iters, args.tol, args.xtol, graph=True)   # warm-up
+        ctx.sync()
+        t64 = []
+        for _ in range(max(1, args.fp64_steps)):
+            ctx.sync()
+            t0 = time.perf_counter()
+            b64.cg(args.iters, args.tol, args.xtol, graph=True)
+            ctx.sync()
+            t64.append(time.perf_counter() - t0)
+        it64 = np.array([r["iters"] for r in b64.reports()])
+        prof64 = b64.profile_passes(max(2, args.kernel_reps // 2))
+        d64 = max(prof64, key=lambda r: r["ms"])
+        fp64 = {"slices_per_s": round(len(Ns) / float(np.mean(t64)), 2),
+                "ms_per_step": round(1e3 * float(np.mean(t64)), 3), "steps": len(t64),
+                "cg_iterations_mean": round(float(it64.mean()), 2),
+                "dominant_pass": f"{PASS_NAMES[d64['pass_']]} N={d64['N']}",
+                "dominant_GBps": round(d64["bytes"] / (d64["ms"] * 1e-3) / 1e9, 1),
+                "note": "FP64 vectors, PSF spectrum, FFTs and twiddles (cgsolver<double>)"}
+        log(f"fp64 batched: {fp64['ms_per_step']} ms/step, {fp64['slices_per_s']} slices/s")
+        b64.close()
+        del b64
+
     # ---- plain path: one solve at a time through the C ABI, host arrays ----
     order = np.arange(len(Ns))
     plain_idx = order[:min(args.plain_slices, len(Ns))]
@@ -470,7 +533,7 @@ def main():
     from oracle.oracle import Oracle
     o64, o32 = Oracle(64), Oracle(32)
     small = sorted(range(len(Ns)), key=lambda i: (Ns[i], As[i], i))[:args.err_slices]
-    errs, errs32, floor64, errs_short = [], [], [], []
+    errs, errs32, floor64, errs_short, errs64, indep64 = [], [], [], [], [], []
     for i in small:
         N, a = int(Ns[i]), int(As[i])
         th = synth.angles(a)
@@ -488,9 +551,16 @@ def main():
         xs, _ = lc.cgsolver(psfs[i], y, np.zeros_like(y), args.short_iters, args.tol, args.xtol)
         x64s, _ = o64.cgsolver(p64, y64, np.zeros(y.shape), args.short_iters, args.tol, args.xtol)
         errs_short.append(rel_l2(xs, x64s))
+        # the GPU's FP64 path, and an FP64 CG independent of the oracle's
+        # rounding (numpy summation order, the oracle's operator)
+        xg64, _ = lc.cgsolver(psfs[i], y64, np.zeros(y.shape), args.iters, args.tol, args.xtol)
+        errs64.append(rel_l2(xg64, x64))
+        indep64.append(max(rel_l2(fp64_cg_np(op, y64, args.iters), x64)
+                           for op in (p64.convolve, fft_operator(p64, N))))
     log(f"rel err vs FP64 oracle ({args.iters} it): GPU {['%.2e' % e for e in errs]}; FP32 CPU "
         f"port {['%.2e' % e for e in errs32]}; FP64 rerun floor {['%.2e' % e for e in floor64]}; "
-        f"GPU at {args.short_iters} it {['%.2e' % e for e in errs_short]}")
+        f"GPU at {args.short_iters} it {['%.2e' % e for e in errs_short]}; GPU FP64 path "
+        f"{['%.2e' % e for e in errs64]}; independent FP64 {['%.2e' % e for e in indep64]}")
 
     # ---- served mixed stream ----
     served = None
@@ -557,9 +627,15 @@ def main():
                             "fp64_floor": [float("%.3e" % e) for e in floor64],
                             "short_iters": args.short_iters,
                             "gpu_short": [float("%.3e" % e) for e in errs_short],
+                            "gpu_fp64": [float("%.3e" % e) for e in errs64],
+                            "fp64_independent": [float("%.3e" % e) for e in indep64],
                             "note": "fp64_floor: FP64 oracle rerun on a right-hand side perturbed "
-                                    "by 1e-15 relative; long CG solves of these under-sampled "
-                                    "systems drift by this much from rounding alone"},
+                                    "by 1e-15 relative; fp64_independent: the farther of two FP64 "
+                                    "CGs with numpy's summation order, on the oracle's operator "
+                                    "and on a numpy-FFT one; long CG "
+                                    "solves of these under-sampled systems drift by this much "
+                                    "from rounding alone"},
+        "fp64_batched": fp64,
         "served": served,
         "lamino3d": lamino,
         "roofline": roofline,

@@ -78,6 +78,8 @@ int lc_geom2d_info(lc_geom2d g, int64_t *info);
 /* PointSpreadFunction2D::kernel_hat() (include/toeplitz2d.h:63-66):
  * complex {M, M/2+1} written to out (host) */
 int lc_psf2d_kernel_hat(lc_ctx ctx, lc_geom2d g, float *out);
+/* PointSpreadFunction2D<double>::kernel_hat(): the same, as double pairs */
+int lc_psf2d_kernel_hat_f64(lc_ctx ctx, lc_geom2d g, double *out);
 
 /* ---- 2-D operators on a stack x of nslices N x N slices ----
  * sysmat2d(x, PointSpreadFunction2D) (src/gradient.cpp:76-79,
@@ -136,6 +138,8 @@ int lc_geom3d_info(lc_geom3d g, int64_t *info);
 /* PointSpreadFunction::kernel_hat() (include/toeplitz.h:55-57): complex
  * {d1, d2, d3/2+1} written to out (host) */
 int lc_psf3d_kernel_hat(lc_ctx ctx, lc_geom3d g, float *out);
+/* PointSpreadFunction<double>::kernel_hat(): the same, as double pairs */
+int lc_psf3d_kernel_hat_f64(lc_ctx ctx, lc_geom3d g, double *out);
 /* sysmat(x, PointSpreadFunction) (src/gradient.cpp:49-52, include/toeplitz.h:101-129)
  * on the {n1, n2, n3} volume x */
 int lc_sysmat3d_psf(lc_ctx ctx, lc_geom3d g, const float *x, float *y, uint32_t flags);

@@ -21,8 +21,11 @@
  * Defining LAMINOCAM_STANDALONE_ARRAY instead provides a minimal
  * tomocam::dims_t / tomocam::Array<T> with that same interface.
  *
- * Precision: the device computes in FP32 (FP64 reductions, PSF and twiddles);
- * double arrays are converted at the boundary.
+ * Precision: Array<float> runs the FP32 path (FP64 reductions, PSF and
+ * twiddles).  Array<double> runs the FP64 path of the Toeplitz operator,
+ * kernel_hat, the solvers and the finite differences (the _f64 entry points),
+ * as the reference's double instantiations do; the NUFFT projectors compute in
+ * FP32 for either type, double arrays being converted at the boundary.
  *
  * Solvers: opt::Function<T> is constructible from any callable, like the
  * reference's std::function alias.  cgsolver / split_bregman run on the device
@@ -134,6 +137,63 @@ template <typename T> struct F32Out {
     }
 };
 
+// element-type dispatch to the FP32 or FP64 entry points
+inline int psf2d_kernel_hat(lc_ctx c, lc_geom2d g, float *o) { return lc_psf2d_kernel_hat(c, g, o); }
+inline int psf2d_kernel_hat(lc_ctx c, lc_geom2d g, double *o) { return lc_psf2d_kernel_hat_f64(c, g, o); }
+inline int psf3d_kernel_hat(lc_ctx c, lc_geom3d g, float *o) { return lc_psf3d_kernel_hat(c, g, o); }
+inline int psf3d_kernel_hat(lc_ctx c, lc_geom3d g, double *o) { return lc_psf3d_kernel_hat_f64(c, g, o); }
+inline int sysmat2d_psf(lc_ctx c, lc_geom2d g, const float *x, int64_t ns, float *y) {
+    return lc_sysmat2d_psf(c, g, x, ns, y, 0);
+}
+inline int sysmat2d_psf(lc_ctx c, lc_geom2d g, const double *x, int64_t ns, double *y) {
+    return lc_sysmat2d_psf_f64(c, g, x, ns, y, 0);
+}
+inline int sysmat3d_psf(lc_ctx c, lc_geom3d g, const float *x, float *y) { return lc_sysmat3d_psf(c, g, x, y, 0); }
+inline int sysmat3d_psf(lc_ctx c, lc_geom3d g, const double *x, double *y) {
+    return lc_sysmat3d_psf_f64(c, g, x, y, 0);
+}
+inline int cg2d(lc_ctx c, lc_geom2d g, const float *y, float *x, int64_t ns, int64_t it, double tol,
+                double xtol, lc_report *r) {
+    return lc_cg2d(c, g, y, x, ns, it, tol, xtol, 0, r);
+}
+inline int cg2d(lc_ctx c, lc_geom2d g, const double *y, double *x, int64_t ns, int64_t it, double tol,
+                double xtol, lc_report *r) {
+    return lc_cg2d_f64(c, g, y, x, ns, it, tol, xtol, 0, r);
+}
+inline int cg3d(lc_ctx c, lc_geom3d g, const float *y, float *x, int64_t it, double tol, double xtol,
+                lc_report *r) {
+    return lc_cg3d(c, g, y, x, it, tol, xtol, 0, r);
+}
+inline int cg3d(lc_ctx c, lc_geom3d g, const double *y, double *x, int64_t it, double tol, double xtol,
+                lc_report *r) {
+    return lc_cg3d_f64(c, g, y, x, it, tol, xtol, 0, r);
+}
+inline int sb2d(lc_ctx c, lc_geom2d g, const float *y, float *x, int64_t ns, double la, double mu,
+                int64_t outer, int64_t inner, double tol, double xtol, lc_report *r) {
+    return lc_split_bregman2d(c, g, y, x, ns, la, mu, outer, inner, tol, xtol, 0, r);
+}
+inline int sb2d(lc_ctx c, lc_geom2d g, const double *y, double *x, int64_t ns, double la, double mu,
+                int64_t outer, int64_t inner, double tol, double xtol, lc_report *r) {
+    return lc_split_bregman2d_f64(c, g, y, x, ns, la, mu, outer, inner, tol, xtol, 0, r);
+}
+inline int sb3d(lc_ctx c, lc_geom3d g, const float *y, float *x, double la, double mu, int64_t outer,
+                int64_t inner, double tol, double xtol, lc_report *r) {
+    return lc_split_bregman3d(c, g, y, x, la, mu, outer, inner, tol, xtol, 0, r);
+}
+inline int sb3d(lc_ctx c, lc_geom3d g, const double *y, double *x, double la, double mu, int64_t outer,
+                int64_t inner, double tol, double xtol, lc_report *r) {
+    return lc_split_bregman3d_f64(c, g, y, x, la, mu, outer, inner, tol, xtol, 0, r);
+}
+inline int fd(int op, lc_ctx c, const float *in, int64_t n1, int64_t n2, int64_t n3, float *out) {
+    return op == 0 ? lc_grad(c, in, n1, n2, n3, out, 0)
+                   : op == 1 ? lc_divergence(c, in, n1, n2, n3, out, 0) : lc_laplacian(c, in, n1, n2, n3, out, 0);
+}
+inline int fd(int op, lc_ctx c, const double *in, int64_t n1, int64_t n2, int64_t n3, double *out) {
+    return op == 0 ? lc_grad_f64(c, in, n1, n2, n3, out, 0)
+                   : op == 1 ? lc_divergence_f64(c, in, n1, n2, n3, out, 0)
+                             : lc_laplacian_f64(c, in, n1, n2, n3, out, 0);
+}
+
 struct Geom2 {
     lc_geom2d h = nullptr;
     ~Geom2() { if (h) lc_geom2d_destroy(h); }
@@ -225,8 +285,8 @@ template <typename T> class PointSpreadFunction2D {
 
     Array<complex_t> kernel_hat() const {
         const size_t H = M_ / 2 + 1;
-        std::vector<float> buf(2 * M_ * H);
-        laminocam::check(lc_psf2d_kernel_hat(laminocam::context(), g_->h, buf.data()));
+        std::vector<T> buf(2 * M_ * H);
+        laminocam::check(laminocam::psf2d_kernel_hat(laminocam::context(), g_->h, buf.data()));
         Array<complex_t> out(dims_t{1, M_, H});
         for (size_t i = 0; i < M_ * H; ++i) out.data()[i] = complex_t((T)buf[2 * i], (T)buf[2 * i + 1]);
         return out;
@@ -236,10 +296,7 @@ template <typename T> class PointSpreadFunction2D {
         const dims_t d = x.dims();
         if (d.n2 != N_ || d.n3 != N_) throw std::invalid_argument("convolve: slice size differs from the PSF's");
         Array<T> y(d);
-        laminocam::F32In<T> in(x.data(), x.size());
-        laminocam::F32Out<T> out(y.data(), y.size());
-        laminocam::check(lc_sysmat2d_psf(laminocam::context(), g_->h, in.p, (int64_t)d.n1, out.p, 0));
-        out.commit();
+        laminocam::check(laminocam::sysmat2d_psf(laminocam::context(), g_->h, x.data(), (int64_t)d.n1, y.data()));
         return y;
     }
 
@@ -268,8 +325,8 @@ template <typename T> class PointSpreadFunction {
 
     Array<complex_t> kernel_hat() const {
         const size_t H = dims_.n3 / 2 + 1, n = dims_.n1 * dims_.n2 * H;
-        std::vector<float> buf(2 * n);
-        laminocam::check(lc_psf3d_kernel_hat(laminocam::context(), g_->h, buf.data()));
+        std::vector<T> buf(2 * n);
+        laminocam::check(laminocam::psf3d_kernel_hat(laminocam::context(), g_->h, buf.data()));
         Array<complex_t> out(dims_t{dims_.n1, dims_.n2, H});
         for (size_t i = 0; i < n; ++i) out.data()[i] = complex_t((T)buf[2 * i], (T)buf[2 * i + 1]);
         return out;
@@ -278,10 +335,7 @@ template <typename T> class PointSpreadFunction {
     Array<T> convolve(const Array<T> &x) const {
         if (!(x.dims() == rd_)) throw std::invalid_argument("convolve: volume dims differ from the PSF's");
         Array<T> y(x.dims());
-        laminocam::F32In<T> in(x.data(), x.size());
-        laminocam::F32Out<T> out(y.data(), y.size());
-        laminocam::check(lc_sysmat3d_psf(laminocam::context(), g_->h, in.p, out.p, 0));
-        out.commit();
+        laminocam::check(laminocam::sysmat3d_psf(laminocam::context(), g_->h, x.data(), y.data()));
         return y;
     }
 
@@ -440,17 +494,15 @@ template <typename T>
 Array<T> cgsolver(const Function<T> &A, const Array<T> &y, const Array<T> &x, size_t max_iter, T tol, T xtol) {
     detail::need_psf(A, "cgsolver");
     Array<T> out(x.dims());
-    laminocam::F32In<T> yin(y.data(), y.size());
-    laminocam::F32Out<T> o(out.data(), out.size());
-    std::memcpy(o.p, laminocam::F32In<T>(x.data(), x.size()).p, sizeof(float) * x.size());
+    std::memcpy(out.data(), x.data(), sizeof(T) * x.size());
     lc_report rep;
     if (A.kind() == 2)
-        laminocam::check(lc_cg2d(laminocam::context(), A.psf2().handle(), yin.p, o.p, (int64_t)y.dims().n1,
-                                 (int64_t)max_iter, (double)tol, (double)xtol, 0, &rep));
+        laminocam::check(laminocam::cg2d(laminocam::context(), A.psf2().handle(), y.data(), out.data(),
+                                         (int64_t)y.dims().n1, (int64_t)max_iter, (double)tol, (double)xtol,
+                                         &rep));
     else
-        laminocam::check(lc_cg3d(laminocam::context(), A.psf3().handle(), yin.p, o.p, (int64_t)max_iter,
-                                 (double)tol, (double)xtol, 0, &rep));
-    o.commit();
+        laminocam::check(laminocam::cg3d(laminocam::context(), A.psf3().handle(), y.data(), out.data(),
+                                         (int64_t)max_iter, (double)tol, (double)xtol, &rep));
     return out;
 }
 
@@ -461,55 +513,46 @@ Array<T> split_bregman(const Function<T> &A, const Array<T> &y, const Array<T> &
                        size_t outer_max, size_t inner_max, T tol, T xtol) {
     detail::need_psf(A, "split_bregman");
     Array<T> out(x0.dims());
-    laminocam::F32In<T> yin(y.data(), y.size());
-    laminocam::F32Out<T> o(out.data(), out.size());
-    std::memcpy(o.p, laminocam::F32In<T>(x0.data(), x0.size()).p, sizeof(float) * x0.size());
+    std::memcpy(out.data(), x0.data(), sizeof(T) * x0.size());
     lc_report rep;
     if (A.kind() == 2)
-        laminocam::check(lc_split_bregman2d(laminocam::context(), A.psf2().handle(), yin.p, o.p,
-                                            (int64_t)y.dims().n1, (double)lambda, (double)mu,
-                                            (int64_t)outer_max, (int64_t)inner_max, (double)tol,
-                                            (double)xtol, 0, &rep));
+        laminocam::check(laminocam::sb2d(laminocam::context(), A.psf2().handle(), y.data(), out.data(),
+                                         (int64_t)y.dims().n1, (double)lambda, (double)mu, (int64_t)outer_max,
+                                         (int64_t)inner_max, (double)tol, (double)xtol, &rep));
     else
-        laminocam::check(lc_split_bregman3d(laminocam::context(), A.psf3().handle(), yin.p, o.p,
-                                            (double)lambda, (double)mu, (int64_t)outer_max,
-                                            (int64_t)inner_max, (double)tol, (double)xtol, 0, &rep));
-    o.commit();
+        laminocam::check(laminocam::sb3d(laminocam::context(), A.psf3().handle(), y.data(), out.data(),
+                                         (double)lambda, (double)mu, (int64_t)outer_max, (int64_t)inner_max,
+                                         (double)tol, (double)xtol, &rep));
     return out;
 }
 
 // include/bregman.h:34-55, src/finitediff.cpp:26-90 (interior-only differences)
 template <typename T> std::array<Array<T>, 3> grad_u(const Array<T> &u) {
     const dims_t d = u.dims();
-    std::vector<float> g(3 * u.size());
-    laminocam::F32In<T> in(u.data(), u.size());
-    laminocam::check(lc_grad(laminocam::context(), in.p, (int64_t)d.n1, (int64_t)d.n2, (int64_t)d.n3, g.data(), 0));
+    std::vector<T> g(3 * u.size());
+    laminocam::check(laminocam::fd(0, laminocam::context(), u.data(), (int64_t)d.n1, (int64_t)d.n2,
+                                   (int64_t)d.n3, g.data()));
     std::array<Array<T>, 3> out{Array<T>(d), Array<T>(d), Array<T>(d)};
-    for (int c = 0; c < 3; ++c)
-        for (size_t i = 0; i < u.size(); ++i) out[c].data()[i] = (T)g[c * u.size() + i];
+    for (int c = 0; c < 3; ++c) std::memcpy(out[c].data(), g.data() + c * u.size(), sizeof(T) * u.size());
     return out;
 }
 
 template <typename T> Array<T> divergence(const std::array<Array<T>, 3> &g) {
     const dims_t d = g[0].dims();
     const size_t n = g[0].size();
-    std::vector<float> in(3 * n);
-    for (int c = 0; c < 3; ++c)
-        for (size_t i = 0; i < n; ++i) in[c * n + i] = (float)g[c].data()[i];
+    std::vector<T> in(3 * n);
+    for (int c = 0; c < 3; ++c) std::memcpy(in.data() + c * n, g[c].data(), sizeof(T) * n);
     Array<T> out(d);
-    laminocam::F32Out<T> o(out.data(), n);
-    laminocam::check(lc_divergence(laminocam::context(), in.data(), (int64_t)d.n1, (int64_t)d.n2, (int64_t)d.n3, o.p, 0));
-    o.commit();
+    laminocam::check(laminocam::fd(1, laminocam::context(), in.data(), (int64_t)d.n1, (int64_t)d.n2,
+                                   (int64_t)d.n3, out.data()));
     return out;
 }
 
 template <typename T> Array<T> laplacian(const Array<T> &u) {
     const dims_t d = u.dims();
     Array<T> out(d);
-    laminocam::F32In<T> in(u.data(), u.size());
-    laminocam::F32Out<T> o(out.data(), out.size());
-    laminocam::check(lc_laplacian(laminocam::context(), in.p, (int64_t)d.n1, (int64_t)d.n2, (int64_t)d.n3, o.p, 0));
-    o.commit();
+    laminocam::check(laminocam::fd(2, laminocam::context(), u.data(), (int64_t)d.n1, (int64_t)d.n2,
+                                   (int64_t)d.n3, out.data()));
     return out;
 }
 

@@ -225,11 +225,14 @@ class PointSpreadFunction2D:
     def psf_size(self):
         return self.info[3]
 
-    def kernel_hat(self):
+    def kernel_hat(self, dtype=np.complex64):
+        """reference-layout spectrum {1, M, M/2+1}; dtype=complex128 for the
+        PointSpreadFunction2D<double> form"""
         M, H = self.info[3], self.info[4]
-        out = np.empty((1, M, H), np.complex64)
-        _lib.check(self.lib.lc_psf2d_kernel_hat(
-            self.ctx.handle, self.handle, out.view(np.float32).ctypes.data_as(C.POINTER(C.c_float))))
+        out = np.empty((1, M, H), dtype)
+        fn = self.lib.lc_psf2d_kernel_hat_f64 if out.dtype == np.complex128 else \
+            self.lib.lc_psf2d_kernel_hat
+        _lib.check(fn(self.ctx.handle, self.handle, C.c_void_p(out.ctypes.data)))
         return out
 
     def convolve(self, x):
@@ -301,11 +304,14 @@ class PointSpreadFunction:
     def dims(self):
         return self.info[6:9]
 
-    def kernel_hat(self):
+    def kernel_hat(self, dtype=np.complex64):
+        """reference-layout spectrum {d1, d2, d3/2+1}; dtype=complex128 for the
+        PointSpreadFunction<double> form"""
         d1, d2, d3 = self.dims()
-        out = np.empty((d1, d2, d3 // 2 + 1), np.complex64)
-        _lib.check(self.lib.lc_psf3d_kernel_hat(
-            self.ctx.handle, self.handle, out.view(np.float32).ctypes.data_as(C.POINTER(C.c_float))))
+        out = np.empty((d1, d2, d3 // 2 + 1), dtype)
+        fn = self.lib.lc_psf3d_kernel_hat_f64 if out.dtype == np.complex128 else \
+            self.lib.lc_psf3d_kernel_hat
+        _lib.check(fn(self.ctx.handle, self.handle, C.c_void_p(out.ctypes.data)))
         return out
 
     def convolve(self, x):

@@ -34,7 +34,7 @@ EXPORTS = [
     "lc_sysmat2d_psf_f64", "lc_cg2d_f64", "lc_split_bregman2d_f64", "lc_sysmat3d_psf_f64",
     "lc_cg3d_f64", "lc_split_bregman3d_f64", "lc_grad_f64", "lc_divergence_f64",
     "lc_laplacian_f64", "lc_batch_create_ex", "lc_batch2d_set_f64", "lc_batch2d_get_f64",
-    "lc_batch2d_device_ptr_f64",
+    "lc_batch2d_device_ptr_f64", "lc_psf2d_kernel_hat_f64", "lc_psf3d_kernel_hat_f64",
 ]
 
 
@@ -88,7 +88,7 @@ def load():
         "lc_geom2d_create": (C.c_int, [vp, f64p, i64, i64, i64, u32, C.POINTER(vp)]),
         "lc_geom2d_destroy": (C.c_int, [vp]),
         "lc_geom2d_info": (C.c_int, [vp, C.POINTER(i64)]),
-        "lc_psf2d_kernel_hat": (C.c_int, [vp, vp, f32p]),
+        "lc_psf2d_kernel_hat": (C.c_int, [vp, vp, vp]),
         "lc_sysmat2d_psf": (C.c_int, [vp, vp, vp, i64, vp, u32]),
         "lc_forward2d": (C.c_int, [vp, vp, vp, i64, vp, u32]),
         "lc_backproj2d": (C.c_int, [vp, vp, vp, i64, vp, u32]),
@@ -116,7 +116,7 @@ def load():
                                        C.POINTER(vp)]),
         "lc_geom3d_destroy": (C.c_int, [vp]),
         "lc_geom3d_info": (C.c_int, [vp, C.POINTER(i64)]),
-        "lc_psf3d_kernel_hat": (C.c_int, [vp, vp, f32p]),
+        "lc_psf3d_kernel_hat": (C.c_int, [vp, vp, vp]),
         "lc_sysmat3d_psf": (C.c_int, [vp, vp, vp, vp, u32]),
         "lc_server_create": (C.c_int, [vp, C.c_uint64, i32, i32, C.POINTER(vp)]),
         "lc_server_submit": (C.c_int, [vp, C.POINTER(LcRequest), C.POINTER(C.c_uint64)]),
@@ -156,6 +156,8 @@ def load():
         "lc_batch2d_set_f64": (C.c_int, [vp, i32, i32, vp, u32]),
         "lc_batch2d_get_f64": (C.c_int, [vp, i32, i32, vp, u32]),
         "lc_batch2d_device_ptr_f64": (C.c_int, [vp, i32, i32, C.POINTER(vp)]),
+        "lc_psf2d_kernel_hat_f64": (C.c_int, [vp, vp, vp]),
+        "lc_psf3d_kernel_hat_f64": (C.c_int, [vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

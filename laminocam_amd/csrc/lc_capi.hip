@@ -150,7 +150,16 @@ int lc_psf2d_kernel_hat(lc_ctx ctx, lc_geom2d g, float *out) {
         LC_REQUIRE(ctx && g && out, "lc_psf2d_kernel_hat: NULL argument");
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         ctx->c.set_current();
-        geom2d_kernel_hat(ctx->c, g->g, out);
+        geom2d_kernel_hat<float>(ctx->c, g->g, out);
+    });
+}
+
+int lc_psf2d_kernel_hat_f64(lc_ctx ctx, lc_geom2d g, double *out) {
+    return guard([&] {
+        LC_REQUIRE(ctx && g && out, "lc_psf2d_kernel_hat_f64: NULL argument");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        ctx->c.set_current();
+        geom2d_kernel_hat<double>(ctx->c, g->g, out);
     });
 }
 
@@ -390,7 +399,16 @@ int lc_psf3d_kernel_hat(lc_ctx ctx, lc_geom3d g, float *out) {
         LC_REQUIRE(ctx && g && out, "lc_psf3d_kernel_hat: NULL argument");
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         ctx->c.set_current();
-        geom3d_kernel_hat(ctx->c, g->g, out);
+        geom3d_kernel_hat<float>(ctx->c, g->g, out);
+    });
+}
+
+int lc_psf3d_kernel_hat_f64(lc_ctx ctx, lc_geom3d g, double *out) {
+    return guard([&] {
+        LC_REQUIRE(ctx && g && out, "lc_psf3d_kernel_hat_f64: NULL argument");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        ctx->c.set_current();
+        geom3d_kernel_hat<double>(ctx->c, g->g, out);
     });
 }
 

@@ -113,16 +113,18 @@ void geom3d_init(Ctx &c, Geom3D &g, const double *theta, const double *gamma, co
 // FP64 closed-form PSF P(m) = sum_i D_ncols(m.u_i) D_nrows(m.v_i), FP64 3-D FFT,
 // stored as the FP32 real spectrum (f64: as the FP64 spectrum khat3_64)
 void geom3d_build_psf(Ctx &c, Geom3D &g, bool f64 = false);
-// reference-layout kernel_hat (complex {M1, M, M/2+1}, unscaled) to host
-void geom3d_kernel_hat(Ctx &c, Geom3D &g, float *out_host);
+// reference-layout kernel_hat (complex {M1, M, M/2+1}, unscaled) to host, as
+// float or double pairs
+template <typename OT> void geom3d_kernel_hat(Ctx &c, Geom3D &g, OT *out_host);
 
 void geom2d_init(Ctx &c, Geom2D &g, const double *theta, int64_t nang, int64_t nrad,
                  int64_t image_n);
 // FP64 closed-form PSF + FP64 FFT, stored as FP32 real spectrum (f64: as the
 // FP64 spectrum khatT64)
 void geom2d_build_psf(Ctx &c, Geom2D &g, bool f64 = false);
-// copy out the reference-layout kernel_hat (complex {M, M/2+1}, unscaled)
-void geom2d_kernel_hat(Ctx &c, Geom2D &g, float *out_host);
+// copy out the reference-layout kernel_hat (complex {M, M/2+1}, unscaled), as
+// float or double pairs
+template <typename OT> void geom2d_kernel_hat(Ctx &c, Geom2D &g, OT *out_host);
 
 // ---------------------------------------------------------------------------
 // device-side tables

@@ -278,9 +278,9 @@ void geom3d_build_psf(Ctx &c, Geom3D &g, bool f64) {
 // PSF (centred layout) for k3 < H, times the phase of the reference's storage
 // with the centre at h = floor(dims/2) on every axis:
 // exp(-2 pi i (k1 h1 / M1 + (k2 + k3) h / M)).  grid (ceil(H / B), M)
-template <int NT, int MAXE>
+template <typename OT, int NT, int MAXE>
 __global__ void __launch_bounds__(NT)
-k_khat3_export(const double2 *P, float2 *out, int M1, int M, int H, int B, FftPlan plan,
+k_khat3_export(const double2 *P, cpx<OT> *out, int M1, int M, int H, int B, FftPlan plan,
                const double2 *__restrict__ tw) {
     double2 *buf = reinterpret_cast<double2 *>(lc_smem);
     const int k30 = blockIdx.x * B;
@@ -301,11 +301,11 @@ k_khat3_export(const double2 *P, float2 *out, int M1, int M, int H, int B, FftPl
         double s, co;
         sincospi(-2.0 * ((double)p2 / (double)M + (double)p1 / (double)M1), &s, &co);
         out[((size_t)k1 * M + k2) * H + k3] =
-            make_float2((float)(v.x * co - v.y * s), (float)(v.x * s + v.y * co));
+            cmake<cpx<OT>>((OT)(v.x * co - v.y * s), (OT)(v.x * s + v.y * co));
     }
 }
 
-void geom3d_kernel_hat(Ctx &c, Geom3D &g, float *out_host) {
+template <typename OT> void geom3d_kernel_hat(Ctx &c, Geom3D &g, OT *out_host) {
     if (g.n2 != g.n3)
         throw Error(LC_EUNSUP, "3-D Toeplitz operator: recon dims n2 != n3 are not supported");
     const int M1 = (int)g.M1, M = (int)g.M, H = (int)g.H;
@@ -334,17 +334,19 @@ void geom3d_kernel_hat(Ctx &c, Geom3D &g, float *out_host) {
         LC_HIP(hipGetLastError());
     }
     const size_t n = (size_t)M1 * M * H;
-    float2 *tmp = reinterpret_cast<float2 *>(c.scratch(1, sizeof(float2) * n));
+    cpx<OT> *tmp = reinterpret_cast<cpx<OT> *>(c.scratch(1, sizeof(cpx<OT>) * n));
     {
         const int B = std::max(1, std::min(NT * MAXE / M1, 64));
-        hipLaunchKernelGGL((k_khat3_export<NT, MAXE>), dim3(ceil_div(H, B), M), dim3(NT),
+        hipLaunchKernelGGL((k_khat3_export<OT, NT, MAXE>), dim3(ceil_div(H, B), M), dim3(NT),
                            sizeof(double2) * lds_round(B * M1), c.stream(), P, tmp, M1, M, H, B,
                            plan1, tw1);
         LC_HIP(hipGetLastError());
     }
-    LC_HIP(hipMemcpyAsync(out_host, tmp, sizeof(float2) * n, hipMemcpyDeviceToHost, c.stream()));
+    LC_HIP(hipMemcpyAsync(out_host, tmp, sizeof(cpx<OT>) * n, hipMemcpyDeviceToHost, c.stream()));
     c.sync();
 }
+template void geom3d_kernel_hat<float>(Ctx &, Geom3D &, float *);
+template void geom3d_kernel_hat<double>(Ctx &, Geom3D &, double *);
 
 // ---------------------------------------------------------------------------
 // 3-D Toeplitz middle pass

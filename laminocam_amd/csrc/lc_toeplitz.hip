@@ -161,8 +161,8 @@ __global__ void __launch_bounds__(NT) k_psf_cols(const double2 *P, OT *khatT, in
 // with the centre at index h = floor(M/2): exp(-2 pi i (kx + ky) h / M).
 // For even M the row and column m = -M/2 have no +M/2 partner, so this
 // spectrum is complex; those entries never reach the cropped convolution.
-template <int NT, int MAXE>
-__global__ void __launch_bounds__(NT) k_khat_export(const double2 *P, float2 *out, int M, int H,
+template <typename OT, int NT, int MAXE>
+__global__ void __launch_bounds__(NT) k_khat_export(const double2 *P, cpx<OT> *out, int M, int H,
                                                     int B, FftPlan plan,
                                                     const double2 *__restrict__ tw) {
     double2 *buf = reinterpret_cast<double2 *>(lc_smem);
@@ -181,7 +181,7 @@ __global__ void __launch_bounds__(NT) k_khat_export(const double2 *P, float2 *ou
         const long long ph = ((long long)(kx + ky) * (M / 2)) % M;
         double s, co;
         sincospi(-2.0 * (double)ph / (double)M, &s, &co);
-        out[(size_t)ky * H + kx] = make_float2((float)(v.x * co - v.y * s), (float)(v.x * s + v.y * co));
+        out[(size_t)ky * H + kx] = cmake<cpx<OT>>((OT)(v.x * co - v.y * s), (OT)(v.x * s + v.y * co));
     }
 }
 
@@ -241,7 +241,7 @@ void geom2d_build_psf(Ctx &c, Geom2D &g, bool f64) {
     else g.psf_ready = true;
 }
 
-void geom2d_kernel_hat(Ctx &c, Geom2D &g, float *out_host) {
+template <typename OT> void geom2d_kernel_hat(Ctx &c, Geom2D &g, OT *out_host) {
     const int M = (int)g.M, H = (int)g.H;
     FftPlan plan;
     LC_REQUIRE(fft_make_plan(M, plan), "PSF size is not 2^p 3^q");
@@ -256,15 +256,17 @@ void geom2d_kernel_hat(Ctx &c, Geom2D &g, float *out_host) {
     hipLaunchKernelGGL((k_rows_c2c<double, -1, NT, MAXE>), dim3(ceil_div(M, nl)), dim3(NT),
                        sizeof(double2) * lds_round(nl * M), c.stream(), P, M, nl, plan, tw);
     LC_HIP(hipGetLastError());
-    float2 *tmp = reinterpret_cast<float2 *>(c.scratch(1, sizeof(float2) * (size_t)M * H));
+    cpx<OT> *tmp = reinterpret_cast<cpx<OT> *>(c.scratch(1, sizeof(cpx<OT>) * (size_t)M * H));
     const int B = std::max(1, NT * MAXE / M);
-    hipLaunchKernelGGL((k_khat_export<NT, MAXE>), dim3(ceil_div(H, B)), dim3(NT),
+    hipLaunchKernelGGL((k_khat_export<OT, NT, MAXE>), dim3(ceil_div(H, B)), dim3(NT),
                        sizeof(double2) * lds_round(B * M), c.stream(), P, tmp, M, H, B, plan, tw);
     LC_HIP(hipGetLastError());
-    LC_HIP(hipMemcpyAsync(out_host, tmp, sizeof(float2) * (size_t)M * H, hipMemcpyDeviceToHost,
+    LC_HIP(hipMemcpyAsync(out_host, tmp, sizeof(cpx<OT>) * (size_t)M * H, hipMemcpyDeviceToHost,
                           c.stream()));
     c.sync();
 }
+template void geom2d_kernel_hat<float>(Ctx &, Geom2D &, float *);
+template void geom2d_kernel_hat<double>(Ctx &, Geom2D &, double *);
 
 // ----------------------------------------------------------------------------
 // launch configuration: a few compiled block shapes per pass; a variant picks

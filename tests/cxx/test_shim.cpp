@@ -121,6 +121,36 @@ int main() {
         double md = 0.0;
         for (size_t i = 0; i < fd.size(); ++i) md = std::fmax(md, std::fabs(fd[i] - fx[i]));
         CHECK("double arrays", md < 1e-5);
+        // Array<double> runs the FP64 Toeplitz operator, kernel_hat, solvers
+        // and differences: A^T A is symmetric to FP64 rounding
+        cpu::PointSpreadFunction2D<double> psfd(gridd, N);
+        Array<double> ud = random_array<double>(dims_t{2, N, N}), wd = random_array<double>(dims_t{2, N, N});
+        Array<double> aud = sysmat2d(ud, psfd), awd = sysmat2d(wd, psfd);
+        const double ld = dot(aud, wd), rd_ = dot(ud, awd);
+        std::printf("  fp64 toeplitz symmetry %.3e\n", std::fabs(ld - rd_) / std::fabs(ld));
+        CHECK("fp64 toeplitz symmetric", std::fabs(ld - rd_) <= 1e-12 * std::fabs(ld));
+        Array<std::complex<double>> khd = psfd.kernel_hat();
+        double kdiff = 0.0, knorm = 0.0;
+        for (size_t i = 0; i < khd.size(); ++i) {
+            kdiff += std::norm(khd[i] - std::complex<double>(kh[i]));
+            knorm += std::norm(khd[i]);
+        }
+        CHECK("fp64 kernel_hat", std::sqrt(kdiff / knorm) < 1e-6);
+        Array<double> yd(dims_t{1, N, N});
+        for (size_t i = 0; i < yd.size(); ++i) yd[i] = y[i];
+        Array<double> x0d(dims_t{1, N, N});
+        Array<double> x5d = opt::cgsolver(opt::toeplitz(psfd), yd, x0d, 5, 1e-12, 0.0);
+        double cdiff = 0.0, cnorm = 0.0;
+        for (size_t i = 0; i < x5d.size(); ++i) {
+            cdiff += (x5d[i] - x5[i]) * (x5d[i] - x5[i]);
+            cnorm += x5d[i] * x5d[i];
+        }
+        std::printf("  cg 5 it fp64 vs fp32 %.3e\n", std::sqrt(cdiff / cnorm));
+        CHECK("fp64 cgsolver", std::sqrt(cdiff / cnorm) < 1e-4);
+        Array<double> vd = random_array<double>(dims_t{5, 32, 32});
+        auto gd = opt::grad_u(vd);
+        CHECK("fp64 laplacian == divergence(grad)",
+              rel_l2(opt::laplacian(vd), opt::divergence(gd)) < 1e-14);
     }
     // ---- 3-D ----
     {

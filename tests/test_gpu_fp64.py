@@ -63,6 +63,30 @@ def test_toeplitz2d_fp64(lc, orc64, N, nang, ns):
     assert y32.dtype == np.float32 and rel_l2(y32, y) < 1e-5
 
 
+@pytest.mark.parametrize("N,nang", [(16, 7), (33, 17), (64, 90), (96, 45)])
+def test_kernel_hat_fp64(lc, orc64, N, nang):
+    # PointSpreadFunction2D<double>::kernel_hat(): the FP64 spectrum, also for
+    # asymmetric angle sets where it is complex
+    announce(f"fp64 kernel_hat N={N} nang={nang}")
+    for th in (ref_theta(nang), np.arange(nang) * np.pi / nang + 0.05):
+        psf = lc.PointSpreadFunction2D(lc.PolarGrid2D(th, N), N)
+        kh = psf.kernel_hat(np.complex128)
+        err = rel_l2(kh, orc64.psf2d(th, N, N, method=0).kernel_hat())
+        print(f"rel_l2 = {err:.3e}")
+        assert kh.dtype == np.complex128 and err < 1e-12
+
+
+def test_kernel_hat3d_fp64(lc, orc64):
+    announce("fp64 kernel_hat3d")
+    th, ga, be = lamino_angles(11)
+    dims = (3, 33, 33)
+    psf = lc.PointSpreadFunction(lc.PolarGrid(th, ga, be, 12, 30), dims)
+    kh = psf.kernel_hat(np.complex128)
+    err = rel_l2(kh, orc64.psf3d(th, ga, be, 12, 30, dims, method=0).kernel_hat())
+    print(f"rel_l2 = {err:.3e}")
+    assert err < 1e-12
+
+
 SHAPES3 = [((5, 16, 16), 7, 9, 16, -30.0), ((3, 33, 33), 11, 12, 30, -30.0),
            ((8, 40, 40), 30, 16, 40, 20.0), ((1, 24, 24), 9, 6, 24, -30.0)]
 
