@@ -230,6 +230,12 @@ def served_stream(args, lc, synth, ctx):
             x, rep, tm = srv.result(tickets[i], timeout=600.0)
             out.append(tm)
         st = srv.stats()
+        # capacity: the same requests submitted at once, timed to the last result
+        t_b0 = time.perf_counter()
+        tk = [srv.submit(psf, sino=sino, tol=args.tol, xtol=args.xtol, **kw)
+              for psf, sino, kw, _, _, _ in reqs]
+        burst = [srv.result(t, timeout=600.0)[2] for t in tk]
+        t_b = time.perf_counter() - t_b0
     lat = np.array([tm[2] - tm[0] for tm in out])
     t_first = min(tm[0] for tm in out)
     t_last = max(tm[2] for tm in out)
@@ -261,6 +267,10 @@ def served_stream(args, lc, synth, ctx):
         "mix": {"3d": int(sum(r[4] for r in reqs)), "split_bregman": int(sum(r[5] for r in reqs))},
         "plain_same_mix": {"slices_per_s": round(plain_rate, 2), "sample_requests": k},
         "speedup_vs_plain_same_mix": round(slices / (t_last - t_first) / plain_rate, 2),
+        "capacity": {"slices_per_s": round(slices / t_b, 2), "seconds": round(t_b, 3),
+                     "speedup_vs_plain_same_mix": round(slices / t_b / plain_rate, 2),
+                     "latency_ms_p50": round(1e3 * float(np.median([b[2] - b[0] for b in burst])), 2),
+                     "note": "every request of the stream submitted at once"},
         "lanes": args.serve_lanes, "budget_gb": args.serve_budget_gb, "batches": int(st["batches"]),
         "lane_seconds": {k[7:]: round(v, 3) for k, v in st.items() if k.startswith("lane_s_")},
         "workload": "Poisson arrivals; 2-D slices N 128-1024 x 180/360/720 angles and 3-D volumes "
@@ -394,6 +404,8 @@ def main():
     ap.add_argument("--serve-lanes", type=int, default=3)
     ap.add_argument("--serve-budget-gb", type=float, default=96.0)
     ap.add_argument("--sb-outer", type=int, default=20)
+    ap.add_argument("--sb-steps", type=int, default=2,
+                    help="timed batched Split-Bregman solves on the slices (0: skip)")
     ap.add_argument("--serve-plain-sample", type=int, default=60,
                     help="served stream: requests re-solved one at a time for the plain rate")
     ap.add_argument("--no-serve", action="store_true", help="skip the served stream")
@@ -524,6 +536,51 @@ def main():
     log(f"plain: {plain_rate:.1f} slices/s over {len(plain_idx)} slices, max |x_plain - x_batched| "
         f"= {max_diff:.3e}")
 
+    # ---- Split-Bregman TV on the same slices: batched vs plain ----
+    sb = None
+    if args.sb_steps > 0:
+        sb_kw = dict(lam=0.1, mu=10.0, outer=args.sb_outer, inner=1)
+        batch.split_bregman(sb_kw["lam"], sb_kw["mu"], sb_kw["outer"], 1, args.tol, args.xtol,
+                            graph=True, x0_zero=True)   # warm-up
+        ctx.sync()
+        t_sb = []
+        for _ in range(args.sb_steps):
+            ctx.sync()
+            t0 = time.perf_counter()
+            batch.split_bregman(sb_kw["lam"], sb_kw["mu"], sb_kw["outer"], 1, args.tol, args.xtol,
+                                graph=True, x0_zero=True)
+            ctx.sync()
+            t_sb.append(time.perf_counter() - t0)
+        outer_done = np.array([r["outer"] for r in batch.reports()])
+        xsb = {int(i): batch.get(int(i), batch.X) for i in plain_idx}
+        lat_sb, diff_sb = [], 0.0
+        for i in plain_idx:
+            t0 = time.perf_counter()
+            x, _ = lc.split_bregman(psfs[i], rhs[i], np.zeros_like(rhs[i]), sb_kw["lam"],
+                                    sb_kw["mu"], sb_kw["outer"], 1, args.tol, args.xtol)
+            lat_sb.append(time.perf_counter() - t0)
+            diff_sb = max(diff_sb, float(np.max(np.abs(x - xsb[int(i)]))))
+        tn = {}
+        for i, t in zip(plain_idx, lat_sb):
+            tn.setdefault(int(Ns[i]), []).append(t)
+        tn = {n: float(np.mean(v)) for n, v in tn.items()}
+        if len(tn) == len(set(int(n) for n in Ns)):
+            sb_plain_total = sum(tn[int(n)] for n in Ns)
+        else:
+            sb_plain_total = float(np.mean(lat_sb)) * len(Ns)
+        sb_rate = len(Ns) / float(np.mean(t_sb))
+        sb_plain_rate = len(Ns) / sb_plain_total
+        sb = {"slices_per_s": round(sb_rate, 2), "ms_per_step": round(1e3 * float(np.mean(t_sb)), 3),
+              "plain_slices_per_s": round(sb_plain_rate, 2),
+              "speedup_vs_plain": round(sb_rate / sb_plain_rate, 2),
+              "outer_mean": round(float(outer_done.mean()), 2),
+              "max_abs_diff_plain_vs_batched": diff_sb,
+              "workload": f"the {len(Ns)} slices above, Split-Bregman isotropic TV, lambda "
+                          f"{sb_kw['lam']}, mu {sb_kw['mu']}, {sb_kw['outer']} outer x 1 inner, "
+                          f"x0 = 0; plain: the first {len(plain_idx)} slices one at a time"}
+        log(f"split-bregman: batched {sb_rate:.1f} slices/s, plain {sb_plain_rate:.1f} slices/s, "
+            f"max |diff| {diff_sb:.3e}")
+
     # ---- relative error vs the FP64 oracle on the smallest slices ----
     # Long CG solves on these under-sampled systems amplify rounding: two FP64
     # runs whose right-hand sides differ by 1e-15 relative drift apart too.
@@ -636,6 +693,7 @@ def main():
                                     "solves of these under-sampled systems drift by this much "
                                     "from rounding alone"},
         "fp64_batched": fp64,
+        "split_bregman_batched": sb,
         "served": served,
         "lamino3d": lamino,
         "roofline": roofline,

@@ -213,6 +213,7 @@ int lc_batch2d_get_f64(lc_batch2d b, int32_t prob, int32_t which, double *dst, u
 int lc_batch2d_device_ptr_f64(lc_batch2d b, int32_t prob, int32_t which, double **out);
 int lc_batch2d_cg(lc_batch2d b, const int32_t *max_iter, double tol, double xtol,
                   uint32_t flags);
+/* flags: LC_GRAPH, LC_TV_ZERO_PAD, LC_TV_ANISO, LC_X0_ZERO (start every problem from x = 0) */
 int lc_batch2d_split_bregman(lc_batch2d b, const float *lambda, const float *mu, int64_t outer,
                              int64_t inner, double tol, double xtol, uint32_t flags);
 int lc_batch2d_reports(lc_batch2d b, lc_report *reps);

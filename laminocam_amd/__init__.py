@@ -564,14 +564,14 @@ class Batch2D:
         _lib.check(self.lib.lc_batch2d_cg(self.handle, arr, float(tol), float(xtol), flags))
 
     def split_bregman(self, lam, mu, outer, inner, tol=1e-5, xtol=1e-5, zero_pad=False,
-                      aniso=False, graph=True):
+                      aniso=False, graph=True, x0_zero=False):
         n = len(self)
         la = np.ascontiguousarray(np.broadcast_to(np.float32(lam), (n,)) if np.isscalar(lam) else lam,
                                   dtype=np.float32)
         m = np.ascontiguousarray(np.broadcast_to(np.float32(mu), (n,)) if np.isscalar(mu) else mu,
                                  dtype=np.float32)
         flags = (LC_GRAPH if graph else 0) | (LC_TV_ZERO_PAD if zero_pad else 0) | \
-            (LC_TV_ANISO if aniso else 0)
+            (LC_TV_ANISO if aniso else 0) | (LC_X0_ZERO if x0_zero else 0)
         _lib.check(self.lib.lc_batch2d_split_bregman(
             self.handle, la.ctypes.data_as(C.POINTER(C.c_float)),
             m.ctypes.data_as(C.POINTER(C.c_float)), int(outer), int(inner), float(tol),

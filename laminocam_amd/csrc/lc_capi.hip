@@ -743,7 +743,7 @@ int lc_batch2d_split_bregman(lc_batch2d b, const float *lambda, const float *mu,
         const bool graph = flags & LC_GRAPH;
         b->b->split_bregman(la, m, (int)outer, (int)inner, tol, xtol,
                             (flags & LC_TV_ZERO_PAD) ? 1 : 0, (flags & LC_TV_ANISO) != 0, graph,
-                            false);
+                            false, (flags & LC_X0_ZERO) != 0);
         auto st = b->b->cg_state();
         auto sb = b->b->sb_state();
         for (int i = 0; i < np; ++i) {

@@ -941,7 +941,7 @@ void Batch2D::sb_outer_t(bool hsync, double tol, double xtol, int inner, const i
 
 void Batch2D::split_bregman(const std::vector<float> &lambda, const std::vector<float> &mu,
                             int outer, int inner, double tol, double xtol, int boundary,
-                            bool aniso, bool graph, bool host_sync) {
+                            bool aniso, bool graph, bool host_sync, bool x0_zero) {
     const int np = nprob();
     LC_REQUIRE((int)lambda.size() == np && (int)mu.size() == np, "sb: one lambda/mu per problem");
     sb_boundary_ = boundary;
@@ -965,6 +965,9 @@ void Batch2D::split_bregman(const std::vector<float> &lambda, const std::vector<
                            3 * total_);
         hipLaunchKernelGGL(k_fill<R>, dim3(1024), dim3(256), 0, c_.stream(), Bv_.as<R>(), (R)0,
                            3 * total_);
+        if (x0_zero)
+            hipLaunchKernelGGL(k_fill<R>, dim3(1024), dim3(256), 0, c_.stream(), X_.as<R>(), (R)0,
+                               total_);
     });
     LC_HIP(hipGetLastError());
     LC_HIP(hipMemcpyAsync(XO_.as(), X_.as(), fb, hipMemcpyDeviceToDevice, c_.stream()));

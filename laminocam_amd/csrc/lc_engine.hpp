@@ -304,10 +304,11 @@ class Batch2D {
     // kernel (the plain, one-solve-at-a-time mode).
     void cg(const std::vector<int> &max_iter, double tol, double xtol, bool graph,
             bool host_sync, bool x0_zero);
-    // Split-Bregman TV (src/bregman.cpp:36-94) on every problem
+    // Split-Bregman TV (src/bregman.cpp:36-94) on every problem, starting from
+    // X() (x0_zero: from x = 0, X() cleared first)
     void split_bregman(const std::vector<float> &lambda, const std::vector<float> &mu,
                        int outer, int inner, double tol, double xtol, int boundary,
-                       bool aniso, bool graph, bool host_sync);
+                       bool aniso, bool graph, bool host_sync, bool x0_zero = false);
 
     std::vector<CgState> cg_state();
     std::vector<SbState> sb_state();

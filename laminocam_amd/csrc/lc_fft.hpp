@@ -247,14 +247,26 @@ constexpr int ct_radix(int L, int s) {
     return 3;
 }
 
-template <typename T, int L, int R, int NS, int SIGN, int NT, int MAXE, bool LFAST>
+// Swizzled LDS addresses of a thread's R elements x0 + r*S.  When the stride
+// S is a multiple of 256 the swizzle (bits 5-7 folded into bits 2-4) commutes
+// with it: swz(x0 + r*S) = swz(x0) + r*S, so one swizzle serves all R and the
+// rest become constant offsets.
+template <int S> __device__ __forceinline__ int swz_step(int sx0, int x0, int r) {
+    if constexpr (S % 256 == 0) return sx0 + r * S;
+    else return swz(x0 + r * S);
+}
+
+// LSH >= 0: the column layout's log2(lines) is a compile-time constant (all
+// strides fold); LSH < 0: it is the runtime lsh.
+template <typename T, int L, int R, int NS, int SIGN, int NT, int MAXE, bool LFAST, int LSH>
 __device__ __forceinline__ void fft_stage_ct(cpx<T> *buf, int nl, int lsh,
                                              const cpx<T> *__restrict__ tw) {
     using C = cpx<T>;
     constexpr int MAXB = (MAXE + R - 1) / R;
     constexpr int LR = L / R;
     constexpr int TWS = L / (NS * R);
-    const int nbf = nl * LR;
+    const int ls = LSH >= 0 ? LSH : lsh;
+    const int nbf = LSH >= 0 ? (LR << (LSH >= 0 ? LSH : 0)) : nl * LR;
     const int tid = threadIdx.x;
     C v[MAXB][R];
     C wa[MAXB], wb[MAXB];
@@ -265,7 +277,7 @@ __device__ __forceinline__ void fft_stage_ct(cpx<T> *buf, int nl, int lsh,
         for (int t = 0; t < MAXB; ++t) {
             int b = tid + t * NT;
             b = b < nbf ? b : nbf - 1;
-            const int j = LFAST ? (b >> lsh) : (b % LR);
+            const int j = LFAST ? (b >> ls) : (b % LR);
             const int k = j % NS;
             wa[t] = tw[k * TWS];
             if constexpr (R == 8) wb[t] = tw[4 * k * TWS];
@@ -275,12 +287,23 @@ __device__ __forceinline__ void fft_stage_ct(cpx<T> *buf, int nl, int lsh,
     for (int t = 0; t < MAXB; ++t) {
         const int b = tid + t * NT;
         if (b < nbf) {
-            int j, line;
-            if (LFAST) { j = b >> lsh; line = b & ((1 << lsh) - 1); }
-            else { line = b / LR; j = b - line * LR; }
+            if (LFAST) {
+                // element j + r LR of line `line` sits at ((j + r LR) << ls) + line = b + r (LR << ls)
+                if constexpr (LSH >= 0) {
+                    constexpr int S = LR << (LSH >= 0 ? LSH : 0);
+                    const int sb = swz(b);
 #pragma unroll
-            for (int r = 0; r < R; ++r)
-                v[t][r] = buf[LFAST ? swz(((j + r * LR) << lsh) + line) : swz(line * L + j + r * LR)];
+                    for (int r = 0; r < R; ++r) v[t][r] = buf[swz_step<S>(sb, b, r)];
+                } else {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) v[t][r] = buf[swz(b + r * (LR << ls))];
+                }
+            } else {
+                const int line = b / LR, j = b - line * LR;
+                const int x0 = line * L + j, sx0 = swz(x0);
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[t][r] = buf[swz_step<LR>(sx0, x0, r)];
+            }
         }
     }
     __syncthreads();
@@ -289,7 +312,7 @@ __device__ __forceinline__ void fft_stage_ct(cpx<T> *buf, int nl, int lsh,
         const int b = tid + t * NT;
         if (b < nbf) {
             int j, line;
-            if (LFAST) { j = b >> lsh; line = b & ((1 << lsh) - 1); }
+            if (LFAST) { j = b >> ls; line = b & ((1 << ls) - 1); }
             else { line = b / LR; j = b - line * LR; }
             const int k = j % NS;
             if constexpr (NS > 1) {
@@ -310,29 +333,43 @@ __device__ __forceinline__ void fft_stage_ct(cpx<T> *buf, int nl, int lsh,
             }
             dftR<R, SIGN>(v[t]);
             const int o = (j - k) * R + k;
+            if (LFAST) {
+                const int ob = (o << ls) + line;
+                if constexpr (LSH >= 0) {
+                    constexpr int S = NS << (LSH >= 0 ? LSH : 0);
+                    const int sb = swz(ob);
 #pragma unroll
-            for (int r = 0; r < R; ++r)
-                buf[LFAST ? swz(((o + r * NS) << lsh) + line) : swz(line * L + o + r * NS)] = v[t][r];
+                    for (int r = 0; r < R; ++r) buf[swz_step<S>(sb, ob, r)] = v[t][r];
+                } else {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) buf[swz(ob + r * (NS << ls))] = v[t][r];
+                }
+            } else {
+                const int ob = line * L + o, sb = swz(ob);
+#pragma unroll
+                for (int r = 0; r < R; ++r) buf[swz_step<NS>(sb, ob, r)] = v[t][r];
+            }
         }
     }
     __syncthreads();
 }
 
-template <typename T, int L, int S, int NS, int SIGN, int NT, int MAXE, bool LFAST>
+template <typename T, int L, int S, int NS, int SIGN, int NT, int MAXE, bool LFAST, int LSH>
 __device__ __forceinline__ void fft_stages_ct(cpx<T> *buf, int nl, int lsh,
                                               const cpx<T> *__restrict__ tw) {
     constexpr int R = ct_radix(L, S);
-    fft_stage_ct<T, L, R, NS, SIGN, NT, MAXE, LFAST>(buf, nl, lsh, tw);
+    fft_stage_ct<T, L, R, NS, SIGN, NT, MAXE, LFAST, LSH>(buf, nl, lsh, tw);
     if constexpr (S + 1 < ct_nst(L))
-        fft_stages_ct<T, L, S + 1, NS * R, SIGN, NT, MAXE, LFAST>(buf, nl, lsh, tw);
+        fft_stages_ct<T, L, S + 1, NS * R, SIGN, NT, MAXE, LFAST, LSH>(buf, nl, lsh, tw);
 }
 
 // Full transform of nl lines of compile-time length L (row layout: line
-// stride L; column layout: nl = 1 << lsh lines interleaved).
-template <typename T, int L, int SIGN, int NT, int MAXE, bool LFAST>
+// stride L; column layout: nl = 1 << lsh lines interleaved, lsh = LSH when
+// LSH >= 0).
+template <typename T, int L, int SIGN, int NT, int MAXE, bool LFAST, int LSH = -1>
 __device__ __forceinline__ void fft_lds_ct(cpx<T> *buf, int nl, int lsh,
                                            const cpx<T> *__restrict__ tw) {
-    fft_stages_ct<T, L, 0, 1, SIGN, NT, MAXE, LFAST>(buf, nl, lsh, tw);
+    fft_stages_ct<T, L, 0, 1, SIGN, NT, MAXE, LFAST, LSH>(buf, nl, lsh, tw);
 }
 
 // FP64 twiddle generation: tw[k] = exp(-2 pi i k / L)

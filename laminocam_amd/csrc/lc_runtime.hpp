@@ -1,6 +1,7 @@
 // Device context, buffers and twiddle cache.
 #pragma once
 
+#include <algorithm>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -61,6 +62,40 @@ class MemPool {
 
 // allocations made by this thread go through this pool when set
 inline thread_local MemPool *tl_pool = nullptr;
+
+// Grow-only pinned host buffer (hipHostMalloc): staging for asynchronous
+// host <-> device copies.  Grows by at least 1.5x so that a sequence of
+// batches of similar size reallocates rarely.
+class PinnedBuf {
+  public:
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf &) = delete;
+    PinnedBuf &operator=(const PinnedBuf &) = delete;
+    ~PinnedBuf() {
+        if (p_) (void)hipHostFree(p_);
+    }
+    // a buffer of at least `bytes`; the caller has no copy in flight on it
+    void *get(size_t bytes) {
+        if (bytes > n_) {
+            const size_t want = std::max(bytes, n_ + n_ / 2);
+            if (p_) (void)hipHostFree(p_);
+            p_ = nullptr;
+            n_ = 0;
+            hipError_t e = hipHostMalloc(&p_, want, hipHostMallocDefault);
+            if (e != hipSuccess) {
+                p_ = nullptr;
+                throw Error(LC_ENOMEM, "hipHostMalloc of " + std::to_string(want) +
+                                           " bytes failed: " + hipGetErrorString(e));
+            }
+            n_ = want;
+        }
+        return p_;
+    }
+
+  private:
+    void *p_ = nullptr;
+    size_t n_ = 0;
+};
 
 // RAII device allocation
 class DBuf {

@@ -205,7 +205,7 @@ void Server::lane_loop(Lane &lane) {
         }
         std::string err;
         try {
-            run_batch(c, batch);
+            run_batch(lane, batch);
         } catch (const std::exception &e) {
             err = e.what();
         }
@@ -226,7 +226,8 @@ void Server::lane_loop(Lane &lane) {
     }
 }
 
-void Server::run_batch(Ctx &c, std::vector<ServerRequest *> &batch) {
+void Server::run_batch(Lane &lane, std::vector<ServerRequest *> &batch) {
+    Ctx &c = *lane.ctx;
     std::lock_guard<std::mutex> lk(c.mu);
     c.set_current();
     const double t0 = now_s();
@@ -236,34 +237,47 @@ void Server::run_batch(Ctx &c, std::vector<ServerRequest *> &batch) {
     Batch2D b(c, ps);
     const double t1 = now_s();
     // inputs: right-hand sides (or sinograms back-projected into y), initial
-    // guesses.  All sinograms of the batch share one staging buffer, so the
-    // uploads and back-projections queue on the stream without host waits.
-    size_t sino_total = 0;
-    for (ServerRequest *r : batch)
-        if (r->has_sino) sino_total += (r->sino.size() + 63) & ~(size_t)63;
-    DBuf sino;
-    if (sino_total) sino.alloc(sizeof(float) * sino_total);
-    size_t soff = 0;
+    // guesses.  Every host array of the batch is packed into the lane's pinned
+    // staging buffer and uploaded with one asynchronous copy per array into
+    // one device staging region, so uploads and back-projections queue on the
+    // stream without host waits.
+    auto pad = [](size_t n) { return (n + 63) & ~(size_t)63; };
+    size_t in_total = 0;
     bool any_x0 = false;
+    for (ServerRequest *r : batch) {
+        in_total += pad(r->has_sino ? r->sino.size() : r->rhs.size()) + pad(r->x0.size());
+        if (!r->x0.empty()) any_x0 = true;
+    }
+    float *hst = static_cast<float *>(lane.stage.get(sizeof(float) * std::max<size_t>(in_total, 1)));
+    DBuf dst_in;
+    dst_in.alloc(sizeof(float) * std::max<size_t>(in_total, 1));
+    float *dst = dst_in.as<float>();
+    size_t off = 0;
     for (size_t i = 0; i < batch.size(); ++i) {
         ServerRequest *r = batch[i];
+        const std::vector<float> &src = r->has_sino ? r->sino : r->rhs;
+        std::memcpy(hst + off, src.data(), sizeof(float) * src.size());
+        float *s = dst + off;
+        copy_in(c, s, hst + off, sizeof(float) * src.size(), false);
+        off += pad(src.size());
         float *y = b.Y() + b.offset((int)i);
         if (r->has_sino) {
-            float *s = sino.as<float>() + soff;
-            soff += (r->sino.size() + 63) & ~(size_t)63;
-            copy_in(c, s, r->sino.data(), sizeof(float) * r->sino.size(), false);
             if (r->g3) backproj3d_dev(c, *r->g3, s, y, nullptr);
             else backproj2d_dev(c, *r->g2, s, y, r->ns);
         } else {
-            copy_in(c, y, r->rhs.data(), sizeof(float) * r->rhs.size(), false);
+            LC_HIP(hipMemcpyAsync(y, s, sizeof(float) * src.size(), hipMemcpyDeviceToDevice,
+                                  c.stream()));
         }
-        if (!r->x0.empty()) any_x0 = true;
-    }
-    for (size_t i = 0; i < batch.size(); ++i) {
-        ServerRequest *r = batch[i];
         float *x = b.X() + b.offset((int)i);
-        if (!r->x0.empty()) copy_in(c, x, r->x0.data(), sizeof(float) * r->x0.size(), false);
-        else LC_HIP(hipMemsetAsync(x, 0, sizeof(float) * (size_t)b.size((int)i), c.stream()));
+        if (!r->x0.empty()) {
+            std::memcpy(hst + off, r->x0.data(), sizeof(float) * r->x0.size());
+            copy_in(c, dst + off, hst + off, sizeof(float) * r->x0.size(), false);
+            LC_HIP(hipMemcpyAsync(x, dst + off, sizeof(float) * r->x0.size(),
+                                  hipMemcpyDeviceToDevice, c.stream()));
+            off += pad(r->x0.size());
+        } else {
+            LC_HIP(hipMemsetAsync(x, 0, sizeof(float) * (size_t)b.size((int)i), c.stream()));
+        }
     }
     c.sync();
     const double t2 = now_s();
@@ -285,10 +299,15 @@ void Server::run_batch(Ctx &c, std::vector<ServerRequest *> &batch) {
     const double t3 = now_s();
     const auto cs = b.cg_state();
     const auto ss = h.solver == 1 ? b.sb_state() : std::vector<SbState>();
+    // results: the batch's whole solution buffer in one copy into pinned
+    // staging (problems are contiguous), then per request on the host
+    float *hx = static_cast<float *>(lane.stage.get(sizeof(float) * (size_t)b.total()));
+    copy_out(c, hx, b.X(), sizeof(float) * (size_t)b.total(), false);
+    c.sync();
     for (size_t i = 0; i < batch.size(); ++i) {
         ServerRequest *r = batch[i];
         r->x.resize((size_t)b.size((int)i));
-        copy_out(c, r->x.data(), b.X() + b.offset((int)i), sizeof(float) * r->x.size(), false);
+        std::memcpy(r->x.data(), hx + b.offset((int)i), sizeof(float) * r->x.size());
         r->rep.iters = cs[i].iters;
         r->rep.outer = h.solver == 1 ? ss[i].outer : 0;
         r->rep.res = cs[i].res;

@@ -56,11 +56,12 @@ class Server {
   private:
     struct Lane {
         MemPool pool;                // outlives the context's buffers
+        PinnedBuf stage;             // host staging of the lane's inputs and results
         std::unique_ptr<Ctx> ctx;
         std::thread th;
     };
     void lane_loop(Lane &lane);
-    void run_batch(Ctx &c, std::vector<ServerRequest *> &batch);
+    void run_batch(Lane &lane, std::vector<ServerRequest *> &batch);
 
     Ctx &main_;
     size_t budget_;
