@@ -31,12 +31,31 @@ namespace {
 
 constexpr int W3 = 8;                       // kernel width (cells)
 constexpr int TZ = 8, TY = 16, TX = 16;     // spread tile (cells)
+static_assert(TX == 16, "tile_idx swizzles x by XOR with 8 within a row of 16");
 constexpr int S3_WAVES = 4;
 constexpr int LN_NT = 256, LN_ME = 32;      // line passes: NT * ME complex in LDS
 
 __device__ __forceinline__ float es3(float z, float beta) {
     const float s = 1.0f - z * z;
     return s > 0.0f ? __expf(beta * (sqrtf(s) - 1.0f)) : 0.0f;
+}
+
+// value of v in lane `lane` (the same for every lane: a scalar broadcast, no LDS)
+__device__ __forceinline__ float rl_f32(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ __forceinline__ double rl_f64(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// spread tile cell (z, y, x) -> LDS index.  x is XOR-ed with bit 1 of y (times
+// 8): the 4 rows x 8 columns a half-wave updates per z step then fall on
+// distinct bank pairs (plain rows y and y + 2 share banks).
+__device__ __forceinline__ int tile_idx(int z, int y, int x) {
+    return (z * TY + y) * TX + (x ^ (((y >> 1) & 1) << 3));
 }
 
 __device__ __forceinline__ int wrapi(int a, int n) {
@@ -205,8 +224,10 @@ k_spread3d(const float2 *__restrict__ C, float2 *__restrict__ F, const double *_
                         const int src = __ffsll((long long)hits) - 1;
                         hits &= hits - 1;
                         const int pp = p0 + src;
-                        const int hk0 = __shfl(k0, src, 64), hk1 = __shfl(k1, src, 64);
-                        const int hj0 = __shfl(j0, src, 64), hj1 = __shfl(j1, src, 64);
+                        const int hk0 = __builtin_amdgcn_readlane(k0, src),
+                                  hk1 = __builtin_amdgcn_readlane(k1, src);
+                        const int hj0 = __builtin_amdgcn_readlane(j0, src),
+                                  hj1 = __builtin_amdgcn_readlane(j1, src);
                         const int nk = hk1 - hk0 + 1;
                         const int ncand = nk * (hj1 - hj0 + 1);
                         double sdx = 0.0, sdy = 0.0;
@@ -249,9 +270,10 @@ k_spread3d(const float2 *__restrict__ C, float2 *__restrict__ F, const double *_
                             while (acc_pts) {
                                 const int s2 = __ffsll((long long)acc_pts) - 1;
                                 acc_pts &= acc_pts - 1;
-                                const double pz = __shfl(gz, s2, 64), py = __shfl(gy, s2, 64),
-                                             px = __shfl(gx, s2, 64);
-                                const float vr = __shfl(cv.x, s2, 64), vi = __shfl(cv.y, s2, 64);
+                                // s2 is uniform: scalar broadcasts
+                                const double pz = rl_f64(gz, s2), py = rl_f64(gy, s2),
+                                             px = rl_f64(gx, s2);
+                                const float vr = rl_f32(cv.x, s2), vi = rl_f32(cv.y, s2);
                                 const int lz0 = (int)ceil(pz - hw), ly0 = (int)ceil(py - hw),
                                           lx0 = (int)ceil(px - hw);
                                 const float wxy = es3((float)((double)(lx0 + dxl) - px) * ihw, g.beta) *
@@ -261,11 +283,11 @@ k_spread3d(const float2 *__restrict__ C, float2 *__restrict__ F, const double *_
                                 const bool inxy = cx >= 0 && cx < TX && cy >= 0 && cy < TY;
 #pragma unroll
                                 for (int dz = 0; dz < W3; ++dz) {
-                                    const float wz = __shfl(wzl, dz, 64);
+                                    const float wz = rl_f32(wzl, dz);   // lane dz holds wz[dz]
                                     const int cz = lz0 + dz - c0[0];
                                     if (inxy && cz >= 0 && cz < TZ) {
                                         const float wgt = wxy * wz;
-                                        float2 &a = my[(cz * TY + cy) * TX + cx];
+                                        float2 &a = my[tile_idx(cz, cy, cx)];
                                         a.x += wgt * vr;
                                         a.y += wgt * vi;
                                     }
@@ -282,7 +304,8 @@ k_spread3d(const float2 *__restrict__ C, float2 *__restrict__ F, const double *_
             s.x += acc[w * TC + i].x;
             s.y += acc[w * TC + i].y;
         }
-        const int cz = i / (TY * TX), cy = (i / TX) % TY, cx = i % TX;
+        // physical index i holds cell (cz, cy, cx) with cx un-swizzled
+        const int cz = i / (TY * TX), cy = (i / TX) % TY, cx = (i % TX) ^ (((cy >> 1) & 1) << 3);
         const int zz = c0[0] + cz, yy = c0[1] + cy, xx = c0[2] + cx;
         if (zz < g.nf[0] && yy < g.nf[1] && xx < g.nf[2])
             F[((int64_t)zz * g.nf[1] + yy) * g.nf[2] + xx] = s;
