@@ -159,13 +159,16 @@ def cpu_baseline(args, iters_by_slice, Ns, rhs_by_n):
     }
 
 
-def served_stream(args, lc, synth, ctx):
+def served_stream(args, lc, synth, ctx, frac3d=None, rate=None):
     """Served mixed stream: Poisson arrivals at a fixed rate of 2-D slices
     (N 128-1024, 180-720 angles) and 3-D laminography volumes (8-32 slices x
     256-511^2, 180-720 projections, 30 degree tilt), 70% CG / 30% Split-Bregman
     TV (lambda U[0.05, 0.2], mu 10), submitted as sinograms / projections to the
     reconstruction server (HBM-budget admission, continuous batching).  Inputs
-    come from a pool of seeded phantoms per geometry built before the clock."""
+    come from a pool of seeded phantoms per geometry built before the clock.
+    frac3d / rate override --serve-3d-frac / --serve-rate."""
+    frac3d = args.serve_3d_frac if frac3d is None else frac3d
+    rate = args.serve_rate if rate is None else rate
     import threading
     rng = synth.Stream((5 << 16) + 1)
     geo2, pool2 = [], []
@@ -179,6 +182,8 @@ def served_stream(args, lc, synth, ctx):
             geo2.append(psf)
             pool2.append(s / np.float32(np.max(np.abs(s))))
     vols = [((8, 256, 256), 180, 64), ((16, 384, 384), 360, 96), ((32, 511, 511), 720, 128)]
+    if frac3d <= 0.0:
+        vols = []
     geo3, pool3 = [], []
     for i, (dims, nproj, nrows) in enumerate(vols):
         th = np.arange(nproj) * (2 * np.pi / nproj)
@@ -189,16 +194,16 @@ def served_stream(args, lc, synth, ctx):
         p = synth.add_gaussian_noise(p, 0.01, synth.Stream((5 << 16) + 400 + i))
         geo3.append(psf)
         pool3.append(p / np.float32(np.max(np.abs(p))))
-    n = max(1, int(round(args.serve_rate * args.serve_seconds)))
-    gaps = -np.log(1.0 - rng.uniform(n)) / args.serve_rate
+    n = max(1, int(round(rate * args.serve_seconds)))
+    gaps = -np.log(1.0 - rng.uniform(n)) / rate
     kinds = rng.uniform(n)
     picks2 = rng.choice(list(range(len(geo2))), n)
-    picks3 = rng.choice(list(range(len(geo3))), n)
+    picks3 = rng.choice(list(range(max(1, len(geo3)))), n)
     solv = rng.uniform(n)
     lams = rng.uniform(n, 0.05, 0.2)
     reqs = []
     for i in range(n):
-        is3 = kinds[i] < args.serve_3d_frac
+        is3 = kinds[i] < frac3d
         psf = geo3[picks3[i]] if is3 else geo2[picks2[i]]
         sino = pool3[picks3[i]] if is3 else pool2[picks2[i]]
         sb = solv[i] < 0.3
@@ -209,7 +214,7 @@ def served_stream(args, lc, synth, ctx):
     with lc.Server(hbm_budget=int(args.serve_budget_gb * (1 << 30)), lanes=args.serve_lanes,
                    ctx=ctx) as srv:
         # warm the lanes' kernels and scratch on one request of each kind
-        for psf, sino in ((geo2[0], pool2[0]), (geo3[0], pool3[0])):
+        for psf, sino in [(geo2[0], pool2[0])] + ([(geo3[0], pool3[0])] if geo3 else []):
             srv.result(srv.submit(psf, sino=sino, max_iter=2), timeout=120.0)
         tickets = [None] * n
         t0 = time.perf_counter()
@@ -259,7 +264,7 @@ def served_stream(args, lc, synth, ctx):
             lc.cgsolver(psf, rhs, x0, kw["max_iter"], args.tol, args.xtol)
     plain_rate = sum(r[3] for r in reqs[:k]) / (time.perf_counter() - t0)
     return {
-        "requests": n, "rate_req_per_s": args.serve_rate, "seconds_of_arrivals": args.serve_seconds,
+        "requests": n, "rate_req_per_s": rate, "seconds_of_arrivals": args.serve_seconds,
         "slices": int(slices), "slices_per_s": round(slices / (t_last - t_first), 2),
         "requests_per_s": round(n / (t_last - t_first), 2),
         "latency_ms": {"p50": round(1e3 * float(np.median(lat)), 2),
@@ -273,8 +278,9 @@ def served_stream(args, lc, synth, ctx):
                      "note": "every request of the stream submitted at once"},
         "lanes": args.serve_lanes, "budget_gb": args.serve_budget_gb, "batches": int(st["batches"]),
         "lane_seconds": {k[7:]: round(v, 3) for k, v in st.items() if k.startswith("lane_s_")},
-        "workload": "Poisson arrivals; 2-D slices N 128-1024 x 180/360/720 angles and 3-D volumes "
-                    "8x256^2 / 16x384^2 / 32x511^2 (180/360/720 projections, 30 deg tilt); 70% CG "
+        "workload": f"Poisson arrivals; 2-D slices N 128-1024 x 180/360/720 angles and 3-D volumes "
+                    f"8x256^2 / 16x384^2 / 32x511^2 (180/360/720 projections, 30 deg tilt) as "
+                    f"{100 * frac3d:g}% of requests; 70% CG "
                     f"({args.iters} it), 30% Split-Bregman ({args.sb_outer} outer x 1 inner, "
                     "lambda U[0.05,0.2], mu 10); sinograms back-projected on the device",
     }
@@ -401,6 +407,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0, help="served stream: requests/s")
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="served stream: seconds of arrivals")
     ap.add_argument("--serve-3d-frac", type=float, default=0.05)
+    ap.add_argument("--serve-2d-rate", type=float, default=600.0,
+                    help="requests/s of the 2-D-only served stream (0: skip it)")
     ap.add_argument("--serve-lanes", type=int, default=3)
     ap.add_argument("--serve-budget-gb", type=float, default=96.0)
     ap.add_argument("--sb-outer", type=int, default=20)
@@ -432,6 +440,9 @@ def main():
 
     if args.serve_only:
         served = served_stream(args, lc, synth, lc.default_context())
+        if args.serve_2d_rate > 0:
+            served["two_d_only"] = served_stream(args, lc, synth, lc.default_context(),
+                                                 frac3d=0.0, rate=args.serve_2d_rate)
         print(json.dumps({"served": served}), flush=True)
         return
 
@@ -626,6 +637,14 @@ def main():
         log(f"served: {served['requests']} requests, {served['slices_per_s']} slices/s, latency "
             f"p50 {served['latency_ms']['p50']} ms p99 {served['latency_ms']['p99']} ms, "
             f"{served['batches']} batches")
+        if args.serve_2d_rate > 0:
+            # the same stream without 3-D volumes: the 2-D slices alone
+            served["two_d_only"] = served_stream(args, lc, synth, ctx, frac3d=0.0,
+                                                 rate=args.serve_2d_rate)
+            s2 = served["two_d_only"]
+            log(f"served 2-D only: {s2['slices_per_s']} slices/s (capacity "
+                f"{s2['capacity']['slices_per_s']}), plain {s2['plain_same_mix']['slices_per_s']}, "
+                f"p50 {s2['latency_ms']['p50']} ms p99 {s2['latency_ms']['p99']} ms")
 
     # ---- 3-D laminography solves ----
     lamino = None

@@ -31,6 +31,17 @@ constexpr int NU_W = 8;        // kernel width (cells); 64 lanes cover the footp
 constexpr int NU_T = 32;       // spread tile edge (cells)
 constexpr int NU_WAVES = 4;    // waves per spread workgroup
 
+// value of v in lane `lane` (uniform lane index: a scalar broadcast, no LDS)
+__device__ __forceinline__ float rl_f32(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ __forceinline__ double rl_f64(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 __device__ __forceinline__ float es_kernel(float z, float beta) {
     const float s = 1.0f - z * z;
     return s > 0.0f ? __expf(beta * (sqrtf(s) - 1.0f)) : 0.0f;
@@ -193,23 +204,48 @@ k_spread2d(const float2 *__restrict__ C, float2 *__restrict__ fine, const double
                 while (hits) {
                     const int src = __ffsll((long long)hits) - 1;
                     hits &= hits - 1;
-                    const int rk0 = __shfl(k0, src, 64), rk1 = __shfl(k1, src, 64);
-                    const double rc = __shfl(cth, src, 64), rs = __shfl(sth, src, 64);
+                    // src is uniform: scalar broadcasts of the ray
+                    const int rk0 = __builtin_amdgcn_readlane(k0, src);
+                    const int rk1 = __builtin_amdgcn_readlane(k1, src);
+                    const double rc = rl_f64(cth, src), rs = rl_f64(sth, src);
                     const int ang = a0 + src;
-                    for (int k = rk0; k <= rk1; ++k) {
-                        const double r = ((double)k + 0.5) * g.dr - M_PI;
-                        const double gx = 0.5 * g.nf + g.sc * r * rc - ox * g.nf;
-                        const double gy = 0.5 * g.nf + g.sc * r * rs - oy * g.nf;
-                        const int lx0 = (int)ceil(gx - hw), ly0 = (int)ceil(gy - hw);
-                        const int cx = lx0 + dx - x0, cy = ly0 + dy - y0;
-                        if (cx >= 0 && cx < NU_T && cy >= 0 && cy < NU_T) {
-                            const float zx = (float)((double)(lx0 + dx) - gx) * inv_hw;
-                            const float zy = (float)((double)(ly0 + dy) - gy) * inv_hw;
-                            const float wgt = es_kernel(zx, g.beta) * es_kernel(zy, g.beta);
-                            const float2 cv = Cs[(int64_t)ang * g.nrad + k];
-                            float2 &a = my[cy * NU_T + cx];
-                            a.x += wgt * cv.x;
-                            a.y += wgt * cv.y;
+                    // 64 samples of the ray at a time: each lane places one
+                    // (footprint corner, offsets, value), then the wave
+                    // spreads them one after the other in ascending k
+                    for (int kb = rk0; kb <= rk1; kb += 64) {
+                        const int k = kb + lane;
+                        int lx0 = 0, ly0 = 0;
+                        float fx = 0.f, fy = 0.f;
+                        float2 cv = make_float2(0.f, 0.f);
+                        bool ok = k <= rk1;
+                        if (ok) {
+                            const double r = ((double)k + 0.5) * g.dr - M_PI;
+                            const double gx = 0.5 * g.nf + g.sc * r * rc - ox * g.nf;
+                            const double gy = 0.5 * g.nf + g.sc * r * rs - oy * g.nf;
+                            lx0 = (int)ceil(gx - hw);
+                            ly0 = (int)ceil(gy - hw);
+                            fx = (float)((double)lx0 - gx);   // footprint start - point, in (-hw - 1, -hw]
+                            fy = (float)((double)ly0 - gy);
+                            ok = lx0 + NU_W - 1 >= x0 && lx0 < x0 + NU_T && ly0 + NU_W - 1 >= y0 &&
+                                 ly0 < y0 + NU_T;
+                            if (ok) cv = Cs[(int64_t)ang * g.nrad + k];
+                        }
+                        unsigned long long pts = __ballot(ok);
+                        while (pts) {
+                            const int s2 = __ffsll((long long)pts) - 1;
+                            pts &= pts - 1;
+                            const int px0 = __builtin_amdgcn_readlane(lx0, s2);
+                            const int py0 = __builtin_amdgcn_readlane(ly0, s2);
+                            const int cx = px0 + dx - x0, cy = py0 + dy - y0;
+                            if (cx >= 0 && cx < NU_T && cy >= 0 && cy < NU_T) {
+                                const float zx = ((float)dx + rl_f32(fx, s2)) * inv_hw;
+                                const float zy = ((float)dy + rl_f32(fy, s2)) * inv_hw;
+                                const float wgt = es_kernel(zx, g.beta) * es_kernel(zy, g.beta);
+                                const float vr = rl_f32(cv.x, s2), vi = rl_f32(cv.y, s2);
+                                float2 &acc1 = my[cy * NU_T + cx];
+                                acc1.x += wgt * vr;
+                                acc1.y += wgt * vi;
+                            }
                         }
                     }
                 }

@@ -11,7 +11,9 @@
 // start on the next idle lane, so their work overlaps on the device with the
 // batches already running.  Each problem keeps its own CG / Bregman scalars,
 // so its result does not depend on the batch it ran in.
+#include <algorithm>
 #include <chrono>
+#include <functional>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -231,6 +233,15 @@ void Server::run_batch(Lane &lane, std::vector<ServerRequest *> &batch) {
     std::lock_guard<std::mutex> lk(c.mu);
     c.set_current();
     const double t0 = now_s();
+    // requests on the same geometry next to each other (stable: arrival order
+    // within a geometry), so their sinograms and solution slots are contiguous
+    // and one back-projection serves them all; a problem's result does not
+    // depend on its place in the batch
+    std::stable_sort(batch.begin(), batch.end(), [](const ServerRequest *a, const ServerRequest *b) {
+        const void *ga = a->g3 ? (const void *)a->g3 : (const void *)a->g2;
+        const void *gb = b->g3 ? (const void *)b->g3 : (const void *)b->g2;
+        return std::less<const void *>()(ga, gb);
+    });
     std::vector<ProbSpec> ps;
     for (ServerRequest *r : batch)
         ps.push_back(r->g3 ? ProbSpec{nullptr, (int)r->g3->n1, r->g3} : ProbSpec{r->g2, r->ns});
@@ -253,20 +264,49 @@ void Server::run_batch(Lane &lane, std::vector<ServerRequest *> &batch) {
     dst_in.alloc(sizeof(float) * std::max<size_t>(in_total, 1));
     float *dst = dst_in.as<float>();
     size_t off = 0;
+    // a run of 2-D sinograms on one geometry, contiguous in staging and in y
+    const Geom2D *run_g = nullptr;
+    float *run_s = nullptr, *run_y = nullptr;
+    int run_ns = 0;
+    size_t run_end = 0;          // staging offset just past the run
+    int64_t run_yend = 0;        // y offset just past the run
+    bool run_tight = false;      // the run's last sinogram has no staging padding
+    auto flush = [&]() {
+        if (run_ns) backproj2d_dev(c, *const_cast<Geom2D *>(run_g), run_s, run_y, run_ns);
+        run_g = nullptr;
+        run_ns = 0;
+    };
     for (size_t i = 0; i < batch.size(); ++i) {
         ServerRequest *r = batch[i];
         const std::vector<float> &src = r->has_sino ? r->sino : r->rhs;
         std::memcpy(hst + off, src.data(), sizeof(float) * src.size());
         float *s = dst + off;
         copy_in(c, s, hst + off, sizeof(float) * src.size(), false);
+        const size_t soff = off;
         off += pad(src.size());
         float *y = b.Y() + b.offset((int)i);
-        if (r->has_sino) {
-            if (r->g3) backproj3d_dev(c, *r->g3, s, y, nullptr);
-            else backproj2d_dev(c, *r->g2, s, y, r->ns);
+        if (r->has_sino && r->g2) {
+            // one back-projection's fine-grid scratch stays near 1 GiB
+            // (about 32 N^2 bytes per slice)
+            const int64_t nn = r->g2->N * r->g2->N;
+            const int run_cap = (int)std::max<int64_t>(1, ((int64_t)1 << 30) / (32 * nn));
+            const bool extends = run_ns && run_g == r->g2 && run_end == soff && run_tight &&
+                                 run_yend == b.offset((int)i) && run_ns + r->ns <= run_cap;
+            if (!extends) {
+                flush();
+                run_g = r->g2;
+                run_s = s;
+                run_y = y;
+            }
+            run_ns += r->ns;
+            run_end = off;
+            run_tight = pad(src.size()) == src.size();
+            run_yend = b.offset((int)i) + b.size((int)i);
         } else {
-            LC_HIP(hipMemcpyAsync(y, s, sizeof(float) * src.size(), hipMemcpyDeviceToDevice,
-                                  c.stream()));
+            flush();
+            if (r->has_sino) backproj3d_dev(c, *r->g3, s, y, nullptr);
+            else LC_HIP(hipMemcpyAsync(y, s, sizeof(float) * src.size(), hipMemcpyDeviceToDevice,
+                                       c.stream()));
         }
         float *x = b.X() + b.offset((int)i);
         if (!r->x0.empty()) {
@@ -278,7 +318,10 @@ void Server::run_batch(Lane &lane, std::vector<ServerRequest *> &batch) {
         } else {
             LC_HIP(hipMemsetAsync(x, 0, sizeof(float) * (size_t)b.size((int)i), c.stream()));
         }
+        // an initial guess staged after the sinogram breaks staging contiguity
+        if (!r->x0.empty()) flush();
     }
+    flush();
     c.sync();
     const double t2 = now_s();
     const ServerRequest &h = *batch.front();
