@@ -409,7 +409,7 @@ def main():
     ap.add_argument("--serve-3d-frac", type=float, default=0.05)
     ap.add_argument("--serve-2d-rate", type=float, default=600.0,
                     help="requests/s of the 2-D-only served stream (0: skip it)")
-    ap.add_argument("--serve-lanes", type=int, default=3)
+    ap.add_argument("--serve-lanes", type=int, default=4)
     ap.add_argument("--serve-budget-gb", type=float, default=96.0)
     ap.add_argument("--sb-outer", type=int, default=20)
     ap.add_argument("--sb-steps", type=int, default=2,
