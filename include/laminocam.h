@@ -220,9 +220,10 @@ int lc_batch2d_reports(lc_batch2d b, lc_report *reps);
 /* y = A x over every problem of the batch (buffers 0 -> 1 unless stated) */
 int lc_batch2d_sysmat(lc_batch2d b, int32_t src, int32_t dst);
 /* Per-launch HIP-event timing of the three Toeplitz passes as one CG iteration
- * issues them (no reference counterpart; measurement only).  Writes one row of
- * 5 doubles per (image size class, pass): N, pass (0 rows+FFT, 1 columns,
- * 2 inverse rows), slices, ms per launch, algorithmic HBM bytes per launch.
+ * issues them, then of the CG vector update (no reference counterpart;
+ * measurement only).  Writes one row of 5 doubles per (image size class,
+ * pass): N, pass (0 rows+FFT, 1 columns, 2 inverse rows; 3 the update over the
+ * whole batch, N = 0), slices, ms per launch, algorithmic HBM bytes per launch.
  * Leaves x and y untouched; clobbers the other solver buffers. */
 int lc_batch2d_profile(lc_batch2d b, int32_t reps, int32_t max_rows, double *rows,
                        int32_t *nrows);

@@ -587,7 +587,7 @@ class Batch2D:
         """HIP-event duration of every Toeplitz pass launch as a CG iteration
         issues it: list of dicts (N, pass, nslices, ms, bytes).  Clobbers the
         solver state and Ax, keeps x and y."""
-        cap = 3 * 64
+        cap = 3 * 64 + 1
         rows = (C.c_double * (5 * cap))()
         n = C.c_int32()
         _lib.check(self.lib.lc_batch2d_profile(self.handle, int(reps), cap, rows, C.byref(n)))

@@ -747,6 +747,33 @@ std::vector<Batch2D::PassTiming> Batch2D::profile_passes(int reps) {
                 acc[3 * i + q] += ms;
             }
     }
+    // the vector update of a CG iteration (x += alpha p, r -= alpha Ap), with a
+    // scratch buffer standing in for x so the solution is left untouched
+    double upd_ms = 0.0;
+    {
+        DBuf xs(esize() * (size_t)total_);
+        hipEvent_t u0, u1;
+        LC_HIP(hipEventCreate(&u0));
+        LC_HIP(hipEventCreate(&u1));
+        for (int r = 0; r <= reps; ++r) {
+            LC_HIP(hipEventRecord(u0, c_.stream()));
+            LC_ETYPE([&](auto z) {
+                using R = decltype(z);
+                hipLaunchKernelGGL((k_cg_update<R, 256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
+                                   dchunk_.as<ChunkRef>(), st, xs.as<R>(), R_.as<R>(), P_.as<R>(),
+                                   AP_.as<R>(), cpartial_.as<double>());
+            });
+            LC_HIP(hipGetLastError());
+            LC_HIP(hipEventRecord(u1, c_.stream()));
+            LC_HIP(hipEventSynchronize(u1));
+            float ms = 0.0f;
+            LC_HIP(hipEventElapsedTime(&ms, u0, u1));
+            if (r > 0) upd_ms += ms;
+        }
+        (void)hipEventDestroy(u0);
+        (void)hipEventDestroy(u1);
+        c_.sync();
+    }
     for (auto &e : ev) (void)hipEventDestroy(e);
     std::vector<PassTiming> out;
     // element bytes e (4 or 8): real vectors e, complex 2e
@@ -761,6 +788,10 @@ std::vector<Batch2D::PassTiming> Batch2D::profile_passes(int reps) {
         for (int q = 0; q < 3; ++q)
             out.push_back(PassTiming{k.N, q, k.nslices, acc[3 * i + q] / reps, bytes[q]});
     }
+    // pass 3: the update over every problem (reads p, Ap, x, r; writes x, r)
+    int nsl = 0;
+    for (auto &k : classes_) nsl += k->nslices;
+    out.push_back(PassTiming{0, 3, nsl, upd_ms / reps, 6.0 * e * (double)total_});
     return out;
 }
 

@@ -289,7 +289,8 @@ class Batch2D {
     float time_apply(int reps);
     // HIP-event time of every launch of the three Toeplitz passes in the form a
     // CG iteration issues them (P1 fused with p = r + beta p, P3 with the
-    // <p, Ap> partials), averaged over reps.  Overwrites P, AP and the solver
+    // <p, Ap> partials), averaged over reps, then of the CG vector update over
+    // the whole batch (pass 3, N = 0).  Overwrites P, AP, R and the solver
     // state, not X or Y.
     struct PassTiming {
         int N, pass, nslices;
