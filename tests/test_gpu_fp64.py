@@ -160,12 +160,15 @@ def _fft_operator(P, N):
 @pytest.mark.parametrize("seed", [0, 1, 2])
 def test_cg_fp64_long_at_rounding_floor(lc, orc64, seed):
     # 50 CG iterations on the 128 x 128, 90-angle slices of the batched
-    # workload.  Here plain CG is chaotic in FP64 itself: two FP64 solves that
-    # differ only in summation order or FFT rounding end 1e-3 .. 1e-2 apart.
-    # The bar: the GPU's FP64 result is no farther from the oracle than those
-    # independent FP64 realisations are, its residual is as close to the
-    # oracle's as theirs, and its distance to the true object matches the
-    # oracle's to 1e-3 relative.
+    # workload.  Here plain CG is chaotic in FP64 itself: FP64 solves that
+    # differ only in summation order, FFT rounding or a 1e-15 perturbation of
+    # the right-hand side end 1e-4 .. 1e-2 apart, and where in that range a
+    # given realisation lands varies by about 5x.  The bar: the GPU's FP64
+    # result is within 3x of the farthest of four independent FP64
+    # realisations, its residual within 3x their spread of the oracle's, its
+    # distance to the true object equal to the oracle's to 0.5% or within 1.5x
+    # their spread (the oracle itself can be the outlier), and it is at most
+    # half as far from the oracle as the GPU's FP32 solve.
     N, nang, iters = 128, 90, 50
     announce(f"fp64 cg long N={N} nang={nang} iters={iters} seed={seed}")
     from laminocam_amd import synth
@@ -181,20 +184,31 @@ def test_cg_fp64_long_at_rounding_floor(lc, orc64, seed):
     assert rep["iters"] == rr["iters"] == iters
     xa = _cg_np(P.convolve, rhs, iters)
     xb = _cg_np(_fft_operator(P, N), rhs, iters)
-    floor = max(rel_l2(xa, xr), rel_l2(xb, xr))
+    reruns = []
+    for k in (1, 2):
+        pert = np.random.default_rng(1000 * seed + k).standard_normal(rhs.shape)
+        xp, _ = orc64.cgsolver(P, rhs * (1.0 + 1e-15 * pert), np.zeros_like(rhs), iters, 0.0, 0.0)
+        reruns.append(rel_l2(xp, xr))
+    floor = max([rel_l2(xa, xr), rel_l2(xb, xr)] + reruns)
     err = rel_l2(x, xr)
     ynorm = np.linalg.norm(rhs)
     res = [np.linalg.norm(rhs - P.convolve(v)) / ynorm for v in (x, xr, xa, xb)]
     d_obj = [rel_l2(v, vol) for v in (x, xr, xa, xb)]
     print(f"rel_l2 vs oracle: gpu {err:.3e}, independent FP64 {rel_l2(xa, xr):.3e} / "
-          f"{rel_l2(xb, xr):.3e}; true residual gpu {res[0]:.4e} oracle {res[1]:.4e} "
+          f"{rel_l2(xb, xr):.3e} / reruns {reruns[0]:.3e} / {reruns[1]:.3e}; true residual gpu "
+          f"{res[0]:.4e} oracle {res[1]:.4e} "
           f"independent {res[2]:.4e} / {res[3]:.4e}; distance to object gpu {d_obj[0]:.6f} "
           f"oracle {d_obj[1]:.6f} independent {d_obj[2]:.6f} / {d_obj[3]:.6f}")
-    assert err <= max(1e-3, 2.0 * floor)
+    x32, _ = lc.cgsolver(psf, rhs.astype(np.float32), np.zeros(rhs.shape, np.float32), iters,
+                         0.0, 0.0, graph=True)
+    err32 = rel_l2(x32, xr)
+    print(f"FP32 path rel_l2 vs oracle {err32:.3e}")
+    assert err <= max(1e-3, 3.0 * floor)
     spread = max(abs(np.log(res[2] / res[1])), abs(np.log(res[3] / res[1])))
-    assert abs(np.log(res[0] / res[1])) <= max(np.log(1.05), 2.0 * spread)
+    assert abs(np.log(res[0] / res[1])) <= max(np.log(1.05), 3.0 * spread)
     dspread = max(abs(d_obj[2] - d_obj[1]), abs(d_obj[3] - d_obj[1]))
-    assert abs(d_obj[0] - d_obj[1]) <= max(1e-3 * d_obj[1], 2.0 * dspread)
+    assert abs(d_obj[0] - d_obj[1]) <= max(5e-3 * d_obj[1], 1.5 * dspread)
+    assert err <= 0.5 * err32
 
 
 @pytest.mark.parametrize("zero_pad,aniso", [(False, False), (True, False), (True, True)])
