@@ -258,9 +258,16 @@ template <int S> __device__ __forceinline__ int swz_step(int sx0, int x0, int r)
 
 // LSH >= 0: the column layout's log2(lines) is a compile-time constant (all
 // strides fold); LSH < 0: it is the runtime lsh.
-template <typename T, int L, int R, int NS, int SIGN, int NT, int MAXE, bool LFAST, int LSH>
+// Pruning for zero-padded linear convolution (every line holds at most
+// (L + 1) / 2 nonzero leading elements, and only that many leading outputs are
+// used): PIN (first stage) treats inputs at index >= (L + 1) / 2 as zero
+// without reading them; POUT (last stage) skips outputs at index >= (L + 1) / 2.
+template <typename T, int L, int R, int NS, int SIGN, int NT, int MAXE, bool LFAST, int LSH,
+          bool PIN = false, bool POUT = false>
 __device__ __forceinline__ void fft_stage_ct(cpx<T> *buf, int nl, int lsh,
                                              const cpx<T> *__restrict__ tw) {
+    // element j + r LR with j >= 0 is at least r LR: past the half when r LR >= (L + 1) / 2
+    constexpr int HALF = (L + 1) / 2;
     using C = cpx<T>;
     constexpr int MAXB = (MAXE + R - 1) / R;
     constexpr int LR = L / R;
@@ -293,16 +300,19 @@ __device__ __forceinline__ void fft_stage_ct(cpx<T> *buf, int nl, int lsh,
                     constexpr int S = LR << (LSH >= 0 ? LSH : 0);
                     const int sb = swz(b);
 #pragma unroll
-                    for (int r = 0; r < R; ++r) v[t][r] = buf[swz_step<S>(sb, b, r)];
+                    for (int r = 0; r < R; ++r)
+                        v[t][r] = (PIN && r * LR >= HALF) ? cmake<C>(0, 0) : buf[swz_step<S>(sb, b, r)];
                 } else {
 #pragma unroll
-                    for (int r = 0; r < R; ++r) v[t][r] = buf[swz(b + r * (LR << ls))];
+                    for (int r = 0; r < R; ++r)
+                        v[t][r] = (PIN && r * LR >= HALF) ? cmake<C>(0, 0) : buf[swz(b + r * (LR << ls))];
                 }
             } else {
                 const int line = b / LR, j = b - line * LR;
                 const int x0 = line * L + j, sx0 = swz(x0);
 #pragma unroll
-                for (int r = 0; r < R; ++r) v[t][r] = buf[swz_step<LR>(sx0, x0, r)];
+                for (int r = 0; r < R; ++r)
+                    v[t][r] = (PIN && r * LR >= HALF) ? cmake<C>(0, 0) : buf[swz_step<LR>(sx0, x0, r)];
             }
         }
     }
@@ -335,41 +345,50 @@ __device__ __forceinline__ void fft_stage_ct(cpx<T> *buf, int nl, int lsh,
             const int o = (j - k) * R + k;
             if (LFAST) {
                 const int ob = (o << ls) + line;
+                // last stage (NS = L / R): output o + r NS = j + r LR, at least r LR
                 if constexpr (LSH >= 0) {
                     constexpr int S = NS << (LSH >= 0 ? LSH : 0);
                     const int sb = swz(ob);
 #pragma unroll
-                    for (int r = 0; r < R; ++r) buf[swz_step<S>(sb, ob, r)] = v[t][r];
+                    for (int r = 0; r < R; ++r)
+                        if (!(POUT && r * LR >= HALF)) buf[swz_step<S>(sb, ob, r)] = v[t][r];
                 } else {
 #pragma unroll
-                    for (int r = 0; r < R; ++r) buf[swz(ob + r * (NS << ls))] = v[t][r];
+                    for (int r = 0; r < R; ++r)
+                        if (!(POUT && r * LR >= HALF)) buf[swz(ob + r * (NS << ls))] = v[t][r];
                 }
             } else {
                 const int ob = line * L + o, sb = swz(ob);
 #pragma unroll
-                for (int r = 0; r < R; ++r) buf[swz_step<NS>(sb, ob, r)] = v[t][r];
+                for (int r = 0; r < R; ++r)
+                    if (!(POUT && r * LR >= HALF)) buf[swz_step<NS>(sb, ob, r)] = v[t][r];
             }
         }
     }
     __syncthreads();
 }
 
-template <typename T, int L, int S, int NS, int SIGN, int NT, int MAXE, bool LFAST, int LSH>
+template <typename T, int L, int S, int NS, int SIGN, int NT, int MAXE, bool LFAST, int LSH,
+          bool PIN, bool POUT>
 __device__ __forceinline__ void fft_stages_ct(cpx<T> *buf, int nl, int lsh,
                                               const cpx<T> *__restrict__ tw) {
     constexpr int R = ct_radix(L, S);
-    fft_stage_ct<T, L, R, NS, SIGN, NT, MAXE, LFAST, LSH>(buf, nl, lsh, tw);
-    if constexpr (S + 1 < ct_nst(L))
-        fft_stages_ct<T, L, S + 1, NS * R, SIGN, NT, MAXE, LFAST, LSH>(buf, nl, lsh, tw);
+    constexpr bool LAST = S + 1 == ct_nst(L);
+    fft_stage_ct<T, L, R, NS, SIGN, NT, MAXE, LFAST, LSH, PIN && S == 0, POUT && LAST>(buf, nl, lsh,
+                                                                                       tw);
+    if constexpr (!LAST)
+        fft_stages_ct<T, L, S + 1, NS * R, SIGN, NT, MAXE, LFAST, LSH, PIN, POUT>(buf, nl, lsh, tw);
 }
 
 // Full transform of nl lines of compile-time length L (row layout: line
 // stride L; column layout: nl = 1 << lsh lines interleaved, lsh = LSH when
-// LSH >= 0).
-template <typename T, int L, int SIGN, int NT, int MAXE, bool LFAST, int LSH = -1>
+// LSH >= 0).  PIN / POUT: the zero-padded-input / leading-output pruning of
+// fft_stage_ct.
+template <typename T, int L, int SIGN, int NT, int MAXE, bool LFAST, int LSH = -1,
+          bool PIN = false, bool POUT = false>
 __device__ __forceinline__ void fft_lds_ct(cpx<T> *buf, int nl, int lsh,
                                            const cpx<T> *__restrict__ tw) {
-    fft_stages_ct<T, L, 0, 1, SIGN, NT, MAXE, LFAST, LSH>(buf, nl, lsh, tw);
+    fft_stages_ct<T, L, 0, 1, SIGN, NT, MAXE, LFAST, LSH, PIN, POUT>(buf, nl, lsh, tw);
 }
 
 // FP64 twiddle generation: tw[k] = exp(-2 pi i k / L)
