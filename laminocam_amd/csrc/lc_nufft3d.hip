@@ -30,9 +30,13 @@ extern __shared__ __align__(16) char lc_smem[];
 namespace {
 
 constexpr int W3 = 8;                       // kernel width (cells)
-constexpr int TZ = 8, TY = 16, TX = 16;     // spread tile (cells)
-static_assert(TX == 16, "tile_idx swizzles x by XOR with 8 within a row of 16");
-constexpr int S3_WAVES = 4;
+// spread tile (cells) and waves per tile: each wave keeps a private LDS copy
+// of the tile (8 KiB), so the block takes 16 KiB and a CU holds ten; measured
+// on the 32 x 511^2 / 720 x 128 x 511 back-projection against 8x16x16 tiles
+// with 4 or 2 waves, 8x8x8 and 16x16x8 tiles, this shape is the fastest
+constexpr int TZ = 8, TY = 16, TX = 8;
+static_assert(TX == 16 || TX == 8, "tile rows of 16 (swizzled) or 8 cells");
+constexpr int S3_WAVES = 2;
 constexpr int LN_NT = 256, LN_ME = 32;      // line passes: NT * ME complex in LDS
 
 __device__ __forceinline__ float es3(float z, float beta) {
@@ -54,8 +58,11 @@ __device__ __forceinline__ double rl_f64(double v, int lane) {
 // spread tile cell (z, y, x) -> LDS index.  x is XOR-ed with bit 1 of y (times
 // 8): the 4 rows x 8 columns a half-wave updates per z step then fall on
 // distinct bank pairs (plain rows y and y + 2 share banks).
+// (rows of 8 cells need no swizzle: 4 rows x 8 columns already cover 32
+// distinct bank pairs)
+__device__ __forceinline__ int tile_swx(int y) { return TX == 16 ? (((y >> 1) & 1) << 3) : 0; }
 __device__ __forceinline__ int tile_idx(int z, int y, int x) {
-    return (z * TY + y) * TX + (x ^ (((y >> 1) & 1) << 3));
+    return (z * TY + y) * TX + (x ^ tile_swx(y));
 }
 
 __device__ __forceinline__ int wrapi(int a, int n) {
@@ -305,7 +312,7 @@ k_spread3d(const float2 *__restrict__ C, float2 *__restrict__ F, const double *_
             s.y += acc[w * TC + i].y;
         }
         // physical index i holds cell (cz, cy, cx) with cx un-swizzled
-        const int cz = i / (TY * TX), cy = (i / TX) % TY, cx = (i % TX) ^ (((cy >> 1) & 1) << 3);
+        const int cz = i / (TY * TX), cy = (i / TX) % TY, cx = (i % TX) ^ tile_swx(cy);
         const int zz = c0[0] + cz, yy = c0[1] + cy, xx = c0[2] + cx;
         if (zz < g.nf[0] && yy < g.nf[1] && xx < g.nf[2])
             F[((int64_t)zz * g.nf[1] + yy) * g.nf[2] + xx] = s;
