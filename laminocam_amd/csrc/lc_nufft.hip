@@ -28,8 +28,18 @@ namespace lc {
 extern __shared__ __align__(16) char lc_smem[];
 
 constexpr int NU_W = 8;        // kernel width (cells); 64 lanes cover the footprint
-constexpr int NU_T = 32;       // spread tile edge (cells)
+// spread tile edge (cells): 16 x 16 tiles (2 KiB per wave copy) keep more
+// tiles in flight than 32 x 32; faster for single slices and on the served
+// stream, where back-projections stack a few slices per geometry
+constexpr int NU_T = 16;
 constexpr int NU_WAVES = 4;    // waves per spread workgroup
+static_assert(NU_T == 16 || NU_T == 32, "spread tile rows of 16 (swizzled) or 32 cells");
+
+// spread tile cell (y, x) -> LDS index.  For rows of 16 cells x is XOR-ed with
+// bit 1 of y (times 8), so the 4 rows x 8 columns a half-wave updates fall on
+// distinct bank pairs (plain rows y and y + 2 would share banks).
+__device__ __forceinline__ int nu_swx(int y) { return NU_T == 16 ? (((y >> 1) & 1) << 3) : 0; }
+__device__ __forceinline__ int nu_tidx(int y, int x) { return y * NU_T + (x ^ nu_swx(y)); }
 
 // value of v in lane `lane` (uniform lane index: a scalar broadcast, no LDS)
 __device__ __forceinline__ float rl_f32(float v, int lane) {
@@ -242,7 +252,7 @@ k_spread2d(const float2 *__restrict__ C, float2 *__restrict__ fine, const double
                                 const float zy = ((float)dy + rl_f32(fy, s2)) * inv_hw;
                                 const float wgt = es_kernel(zx, g.beta) * es_kernel(zy, g.beta);
                                 const float vr = rl_f32(cv.x, s2), vi = rl_f32(cv.y, s2);
-                                float2 &acc1 = my[cy * NU_T + cx];
+                                float2 &acc1 = my[nu_tidx(cy, cx)];
                                 acc1.x += wgt * vr;
                                 acc1.y += wgt * vi;
                             }
@@ -261,7 +271,7 @@ k_spread2d(const float2 *__restrict__ C, float2 *__restrict__ fine, const double
             s.x += acc[w * NU_T * NU_T + i].x;
             s.y += acc[w * NU_T * NU_T + i].y;
         }
-        const int cy = i / NU_T, cx = i - cy * NU_T;
+        const int cy = i / NU_T, cx = (i - cy * NU_T) ^ nu_swx(cy);
         const int gxx = x0 + cx, gyy = y0 + cy;
         if (gxx < g.nf && gyy < g.nf) fs[(int64_t)gyy * g.nf + gxx] = s;
     }
