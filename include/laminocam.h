@@ -267,6 +267,78 @@ int lc_server_stats(lc_server s, double *stats);
 /* finishes every queued request, then stops the lanes */
 int lc_server_destroy(lc_server s);
 
+/* ---- whole-solve drivers ----
+ * ReconParams (include/recon_params.h:45-54): field for field, same types and
+ * defaults (lc_recon_params_default).  The regularizer values follow
+ * enum class Regularizer { SPLIT_BREGMAN, UNCONSTRAINED }. */
+#define LC_REG_SPLIT_BREGMAN 0
+#define LC_REG_UNCONSTRAINED 1
+typedef struct {
+    int32_t regularizer;    /* LC_REG_UNCONSTRAINED (CG) or LC_REG_SPLIT_BREGMAN */
+    int32_t pad;
+    int64_t recon_dims[3];  /* {n1, n2, n3}, n2 == n3 */
+    int64_t max_iters;      /* 100: CG iterations, or Split-Bregman outer iterations */
+    int64_t inner_iters;    /* 1: CG iterations per Split-Bregman outer iteration */
+    float lambda;           /* 0.1 */
+    float mu;               /* 10 */
+    float tol;              /* 1e-5 */
+    float xtol;             /* 1e-5 */
+    float pad_factor;       /* 1.4142 (3-D driver only) */
+    float pad2;
+} lc_recon_params;
+void lc_recon_params_default(lc_recon_params *p);
+
+/* prepare_sinogram (include/mbir2d_prep.h:44-65): the ndatasets projection
+ * stacks projs[d] {nang[d], nrows, ncols} are stacked, divided by their global
+ * maximum (when it is > 0) and written as the sinogram {nrows, sum nang, ncols}
+ * (one slice per detector row).  The maximum is returned in *max_out (may be
+ * NULL).  The angles concatenate on the host.  LC_DEVICE_PTRS: projs[d], sino
+ * and max_out are device pointers. */
+int lc_prepare_sinogram(lc_ctx ctx, int32_t ndatasets, const float *const *projs,
+                        const int64_t *nang, int64_t nrows, int64_t ncols, float *sino,
+                        float *max_out, uint32_t flags);
+int lc_prepare_sinogram_f64(lc_ctx ctx, int32_t ndatasets, const double *const *projs,
+                            const int64_t *nang, int64_t nrows, int64_t ncols, double *sino,
+                            double *max_out, uint32_t flags);
+
+/* The 2-D slice-stack solve of cpu_mbir2d (src/mbir2d.cpp:66-100) and
+ * gpu::MBIR2D (src/gpu/mbir2d.cu:36-85): PolarGrid2D(theta, ncols),
+ * PointSpreadFunction2D(grid, recon_dims[2]), rhs = backproj2d of the first
+ * recon_dims[0] slices of the prepared sinogram {nslices, nang, ncols}, then
+ * CG or Split-Bregman (max_iters outer x inner_iters) from x0 = 0 on the whole
+ * stack.  recon receives {recon_dims}.  recon_dims[0] <= nslices and
+ * recon_dims[1] == recon_dims[2] (LC_EINVAL otherwise).  flags: LC_GRAPH,
+ * LC_TV_ZERO_PAD, LC_TV_ANISO, LC_DEVICE_PTRS.  The _f64 form solves in FP64;
+ * its back-projection runs in FP32, as every projector here. */
+int lc_mbir2d(lc_ctx ctx, const float *sino, int64_t nslices, int64_t nang, int64_t ncols,
+              const double *theta, const lc_recon_params *params, float *recon, lc_report *rep,
+              uint32_t flags);
+int lc_mbir2d_f64(lc_ctx ctx, const double *sino, int64_t nslices, int64_t nang, int64_t ncols,
+                  const double *theta, const lc_recon_params *params, double *recon,
+                  lc_report *rep, uint32_t flags);
+
+/* MBIR (src/mbir.cpp:34-161), the 3-D laminography solve over ndatasets
+ * datasets (Dataset_t, include/recon_params.h:32-39): projs[d] {nang[d],
+ * nrows, ncols} with one gamma[d] and beta[d] per dataset; theta and shifts
+ * ({dx, dy} per projection, may be NULL for none) concatenate over the
+ * datasets.  Steps: stack, divide by the global maximum (when > 0), pad every
+ * projection symmetrically with zeros to size_t(n * pad_factor), made odd by
+ * -1 when even (skipped when pad_factor - 1 < 1e-6), grid and PSF on the
+ * padded detector for the padded volume (each recon dim padded the same way),
+ * rhs = backproj(..., shifts), CG or Split-Bregman from x0 = 0, symmetric crop
+ * to recon_dims into recon.  padded_dims (may be NULL) receives {volume n1,
+ * n2, n3, detector rows, cols}.  A pad factor that makes the padded volume
+ * smaller than recon_dims is LC_EINVAL.  flags as lc_mbir2d. */
+int lc_mbir3d(lc_ctx ctx, int32_t ndatasets, const float *const *projs, const int64_t *nang,
+              int64_t nrows, int64_t ncols, const double *theta, const double *gamma,
+              const double *beta, const double *shifts, const lc_recon_params *params,
+              float *recon, int64_t *padded_dims, lc_report *rep, uint32_t flags);
+int lc_mbir3d_f64(lc_ctx ctx, int32_t ndatasets, const double *const *projs,
+                  const int64_t *nang, int64_t nrows, int64_t ncols, const double *theta,
+                  const double *gamma, const double *beta, const double *shifts,
+                  const lc_recon_params *params, double *recon, int64_t *padded_dims,
+                  lc_report *rep, uint32_t flags);
+
 /* ---- Toeplitz kernel shapes per image size (no reference counterpart) ----
  * variant[0] = row-pass shape, variant[1] = column-pass shape,
  * variant[2] = columns per column-pass block.  lc_autotune2d times the

@@ -11,6 +11,9 @@
  *                                    forward2d, backproj2d, sysmat2d (2)
  *   include/optimize.h:31-91         opt::Function, opt::cgsolver, opt::split_bregman
  *   include/bregman.h:34-55          opt::grad_u, opt::divergence, opt::laplacian
+ *   include/mbir2d_prep.h:30-65      Sinogram2D, prepare_sinogram
+ *   include/tomocam.h:31-33          MBIR (src/mbir.cpp:34-161)
+ *   include/gpu/tomocam.h:60-75      gpu::MBIR, gpu::MBIR2D (src/gpu/mbir2d.cu:36-85)
  * Errors: a nonzero status becomes std::invalid_argument (bad arguments,
  * unsupported shapes) or std::runtime_error (device failures), as the
  * reference throws (src/projection.cpp:105-108, include/finufft_plan.h:106).
@@ -19,7 +22,9 @@
  * Array(dims_t) constructor of tomocam::Array<T> (include/array.h:61-110), so
  * they compile against the reference's array.h when it is included first.
  * Defining LAMINOCAM_STANDALONE_ARRAY instead provides a minimal
- * tomocam::dims_t / tomocam::Array<T> with that same interface.
+ * tomocam::dims_t / tomocam::Array<T> with that same interface, and the
+ * driver types Dataset_t, Regularizer and ReconParams of
+ * include/recon_params.h:32-54 (otherwise include that header first).
  *
  * Precision: Array<float> runs the FP32 path (FP64 reductions, PSF and
  * twiddles).  Array<double> runs the FP64 path of the Toeplitz operator,
@@ -555,7 +560,169 @@ template <typename T> Array<T> laplacian(const Array<T> &u) {
                                    (int64_t)d.n3, out.data()));
     return out;
 }
-
 } // namespace tomocam::opt
+
+// ---------------------------------------------------------------------------
+// whole-solve drivers
+// ---------------------------------------------------------------------------
+#ifdef LAMINOCAM_STANDALONE_ARRAY
+namespace tomocam {
+// include/recon_params.h:32-54: the fields the drivers read
+template <typename T> struct Dataset_t {
+    Array<T> projs;                         // {n_angles, n_rows, n_cols}
+    std::vector<T> angles;
+    T gamma;
+    T beta = T(0);
+    std::vector<std::array<T, 2>> shifts;   // per-projection [dx, dy] in pixels
+};
+enum class Regularizer { SPLIT_BREGMAN, UNCONSTRAINED };
+struct ReconParams {
+    Regularizer regularizer = Regularizer::UNCONSTRAINED;
+    std::array<size_t, 3> recon_dims = {0, 0, 0};
+    size_t maxIters = 100;
+    size_t innerIters = 1;
+    float lambda = 0.1f;
+    float mu = 10.0f;
+    float tol = 1e-5f;
+    float xtol = 1e-5f;
+    float PAD_FACTOR = 1.4142f;
+};
+} // namespace tomocam
+#endif
+
+namespace laminocam {
+
+inline lc_recon_params recon_params(const tomocam::ReconParams &p) {
+    lc_recon_params c;
+    lc_recon_params_default(&c);
+    c.regularizer = p.regularizer == tomocam::Regularizer::SPLIT_BREGMAN ? LC_REG_SPLIT_BREGMAN
+                                                                         : LC_REG_UNCONSTRAINED;
+    for (int i = 0; i < 3; ++i) c.recon_dims[i] = (int64_t)p.recon_dims[i];
+    c.max_iters = (int64_t)p.maxIters;
+    c.inner_iters = (int64_t)p.innerIters;
+    c.lambda = p.lambda;
+    c.mu = p.mu;
+    c.tol = p.tol;
+    c.xtol = p.xtol;
+    c.pad_factor = p.PAD_FACTOR;
+    return c;
+}
+
+inline int prepare_sinogram(lc_ctx c, int32_t nd, const float *const *p, const int64_t *na, int64_t nr,
+                            int64_t nc, float *out) {
+    return lc_prepare_sinogram(c, nd, p, na, nr, nc, out, nullptr, 0);
+}
+inline int prepare_sinogram(lc_ctx c, int32_t nd, const double *const *p, const int64_t *na, int64_t nr,
+                            int64_t nc, double *out) {
+    return lc_prepare_sinogram_f64(c, nd, p, na, nr, nc, out, nullptr, 0);
+}
+inline int mbir2d(lc_ctx c, const float *s, int64_t ns, int64_t na, int64_t nc, const double *th,
+                  const lc_recon_params *p, float *x) {
+    return lc_mbir2d(c, s, ns, na, nc, th, p, x, nullptr, 0);
+}
+inline int mbir2d(lc_ctx c, const double *s, int64_t ns, int64_t na, int64_t nc, const double *th,
+                  const lc_recon_params *p, double *x) {
+    return lc_mbir2d_f64(c, s, ns, na, nc, th, p, x, nullptr, 0);
+}
+inline int mbir3d(lc_ctx c, int32_t nd, const float *const *pr, const int64_t *na, int64_t nr, int64_t nc,
+                  const double *th, const double *ga, const double *be, const double *sh,
+                  const lc_recon_params *p, float *x) {
+    return lc_mbir3d(c, nd, pr, na, nr, nc, th, ga, be, sh, p, x, nullptr, nullptr, 0);
+}
+inline int mbir3d(lc_ctx c, int32_t nd, const double *const *pr, const int64_t *na, int64_t nr, int64_t nc,
+                  const double *th, const double *ga, const double *be, const double *sh,
+                  const lc_recon_params *p, double *x) {
+    return lc_mbir3d_f64(c, nd, pr, na, nr, nc, th, ga, be, sh, p, x, nullptr, nullptr, 0);
+}
+
+// projection pointers and counts of the datasets; every dataset shares one detector size
+template <typename T> struct DatasetView {
+    std::vector<const T *> projs;
+    std::vector<int64_t> nang;
+    std::vector<double> theta;
+    size_t nrows = 0, ncols = 0;
+    explicit DatasetView(const std::vector<tomocam::Dataset_t<T>> &ds) {
+        if (ds.empty()) throw std::invalid_argument("no datasets");
+        nrows = ds[0].projs.nrows();
+        ncols = ds[0].projs.ncols();
+        for (const auto &d : ds) {
+            if (d.projs.nrows() != nrows || d.projs.ncols() != ncols)
+                throw std::invalid_argument("datasets differ in detector size");
+            if (d.angles.size() != d.projs.nslices())
+                throw std::invalid_argument("one angle per projection");
+            projs.push_back(d.projs.data());
+            nang.push_back((int64_t)d.projs.nslices());
+            theta.insert(theta.end(), d.angles.begin(), d.angles.end());
+        }
+    }
+};
+
+} // namespace laminocam
+
+namespace tomocam {
+
+template <typename T> struct Sinogram2D {
+    Array<T> data;          // {n_slices, n_angles, n_cols}
+    std::vector<T> theta;   // one angle per projection (radians)
+};
+
+// include/mbir2d_prep.h:44-65: stack, divide by the global maximum (when > 0),
+// {n_angles, n_rows, n_cols} -> {n_rows, n_angles, n_cols}; on the device
+template <typename T> Sinogram2D<T> prepare_sinogram(const std::vector<Dataset_t<T>> &datasets) {
+    laminocam::DatasetView<T> v(datasets);
+    Sinogram2D<T> out;
+    out.data = Array<T>(dims_t{v.nrows, v.theta.size(), v.ncols});
+    laminocam::check(laminocam::prepare_sinogram(laminocam::context(), (int32_t)v.projs.size(),
+                                                 v.projs.data(), v.nang.data(), (int64_t)v.nrows,
+                                                 (int64_t)v.ncols, out.data.data()));
+    out.theta.assign(v.theta.begin(), v.theta.end());
+    return out;
+}
+
+// src/mbir.cpp:34-161: the 3-D laminography reconstruction, cropped to
+// params.recon_dims.  As there, every projection needs a shift.
+template <typename T> Array<T> MBIR(const std::vector<Dataset_t<T>> &datasets, const ReconParams &params) {
+    laminocam::DatasetView<T> v(datasets);
+    std::vector<double> gamma, beta, sh;
+    for (const auto &d : datasets) {
+        gamma.push_back((double)d.gamma);
+        beta.push_back((double)d.beta);
+        for (const auto &s : d.shifts) {
+            sh.push_back((double)s[0]);
+            sh.push_back((double)s[1]);
+        }
+    }
+    if (sh.size() != 2 * v.theta.size())
+        throw std::invalid_argument("shifts size must match the number of slices in the projection");
+    const lc_recon_params p = laminocam::recon_params(params);
+    Array<T> out(dims_t{params.recon_dims[0], params.recon_dims[1], params.recon_dims[2]});
+    laminocam::check(laminocam::mbir3d(laminocam::context(), (int32_t)v.projs.size(), v.projs.data(),
+                                       v.nang.data(), (int64_t)v.nrows, (int64_t)v.ncols, v.theta.data(),
+                                       gamma.data(), beta.data(), sh.data(), &p, out.data()));
+    return out;
+}
+
+namespace gpu {
+// include/gpu/tomocam.h:60-75: the device drivers are the same calls
+template <typename T> Array<T> MBIR(const std::vector<Dataset_t<T>> &datasets, const ReconParams &params) {
+    return tomocam::MBIR<T>(datasets, params);
+}
+
+// src/gpu/mbir2d.cu:36-85 (and cpu_mbir2d, src/mbir2d.cpp:66-100): the
+// slice-stack reconstruction of a prepared sinogram {n_slices, n_angles, n_cols}
+template <typename T>
+Array<T> MBIR2D(const Array<T> &sinogram, const std::vector<T> &theta, const ReconParams &params) {
+    if (theta.size() != sinogram.nrows()) throw std::invalid_argument("one angle per sinogram row");
+    const std::vector<double> th(theta.begin(), theta.end());
+    const lc_recon_params p = laminocam::recon_params(params);
+    Array<T> out(dims_t{params.recon_dims[0], params.recon_dims[1], params.recon_dims[2]});
+    laminocam::check(laminocam::mbir2d(laminocam::context(), sinogram.data(), (int64_t)sinogram.nslices(),
+                                       (int64_t)sinogram.nrows(), (int64_t)sinogram.ncols(), th.data(), &p,
+                                       out.data()));
+    return out;
+}
+} // namespace gpu
+
+} // namespace tomocam
 
 #endif // LAMINOCAM_TOMOCAM_HPP

@@ -12,6 +12,8 @@ Python mirror of the reference's C++ API over the C ABI of liblaminocam.so
   cgsolver               src/conjgrad.cpp:33-76
   split_bregman          src/bregman.cpp:36-94
   grad_u / divergence / laplacian   src/finitediff.cpp:26-90, include/bregman.h:52-55
+  prepare_sinogram, MBIR2D, MBIR    include/mbir2d_prep.h:44-65, src/mbir2d.cpp:66-100,
+                                    src/mbir.cpp:34-161 (laminocam_amd.drivers)
 
 All compute runs in HIP kernels; nothing here falls back to the CPU.
 """
@@ -29,7 +31,8 @@ __all__ = ["Context", "PolarGrid2D", "PointSpreadFunction2D", "PolarGrid", "Poin
            "sysmat", "sysmat2d", "forward", "backproj", "forward2d",
            "backproj2d", "cgsolver",
            "split_bregman", "grad_u", "divergence", "laplacian", "Batch2D", "LcError",
-           "default_context", "Server"]
+           "default_context", "Server", "Dataset", "Regularizer", "ReconParams", "Sinogram2D",
+           "prepare_sinogram", "MBIR2D", "MBIR"]
 
 _default_ctx = None
 
@@ -714,3 +717,7 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+from .drivers import (MBIR, MBIR2D, Dataset, ReconParams, Regularizer,  # noqa: E402
+                      Sinogram2D, prepare_sinogram)

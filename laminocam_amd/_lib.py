@@ -18,6 +18,9 @@ LC_TV_ANISO = 0x10
 LC_GEOM_LAZY_PSF = 0x20
 LC_FP64 = 0x40
 
+LC_REG_SPLIT_BREGMAN = 0
+LC_REG_UNCONSTRAINED = 1
+
 # every symbol declared in include/laminocam.h
 EXPORTS = [
     "lc_abi_version", "lc_last_error", "lc_ctx_create", "lc_ctx_destroy", "lc_ctx_sync",
@@ -35,6 +38,8 @@ EXPORTS = [
     "lc_cg3d_f64", "lc_split_bregman3d_f64", "lc_grad_f64", "lc_divergence_f64",
     "lc_laplacian_f64", "lc_batch_create_ex", "lc_batch2d_set_f64", "lc_batch2d_get_f64",
     "lc_batch2d_device_ptr_f64", "lc_psf2d_kernel_hat_f64", "lc_psf3d_kernel_hat_f64",
+    "lc_recon_params_default", "lc_prepare_sinogram", "lc_prepare_sinogram_f64", "lc_mbir2d",
+    "lc_mbir2d_f64", "lc_mbir3d", "lc_mbir3d_f64",
 ]
 
 
@@ -54,6 +59,14 @@ class LcRequest(C.Structure):
                 ("inner", C.c_int32), ("tol", C.c_double), ("xtol", C.c_double),
                 ("lam", C.c_double), ("mu", C.c_double), ("flags", C.c_uint32),
                 ("sino", C.c_void_p), ("rhs", C.c_void_p), ("x0", C.c_void_p)]
+
+
+class LcReconParams(C.Structure):
+    """lc_recon_params of laminocam.h (ReconParams, include/recon_params.h:45-54)"""
+    _fields_ = [("regularizer", C.c_int32), ("pad", C.c_int32), ("recon_dims", C.c_int64 * 3),
+                ("max_iters", C.c_int64), ("inner_iters", C.c_int64), ("lam", C.c_float),
+                ("mu", C.c_float), ("tol", C.c_float), ("xtol", C.c_float),
+                ("pad_factor", C.c_float), ("pad2", C.c_float)]
 
 
 class LcError(RuntimeError):
@@ -158,6 +171,21 @@ def load():
         "lc_batch2d_device_ptr_f64": (C.c_int, [vp, i32, i32, C.POINTER(vp)]),
         "lc_psf2d_kernel_hat_f64": (C.c_int, [vp, vp, vp]),
         "lc_psf3d_kernel_hat_f64": (C.c_int, [vp, vp, vp]),
+        "lc_recon_params_default": (None, [C.POINTER(LcReconParams)]),
+        "lc_prepare_sinogram": (C.c_int, [vp, i32, C.POINTER(vp), C.POINTER(i64), i64, i64, vp, vp,
+                                          u32]),
+        "lc_prepare_sinogram_f64": (C.c_int, [vp, i32, C.POINTER(vp), C.POINTER(i64), i64, i64, vp,
+                                              vp, u32]),
+        "lc_mbir2d": (C.c_int, [vp, vp, i64, i64, i64, f64p, C.POINTER(LcReconParams), vp,
+                                C.POINTER(LcReport), u32]),
+        "lc_mbir2d_f64": (C.c_int, [vp, vp, i64, i64, i64, f64p, C.POINTER(LcReconParams), vp,
+                                    C.POINTER(LcReport), u32]),
+        "lc_mbir3d": (C.c_int, [vp, i32, C.POINTER(vp), C.POINTER(i64), i64, i64, f64p, f64p,
+                                f64p, f64p, C.POINTER(LcReconParams), vp, C.POINTER(i64),
+                                C.POINTER(LcReport), u32]),
+        "lc_mbir3d_f64": (C.c_int, [vp, i32, C.POINTER(vp), C.POINTER(i64), i64, i64, f64p, f64p,
+                                    f64p, f64p, C.POINTER(LcReconParams), vp, C.POINTER(i64),
+                                    C.POINTER(LcReport), u32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

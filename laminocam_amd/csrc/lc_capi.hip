@@ -945,3 +945,80 @@ int lc_server_destroy(lc_server s) {
 }
 
 } // extern "C"
+
+// ---- whole-solve drivers ----
+extern "C" {
+
+void lc_recon_params_default(lc_recon_params *p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->regularizer = LC_REG_UNCONSTRAINED;
+    p->max_iters = 100;
+    p->inner_iters = 1;
+    p->lambda = 0.1f;
+    p->mu = 10.0f;
+    p->tol = 1e-5f;
+    p->xtol = 1e-5f;
+    p->pad_factor = 1.4142f;
+}
+
+int lc_prepare_sinogram(lc_ctx ctx, int32_t ndatasets, const float *const *projs,
+                        const int64_t *nang, int64_t nrows, int64_t ncols, float *sino,
+                        float *max_out, uint32_t flags) {
+    return guard([&] {
+        LC_REQUIRE(ctx, "lc_prepare_sinogram: NULL context");
+        prepare_sinogram<float>(ctx->c, ndatasets, projs, nang, nrows, ncols, sino, max_out, flags);
+    });
+}
+
+int lc_prepare_sinogram_f64(lc_ctx ctx, int32_t ndatasets, const double *const *projs,
+                            const int64_t *nang, int64_t nrows, int64_t ncols, double *sino,
+                            double *max_out, uint32_t flags) {
+    return guard([&] {
+        LC_REQUIRE(ctx, "lc_prepare_sinogram_f64: NULL context");
+        prepare_sinogram<double>(ctx->c, ndatasets, projs, nang, nrows, ncols, sino, max_out, flags);
+    });
+}
+
+int lc_mbir2d(lc_ctx ctx, const float *sino, int64_t nslices, int64_t nang, int64_t ncols,
+              const double *theta, const lc_recon_params *params, float *recon, lc_report *rep,
+              uint32_t flags) {
+    return guard([&] {
+        LC_REQUIRE(ctx, "lc_mbir2d: NULL context");
+        mbir2d<float>(ctx->c, sino, nslices, nang, ncols, theta, params, recon, rep, flags);
+    });
+}
+
+int lc_mbir2d_f64(lc_ctx ctx, const double *sino, int64_t nslices, int64_t nang, int64_t ncols,
+                  const double *theta, const lc_recon_params *params, double *recon,
+                  lc_report *rep, uint32_t flags) {
+    return guard([&] {
+        LC_REQUIRE(ctx, "lc_mbir2d_f64: NULL context");
+        mbir2d<double>(ctx->c, sino, nslices, nang, ncols, theta, params, recon, rep, flags);
+    });
+}
+
+int lc_mbir3d(lc_ctx ctx, int32_t ndatasets, const float *const *projs, const int64_t *nang,
+              int64_t nrows, int64_t ncols, const double *theta, const double *gamma,
+              const double *beta, const double *shifts, const lc_recon_params *params,
+              float *recon, int64_t *padded_dims, lc_report *rep, uint32_t flags) {
+    return guard([&] {
+        LC_REQUIRE(ctx, "lc_mbir3d: NULL context");
+        mbir3d<float>(ctx->c, ndatasets, projs, nang, nrows, ncols, theta, gamma, beta, shifts, params,
+                      recon, padded_dims, rep, flags);
+    });
+}
+
+int lc_mbir3d_f64(lc_ctx ctx, int32_t ndatasets, const double *const *projs,
+                  const int64_t *nang, int64_t nrows, int64_t ncols, const double *theta,
+                  const double *gamma, const double *beta, const double *shifts,
+                  const lc_recon_params *params, double *recon, int64_t *padded_dims,
+                  lc_report *rep, uint32_t flags) {
+    return guard([&] {
+        LC_REQUIRE(ctx, "lc_mbir3d_f64: NULL context");
+        mbir3d<double>(ctx->c, ndatasets, projs, nang, nrows, ncols, theta, gamma, beta, shifts, params,
+                       recon, padded_dims, rep, flags);
+    });
+}
+
+} // extern "C"

@@ -349,4 +349,16 @@ class Batch2D {
 void fd_launch(Ctx &c, int op, const float *in, float *out, int n1, int n2, int n3, int boundary);
 void fd_launch(Ctx &c, int op, const double *in, double *out, int n1, int n2, int n3, int boundary);
 
+// whole-solve drivers (lc_driver.hip), T = float or double
+template <typename T>
+void prepare_sinogram(Ctx &c, int32_t nds, const T *const *projs, const int64_t *nang, int64_t nrows,
+                      int64_t ncols, T *sino, T *max_out, uint32_t flags);
+template <typename T>
+void mbir2d(Ctx &c, const T *sino, int64_t nslices, int64_t nang, int64_t ncols, const double *theta,
+            const lc_recon_params *p, T *recon, lc_report *rep, uint32_t flags);
+template <typename T>
+void mbir3d(Ctx &c, int32_t nds, const T *const *projs, const int64_t *nang, int64_t nrows, int64_t ncols,
+            const double *theta, const double *gamma, const double *beta, const double *shifts,
+            const lc_recon_params *p, T *recon, int64_t *padded_dims, lc_report *rep, uint32_t flags);
+
 } // namespace lc

@@ -168,7 +168,6 @@ static bool same_group(const ServerRequest &a, const ServerRequest &b) {
 }
 
 void Server::lane_loop(Lane &lane) {
-    Ctx &c = *lane.ctx;
     tl_pool = &lane.pool;   // this lane's batches reuse device blocks instead of hipFree
     for (;;) {
         std::vector<ServerRequest *> batch;

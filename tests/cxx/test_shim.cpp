@@ -5,6 +5,8 @@
 //   tests/test_sysmat_toeplitz.cpp   sysmat(x, grid) == sysmat(x, psf)
 //   tests/test_cg_normal.cpp         CG on A x = A x_true reduces the residual
 //   src/projection.cpp:105-108       shift count mismatch -> std::invalid_argument
+//   include/mbir2d_prep.h:44-65      prepare_sinogram: normalised, {rows, angles, cols}
+//   src/mbir.cpp:34-161              MBIR == the same steps through the API, cropped
 // Seeded inputs from a SplitMix64 stream; exits nonzero on the first failure.
 #define LAMINOCAM_STANDALONE_ARRAY
 #include "laminocam_tomocam.hpp"
@@ -180,6 +182,83 @@ int main() {
         Array<float> y = sysmat(x, psf);
         Array<float> x3 = opt::cgsolver(opt::toeplitz(psf), y, Array<float>(rd), 4, 1e-12f, 0.0f);
         CHECK("cgsolver 3-D runs", std::isfinite(dot(x3, x3)));
+    }
+    // ---- drivers ----
+    {
+        // two datasets of 5 and 4 projections {n, 3 rows, 16 cols}
+        std::vector<Dataset_t<float>> ds(2);
+        const size_t nr = 3, nc = 16, na[2] = {5, 4};
+        float mx = 0.0f;
+        for (int d = 0; d < 2; ++d) {
+            ds[d].projs = random_array<float>(dims_t{na[d], nr, nc}, 0.1, 2.0);
+            for (size_t i = 0; i < ds[d].projs.size(); ++i) mx = std::max(mx, ds[d].projs[i]);
+            for (size_t i = 0; i < na[d]; ++i) {
+                ds[d].angles.push_back((float)(2.0 * M_PI * i / na[d] + 0.3 * d));
+                ds[d].shifts.push_back({(float)(urand() - 0.5), (float)(urand() - 0.5)});
+            }
+            ds[d].gamma = (float)(-30.0 * M_PI / 180.0);
+        }
+        Sinogram2D<float> sg = prepare_sinogram(ds);
+        bool ok = (sg.data.dims() == dims_t{nr, 9, nc}) && sg.theta.size() == 9;
+        for (size_t r = 0; r < nr && ok; ++r)
+            for (size_t a = 0; a < 9 && ok; ++a)
+                for (size_t c = 0; c < nc; ++c) {
+                    const Array<float> &pj = a < 5 ? ds[0].projs : ds[1].projs;
+                    const size_t aa = a < 5 ? a : a - 5;
+                    ok = ok && sg.data[(r * 9 + a) * nc + c] == pj[(aa * nr + r) * nc + c] / mx;
+                }
+        CHECK("prepare_sinogram", ok);
+
+        ReconParams rp;
+        rp.recon_dims = {3, 12, 12};
+        rp.maxIters = 4;
+        ReconParams rp2 = rp;
+        rp2.recon_dims = {3, 16, 16};
+        Array<float> rec2 = gpu::MBIR2D(sg.data, sg.theta, rp2);
+        CHECK("MBIR2D runs", (rec2.dims() == dims_t{3, 16, 16}) && std::isfinite(dot(rec2, rec2)));
+
+        // MBIR against its steps: normalise, pad 3 x 16 -> 3 x 21 (int(16 * 1.4142) = 22
+        // -> 21; int(3 * 1.4142) = 4 -> 3), grid, backproj with shifts into the padded
+        // volume 3 x 15 x 15, CG from zeros, symmetric crop to 3 x 12 x 12
+        Array<float> rec = MBIR(ds, rp);
+        const size_t n3p = 21;
+        const dims_t odims{3, 15, 15};
+        Array<float> padded(dims_t{9, nr, n3p});
+        std::vector<float> th, ga(9, ds[0].gamma), be(9, 0.0f);
+        std::vector<std::array<float, 2>> sh;
+        for (int d = 0; d < 2; ++d) {
+            th.insert(th.end(), ds[d].angles.begin(), ds[d].angles.end());
+            sh.insert(sh.end(), ds[d].shifts.begin(), ds[d].shifts.end());
+        }
+        for (size_t a = 0; a < 9; ++a)
+            for (size_t r = 0; r < nr; ++r)
+                for (size_t c = 0; c < nc; ++c) {
+                    const Array<float> &pj = a < 5 ? ds[0].projs : ds[1].projs;
+                    const size_t aa = a < 5 ? a : a - 5;
+                    padded[(a * nr + r) * n3p + c + 2] = pj[(aa * nr + r) * nc + c] / mx;
+                }
+        cpu::PolarGrid<float> pg(th, ga, be, nr, n3p);
+        Array<float> rhs = backproj(padded, pg, odims, sh);
+        cpu::PointSpreadFunction<float> pp(pg, odims);
+        Array<float> xs = opt::cgsolver(opt::toeplitz(pp), rhs, Array<float>(odims), 4, 1e-5f, 1e-5f);
+        bool same = (rec.dims() == dims_t{3, 12, 12});
+        for (size_t i = 0; i < 3 && same; ++i)
+            for (size_t j = 0; j < 12; ++j)
+                for (size_t k = 0; k < 12; ++k)
+                    same = same && rec[(i * 12 + j) * 12 + k] == xs[(i * 15 + j + 1) * 15 + k + 1];
+        CHECK("MBIR == its steps, cropped", same);
+        std::vector<std::array<float, 2>> short_sh(3);
+        std::vector<Dataset_t<float>> bad(1);
+        bad[0].projs = random_array<float>(dims_t{4, nr, nc});
+        bad[0].angles.assign(4, 0.0f);
+        bad[0].shifts = short_sh;
+        bool threw = false;
+        try {
+            MBIR(bad, rp);
+        } catch (const std::invalid_argument &) {
+            threw = true;
+        }
+        CHECK("MBIR shift count mismatch", threw);
     }
     std::printf("%s: %d failure(s)\n", g_fail ? "FAILED" : "OK", g_fail);
     return g_fail ? 1 : 0;
