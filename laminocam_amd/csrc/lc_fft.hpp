@@ -70,20 +70,20 @@ __device__ __forceinline__ int fdiv(int b, int d, float inv) {
 }
 
 // multiply by SIGN * i
-template <int SIGN, typename C> __device__ __forceinline__ C mul_si(C a) {
+template <int SIGN, typename C> __host__ __device__ __forceinline__ C mul_si(C a) {
     C r;
     if (SIGN < 0) { r.x = a.y; r.y = -a.x; }
     else { r.x = -a.y; r.y = a.x; }
     return r;
 }
 
-template <int SIGN, typename C> __device__ __forceinline__ void dft2(C *v) {
+template <int SIGN, typename C> __host__ __device__ __forceinline__ void dft2(C *v) {
     C a = v[0], b = v[1];
     v[0] = a + b;
     v[1] = a - b;
 }
 
-template <int SIGN, typename C> __device__ __forceinline__ void dft4(C *v) {
+template <int SIGN, typename C> __host__ __device__ __forceinline__ void dft4(C *v) {
     C a0 = v[0] + v[2], a1 = v[0] - v[2];
     C b0 = v[1] + v[3], b1 = v[1] - v[3];
     C jb1 = mul_si<SIGN>(b1);
@@ -93,7 +93,7 @@ template <int SIGN, typename C> __device__ __forceinline__ void dft4(C *v) {
     v[3] = a1 - jb1;
 }
 
-template <int SIGN, typename C> __device__ __forceinline__ void dft3(C *v) {
+template <int SIGN, typename C> __host__ __device__ __forceinline__ void dft3(C *v) {
     using R = decltype(C{}.x);
     const R h = (R)0.8660254037844386467637231707529362;  // sqrt(3)/2
     C t1 = v[1] + v[2];
@@ -105,7 +105,7 @@ template <int SIGN, typename C> __device__ __forceinline__ void dft3(C *v) {
     v[2] = t2 - jd;
 }
 
-template <int SIGN, typename C> __device__ __forceinline__ void dft8(C *v) {
+template <int SIGN, typename C> __host__ __device__ __forceinline__ void dft8(C *v) {
     using R = decltype(C{}.x);
     const R s = (R)0.7071067811865475244008443621048490;
     C e[4] = {v[0], v[2], v[4], v[6]};
@@ -129,11 +129,48 @@ template <int SIGN, typename C> __device__ __forceinline__ void dft8(C *v) {
     v[7] = e[3] - o3;
 }
 
-template <int R, int SIGN, typename C> __device__ __forceinline__ void dftR(C *v) {
+// 16 points as 4 x 4: DFT_4 over n1 of v[4 n1 + n2], twiddle w^{n2 k1}, DFT_4
+// over n2; output k1 + 4 k2 (w = e^{SIGN 2 pi i / 16})
+template <int SIGN, typename C> __host__ __device__ __forceinline__ void dft16(C *v) {
+    using R = decltype(C{}.x);
+    const R c1 = (R)0.9238795325112867561281831893967882;   // cos(pi/8)
+    const R s1 = (R)0.3826834323650897717284599840303989;   // sin(pi/8)
+    const R h = (R)0.7071067811865475244008443621048490;
+    const C w1 = cmake<C>(c1, SIGN * s1), w2 = cmake<C>(h, SIGN * h), w3 = cmake<C>(s1, SIGN * c1);
+    const C w6 = cmake<C>(-h, SIGN * h), w9 = cmake<C>(-c1, -SIGN * s1);
+    C a[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+        a[n2][0] = v[n2];
+        a[n2][1] = v[4 + n2];
+        a[n2][2] = v[8 + n2];
+        a[n2][3] = v[12 + n2];
+        dft4<SIGN>(a[n2]);
+    }
+    a[1][1] = cmul(a[1][1], w1);
+    a[1][2] = cmul(a[1][2], w2);
+    a[1][3] = cmul(a[1][3], w3);
+    a[2][1] = cmul(a[2][1], w2);
+    a[2][2] = mul_si<SIGN>(a[2][2]);
+    a[2][3] = cmul(a[2][3], w6);
+    a[3][1] = cmul(a[3][1], w3);
+    a[3][2] = cmul(a[3][2], w6);
+    a[3][3] = cmul(a[3][3], w9);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        C t[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+        dft4<SIGN>(t);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+    }
+}
+
+template <int R, int SIGN, typename C> __host__ __device__ __forceinline__ void dftR(C *v) {
     if constexpr (R == 2) dft2<SIGN>(v);
     else if constexpr (R == 3) dft3<SIGN>(v);
     else if constexpr (R == 4) dft4<SIGN>(v);
-    else dft8<SIGN>(v);
+    else if constexpr (R == 8) dft8<SIGN>(v);
+    else dft16<SIGN>(v);
 }
 
 template <typename T, int R, int SIGN, int NT, int MAXE, bool LFAST>
@@ -237,14 +274,27 @@ __device__ void fft_lds(cpx<T> *buf, int nl, int ls, const FftPlan &plan,
 // ---------------------------------------------------------------------------
 constexpr int ct_twos(int L) { int t = 0; while (L % 2 == 0) { L /= 2; ++t; } return t; }
 constexpr int ct_threes(int L) { int t = 0; while (L % 3 == 0) { L /= 3; ++t; } return t; }
-constexpr int ct_nst(int L) {
-    return ct_twos(L) / 3 + (ct_twos(L) % 3 ? 1 : 0) + ct_threes(L);
+// radices: as many RM (16 or 8) as divide L, one of 8 / 4 / 2 for the
+// remaining power of two, and the 3s.  Two orders, chosen per length by
+// timing the fused column convolution (fft_conv_ct, whose middle runs in the
+// last stage) on gfx950:
+//   RMs, remainder, 3s                      (L = 1024: 16 16 4; 1536: 8 8 8 3)
+//   one RM, remainder, 3s, the other RMs    (L = 2048: 16 8 16; 768: 8 4 3 8)
+constexpr int ct_lg(int RM) { return RM == 16 ? 4 : 3; }
+constexpr bool ct_big_last(int L) { return L == 2048 || L == 768; }
+constexpr int ct_nst(int L, int RM = 8) {
+    return ct_twos(L) / ct_lg(RM) + (ct_twos(L) % ct_lg(RM) ? 1 : 0) + ct_threes(L);
 }
-constexpr int ct_radix(int L, int s) {
-    const int n8 = ct_twos(L) / 3, rem = ct_twos(L) % 3;
-    if (s < n8) return 8;
-    if (rem && s == n8) return rem == 2 ? 4 : 2;
-    return 3;
+constexpr int ct_radix(int L, int s, int RM = 8) {
+    const int nb = ct_twos(L) / ct_lg(RM), rem = ct_twos(L) % ct_lg(RM);
+    const int nrem = rem ? 1 : 0, th = ct_threes(L);
+    const int lead = ct_big_last(L) ? (nb > 0 ? 1 : 0) : nb;   // RMs before the remainder
+    if (s < lead) return RM;
+    s -= lead;
+    if (s < nrem) return 1 << rem;
+    s -= nrem;
+    if (s < th) return 3;
+    return RM;
 }
 
 // Swizzled LDS addresses of a thread's R elements x0 + r*S.  When the stride
@@ -276,7 +326,7 @@ __device__ __forceinline__ void fft_stage_ct(cpx<T> *buf, int nl, int lsh,
     const int nbf = LSH >= 0 ? (LR << (LSH >= 0 ? LSH : 0)) : nl * LR;
     const int tid = threadIdx.x;
     C v[MAXB][R];
-    C wa[MAXB], wb[MAXB];
+    C wa[MAXB], wb[MAXB], wc[MAXB];
     // twiddle loads first (k = 0 reads w^0 = 1, an exact multiply) so their
     // latency overlaps the LDS reads and the barrier
     if constexpr (NS > 1) {
@@ -287,7 +337,8 @@ __device__ __forceinline__ void fft_stage_ct(cpx<T> *buf, int nl, int lsh,
             const int j = LFAST ? (b >> ls) : (b % LR);
             const int k = j % NS;
             wa[t] = tw[k * TWS];
-            if constexpr (R == 8) wb[t] = tw[4 * k * TWS];
+            if constexpr (R >= 8) wb[t] = tw[4 * k * TWS];
+            if constexpr (R == 16) wc[t] = tw[8 * k * TWS];
         }
     }
 #pragma unroll
@@ -331,12 +382,25 @@ __device__ __forceinline__ void fft_stage_ct(cpx<T> *buf, int nl, int lsh,
                 if (SIGN > 0) w[1].y = -w[1].y;
                 if constexpr (R >= 3) w[2] = cmul(w[1], w[1]);
                 if constexpr (R >= 4) w[3] = cmul(w[2], w[1]);
-                if constexpr (R == 8) {
+                if constexpr (R >= 8) {
                     w[4] = wb[t];
                     if (SIGN > 0) w[4].y = -w[4].y;
                     w[5] = cmul(w[4], w[1]);
                     w[6] = cmul(w[4], w[2]);
                     w[7] = cmul(w[4], w[3]);
+                }
+                if constexpr (R == 16) {
+                    // w^8 from the table, w^12 = w^8 w^4: every power within
+                    // two products of a table entry
+                    w[8] = wc[t];
+                    if (SIGN > 0) w[8].y = -w[8].y;
+                    w[9] = cmul(w[8], w[1]);
+                    w[10] = cmul(w[8], w[2]);
+                    w[11] = cmul(w[8], w[3]);
+                    w[12] = cmul(w[8], w[4]);
+                    w[13] = cmul(w[12], w[1]);
+                    w[14] = cmul(w[12], w[2]);
+                    w[15] = cmul(w[12], w[3]);
                 }
 #pragma unroll
                 for (int r = 1; r < R; ++r) v[t][r] = cmul(v[t][r], w[r]);
@@ -368,16 +432,30 @@ __device__ __forceinline__ void fft_stage_ct(cpx<T> *buf, int nl, int lsh,
     __syncthreads();
 }
 
+// Radix-16 stages for FP32 power-of-two lengths >= 1024 when every thread
+// holds at least 16 elements: one stage (an LDS round trip and two barriers)
+// fewer than radix 8.  Elsewhere radix 8: the stage count is the same or
+// the larger radix measured slower (lower occupancy).
+template <typename T, int MAXE, int L> constexpr int ct_rmax() {
+    return (sizeof(T) == 4 && MAXE >= 16 && L >= 1024 && ct_threes(L) == 0) ? 16 : 8;
+}
+
+// Stages S .. SEND - 1 of the plan (REV: the radices in reverse order).  PIN
+// prunes the plan's first stage, POUT its last.
 template <typename T, int L, int S, int NS, int SIGN, int NT, int MAXE, bool LFAST, int LSH,
-          bool PIN, bool POUT>
+          bool PIN, bool POUT, bool REV = false, int SEND = -1>
 __device__ __forceinline__ void fft_stages_ct(cpx<T> *buf, int nl, int lsh,
                                               const cpx<T> *__restrict__ tw) {
-    constexpr int R = ct_radix(L, S);
-    constexpr bool LAST = S + 1 == ct_nst(L);
-    fft_stage_ct<T, L, R, NS, SIGN, NT, MAXE, LFAST, LSH, PIN && S == 0, POUT && LAST>(buf, nl, lsh,
-                                                                                       tw);
+    constexpr int RM = ct_rmax<T, MAXE, L>();
+    constexpr int NST = ct_nst(L, RM);
+    constexpr int END = SEND < 0 ? NST : SEND;
+    constexpr int R = ct_radix(L, REV ? NST - 1 - S : S, RM);
+    constexpr bool LAST = S + 1 == END;
+    fft_stage_ct<T, L, R, NS, SIGN, NT, MAXE, LFAST, LSH, PIN && S == 0, POUT && S + 1 == NST>(
+        buf, nl, lsh, tw);
     if constexpr (!LAST)
-        fft_stages_ct<T, L, S + 1, NS * R, SIGN, NT, MAXE, LFAST, LSH, PIN, POUT>(buf, nl, lsh, tw);
+        fft_stages_ct<T, L, S + 1, NS * R, SIGN, NT, MAXE, LFAST, LSH, PIN, POUT, REV, SEND>(buf, nl,
+                                                                                           lsh, tw);
 }
 
 // Full transform of nl lines of compile-time length L (row layout: line
@@ -389,6 +467,119 @@ template <typename T, int L, int SIGN, int NT, int MAXE, bool LFAST, int LSH = -
 __device__ __forceinline__ void fft_lds_ct(cpx<T> *buf, int nl, int lsh,
                                            const cpx<T> *__restrict__ tw) {
     fft_stages_ct<T, L, 0, 1, SIGN, NT, MAXE, LFAST, LSH, PIN, POUT>(buf, nl, lsh, tw);
+}
+
+// ---------------------------------------------------------------------------
+// Column-layout circular convolution with a real spectrum:
+//   line <- IFFT(kr .* FFT(line))   (unnormalised inverse)
+// The forward transform's last stage leaves butterfly j holding outputs
+// j + r L/R (r < R) in registers; the inverse transform, with its radices in
+// reverse order, starts with a radix-R stage over those same indices.  So the
+// two stages and the spectrum multiply run in registers: no LDS round trip
+// between the transforms.  kr[t][r] is the spectrum at index j + r L/R of
+// butterfly t of this thread (see conv_mid_index).  The forward input is
+// pruned (PIN) and the inverse output (POUT) as in fft_stage_ct.
+// ---------------------------------------------------------------------------
+template <typename T, int L, int MAXE> struct ConvMid {
+    static constexpr int RM = ct_rmax<T, MAXE, L>();
+    static constexpr int NST = ct_nst(L, RM);
+    static constexpr int R = ct_radix(L, NST - 1, RM);   // forward last = inverse first
+    static constexpr int LR = L / R;
+    static constexpr int MAXB = (MAXE + R - 1) / R;
+};
+
+// line and index j of this thread's mid-stage butterfly t (false if none)
+template <typename T, int L, int NT, int MAXE, int LSH>
+__device__ __forceinline__ bool conv_mid_index(int t, int nl, int lsh, int &line, int &j) {
+    constexpr int LR = ConvMid<T, L, MAXE>::LR;
+    const int ls = LSH >= 0 ? LSH : lsh;
+    const int b = threadIdx.x + t * NT;
+    if (b >= nl * LR) return false;
+    j = b >> ls;
+    line = b & ((1 << ls) - 1);
+    return true;
+}
+
+template <typename T, int L, int NT, int MAXE, int LSH>
+__device__ __forceinline__ void fft_conv_mid(cpx<T> *buf, int nl, int lsh,
+                                             const cpx<T> *__restrict__ tw,
+                                             const T (&kr)[ConvMid<T, L, MAXE>::MAXB]
+                                                          [ConvMid<T, L, MAXE>::R]) {
+    using C = cpx<T>;
+    using CM = ConvMid<T, L, MAXE>;
+    constexpr int R = CM::R, LR = CM::LR, MAXB = CM::MAXB;
+    const int ls = LSH >= 0 ? LSH : lsh;
+    C v[MAXB][R];
+    C wa[MAXB], wb[MAXB], wc[MAXB];
+    // forward last stage: NS = L / R, so k = j and the twiddle step is 1
+#pragma unroll
+    for (int t = 0; t < MAXB; ++t) {
+        int line, j;
+        if (!conv_mid_index<T, L, NT, MAXE, LSH>(t, nl, lsh, line, j)) j = 0;
+        wa[t] = tw[j];
+        if constexpr (R >= 8) wb[t] = tw[4 * j];
+        if constexpr (R == 16) wc[t] = tw[8 * j];
+    }
+#pragma unroll
+    for (int t = 0; t < MAXB; ++t) {
+        int line, j;
+        if (conv_mid_index<T, L, NT, MAXE, LSH>(t, nl, lsh, line, j)) {
+            const int b = (j << ls) + line;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[t][r] = buf[swz(b + r * (LR << ls))];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < MAXB; ++t) {
+        int line, j;
+        if (conv_mid_index<T, L, NT, MAXE, LSH>(t, nl, lsh, line, j)) {
+            C w[R];
+            w[1] = wa[t];
+            if constexpr (R >= 3) w[2] = cmul(w[1], w[1]);
+            if constexpr (R >= 4) w[3] = cmul(w[2], w[1]);
+            if constexpr (R >= 8) {
+                w[4] = wb[t];
+                w[5] = cmul(w[4], w[1]);
+                w[6] = cmul(w[4], w[2]);
+                w[7] = cmul(w[4], w[3]);
+            }
+            if constexpr (R == 16) {
+                w[8] = wc[t];
+                w[9] = cmul(w[8], w[1]);
+                w[10] = cmul(w[8], w[2]);
+                w[11] = cmul(w[8], w[3]);
+                w[12] = cmul(w[8], w[4]);
+                w[13] = cmul(w[12], w[1]);
+                w[14] = cmul(w[12], w[2]);
+                w[15] = cmul(w[12], w[3]);
+            }
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[t][r] = cmul(v[t][r], w[r]);
+            dftR<R, -1>(v[t]);
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[t][r] = cscale(v[t][r], kr[t][r]);
+            // inverse first stage (NS = 1, no twiddles): outputs j R + r
+            dftR<R, 1>(v[t]);
+            const int ob = ((j * R) << ls) + line;
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[swz(ob + (r << ls))] = v[t][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int L, int NT, int MAXE, int LSH>
+__device__ __forceinline__ void fft_conv_ct(cpx<T> *buf, int nl, int lsh,
+                                            const cpx<T> *__restrict__ tw,
+                                            const T (&kr)[ConvMid<T, L, MAXE>::MAXB]
+                                                         [ConvMid<T, L, MAXE>::R]) {
+    using CM = ConvMid<T, L, MAXE>;
+    static_assert(CM::NST >= 2, "fused convolution needs at least two stages");
+    fft_stages_ct<T, L, 0, 1, -1, NT, MAXE, true, LSH, true, false, false, CM::NST - 1>(buf, nl, lsh,
+                                                                                      tw);
+    fft_conv_mid<T, L, NT, MAXE, LSH>(buf, nl, lsh, tw, kr);
+    fft_stages_ct<T, L, 1, CM::R, 1, NT, MAXE, true, LSH, false, true, true>(buf, nl, lsh, tw);
 }
 
 // FP64 twiddle generation: tw[k] = exp(-2 pi i k / L)
