@@ -727,11 +727,48 @@ def main():
         "lamino3d": lamino,
         "roofline": roofline,
         "roofline_toeplitz": roofline_toeplitz,
+        "hbm_copy_reference": copy_reference(),
         "cpu_baseline": cpu,
         "kernel_passes": [{"N": r["N"], "pass": PASS_NAMES[r["pass_"]], "ms": round(r["ms"], 4),
                            "GBps": round(r["bytes"] / (r["ms"] * 1e-3) / 1e9, 1)} for r in prof],
     }
     print(json.dumps(out), flush=True)
+
+
+def copy_reference(nbytes=1 << 30, reps=10):
+    """Device-to-device hipMemcpyAsync of nbytes on the null stream, timed with
+    HIP events: the HBM rate the runtime's own copy reaches on this GPU, a
+    practical ceiling for kernels that read and write equal amounts."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        return None
+    try:
+        hip.hipMemset(a, 0, C.c_size_t(nbytes))
+        hip.hipEventCreate(C.byref(e0))
+        hip.hipEventCreate(C.byref(e1))
+        d2d = 3   # hipMemcpyDeviceToDevice
+        for _ in range(3):
+            hip.hipMemcpyAsync(b, a, C.c_size_t(nbytes), d2d, None)
+        hip.hipDeviceSynchronize()
+        hip.hipEventRecord(e0, None)
+        for _ in range(reps):
+            hip.hipMemcpyAsync(b, a, C.c_size_t(nbytes), d2d, None)
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = C.c_float()
+        hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+        gbps = 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9
+        return {"GB/s": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBS, 4),
+                "how": f"hipMemcpyAsync device to device, {nbytes >> 20} MiB, {reps} copies, "
+                       "HIP events (read + write bytes)"}
+    finally:
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+        hip.hipFree(a)
+        hip.hipFree(b)
 
 
 if __name__ == "__main__":
