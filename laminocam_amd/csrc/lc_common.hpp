@@ -77,6 +77,15 @@ struct Error : public std::runtime_error {
 
 inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
+// orders a wave's LDS writes before its later LDS reads of other lanes' data
+// (and reads before later overwrites); a wave executes in lockstep, so no
+// barrier across waves is needed
+__device__ __forceinline__ void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // blockIdx remap so that consecutive logical blocks land on the same XCD.
 // Hardware dispatch sends block b to XCD (b mod 8); giving XCD x a contiguous
 // run of logical ids lets its private L2 share the cache lines neighbouring

@@ -17,6 +17,7 @@ struct Nufft2D {
     int w = 0;            // kernel width in cells
     double beta = 0.0;    // kernel shape
     DBuf corr;            // float[N]
+    DBuf cs;              // double2[nang]: (cos theta, sin theta), FP64
     FftPlan plan_nf{};    // fine-grid FFT
     FftPlan plan_det{};   // detector FFT (valid if det_fast)
     bool det_fast = false;

@@ -180,15 +180,18 @@ __device__ __forceinline__ void ray_box(double c, double s, const SpreadGeom &g,
 // grid: (ntx * ntx, nslices).  C: per slice nang x nrad complex (detector-
 // transformed sinogram).  fine: per slice nf x nf complex.
 __global__ void __launch_bounds__(64 * NU_WAVES)
-k_spread2d(const float2 *__restrict__ C, float2 *__restrict__ fine, const double *__restrict__ theta,
+k_spread2d(const float2 *__restrict__ C, float2 *__restrict__ fine, const double2 *__restrict__ cs,
            SpreadGeom g) {
     float2 *acc = reinterpret_cast<float2 *>(lc_smem);   // [NU_WAVES][T*T]
+    // [NU_WAVES][64][2 NU_W]: kernel weights of the wave's current 64 points
+    float *wtab = reinterpret_cast<float *>(acc + NU_WAVES * NU_T * NU_T);
     const int slice = blockIdx.y;
     const int tile = blockIdx.x;
     const int ty = tile / g.ntx, tx = tile - ty * g.ntx;
     const int x0 = tx * NU_T, y0 = ty * NU_T;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     float2 *my = acc + wave * NU_T * NU_T;
+    float *wt = wtab + wave * 64 * 2 * NU_W;
     for (int i = lane; i < NU_T * NU_T; i += 64) my[i] = make_float2(0.f, 0.f);
     __syncthreads();
     const float2 *Cs = C + (int64_t)slice * g.nang * g.nrad;
@@ -207,7 +210,9 @@ k_spread2d(const float2 *__restrict__ C, float2 *__restrict__ fine, const double
                 int k0 = 1, k1 = 0;
                 double cth = 0.0, sth = 0.0;
                 if (ai < g.nang) {
-                    sincos(theta[ai], &sth, &cth);
+                    const double2 t = cs[ai];
+                    cth = t.x;
+                    sth = t.y;
                     ray_box(cth, sth, g, bx0, bx1, by0, by1, k0, k1);
                 }
                 unsigned long long hits = __ballot(k0 <= k1);
@@ -240,6 +245,18 @@ k_spread2d(const float2 *__restrict__ C, float2 *__restrict__ fine, const double
                                  ly0 < y0 + NU_T;
                             if (ok) cv = Cs[(int64_t)ang * g.nrad + k];
                         }
+                        // the point's 2 x 8 kernel weights, once per point (not
+                        // per footprint cell): the lanes of the spread loop read
+                        // them back from the wave's table
+                        wave_lds_fence();
+                        if (ok) {
+#pragma unroll
+                            for (int d = 0; d < NU_W; ++d) {
+                                wt[lane * 2 * NU_W + d] = es_kernel(((float)d + fx) * inv_hw, g.beta);
+                                wt[lane * 2 * NU_W + NU_W + d] = es_kernel(((float)d + fy) * inv_hw, g.beta);
+                            }
+                        }
+                        wave_lds_fence();
                         unsigned long long pts = __ballot(ok);
                         while (pts) {
                             const int s2 = __ffsll((long long)pts) - 1;
@@ -248,9 +265,7 @@ k_spread2d(const float2 *__restrict__ C, float2 *__restrict__ fine, const double
                             const int py0 = __builtin_amdgcn_readlane(ly0, s2);
                             const int cx = px0 + dx - x0, cy = py0 + dy - y0;
                             if (cx >= 0 && cx < NU_T && cy >= 0 && cy < NU_T) {
-                                const float zx = ((float)dx + rl_f32(fx, s2)) * inv_hw;
-                                const float zy = ((float)dy + rl_f32(fy, s2)) * inv_hw;
-                                const float wgt = es_kernel(zx, g.beta) * es_kernel(zy, g.beta);
+                                const float wgt = wt[s2 * 2 * NU_W + dx] * wt[s2 * 2 * NU_W + NU_W + dy];
                                 const float vr = rl_f32(cv.x, s2), vi = rl_f32(cv.y, s2);
                                 float2 &acc1 = my[nu_tidx(cy, cx)];
                                 acc1.x += wgt * vr;
@@ -281,13 +296,12 @@ k_spread2d(const float2 *__restrict__ C, float2 *__restrict__ fine, const double
 // Type-2 interpolation: one thread per point
 // ---------------------------------------------------------------------------
 __global__ void k_interp2d(const float2 *__restrict__ fine, float2 *__restrict__ C,
-                           const double *__restrict__ theta, SpreadGeom g, int npts) {
+                           const double2 *__restrict__ cs, SpreadGeom g, int npts) {
     const int slice = blockIdx.y;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npts) return;
     const int ang = p / g.nrad, k = p - ang * g.nrad;
-    double s, c;
-    sincos(theta[ang], &s, &c);
+    const double c = cs[ang].x, s = cs[ang].y;
     const double r = ((double)k + 0.5) * g.dr - M_PI;
     const double gx = 0.5 * g.nf + g.sc * r * c, gy = 0.5 * g.nf + g.sc * r * s;
     const double hw = 0.5 * NU_W;
@@ -476,8 +490,24 @@ static double es_phihat(double beta, int w, double xi) {
     return hw * 0.5 * M_PI * s;
 }
 
+// (cos, sin) of every projection angle, FP64, once per geometry: the spread
+// tests every angle against every tile and the interpolation reads one per
+// point
+__global__ void k_angle_table(const double *__restrict__ theta, double2 *__restrict__ cs, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        double s, c;
+        sincos(theta[i], &s, &c);
+        cs[i] = make_double2(c, s);
+    }
+}
+
 void nufft2d_init(Ctx &c, Geom2D &g) {
     Nufft2D &u = g.nu;
+    u.cs.alloc(sizeof(double2) * g.nang);
+    hipLaunchKernelGGL(k_angle_table, dim3(ceil_div(g.nang, 256)), dim3(256), 0, c.stream(),
+                       g.d_theta.as<double>(), u.cs.as<double2>(), (int)g.nang);
+    LC_HIP(hipGetLastError());
     const int N = (int)g.N;
     int64_t nf = 2 * N;
     while (!is_2_3_smooth(nf) || nf < 2 * NU_W) ++nf;
@@ -565,9 +595,9 @@ void type1(Ctx &c, const Geom2D &g, const NuWork &w, float *vol, int ns) {
     const Nufft2D &u = g.nu;
     const int N = (int)g.N, nf = u.nf;
     const SpreadGeom sg = spread_geom(g);
-    const size_t shs = sizeof(float2) * NU_WAVES * NU_T * NU_T;
+    const size_t shs = sizeof(float2) * NU_WAVES * NU_T * NU_T + sizeof(float) * NU_WAVES * 64 * 2 * NU_W;
     hipLaunchKernelGGL(k_spread2d, dim3(sg.ntx * sg.ntx, ns), dim3(64 * NU_WAVES), shs, c.stream(),
-                       w.C, w.F, g.d_theta.as<double>(), sg);
+                       w.C, w.F, g.nu.cs.as<double2>(), sg);
     LC_HIP(hipGetLastError());
     const float2 *tw = c.twiddles<float>(nf);
     const int nl = std::min(NU_NT * NU_ME / nf, nf);
@@ -598,7 +628,7 @@ void type2(Ctx &c, const Geom2D &g, const NuWork &w, const float *vol, int ns) {
     LC_HIP(hipGetLastError());
     const SpreadGeom sg = spread_geom(g);
     hipLaunchKernelGGL(k_interp2d, dim3(ceil_div(npts, 256), ns), dim3(256), 0, c.stream(), w.F, w.C,
-                       g.d_theta.as<double>(), sg, (int)npts);
+                       g.nu.cs.as<double2>(), sg, (int)npts);
     LC_HIP(hipGetLastError());
 }
 

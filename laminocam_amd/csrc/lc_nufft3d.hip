@@ -148,6 +148,25 @@ k_interp3d(const float2 *__restrict__ F, float2 *__restrict__ C, const double *_
     C[q] = make_float2(re, im);
 }
 
+// include/fft.h:53-74 (phase_shift2d): detector spectrum C[p][j][k] *=
+// exp(-i (qx_k dx_p + qy_j dy_p)), qx_k = (k + 1/2) 2 pi / ncols - pi,
+// qy_j = (j + 1/2) 2 pi / nrows - pi; once per point, before the spread
+__global__ void k_phase_shift3d(float2 *__restrict__ C, const double *__restrict__ shifts, Grid3 g,
+                                int64_t np) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const int64_t pj = i / g.ncols;
+    const int kk = (int)(i - pj * g.ncols);
+    const int pp = (int)(pj / g.nrows), jj = (int)(pj - (int64_t)pp * g.nrows);
+    const double qx = ((double)kk + 0.5) * g.dX - M_PI;
+    const double qy = ((double)jj + 0.5) * g.dY - M_PI;
+    double s, c;
+    sincos(qx * shifts[2 * pp] + qy * shifts[2 * pp + 1], &s, &c);
+    const float cr = (float)c, si = (float)-s;
+    const float2 cv = C[i];
+    C[i] = make_float2(cv.x * cr - cv.y * si, cv.x * si + cv.y * cr);
+}
+
 // ---------------------------------------------------------------------------
 // type-1 spreading: each workgroup owns a TZ x TY x TX tile of the fine grid.
 // For every periodic image of the tile, lanes test 64 projections at a time:
@@ -160,7 +179,7 @@ k_interp3d(const float2 *__restrict__ F, float2 *__restrict__ C, const double *_
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(64 * S3_WAVES)
 k_spread3d(const float2 *__restrict__ C, float2 *__restrict__ F, const double *__restrict__ rot,
-           const double *__restrict__ shifts, Grid3 g, int ntz, int nty, int ntx) {
+           Grid3 g, int ntz, int nty, int ntx) {
     constexpr int TC = TZ * TY * TX;
     float2 *acc = reinterpret_cast<float2 *>(lc_smem);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -237,11 +256,6 @@ k_spread3d(const float2 *__restrict__ C, float2 *__restrict__ F, const double *_
                                   hj1 = __builtin_amdgcn_readlane(j1, src);
                         const int nk = hk1 - hk0 + 1;
                         const int ncand = nk * (hj1 - hj0 + 1);
-                        double sdx = 0.0, sdy = 0.0;
-                        if (shifts) {
-                            sdx = shifts[2 * pp];
-                            sdy = shifts[2 * pp + 1];
-                        }
                         for (int cb = 0; cb < ncand; cb += 64) {
                             const int ci = cb + lane;
                             bool ok = false;
@@ -261,18 +275,7 @@ k_spread3d(const float2 *__restrict__ C, float2 *__restrict__ F, const double *_
                                      gx >= c0[2] - hw && gx <= c0[2] + TX - 1 + hw;
                             }
                             float2 cv = make_float2(0.f, 0.f);
-                            if (ok) {
-                                cv = C[((int64_t)pp * g.nrows + jj) * g.ncols + kk];
-                                if (shifts) {
-                                    // include/fft.h:53-74: C *= exp(-i (qx dx + qy dy))
-                                    const double qx = ((double)kk + 0.5) * g.dX - M_PI;
-                                    const double qy = ((double)jj + 0.5) * g.dY - M_PI;
-                                    double s, c;
-                                    sincos(qx * sdx + qy * sdy, &s, &c);
-                                    const float cr = (float)c, si = (float)-s;
-                                    cv = make_float2(cv.x * cr - cv.y * si, cv.x * si + cv.y * cr);
-                                }
-                            }
+                            if (ok) cv = C[((int64_t)pp * g.nrows + jj) * g.ncols + kk];
                             unsigned long long acc_pts = __ballot(ok);
                             while (acc_pts) {
                                 const int s2 = __ffsll((long long)acc_pts) - 1;
@@ -730,10 +733,16 @@ void type1_3d(Ctx &c, const Geom3D &g, const Work3 &w, float *vol, bool mask, co
     const Nufft3D &u = g.nu;
     const int n1 = (int)g.n1, n2 = (int)g.n2, n3 = (int)g.n3;
     const Grid3 gg = grid3(g, mask);
+    if (shifts) {
+        const int64_t np = (int64_t)g.nproj * g.nrows * g.ncols;
+        hipLaunchKernelGGL(k_phase_shift3d, dim3(ceil_div(np, 256)), dim3(256), 0, c.stream(), w.C,
+                           shifts, gg, np);
+        LC_HIP(hipGetLastError());
+    }
     const int ntz = ceil_div(u.nf[0], TZ), nty = ceil_div(u.nf[1], TY), ntx = ceil_div(u.nf[2], TX);
     const size_t shs = sizeof(float2) * S3_WAVES * TZ * TY * TX;
     hipLaunchKernelGGL(k_spread3d, dim3(ntz * nty * ntx), dim3(64 * S3_WAVES), shs, c.stream(), w.C,
-                       w.F, g.d_rot.as<double>(), shifts, gg, ntz, nty, ntx);
+                       w.F, g.d_rot.as<double>(), gg, ntz, nty, ntx);
     LC_HIP(hipGetLastError());
     const float *corr = g.nu.corr.as<float>();
     // x: rows (nf0*nf1, nf2) -> (nf0*nf1, n3)
