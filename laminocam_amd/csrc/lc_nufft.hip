@@ -215,24 +215,35 @@ k_spread2d(const float2 *__restrict__ C, float2 *__restrict__ fine, const double
                     sth = t.y;
                     ray_box(cth, sth, g, bx0, bx1, by0, by1, k0, k1);
                 }
-                unsigned long long hits = __ballot(k0 <= k1);
-                while (hits) {
-                    const int src = __ffsll((long long)hits) - 1;
-                    hits &= hits - 1;
-                    // src is uniform: scalar broadcasts of the ray
-                    const int rk0 = __builtin_amdgcn_readlane(k0, src);
-                    const int rk1 = __builtin_amdgcn_readlane(k1, src);
-                    const double rc = rl_f64(cth, src), rs = rl_f64(sth, src);
-                    const int ang = a0 + src;
-                    // 64 samples of the ray at a time: each lane places one
-                    // (footprint corner, offsets, value), then the wave
-                    // spreads them one after the other in ascending k
-                    for (int kb = rk0; kb <= rk1; kb += 64) {
-                        const int k = kb + lane;
+                // samples of the hit rays packed 64 to a batch, in ray order
+                // then ascending k (the order a ray-by-ray walk spreads
+                // them): lane q of the packed sequence finds its ray by a
+                // binary search over the wave's inclusive prefix sums
+                const int nk = k0 <= k1 ? k1 - k0 + 1 : 0;
+                int incl = nk;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int t = __shfl_up(incl, o, 64);
+                    if (lane >= o) incl += t;
+                }
+                const int total = __shfl(incl, 63, 64);
+                for (int base = 0; base < total; base += 64) {
+                    {
+                        const int q = base + lane;
+                        int rr = 0;
+#pragma unroll
+                        for (int step = 32; step > 0; step >>= 1)
+                            if (__shfl(incl, rr + step - 1, 64) <= q) rr += step;
+                        rr = rr < 64 ? rr : 63;
+                        const int rincl = __shfl(incl, rr, 64), rnk = __shfl(nk, rr, 64);
+                        const int rk0 = __shfl(k0, rr, 64);
+                        const double rc = __shfl(cth, rr, 64), rs = __shfl(sth, rr, 64);
+                        const int k = rk0 + (q - (rincl - rnk));
+                        const int ang = a0 + rr;
                         int lx0 = 0, ly0 = 0;
                         float fx = 0.f, fy = 0.f;
                         float2 cv = make_float2(0.f, 0.f);
-                        bool ok = k <= rk1;
+                        bool ok = q < total;
                         if (ok) {
                             const double r = ((double)k + 0.5) * g.dr - M_PI;
                             const double gx = 0.5 * g.nf + g.sc * r * rc - ox * g.nf;
