@@ -274,30 +274,43 @@ k_spread3d(const float2 *__restrict__ C, float2 *__restrict__ F, const double *_
                                      gy >= c0[1] - hw && gy <= c0[1] + TY - 1 + hw &&
                                      gx >= c0[2] - hw && gx <= c0[2] + TX - 1 + hw;
                             }
+                            // per point, by the lane that owns it: the footprint corner
+                            // (lz0, ly0, lx0) and the offsets corner - point in FP32,
+                            // in (-hw - 1, -hw]
                             float2 cv = make_float2(0.f, 0.f);
-                            if (ok) cv = C[((int64_t)pp * g.nrows + jj) * g.ncols + kk];
+                            int lz0 = 0, ly0 = 0, lx0 = 0;
+                            float fz = 0.f, fy = 0.f, fx = 0.f;
+                            if (ok) {
+                                cv = C[((int64_t)pp * g.nrows + jj) * g.ncols + kk];
+                                lz0 = (int)ceil(gz - hw);
+                                ly0 = (int)ceil(gy - hw);
+                                lx0 = (int)ceil(gx - hw);
+                                fz = (float)((double)lz0 - gz);
+                                fy = (float)((double)ly0 - gy);
+                                fx = (float)((double)lx0 - gx);
+                            }
                             unsigned long long acc_pts = __ballot(ok);
                             while (acc_pts) {
                                 const int s2 = __ffsll((long long)acc_pts) - 1;
                                 acc_pts &= acc_pts - 1;
                                 // s2 is uniform: scalar broadcasts
-                                const double pz = rl_f64(gz, s2), py = rl_f64(gy, s2),
-                                             px = rl_f64(gx, s2);
+                                const int pz0 = __builtin_amdgcn_readlane(lz0, s2);
+                                const int py0 = __builtin_amdgcn_readlane(ly0, s2);
+                                const int px0 = __builtin_amdgcn_readlane(lx0, s2);
                                 const float vr = rl_f32(cv.x, s2), vi = rl_f32(cv.y, s2);
-                                const int lz0 = (int)ceil(pz - hw), ly0 = (int)ceil(py - hw),
-                                          lx0 = (int)ceil(px - hw);
-                                const float wxy = es3((float)((double)(lx0 + dxl) - px) * ihw, g.beta) *
-                                                  es3((float)((double)(ly0 + dyl) - py) * ihw, g.beta);
-                                const float wzl = es3((float)((double)(lz0 + dxl) - pz) * ihw, g.beta);
-                                const int cx = lx0 + dxl - c0[2], cy = ly0 + dyl - c0[1];
+                                const float wxy = es3(((float)dxl + rl_f32(fx, s2)) * ihw, g.beta) *
+                                                  es3(((float)dyl + rl_f32(fy, s2)) * ihw, g.beta);
+                                // lane dz (< W3) holds wz[dz]
+                                const float wzl = es3(((float)dxl + rl_f32(fz, s2)) * ihw, g.beta);
+                                const int cx = px0 + dxl - c0[2], cy = py0 + dyl - c0[1];
                                 const bool inxy = cx >= 0 && cx < TX && cy >= 0 && cy < TY;
-#pragma unroll
-                                for (int dz = 0; dz < W3; ++dz) {
-                                    const float wz = rl_f32(wzl, dz);   // lane dz holds wz[dz]
-                                    const int cz = lz0 + dz - c0[0];
-                                    if (inxy && cz >= 0 && cz < TZ) {
+                                // the footprint's z steps inside the tile (uniform bounds)
+                                const int dz0 = max(0, c0[0] - pz0), dz1 = min(W3, c0[0] + TZ - pz0);
+                                for (int dz = dz0; dz < dz1; ++dz) {
+                                    const float wz = rl_f32(wzl, dz);
+                                    if (inxy) {
                                         const float wgt = wxy * wz;
-                                        float2 &a = my[tile_idx(cz, cy, cx)];
+                                        float2 &a = my[tile_idx(pz0 + dz - c0[0], cy, cx)];
                                         a.x += wgt * vr;
                                         a.y += wgt * vi;
                                     }
