@@ -52,9 +52,13 @@ __device__ __forceinline__ double rl_f64(double v, int lane) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// exp(beta (sqrt(1 - z^2) - 1)) for |z| < 1, else 0.  The hardware square
+// root (within 1 ulp; s lies in (0, 1]) instead of the correctly rounded
+// sequence: one instruction instead of about fifteen, a relative weight
+// error near 1e-6, inside the kernel's own approximation error
 __device__ __forceinline__ float es_kernel(float z, float beta) {
     const float s = 1.0f - z * z;
-    return s > 0.0f ? __expf(beta * (sqrtf(s) - 1.0f)) : 0.0f;
+    return s > 0.0f ? __expf(beta * (__builtin_amdgcn_sqrtf(s) - 1.0f)) : 0.0f;
 }
 
 // ---------------------------------------------------------------------------
