@@ -73,8 +73,12 @@ k_det_fft(const float *__restrict__ real_in, const float2 *__restrict__ cplx_in,
           float2 *__restrict__ cplx_out, float *__restrict__ real_out, int nrows, int n, int nl,
           FftPlan plan, const float2 *__restrict__ tw, float scale) {
     float2 *buf = reinterpret_cast<float2 *>(lc_smem);
-    const int row0 = blockIdx.x * nl;
-    const int nloc = min(nl, nrows - row0);
+    // blocks never straddle slices (blockIdx.y = slice, nrows rows each): a
+    // row's place in its block, which its rounding depends on, is then the
+    // same whether the slice is transformed alone or inside a stack
+    const int lrow0 = blockIdx.x * nl;
+    const int nloc = min(nl, nrows - lrow0);
+    const int64_t row0 = (int64_t)blockIdx.y * nrows + lrow0;
     const int dfw = n / 2, dinv = n / 2 + (n & 1);
     // input shift: fftshift for backproj (SIGN -1), ifftshift for forward (+1)
     const int din = SIGN < 0 ? dfw : dinv;
@@ -85,7 +89,7 @@ k_det_fft(const float *__restrict__ real_in, const float2 *__restrict__ cplx_in,
         if (l < nloc) {
             int src = k + din;
             if (src >= n) src -= n;
-            const int64_t q = (int64_t)(row0 + l) * n + src;
+            const int64_t q = (row0 + l) * n + src;
             v = SIGN < 0 ? make_float2(real_in[q], 0.f) : cplx_in[q];
         }
         buf[lds_row(l, k, n)] = v;
@@ -97,7 +101,7 @@ k_det_fft(const float *__restrict__ real_in, const float2 *__restrict__ cplx_in,
         int src = k + dout;
         if (src >= n) src -= n;
         const float2 v = buf[lds_row(l, src, n)];
-        const int64_t q = (int64_t)(row0 + l) * n + k;
+        const int64_t q = (row0 + l) * n + k;
         if (SIGN < 0) cplx_out[q] = v;
         else real_out[q] = v.x * scale;
     }
@@ -561,22 +565,22 @@ SpreadGeom spread_geom(const Geom2D &g) {
     return s;
 }
 
+// ns slices of g.nang sinogram rows each
 void det_transform(Ctx &c, const Geom2D &g, int sign, const float *real_in, const float2 *cplx_in,
-                   float2 *cplx_out, float *real_out, int nrows) {
-    const int n = (int)g.nrad;
+                   float2 *cplx_out, float *real_out, int ns) {
+    const int n = (int)g.nrad, nang = (int)g.nang, nrows = ns * nang;
     const float2 *tw = c.twiddles<float>(n);
     const float scale = 1.0f / (float)n;
     if (g.nu.det_fast && n <= NU_NT * NU_ME) {
         const int nl = std::max(1, std::min(NU_NT * NU_ME / n, 64));
         const size_t sh = sizeof(float2) * lds_round(nl * n);
+        const dim3 grid(ceil_div(nang, nl), ns);
         if (sign < 0)
-            hipLaunchKernelGGL((k_det_fft<-1, NU_NT, NU_ME>), dim3(ceil_div(nrows, nl)), dim3(NU_NT), sh,
-                               c.stream(), real_in, cplx_in, cplx_out, real_out, nrows, n, nl,
-                               g.nu.plan_det, tw, scale);
+            hipLaunchKernelGGL((k_det_fft<-1, NU_NT, NU_ME>), grid, dim3(NU_NT), sh, c.stream(), real_in,
+                               cplx_in, cplx_out, real_out, nang, n, nl, g.nu.plan_det, tw, scale);
         else
-            hipLaunchKernelGGL((k_det_fft<1, NU_NT, NU_ME>), dim3(ceil_div(nrows, nl)), dim3(NU_NT), sh,
-                               c.stream(), real_in, cplx_in, cplx_out, real_out, nrows, n, nl,
-                               g.nu.plan_det, tw, scale);
+            hipLaunchKernelGGL((k_det_fft<1, NU_NT, NU_ME>), grid, dim3(NU_NT), sh, c.stream(), real_in,
+                               cplx_in, cplx_out, real_out, nang, n, nl, g.nu.plan_det, tw, scale);
     } else {
         dim3 grid(nrows);
         if (sign < 0)
@@ -658,7 +662,7 @@ void nu_check(Ctx &c, Geom2D &g, int ns) {
 void backproj2d_dev(Ctx &c, Geom2D &g, const float *sino, float *vol, int ns) {
     nu_check(c, g, ns);
     const NuWork w = nu_work(c, g, ns);
-    det_transform(c, g, -1, sino, nullptr, w.C, nullptr, (int)(ns * g.nang));
+    det_transform(c, g, -1, sino, nullptr, w.C, nullptr, ns);
     type1(c, g, w, vol, ns);
 }
 
@@ -667,7 +671,7 @@ void forward2d_dev(Ctx &c, Geom2D &g, const float *vol, float *sino, int ns) {
     nu_check(c, g, ns);
     const NuWork w = nu_work(c, g, ns);
     type2(c, g, w, vol, ns);
-    det_transform(c, g, 1, nullptr, w.C, nullptr, sino, (int)(ns * g.nang));
+    det_transform(c, g, 1, nullptr, w.C, nullptr, sino, ns);
 }
 
 // sysmat2d(x, grid) = Re(T1(T2 x)) / nrad (src/gradient.cpp:59-69)

@@ -85,11 +85,14 @@ def test_projectors_adjoint(lc, N, nang, nrad):
     assert abs(lhs - rhs) <= 1e-5 * (abs(lhs) + abs(rhs))
 
 
-def test_projector_stack_consistency(lc):
-    # slice i of a stack == slice i alone (bitwise: per-slice work is independent)
-    announce("projector stack consistency")
+@pytest.mark.parametrize("N,nang", [(80, 40), (64, 90), (128, 180)])
+def test_projector_stack_consistency(lc, N, nang):
+    # slice i of a stack == slice i alone (bitwise: per-slice work is independent).
+    # N = 80 takes the direct detector DFT; 64 and 128 the LDS FFT, whose
+    # blocks hold several sinogram rows (90 and 180 rows are not multiples of
+    # the rows per block, so a stack would shift later slices' rows)
+    announce(f"projector stack consistency N={N} nang={nang}")
     rng = np.random.default_rng(9)
-    N, nang = 80, 40
     g = lc.PolarGrid2D(uniform_theta(nang), N)
     vol = rng.random((4, N, N)).astype(np.float32)
     sino = lc.forward2d(vol, g)
