@@ -140,8 +140,8 @@ int lc_geom2d_info(lc_geom2d g, int64_t *info) {
         info[0] = g->g.nang;
         info[1] = g->g.nrad;
         info[2] = g->g.N;
-        info[3] = g->g.M;
-        info[4] = g->g.H;
+        info[3] = g->g.Mr;
+        info[4] = g->g.Mr / 2 + 1;
     });
 }
 
@@ -389,7 +389,7 @@ int lc_geom3d_info(lc_geom3d g, int64_t *info) {
     return guard([&] {
         LC_REQUIRE(g && info, "lc_geom3d_info: NULL argument");
         const Geom3D &q = g->g;
-        const int64_t v[9] = {q.nproj, q.nrows, q.ncols, q.n1, q.n2, q.n3, q.M1, q.M, q.M};
+        const int64_t v[9] = {q.nproj, q.nrows, q.ncols, q.n1, q.n2, q.n3, q.M1, q.Mr, q.Mr};
         for (int i = 0; i < 9; ++i) info[i] = v[i];
     });
 }
@@ -799,7 +799,7 @@ int lc_get_tuning2d(lc_ctx ctx, int64_t image_n, int32_t *variant) {
         auto it = ctx->c.toep_tuning.find((int)image_n);
         ToepVariant v = it != ctx->c.toep_tuning.end()
                             ? it->second
-                            : toep_default_variant((int)next_fast_dim(2 * image_n - 1));
+                            : toep_default_variant((int)toep_conv_size(image_n));
         variant[0] = v.row;
         variant[1] = v.col;
         variant[2] = v.B;

@@ -525,7 +525,7 @@ void Batch2D::build() {
         auto k = std::make_unique<ToepClass>();
         const Geom3D *g3 = probs_[kv.second.front()].g3;
         k->N = kv.first;
-        k->M = g3 ? (int)g3->M : (int)next_fast_dim(2 * k->N - 1);
+        k->M = g3 ? (int)g3->M : (int)toep_conv_size(k->N);
         k->H = k->M / 2 + 1;
         LC_REQUIRE(fft_make_plan(k->M, k->plan), "batch: PSF size not 2^p 3^q");
         if (f64_) k->tw64 = c_.twiddles<double>(k->M);

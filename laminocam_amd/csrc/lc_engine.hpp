@@ -30,6 +30,8 @@ struct Nufft2D {
 // ---------------------------------------------------------------------------
 struct Geom2D {
     int64_t nang = 0, nrad = 0, N = 0, M = 0, H = 0;
+    int64_t Mr = 0;    // the reference's PSF size next_fast_dim(2N - 1) (kernel_hat, psf_size);
+                       // M = toep_conv_size(N) is the size the Toeplitz apply convolves at
     std::vector<double> theta;
     DBuf d_theta;      // double[nang]
     DBuf khatT;        // float[H][M]: real spectrum of the centred PSF / (M^2 N)
@@ -48,6 +50,9 @@ void sysmat2d_nufft_dev(Ctx &c, Geom2D &g, const float *x, float *y, int ns);
 
 // next_fast_dim of include/toeplitz2d.h:48-57
 int64_t next_fast_dim(int64_t n);
+// circular-convolution size of the Toeplitz apply for an n-point axis (see
+// lc_toeplitz.hip); the reference's PSF size stays next_fast_dim(2 n - 1)
+int64_t toep_conv_size(int64_t n);
 
 // D_n(t) = sin(pi t) / sin(pi t / n) evaluated at t = k n + d as
 // (-1)^{k(n-1)} sin(pi d) / sin(pi d / n), exact at the removable zeros
@@ -94,7 +99,8 @@ struct Geom3D {
     int64_t nproj = 0, nrows = 0, ncols = 0;
     int64_t n1 = 0, n2 = 0, n3 = 0;   // recon dims
     int64_t N = 0;                    // n2 (== n3 for the Toeplitz operator)
-    int64_t M1 = 0, M = 0, H = 0;     // PSF dims {M1, M, M}; H = M/2 + 1
+    int64_t M1 = 0, M = 0, H = 0;     // convolution dims {M1, M, M}; H = M/2 + 1
+    int64_t Mr = 0;                   // the reference's in-plane PSF size next_fast_dim(2N - 1)
     std::vector<double> rot;          // per projection: u (3) then v (3)
     DBuf d_rot;                       // double[nproj][6]
     DBuf khat3;                       // float[M][H][M1]: real spectrum of the centred PSF / (M1 M^2 N^2)

@@ -58,7 +58,8 @@ void geom3d_init(Ctx &c, Geom3D &g, const double *theta, const double *gamma, co
     g.n3 = recon_dims[2];
     g.N = recon_dims[1];
     g.M1 = next_fast_dim(2 * g.n1 - 1);
-    g.M = next_fast_dim(2 * g.N - 1);
+    g.Mr = next_fast_dim(2 * g.N - 1);
+    g.M = toep_conv_size(g.N);
     g.H = g.M / 2 + 1;
     LC_REQUIRE(g.M <= 4096 && g.M1 <= 4096, "3-D geometry: volume too large");
     g.rot.resize(6 * nproj);
@@ -308,7 +309,8 @@ k_khat3_export(const double2 *P, cpx<OT> *out, int M1, int M, int H, int B, FftP
 template <typename OT> void geom3d_kernel_hat(Ctx &c, Geom3D &g, OT *out_host) {
     if (g.n2 != g.n3)
         throw Error(LC_EUNSUP, "3-D Toeplitz operator: recon dims n2 != n3 are not supported");
-    const int M1 = (int)g.M1, M = (int)g.M, H = (int)g.H;
+    // the reference's layout and size, independent of the convolution size
+    const int M1 = (int)g.M1, M = (int)g.Mr, H = M / 2 + 1;
     FftPlan plan, plan1;
     LC_REQUIRE(fft_make_plan(M, plan) && fft_make_plan(M1, plan1), "3-D PSF size is not 2^p 3^q");
     double2 *P = reinterpret_cast<double2 *>(c.scratch(0, sizeof(double2) * (size_t)M1 * M * M));

@@ -34,6 +34,20 @@ int64_t next_fast_dim(int64_t n) {
     return best;
 }
 
+// Any circular-convolution size >= 2n - 1 gives the same linear convolution
+// after the crop: the PSF values at |m| <= n - 1 do not depend on it, and the
+// unnormalised FFT round trip's factor is divided out with it.  The FFT
+// lengths compiled into the Toeplitz kernels run much faster than runtime
+// plans (1458 = 2 3^6 takes seven radix stages, 1536 four), so take the
+// smallest compiled length within 25% of next_fast_dim(2n - 1), else that.
+int64_t toep_conv_size(int64_t n) {
+    const int64_t m = next_fast_dim(2 * n - 1);
+    static const int64_t compiled[] = {256, 384, 512, 768, 1024, 1536, 2048};
+    for (int64_t L : compiled)
+        if (L >= m) return L * 4 <= m * 5 ? L : m;
+    return m;
+}
+
 // ----------------------------------------------------------------------------
 // FP64 PSF: P(m) = sum_i D_nrad(m_x cos th_i + m_y sin th_i),
 // D_n(t) = sin(pi t)/sin(pi t/n) (the radial sum of exp(i r_k t) over the
@@ -192,7 +206,8 @@ void geom2d_init(Ctx &c, Geom2D &g, const double *theta, int64_t nang, int64_t n
     g.nang = nang;
     g.nrad = nrad;
     g.N = image_n;
-    g.M = next_fast_dim(2 * image_n - 1);
+    g.Mr = next_fast_dim(2 * image_n - 1);
+    g.M = toep_conv_size(image_n);
     g.H = g.M / 2 + 1;
     LC_REQUIRE(g.M <= 4096, "geometry: image size above 2048 is not supported");
     g.theta.assign(theta, theta + nang);
@@ -242,7 +257,8 @@ void geom2d_build_psf(Ctx &c, Geom2D &g, bool f64) {
 }
 
 template <typename OT> void geom2d_kernel_hat(Ctx &c, Geom2D &g, OT *out_host) {
-    const int M = (int)g.M, H = (int)g.H;
+    // the reference's layout and size, independent of the convolution size
+    const int M = (int)g.Mr, H = M / 2 + 1;
     FftPlan plan;
     LC_REQUIRE(fft_make_plan(M, plan), "PSF size is not 2^p 3^q");
     double2 *P = reinterpret_cast<double2 *>(c.scratch(0, sizeof(double2) * (size_t)M * M));
