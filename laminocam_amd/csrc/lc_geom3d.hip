@@ -23,6 +23,8 @@
 // per k2.
 #include <cmath>
 
+#include <type_traits>
+
 #include "lc_engine.hpp"
 
 namespace lc {
@@ -363,8 +365,10 @@ __device__ __forceinline__ bool prob_active3(const CgState *st, int prob) {
     return st == nullptr || st[prob].active != 0;
 }
 
-// P2f: U block (N x B, slice i1) -> FFT_M down columns -> W[kb][k2][i1][c]
-template <typename R, int NT, int MAXE>
+// P2f: U block (N x B, slice i1) -> FFT_M down columns -> W[kb][k2][i1][c].
+// LF > 0: M = LF at compile time (unrolled stages, B = 1 << g.lsh lines), with
+// the zero rows N.. of the input pruned from the first stage.
+template <typename R, int NT, int MAXE, int LF>
 __global__ void __launch_bounds__(NT)
 k_t3_cols_fwd(const SliceRef *__restrict__ sl, const CgState *__restrict__ st,
               const cpx<R> *__restrict__ U, cpx<R> *__restrict__ W, ToepGeom g, int n1, FftPlan plan,
@@ -383,7 +387,8 @@ k_t3_cols_fwd(const SliceRef *__restrict__ sl, const CgState *__restrict__ st,
         buf[lds_col(c, i, B)] = i < N ? blk[idx] : cmake<C>(0, 0);
     }
     __syncthreads();
-    fft_lds<R, -1, NT, MAXE, true>(buf, B, 0, plan, tw);
+    if constexpr (LF > 0) fft_lds_ct<R, LF, -1, NT, MAXE, true, -1, true, false>(buf, B, g.lsh, tw);
+    else fft_lds<R, -1, NT, MAXE, true>(buf, B, 0, plan, tw);
     C *w = W + (int64_t)kb * M * n1 * B;
     for (int idx = threadIdx.x; idx < M * B; idx += NT) {
         const int k2 = idx / B, c = idx - k2 * B;
@@ -434,7 +439,8 @@ k_t3_slices(const SliceRef *__restrict__ sl, const CgState *__restrict__ st, cpx
 }
 
 // P2i: W[kb][:][i1][:] -> inverse FFT_M down columns -> keep N rows -> U block
-template <typename R, int NT, int MAXE>
+// (LF > 0: compiled length; the unused outputs N.. pruned from the last stage)
+template <typename R, int NT, int MAXE, int LF>
 __global__ void __launch_bounds__(NT)
 k_t3_cols_inv(const SliceRef *__restrict__ sl, const CgState *__restrict__ st,
               const cpx<R> *__restrict__ W, cpx<R> *__restrict__ U, ToepGeom g, int n1, FftPlan plan,
@@ -453,7 +459,8 @@ k_t3_cols_inv(const SliceRef *__restrict__ sl, const CgState *__restrict__ st,
         buf[lds_col(c, k2, B)] = w[((int64_t)k2 * n1 + i1) * B + c];
     }
     __syncthreads();
-    fft_lds<R, 1, NT, MAXE, true>(buf, B, 0, plan, tw);
+    if constexpr (LF > 0) fft_lds_ct<R, LF, 1, NT, MAXE, true, -1, false, true>(buf, B, g.lsh, tw);
+    else fft_lds<R, 1, NT, MAXE, true>(buf, B, 0, plan, tw);
     C *blk = U + s.uoff + (int64_t)kb * N * B;
     for (int idx = threadIdx.x; idx < N * B; idx += NT) {
         const int i = idx / B, c = idx - i * B;
@@ -478,17 +485,37 @@ template <typename R> void toep3d_mid(Ctx &c, const ToepClass &k, const CgState 
     C *W = k.W.as<C>();
     const C *tw = toep_tw<R>(k);
     const size_t shc = sizeof(C) * lds_round(k.B * k.M);
-    hipLaunchKernelGGL((k_t3_cols_fwd<R, T3_NT, ME>), dim3(k.nblk, k.n1), dim3(T3_NT), shc,
-                       c.stream(), sl, st, U, W, k.g, k.n1, k.plan, tw);
-    LC_HIP(hipGetLastError());
+    // FP32 column FFTs at the Toeplitz kernels' compiled lengths, runtime plans otherwise
+    const int lf = f64 ? 0 : k.M;
+    auto cols = [&](auto dir) {
+        constexpr bool FWD = decltype(dir)::value;
+#define LC_T3_COLS(L)                                                                             \
+    if constexpr (FWD)                                                                            \
+        hipLaunchKernelGGL((k_t3_cols_fwd<R, T3_NT, ME, L>), dim3(k.nblk, k.n1), dim3(T3_NT), shc, \
+                           c.stream(), sl, st, U, W, k.g, k.n1, k.plan, tw);                      \
+    else                                                                                          \
+        hipLaunchKernelGGL((k_t3_cols_inv<R, T3_NT, ME, L>), dim3(k.nblk, k.n1), dim3(T3_NT), shc, \
+                           c.stream(), sl, st, W, U, k.g, k.n1, k.plan, tw)
+        switch (lf) {
+        case 256: LC_T3_COLS(256); break;
+        case 384: LC_T3_COLS(384); break;
+        case 512: LC_T3_COLS(512); break;
+        case 768: LC_T3_COLS(768); break;
+        case 1024: LC_T3_COLS(1024); break;
+        case 1536: LC_T3_COLS(1536); break;
+        case 2048: LC_T3_COLS(2048); break;
+        default: LC_T3_COLS(0); break;
+        }
+#undef LC_T3_COLS
+        LC_HIP(hipGetLastError());
+    };
+    cols(std::true_type{});
     const size_t shb = sizeof(C) * lds_round(k.kb2 * k.B * k.M1);
     hipLaunchKernelGGL((k_t3_slices<R, T3B_NT, T3B_ME>), dim3(k.nblk, ceil_div(k.M, k.kb2)),
                        dim3(T3B_NT), shb, c.stream(), sl, st, W, t3_khat<R>(*k.g3), k.g, k.n1,
                        k.M1, k.kb2, k.plan1, t3_tw1<R>(k));
     LC_HIP(hipGetLastError());
-    hipLaunchKernelGGL((k_t3_cols_inv<R, T3_NT, ME>), dim3(k.nblk, k.n1), dim3(T3_NT), shc,
-                       c.stream(), sl, st, W, U, k.g, k.n1, k.plan, tw);
-    LC_HIP(hipGetLastError());
+    cols(std::false_type{});
 }
 template void toep3d_mid<float>(Ctx &, const ToepClass &, const CgState *, float2 *);
 template void toep3d_mid<double>(Ctx &, const ToepClass &, const CgState *, double2 *);
