@@ -438,6 +438,57 @@ k_t3_slices(const SliceRef *__restrict__ sl, const CgState *__restrict__ st, cpx
     }
 }
 
+// P2b with a compiled slice-axis length L1 (a power of two): NL = NT MAXE / L1
+// lines per block (kb2 = NL / B rows k2), only the n1 <= (L1 + 1) / 2 leading
+// rows loaded, and the forward FFT, the khat3 multiply and the inverse FFT run
+// as one fused convolution (fft_conv_ct).  grid (nblk, ceil(M / kb2))
+template <typename R, int NT, int MAXE, int L1>
+__global__ void __launch_bounds__(NT)
+k_t3_slices_ct(const SliceRef *__restrict__ sl, const CgState *__restrict__ st, cpx<R> *__restrict__ W,
+               const R *__restrict__ khat3, ToepGeom g, int n1, int kb2,
+               const cpx<R> *__restrict__ tw1) {
+    using C = cpx<R>;
+    using CM = ConvMid<R, L1, MAXE>;
+    constexpr int NL = NT * MAXE / L1;
+    constexpr int LSH = NL == 1 ? 0 : NL == 2 ? 1 : NL == 4 ? 2 : NL == 8 ? 3 : NL == 16 ? 4 : NL == 32 ? 5
+                      : NL == 64 ? 6 : NL == 128 ? 7 : 8;
+    static_assert((1 << LSH) == NL, "lines per block must be a power of two up to 256");
+    constexpr int HALF = (L1 + 1) / 2;
+    if (!prob_active3(st, sl[0].prob)) return;
+    const int M = g.M, H = g.H, B = g.B, bsh = g.lsh;
+    const int kb = blockIdx.x, k20 = blockIdx.y * kb2;
+    const int nk2 = min(kb2, M - k20);
+    C *buf = reinterpret_cast<C *>(lc_smem);
+    C *w = W + ((int64_t)kb * M + k20) * n1 * B;   // tile [j][i1][c], contiguous
+    // spectrum values of this thread's middle-stage butterflies (line, index)
+    R kr[CM::MAXB][CM::R];
+#pragma unroll
+    for (int t = 0; t < CM::MAXB; ++t) {
+        int line = 0, jm = 0;
+        const bool has = conv_mid_index<R, L1, NT, MAXE, LSH>(t, NL, LSH, line, jm);
+        const int j = line >> bsh, c = line & (B - 1);
+        const int k3 = kb * B + c;
+        const bool ok = has && j < nk2 && k3 < H;
+        const R *kc = khat3 + ((int64_t)(ok ? k20 + j : 0) * H + (ok ? k3 : 0)) * L1;
+#pragma unroll
+        for (int r = 0; r < CM::R; ++r) kr[t][r] = ok ? kc[jm + r * CM::LR] : (R)0;
+    }
+    for (int idx = threadIdx.x; idx < NL * HALF; idx += NT) {
+        const int e = idx >> LSH, line = idx & (NL - 1);
+        const int j = line >> bsh, c = line & (B - 1);
+        C v = cmake<C>(0, 0);
+        if (j < nk2 && e < n1) v = w[((int64_t)j * n1 + e) * B + c];
+        buf[swz((e << LSH) + line)] = v;
+    }
+    __syncthreads();
+    fft_conv_ct<R, L1, NT, MAXE, LSH>(buf, NL, LSH, tw1, kr);
+    for (int idx = threadIdx.x; idx < NL * n1; idx += NT) {
+        const int e = idx >> LSH, line = idx & (NL - 1);
+        const int j = line >> bsh, c = line & (B - 1);
+        if (j < nk2) w[((int64_t)j * n1 + e) * B + c] = buf[swz((e << LSH) + line)];
+    }
+}
+
 // P2i: W[kb][:][i1][:] -> inverse FFT_M down columns -> keep N rows -> U block
 // (LF > 0: compiled length; the unused outputs N.. pruned from the last stage)
 template <typename R, int NT, int MAXE, int LF>
@@ -511,9 +562,22 @@ template <typename R> void toep3d_mid(Ctx &c, const ToepClass &k, const CgState 
     };
     cols(std::true_type{});
     const size_t shb = sizeof(C) * lds_round(k.kb2 * k.B * k.M1);
-    hipLaunchKernelGGL((k_t3_slices<R, T3B_NT, T3B_ME>), dim3(k.nblk, ceil_div(k.M, k.kb2)),
-                       dim3(T3B_NT), shb, c.stream(), sl, st, W, t3_khat<R>(*k.g3), k.g, k.n1,
-                       k.M1, k.kb2, k.plan1, t3_tw1<R>(k));
+    const dim3 gb(k.nblk, ceil_div(k.M, k.kb2));
+    // FP32 slice-axis lengths 16..128 with a full block of lines: compiled,
+    // fused convolution; anything else: runtime plan
+    const bool ct1 = !f64 && k.kb2 * k.B * k.M1 == T3B_NT * T3B_ME &&
+                     (k.M1 == 16 || k.M1 == 32 || k.M1 == 64 || k.M1 == 128);
+#define LC_T3_SL(L1)                                                                              \
+    hipLaunchKernelGGL((k_t3_slices_ct<R, T3B_NT, T3B_ME, L1>), gb, dim3(T3B_NT), shb, c.stream(),  \
+                       sl, st, W, t3_khat<R>(*k.g3), k.g, k.n1, k.kb2, t3_tw1<R>(k))
+    if (ct1 && k.M1 == 16) LC_T3_SL(16);
+    else if (ct1 && k.M1 == 32) LC_T3_SL(32);
+    else if (ct1 && k.M1 == 64) LC_T3_SL(64);
+    else if (ct1 && k.M1 == 128) LC_T3_SL(128);
+    else
+        hipLaunchKernelGGL((k_t3_slices<R, T3B_NT, T3B_ME>), gb, dim3(T3B_NT), shb, c.stream(), sl, st, W,
+                           t3_khat<R>(*k.g3), k.g, k.n1, k.M1, k.kb2, k.plan1, t3_tw1<R>(k));
+#undef LC_T3_SL
     LC_HIP(hipGetLastError());
     cols(std::false_type{});
 }
