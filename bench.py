@@ -353,6 +353,47 @@ def lamino_leg(args, lc, synth, ctx):
     return out
 
 
+def nufft_probe(reps=2):
+    """Back-projections for counter runs of the spreads: 8 slices of 1024^2
+    from 720 angles x 1024 bins (2-D), and a 32 x 511^2 volume from 720
+    projections of 128 x 511 at 30 degrees tilt (3-D); device-resident
+    inputs, HIP-event times printed as one JSON line."""
+    import ctypes as C
+    import laminocam_amd as lc
+    from laminocam_amd import synth
+    ctx = lc.default_context()
+    hip = C.CDLL("libamdhip64.so")
+    out = {}
+
+    def timed(name, fn):
+        fn()
+        ctx.sync()
+        e0, e1 = C.c_void_p(), C.c_void_p()
+        hip.hipEventCreate(C.byref(e0))
+        hip.hipEventCreate(C.byref(e1))
+        hip.hipEventRecord(e0, None)
+        for _ in range(reps):
+            fn()
+        ctx.sync()
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = C.c_float()
+        hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+        out[name] = round(ms.value / reps, 3)
+
+    N, nang = 1024, 720
+    g2 = lc.PolarGrid2D(synth.angles(nang), N)
+    s2 = np.random.default_rng(0).standard_normal((8, nang, N)).astype(np.float32)
+    timed("backproj2d_8x1024_ms", lambda: lc.backproj2d(s2, g2, (8, N, N)))
+    dims, nproj, nrows = (32, 511, 511), 720, 128
+    th = np.arange(nproj) * (2 * np.pi / nproj)
+    g3 = lc.PolarGrid(th, np.full(nproj, -np.pi / 6), np.zeros(nproj), nrows, dims[2])
+    p3 = np.random.default_rng(2).standard_normal(g3.dims()).astype(np.float32)
+    timed("backproj3d_32x511_ms", lambda: lc.backproj(p3, g3, dims))
+    out["note"] = "host-inclusive: each call copies its input in and its result out"
+    print(json.dumps({"nufft_probe": out}), flush=True)
+
+
 def pmc_probe(args):
     """Short run for rocprofv3 --pmc: a device copy of known size (calibration)
     and a few batched Toeplitz applies of the benchmark workload."""
@@ -421,6 +462,8 @@ def main():
     ap.add_argument("--lamino-iters", type=int, default=100)
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the served stream and print its JSON (profiling)")
+    ap.add_argument("--nufft-probe", action="store_true",
+                    help="only the 2-D and 3-D back-projections (for counter runs of the spreads)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only run a calibration copy and a few Toeplitz applies (for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -433,6 +476,9 @@ def main():
 
     if args.pmc_probe:
         pmc_probe(args)
+        return
+    if args.nufft_probe:
+        nufft_probe()
         return
 
     import laminocam_amd as lc
