@@ -314,7 +314,8 @@ void Server::run_batch(Lane &lane, std::vector<ServerRequest *> &batch) {
             LC_HIP(hipMemcpyAsync(x, dst + off, sizeof(float) * r->x0.size(),
                                   hipMemcpyDeviceToDevice, c.stream()));
             off += pad(r->x0.size());
-        } else {
+        } else if (any_x0) {
+            // (with no initial guess in the batch the solver clears x itself)
             LC_HIP(hipMemsetAsync(x, 0, sizeof(float) * (size_t)b.size((int)i), c.stream()));
         }
         // an initial guess staged after the sinogram breaks staging contiguity
@@ -335,7 +336,7 @@ void Server::run_batch(Lane &lane, std::vector<ServerRequest *> &batch) {
             mu.push_back(r->mu);
         }
         b.split_bregman(la, mu, h.outer, h.inner, h.tol, h.xtol, (h.flags & LC_TV_ZERO_PAD) ? 1 : 0,
-                        (h.flags & LC_TV_ANISO) != 0, true, false);
+                        (h.flags & LC_TV_ANISO) != 0, true, false, !any_x0);
     }
     c.sync();
     const double t3 = now_s();
