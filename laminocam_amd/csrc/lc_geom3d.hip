@@ -144,9 +144,16 @@ k_psf3d_gen(double2 *P, const double *rot, int nproj, int nrows, int ncols, int 
                     r = a2r * sv.x - a2i * sv.y; im = a2r * sv.y + a2i * sv.x; a2r = r; a2i = im;
                     r = b2r * sv.z - b2i * sv.w; im = b2r * sv.w + b2i * sv.z; b2r = r; b2i = im;
                 }
-                const double d1 = (fabs(b1i) > 1e-3) ? a1i / b1i : dirichlet_d(t1, ncols, inv_c);
-                const double d2 = (fabs(b2i) > 1e-3) ? a2i / b2i : dirichlet_d(t2, nrows, inv_r);
-                acc[j] += d1 * d2;
+                // D_ncols(t1) D_nrows(t2) = (a1i a2i) / (b1i b2i): one FP64
+                // division per term away from the removable zeros
+                const bool f1 = fabs(b1i) > 1e-3, f2 = fabs(b2i) > 1e-3;
+                if (f1 && f2) {
+                    acc[j] += (a1i * a2i) / (b1i * b2i);
+                } else {
+                    const double d1 = f1 ? a1i / b1i : dirichlet_d(t1, ncols, inv_c);
+                    const double d2 = f2 ? a2i / b2i : dirichlet_d(t2, nrows, inv_r);
+                    acc[j] += d1 * d2;
+                }
             }
         }
     }
