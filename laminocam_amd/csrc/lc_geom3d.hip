@@ -76,12 +76,12 @@ void geom3d_init(Ctx &c, Geom3D &g, const double *theta, const double *gamma, co
 
 // ---------------------------------------------------------------------------
 // FP64 PSF on the centred periodic index space (index a <-> m = a for
-// a < (L+1)/2, a - L otherwise).  One thread per (iz, iy, run of 8 ix) for
+// a < (L+1)/2, a - L otherwise).  One thread per (iz, iy, run of P3_RUN ix) for
 // iz <= M1/2; iz strictly inside (0, M1/2) also writes P(-m) = P(m).
 // Along a run both Dirichlet arguments advance by fixed steps, so their
 // numerator and denominator phases advance by fixed rotations.
 // ---------------------------------------------------------------------------
-constexpr int P3_RUN = 8;
+constexpr int P3_RUN = 16;
 constexpr int P3_CHUNK = 256;
 
 __global__ void __launch_bounds__(256)
