@@ -308,8 +308,9 @@ int lc_prepare_sinogram_f64(lc_ctx ctx, int32_t ndatasets, const double *const *
  * CG or Split-Bregman (max_iters outer x inner_iters) from x0 = 0 on the whole
  * stack.  recon receives {recon_dims}.  recon_dims[0] <= nslices and
  * recon_dims[1] == recon_dims[2] (LC_EINVAL otherwise).  flags: LC_GRAPH,
- * LC_TV_ZERO_PAD, LC_TV_ANISO, LC_DEVICE_PTRS.  The _f64 form solves in FP64;
- * its back-projection runs in FP32, as every projector here. */
+ * LC_TV_ZERO_PAD, LC_TV_ANISO, LC_DEVICE_PTRS (sino and recon are device
+ * pointers; theta stays on the host).  The _f64 form solves in FP64; its
+ * back-projection runs in FP32, as every projector here. */
 int lc_mbir2d(lc_ctx ctx, const float *sino, int64_t nslices, int64_t nang, int64_t ncols,
               const double *theta, const lc_recon_params *params, float *recon, lc_report *rep,
               uint32_t flags);
@@ -328,7 +329,9 @@ int lc_mbir2d_f64(lc_ctx ctx, const double *sino, int64_t nslices, int64_t nang,
  * rhs = backproj(..., shifts), CG or Split-Bregman from x0 = 0, symmetric crop
  * to recon_dims into recon.  padded_dims (may be NULL) receives {volume n1,
  * n2, n3, detector rows, cols}.  A pad factor that makes the padded volume
- * smaller than recon_dims is LC_EINVAL.  flags as lc_mbir2d. */
+ * smaller than recon_dims is LC_EINVAL.  flags as lc_mbir2d; with
+ * LC_DEVICE_PTRS, projs[d], shifts and recon are device pointers (theta,
+ * gamma, beta stay on the host). */
 int lc_mbir3d(lc_ctx ctx, int32_t ndatasets, const float *const *projs, const int64_t *nang,
               int64_t nrows, int64_t ncols, const double *theta, const double *gamma,
               const double *beta, const double *shifts, const lc_recon_params *params,
