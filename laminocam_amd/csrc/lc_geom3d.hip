@@ -389,7 +389,9 @@ k_t3_cols_fwd(const SliceRef *__restrict__ sl, const CgState *__restrict__ st,
     const int N = g.N, M = g.M, B = g.B;
     C *buf = reinterpret_cast<C *>(lc_smem);
     const C *blk = U + s.uoff + (int64_t)kb * N * B;
-    for (int idx = threadIdx.x; idx < M * B; idx += NT) {
+    // a compiled length's pruned first stage never reads rows >= (M + 1) / 2
+    const int rows = LF > 0 ? (LF + 1) / 2 : M;
+    for (int idx = threadIdx.x; idx < rows * B; idx += NT) {
         const int i = idx / B, c = idx - i * B;
         buf[lds_col(c, i, B)] = i < N ? blk[idx] : cmake<C>(0, 0);
     }
