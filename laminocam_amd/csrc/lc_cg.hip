@@ -27,6 +27,10 @@
 
 namespace lc {
 
+// threads per block of the CG vector update (CHUNK / (4 CG_UPD_NT) four-wide
+// loads of each vector per thread)
+constexpr int CG_UPD_NT = 512;
+
 // ----------------------------------------------------------------------------
 // reductions
 // ----------------------------------------------------------------------------
@@ -134,12 +138,17 @@ __global__ void k_cg_fin1(const ProbDev *__restrict__ pd, CgState *st, const dou
     }
 }
 
-// x += alpha p; r -= alpha Ap; partials <r,r>, ||alpha p||^2, ||x||^2
+// x += alpha p; r -= alpha Ap; partials <r,r>, ||alpha p||^2, ||x||^2.
+// Chunks start 4-element aligned (problems are 256-byte aligned, chunks
+// CHUNK elements long): whole chunks issue all their 4-wide loads before any
+// arithmetic; a problem's last, shorter chunk takes 4-wide steps and a scalar
+// tail.
 template <typename R, int NT>
 __global__ void __launch_bounds__(NT)
 k_cg_update(const ChunkRef *__restrict__ ch, const CgState *__restrict__ st,
             R *__restrict__ x, R *__restrict__ r, const R *__restrict__ p,
             const R *__restrict__ ap, double *__restrict__ cpart) {
+    using V = vec4<R>;
     const ChunkRef c = ch[blockIdx.x];
     const CgState s = st[c.prob];
     if (!s.active) return;
@@ -148,15 +157,58 @@ k_cg_update(const ChunkRef *__restrict__ ch, const CgState *__restrict__ st,
     // CG then amplifies)
     const double alpha = s.alpha;
     double rr = 0.0, ss = 0.0, xx = 0.0;
-    for (int64_t i = c.begin + threadIdx.x; i < c.end; i += NT) {
-        const R step = (R)((double)p[i] * alpha);
-        const R xi = x[i] + step;
-        const R ri = r[i] - (R)((double)ap[i] * alpha);
-        x[i] = xi;
-        r[i] = ri;
-        rr += (double)ri * ri;
+    auto upd = [&](R pv, R apv, R &xv, R &rv) {
+        const R step = (R)((double)pv * alpha);
+        xv = xv + step;
+        rv = rv - (R)((double)apv * alpha);
+        rr += (double)rv * rv;
         ss += (double)step * step;
-        xx += (double)xi * xi;
+        xx += (double)xv * xv;
+    };
+    auto upd4 = [&](const V &pv, const V &av, V &xv, V &rv) {
+        upd(pv.x, av.x, xv.x, rv.x);
+        upd(pv.y, av.y, xv.y, rv.y);
+        upd(pv.z, av.z, xv.z, rv.z);
+        upd(pv.w, av.w, xv.w, rv.w);
+    };
+    V *x4 = reinterpret_cast<V *>(x + c.begin);
+    V *r4 = reinterpret_cast<V *>(r + c.begin);
+    const V *p4 = reinterpret_cast<const V *>(p + c.begin);
+    const V *a4 = reinterpret_cast<const V *>(ap + c.begin);
+    const int64_t n4 = (c.end - c.begin) >> 2;
+    constexpr int U = CHUNK / 4 / NT;
+    static_assert(U >= 1 && U * 4 * NT == CHUNK, "CHUNK must be a multiple of 4 NT");
+    if (n4 == CHUNK / 4) {
+        V pv[U], av[U], xv[U], rv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = threadIdx.x + u * NT;
+            pv[u] = p4[i];
+            av[u] = a4[i];
+            xv[u] = x4[i];
+            rv[u] = r4[i];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            upd4(pv[u], av[u], xv[u], rv[u]);
+            const int i = threadIdx.x + u * NT;
+            x4[i] = xv[u];
+            r4[i] = rv[u];
+        }
+    } else {
+        for (int64_t i = threadIdx.x; i < n4; i += NT) {
+            const V pv = p4[i], av = a4[i];
+            V xv = x4[i], rv = r4[i];
+            upd4(pv, av, xv, rv);
+            x4[i] = xv;
+            r4[i] = rv;
+        }
+        for (int64_t i = c.begin + 4 * n4 + threadIdx.x; i < c.end; i += NT) {
+            R xv = x[i], rv = r[i];
+            upd(p[i], ap[i], xv, rv);
+            x[i] = xv;
+            r[i] = rv;
+        }
     }
     __shared__ double sh[NT / 64];
     const double a = block_sum<NT>(rr, sh);
@@ -759,7 +811,7 @@ std::vector<Batch2D::PassTiming> Batch2D::profile_passes(int reps) {
             LC_HIP(hipEventRecord(u0, c_.stream()));
             LC_ETYPE([&](auto z) {
                 using R = decltype(z);
-                hipLaunchKernelGGL((k_cg_update<R, 256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
+                hipLaunchKernelGGL((k_cg_update<R, CG_UPD_NT>), dim3(nchunk_), dim3(CG_UPD_NT), 0, c_.stream(),
                                    dchunk_.as<ChunkRef>(), st, xs.as<R>(), R_.as<R>(), P_.as<R>(),
                                    AP_.as<R>(), cpartial_.as<double>());
             });
@@ -889,7 +941,7 @@ template <typename R> void Batch2D::cg_iteration_t(bool host_sync, bool sb) {
                        st, partial_.as<double>(), cpartial_.as<double>(), sb ? 1 : 0);
     LC_HIP(hipGetLastError());
     hs();
-    hipLaunchKernelGGL((k_cg_update<R, 256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
+    hipLaunchKernelGGL((k_cg_update<R, CG_UPD_NT>), dim3(nchunk_), dim3(CG_UPD_NT), 0, c_.stream(),
                        dchunk_.as<ChunkRef>(), st, X, Rv, P, AP, cpartial_.as<double>());
     LC_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_cg_fin2, dim3(nprob()), dim3(64), 0, c_.stream(), dprob_.as<ProbDev>(),
