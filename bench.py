@@ -212,7 +212,7 @@ def served_stream(args, lc, synth, ctx, frac3d=None, rate=None):
             else dict(solver="cg", max_iter=args.iters)
         reqs.append((psf, sino, kw, slices, is3, sb))
     with lc.Server(hbm_budget=int(args.serve_budget_gb * (1 << 30)), lanes=args.serve_lanes,
-                   ctx=ctx) as srv:
+                   max_batch=args.serve_max_batch, ctx=ctx) as srv:
         # warm the lanes' kernels and scratch on one request of each kind
         for psf, sino in [(geo2[0], pool2[0])] + ([(geo3[0], pool3[0])] if geo3 else []):
             srv.result(srv.submit(psf, sino=sino, max_iter=2), timeout=120.0)
@@ -225,7 +225,7 @@ def served_stream(args, lc, synth, ctx, frac3d=None, rate=None):
                 d = arrive[i] - time.perf_counter()
                 if d > 0:
                     time.sleep(d)
-                tickets[i] = srv.submit(psf, sino=sino, tol=args.tol, xtol=args.xtol, **kw)
+                tickets[i] = srv.submit(psf, sino=sino, tol=args.tol, xtol=args.xtol, borrow=True, **kw)
 
         th = threading.Thread(target=submitter)
         th.start()
@@ -237,10 +237,11 @@ def served_stream(args, lc, synth, ctx, frac3d=None, rate=None):
         st = srv.stats()
         # capacity: the same requests submitted at once, timed to the last result
         t_b0 = time.perf_counter()
-        tk = [srv.submit(psf, sino=sino, tol=args.tol, xtol=args.xtol, **kw)
+        tk = [srv.submit(psf, sino=sino, tol=args.tol, xtol=args.xtol, borrow=True, **kw)
               for psf, sino, kw, _, _, _ in reqs]
         burst = [srv.result(t, timeout=600.0)[2] for t in tk]
         t_b = time.perf_counter() - t_b0
+        st_b = srv.stats()
     lat = np.array([tm[2] - tm[0] for tm in out])
     t_first = min(tm[0] for tm in out)
     t_last = max(tm[2] for tm in out)
@@ -275,14 +276,21 @@ def served_stream(args, lc, synth, ctx, frac3d=None, rate=None):
         "capacity": {"slices_per_s": round(slices / t_b, 2), "seconds": round(t_b, 3),
                      "speedup_vs_plain_same_mix": round(slices / t_b / plain_rate, 2),
                      "latency_ms_p50": round(1e3 * float(np.median([b[2] - b[0] for b in burst])), 2),
+                     "batches": int(st_b["batches"] - st["batches"]),
+                     "lane_seconds": {k[7:]: round(st_b[k] - v, 3) for k, v in st.items()
+                                      if k.startswith("lane_s_")},
+                     "last_submit_s": round(max(b[0] for b in burst) - min(b[0] for b in burst), 3),
                      "note": "every request of the stream submitted at once"},
-        "lanes": args.serve_lanes, "budget_gb": args.serve_budget_gb, "batches": int(st["batches"]),
+        "lanes": args.serve_lanes, "max_batch": args.serve_max_batch, "budget_gb": args.serve_budget_gb,
+        "batches": int(st["batches"]),
         "lane_seconds": {k[7:]: round(v, 3) for k, v in st.items() if k.startswith("lane_s_")},
         "workload": f"Poisson arrivals; 2-D slices N 128-1024 x 180/360/720 angles and 3-D volumes "
                     f"8x256^2 / 16x384^2 / 32x511^2 (180/360/720 projections, 30 deg tilt) as "
                     f"{100 * frac3d:g}% of requests; 70% CG "
                     f"({args.iters} it), 30% Split-Bregman ({args.sb_outer} outer x 1 inner, "
-                    "lambda U[0.05,0.2], mu 10); sinograms back-projected on the device",
+                    "lambda U[0.05,0.2], mu 10); sinograms back-projected on the device; "
+                    "host arrays in and out (inputs lent to the server, solutions written "
+                    "into the caller's arrays)",
     }
 
 
@@ -452,6 +460,8 @@ def main():
                     help="requests/s of the 2-D-only served stream (0: skip it)")
     ap.add_argument("--serve-lanes", type=int, default=4)
     ap.add_argument("--serve-budget-gb", type=float, default=96.0)
+    ap.add_argument("--serve-max-batch", type=int, default=128,
+                    help="served stream: most requests one lane batch takes")
     ap.add_argument("--sb-outer", type=int, default=20)
     ap.add_argument("--sb-steps", type=int, default=2,
                     help="timed batched Split-Bregman solves on the slices (0: skip)")

@@ -42,6 +42,9 @@ extern "C" {
 #define LC_GEOM_LAZY_PSF 0x20u /* lc_geom2d_create: build the PSF on first use
                                  (geometries used only for projection) */
 #define LC_FP64 0x40u        /* lc_batch_create_ex: FP64 vectors, spectra and FFTs */
+#define LC_REQ_BORROW 0x80u  /* lc_request: sino / rhs / x0 are read in place when the request
+                                runs instead of copied at submit; they must stay valid and
+                                unchanged until the request finishes */
 
 typedef struct lc_ctx_s *lc_ctx;
 typedef struct lc_geom2d_s *lc_geom2d;
@@ -245,21 +248,25 @@ typedef struct {
     int32_t outer, inner;  /* Split-Bregman iterations */
     double tol, xtol;
     double lambda, mu;     /* Split-Bregman */
-    uint32_t flags;        /* LC_TV_ZERO_PAD, LC_TV_ANISO */
+    uint32_t flags;        /* LC_TV_ZERO_PAD, LC_TV_ANISO, LC_REQ_BORROW */
     const float *sino;     /* sinogram {nslices, nang, nrad} / projections {nproj, nrows, ncols},
                               back-projected on the device; or NULL and: */
     const float *rhs;      /* the right-hand side A^T y */
     const float *x0;       /* initial guess, NULL for zero */
+    float *x_out;          /* NULL: the server keeps the solution until lc_server_result copies
+                              it out; or a host buffer of the problem size that the lane writes
+                              the solution into as the request finishes (valid until then) */
 } lc_request;
 int lc_server_create(lc_ctx ctx, uint64_t hbm_budget, int32_t lanes, int32_t max_batch,
                      lc_server *out);
-/* copies the inputs; returns a ticket */
+/* copies the inputs (unless LC_REQ_BORROW); returns a ticket */
 int lc_server_submit(lc_server s, const lc_request *r, uint64_t *ticket);
 int lc_server_poll(lc_server s, uint64_t ticket, int32_t *done);
 /* timeout_s < 0 waits without limit; *done = 0 on timeout */
 int lc_server_wait(lc_server s, uint64_t ticket, double timeout_s, int32_t *done);
-/* copies the solution into x (problem size) and releases the ticket;
- * times[3] = submit, start, done (seconds, steady clock) */
+/* copies the solution into x (problem size) and releases the ticket; a request
+ * submitted with x_out already holds its solution there, and x may be NULL (or
+ * x_out: nothing is copied); times[3] = submit, start, done (seconds, steady clock) */
 int lc_server_result(lc_server s, uint64_t ticket, float *x, lc_report *rep, double *times);
 /* stats[10] = in-flight bytes, queued, running, finished, batches, budget, and
  * summed lane seconds building batches, loading inputs, solving, reading back */

@@ -621,10 +621,12 @@ class Batch2D:
 class Server:
     """Reconstruction server: continuous batching with HBM-budget admission
     over `lanes` device lanes (lc_server_* in laminocam.h).  submit() copies
-    the inputs and returns a ticket; result() waits and returns
-    (x, report, (t_submit, t_start, t_done))."""
+    the inputs (or, with borrow=True, keeps a reference to them and the lane
+    reads them in place: do not modify them until the result is taken) and
+    returns a ticket; the lane writes the solution straight into an array
+    that result() then returns with (report, (t_submit, t_start, t_done))."""
 
-    def __init__(self, hbm_budget: int = 64 << 30, lanes: int = 2, max_batch: int = 512,
+    def __init__(self, hbm_budget: int = 64 << 30, lanes: int = 2, max_batch: int = 128,
                  ctx: Context | None = None):
         self.ctx = ctx or default_context()
         self.lib = self.ctx.lib
@@ -632,11 +634,11 @@ class Server:
         _lib.check(self.lib.lc_server_create(self.ctx.handle, int(hbm_budget), int(lanes),
                                              int(max_batch), C.byref(h)))
         self.handle = h
-        self._shapes = {}
+        self._held = {}   # ticket -> (output array, borrowed inputs)
 
     def submit(self, psf, *, sino=None, rhs=None, x0=None, solver="cg", max_iter=50,
                tol=1e-5, xtol=1e-5, lam=0.1, mu=10.0, outer=1, inner=1, zero_pad=False,
-               aniso=False):
+               aniso=False, borrow=False):
         """psf: PointSpreadFunction2D (a stack) or PointSpreadFunction (a volume).
         sino: sinogram {ns, nang, nrad} / projections {nproj, nrows, ncols}
         (back-projected on the device), or rhs: the right-hand side."""
@@ -655,7 +657,8 @@ class Server:
         r.solver = 0 if solver == "cg" else 1
         r.max_iter, r.outer, r.inner = int(max_iter), int(outer), int(inner)
         r.tol, r.xtol, r.lam, r.mu = float(tol), float(xtol), float(lam), float(mu)
-        r.flags = (LC_TV_ZERO_PAD if zero_pad else 0) | (LC_TV_ANISO if aniso else 0)
+        r.flags = (LC_TV_ZERO_PAD if zero_pad else 0) | (LC_TV_ANISO if aniso else 0) | \
+            (_lib.LC_REQ_BORROW if borrow else 0)
         if sino is not None:
             s = _f32(sino)
             keep.append(s)
@@ -670,9 +673,11 @@ class Server:
             x = _f32(x0)
             keep.append(x)
             r.x0 = x.ctypes.data
+        out = np.empty(shape, np.float32)
+        r.x_out = out.ctypes.data
         t = C.c_uint64()
         _lib.check(self.lib.lc_server_submit(self.handle, C.byref(r), C.byref(t)))
-        self._shapes[t.value] = shape
+        self._held[t.value] = (out, keep if borrow else None)
         return t.value
 
     def done(self, ticket) -> bool:
@@ -686,11 +691,10 @@ class Server:
                                            C.byref(d)))
         if not d.value:
             raise TimeoutError(f"request {ticket} not finished after {timeout} s")
-        shape = self._shapes.pop(ticket)
-        x = np.empty(shape, np.float32)
+        x, _ = self._held.pop(ticket)
         rep = LcReport()
         times = (C.c_double * 3)()
-        _lib.check(self.lib.lc_server_result(self.handle, C.c_uint64(ticket), _ptr(x),
+        _lib.check(self.lib.lc_server_result(self.handle, C.c_uint64(ticket), None,
                                              C.byref(rep), times))
         return x, rep.as_dict(), tuple(times)
 
@@ -703,8 +707,9 @@ class Server:
 
     def close(self):
         if self.handle:
-            self.lib.lc_server_destroy(self.handle)
+            self.lib.lc_server_destroy(self.handle)   # finishes every queued request
             self.handle = None
+            self._held.clear()
 
     def __enter__(self):
         return self

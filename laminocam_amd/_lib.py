@@ -17,6 +17,7 @@ LC_TV_ZERO_PAD = 0x8
 LC_TV_ANISO = 0x10
 LC_GEOM_LAZY_PSF = 0x20
 LC_FP64 = 0x40
+LC_REQ_BORROW = 0x80
 
 LC_REG_SPLIT_BREGMAN = 0
 LC_REG_UNCONSTRAINED = 1
@@ -58,7 +59,8 @@ class LcRequest(C.Structure):
                 ("solver", C.c_int32), ("max_iter", C.c_int32), ("outer", C.c_int32),
                 ("inner", C.c_int32), ("tol", C.c_double), ("xtol", C.c_double),
                 ("lam", C.c_double), ("mu", C.c_double), ("flags", C.c_uint32),
-                ("sino", C.c_void_p), ("rhs", C.c_void_p), ("x0", C.c_void_p)]
+                ("sino", C.c_void_p), ("rhs", C.c_void_p), ("x0", C.c_void_p),
+                ("x_out", C.c_void_p)]
 
 
 class LcReconParams(C.Structure):

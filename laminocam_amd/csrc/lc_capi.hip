@@ -892,13 +892,26 @@ int lc_server_submit(lc_server s, const lc_request *r, uint64_t *ticket) {
         q->lambda = (float)r->lambda;
         q->mu = (float)r->mu;
         q->flags = r->flags;
-        if (r->sino) {
-            q->has_sino = true;
-            q->sino.assign(r->sino, r->sino + nsino);
+        const bool borrow = (r->flags & LC_REQ_BORROW) != 0;
+        q->has_sino = r->sino != nullptr;
+        q->in_n = (size_t)(r->sino ? nsino : nvol);
+        const float *in = r->sino ? r->sino : r->rhs;
+        if (borrow) {
+            q->in = in;
         } else {
-            q->rhs.assign(r->rhs, r->rhs + nvol);
+            (r->sino ? q->sino : q->rhs).assign(in, in + q->in_n);
+            q->in = (r->sino ? q->sino : q->rhs).data();
         }
-        if (r->x0) q->x0.assign(r->x0, r->x0 + nvol);
+        if (r->x0) {
+            q->x0_n = (size_t)nvol;
+            if (borrow) {
+                q->x0p = r->x0;
+            } else {
+                q->x0.assign(r->x0, r->x0 + nvol);
+                q->x0p = q->x0.data();
+            }
+        }
+        q->x_out = r->x_out;
         *ticket = s->s->submit(std::move(q));
     });
 }
@@ -920,10 +933,16 @@ int lc_server_wait(lc_server s, uint64_t ticket, double timeout_s, int32_t *done
 
 int lc_server_result(lc_server s, uint64_t ticket, float *x, lc_report *rep, double *times) {
     return guard([&] {
-        LC_REQUIRE(s && x, "lc_server_result: NULL argument");
+        LC_REQUIRE(s, "lc_server_result: NULL server");
         auto r = s->s->take(ticket);
         if (r->state == 3) throw Error(LC_EINTERNAL, "server: request failed: " + r->err);
-        std::memcpy(x, r->x.data(), sizeof(float) * r->x.size());
+        if (!r->x_out) {
+            LC_REQUIRE(x, "lc_server_result: NULL x for a request without x_out");
+            std::memcpy(x, r->x.data(), sizeof(float) * r->x.size());
+        } else if (x && x != r->x_out) {
+            std::memcpy(x, r->x_out, sizeof(float) * (size_t)(r->g3 ? r->g3->n1 * r->g3->n2 * r->g3->n3
+                                                                    : (int64_t)r->ns * r->g2->N * r->g2->N));
+        }
         if (rep) *rep = r->rep;
         if (times) {
             times[0] = r->t_submit;

@@ -9,7 +9,9 @@
 // HBM budget and shares its solver settings, builds one batch, and runs the
 // solve from captured graphs.  Requests arriving while other lanes are busy
 // start on the next idle lane, so their work overlaps on the device with the
-// batches already running.  Each problem keeps its own CG / Bregman scalars,
+// batches already running.  A batch's results are copied to the callers'
+// host arrays by a worker while its lane goes on to the next batch.  Each
+// problem keeps its own CG / Bregman scalars,
 // so its result does not depend on the batch it ran in.
 #include <algorithm>
 #include <chrono>
@@ -92,15 +94,19 @@ uint64_t Server::submit(std::unique_ptr<ServerRequest> r) {
     // geometry state the lanes only read: PSF and NUFFT tables, built here
     {
         std::lock_guard<std::mutex> lk(main_.mu);
-        main_.set_current();
-        if (r->g3) {
-            if (!r->g3->psf_ready) geom3d_build_psf(main_, *r->g3);
-            if (r->has_sino && !r->g3->nu.ready) nufft3d_init(main_, *r->g3);
-        } else {
-            if (!r->g2->psf_ready) geom2d_build_psf(main_, *r->g2);
-            if (r->has_sino && !r->g2->nu.ready) nufft2d_init(main_, *r->g2);
+        const bool build = r->g3 ? !r->g3->psf_ready || (r->has_sino && !r->g3->nu.ready)
+                                 : !r->g2->psf_ready || (r->has_sino && !r->g2->nu.ready);
+        if (build) {   // (a request on a ready geometry launches nothing here)
+            main_.set_current();
+            if (r->g3) {
+                if (!r->g3->psf_ready) geom3d_build_psf(main_, *r->g3);
+                if (r->has_sino && !r->g3->nu.ready) nufft3d_init(main_, *r->g3);
+            } else {
+                if (!r->g2->psf_ready) geom2d_build_psf(main_, *r->g2);
+                if (r->has_sino && !r->g2->nu.ready) nufft2d_init(main_, *r->g2);
+            }
+            main_.sync();
         }
-        main_.sync();
     }
     r->bytes = request_bytes(*r);
     LC_REQUIRE(r->bytes <= budget_, "server: request needs " + std::to_string(r->bytes) +
@@ -174,7 +180,12 @@ void Server::lane_loop(Lane &lane) {
         {
             std::unique_lock<std::mutex> lk(m_);
             for (;;) {
-                if (stop_ && queue_.empty()) return;
+                if (stop_ && queue_.empty()) {
+                    // (the copy-out takes m_ to mark its requests done)
+                    lk.unlock();
+                    if (lane.out_job.valid()) lane.out_job.get();
+                    return;
+                }
                 if (!queue_.empty()) {
                     // first-come requests of the head's group that fit the free budget
                     ServerRequest *head = reqs_[queue_.front()].get();
@@ -205,29 +216,54 @@ void Server::lane_loop(Lane &lane) {
             ++batches_;
         }
         std::string err;
+        std::vector<OutCopy> outs;
         try {
-            run_batch(lane, batch);
+            outs = run_batch(lane, batch);
         } catch (const std::exception &e) {
             err = e.what();
         }
         {
+            // the batch's device memory is released: other lanes may admit now
             std::lock_guard<std::mutex> lk(m_);
-            const double t = now_s();
-            for (ServerRequest *r : batch) {
-                inflight_ -= r->bytes;
-                r->t_done = t;
-                r->state = err.empty() ? 2 : 3;
-                r->err = err;
-            }
-            running_ -= batch.size();
-            done_ += batch.size();
+            for (ServerRequest *r : batch) inflight_ -= r->bytes;
         }
-        cv_done_.notify_all();
-        cv_work_.notify_all();   // memory freed: other lanes may admit now
+        cv_work_.notify_all();
+        if (!err.empty()) {
+            finish(batch, err);
+            continue;
+        }
+        // the host copies of the results run beside the lane's next batch
+        lane.out_job = std::async(std::launch::async, [this, batch, outs = std::move(outs)] {
+            for (const OutCopy &o : outs) {
+                float *xo = o.r->x_out;
+                if (!xo) {
+                    o.r->x.resize(o.n);
+                    xo = o.r->x.data();
+                }
+                std::memcpy(xo, o.src, sizeof(float) * o.n);
+            }
+            finish(batch, std::string());
+        });
     }
 }
 
-void Server::run_batch(Lane &lane, std::vector<ServerRequest *> &batch) {
+// marks a batch's requests done (or failed) and wakes their waiters
+void Server::finish(const std::vector<ServerRequest *> &batch, const std::string &err) {
+    {
+        std::lock_guard<std::mutex> lk(m_);
+        const double t = now_s();
+        for (ServerRequest *r : batch) {
+            r->t_done = t;
+            r->state = err.empty() ? 2 : 3;
+            r->err = err;
+        }
+        running_ -= batch.size();
+        done_ += batch.size();
+    }
+    cv_done_.notify_all();
+}
+
+std::vector<Server::OutCopy> Server::run_batch(Lane &lane, std::vector<ServerRequest *> &batch) {
     Ctx &c = *lane.ctx;
     std::lock_guard<std::mutex> lk(c.mu);
     c.set_current();
@@ -255,8 +291,8 @@ void Server::run_batch(Lane &lane, std::vector<ServerRequest *> &batch) {
     size_t in_total = 0;
     bool any_x0 = false;
     for (ServerRequest *r : batch) {
-        in_total += pad(r->has_sino ? r->sino.size() : r->rhs.size()) + pad(r->x0.size());
-        if (!r->x0.empty()) any_x0 = true;
+        in_total += pad(r->in_n) + (r->x0p ? pad(r->x0_n) : 0);
+        if (r->x0p) any_x0 = true;
     }
     float *hst = static_cast<float *>(lane.stage.get(sizeof(float) * std::max<size_t>(in_total, 1)));
     DBuf dst_in;
@@ -277,12 +313,12 @@ void Server::run_batch(Lane &lane, std::vector<ServerRequest *> &batch) {
     };
     for (size_t i = 0; i < batch.size(); ++i) {
         ServerRequest *r = batch[i];
-        const std::vector<float> &src = r->has_sino ? r->sino : r->rhs;
-        std::memcpy(hst + off, src.data(), sizeof(float) * src.size());
+        const size_t src_n = r->in_n;
+        std::memcpy(hst + off, r->in, sizeof(float) * src_n);
         float *s = dst + off;
-        copy_in(c, s, hst + off, sizeof(float) * src.size(), false);
+        copy_in(c, s, hst + off, sizeof(float) * src_n, false);
         const size_t soff = off;
-        off += pad(src.size());
+        off += pad(src_n);
         float *y = b.Y() + b.offset((int)i);
         if (r->has_sino && r->g2) {
             // one back-projection's fine-grid scratch stays near 1 GiB
@@ -299,27 +335,27 @@ void Server::run_batch(Lane &lane, std::vector<ServerRequest *> &batch) {
             }
             run_ns += r->ns;
             run_end = off;
-            run_tight = pad(src.size()) == src.size();
+            run_tight = pad(src_n) == src_n;
             run_yend = b.offset((int)i) + b.size((int)i);
         } else {
             flush();
             if (r->has_sino) backproj3d_dev(c, *r->g3, s, y, nullptr);
-            else LC_HIP(hipMemcpyAsync(y, s, sizeof(float) * src.size(), hipMemcpyDeviceToDevice,
+            else LC_HIP(hipMemcpyAsync(y, s, sizeof(float) * src_n, hipMemcpyDeviceToDevice,
                                        c.stream()));
         }
         float *x = b.X() + b.offset((int)i);
-        if (!r->x0.empty()) {
-            std::memcpy(hst + off, r->x0.data(), sizeof(float) * r->x0.size());
-            copy_in(c, dst + off, hst + off, sizeof(float) * r->x0.size(), false);
-            LC_HIP(hipMemcpyAsync(x, dst + off, sizeof(float) * r->x0.size(),
+        if (r->x0p) {
+            std::memcpy(hst + off, r->x0p, sizeof(float) * r->x0_n);
+            copy_in(c, dst + off, hst + off, sizeof(float) * r->x0_n, false);
+            LC_HIP(hipMemcpyAsync(x, dst + off, sizeof(float) * r->x0_n,
                                   hipMemcpyDeviceToDevice, c.stream()));
-            off += pad(r->x0.size());
+            off += pad(r->x0_n);
         } else if (any_x0) {
             // (with no initial guess in the batch the solver clears x itself)
             LC_HIP(hipMemsetAsync(x, 0, sizeof(float) * (size_t)b.size((int)i), c.stream()));
         }
         // an initial guess staged after the sinogram breaks staging contiguity
-        if (!r->x0.empty()) flush();
+        if (r->x0p) flush();
     }
     flush();
     c.sync();
@@ -342,15 +378,18 @@ void Server::run_batch(Lane &lane, std::vector<ServerRequest *> &batch) {
     const double t3 = now_s();
     const auto cs = b.cg_state();
     const auto ss = h.solver == 1 ? b.sb_state() : std::vector<SbState>();
-    // results: the batch's whole solution buffer in one copy into pinned
-    // staging (problems are contiguous), then per request on the host
-    float *hx = static_cast<float *>(lane.stage.get(sizeof(float) * (size_t)b.total()));
+    // results: the batch's whole solution buffer in one copy into the result
+    // staging (problems are contiguous), once the previous batch's copy-out
+    // has finished reading it; the per-request host copies are returned to
+    // the lane loop, which runs them beside the lane's next batch
+    std::vector<OutCopy> outs;
+    if (lane.out_job.valid()) lane.out_job.get();
+    float *hx = static_cast<float *>(lane.ostage.get(sizeof(float) * (size_t)b.total()));
     copy_out(c, hx, b.X(), sizeof(float) * (size_t)b.total(), false);
     c.sync();
     for (size_t i = 0; i < batch.size(); ++i) {
         ServerRequest *r = batch[i];
-        r->x.resize((size_t)b.size((int)i));
-        std::memcpy(r->x.data(), hx + b.offset((int)i), sizeof(float) * r->x.size());
+        outs.push_back(OutCopy{r, hx + b.offset((int)i), (size_t)b.size((int)i)});
         r->rep.iters = cs[i].iters;
         r->rep.outer = h.solver == 1 ? ss[i].outer : 0;
         r->rep.res = cs[i].res;
@@ -365,6 +404,7 @@ void Server::run_batch(Lane &lane, std::vector<ServerRequest *> &batch) {
     t_input_ += t2 - t1;
     t_solve_ += t3 - t2;
     t_output_ += t4 - t3;
+    return outs;
 }
 
 } // namespace lc

@@ -7,6 +7,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <future>
 #include <thread>
 #include <vector>
 
@@ -29,7 +30,15 @@ struct ServerRequest {
     std::vector<float> sino;   // sinogram / projections (back-projected on the device)
     std::vector<float> rhs;    // right-hand side when no sinogram
     std::vector<float> x0;     // empty: start from zero
-    // results
+    // the input the lane reads (sinogram or right-hand side) and the initial
+    // guess (x0p == nullptr: zero): into the vectors above, or the caller's
+    // arrays when borrowed
+    const float *in = nullptr;
+    size_t in_n = 0;
+    const float *x0p = nullptr;
+    size_t x0_n = 0;
+    // results: into the caller's x_out when given, else into x
+    float *x_out = nullptr;
     std::vector<float> x;
     lc_report rep{};
     size_t bytes = 0;          // device bytes held while running
@@ -54,14 +63,23 @@ class Server {
     void stats(double *out);
 
   private:
+    // one request's solution, from the lane's result staging to its host array
+    struct OutCopy {
+        ServerRequest *r;
+        const float *src;
+        size_t n;
+    };
     struct Lane {
         MemPool pool;                // outlives the context's buffers
-        PinnedBuf stage;             // host staging of the lane's inputs and results
+        PinnedBuf stage;             // host staging of the lane's inputs
+        PinnedBuf ostage;            // host staging of the lane's results
+        std::future<void> out_job;   // the previous batch's copy-out (reads ostage)
         std::unique_ptr<Ctx> ctx;
         std::thread th;
     };
     void lane_loop(Lane &lane);
-    void run_batch(Lane &lane, std::vector<ServerRequest *> &batch);
+    std::vector<OutCopy> run_batch(Lane &lane, std::vector<ServerRequest *> &batch);
+    void finish(const std::vector<ServerRequest *> &batch, const std::string &err);
 
     Ctx &main_;
     size_t budget_;

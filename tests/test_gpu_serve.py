@@ -51,7 +51,7 @@ def solve_alone(lc, r):
         rhs = lc.backproj2d(r["sino"], r["grid"], (r["sino"].shape[0], N, N))
     else:
         rhs = r["rhs"]
-    x0 = np.zeros(rhs.shape, np.float32)
+    x0 = r["x0"] if "x0" in r else np.zeros(rhs.shape, np.float32)
     if r["solver"] == "cg":
         return lc.cgsolver(p, rhs, x0, r["max_iter"], 1e-5, 1e-5)[0]
     return lc.split_bregman(p, rhs, x0, r["lam"], r["mu"], r["outer"], r["inner"], 1e-5, 1e-5)[0]
@@ -117,3 +117,55 @@ def test_server_threads_submit_concurrently(lc, problems):
             t.join(timeout=180.0)
     for x, xr in zip(res, ref):
         assert x is not None and np.array_equal(x, xr)
+
+
+def test_server_borrowed_inputs_and_copy_out(lc, problems):
+    # three ways in and out, one result: inputs copied at submit (the caller's
+    # arrays are overwritten right after), inputs borrowed (read in place when
+    # the request runs), and the C-level copy-out path (x_out NULL: the server
+    # keeps the solution until lc_server_result copies it); with initial guesses
+    announce("server borrowed inputs and copy-out")
+    import ctypes as C
+    from laminocam_amd import _lib
+    rng = np.random.default_rng(23)
+    reqs = []
+    for r in problems[:4]:
+        r = dict(r)
+        p = r["psf"]
+        shape = tuple(p.recon_dims) if isinstance(p, lc.PointSpreadFunction) else \
+            ((r["sino"].shape[0] if "sino" in r else r["rhs"].shape[0]), p.image_size, p.image_size)
+        r["x0"] = (0.1 * rng.random(shape)).astype(np.float32)
+        reqs.append(r)
+    ref = [solve_alone(lc, r) for r in reqs]
+    with lc.Server(hbm_budget=8 << 30, lanes=2) as srv:
+        copied = []
+        for r in reqs:
+            rc = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in r.items()}
+            copied.append(submit(srv, rc))
+            for v in rc.values():
+                if isinstance(v, np.ndarray):
+                    v[...] = 7.0    # the server holds its own copy
+        borrowed = [srv.submit(r["psf"], borrow=True,
+                               **{k: v for k, v in r.items() if k not in ("psf", "grid")})
+                    for r in reqs]
+        # C level, no x_out: a 2-D CG request from its right-hand side
+        r0 = reqs[0]
+        N = r0["psf"].image_size
+        rhs0 = lc.backproj2d(r0["sino"], r0["grid"], (r0["sino"].shape[0], N, N))
+        q = _lib.LcRequest()
+        q.g2, q.nslices, q.solver, q.max_iter = r0["psf"].handle.value, rhs0.shape[0], 0, r0["max_iter"]
+        q.outer, q.inner, q.tol, q.xtol = 1, 1, 1e-5, 1e-5
+        q.rhs, q.x0 = rhs0.ctypes.data, r0["x0"].ctypes.data
+        t = C.c_uint64()
+        _lib.check(srv.lib.lc_server_submit(srv.handle, C.byref(q), C.byref(t)))
+        d = C.c_int32()
+        _lib.check(srv.lib.lc_server_wait(srv.handle, t, 120.0, C.byref(d)))
+        assert d.value == 1
+        xc = np.empty(rhs0.shape, np.float32)
+        _lib.check(srv.lib.lc_server_result(srv.handle, t, xc.ctypes.data, None, None))
+        out_c = [srv.result(k, timeout=120.0)[0] for k in copied]
+        out_b = [srv.result(k, timeout=120.0)[0] for k in borrowed]
+    for i, xr in enumerate(ref):
+        assert np.array_equal(out_c[i], xr), f"copied request {i} differs from its solo solve"
+        assert np.array_equal(out_b[i], xr), f"borrowed request {i} differs from its solo solve"
+    assert np.array_equal(xc, ref[0])
