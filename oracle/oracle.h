@@ -1,7 +1,7 @@
 /*
  * CPU oracle for the LaminoCam reconstruction hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: the HIP product path
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle: the HIP product path
  * never links, loads or calls it.  Only tests/, the smoke check and the
  * cpu_baseline leg of bench.py use it.
  *

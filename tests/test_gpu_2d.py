@@ -101,7 +101,8 @@ def test_cg_matches_oracle(lc, orc64, N, nang, ns, iters):
 def test_cg_long_no_worse_than_fp32_reference(lc, orc64, orc32, N, nang, iters):
     # Past ~10 iterations plain CG on these undersampled systems amplifies
     # rounding chaotically: two FP64 runs that differ only in summation order
-    # drift apart by up to 1e-2 ("solver parity").  The bar here is
+    # drift apart by up to 1e-2 (the bench's rel_err_vs_fp64 record measures
+    # this on every run as fp64_independent).  The bar here is
     # that the GPU result is as close to the FP64 oracle as the FP32
     # restatement of the reference's own float path, and that it reaches the
     # same residual.
