@@ -197,3 +197,29 @@ def test_batch_same_alone_and_in_mix(lc, orc64):
         assert np.array_equal(alone.get(0, alone.X), batch.get(i, batch.X))
         xs, _ = lc.cgsolver(psfs[i], rhss[i], np.zeros_like(rhss[i]), iters[i], 1e-5, 1e-5)
         assert np.array_equal(xs, batch.get(i, batch.X))
+
+
+@pytest.mark.parametrize("N", [128, 384, 1024])
+def test_row_shapes_agree(lc, orc64, N):
+    # both compiled row-pass shapes (256x8, 512x8) give the oracle's
+    # Toeplitz apply, and the CG-fused row passes agree across shapes
+    announce(f"row shapes N={N}")
+    ctx = lc.Context(0)
+    try:
+        ctx.load_tuning()
+        _, col, B = ctx.get_tuning(N)
+        rng = np.random.default_rng(5 + N)
+        th = ref_theta(16)
+        x = rng.random((1, N, N)).astype(np.float32)
+        psf = lc.PointSpreadFunction2D(lc.PolarGrid2D(th, N), N, ctx=ctx)
+        yr = orc64.psf2d(th, N, N, method=0).convolve(x)
+        rhs = (yr / np.max(np.abs(yr))).astype(np.float32)
+        xs = []
+        for row in range(2):
+            ctx.set_tuning(N, (row, col, B))
+            psf2 = lc.PointSpreadFunction2D(lc.PolarGrid2D(th, N), N, ctx=ctx)
+            assert rel_l2(psf2.convolve(x), yr) < 1e-5, row
+            xs.append(lc.cgsolver(psf2, rhs, np.zeros_like(rhs), 4, 1e-7, 1e-7)[0])
+        assert rel_l2(xs[1], xs[0]) < 1e-5
+    finally:
+        ctx.close()
