@@ -583,9 +583,10 @@ void Batch2D::build() {
         if (f64_) k->tw64 = c_.twiddles<double>(k->M);
         else k->tw = c_.twiddles<float>(k->M);
         if (g3) {
-            // runtime-plan shapes: row shape by M, column blocks B * M <= the 3-D cap
+            // row shape by M (512 x 8 above 1024: 1.7% faster per laminography
+            // CG iteration at M = 1536 than 256 x 8), column blocks B * M <= the 3-D cap
             ToepVariant v;
-            v.row = k->M > 2048 ? 1 : 0;
+            v.row = k->M > 1024 ? 1 : 0;
             v.col = f64_ ? 0 : 2;   // block-width bound only: 4096 (FP64) or 8192 elements
             v.B = 0;
             toep_configure(*k, v);
