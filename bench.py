@@ -470,6 +470,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the served stream")
     ap.add_argument("--no-lamino", action="store_true", help="skip the 3-D laminography solves")
     ap.add_argument("--lamino-iters", type=int, default=100)
+    ap.add_argument("--lamino-only", action="store_true",
+                    help="run only the 3-D laminography solves (one JSON line)")
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the served stream and print its JSON (profiling)")
     ap.add_argument("--nufft-probe", action="store_true",
@@ -494,6 +496,9 @@ def main():
     import laminocam_amd as lc
     from laminocam_amd import synth
 
+    if args.lamino_only:
+        print(json.dumps({"lamino3d": lamino_leg(args, lc, synth, lc.default_context())}), flush=True)
+        return
     if args.serve_only:
         served = served_stream(args, lc, synth, lc.default_context())
         if args.serve_2d_rate > 0:
