@@ -97,59 +97,74 @@ __device__ __forceinline__ bool inside_pi(double x, double y, double z) {
 }
 
 // ---------------------------------------------------------------------------
-// type-2 interpolation: one thread per point
+// type-2 interpolation: a wave per point.  Each lane first finds the footprint
+// of one of the wave's 64 consecutive points; the wave then takes the points
+// in turn with its lanes over the 8 x 8 (y, x) footprint, walking the 8 z
+// planes (each step reads 8 rows of 8 contiguous fine-grid values), and sums
+// the 64 lanes in a fixed order.  Lane i writes point i, so the stores are
+// contiguous.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 k_interp3d(const float2 *__restrict__ F, float2 *__restrict__ C, const double *__restrict__ rot,
            Grid3 g) {
+    static_assert(W3 == 8, "the lane layout assumes a footprint of 8 per axis");
     const int64_t npts = (int64_t)g.nproj * g.nrows * g.ncols;
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= npts) return;
-    const int k = (int)(q % g.ncols);
-    const int64_t t = q / g.ncols;
-    const int j = (int)(t % g.nrows), p = (int)(t / g.nrows);
-    double x, y, z;
-    point3(rot, p, j, k, g, x, y, z);
-    if (g.mask && !inside_pi(x, y, z)) {
-        C[q] = make_float2(0.f, 0.f);
-        return;
-    }
+    const int lane = threadIdx.x & 63;
+    const int64_t base = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64;
+    if (base >= npts) return;   // whole wave
+    const int64_t q = base + lane;
     const double hw = 0.5 * W3;
     const float ihw = 1.0f / (float)hw;
-    const double gz = 0.5 * g.nf[0] + z * (g.nf[0] / (2.0 * M_PI));
-    const double gy = 0.5 * g.nf[1] + y * (g.nf[1] / (2.0 * M_PI));
-    const double gx = 0.5 * g.nf[2] + x * (g.nf[2] / (2.0 * M_PI));
-    const int z0 = (int)ceil(gz - hw), y0 = (int)ceil(gy - hw), x0 = (int)ceil(gx - hw);
-    float wz[W3], wy[W3], wx[W3];
-    int xi[W3];
-#pragma unroll
-    for (int d = 0; d < W3; ++d) {
-        wz[d] = es3((float)((double)(z0 + d) - gz) * ihw, g.beta);
-        wy[d] = es3((float)((double)(y0 + d) - gy) * ihw, g.beta);
-        wx[d] = es3((float)((double)(x0 + d) - gx) * ihw, g.beta);
-        xi[d] = wrapi(x0 + d, g.nf[2]);
+    double gz = 0.0, gy = 0.0, gx = 0.0;
+    int z0 = 0, y0 = 0, x0 = 0, ok = 0;
+    if (q < npts) {
+        const int k = (int)(q % g.ncols);
+        const int64_t t = q / g.ncols;
+        const int j = (int)(t % g.nrows), p = (int)(t / g.nrows);
+        double x, y, z;
+        point3(rot, p, j, k, g, x, y, z);
+        ok = (!g.mask || inside_pi(x, y, z)) ? 1 : 0;
+        gz = 0.5 * g.nf[0] + z * (g.nf[0] / (2.0 * M_PI));
+        gy = 0.5 * g.nf[1] + y * (g.nf[1] / (2.0 * M_PI));
+        gx = 0.5 * g.nf[2] + x * (g.nf[2] / (2.0 * M_PI));
+        z0 = (int)ceil(gz - hw);
+        y0 = (int)ceil(gy - hw);
+        x0 = (int)ceil(gx - hw);
     }
-    float re = 0.f, im = 0.f;
-    for (int a = 0; a < W3; ++a) {
-        const int zz = wrapi(z0 + a, g.nf[0]);
-        float pr = 0.f, pi = 0.f;
-        for (int b = 0; b < W3; ++b) {
-            const int yy = wrapi(y0 + b, g.nf[1]);
-            const float2 *row = F + ((int64_t)zz * g.nf[1] + yy) * g.nf[2];
-            float rr = 0.f, ri = 0.f;
+    const int iy = lane >> 3, ix = lane & 7;
+    const int nlast = (int)(npts - 1 - base < 63 ? npts - 1 - base : 63);
+    float2 res = make_float2(0.f, 0.f);
+    for (int j = 0; j <= nlast; ++j) {
+        if (!__builtin_amdgcn_readlane(ok, j)) continue;   // masked point: 0
+        const double gzj = rl_f64(gz, j), gyj = rl_f64(gy, j), gxj = rl_f64(gx, j);
+        const int z0j = __builtin_amdgcn_readlane(z0, j);
+        const int y0j = __builtin_amdgcn_readlane(y0, j);
+        const int x0j = __builtin_amdgcn_readlane(x0, j);
+        // lane (iy, ix) holds the y weight of row iy, the x weight of column
+        // ix, and the z weight of plane ix (read by every lane in the z walk)
+        const float wzl = es3((float)((double)(z0j + ix) - gzj) * ihw, g.beta);
+        const float wyl = es3((float)((double)(y0j + iy) - gyj) * ihw, g.beta);
+        const float wxl = es3((float)((double)(x0j + ix) - gxj) * ihw, g.beta);
+        const int yy = wrapi(y0j + iy, g.nf[1]), xx = wrapi(x0j + ix, g.nf[2]);
+        float rr = 0.f, ri = 0.f;
 #pragma unroll
-            for (int c = 0; c < W3; ++c) {
-                const float2 v = row[xi[c]];
-                rr += wx[c] * v.x;
-                ri += wx[c] * v.y;
-            }
-            pr += wy[b] * rr;
-            pi += wy[b] * ri;
+        for (int a = 0; a < W3; ++a) {
+            const float wza = rl_f32(wzl, a);
+            const int zz = wrapi(z0j + a, g.nf[0]);
+            const float2 v = F[((int64_t)zz * g.nf[1] + yy) * g.nf[2] + xx];
+            rr += wza * v.x;
+            ri += wza * v.y;
         }
-        re += wz[a] * pr;
-        im += wz[a] * pi;
+        const float w = wyl * wxl;
+        float sr = w * rr, si = w * ri;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            sr += __shfl_xor(sr, off, 64);
+            si += __shfl_xor(si, off, 64);
+        }
+        if (lane == j) res = make_float2(sr, si);
     }
-    C[q] = make_float2(re, im);
+    if (q < npts) C[q] = res;
 }
 
 // include/fft.h:53-74 (phase_shift2d): detector spectrum C[p][j][k] *=
@@ -740,8 +755,9 @@ void type2_3d(Ctx &c, const Geom3D &g, const Work3 &w, const float *vol, bool ma
                              corr + u.coff[2], 1.0f, u.plan[2]);
     const Grid3 gg = grid3(g, mask);
     const int64_t npts = g.nproj * g.nrows * g.ncols;
-    hipLaunchKernelGGL(k_interp3d, dim3(ceil_div(npts, 256)), dim3(256), 0, c.stream(), w.F, w.C,
-                       g.d_rot.as<double>(), gg);
+    // a wave of 64 points, four waves per block
+    hipLaunchKernelGGL(k_interp3d, dim3(ceil_div(ceil_div(npts, (int64_t)64), (int64_t)4)), dim3(256), 0,
+                       c.stream(), w.F, w.C, g.d_rot.as<double>(), gg);
     LC_HIP(hipGetLastError());
 }
 
