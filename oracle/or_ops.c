@@ -652,17 +652,68 @@ static void sb_op(void *c, const REAL *in, REAL *out) {
     for (int64_t i = 0; i < n; ++i) out[i] -= s->tmp[i] * (REAL)s->mu;
 }
 
-/* src/bregman.cpp:36-94.  aniso=1 replaces the isotropic shrink by the
- * per-component soft threshold (an option the reference does not have). */
-int or_split_bregman(or_op_fn A, void *ctx, const REAL *y, const REAL *x0,
-                     int64_t n1, int64_t n2, int64_t n3, double lambda,
-                     double mu, int64_t outer, int64_t inner, double tol,
-                     double xtol, int boundary, int aniso, REAL *x,
-                     or_report *rep) {
+/* The three Split-Bregman steps of one outer iteration, each exposed so that
+ * it can be checked on its own.
+ *
+ * rhs = yT - mu * div(d - b)                       src/bregman.cpp:58-61 */
+void or_sb_rhs(const REAL *y, const REAL *d, const REAL *b, int64_t n1,
+               int64_t n2, int64_t n3, int boundary, double mu, REAL *rhs) {
+    const int64_t n = n1 * n2 * n3;
+    REAL *g = (REAL *)malloc(sizeof(REAL) * 3 * (size_t)n);
+    for (int64_t i = 0; i < 3 * n; ++i) g[i] = d[i] - b[i];
+    or_div(g, n1, n2, n3, boundary, rhs);
+    for (int64_t i = 0; i < n; ++i) rhs[i] = y[i] - rhs[i] * (REAL)mu;
+    free(g);
+}
+
+/* d from the gradient g of the new x and the Bregman variable b, n points of
+ * 3 components each (src/bregman.cpp:66-80):
+ *   isotropic:   sk = sqrt(sum_c (g_c + b_c)^2),
+ *                d_c = max(0, sk - lambda/mu) * (g_c + b_c) / (sk + 1e-8)
+ *   anisotropic: d_c = sign(v) max(0, |v| - lambda/mu), v = g_c + b_c
+ *                (an option the reference does not have). */
+void or_sb_shrink(const REAL *g, const REAL *b, int64_t n, double lambda,
+                  double mu, int aniso, REAL *d) {
+    const REAL thr = (REAL)(lambda / mu);
+    for (int64_t i = 0; i < n; ++i) {
+        if (!aniso) {
+            double ss = 0;
+            for (int c = 0; c < 3; ++c) {
+                const REAL v = g[c * n + i] + b[c * n + i];
+                ss += (double)v * v;
+            }
+            const REAL sk = (REAL)sqrt(ss);
+            const REAL sh = (sk - thr > 0) ? sk - thr : 0;
+            for (int c = 0; c < 3; ++c) {
+                const REAL v = (g[c * n + i] + b[c * n + i]) / (sk + (REAL)1e-8);
+                d[c * n + i] = sh * v;
+            }
+        } else {
+            for (int c = 0; c < 3; ++c) {
+                const REAL v = g[c * n + i] + b[c * n + i];
+                const REAL av = v < 0 ? -v : v;
+                const REAL sh = (av - thr > 0) ? av - thr : 0;
+                d[c * n + i] = (v < 0) ? -sh : sh;
+            }
+        }
+    }
+}
+
+/* Bregman update b += g - d over 3 n values (src/bregman.cpp:82-84) */
+void or_sb_bregman_update(REAL *b, const REAL *g, const REAL *d, int64_t n) {
+    for (int64_t i = 0; i < 3 * n; ++i) b[i] += g[i] - d[i];
+}
+
+/* src/bregman.cpp:36-94 with the splitting variables d and b (3 n values
+ * each) supplied by the caller and updated in place; the reference starts
+ * them at zero (src/bregman.cpp:44-48), as or_split_bregman does. */
+int or_split_bregman_state(or_op_fn A, void *ctx, const REAL *y, const REAL *x0,
+                           int64_t n1, int64_t n2, int64_t n3, double lambda,
+                           double mu, int64_t outer, int64_t inner, double tol,
+                           double xtol, int boundary, int aniso, REAL *d,
+                           REAL *b, REAL *x, or_report *rep) {
     const int64_t n = n1 * n2 * n3;
     REAL *xold = (REAL *)malloc(sizeof(REAL) * (size_t)n);
-    REAL *d = (REAL *)calloc(3 * (size_t)n, sizeof(REAL));
-    REAL *b = (REAL *)calloc(3 * (size_t)n, sizeof(REAL));
     REAL *g = (REAL *)malloc(sizeof(REAL) * 3 * (size_t)n);
     REAL *rhs = (REAL *)malloc(sizeof(REAL) * (size_t)n);
     REAL *xn = (REAL *)malloc(sizeof(REAL) * (size_t)n);
@@ -671,37 +722,13 @@ int or_split_bregman(or_op_fn A, void *ctx, const REAL *y, const REAL *x0,
     memcpy(x, x0, sizeof(REAL) * (size_t)n);
     memcpy(xold, x0, sizeof(REAL) * (size_t)n);
     or_report rp = {0, 0, 0, 0, 0}, cg;
-    const REAL thr = (REAL)(lambda / mu);
     for (int64_t it = 0; it < outer; ++it) {
-        for (int64_t i = 0; i < 3 * n; ++i) g[i] = d[i] - b[i];
-        or_div(g, n1, n2, n3, boundary, rhs);
-        for (int64_t i = 0; i < n; ++i) rhs[i] = y[i] - rhs[i] * (REAL)mu;
+        or_sb_rhs(y, d, b, n1, n2, n3, boundary, mu, rhs);
         or_cgsolver(sb_op, &s, rhs, x, n, inner, tol, xtol, xn, &cg);
         memcpy(x, xn, sizeof(REAL) * (size_t)n);
         or_grad(x, n1, n2, n3, boundary, g);
-        for (int64_t i = 0; i < n; ++i) {
-            if (!aniso) {
-                double ss = 0;
-                for (int c = 0; c < 3; ++c) {
-                    const REAL v = g[c * n + i] + b[c * n + i];
-                    ss += (double)v * v;
-                }
-                const REAL sk = (REAL)sqrt(ss);
-                const REAL sh = (sk - thr > 0) ? sk - thr : 0;
-                for (int c = 0; c < 3; ++c) {
-                    const REAL v = (g[c * n + i] + b[c * n + i]) / (sk + (REAL)1e-8);
-                    d[c * n + i] = sh * v;
-                }
-            } else {
-                for (int c = 0; c < 3; ++c) {
-                    const REAL v = g[c * n + i] + b[c * n + i];
-                    const REAL av = v < 0 ? -v : v;
-                    const REAL sh = (av - thr > 0) ? av - thr : 0;
-                    d[c * n + i] = (v < 0) ? -sh : sh;
-                }
-            }
-        }
-        for (int64_t i = 0; i < 3 * n; ++i) b[i] += g[i] - d[i];
+        or_sb_shrink(g, b, n, lambda, mu, aniso, d);
+        or_sb_bregman_update(b, g, d, n);
         double dn = 0, on = 0;
         for (int64_t i = 0; i < n; ++i) {
             const double df = (double)x[i] - (double)xold[i];
@@ -720,6 +747,23 @@ int or_split_bregman(or_op_fn A, void *ctx, const REAL *y, const REAL *x0,
         }
     }
     if (rep) *rep = rp;
-    free(xold); free(d); free(b); free(g); free(rhs); free(xn); free(s.tmp);
+    free(xold); free(g); free(rhs); free(xn); free(s.tmp);
     return 0;
+}
+
+/* src/bregman.cpp:36-94: d and b start at zero.  aniso=1 replaces the
+ * isotropic shrink by the per-component soft threshold. */
+int or_split_bregman(or_op_fn A, void *ctx, const REAL *y, const REAL *x0,
+                     int64_t n1, int64_t n2, int64_t n3, double lambda,
+                     double mu, int64_t outer, int64_t inner, double tol,
+                     double xtol, int boundary, int aniso, REAL *x,
+                     or_report *rep) {
+    const int64_t n = n1 * n2 * n3;
+    REAL *d = (REAL *)calloc(3 * (size_t)n, sizeof(REAL));
+    REAL *b = (REAL *)calloc(3 * (size_t)n, sizeof(REAL));
+    const int r = or_split_bregman_state(A, ctx, y, x0, n1, n2, n3, lambda, mu, outer,
+                                         inner, tol, xtol, boundary, aniso, d, b, x, rep);
+    free(d);
+    free(b);
+    return r;
 }

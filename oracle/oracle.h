@@ -150,6 +150,21 @@ int or_split_bregman(or_op_fn A, void *ctx, const REAL *y, const REAL *x0,
                      double xtol, int boundary, int aniso, REAL *x_out,
                      or_report *rep);
 
+/* the steps of one Split-Bregman outer iteration (src/bregman.cpp:58-84):
+ * rhs = y - mu div(d - b); d = shrink(g + b, lambda/mu); b += g - d.
+ * d, b, g hold 3 arrays of n values back to back. */
+void or_sb_rhs(const REAL *y, const REAL *d, const REAL *b, int64_t n1,
+               int64_t n2, int64_t n3, int boundary, double mu, REAL *rhs);
+void or_sb_shrink(const REAL *g, const REAL *b, int64_t n, double lambda,
+                  double mu, int aniso, REAL *d);
+void or_sb_bregman_update(REAL *b, const REAL *g, const REAL *d, int64_t n);
+/* or_split_bregman with caller-held d and b (3 n values each, in/out) */
+int or_split_bregman_state(or_op_fn A, void *ctx, const REAL *y, const REAL *x0,
+                           int64_t n1, int64_t n2, int64_t n3, double lambda,
+                           double mu, int64_t outer, int64_t inner, double tol,
+                           double xtol, int boundary, int aniso, REAL *d,
+                           REAL *b, REAL *x_out, or_report *rep);
+
 /* convenience: Toeplitz operator context usable as or_op_fn */
 typedef struct {
     const or_psf *psf;

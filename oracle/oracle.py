@@ -103,6 +103,13 @@ class Oracle:
                                        i64, C.c_double, C.c_double, i64, i64,
                                        C.c_double, C.c_double, C.c_int, C.c_int,
                                        P, C.POINTER(Report)]
+        L.or_split_bregman_state.argtypes = [C.c_void_p, C.c_void_p, P, P, i64, i64,
+                                             i64, C.c_double, C.c_double, i64, i64,
+                                             C.c_double, C.c_double, C.c_int, C.c_int,
+                                             P, P, P, C.POINTER(Report)]
+        L.or_sb_rhs.argtypes = [P, P, P, i64, i64, i64, C.c_int, C.c_double, P]
+        L.or_sb_shrink.argtypes = [P, P, i64, C.c_double, C.c_double, C.c_int, P]
+        L.or_sb_bregman_update.argtypes = [P, P, P, i64]
         L.or_num_threads.restype = C.c_int
         L.or_set_num_threads.argtypes = [C.c_int]
         self._opfn_type = C.CFUNCTYPE(None, C.c_void_p, P, P)
@@ -389,8 +396,35 @@ class Oracle:
                                  y.size, max_iter, tol, xtol, self._p(out), C.byref(rep))
         return out, rep.as_dict()
 
+    # ---- Split-Bregman steps (src/bregman.cpp:58-84) ----
+    def sb_rhs(self, y, d, b, mu, boundary=0):
+        """yT - mu * div(d - b); d, b of shape (3,) + y.shape"""
+        y, d, b = self._r(y), self._r(d), self._r(b)
+        out = np.empty_like(y)
+        self.lib.or_sb_rhs(self._p(y), self._p(d), self._p(b), *_as3(y.shape), boundary,
+                           mu, self._p(out))
+        return out
+
+    def sb_shrink(self, g, b, lam, mu, aniso=False):
+        """d = shrink(g + b, lam/mu); g, b of shape (3, ...)"""
+        g, b = self._r(g), self._r(b)
+        d = np.empty_like(g)
+        self.lib.or_sb_shrink(self._p(g), self._p(b), g[0].size, lam, mu, int(aniso),
+                              self._p(d))
+        return d
+
+    def sb_bregman_update(self, b, g, d):
+        """returns b + g - d"""
+        b = self._r(b).copy()
+        g, d = self._r(g), self._r(d)
+        self.lib.or_sb_bregman_update(self._p(b), self._p(g), self._p(d), b[0].size)
+        return b
+
     def split_bregman(self, A, y, x0, lam, mu, outer, inner, tol, xtol,
-                      boundary=0, aniso=False):
+                      boundary=0, aniso=False, state=None):
+        """src/bregman.cpp:36-94.  state=(d, b): arrays of shape (3,) + y.shape
+        that start the splitting variables and receive their final values
+        (the reference starts them at zero)."""
         y = self._r(y)
         x0 = self._r(x0)
         out = np.empty_like(y)
@@ -405,9 +439,21 @@ class Oracle:
             holder["n"] = y.size
             holder["shape"] = y.shape
             fnp, ctxp = C.cast(keep, C.c_void_p), None
-        self.lib.or_split_bregman(fnp, ctxp, self._p(y), self._p(x0), n1, n2, n3,
-                                  lam, mu, outer, inner, tol, xtol, boundary,
-                                  int(aniso), self._p(out), C.byref(rep))
+        if state is None:
+            self.lib.or_split_bregman(fnp, ctxp, self._p(y), self._p(x0), n1, n2, n3,
+                                      lam, mu, outer, inner, tol, xtol, boundary,
+                                      int(aniso), self._p(out), C.byref(rep))
+        else:
+            d, b = state
+            for a in (d, b):
+                if (not isinstance(a, np.ndarray) or a.dtype != self.dtype
+                        or a.shape != (3,) + y.shape or not a.flags.c_contiguous):
+                    raise ValueError("state arrays must be C-contiguous, of the oracle's "
+                                     "dtype and of shape (3,) + y.shape")
+            self.lib.or_split_bregman_state(fnp, ctxp, self._p(y), self._p(x0), n1, n2, n3,
+                                            lam, mu, outer, inner, tol, xtol, boundary,
+                                            int(aniso), self._p(d), self._p(b),
+                                            self._p(out), C.byref(rep))
         return out, rep.as_dict()
 
 

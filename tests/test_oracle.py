@@ -9,7 +9,7 @@ transposed operand, a wrong normalisation) fails it:
   * Toeplitz == NUFFT normal operator (tests/test_sysmat_toeplitz*.cpp)
   * adjoint / symmetry / PSD (tests/test_adjoint*.cpp)
   * CG on normal equations (tests/test_cg_normal.cpp), diagonal system (tests/test_cg.cu)
-  * shrink / compute_sk closed forms (tests/test_bregman_gpu_cpu.cu)
+  * the Split-Bregman steps: tests/test_oracle_bregman.py
 """
 import os
 
@@ -320,14 +320,6 @@ def test_finite_differences_closed_forms(orc64):
         gg = np.random.default_rng(2).random((3, n1, n2, n3))
         if bd == 1:
             assert np.isclose(np.vdot(orc64.grad(r, bd), gg), -np.vdot(r, orc64.div(gg, bd)))
-
-
-def test_shrink_closed_form(orc64):
-    # tests/test_bregman_gpu_cpu.cu:144-212 formulas through a 1-iteration SB:
-    # with A = 0 the x-update leaves x = 0, so d = shrink(b = 0) = 0 and b stays 0
-    y = np.zeros((3, 4, 4))
-    x, rep = orc64.split_bregman(lambda v: v, y, y, 0.1, 1.0, 1, 1, 1e-5, 1e-5)
-    assert not np.any(x)
 
 
 def test_split_bregman_single_slice_is_restarted_cg(orc64):
