@@ -109,6 +109,21 @@ int lc_sysmat2d_nufft(lc_ctx ctx, lc_geom2d g, const float *x, int64_t nslices, 
  * (src/conjgrad.cpp:33-76).  x holds x0 on entry and the solution on exit. */
 int lc_cg2d(lc_ctx ctx, lc_geom2d g, const float *rhs, float *x, int64_t nslices,
             int64_t max_iter, double tol, double xtol, uint32_t flags, lc_report *rep);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): lc_cg2d with every device buffer of the solve (vectors,
+ * Toeplitz intermediate, tables, partial sums) carved from a fixed array of
+ * LC_NOALLOC_SCRATCH_MIB MiB that the library holds on each GPU until the
+ * process ends; the call makes no hipMalloc or hipFree.  rhs and x are device
+ * pointers (LC_DEVICE_PTRS is required); LC_X0_ZERO as lc_cg2d; LC_GRAPH is
+ * refused.  The geometry must belong to ctx and have its PSF built.  A problem
+ * whose buffers do not fit the array fails with LC_EINVAL before anything runs
+ * (one 1024^2 slice, 8 slices of 512^2 or 32 of 256^2 fit).  Calls on one GPU
+ * share the array: they run one at a time, each overwriting what the last
+ * one left.  Same iterates and result as lc_cg2d. */
+#define LC_NOALLOC_SCRATCH_MIB 64
+int lc_cg2d_noalloc(lc_ctx ctx, lc_geom2d g, const float *rhs, float *x, int64_t nslices,
+                    int64_t max_iter, double tol, double xtol, uint32_t flags, lc_report *rep);
 /* opt::split_bregman (src/bregman.cpp:36-94) with A = sysmat2d(., psf) */
 int lc_split_bregman2d(lc_ctx ctx, lc_geom2d g, const float *rhs, float *x, int64_t nslices,
                        double lambda, double mu, int64_t outer, int64_t inner, double tol,

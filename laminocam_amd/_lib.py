@@ -40,7 +40,7 @@ EXPORTS = [
     "lc_laplacian_f64", "lc_batch_create_ex", "lc_batch2d_set_f64", "lc_batch2d_get_f64",
     "lc_batch2d_device_ptr_f64", "lc_psf2d_kernel_hat_f64", "lc_psf3d_kernel_hat_f64",
     "lc_recon_params_default", "lc_prepare_sinogram", "lc_prepare_sinogram_f64", "lc_mbir2d",
-    "lc_mbir2d_f64", "lc_mbir3d", "lc_mbir3d_f64",
+    "lc_mbir2d_f64", "lc_mbir3d", "lc_mbir3d_f64", "lc_cg2d_noalloc",
 ]
 
 
@@ -110,6 +110,8 @@ def load():
         "lc_sysmat2d_nufft": (C.c_int, [vp, vp, vp, i64, vp, u32]),
         "lc_cg2d": (C.c_int, [vp, vp, vp, vp, i64, i64, C.c_double, C.c_double, u32,
                               C.POINTER(LcReport)]),
+        "lc_cg2d_noalloc": (C.c_int, [vp, vp, vp, vp, i64, i64, C.c_double, C.c_double, u32,
+                                      C.POINTER(LcReport)]),
         "lc_split_bregman2d": (C.c_int, [vp, vp, vp, vp, i64, C.c_double, C.c_double, i64,
                                          i64, C.c_double, C.c_double, u32,
                                          C.POINTER(LcReport)]),

@@ -88,7 +88,14 @@ const char *lc_last_error(void) { return g_err.c_str(); }
 int lc_ctx_create(int device, lc_ctx *out) {
     return guard([&] {
         LC_REQUIRE(out, "lc_ctx_create: out is NULL");
-        *out = new lc_ctx_s(device);
+        auto *c = new lc_ctx_s(device);   // (makes the device current)
+        try {
+            noalloc_setup(device);
+        } catch (...) {
+            delete c;
+            throw;
+        }
+        *out = c;
     });
 }
 
@@ -322,6 +329,18 @@ int lc_cg2d(lc_ctx ctx, lc_geom2d g, const float *rhs, float *x, int64_t nslices
 int lc_cg2d_f64(lc_ctx ctx, lc_geom2d g, const double *rhs, double *x, int64_t nslices,
                 int64_t max_iter, double tol, double xtol, uint32_t flags, lc_report *rep) {
     return cg2d_t(ctx, g, rhs, x, nslices, max_iter, tol, xtol, flags, rep);
+}
+
+int lc_cg2d_noalloc(lc_ctx ctx, lc_geom2d g, const float *rhs, float *x, int64_t nslices,
+                    int64_t max_iter, double tol, double xtol, uint32_t flags, lc_report *rep) {
+    return guard([&] {
+        LC_REQUIRE(ctx && g && rhs && x, "lc_cg2d_noalloc: NULL argument");
+        LC_REQUIRE(nslices >= 1 && nslices <= (1 << 20), "lc_cg2d_noalloc: bad nslices");
+        LC_REQUIRE(g->ctx == ctx, "lc_cg2d_noalloc: geometry of another context");
+        const CgState s = cg2d_noalloc(ctx->c, g->g, rhs, x, (int)nslices, max_iter, tol, xtol,
+                                       flags);
+        if (rep) fill_report(s, rep);
+    });
 }
 
 } // extern "C"

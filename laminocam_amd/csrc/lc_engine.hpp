@@ -356,6 +356,14 @@ class Batch2D {
 void fd_launch(Ctx &c, int op, const float *in, float *out, int n1, int n2, int n3, int boundary);
 void fd_launch(Ctx &c, int op, const double *in, double *out, int n1, int n2, int n3, int boundary);
 
+// allocation-free CG (lc_noalloc.hip): noalloc_setup looks up the device
+// address of the library's fixed scratch array on `device` (current) once;
+// cg2d_noalloc runs lc_cg2d on device pointers with every device buffer
+// carved from that array
+void noalloc_setup(int device);
+CgState cg2d_noalloc(Ctx &c, Geom2D &g, const float *rhs, float *x, int ns, int64_t max_iter,
+                     double tol, double xtol, uint32_t flags);
+
 // whole-solve drivers (lc_driver.hip), T = float or double
 template <typename T>
 void prepare_sinogram(Ctx &c, int32_t nds, const T *const *projs, const int64_t *nang, int64_t nrows,

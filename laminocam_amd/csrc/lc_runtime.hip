@@ -27,6 +27,7 @@ void Ctx::sync() { LC_HIP(hipStreamSynchronize(stream_)); }
 void *Ctx::scratch(int slot, size_t bytes) {
     if ((int)scratch_.size() <= slot) scratch_.resize(slot + 1);
     if (scratch_[slot].bytes() < bytes) {
+        LC_REQUIRE(!tl_arena, "context scratch cannot grow inside an arena");
         LC_HIP(hipStreamSynchronize(stream_));
         scratch_[slot].alloc(bytes);
     }

@@ -63,6 +63,25 @@ class MemPool {
 // allocations made by this thread go through this pool when set
 inline thread_local MemPool *tl_pool = nullptr;
 
+// A fixed device region handed out front to back with no hipMalloc: while
+// tl_arena is set, the DBuf allocations of this thread come from it (256-byte
+// aligned) and are never freed.  A request beyond its end fails with
+// LC_EINVAL.
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+    const char *who = "";
+    void *take(size_t bytes) {
+        const size_t off = (used + 255) & ~(size_t)255;
+        if (off > cap || bytes > cap - off)
+            throw Error(LC_EINVAL, std::string(who) + ": the problem needs more than the " +
+                                       std::to_string(cap >> 20) + " MiB device scratch");
+        used = off + bytes;
+        return base + off;
+    }
+};
+inline thread_local Arena *tl_arena = nullptr;
+
 // Grow-only pinned host buffer (hipHostMalloc): staging for asynchronous
 // host <-> device copies.  Grows by at least 1.5x so that a sequence of
 // batches of similar size reallocates rarely.
@@ -105,20 +124,27 @@ class DBuf {
     ~DBuf() { release(); }
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
-    DBuf(DBuf &&o) noexcept : p_(o.p_), n_(o.n_), cap_(o.cap_), pool_(o.pool_) {
-        o.p_ = nullptr; o.n_ = 0; o.cap_ = 0; o.pool_ = nullptr;
+    DBuf(DBuf &&o) noexcept
+        : p_(o.p_), n_(o.n_), cap_(o.cap_), pool_(o.pool_), arena_(o.arena_) {
+        o.p_ = nullptr; o.n_ = 0; o.cap_ = 0; o.pool_ = nullptr; o.arena_ = false;
     }
     DBuf &operator=(DBuf &&o) noexcept {
         if (this != &o) {
             release();
-            p_ = o.p_; n_ = o.n_; cap_ = o.cap_; pool_ = o.pool_;
-            o.p_ = nullptr; o.n_ = 0; o.cap_ = 0; o.pool_ = nullptr;
+            p_ = o.p_; n_ = o.n_; cap_ = o.cap_; pool_ = o.pool_; arena_ = o.arena_;
+            o.p_ = nullptr; o.n_ = 0; o.cap_ = 0; o.pool_ = nullptr; o.arena_ = false;
         }
         return *this;
     }
     void alloc(size_t bytes) {
         release();
         if (bytes == 0) return;
+        if (tl_arena) {
+            p_ = tl_arena->take(bytes);
+            n_ = cap_ = bytes;
+            arena_ = true;
+            return;
+        }
         if (tl_pool) {
             p_ = tl_pool->get(bytes, cap_);
             pool_ = tl_pool;
@@ -138,13 +164,14 @@ class DBuf {
         if (bytes > n_) alloc(bytes);
     }
     void release() {
-        if (p_) {
+        if (p_ && !arena_) {
             if (pool_) pool_->put(p_, cap_);
             else (void)hipFree(p_);
         }
         p_ = nullptr;
         n_ = cap_ = 0;
         pool_ = nullptr;
+        arena_ = false;
     }
     template <typename T = void> T *as() const { return reinterpret_cast<T *>(p_); }
     size_t bytes() const { return n_; }
@@ -153,6 +180,7 @@ class DBuf {
     void *p_ = nullptr;
     size_t n_ = 0, cap_ = 0;
     MemPool *pool_ = nullptr;
+    bool arena_ = false;   // carved from tl_arena: nothing to free
 };
 
 // Block shapes of the three Toeplitz passes for one image size:
@@ -192,6 +220,10 @@ template <typename T> const cpx<T> *Ctx::twiddles(int L) {
     auto key = std::make_pair(L, (int)sizeof(T));
     auto it = tw_.find(key);
     if (it != tw_.end()) return it->second.as<cpx<T>>();
+    // a cached table must not live in an arena, whose memory the next user overwrites
+    LC_REQUIRE(!tl_arena, "no FFT twiddles of length " + std::to_string(L) +
+                              " prepared on this context (create the geometry on it without "
+                              "LC_GEOM_LAZY_PSF)");
     DBuf b(sizeof(cpx<T>) * (size_t)L);
     hipLaunchKernelGGL(k_twiddles<T>, dim3(ceil_div(L, 256)), dim3(256), 0, stream_,
                        b.as<cpx<T>>(), L);

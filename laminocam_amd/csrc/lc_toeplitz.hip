@@ -250,6 +250,10 @@ void geom2d_build_psf(Ctx &c, Geom2D &g, bool f64) {
                            M, H, B, plan, tw, scale);
     }
     LC_HIP(hipGetLastError());
+    // the apply's FFT twiddles, so that a solve on this context finds them
+    // without allocating
+    if (f64) (void)c.twiddles<double>((int)g.M);
+    else (void)c.twiddles<float>((int)g.M);
     // P lives in a context scratch slot that the next call may reuse
     c.sync();
     if (f64) g.psf64_ready = true;
