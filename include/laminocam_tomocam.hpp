@@ -32,16 +32,20 @@
  * as the reference's double instantiations do; the NUFFT projectors compute in
  * FP32 for either type, double arrays being converted at the boundary.
  *
- * Solvers: opt::Function<T> is constructible from any callable, like the
- * reference's std::function alias.  cgsolver / split_bregman run on the device
- * when the operator is a Toeplitz PSF (opt::Function<T>(psf) or
- * opt::toeplitz(psf)); an operator known only as an opaque callable cannot be
- * moved to the device and is rejected with std::invalid_argument.
+ * Solvers: opt::Function<T> is the reference's std::function alias.  cgsolver
+ * and split_bregman run the whole iteration on the device when the function
+ * object holds an opt::Toeplitz2D / opt::Toeplitz3D (made by
+ * opt::toeplitz(psf)); any other callable, such as the reference's own
+ * [&psf](x) { return sysmat2d(x, psf); }, is applied as given by a host loop
+ * that follows src/conjgrad.cpp:33-76 and src/bregman.cpp:36-94, with FP64
+ * inner products and norms.
  */
 #ifndef LAMINOCAM_TOMOCAM_HPP
 #define LAMINOCAM_TOMOCAM_HPP
 
+#include <algorithm>
 #include <array>
+#include <cmath>
 #include <complex>
 #include <cstdint>
 #include <cstring>
@@ -83,10 +87,60 @@ template <typename T> class Array {
     T &operator[](size_t i) { return v_[i]; }
     const T &operator[](size_t i) const { return v_[i]; }
 
+    // the element-wise arithmetic of include/array.h:146-235
+    Array clone() const { return *this; }
+    static Array ones(dims_t d) {
+        Array a(d);
+        for (auto &v : a.v_) v = T(1);
+        return a;
+    }
+    Array &operator*=(T s) {
+        for (auto &v : v_) v *= s;
+        return *this;
+    }
+    Array operator*(T s) const { return Array(*this) *= s; }
+    Array &operator/=(T s) {
+        if (s == T(0)) throw std::runtime_error("Division by zero in Array<T>::operator/=");
+        for (auto &v : v_) v /= s;
+        return *this;
+    }
+    Array operator/(T s) const { return Array(*this) /= s; }
+    Array &operator+=(const Array &o) {
+        check_(o);
+        for (size_t i = 0; i < v_.size(); ++i) v_[i] += o.v_[i];
+        return *this;
+    }
+    Array operator+(const Array &o) const { return Array(*this) += o; }
+    Array &operator-=(const Array &o) {
+        check_(o);
+        for (size_t i = 0; i < v_.size(); ++i) v_[i] -= o.v_[i];
+        return *this;
+    }
+    Array operator-(const Array &o) const { return Array(*this) -= o; }
+    Array &operator*=(const Array &o) {
+        check_(o);
+        for (size_t i = 0; i < v_.size(); ++i) v_[i] *= o.v_[i];
+        return *this;
+    }
+    Array operator*(const Array &o) const { return Array(*this) *= o; }
+
   private:
+    void check_(const Array &o) const {
+        if (!(o.dims_ == dims_)) throw std::invalid_argument("Array: dimensions differ");
+    }
     dims_t dims_;
     std::vector<T> v_;
 };
+
+// include/array_ops.h:63-82: reductions (accumulated in FP64 here)
+namespace array {
+template <typename T> T dot(const Array<T> &a, const Array<T> &b) {
+    double s = 0.0;
+    for (size_t i = 0; i < a.size(); ++i) s += (double)a[i] * (double)b[i];
+    return (T)s;
+}
+template <typename T> T norm2(const Array<T> &a) { return (T)std::sqrt((double)dot(a, a)); }
+} // namespace array
 } // namespace tomocam
 #endif
 
@@ -212,9 +266,14 @@ struct Geom3 {
 
 namespace tomocam::cpu {
 
-// include/polar_grid2d.h:35-66.  The points x = r cos(theta), y = r sin(theta)
-// are generated on the device; the angles and the radial count are kept.
+// include/polar_grid2d.h:35-66.  The device generates the points
+// x = r cos(theta), y = r sin(theta), r = (k + 1/2) 2 pi / ncols - pi, in its
+// kernels from the angle table; the members npts, x, y hold the same points on
+// the host, computed in T, for callers that read them.
 template <typename T> struct PolarGrid2D {
+    size_t npts = 0;
+    Array<T> x;
+    Array<T> y;
     std::vector<double> theta;
     size_t ncols_ = 0;
     size_t nangles() const { return theta.size(); }
@@ -222,7 +281,20 @@ template <typename T> struct PolarGrid2D {
     dims_t dims() const { return dims_t{1, theta.size(), ncols_}; }
     size_t size() const { return theta.size() * ncols_; }
     PolarGrid2D() = default;
-    PolarGrid2D(const std::vector<T> &th, size_t ncols) : theta(th.begin(), th.end()), ncols_(ncols) {}
+    PolarGrid2D(const std::vector<T> &th, size_t ncols)
+        : x(dims_t{1, th.size(), ncols}), y(dims_t{1, th.size(), ncols}), theta(th.begin(), th.end()),
+          ncols_(ncols) {
+        npts = th.size() * ncols;
+        const T dr = T(2 * M_PI) / static_cast<T>(ncols);
+        for (size_t i = 0; i < th.size(); ++i) {
+            const T c = std::cos(th[i]), s = std::sin(th[i]);
+            for (size_t k = 0; k < ncols; ++k) {
+                const T r = T(k + 0.5) * dr - T(M_PI);
+                x.data()[i * ncols + k] = c * r;
+                y.data()[i * ncols + k] = s * r;
+            }
+        }
+    }
 
     // device geometry for N x N images (PSF built on demand)
     lc_geom2d geometry(size_t N) const {
@@ -240,7 +312,16 @@ template <typename T> struct PolarGrid2D {
 };
 
 // include/polar_grid.h:35-94: points R_i (qX, qY, 0), R_i = RotationTranspose(theta, gamma, beta)
+// (include/rotation.h:28-56), qX = (k + 1/2) 2 pi / ncols - pi, qY likewise over
+// nrows; w = 0 where a coordinate exceeds pi in magnitude, else 1.  The device
+// forms the two in-plane columns of R_i in FP64 and generates the points in
+// its kernels; npts, x, y, z, w hold them on the host, computed in T.
 template <typename T> struct PolarGrid {
+    size_t npts = 0;
+    Array<T> x;
+    Array<T> y;
+    Array<T> z;
+    Array<T> w;
     std::vector<double> theta, gamma, beta;
     size_t nrows_ = 0, ncols_ = 0;
     dims_t dims() const { return dims_t{theta.size(), nrows_, ncols_}; }
@@ -252,6 +333,35 @@ template <typename T> struct PolarGrid {
           nrows_(nrows), ncols_(ncols) {
         if (ga.size() != th.size() || be.size() != th.size())
             throw std::invalid_argument("PolarGrid: theta, gamma and beta sizes differ");
+        const dims_t d{th.size(), nrows, ncols};
+        x = Array<T>(d);
+        y = Array<T>(d);
+        z = Array<T>(d);
+        w = Array<T>(d);
+        npts = d.size();
+        const T dX = T(2 * M_PI) / static_cast<T>(ncols), dY = T(2 * M_PI) / static_cast<T>(nrows);
+        for (size_t i = 0; i < th.size(); ++i) {
+            const T ct = std::cos(th[i]), st = std::sin(th[i]);
+            const T cb = std::cos(be[i]), sb = std::sin(be[i]);
+            const T cg = std::cos(ga[i]), sg = std::sin(ga[i]);
+            // columns 0 and 1 of the transposed rotation; q has no third component
+            const T c0[3] = {cb * cg, -cb * sg, sb};
+            const T c1[3] = {ct * sg + st * sb * cg, ct * cg - st * sb * sg, -st * cb};
+            for (size_t j = 0; j < nrows; ++j) {
+                const T qY = T(j + 0.5) * dY - T(M_PI);
+                for (size_t k = 0; k < ncols; ++k) {
+                    const T qX = T(k + 0.5) * dX - T(M_PI);
+                    const size_t o = (i * nrows + j) * ncols + k;
+                    const T px = c0[0] * qX + c1[0] * qY, py = c0[1] * qX + c1[1] * qY,
+                            pz = c0[2] * qX + c1[2] * qY;
+                    x.data()[o] = px;
+                    y.data()[o] = py;
+                    z.data()[o] = pz;
+                    const T pi = T(M_PI);
+                    w.data()[o] = (std::abs(px) > pi || std::abs(py) > pi || std::abs(pz) > pi) ? T(0) : T(1);
+                }
+            }
+        }
     }
 
     lc_geom3d geometry(dims_t rd, bool lazy = true) const {
@@ -457,79 +567,39 @@ template <typename T> Array<T> sysmat(const Array<T> &x, const cpu::PointSpreadF
 
 namespace tomocam::opt {
 
-// include/optimize.h:31-32: a callable Array -> Array.  It also records the
-// Toeplitz PSF it applies when built from one, which lets the solvers run the
-// whole iteration on the device.
-template <typename T> class Function {
-  public:
-    Function() = default;
-    template <typename F,
-              typename = typename std::enable_if<!std::is_same<typename std::decay<F>::type, Function>::value>::type>
-    Function(F &&f) : fn_(std::forward<F>(f)) {}
-    Function(const cpu::PointSpreadFunction2D<T> &psf)
-        : fn_([psf](const Array<T> &x) { return psf.convolve(x); }), psf2_(psf), kind_(2) {}
-    Function(const cpu::PointSpreadFunction<T> &psf)
-        : fn_([psf](const Array<T> &x) { return psf.convolve(x); }), psf3_(psf), kind_(3) {}
-    Array<T> operator()(const Array<T> &x) const { return fn_(x); }
-    int kind() const { return kind_; }
-    const cpu::PointSpreadFunction2D<T> &psf2() const { return psf2_; }
-    const cpu::PointSpreadFunction<T> &psf3() const { return psf3_; }
+// include/optimize.h:31-61
+template <typename T> using Function = std::function<Array<T>(const Array<T> &)>;
+template <typename T> using Residual = std::function<T(const Array<T> &)>;
 
-  private:
-    std::function<Array<T>(const Array<T> &)> fn_;
-    cpu::PointSpreadFunction2D<T> psf2_;
-    cpu::PointSpreadFunction<T> psf3_;
-    int kind_ = 0;   // 0 opaque callable, 2 / 3 Toeplitz PSF
+enum class Optimizer { SPLIT_BREGMAN, CG_SOLVER };
+
+template <typename T> struct OptimizerConfig {
+    Optimizer method = Optimizer::SPLIT_BREGMAN;
+    T lambda = T(0.1);
+    T mu = T(5.0);
+    T tol = T(1e-5);
+    T xtol = T(1e-5);
+    size_t outer_max = 50;
+    size_t inner_max = 4;
+    OptimizerConfig() = default;
+    OptimizerConfig(Optimizer meth, T lam, T m, size_t out_max, size_t in_max, T tol1, T tol2)
+        : method(meth), lambda(lam), mu(m), tol(tol1), xtol(tol2), outer_max(out_max), inner_max(in_max) {}
 };
 
-template <typename T> Function<T> toeplitz(const cpu::PointSpreadFunction2D<T> &psf) { return Function<T>(psf); }
-template <typename T> Function<T> toeplitz(const cpu::PointSpreadFunction<T> &psf) { return Function<T>(psf); }
-
-namespace detail {
-template <typename T> void need_psf(const Function<T> &A, const char *who) {
-    if (A.kind() == 0)
-        throw std::invalid_argument(std::string(who) +
-                                    ": the operator is an opaque callable; build it from the "
-                                    "PointSpreadFunction (opt::toeplitz(psf)) to solve on the device");
-}
-} // namespace detail
-
-// include/optimize.h:89-91, src/conjgrad.cpp:33-76
-template <typename T>
-Array<T> cgsolver(const Function<T> &A, const Array<T> &y, const Array<T> &x, size_t max_iter, T tol, T xtol) {
-    detail::need_psf(A, "cgsolver");
-    Array<T> out(x.dims());
-    std::memcpy(out.data(), x.data(), sizeof(T) * x.size());
-    lc_report rep;
-    if (A.kind() == 2)
-        laminocam::check(laminocam::cg2d(laminocam::context(), A.psf2().handle(), y.data(), out.data(),
-                                         (int64_t)y.dims().n1, (int64_t)max_iter, (double)tol, (double)xtol,
-                                         &rep));
-    else
-        laminocam::check(laminocam::cg3d(laminocam::context(), A.psf3().handle(), y.data(), out.data(),
-                                         (int64_t)max_iter, (double)tol, (double)xtol, &rep));
-    return out;
-}
-
-// include/optimize.h:74-77, src/bregman.cpp:36-94 (isotropic shrink,
-// interior-only differences as the reference CPU path)
-template <typename T>
-Array<T> split_bregman(const Function<T> &A, const Array<T> &y, const Array<T> &x0, T lambda, T mu,
-                       size_t outer_max, size_t inner_max, T tol, T xtol) {
-    detail::need_psf(A, "split_bregman");
-    Array<T> out(x0.dims());
-    std::memcpy(out.data(), x0.data(), sizeof(T) * x0.size());
-    lc_report rep;
-    if (A.kind() == 2)
-        laminocam::check(laminocam::sb2d(laminocam::context(), A.psf2().handle(), y.data(), out.data(),
-                                         (int64_t)y.dims().n1, (double)lambda, (double)mu, (int64_t)outer_max,
-                                         (int64_t)inner_max, (double)tol, (double)xtol, &rep));
-    else
-        laminocam::check(laminocam::sb3d(laminocam::context(), A.psf3().handle(), y.data(), out.data(),
-                                         (double)lambda, (double)mu, (int64_t)outer_max, (int64_t)inner_max,
-                                         (double)tol, (double)xtol, &rep));
-    return out;
-}
+// The Toeplitz normal operator x -> sysmat(x, psf) as a function object.  A
+// Function<T> holding one of these is recognised by cgsolver and
+// split_bregman (std::function::target), which then keep the iteration on the
+// device.
+template <typename T> struct Toeplitz2D {
+    cpu::PointSpreadFunction2D<T> psf;
+    Array<T> operator()(const Array<T> &x) const { return psf.convolve(x); }
+};
+template <typename T> struct Toeplitz3D {
+    cpu::PointSpreadFunction<T> psf;
+    Array<T> operator()(const Array<T> &x) const { return psf.convolve(x); }
+};
+template <typename T> Toeplitz2D<T> toeplitz(const cpu::PointSpreadFunction2D<T> &psf) { return {psf}; }
+template <typename T> Toeplitz3D<T> toeplitz(const cpu::PointSpreadFunction<T> &psf) { return {psf}; }
 
 // include/bregman.h:34-55, src/finitediff.cpp:26-90 (interior-only differences)
 template <typename T> std::array<Array<T>, 3> grad_u(const Array<T> &u) {
@@ -560,6 +630,175 @@ template <typename T> Array<T> laplacian(const Array<T> &u) {
                                    (int64_t)d.n3, out.data()));
     return out;
 }
+namespace detail {
+template <typename T> double dot(const T *a, const T *b, size_t n) {
+    double s = 0.0;
+    for (size_t i = 0; i < n; ++i) s += (double)a[i] * (double)b[i];
+    return s;
+}
+
+template <typename T> Array<T> call_op(const Function<T> &A, const Array<T> &x) {
+    if (!A) throw std::invalid_argument("the operator is an empty function");
+    Array<T> y = A(x);
+    if (y.size() != x.size()) throw std::invalid_argument("the operator changed the array size");
+    return y;
+}
+
+// src/conjgrad.cpp:33-76 on the host, for an operator given only as a
+// callable: r = y - A x, p = r; per iteration alpha = <r,r>/<p,Ap> (stop when
+// |<p,Ap>| < 1e-10), x += alpha p, r -= alpha Ap, p = r + (<r',r'>/<r,r>) p;
+// stop when ||r||/(||y||+1e-8) < tol or ||alpha p||/(||x||+1e-8) < xtol.
+// Inner products and norms accumulate in FP64.
+template <typename T>
+Array<T> cg_host(const Function<T> &A, const Array<T> &y, const Array<T> &x0, size_t max_iter, T tol, T xtol) {
+    const size_t n = y.size();
+    if (x0.size() != n) throw std::invalid_argument("cgsolver: x0 and y differ in size");
+    Array<T> x(x0.dims());
+    std::memcpy(x.data(), x0.data(), sizeof(T) * n);
+    Array<T> r = call_op(A, x);
+    for (size_t i = 0; i < n; ++i) r.data()[i] = y.data()[i] - r.data()[i];
+    const double ynorm = std::sqrt(dot(y.data(), y.data(), n));
+    Array<T> p(x0.dims());
+    std::memcpy(p.data(), r.data(), sizeof(T) * n);
+    double rs_old = dot(r.data(), r.data(), n);
+    for (size_t it = 0; it < max_iter; ++it) {
+        const Array<T> Ap = call_op(A, p);
+        const double pAp = dot(p.data(), Ap.data(), n);
+        if (std::fabs(pAp) < 1.0e-10) break;
+        const double alpha = rs_old / pAp;
+        double step2 = 0.0;
+        T *xp = x.data(), *rp = r.data(), *pp = p.data();
+        const T *ap = Ap.data();
+        for (size_t i = 0; i < n; ++i) {
+            const T st = (T)(pp[i] * alpha);
+            step2 += (double)st * st;
+            xp[i] += st;
+            rp[i] -= (T)(ap[i] * alpha);
+        }
+        const double rs_new = dot(rp, rp, n);
+        const double beta = rs_new / rs_old;
+        for (size_t i = 0; i < n; ++i) pp[i] = rp[i] + (T)(pp[i] * beta);
+        rs_old = rs_new;
+        const double res = std::sqrt(rs_new) / (ynorm + 1e-8);
+        const double dx = std::sqrt(step2) / (std::sqrt(dot(xp, xp, n)) + 1e-8);
+        if (res < (double)tol || dx < (double)xtol) break;
+    }
+    return x;
+}
+
+// src/bregman.cpp:36-94 on the host around an operator given only as a
+// callable; the finite differences run on the device (grad_u, divergence,
+// laplacian above).  Per outer iteration: rhs = y - mu div(d - b);
+// x = CG(A - mu laplacian, rhs, x, inner_max); g = grad_u(x);
+// sk = sqrt(sum_c (g_c + b_c)^2); d_c = max(0, sk - lambda/mu) (g_c + b_c) /
+// (sk + 1e-8); b += g - d; stop when ||x - x_old|| / (||x_old|| + 1e-8) < xtol.
+template <typename T>
+Array<T> sb_host(const Function<T> &A, const Array<T> &y, const Array<T> &x0, T lambda, T mu, size_t outer_max,
+                 size_t inner_max, T tol, T xtol) {
+    const dims_t dm = x0.dims();
+    const size_t n = x0.size();
+    if (y.size() != n) throw std::invalid_argument("split_bregman: x0 and y differ in size");
+    Array<T> x(dm), x_old(dm);
+    std::memcpy(x.data(), x0.data(), sizeof(T) * n);
+    std::memcpy(x_old.data(), x0.data(), sizeof(T) * n);
+    std::array<Array<T>, 3> d{Array<T>(dm), Array<T>(dm), Array<T>(dm)};
+    std::array<Array<T>, 3> b{Array<T>(dm), Array<T>(dm), Array<T>(dm)};
+    for (int c = 0; c < 3; ++c) {
+        std::fill(d[c].data(), d[c].data() + n, T(0));
+        std::fill(b[c].data(), b[c].data() + n, T(0));
+    }
+    const Function<T> Ap = [&A, mu](const Array<T> &u) {
+        Array<T> out = call_op(A, u);
+        const Array<T> lap = laplacian(u);
+        for (size_t i = 0; i < out.size(); ++i) out.data()[i] -= lap.data()[i] * mu;
+        return out;
+    };
+    const T thr = lambda / mu;
+    for (size_t it = 0; it < outer_max; ++it) {
+        std::array<Array<T>, 3> dmb{Array<T>(dm), Array<T>(dm), Array<T>(dm)};
+        for (int c = 0; c < 3; ++c)
+            for (size_t i = 0; i < n; ++i) dmb[c].data()[i] = d[c].data()[i] - b[c].data()[i];
+        const Array<T> dv = divergence(dmb);
+        Array<T> rhs(dm);
+        for (size_t i = 0; i < n; ++i) rhs.data()[i] = y.data()[i] - dv.data()[i] * mu;
+        Array<T> xn = cg_host(Ap, rhs, x, inner_max, tol, xtol);
+        std::memcpy(x.data(), xn.data(), sizeof(T) * n);
+        const std::array<Array<T>, 3> g = grad_u(x);
+        for (size_t i = 0; i < n; ++i) {
+            T v[3];
+            double ss = 0.0;
+            for (int c = 0; c < 3; ++c) {
+                v[c] = g[c].data()[i] + b[c].data()[i];
+                ss += (double)v[c] * v[c];
+            }
+            const T sk = (T)std::sqrt(ss);
+            const T sh = sk - thr > T(0) ? sk - thr : T(0);
+            for (int c = 0; c < 3; ++c) d[c].data()[i] = sh * (v[c] / (sk + T(1e-8)));
+        }
+        for (int c = 0; c < 3; ++c)
+            for (size_t i = 0; i < n; ++i) b[c].data()[i] += g[c].data()[i] - d[c].data()[i];
+        double dn = 0.0, on = 0.0;
+        for (size_t i = 0; i < n; ++i) {
+            const double df = (double)x.data()[i] - (double)x_old.data()[i];
+            dn += df * df;
+            on += (double)x_old.data()[i] * x_old.data()[i];
+        }
+        std::memcpy(x_old.data(), x.data(), sizeof(T) * n);
+        if (std::sqrt(dn) / (std::sqrt(on) + 1e-8) < (double)xtol) break;
+    }
+    return x;
+}
+} // namespace detail
+
+// The operator parameter is a non-deduced context: T comes from the arrays, so
+// a lambda or an opt::toeplitz(psf) object converts to Function<T> in place.
+// The function type is the reference's all the same.
+template <typename T> struct same_type { using type = T; };
+template <typename T> using OperatorArg = typename same_type<Function<T>>::type;
+
+// include/optimize.h:89-91, src/conjgrad.cpp:33-76
+template <typename T>
+Array<T> cgsolver(const OperatorArg<T> &A, const Array<T> &y, const Array<T> &x, size_t max_iter, T tol, T xtol) {
+    const auto *t2 = A.template target<Toeplitz2D<T>>();
+    const auto *t3 = A.template target<Toeplitz3D<T>>();
+    if (!t2 && !t3) return detail::cg_host(A, y, x, max_iter, tol, xtol);
+    if (x.size() != y.size()) throw std::invalid_argument("cgsolver: x0 and y differ in size");
+    Array<T> out(x.dims());
+    std::memcpy(out.data(), x.data(), sizeof(T) * x.size());
+    lc_report rep;
+    if (t2)
+        laminocam::check(laminocam::cg2d(laminocam::context(), t2->psf.handle(), y.data(), out.data(),
+                                         (int64_t)y.dims().n1, (int64_t)max_iter, (double)tol, (double)xtol,
+                                         &rep));
+    else
+        laminocam::check(laminocam::cg3d(laminocam::context(), t3->psf.handle(), y.data(), out.data(),
+                                         (int64_t)max_iter, (double)tol, (double)xtol, &rep));
+    return out;
+}
+
+// include/optimize.h:74-77, src/bregman.cpp:36-94 (isotropic shrink,
+// interior-only differences as the reference CPU path)
+template <typename T>
+Array<T> split_bregman(const OperatorArg<T> &A, const Array<T> &y, const Array<T> &x0, T lambda, T mu,
+                       size_t outer_max, size_t inner_max, T tol, T xtol) {
+    const auto *t2 = A.template target<Toeplitz2D<T>>();
+    const auto *t3 = A.template target<Toeplitz3D<T>>();
+    if (!t2 && !t3) return detail::sb_host(A, y, x0, lambda, mu, outer_max, inner_max, tol, xtol);
+    if (x0.size() != y.size()) throw std::invalid_argument("split_bregman: x0 and y differ in size");
+    Array<T> out(x0.dims());
+    std::memcpy(out.data(), x0.data(), sizeof(T) * x0.size());
+    lc_report rep;
+    if (t2)
+        laminocam::check(laminocam::sb2d(laminocam::context(), t2->psf.handle(), y.data(), out.data(),
+                                         (int64_t)y.dims().n1, (double)lambda, (double)mu, (int64_t)outer_max,
+                                         (int64_t)inner_max, (double)tol, (double)xtol, &rep));
+    else
+        laminocam::check(laminocam::sb3d(laminocam::context(), t3->psf.handle(), y.data(), out.data(),
+                                         (double)lambda, (double)mu, (int64_t)outer_max, (int64_t)inner_max,
+                                         (double)tol, (double)xtol, &rep));
+    return out;
+}
+
 } // namespace tomocam::opt
 
 // ---------------------------------------------------------------------------

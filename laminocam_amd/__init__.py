@@ -618,6 +618,44 @@ class Batch2D:
             pass
 
 
+def _request_arrays(psf, sino, rhs, x0):
+    """Shapes of one server request, checked before any pointer reaches the
+    library, which copies (or, for a borrowed request, later reads) exactly
+    nslices x the geometry's sizes from each array.  Returns (nslices, the
+    solution's shape, {field: contiguous float32 array})."""
+    if (sino is None) == (rhs is None):
+        raise ValueError("give exactly one of sino and rhs")
+    arrays = {}
+    if isinstance(psf, PointSpreadFunction):
+        shape = tuple(psf.recon_dims)
+        ns = shape[0]
+    else:
+        a = np.asarray(sino if sino is not None else rhs)
+        ns = int(a.shape[0]) if a.ndim == 3 else 1
+        shape = (ns, psf.image_size, psf.image_size)
+    if sino is not None:
+        s = _f32(sino)
+        if isinstance(psf, PointSpreadFunction):
+            want = tuple(psf.grid.dims())
+        else:
+            want = (ns, psf.grid.nangles(), psf.grid.nradial())
+        got = s.shape if s.ndim == 3 else (1,) + s.shape
+        if tuple(got) != want:
+            raise ValueError(f"sinogram shape {s.shape} does not match {want}")
+        arrays["sino"] = s
+    else:
+        y = _f32(rhs)
+        if y.size != int(np.prod(shape)):
+            raise ValueError(f"rhs size {y.size} does not match {shape}")
+        arrays["rhs"] = y
+    if x0 is not None:
+        x = _f32(x0)
+        if x.size != int(np.prod(shape)):
+            raise ValueError(f"x0 size {x.size} does not match {shape}")
+        arrays["x0"] = x
+    return ns, shape, arrays
+
+
 class Server:
     """Reconstruction server: continuous batching with HBM-budget admission
     over `lanes` device lanes (lc_server_* in laminocam.h).  submit() copies
@@ -642,37 +680,22 @@ class Server:
         """psf: PointSpreadFunction2D (a stack) or PointSpreadFunction (a volume).
         sino: sinogram {ns, nang, nrad} / projections {nproj, nrows, ncols}
         (back-projected on the device), or rhs: the right-hand side."""
+        ns, shape, arrays = _request_arrays(psf, sino, rhs, x0)
         r = _lib.LcRequest()
-        keep = []
         if isinstance(psf, PointSpreadFunction):
             r.g3 = psf.handle.value
-            shape = psf.recon_dims
-            ns = shape[0]
         else:
             r.g2 = psf.handle.value
-            a = sino if sino is not None else rhs
-            ns = int(np.asarray(a).shape[0]) if np.asarray(a).ndim == 3 else 1
-            shape = (ns, psf.image_size, psf.image_size)
         r.nslices = ns
         r.solver = 0 if solver == "cg" else 1
         r.max_iter, r.outer, r.inner = int(max_iter), int(outer), int(inner)
         r.tol, r.xtol, r.lam, r.mu = float(tol), float(xtol), float(lam), float(mu)
         r.flags = (LC_TV_ZERO_PAD if zero_pad else 0) | (LC_TV_ANISO if aniso else 0) | \
             (_lib.LC_REQ_BORROW if borrow else 0)
-        if sino is not None:
-            s = _f32(sino)
-            keep.append(s)
-            r.sino = s.ctypes.data
-        else:
-            y = _f32(rhs)
-            if y.size != int(np.prod(shape)):
-                raise ValueError(f"rhs size {y.size} does not match {shape}")
-            keep.append(y)
-            r.rhs = y.ctypes.data
-        if x0 is not None:
-            x = _f32(x0)
-            keep.append(x)
-            r.x0 = x.ctypes.data
+        keep = []
+        for name, a in arrays.items():
+            keep.append(a)
+            setattr(r, name, a.ctypes.data)
         out = np.empty(shape, np.float32)
         r.x_out = out.ctypes.data
         t = C.c_uint64()

@@ -68,21 +68,26 @@ def build(verbose=False, jobs=None):
     return LIB
 
 
-CXX_TEST_SRC = os.path.join(HERE, "..", "tests", "cxx", "test_shim.cpp")
-CXX_TEST_BIN = os.path.join(HERE, "..", "tests", "cxx", "_bin", "test_shim")
+CXX_DIR = os.path.join(HERE, "..", "tests", "cxx")
+CXX_TESTS = ("test_shim", "test_host_api")
+CXX_TEST_BIN = os.path.join(CXX_DIR, "_bin", "test_shim")
 
 
 def build_cxx_test():
-    """g++ build of the C++ API test program against the in-tree library."""
-    os.makedirs(os.path.dirname(CXX_TEST_BIN), exist_ok=True)
+    """g++ build of the C++ API test programs against the in-tree library."""
+    os.makedirs(os.path.join(CXX_DIR, "_bin"), exist_ok=True)
     cxx = os.environ.get("CXX") or shutil.which("g++") or "g++"
-    cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-I", os.path.join(HERE, "..", "include"),
-           CXX_TEST_SRC, "-o", CXX_TEST_BIN, "-L", LIBDIR, "-llaminocam",
-           "-Wl,-rpath,$ORIGIN/../../../laminocam_amd/lib"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"C++ test build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
-    return CXX_TEST_BIN
+    out = []
+    for name in CXX_TESTS:
+        dst = os.path.join(CXX_DIR, "_bin", name)
+        cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-I", os.path.join(HERE, "..", "include"),
+               os.path.join(CXX_DIR, name + ".cpp"), "-o", dst, "-L", LIBDIR, "-llaminocam",
+               "-Wl,-rpath,$ORIGIN/../../../laminocam_amd/lib"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"C++ test build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        out.append(dst)
+    return out
 
 
 if __name__ == "__main__":

@@ -59,3 +59,43 @@ def test_struct_layouts_match_the_header(tmp_path):
         assert got[(cname, "size")] == C.sizeof(py), cname
         for f, _ in py._fields_:
             assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+
+
+def test_server_request_shapes_are_checked_before_the_library_reads_them():
+    # the library copies (or, borrowed, later reads) nslices x the geometry's
+    # sizes from each array: every wrong shape must stop at the host check
+    import numpy as np
+    import pytest
+    import laminocam_amd as lc
+
+    g2 = lc.PolarGrid2D(np.linspace(0, np.pi, 30, endpoint=False), 16)
+    psf2 = object.__new__(lc.PointSpreadFunction2D)   # fields the check reads; no device
+    psf2.grid, psf2.image_size = g2, 16
+    g3 = lc.PolarGrid(np.linspace(0, 2 * np.pi, 12, endpoint=False), -0.5, 0.0, 8, 16)
+    psf3 = object.__new__(lc.PointSpreadFunction)
+    psf3.grid, psf3.recon_dims = g3, (4, 16, 16)
+
+    ns, shape, arr = lc._request_arrays(psf2, np.zeros((3, 30, 16)), None, None)
+    assert ns == 3 and shape == (3, 16, 16) and arr["sino"].dtype == np.float32
+    ns, shape, arr = lc._request_arrays(psf2, np.zeros((30, 16)), None, np.zeros((16, 16)))
+    assert ns == 1 and set(arr) == {"sino", "x0"}
+    ns, shape, arr = lc._request_arrays(psf3, np.zeros((12, 8, 16)), None, None)
+    assert ns == 4 and shape == (4, 16, 16)
+    ns, shape, arr = lc._request_arrays(psf3, None, np.zeros((4, 16, 16)), np.zeros((4, 16, 16)))
+    assert set(arr) == {"rhs", "x0"}
+
+    bad = [
+        (psf2, np.zeros((3, 29, 16)), None, None),        # one angle short
+        (psf2, np.zeros((3, 30, 15)), None, None),        # one bin short
+        (psf2, np.zeros((3, 30, 16)), None, np.zeros((2, 16, 16))),   # x0 of 2 slices
+        (psf2, None, np.zeros((3, 16, 15)), None),        # rhs too small
+        (psf3, np.zeros((12, 8, 15)), None, None),        # projections too small
+        (psf3, np.zeros((11, 8, 16)), None, None),        # one projection short
+        (psf3, None, np.zeros((4, 16, 16)), np.zeros((4, 16, 15))),   # x0 too small
+        (psf3, None, np.zeros((3, 16, 16)), None),        # rhs of 3 slices
+        (psf2, None, None, None),                         # no input
+        (psf2, np.zeros((3, 30, 16)), np.zeros((3, 16, 16)), None),   # both inputs
+    ]
+    for case in bad:
+        with pytest.raises(ValueError):
+            lc._request_arrays(*case)
