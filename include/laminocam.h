@@ -45,6 +45,9 @@ extern "C" {
 #define LC_REQ_BORROW 0x80u  /* lc_request: sino / rhs / x0 are read in place when the request
                                 runs instead of copied at submit; they must stay valid and
                                 unchanged until the request finishes */
+#define LC_REQ_TEST_FAIL 0x100u /* lc_request, a testing aid: the lane raises a host error when
+                                   it builds a batch holding this request, before any device
+                                   work, so that per-request failure isolation can be tested */
 
 typedef struct lc_ctx_s *lc_ctx;
 typedef struct lc_geom2d_s *lc_geom2d;
@@ -251,8 +254,13 @@ int lc_batch2d_profile(lc_batch2d b, int32_t reps, int32_t max_rows, double *row
  * each of `lanes` device lanes (own stream, own thread) takes every queued
  * request that fits the free budget and shares the head request's solver
  * settings, runs them as one batch from captured graphs, and releases their
- * memory.  Lanes overlap on the device.  Each problem keeps its own solver
- * scalars: its result is the same alone and in any mix. */
+ * memory.  When the head request does not fit, smaller requests behind it
+ * may start, for at most 8 batches; then the lanes wait until it fits.
+ * Lanes overlap on the device.  Each problem keeps its own solver scalars:
+ * its result is the same alone and in any mix.  A batch that fails is rerun
+ * one request at a time, so only a request that fails on its own fails; a
+ * request whose solution holds a NaN or an infinity fails too.  A failed
+ * request reports its message through lc_server_result (LC_EINTERNAL). */
 typedef struct lc_server_s *lc_server;
 typedef struct {
     lc_geom2d g2;          /* a 2-D stack of nslices slices on g2, or */
@@ -286,6 +294,11 @@ int lc_server_result(lc_server s, uint64_t ticket, float *x, lc_report *rep, dou
 /* stats[10] = in-flight bytes, queued, running, finished, batches, budget, and
  * summed lane seconds building batches, loading inputs, solving, reading back */
 int lc_server_stats(lc_server s, double *stats);
+/* the most recent batches (up to max_rows, oldest first, the server keeps the
+ * last 65536), 6 doubles each: lane, requests, host start and end (seconds,
+ * steady clock), device start and end of the solve (ms since the server was
+ * created, from HIP events on the lane's stream) */
+int lc_server_timeline(lc_server s, int32_t max_rows, double *rows, int32_t *nrows);
 /* finishes every queued request, then stops the lanes */
 int lc_server_destroy(lc_server s);
 

@@ -676,10 +676,12 @@ class Server:
 
     def submit(self, psf, *, sino=None, rhs=None, x0=None, solver="cg", max_iter=50,
                tol=1e-5, xtol=1e-5, lam=0.1, mu=10.0, outer=1, inner=1, zero_pad=False,
-               aniso=False, borrow=False):
+               aniso=False, borrow=False, test_fail=False):
         """psf: PointSpreadFunction2D (a stack) or PointSpreadFunction (a volume).
         sino: sinogram {ns, nang, nrad} / projections {nproj, nrows, ncols}
-        (back-projected on the device), or rhs: the right-hand side."""
+        (back-projected on the device), or rhs: the right-hand side.
+        test_fail (a testing aid) makes the lane raise a host error on a batch
+        holding this request, before any device work."""
         ns, shape, arrays = _request_arrays(psf, sino, rhs, x0)
         r = _lib.LcRequest()
         if isinstance(psf, PointSpreadFunction):
@@ -691,7 +693,7 @@ class Server:
         r.max_iter, r.outer, r.inner = int(max_iter), int(outer), int(inner)
         r.tol, r.xtol, r.lam, r.mu = float(tol), float(xtol), float(lam), float(mu)
         r.flags = (LC_TV_ZERO_PAD if zero_pad else 0) | (LC_TV_ANISO if aniso else 0) | \
-            (_lib.LC_REQ_BORROW if borrow else 0)
+            (_lib.LC_REQ_BORROW if borrow else 0) | (_lib.LC_REQ_TEST_FAIL if test_fail else 0)
         keep = []
         for name, a in arrays.items():
             keep.append(a)
@@ -727,6 +729,17 @@ class Server:
         return dict(inflight_bytes=s[0], queued=s[1], running=s[2], finished=s[3],
                     batches=s[4], budget=s[5], lane_s_build=s[6], lane_s_input=s[7],
                     lane_s_solve=s[8], lane_s_output=s[9])
+
+    def timeline(self, max_rows: int = 65536):
+        """The most recent batches, oldest first: a float64 array of rows
+        (lane, requests, host start s, host end s, device solve start ms,
+        device solve end ms); device times count from the server's creation."""
+        rows = np.zeros((max_rows, 6), np.float64)
+        n = C.c_int32()
+        _lib.check(self.lib.lc_server_timeline(self.handle, int(max_rows),
+                                               rows.ctypes.data_as(C.POINTER(C.c_double)),
+                                               C.byref(n)))
+        return rows[:n.value].copy()
 
     def close(self):
         if self.handle:

@@ -18,6 +18,7 @@ LC_TV_ANISO = 0x10
 LC_GEOM_LAZY_PSF = 0x20
 LC_FP64 = 0x40
 LC_REQ_BORROW = 0x80
+LC_REQ_TEST_FAIL = 0x100
 
 LC_REG_SPLIT_BREGMAN = 0
 LC_REG_UNCONSTRAINED = 1
@@ -32,7 +33,7 @@ EXPORTS = [
     "lc_batch2d_reports", "lc_batch2d_sysmat", "lc_batch2d_profile",
     "lc_geom3d_create", "lc_geom3d_destroy", "lc_geom3d_info", "lc_psf3d_kernel_hat",
     "lc_server_create", "lc_server_submit", "lc_server_poll", "lc_server_wait",
-    "lc_server_result", "lc_server_stats", "lc_server_destroy",
+    "lc_server_result", "lc_server_stats", "lc_server_timeline", "lc_server_destroy",
     "lc_sysmat3d_psf", "lc_forward3d", "lc_backproj3d", "lc_sysmat3d_nufft", "lc_cg3d", "lc_split_bregman3d", "lc_batch_create", "lc_autotune2d", "lc_set_tuning2d",
     "lc_get_tuning2d",
     "lc_sysmat2d_psf_f64", "lc_cg2d_f64", "lc_split_bregman2d_f64", "lc_sysmat3d_psf_f64",
@@ -141,6 +142,7 @@ def load():
         "lc_server_wait": (C.c_int, [vp, C.c_uint64, C.c_double, C.POINTER(i32)]),
         "lc_server_result": (C.c_int, [vp, C.c_uint64, vp, C.POINTER(LcReport), f64p]),
         "lc_server_stats": (C.c_int, [vp, f64p]),
+        "lc_server_timeline": (C.c_int, [vp, i32, f64p, C.POINTER(i32)]),
         "lc_server_destroy": (C.c_int, [vp]),
         "lc_forward3d": (C.c_int, [vp, vp, vp, vp, u32]),
         "lc_backproj3d": (C.c_int, [vp, vp, vp, vp, vp, u32]),

@@ -978,6 +978,24 @@ int lc_server_stats(lc_server s, double *stats) {
     });
 }
 
+int lc_server_timeline(lc_server s, int32_t max_rows, double *rows, int32_t *nrows) {
+    return guard([&] {
+        LC_REQUIRE(s && nrows && max_rows >= 0 && (rows || max_rows == 0),
+                   "lc_server_timeline: bad argument");
+        const std::vector<BatchRecord> v = s->s->timeline((size_t)max_rows);
+        for (size_t i = 0; i < v.size(); ++i) {
+            double *o = rows + 6 * i;
+            o[0] = v[i].lane;
+            o[1] = v[i].nreq;
+            o[2] = v[i].h0;
+            o[3] = v[i].h1;
+            o[4] = v[i].d0;
+            o[5] = v[i].d1;
+        }
+        *nrows = (int32_t)v.size();
+    });
+}
+
 int lc_server_destroy(lc_server s) {
     return guard([&] { delete s; });
 }

@@ -971,15 +971,8 @@ void Batch2D::cg(const std::vector<int> &max_iter, double tol, double xtol, bool
     LC_HIP(hipGetLastError());
     cg_setup(x0_zero, false, max_iter, tol, xtol);
     if (graph && !host_sync) {
-        hipGraph_t g;
-        hipGraphExec_t ge;
-        LC_HIP(hipStreamBeginCapture(c_.stream(), hipStreamCaptureModeThreadLocal));
-        cg_iteration(false, false);
-        LC_HIP(hipStreamEndCapture(c_.stream(), &g));
-        LC_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-        for (int it = 0; it < iters; ++it) LC_HIP(hipGraphLaunch(ge, c_.stream()));
-        LC_HIP(hipGraphExecDestroy(ge));
-        LC_HIP(hipGraphDestroy(g));
+        CapturedGraph gr(c_, [&] { cg_iteration(false, false); });
+        for (int it = 0; it < iters; ++it) gr.launch();
     } else {
         for (int it = 0; it < iters; ++it) {
             cg_iteration(host_sync, false);
@@ -1077,15 +1070,8 @@ void Batch2D::split_bregman(const std::vector<float> &lambda, const std::vector<
         });
     };
     if (graph && !host_sync) {
-        hipGraph_t g;
-        hipGraphExec_t ge;
-        LC_HIP(hipStreamBeginCapture(c_.stream(), hipStreamCaptureModeThreadLocal));
-        outer_iter(false);
-        LC_HIP(hipStreamEndCapture(c_.stream(), &g));
-        LC_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-        for (int it = 0; it < outer; ++it) LC_HIP(hipGraphLaunch(ge, c_.stream()));
-        LC_HIP(hipGraphExecDestroy(ge));
-        LC_HIP(hipGraphDestroy(g));
+        CapturedGraph gr(c_, [&] { outer_iter(false); });
+        for (int it = 0; it < outer; ++it) gr.launch();
     } else {
         for (int it = 0; it < outer; ++it) {
             outer_iter(host_sync);

@@ -9,6 +9,7 @@ Ctx::Ctx(int device) : device_(device) {
     LC_REQUIRE(device >= 0 && device < n, "lc_ctx_create: device index out of range");
     LC_HIP(hipSetDevice(device));
     // default-priority, non-blocking stream
+    std::lock_guard<std::recursive_mutex> cl(capture_mutex());
     LC_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
 }
 
@@ -17,6 +18,7 @@ Ctx::~Ctx() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     tw_.clear();
     scratch_.clear();
+    std::lock_guard<std::recursive_mutex> cl(capture_mutex());
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 

@@ -12,6 +12,18 @@
 
 namespace lc {
 
+// One process-wide lock taken around every stream capture (begin capture to
+// graph instantiation) and around every call that allocates or frees device
+// or pinned host memory.  hipFree and hipHostFree synchronise the whole
+// device, and a capture on one thread's stream must not race with a
+// device-wide synchronisation or an allocation from another thread (serving
+// lanes capture concurrently, a submitting thread builds geometries beside
+// them).  Recursive: an allocation inside a held section takes it again.
+inline std::recursive_mutex &capture_mutex() {
+    static std::recursive_mutex m;
+    return m;
+}
+
 // Cache of device blocks for one worker thread.  hipFree synchronises the
 // whole device, so a thread whose buffers come and go (a serving lane, one
 // batch after another) keeps them here and reuses them; blocks are sized in
@@ -23,6 +35,7 @@ class MemPool {
     MemPool(const MemPool &) = delete;
     MemPool &operator=(const MemPool &) = delete;
     ~MemPool() {
+        std::lock_guard<std::recursive_mutex> cl(capture_mutex());
         for (auto &kv : free_) (void)hipFree(kv.second);
     }
     static size_t bin(size_t bytes) {
@@ -44,6 +57,7 @@ class MemPool {
             return p;
         }
         void *p = nullptr;
+        std::lock_guard<std::recursive_mutex> cl(capture_mutex());
         hipError_t e = hipMalloc(&p, got);
         if (e != hipSuccess)
             throw Error(LC_ENOMEM, "hipMalloc of " + std::to_string(got) + " bytes failed: " +
@@ -91,11 +105,13 @@ class PinnedBuf {
     PinnedBuf(const PinnedBuf &) = delete;
     PinnedBuf &operator=(const PinnedBuf &) = delete;
     ~PinnedBuf() {
+        std::lock_guard<std::recursive_mutex> cl(capture_mutex());
         if (p_) (void)hipHostFree(p_);
     }
     // a buffer of at least `bytes`; the caller has no copy in flight on it
     void *get(size_t bytes) {
         if (bytes > n_) {
+            std::lock_guard<std::recursive_mutex> cl(capture_mutex());
             const size_t want = std::max(bytes, n_ + n_ / 2);
             if (p_) (void)hipHostFree(p_);
             p_ = nullptr;
@@ -151,6 +167,7 @@ class DBuf {
             n_ = bytes;
             return;
         }
+        std::lock_guard<std::recursive_mutex> cl(capture_mutex());
         hipError_t e = hipMalloc(&p_, bytes);
         if (e != hipSuccess) {
             p_ = nullptr;
@@ -165,8 +182,12 @@ class DBuf {
     }
     void release() {
         if (p_ && !arena_) {
-            if (pool_) pool_->put(p_, cap_);
-            else (void)hipFree(p_);
+            if (pool_) {
+                pool_->put(p_, cap_);
+            } else {
+                std::lock_guard<std::recursive_mutex> cl(capture_mutex());
+                (void)hipFree(p_);
+            }
         }
         p_ = nullptr;
         n_ = cap_ = 0;
@@ -232,6 +253,47 @@ template <typename T> const cpx<T> *Ctx::twiddles(int L) {
     tw_.emplace(key, std::move(b));
     return p;
 }
+
+// An executable graph captured from the launches `body` issues on a
+// context's stream.  Capture and instantiation run under capture_mutex().
+// The destructor first waits for the stream: an executable graph is not
+// destroyed while launches of it are still queued or running.
+class CapturedGraph {
+  public:
+    template <typename F> CapturedGraph(Ctx &c, F &&body) : c_(c) {
+        std::lock_guard<std::recursive_mutex> cl(capture_mutex());
+        LC_HIP(hipStreamBeginCapture(c.stream(), hipStreamCaptureModeThreadLocal));
+        try {
+            body();
+        } catch (...) {
+            hipGraph_t partial = nullptr;
+            if (hipStreamEndCapture(c.stream(), &partial) == hipSuccess && partial)
+                (void)hipGraphDestroy(partial);
+            throw;
+        }
+        LC_HIP(hipStreamEndCapture(c.stream(), &g_));
+        const hipError_t e = hipGraphInstantiate(&ge_, g_, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(g_);
+            g_ = nullptr;
+            LC_HIP(e);
+        }
+    }
+    CapturedGraph(const CapturedGraph &) = delete;
+    CapturedGraph &operator=(const CapturedGraph &) = delete;
+    ~CapturedGraph() {
+        (void)hipStreamSynchronize(c_.stream());
+        std::lock_guard<std::recursive_mutex> cl(capture_mutex());
+        if (ge_) (void)hipGraphExecDestroy(ge_);
+        if (g_) (void)hipGraphDestroy(g_);
+    }
+    void launch() { LC_HIP(hipGraphLaunch(ge_, c_.stream())); }
+
+  private:
+    Ctx &c_;
+    hipGraph_t g_ = nullptr;
+    hipGraphExec_t ge_ = nullptr;
+};
 
 // copy helpers honouring a "pointers are on the device" flag
 void copy_in(Ctx &c, void *dst, const void *src, size_t bytes, bool src_dev);

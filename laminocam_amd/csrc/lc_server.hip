@@ -15,6 +15,7 @@
 // so its result does not depend on the batch it ran in.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <functional>
 #include <condition_variable>
 #include <cstring>
@@ -71,10 +72,17 @@ Server::Server(Ctx &main, size_t budget, int nlanes, int max_batch)
     : main_(main), budget_(budget), max_batch_(max_batch) {
     LC_REQUIRE(nlanes >= 1 && nlanes <= 16, "server: 1 to 16 lanes");
     LC_REQUIRE(max_batch >= 1, "server: max_batch must be >= 1");
+    main.set_current();
+    LC_HIP(hipEventCreate(&epoch_));
+    LC_HIP(hipEventRecord(epoch_, main.stream()));
+    LC_HIP(hipEventSynchronize(epoch_));
     for (int i = 0; i < nlanes; ++i) {
         auto l = std::make_unique<Lane>();
+        l->index = i;
         l->ctx = std::make_unique<Ctx>(main.device());
         l->ctx->toep_tuning = main.toep_tuning;
+        LC_HIP(hipEventCreate(&l->ev0));
+        LC_HIP(hipEventCreate(&l->ev1));
         lanes_.push_back(std::move(l));
     }
     for (auto &l : lanes_) l->th = std::thread([this, lp = l.get()] { lane_loop(*lp); });
@@ -86,8 +94,18 @@ Server::~Server() {
         stop_ = true;
     }
     cv_work_.notify_all();
-    for (auto &l : lanes_)
+    for (auto &l : lanes_) {
         if (l->th.joinable()) l->th.join();
+        if (l->ev0) (void)hipEventDestroy(l->ev0);
+        if (l->ev1) (void)hipEventDestroy(l->ev1);
+    }
+    if (epoch_) (void)hipEventDestroy(epoch_);
+}
+
+std::vector<BatchRecord> Server::timeline(size_t max_rows) {
+    std::lock_guard<std::mutex> lk(m_);
+    const size_t n = std::min(max_rows, records_.size());
+    return std::vector<BatchRecord>(records_.end() - (ptrdiff_t)n, records_.end());
 }
 
 uint64_t Server::submit(std::unique_ptr<ServerRequest> r) {
@@ -97,6 +115,8 @@ uint64_t Server::submit(std::unique_ptr<ServerRequest> r) {
         const bool build = r->g3 ? !r->g3->psf_ready || (r->has_sino && !r->g3->nu.ready)
                                  : !r->g2->psf_ready || (r->has_sino && !r->g2->nu.ready);
         if (build) {   // (a request on a ready geometry launches nothing here)
+            // no lane captures while this thread allocates and synchronises
+            std::lock_guard<std::recursive_mutex> cl(capture_mutex());
             main_.set_current();
             if (r->g3) {
                 if (!r->g3->psf_ready) geom3d_build_psf(main_, *r->g3);
@@ -173,6 +193,10 @@ static bool same_group(const ServerRequest &a, const ServerRequest &b) {
     return true;
 }
 
+// a head request that does not fit the free budget lets smaller requests
+// start past it for at most this many batches; then the lanes wait until it fits
+constexpr int kMaxBypass = 8;
+
 void Server::lane_loop(Lane &lane) {
     tl_pool = &lane.pool;   // this lane's batches reuse device blocks instead of hipFree
     for (;;) {
@@ -187,13 +211,29 @@ void Server::lane_loop(Lane &lane) {
                     return;
                 }
                 if (!queue_.empty()) {
-                    // first-come requests of the head's group that fit the free budget
-                    ServerRequest *head = reqs_[queue_.front()].get();
+                    // the batch's anchor: the head when it fits, else the first
+                    // queued request that fits (unless the head has been passed
+                    // over kMaxBypass times); then every first-come request of
+                    // the anchor's group that fits the remaining budget
                     size_t room = budget_ - inflight_;
+                    ServerRequest *head = reqs_[queue_.front()].get();
+                    ServerRequest *anchor = nullptr;
                     if (head->bytes <= room) {
+                        anchor = head;
+                    } else if (head->bypassed < kMaxBypass) {
+                        for (uint64_t id : queue_) {
+                            ServerRequest *r = reqs_[id].get();
+                            if (r->bytes <= room) {
+                                anchor = r;
+                                break;
+                            }
+                        }
+                    }
+                    if (anchor) {
+                        if (anchor != head) ++head->bypassed;
                         for (auto it = queue_.begin(); it != queue_.end() && (int)batch.size() < max_batch_;) {
                             ServerRequest *r = reqs_[*it].get();
-                            if (same_group(*head, *r) && r->bytes <= room) {
+                            if (same_group(*anchor, *r) && r->bytes <= room) {
                                 room -= r->bytes;
                                 batch.push_back(r);
                                 it = queue_.erase(it);
@@ -215,12 +255,29 @@ void Server::lane_loop(Lane &lane) {
             running_ += batch.size();
             ++batches_;
         }
-        std::string err;
+        std::map<ServerRequest *, std::string> errs;
         std::vector<OutCopy> outs;
+        bool isolated = false;
         try {
             outs = run_batch(lane, batch);
         } catch (const std::exception &e) {
-            err = e.what();
+            if (batch.size() == 1) {
+                errs[batch[0]] = e.what();
+            } else {
+                // the batch failed: each request runs alone, so only a request
+                // that fails on its own fails; its result is delivered at once
+                // (the next run reuses the result staging)
+                isolated = true;
+                for (ServerRequest *r : batch) {
+                    std::vector<ServerRequest *> one{r};
+                    try {
+                        const std::vector<OutCopy> o = run_batch(lane, one);
+                        deliver(o, errs);
+                    } catch (const std::exception &e1) {
+                        errs[r] = e1.what();
+                    }
+                }
+            }
         }
         {
             // the batch's device memory is released: other lanes may admit now
@@ -228,34 +285,46 @@ void Server::lane_loop(Lane &lane) {
             for (ServerRequest *r : batch) inflight_ -= r->bytes;
         }
         cv_work_.notify_all();
-        if (!err.empty()) {
-            finish(batch, err);
+        if (isolated || !errs.empty()) {
+            finish(batch, errs);
             continue;
         }
         // the host copies of the results run beside the lane's next batch
         lane.out_job = std::async(std::launch::async, [this, batch, outs = std::move(outs)] {
-            for (const OutCopy &o : outs) {
-                float *xo = o.r->x_out;
-                if (!xo) {
-                    o.r->x.resize(o.n);
-                    xo = o.r->x.data();
-                }
-                std::memcpy(xo, o.src, sizeof(float) * o.n);
-            }
-            finish(batch, std::string());
+            std::map<ServerRequest *, std::string> e;
+            deliver(outs, e);
+            finish(batch, e);
         });
     }
 }
 
-// marks a batch's requests done (or failed) and wakes their waiters
-void Server::finish(const std::vector<ServerRequest *> &batch, const std::string &err) {
+void Server::deliver(const std::vector<OutCopy> &outs, std::map<ServerRequest *, std::string> &errs) {
+    for (const OutCopy &o : outs) {
+        float *xo = o.r->x_out;
+        if (!xo) {
+            o.r->x.resize(o.n);
+            xo = o.r->x.data();
+        }
+        std::memcpy(xo, o.src, sizeof(float) * o.n);
+        size_t bad = 0;
+        for (size_t i = 0; i < o.n; ++i) bad += !std::isfinite(xo[i]);
+        if (bad)
+            errs[o.r] = "the solution has " + std::to_string(bad) + " non-finite values of " +
+                        std::to_string(o.n);
+    }
+}
+
+// marks a batch's requests done (or failed, those with a message in errs)
+// and wakes their waiters
+void Server::finish(const std::vector<ServerRequest *> &batch, const std::map<ServerRequest *, std::string> &errs) {
     {
         std::lock_guard<std::mutex> lk(m_);
         const double t = now_s();
         for (ServerRequest *r : batch) {
             r->t_done = t;
-            r->state = err.empty() ? 2 : 3;
-            r->err = err;
+            auto it = errs.find(r);
+            r->state = it == errs.end() ? 2 : 3;
+            r->err = it == errs.end() ? std::string() : it->second;
         }
         running_ -= batch.size();
         done_ += batch.size();
@@ -268,6 +337,8 @@ std::vector<Server::OutCopy> Server::run_batch(Lane &lane, std::vector<ServerReq
     std::lock_guard<std::mutex> lk(c.mu);
     c.set_current();
     const double t0 = now_s();
+    for (const ServerRequest *r : batch)
+        LC_REQUIRE(!(r->flags & LC_REQ_TEST_FAIL), "request marked LC_REQ_TEST_FAIL");
     // requests on the same geometry next to each other (stable: arrival order
     // within a geometry), so their sinograms and solution slots are contiguous
     // and one back-projection serves them all; a problem's result does not
@@ -361,6 +432,7 @@ std::vector<Server::OutCopy> Server::run_batch(Lane &lane, std::vector<ServerReq
     c.sync();
     const double t2 = now_s();
     const ServerRequest &h = *batch.front();
+    LC_HIP(hipEventRecord(lane.ev0, c.stream()));
     if (h.solver == 0) {
         std::vector<int> mi;
         for (ServerRequest *r : batch) mi.push_back(r->max_iter);
@@ -374,8 +446,17 @@ std::vector<Server::OutCopy> Server::run_batch(Lane &lane, std::vector<ServerReq
         b.split_bregman(la, mu, h.outer, h.inner, h.tol, h.xtol, (h.flags & LC_TV_ZERO_PAD) ? 1 : 0,
                         (h.flags & LC_TV_ANISO) != 0, true, false, !any_x0);
     }
+    LC_HIP(hipEventRecord(lane.ev1, c.stream()));
     c.sync();
     const double t3 = now_s();
+    BatchRecord rec;
+    rec.lane = lane.index;
+    rec.nreq = (int)batch.size();
+    float d0 = 0.0f, d1 = 0.0f;
+    LC_HIP(hipEventElapsedTime(&d0, epoch_, lane.ev0));
+    LC_HIP(hipEventElapsedTime(&d1, epoch_, lane.ev1));
+    rec.d0 = d0;
+    rec.d1 = d1;
     const auto cs = b.cg_state();
     const auto ss = h.solver == 1 ? b.sb_state() : std::vector<SbState>();
     // results: the batch's whole solution buffer in one copy into the result
@@ -399,7 +480,11 @@ std::vector<Server::OutCopy> Server::run_batch(Lane &lane, std::vector<ServerReq
     }
     c.sync();
     const double t4 = now_s();
+    rec.h0 = t0;
+    rec.h1 = t4;
     std::lock_guard<std::mutex> lk2(m_);
+    records_.push_back(rec);
+    if (records_.size() > 65536) records_.pop_front();
     t_build_ += t1 - t0;
     t_input_ += t2 - t1;
     t_solve_ += t3 - t2;

@@ -44,7 +44,15 @@ struct ServerRequest {
     size_t bytes = 0;          // device bytes held while running
     double t_submit = 0, t_start = 0, t_done = 0;
     int state = 0;             // 0 queued, 1 running, 2 done, 3 failed
+    int bypassed = 0;          // batches admitted past this request while it headed the queue
     std::string err;
+};
+
+// one batch a lane ran: lane, requests, host start / end (seconds, steady
+// clock), device start / end of the solve (ms since the server started)
+struct BatchRecord {
+    int lane = 0, nreq = 0;
+    double h0 = 0, h1 = 0, d0 = 0, d1 = 0;
 };
 
 size_t request_bytes(const ServerRequest &r);
@@ -61,6 +69,8 @@ class Server {
     // summed lane seconds spent building batches, loading inputs (incl.
     // back-projection), solving, and reading results back
     void stats(double *out);
+    // the most recent batch records, oldest first (at most max_rows)
+    std::vector<BatchRecord> timeline(size_t max_rows);
 
   private:
     // one request's solution, from the lane's result staging to its host array
@@ -70,16 +80,21 @@ class Server {
         size_t n;
     };
     struct Lane {
+        int index = 0;
         MemPool pool;                // outlives the context's buffers
         PinnedBuf stage;             // host staging of the lane's inputs
         PinnedBuf ostage;            // host staging of the lane's results
         std::future<void> out_job;   // the previous batch's copy-out (reads ostage)
         std::unique_ptr<Ctx> ctx;
+        hipEvent_t ev0 = nullptr, ev1 = nullptr;   // device start / end of a batch's solve
         std::thread th;
     };
     void lane_loop(Lane &lane);
     std::vector<OutCopy> run_batch(Lane &lane, std::vector<ServerRequest *> &batch);
-    void finish(const std::vector<ServerRequest *> &batch, const std::string &err);
+    // host copies of a batch's results; a request whose solution is not
+    // finite fails with that message (errs[i], empty for success)
+    static void deliver(const std::vector<OutCopy> &outs, std::map<ServerRequest *, std::string> &errs);
+    void finish(const std::vector<ServerRequest *> &batch, const std::map<ServerRequest *, std::string> &errs);
 
     Ctx &main_;
     size_t budget_;
@@ -93,6 +108,8 @@ class Server {
     size_t inflight_ = 0, queued_ = 0, running_ = 0, done_ = 0, batches_ = 0;
     double t_build_ = 0, t_input_ = 0, t_solve_ = 0, t_output_ = 0;
     bool stop_ = false;
+    hipEvent_t epoch_ = nullptr;        // recorded when the server starts
+    std::deque<BatchRecord> records_;   // the most recent batches (guarded by m_)
 };
 
 } // namespace lc

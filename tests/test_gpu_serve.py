@@ -169,3 +169,101 @@ def test_server_borrowed_inputs_and_copy_out(lc, problems):
         assert np.array_equal(out_c[i], xr), f"copied request {i} differs from its solo solve"
         assert np.array_equal(out_b[i], xr), f"borrowed request {i} differs from its solo solve"
     assert np.array_equal(xc, ref[0])
+
+
+def test_server_isolates_a_non_finite_request(lc, problems):
+    # a request whose sinogram holds a NaN fails with a message; the requests
+    # batched with it are bitwise their solo solves (nothing leaks across
+    # problems of one batch)
+    announce("server non-finite request isolation")
+    reqs = [dict(r) for r in problems if r["solver"] == "cg" and "sino" in r][:3]
+    bad = dict(reqs[0])
+    bad["sino"] = reqs[0]["sino"].copy()
+    bad["sino"][0, 3, 5] = np.nan
+    ref = [solve_alone(lc, r) for r in reqs]
+    with lc.Server(hbm_budget=8 << 30, lanes=1) as srv:
+        tk = [submit(srv, r) for r in reqs[:1]] + [submit(srv, bad)] + [submit(srv, r) for r in reqs[1:]]
+        good = [srv.result(tk[0], timeout=120.0)[0]] + \
+            [srv.result(t, timeout=120.0)[0] for t in tk[2:]]
+        with pytest.raises(lc.LcError, match="non-finite"):
+            srv.result(tk[1], timeout=120.0)
+        st = srv.stats()
+    for x, xr in zip(good, ref):
+        assert np.array_equal(x, xr)
+    assert st["finished"] == 4 and st["inflight_bytes"] == 0
+
+
+def test_server_isolates_a_failing_request(lc, problems):
+    # a request that makes its batch raise (LC_REQ_TEST_FAIL, raised on the
+    # host before any device work) fails alone: the batch is rerun one request
+    # at a time and the others return their solo results
+    announce("server failing request isolation")
+    reqs = [r for r in problems if r["solver"] == "cg"][:4]
+    ref = [solve_alone(lc, r) for r in reqs]
+    with lc.Server(hbm_budget=8 << 30, lanes=1) as srv:
+        # hold the lane on a first request so the next ones queue into one batch
+        t_hold = submit(srv, reqs[0])
+        tk = [submit(srv, r) for r in reqs[1:3]]
+        t_bad = srv.submit(reqs[3]["psf"], test_fail=True,
+                           **{k: v for k, v in reqs[3].items() if k not in ("psf", "grid")})
+        tk.append(submit(srv, reqs[3]))
+        x_hold = srv.result(t_hold, timeout=120.0)[0]
+        out = [srv.result(t, timeout=120.0)[0] for t in tk]
+        with pytest.raises(lc.LcError, match="LC_REQ_TEST_FAIL"):
+            srv.result(t_bad, timeout=120.0)
+        st = srv.stats()
+    assert np.array_equal(x_hold, ref[0])
+    for x, xr in zip(out, ref[1:]):
+        assert np.array_equal(x, xr)
+    assert st["finished"] == 5 and st["inflight_bytes"] == 0
+
+
+def _stack_problem(lc, N, ns, nang, seed):
+    rng = np.random.default_rng(seed)
+    g = lc.PolarGrid2D(np.arange(nang) * np.pi / nang, N)
+    p = lc.PointSpreadFunction2D(g, N)
+    return p, p.convolve(rng.random((ns, N, N)).astype(np.float32))
+
+
+def test_server_lanes_overlap_on_the_device(lc):
+    # two groups that cannot share a batch (CG, Split-Bregman) go to two lanes;
+    # the HIP-event intervals of their solves overlap on the device
+    announce("server lanes overlap")
+    p, y = _stack_problem(lc, 256, 24, 180, 5)
+    with lc.Server(hbm_budget=16 << 30, lanes=2) as srv:
+        t_cg = srv.submit(p, rhs=y, solver="cg", max_iter=150, tol=0.0, xtol=0.0)
+        t_sb = srv.submit(p, rhs=y, solver="sb", lam=0.05, mu=10.0, outer=40, inner=2, tol=0.0,
+                          xtol=0.0)
+        srv.result(t_cg, timeout=120.0)
+        srv.result(t_sb, timeout=120.0)
+        tl = srv.timeline()
+    assert len(tl) == 2 and tl[0][0] != tl[1][0], tl
+    (a0, a1), (b0, b1) = tl[0][4:6], tl[1][4:6]
+    print(f"device intervals (ms): lane {int(tl[0][0])} [{a0:.2f}, {a1:.2f}], "
+          f"lane {int(tl[1][0])} [{b0:.2f}, {b1:.2f}]")
+    assert a0 < a1 and b0 < b1
+    assert max(a0, b0) < min(a1, b1), "the two lanes' solves did not overlap"
+
+
+def test_server_admits_past_a_head_that_does_not_fit(lc):
+    # budget for one large request and a little more: while the first large
+    # request runs, the second does not fit, and small requests behind it
+    # start before it instead of waiting
+    announce("server admission past the head")
+    pb, yb = _stack_problem(lc, 256, 32, 180, 6)
+    ps, ys = _stack_problem(lc, 64, 1, 90, 7)
+    big_bytes = 32 * (20 * 256 * 256 + 8 * (257 + 16) * 256) + (1 << 20)
+    with lc.Server(hbm_budget=int(1.5 * big_bytes), lanes=2) as srv:
+        t1 = srv.submit(pb, rhs=yb, solver="cg", max_iter=400, tol=0.0, xtol=0.0)
+        time.sleep(0.01)
+        t2 = srv.submit(pb, rhs=yb, solver="cg", max_iter=400, tol=0.0, xtol=0.0)
+        ts = [srv.submit(ps, rhs=ys, solver="sb", lam=0.05, mu=10.0, outer=3, inner=1)
+              for _ in range(4)]
+        r1 = srv.result(t1, timeout=120.0)
+        r2 = srv.result(t2, timeout=120.0)
+        rs = [srv.result(t, timeout=120.0) for t in ts]
+    start2 = r2[2][1]
+    print(f"big request 2 starts {start2 - r1[2][1]:.3f} s after big 1; smalls start "
+          f"{[round(r[2][1] - r1[2][1], 3) for r in rs]}")
+    assert all(r[2][1] < start2 for r in rs)
+    assert np.array_equal(r1[0], r2[0])
