@@ -605,7 +605,7 @@ void Batch2D::build() {
             ToepVariant v = it != c_.toep_tuning.end() ? it->second : toep_default_variant(k->M);
             // a tuned FP32 column shape may not fit 16-byte elements
             if (v.col >= toep_num_col_shapes_for(f64_)) v = toep_default_variant(k->M);
-            toep_configure(*k, v);
+            toep_configure(*k, toep_fit_variant(v, k->N, k->M, f64_));
         }
         std::vector<SliceRef> sl;
         for (int p : kv.second) {

@@ -229,6 +229,11 @@ template <> inline const double2 *toep_tw<double>(const ToepClass &k) { return k
 // block shapes for a class; U elements per slice (independent of the shapes)
 void toep_configure(ToepClass &k, ToepVariant v);
 ToepVariant toep_default_variant(int M);
+// An odd image size N needs column blocks of at least two columns (the block
+// copies move pairs of complex values).  A column shape too small for 2 M
+// values is replaced by the first shape that holds them (block width then
+// chosen by toep_configure).
+ToepVariant toep_fit_variant(ToepVariant v, int N, int M, bool f64);
 int64_t toep_u_size(const ToepClass &k);
 int toep_num_row_shapes();
 int toep_num_col_shapes();

@@ -341,6 +341,18 @@ ToepVariant toep_default_variant(int M) {
     return v;
 }
 
+ToepVariant toep_fit_variant(ToepVariant v, int N, int M, bool f64) {
+    if (!(N & 1) || v.col < 0 || v.col >= N_COL_SHAPES || toep_col_cap(v.col) >= 2 * M) return v;
+    for (int col : {0, 2, 1}) {
+        if (col < toep_num_col_shapes_for(f64) && toep_col_cap(col) >= 2 * M) {
+            v.col = col;
+            v.B = 0;
+            return v;
+        }
+    }
+    return v;   // toep_configure reports the size as unsupported
+}
+
 void toep_configure(ToepClass &k, ToepVariant v) {
     LC_REQUIRE(v.row >= 0 && v.row < N_ROW_SHAPES && v.col >= 0 && v.col < N_COL_SHAPES,
                "toeplitz: bad kernel variant");

@@ -44,7 +44,11 @@ def test_kernel_hat_asymmetric_angles(lc, orc64, N, nang):
 
 
 @pytest.mark.parametrize("N,nang,ns", [(16, 7, 5), (32, 13, 8), (31, 17, 2), (64, 90, 3),
-                                       (96, 60, 2), (128, 180, 2), (192, 90, 1), (256, 180, 1)])
+                                       (96, 60, 2), (128, 180, 2), (192, 90, 1), (256, 180, 1),
+                                       (512, 180, 1), (768, 90, 1),
+                                       # odd sizes above 512 (the reference forces odd
+                                       # recon sizes, include/config.h:377)
+                                       (513, 60, 1), (721, 90, 2), (1023, 45, 1)])
 def test_toeplitz_matches_oracle(lc, orc64, N, nang, ns):
     announce(f"toeplitz N={N} nang={nang} ns={ns}")
     rng = np.random.default_rng(N * 1000 + nang)
@@ -199,10 +203,11 @@ def test_batch_same_alone_and_in_mix(lc, orc64):
         assert np.array_equal(xs, batch.get(i, batch.X))
 
 
-@pytest.mark.parametrize("N", [128, 384, 1024])
+@pytest.mark.parametrize("N", [128, 384, 576, 721, 1024])
 def test_row_shapes_agree(lc, orc64, N):
     # both compiled row-pass shapes (256x8, 512x8) give the oracle's
-    # Toeplitz apply, and the CG-fused row passes agree across shapes
+    # Toeplitz apply, and the CG-fused row passes agree across shapes; N = 576
+    # and 721 take FFT lengths planned at run time (M = 1152, 1458)
     announce(f"row shapes N={N}")
     ctx = lc.Context(0)
     try:

@@ -57,6 +57,23 @@ def test_toeplitz3d_matches_oracle(lc, orc64, dims, nproj, nrows, ncols, tilt):
     assert err < 1e-5
 
 
+@pytest.mark.parametrize("dims,nproj,nrows", [((3, 721, 721), 6, 8), ((2, 576, 576), 5, 6)])
+def test_toeplitz3d_large_inplane_matches_oracle(lc, orc64, dims, nproj, nrows):
+    # in-plane PSF lengths above 1024 (1458 as in the laminography volume,
+    # 1152), which take the 512-thread row passes, on a short slice axis
+    announce(f"toeplitz3d large in-plane dims={dims}")
+    rng = np.random.default_rng(dims[1])
+    th, ga, be = lamino_angles(nproj)
+    psf = lc.PointSpreadFunction(lc.PolarGrid(th, ga, be, nrows, dims[2]), dims)
+    assert psf.dims()[1] > 1024
+    x = rng.random(dims).astype(np.float32)
+    y = psf.convolve(x)
+    yr = orc64.psf3d(th, ga, be, nrows, dims[2], dims, method=0).convolve(x)
+    err = rel_l2(y, yr)
+    print(f"toeplitz3d {dims} rel_l2 = {err:.3e}")
+    assert err < 1e-5
+
+
 def test_toeplitz3d_lamino_size_symmetric_psd(lc):
     # the padded laminography volume of the 3-D configuration: 29 x 721 x 721
     # on 360 projections of 89 x 721 at 30 degrees tilt.  A^T A is symmetric and

@@ -42,7 +42,10 @@ def test_projectors_match_golden(lc, case):
 # detector widths (direct-DFT path), full-circle angles, stacks
 CASES = [(16, 9, 16, 1, np.pi), (24, 30, 24, 2, np.pi), (31, 17, 33, 1, np.pi),
          (64, 90, 64, 3, np.pi), (64, 45, 90, 1, 2 * np.pi), (100, 60, 100, 1, np.pi),
-         (128, 180, 128, 1, np.pi), (256, 180, 256, 1, np.pi), (192, 90, 192, 1, np.pi)]
+         (128, 180, 128, 1, np.pi), (256, 180, 256, 1, np.pi), (192, 90, 192, 1, np.pi),
+         # the image sizes of the batched workload at its largest angle count
+         (384, 360, 384, 1, np.pi), (512, 720, 512, 1, np.pi), (768, 720, 768, 1, np.pi),
+         (1024, 720, 1024, 1, np.pi)]
 
 
 @pytest.mark.parametrize("N,nang,nrad,ns,span", CASES)
