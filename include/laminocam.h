@@ -45,6 +45,10 @@ extern "C" {
 #define LC_REQ_BORROW 0x80u  /* lc_request: sino / rhs / x0 are read in place when the request
                                 runs instead of copied at submit; they must stay valid and
                                 unchanged until the request finishes */
+#define LC_HOST_SYNC 0x200u  /* lc_batch2d_cg / lc_batch2d_split_bregman: per-op launches that
+                                read the solver scalars back after every iteration and stop
+                                once every problem has stopped, the mode of the one-problem
+                                entry points (without their allocations and copies) */
 #define LC_REQ_TEST_FAIL 0x100u /* lc_request, a testing aid: the lane raises a host error when
                                    it builds a batch holding this request, before any device
                                    work, so that per-request failure isolation can be tested */
@@ -376,6 +380,34 @@ int lc_mbir3d_f64(lc_ctx ctx, int32_t ndatasets, const double *const *projs,
                   const double *gamma, const double *beta, const double *shifts,
                   const lc_recon_params *params, double *recon, int64_t *padded_dims,
                   lc_report *rep, uint32_t flags);
+
+/* ---- memory model ----
+ * lc_reference_memory_model: the reference's analytic device-memory model of
+ * its single-device solvers, as a query (scripts/estimate_bregman_memory.py:18-62,
+ * its fit rule src/gpu/mbir2d_multi.cu:459-470).  For a volume of N = n1 n2 n3
+ * elements of elem_bytes bytes (4 or 8), out[5] =
+ *   typical high-water mark   Split-Bregman 22 N, CG 11 N
+ *   conservative mark         Split-Bregman 24 N, CG 11 N
+ *   inputs                    2 N (right-hand side and initial guess)
+ *   the fit rule's working set  Split-Bregman 27 N, CG 11 N
+ *   N
+ * all in bytes except N.  The script models Split-Bregman only; for CG the
+ * reference states only the fit rule's 11 N, used for all three figures.
+ * lc_volume_fits: the reference's fit rule (gpu::volume_fits_one_gpu,
+ * include/gpu/tomocam.h:91-93): fit working set < 0.8 x free_bytes.
+ * lc_volume_fits_one_gpu: the same with the free memory of the context's
+ * device (hipMemGetInfo).
+ * lc_request_bytes (no reference counterpart): the device bytes this
+ * library's server charges against its HBM budget for a request (solver
+ * vectors, Toeplitz intermediates, Split-Bregman fields, back-projection
+ * scratch); what admission uses. */
+int lc_reference_memory_model(const int64_t *dims, int32_t elem_bytes, int32_t regularizer,
+                              double *out);
+int lc_volume_fits(const lc_recon_params *params, int32_t elem_bytes, uint64_t free_bytes,
+                   int32_t *fits);
+int lc_volume_fits_one_gpu(lc_ctx ctx, const lc_recon_params *params, int32_t elem_bytes,
+                           int32_t *fits);
+int lc_request_bytes(const lc_request *r, uint64_t *bytes);
 
 /* ---- Toeplitz kernel shapes per image size (no reference counterpart) ----
  * variant[0] = row-pass shape, variant[1] = column-pass shape,

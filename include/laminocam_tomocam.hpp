@@ -14,6 +14,8 @@
  *   include/mbir2d_prep.h:30-65      Sinogram2D, prepare_sinogram
  *   include/tomocam.h:31-33          MBIR (src/mbir.cpp:34-161)
  *   include/gpu/tomocam.h:60-75      gpu::MBIR, gpu::MBIR2D (src/gpu/mbir2d.cu:36-85)
+ *   include/gpu/tomocam.h:91-93      gpu::volume_fits_one_gpu
+ *   include/config.h:284-343         broadcast_shifts, AlignmentTable, apply_alignment
  * Errors: a nonzero status becomes std::invalid_argument (bad arguments,
  * unsupported shapes) or std::runtime_error (device failures), as the
  * reference throws (src/projection.cpp:105-108, include/finufft_plan.h:106).
@@ -900,6 +902,47 @@ template <typename T> struct DatasetView {
 
 namespace tomocam {
 
+// include/config.h:284-296: the per-projection [dx, dy] shifts of one dataset;
+// none gives zeros, one pair (the legacy 'cor-offset') is repeated for every
+// projection, otherwise there is one pair per projection
+template <typename T>
+std::vector<std::array<T, 2>> broadcast_shifts(const std::vector<std::array<T, 2>> &shifts, size_t nproj) {
+    if (shifts.empty()) return std::vector<std::array<T, 2>>(nproj, std::array<T, 2>{T(0), T(0)});
+    if (shifts.size() == 1) return std::vector<std::array<T, 2>>(nproj, shifts[0]);
+    if (shifts.size() != nproj)
+        throw std::runtime_error("shifts file has " + std::to_string(shifts.size()) + " entries but " +
+                                 std::to_string(nproj) + " projections were loaded");
+    return shifts;
+}
+
+// The contents of an alignment file (include/config.h:298-343), parsed by
+// the caller: optional tilts in degrees, the corrected angles in degrees, and
+// optional per-projection shifts in pixels.
+template <typename T> struct AlignmentTable {
+    bool has_gamma = false, has_beta = false, has_shifts = false;
+    T gamma_deg = T(0), beta_deg = T(0);
+    std::vector<T> corrected_angles_deg;
+    std::vector<std::array<T, 2>> shifts_px;
+};
+
+// include/config.h:298-343: the alignment override of a dataset.  gamma =
+// -gamma_deg (the parser's sign convention) and beta = beta_deg, in radians,
+// when given; the angles become the corrected angles; the shifts become
+// shifts_px when given, else stay the dataset's (completed by
+// broadcast_shifts), and must number one per corrected angle.
+template <typename T> void apply_alignment(Dataset_t<T> &ds, const AlignmentTable<T> &a) {
+    ds.shifts = broadcast_shifts<T>(ds.shifts, ds.angles.size());
+    if (a.has_gamma) ds.gamma = -a.gamma_deg * T(M_PI) / T(180);
+    if (a.has_beta) ds.beta = a.beta_deg * T(M_PI) / T(180);
+    if (a.corrected_angles_deg.empty()) throw std::runtime_error("Alignment TOML missing 'corrected_angles_deg'");
+    ds.angles.clear();
+    for (T v : a.corrected_angles_deg) ds.angles.push_back(v * T(M_PI) / T(180));
+    if (a.has_shifts) ds.shifts = a.shifts_px;
+    if (ds.shifts.size() != ds.angles.size())
+        throw std::runtime_error("Alignment TOML: shifts_px has " + std::to_string(ds.shifts.size()) +
+                                 " entries but corrected_angles_deg has " + std::to_string(ds.angles.size()));
+}
+
 template <typename T> struct Sinogram2D {
     Array<T> data;          // {n_slices, n_angles, n_cols}
     std::vector<T> theta;   // one angle per projection (radians)
@@ -942,6 +985,16 @@ template <typename T> Array<T> MBIR(const std::vector<Dataset_t<T>> &datasets, c
 }
 
 namespace gpu {
+// include/gpu/tomocam.h:91-93 (rule src/gpu/mbir2d_multi.cu:459-470): the
+// single-device solvers' working set, Split-Bregman 27 N and CG 11 N elements
+// of T, below 0.8 of the device's free memory
+template <typename T> bool volume_fits_one_gpu(const ReconParams &params) {
+    const lc_recon_params p = laminocam::recon_params(params);
+    int32_t fits = 0;
+    laminocam::check(lc_volume_fits_one_gpu(laminocam::context(), &p, (int32_t)sizeof(T), &fits));
+    return fits != 0;
+}
+
 // include/gpu/tomocam.h:60-75: the device drivers are the same calls
 template <typename T> Array<T> MBIR(const std::vector<Dataset_t<T>> &datasets, const ReconParams &params) {
     return tomocam::MBIR<T>(datasets, params);

@@ -32,7 +32,7 @@ __all__ = ["Context", "PolarGrid2D", "PointSpreadFunction2D", "PolarGrid", "Poin
            "backproj2d", "cgsolver",
            "split_bregman", "grad_u", "divergence", "laplacian", "Batch2D", "LcError",
            "default_context", "Server", "Dataset", "Regularizer", "ReconParams", "Sinogram2D",
-           "prepare_sinogram", "MBIR2D", "MBIR"]
+           "prepare_sinogram", "MBIR2D", "MBIR", "estimate_memory", "volume_fits_one_gpu"]
 
 _default_ctx = None
 
@@ -559,22 +559,26 @@ class Batch2D:
     def sysmat(self, src=0, dst=2):
         _lib.check(self.lib.lc_batch2d_sysmat(self.handle, src, dst))
 
-    def cg(self, max_iter, tol=1e-5, xtol=1e-5, graph=True, x0_zero=True):
+    def cg(self, max_iter, tol=1e-5, xtol=1e-5, graph=True, x0_zero=True, host_sync=False):
+        """host_sync (with graph=False): the one-problem entry points' mode,
+        scalars read back after every iteration"""
         n = len(self)
         mi = [int(max_iter)] * n if np.isscalar(max_iter) else [int(v) for v in max_iter]
         arr = (C.c_int32 * n)(*mi)
-        flags = (LC_GRAPH if graph else 0) | (LC_X0_ZERO if x0_zero else 0)
+        flags = (LC_GRAPH if graph else 0) | (LC_X0_ZERO if x0_zero else 0) | \
+            (_lib.LC_HOST_SYNC if host_sync else 0)
         _lib.check(self.lib.lc_batch2d_cg(self.handle, arr, float(tol), float(xtol), flags))
 
     def split_bregman(self, lam, mu, outer, inner, tol=1e-5, xtol=1e-5, zero_pad=False,
-                      aniso=False, graph=True, x0_zero=False):
+                      aniso=False, graph=True, x0_zero=False, host_sync=False):
         n = len(self)
         la = np.ascontiguousarray(np.broadcast_to(np.float32(lam), (n,)) if np.isscalar(lam) else lam,
                                   dtype=np.float32)
         m = np.ascontiguousarray(np.broadcast_to(np.float32(mu), (n,)) if np.isscalar(mu) else mu,
                                  dtype=np.float32)
         flags = (LC_GRAPH if graph else 0) | (LC_TV_ZERO_PAD if zero_pad else 0) | \
-            (LC_TV_ANISO if aniso else 0) | (LC_X0_ZERO if x0_zero else 0)
+            (LC_TV_ANISO if aniso else 0) | (LC_X0_ZERO if x0_zero else 0) | \
+            (_lib.LC_HOST_SYNC if host_sync else 0)
         _lib.check(self.lib.lc_batch2d_split_bregman(
             self.handle, la.ctypes.data_as(C.POINTER(C.c_float)),
             m.ctypes.data_as(C.POINTER(C.c_float)), int(outer), int(inner), float(tol),
@@ -705,6 +709,25 @@ class Server:
         self._held[t.value] = (out, keep if borrow else None)
         return t.value
 
+    def request_bytes(self, psf, *, sino=None, rhs=None, solver="cg"):
+        """device bytes the server charges against its budget for such a
+        request (lc_request_bytes)"""
+        ns, _, _ = _request_arrays(psf, sino, rhs, None)
+        r = _lib.LcRequest()
+        if isinstance(psf, PointSpreadFunction):
+            r.g3 = psf.handle.value
+        else:
+            r.g2 = psf.handle.value
+        r.nslices = ns
+        r.solver = 0 if solver == "cg" else 1
+        keep = None
+        if sino is not None:
+            keep = _f32(sino)
+            r.sino = keep.ctypes.data
+        out = C.c_uint64()
+        _lib.check(self.lib.lc_request_bytes(C.byref(r), C.byref(out)))
+        return int(out.value)
+
     def done(self, ticket) -> bool:
         d = C.c_int32()
         _lib.check(self.lib.lc_server_poll(self.handle, C.c_uint64(ticket), C.byref(d)))
@@ -761,4 +784,4 @@ class Server:
 
 
 from .drivers import (MBIR, MBIR2D, Dataset, ReconParams, Regularizer,  # noqa: E402
-                      Sinogram2D, prepare_sinogram)
+                      Sinogram2D, estimate_memory, prepare_sinogram, volume_fits_one_gpu)

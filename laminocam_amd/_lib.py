@@ -19,6 +19,7 @@ LC_GEOM_LAZY_PSF = 0x20
 LC_FP64 = 0x40
 LC_REQ_BORROW = 0x80
 LC_REQ_TEST_FAIL = 0x100
+LC_HOST_SYNC = 0x200
 
 LC_REG_SPLIT_BREGMAN = 0
 LC_REG_UNCONSTRAINED = 1
@@ -42,6 +43,7 @@ EXPORTS = [
     "lc_batch2d_device_ptr_f64", "lc_psf2d_kernel_hat_f64", "lc_psf3d_kernel_hat_f64",
     "lc_recon_params_default", "lc_prepare_sinogram", "lc_prepare_sinogram_f64", "lc_mbir2d",
     "lc_mbir2d_f64", "lc_mbir3d", "lc_mbir3d_f64", "lc_cg2d_noalloc",
+    "lc_reference_memory_model", "lc_volume_fits", "lc_volume_fits_one_gpu", "lc_request_bytes",
 ]
 
 
@@ -178,6 +180,10 @@ def load():
         "lc_psf2d_kernel_hat_f64": (C.c_int, [vp, vp, vp]),
         "lc_psf3d_kernel_hat_f64": (C.c_int, [vp, vp, vp]),
         "lc_recon_params_default": (None, [C.POINTER(LcReconParams)]),
+        "lc_reference_memory_model": (C.c_int, [C.POINTER(i64), i32, i32, f64p]),
+        "lc_volume_fits": (C.c_int, [C.POINTER(LcReconParams), i32, C.c_uint64, C.POINTER(i32)]),
+        "lc_volume_fits_one_gpu": (C.c_int, [vp, C.POINTER(LcReconParams), i32, C.POINTER(i32)]),
+        "lc_request_bytes": (C.c_int, [C.POINTER(LcRequest), C.POINTER(C.c_uint64)]),
         "lc_prepare_sinogram": (C.c_int, [vp, i32, C.POINTER(vp), C.POINTER(i64), i64, i64, vp, vp,
                                           u32]),
         "lc_prepare_sinogram_f64": (C.c_int, [vp, i32, C.POINTER(vp), C.POINTER(i64), i64, i64, vp,

@@ -744,8 +744,8 @@ int lc_batch2d_cg(lc_batch2d b, const int32_t *max_iter, double tol, double xtol
         std::lock_guard<std::mutex> lk(b->ctx->c.mu);
         b->ctx->c.set_current();
         std::vector<int> mi(max_iter, max_iter + b->b->nprob());
-        const bool graph = flags & LC_GRAPH;
-        b->b->cg(mi, tol, xtol, graph, false, (flags & LC_X0_ZERO) != 0);
+        const bool graph = flags & LC_GRAPH, hsync = (flags & LC_HOST_SYNC) && !graph;
+        b->b->cg(mi, tol, xtol, graph, hsync, (flags & LC_X0_ZERO) != 0);
         auto st = b->b->cg_state();
         for (int i = 0; i < b->b->nprob(); ++i) fill_report(st[i], &b->reps[i]);
     });
@@ -759,10 +759,10 @@ int lc_batch2d_split_bregman(lc_batch2d b, const float *lambda, const float *mu,
         b->ctx->c.set_current();
         const int np = b->b->nprob();
         std::vector<float> la(lambda, lambda + np), m(mu, mu + np);
-        const bool graph = flags & LC_GRAPH;
+        const bool graph = flags & LC_GRAPH, hsync = (flags & LC_HOST_SYNC) && !graph;
         b->b->split_bregman(la, m, (int)outer, (int)inner, tol, xtol,
                             (flags & LC_TV_ZERO_PAD) ? 1 : 0, (flags & LC_TV_ANISO) != 0, graph,
-                            false, (flags & LC_X0_ZERO) != 0);
+                            hsync, (flags & LC_X0_ZERO) != 0);
         auto st = b->b->cg_state();
         auto sb = b->b->sb_state();
         for (int i = 0; i < np; ++i) {
@@ -993,6 +993,67 @@ int lc_server_timeline(lc_server s, int32_t max_rows, double *rows, int32_t *nro
             o[5] = v[i].d1;
         }
         *nrows = (int32_t)v.size();
+    });
+}
+
+int lc_reference_memory_model(const int64_t *dims, int32_t elem_bytes, int32_t regularizer,
+                              double *out) {
+    return guard([&] {
+        LC_REQUIRE(dims && out, "lc_reference_memory_model: NULL argument");
+        LC_REQUIRE(elem_bytes == 4 || elem_bytes == 8, "lc_reference_memory_model: elem_bytes 4 or 8");
+        LC_REQUIRE(regularizer == LC_REG_SPLIT_BREGMAN || regularizer == LC_REG_UNCONSTRAINED,
+                   "lc_reference_memory_model: unknown regularizer");
+        LC_REQUIRE(dims[0] >= 0 && dims[1] >= 0 && dims[2] >= 0, "lc_reference_memory_model: negative dims");
+        const double N = (double)dims[0] * (double)dims[1] * (double)dims[2];
+        const bool sb = regularizer == LC_REG_SPLIT_BREGMAN;
+        const double e = (double)elem_bytes;
+        out[0] = (sb ? 22.0 : 11.0) * N * e;
+        out[1] = (sb ? 24.0 : 11.0) * N * e;
+        out[2] = 2.0 * N * e;
+        out[3] = (sb ? 27.0 : 11.0) * N * e;
+        out[4] = N;
+    });
+}
+
+int lc_volume_fits(const lc_recon_params *p, int32_t elem_bytes, uint64_t free_bytes, int32_t *fits) {
+    return guard([&] {
+        LC_REQUIRE(p && fits, "lc_volume_fits: NULL argument");
+        LC_REQUIRE(elem_bytes == 4 || elem_bytes == 8, "lc_volume_fits: elem_bytes 4 or 8");
+        const uint64_t N = (uint64_t)p->recon_dims[0] * (uint64_t)p->recon_dims[1] *
+                           (uint64_t)p->recon_dims[2];
+        const uint64_t factor = p->regularizer == LC_REG_SPLIT_BREGMAN ? 27 : 11;
+        // size_t arithmetic and the truncating cast of the reference's rule
+        *fits = factor * N * (uint64_t)elem_bytes < (uint64_t)(0.8 * (double)free_bytes) ? 1 : 0;
+    });
+}
+
+int lc_volume_fits_one_gpu(lc_ctx ctx, const lc_recon_params *p, int32_t elem_bytes, int32_t *fits) {
+    return guard([&] {
+        LC_REQUIRE(ctx && p && fits, "lc_volume_fits_one_gpu: NULL argument");
+        ctx->c.set_current();
+        size_t free_b = 0, total_b = 0;
+        LC_HIP(hipMemGetInfo(&free_b, &total_b));
+        const int st = lc_volume_fits(p, elem_bytes, (uint64_t)free_b, fits);
+        if (st != LC_OK) throw Error(st, lc_last_error());
+    });
+}
+
+int lc_request_bytes(const lc_request *r, uint64_t *bytes) {
+    return guard([&] {
+        LC_REQUIRE(r && bytes, "lc_request_bytes: NULL argument");
+        LC_REQUIRE((r->g2 != nullptr) != (r->g3 != nullptr), "lc_request_bytes: need exactly one geometry");
+        ServerRequest q;
+        if (r->g3) {
+            q.g3 = &r->g3->g;
+            q.ns = (int)q.g3->n1;
+        } else {
+            LC_REQUIRE(r->nslices >= 1, "lc_request_bytes: nslices must be >= 1");
+            q.g2 = &r->g2->g;
+            q.ns = r->nslices;
+        }
+        q.solver = r->solver;
+        q.has_sino = r->sino != nullptr;
+        *bytes = (uint64_t)request_bytes(q);
     });
 }
 

@@ -276,24 +276,32 @@ __device__ __forceinline__ R grad_c(const R *u, const Idx3 &q, int c, int s, int
     return (R)0.5 * (up - um);
 }
 
+// divergence of a 3-component field at (s,a,b); G(c, s, a, b) reads
+// component c at an in-range point
+template <typename R, typename G>
+__device__ __forceinline__ R div_gen(const G &g, const Idx3 &q, int s, int a, int b, int boundary) {
+    if (boundary == 0) {
+        if (!interior(q, s, a, b)) return (R)0;
+        return (g(0, s + 1, a, b) - g(0, s - 1, a, b)) / (R)2 +
+               (g(1, s, a + 1, b) - g(1, s, a - 1, b)) / (R)2 +
+               (g(2, s, a, b + 1) - g(2, s, a, b - 1)) / (R)2;
+    }
+    const R g0p = inside(q, s + 1, a, b) ? g(0, s + 1, a, b) : (R)0;
+    const R g0m = inside(q, s - 1, a, b) ? g(0, s - 1, a, b) : (R)0;
+    const R g1p = inside(q, s, a + 1, b) ? g(1, s, a + 1, b) : (R)0;
+    const R g1m = inside(q, s, a - 1, b) ? g(1, s, a - 1, b) : (R)0;
+    const R g2p = inside(q, s, a, b + 1) ? g(2, s, a, b + 1) : (R)0;
+    const R g2m = inside(q, s, a, b - 1) ? g(2, s, a, b - 1) : (R)0;
+    const R h = (R)0.5;
+    return h * (g0p - g0m) + h * (g1p - g1m) + h * (g2p - g2m);
+}
+
 // divergence of a 3-component field g (component c at g + c*cstride) at (s,a,b)
 template <typename R>
 __device__ __forceinline__ R div_at(const R *g, int64_t cstride, const Idx3 &q, int s, int a, int b,
                                     int boundary) {
-    if (boundary == 0) {
-        if (!interior(q, s, a, b)) return (R)0;
-        return (at3(g, q, s + 1, a, b) - at3(g, q, s - 1, a, b)) / (R)2 +
-               (at3(g + cstride, q, s, a + 1, b) - at3(g + cstride, q, s, a - 1, b)) / (R)2 +
-               (at3(g + 2 * cstride, q, s, a, b + 1) - at3(g + 2 * cstride, q, s, a, b - 1)) / (R)2;
-    }
-    const R g0p = inside(q, s + 1, a, b) ? at3(g, q, s + 1, a, b) : (R)0;
-    const R g0m = inside(q, s - 1, a, b) ? at3(g, q, s - 1, a, b) : (R)0;
-    const R g1p = inside(q, s, a + 1, b) ? at3(g + cstride, q, s, a + 1, b) : (R)0;
-    const R g1m = inside(q, s, a - 1, b) ? at3(g + cstride, q, s, a - 1, b) : (R)0;
-    const R g2p = inside(q, s, a, b + 1) ? at3(g + 2 * cstride, q, s, a, b + 1) : (R)0;
-    const R g2m = inside(q, s, a, b - 1) ? at3(g + 2 * cstride, q, s, a, b - 1) : (R)0;
-    const R h = (R)0.5;
-    return h * (g0p - g0m) + h * (g1p - g1m) + h * (g2p - g2m);
+    auto get = [&](int c, int ss, int aa, int bb) { return at3(g + c * cstride, q, ss, aa, bb); };
+    return div_gen<R>(get, q, s, a, b, boundary);
 }
 
 // laplacian = div(grad u) evaluated directly from u
@@ -359,33 +367,26 @@ k_sb_lapdot(const ChunkRef *__restrict__ ch, const ProbDev *__restrict__ pd,
     }
 }
 
-// rhs = y0 - mu * div(d - b)   (src/bregman.cpp:60-62)
+// rhs = y0 - mu * div(d - b)   (src/bregman.cpp:58-61); d - b is formed at
+// each stencil point as it is read (component stride = total), rounded
+// exactly as a stored d - b would be
 template <typename R, int NT>
 __global__ void __launch_bounds__(NT)
 k_sb_rhs(const ChunkRef *__restrict__ ch, const ProbDev *__restrict__ pd,
          const SbState *__restrict__ sb, const R *__restrict__ y0, R *__restrict__ rhs,
-         const R *__restrict__ tmp, int64_t total) {
+         const R *__restrict__ D, const R *__restrict__ Bv, int64_t total) {
     const ChunkRef c = ch[blockIdx.x];
     if (!sb[c.prob].active) return;
     const ProbDev d = pd[c.prob];
-    // tmp holds d - b for the three components (component stride = total)
+    const R *dp = D + d.off, *bp = Bv + d.off;
     for (int64_t i = c.begin + threadIdx.x; i < c.end; i += NT) {
         const Idx3 q = idx_of(d, i);
-        const R *g = tmp + d.off;
-        const R dv = div_at(g, total, q, q.s, q.a, q.b, d.boundary);
+        auto dmb = [&](int k, int ss, int aa, int bb) {
+            return at3(dp + k * total, q, ss, aa, bb) - at3(bp + k * total, q, ss, aa, bb);
+        };
+        const R dv = div_gen<R>(dmb, q, q.s, q.a, q.b, d.boundary);
         rhs[i] = y0[i] - dv * (R)d.mu;
     }
-}
-
-template <typename R, int NT>
-__global__ void __launch_bounds__(NT)
-k_sb_dmb(const ChunkRef *__restrict__ ch, const SbState *__restrict__ sb,
-         const R *__restrict__ D, const R *__restrict__ Bv, R *__restrict__ tmp,
-         int64_t total) {
-    const ChunkRef c = ch[blockIdx.x];
-    if (!sb[c.prob].active) return;
-    for (int64_t i = c.begin + threadIdx.x; i < c.end; i += NT)
-        for (int k = 0; k < 3; ++k) tmp[k * total + i] = D[k * total + i] - Bv[k * total + i];
 }
 
 // isotropic (or per-component) shrink and Bregman update
@@ -995,11 +996,9 @@ void Batch2D::sb_outer_t(bool hsync, double tol, double xtol, int inner, const i
     const int np = nprob();
     const SbState *sbs = dsb_.as<SbState>();
     // rhs = y0 - mu div(d - b)
-    hipLaunchKernelGGL((k_sb_dmb<R, 256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
-                       dchunk_.as<ChunkRef>(), sbs, Dv_.as<R>(), Bv_.as<R>(), T_.as<R>(), total_);
     hipLaunchKernelGGL((k_sb_rhs<R, 256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
                        dchunk_.as<ChunkRef>(), dprob_.as<ProbDev>(), sbs, Y0_.as<R>(), Y_.as<R>(),
-                       T_.as<R>(), total_);
+                       Dv_.as<R>(), Bv_.as<R>(), total_);
     LC_HIP(hipGetLastError());
     // x = cgsolver(A - mu lap, rhs, x, inner)
     hipLaunchKernelGGL(k_cg_arm, dim3(ceil_div(np, 64)), dim3(64), 0, c_.stream(),
@@ -1034,7 +1033,6 @@ void Batch2D::split_bregman(const std::vector<float> &lambda, const std::vector<
     Bv_.reserve(3 * fb);
     XO_.reserve(fb);
     Y0_.reserve(fb);
-    T_.reserve(3 * fb);
     // d = b = 0, x_old = x0, y0 = A^T y
     LC_ETYPE([&](auto z) {
         using R = decltype(z);

@@ -229,6 +229,9 @@ template <> inline const double2 *toep_tw<double>(const ToepClass &k) { return k
 // block shapes for a class; U elements per slice (independent of the shapes)
 void toep_configure(ToepClass &k, ToepVariant v);
 ToepVariant toep_default_variant(int M);
+// fine-grid sizes of the 2-D and 3-D NUFFTs (per axis), as their init sets them
+int nufft2d_fine_n(int64_t N);
+int nufft3d_fine_n(int64_t n);
 // An odd image size N needs column blocks of at least two columns (the block
 // copies move pairs of complex values).  A column shape too small for 2 M
 // values is replaced by the first shape that holds them (block width then
@@ -348,7 +351,7 @@ class Batch2D {
     std::vector<std::unique_ptr<ToepClass>> classes_;
     int64_t total_ = 0, total_u_ = 0;
     int nchunk_ = 0, nslot_ = 0;
-    DBuf X_, R_, P_, AP_, Y_, U_, T_;
+    DBuf X_, R_, P_, AP_, Y_, U_;
     DBuf dprob_, dchunk_, dstate_, dsb_, partial_, cpartial_;
     // Split-Bregman buffers (allocated on demand)
     DBuf Dv_, Bv_, XO_, Y0_;

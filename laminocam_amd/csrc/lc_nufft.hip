@@ -521,6 +521,12 @@ __global__ void k_angle_table(const double *__restrict__ theta, double2 *__restr
     }
 }
 
+int nufft2d_fine_n(int64_t N) {
+    int64_t nf = 2 * N;
+    while (!is_2_3_smooth(nf) || nf < 2 * NU_W) ++nf;
+    return (int)nf;
+}
+
 void nufft2d_init(Ctx &c, Geom2D &g) {
     Nufft2D &u = g.nu;
     u.cs.alloc(sizeof(double2) * g.nang);
@@ -528,8 +534,7 @@ void nufft2d_init(Ctx &c, Geom2D &g) {
                        g.d_theta.as<double>(), u.cs.as<double2>(), (int)g.nang);
     LC_HIP(hipGetLastError());
     const int N = (int)g.N;
-    int64_t nf = 2 * N;
-    while (!is_2_3_smooth(nf) || nf < 2 * NU_W) ++nf;
+    const int64_t nf = nufft2d_fine_n(N);
     u.nf = (int)nf;
     u.w = NU_W;
     u.beta = 2.30 * NU_W;

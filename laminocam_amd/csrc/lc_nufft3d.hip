@@ -586,6 +586,8 @@ double es_phihat3(double beta, double xi) {
 
 } // namespace
 
+int nufft3d_fine_n(int64_t n) { return fine_size(n); }
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------

@@ -47,9 +47,11 @@ size_t request_bytes(const ServerRequest &r) {
         extra = 8 * (H + 16) * M * ns;   // W: column blocks padded to B <= 16
         if (r.has_sino) {
             const Nufft3D &u = g.nu;
+            int64_t nf[3];
+            for (int a = 0; a < 3; ++a)
+                nf[a] = u.ready ? u.nf[a] : nufft3d_fine_n(a == 0 ? g.n1 : a == 1 ? g.n2 : g.n3);
             bp = 8 * (g.nproj * g.nrows * g.ncols) * 2 +
-                 8 * ((int64_t)u.nf[0] * u.nf[1] * u.nf[2] + (int64_t)u.nf[0] * u.nf[1] * g.n3 +
-                      (int64_t)u.nf[0] * g.n2 * g.n3);
+                 8 * (nf[0] * nf[1] * nf[2] + nf[0] * nf[1] * g.n3 + nf[0] * g.n2 * g.n3);
         }
     } else {
         const Geom2D &g = *r.g2;
@@ -58,7 +60,7 @@ size_t request_bytes(const ServerRequest &r) {
         M = g.M;
         H = g.H;
         if (r.has_sino) {
-            const int64_t nf = g.nu.nf;
+            const int64_t nf = g.nu.ready ? g.nu.nf : nufft2d_fine_n(g.N);
             bp = 4 * ns * g.nang * g.nrad + 8 * ns * (g.nang * g.nrad + nf * nf + nf * N);
         }
     }

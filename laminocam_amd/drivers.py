@@ -7,6 +7,8 @@ lc_prepare_sinogram), mirroring the reference's callers of the hot path:
   prepare_sinogram  prepare_sinogram    include/mbir2d_prep.h:44-65
   MBIR2D            cpu_mbir2d / gpu::MBIR2D   src/mbir2d.cpp:66-100, src/gpu/mbir2d.cu:36-85
   MBIR              MBIR                src/mbir.cpp:34-161
+  broadcast_shifts, apply_alignment     include/config.h:284-343 (per-dataset shifts
+                                        and the alignment-file override)
 
 Normalisation, transposes, padding, back-projection, the solve and the crop
 all run on the GPU; the host only stacks pointers and angles.
@@ -25,7 +27,8 @@ from ._lib import (LC_GRAPH, LC_REG_SPLIT_BREGMAN, LC_REG_UNCONSTRAINED, LC_TV_A
                    LC_TV_ZERO_PAD, LcReconParams, LcReport)
 
 __all__ = ["Dataset", "Regularizer", "ReconParams", "Sinogram2D", "prepare_sinogram", "MBIR2D",
-           "MBIR", "padded_dims"]
+           "MBIR", "padded_dims", "estimate_memory", "volume_fits_one_gpu", "broadcast_shifts",
+           "apply_alignment", "load_alignment"]
 
 
 class Regularizer(enum.IntEnum):
@@ -199,3 +202,87 @@ def padded_dims(params: ReconParams, nrows: int, ncols: int, dtype=np.float32):
     else:
         det = (pad(int(nrows), f32, np.float32), pad(int(ncols), f32, np.float32))
     return vol + det
+
+
+def estimate_memory(shape, dtype=np.float32, regularizer=Regularizer.SPLIT_BREGMAN):
+    """The reference's analytic device-memory model of its single-device
+    solvers (scripts/estimate_bregman_memory.py:18-62 and the fit rule of
+    src/gpu/mbir2d_multi.cu:459-470) for a volume `shape`: bytes of the typical
+    and conservative high-water marks (Split-Bregman 22 N / 24 N; CG 11 N),
+    of the inputs (2 N) and of the fit rule's working set (27 N / 11 N)."""
+    L = _lib.load()
+    d = (C.c_int64 * 3)(*[int(v) for v in shape])
+    out = (C.c_double * 5)()
+    _lib.check(L.lc_reference_memory_model(d, int(np.dtype(dtype).itemsize), int(regularizer), out))
+    return {"typical_bytes": out[0], "conservative_bytes": out[1], "input_bytes": out[2],
+            "fit_rule_bytes": out[3], "N": int(out[4])}
+
+
+def volume_fits_one_gpu(params: ReconParams, dtype=np.float32, free_bytes=None, ctx=None) -> bool:
+    """gpu::volume_fits_one_gpu (include/gpu/tomocam.h:91-93): the fit rule's
+    working set (Split-Bregman 27 N, CG 11 N elements) below 0.8 of the free
+    device memory; free_bytes given: against that figure instead (no device
+    call)."""
+    L = _lib.load()
+    p = params.to_c()
+    fits = C.c_int32()
+    eb = int(np.dtype(dtype).itemsize)
+    if free_bytes is not None:
+        _lib.check(L.lc_volume_fits(C.byref(p), eb, int(free_bytes), C.byref(fits)))
+    else:
+        ctx = ctx or default_context()
+        _lib.check(L.lc_volume_fits_one_gpu(ctx.handle, C.byref(p), eb, C.byref(fits)))
+    return bool(fits.value)
+
+
+def broadcast_shifts(shifts, nproj: int):
+    """Per-projection [dx, dy] shifts of one [[input]] dataset
+    (include/config.h:284-296): none gives zeros, a single pair (the legacy
+    'cor-offset') is repeated for every projection, otherwise there must be one
+    pair per projection (ValueError)."""
+    s = np.zeros((0, 2)) if shifts is None else np.asarray(shifts, dtype=np.float64).reshape(-1, 2)
+    if len(s) == 0:
+        return np.zeros((nproj, 2))
+    if len(s) == 1:
+        return np.repeat(s, nproj, axis=0)
+    if len(s) != nproj:
+        raise ValueError(f"shifts file has {len(s)} entries but {nproj} projections were loaded")
+    return s.copy()
+
+
+def apply_alignment(dataset: Dataset, table: dict) -> Dataset:
+    """The alignment override of an [[input]] dataset (include/config.h:298-343):
+    `table` is the parsed alignment file.  'gamma_deg' and 'beta_deg' (optional)
+    replace the tilts, gamma with the parser's sign flip (gamma = -gamma_deg,
+    radians); 'corrected_angles_deg' (required) replaces the angles; 'shifts_px'
+    (optional, [dx, dy] pairs) replaces the shifts.  The shifts must then
+    number one per corrected angle.  The dataset's shifts are first completed
+    as broadcast_shifts does.  Returns a new Dataset; projections are shared."""
+    import math
+    gamma, beta = float(dataset.gamma), float(dataset.beta)
+    if "gamma_deg" in table:
+        gamma = -float(table["gamma_deg"]) * math.pi / 180.0
+    if "beta_deg" in table:
+        beta = float(table["beta_deg"]) * math.pi / 180.0
+    if "corrected_angles_deg" not in table:
+        raise ValueError("Alignment TOML missing 'corrected_angles_deg'")
+    angles = np.asarray([float(a) * math.pi / 180.0 for a in table["corrected_angles_deg"]])
+    shifts = broadcast_shifts(dataset.shifts, len(np.asarray(dataset.angles)))
+    if "shifts_px" in table:
+        pairs = []
+        for e in table["shifts_px"]:
+            if not hasattr(e, "__len__") or len(e) < 2:
+                raise ValueError("Alignment TOML 'shifts_px' entries must be [dx, dy] pairs")
+            pairs.append((float(e[0]), float(e[1])))
+        shifts = np.asarray(pairs, dtype=np.float64).reshape(-1, 2)
+    if len(shifts) != len(angles):
+        raise ValueError(f"Alignment TOML: shifts_px has {len(shifts)} entries but "
+                         f"corrected_angles_deg has {len(angles)}")
+    return Dataset(projs=dataset.projs, angles=angles, gamma=gamma, beta=beta, shifts=shifts)
+
+
+def load_alignment(path: str) -> dict:
+    """Parse an alignment TOML file (the table apply_alignment takes)."""
+    import tomli
+    with open(path, "rb") as f:
+        return tomli.load(f)

@@ -409,6 +409,21 @@ or_psf *or_psf3d_create(const REAL *theta, const REAL *gamma, const REAL *beta,
     return p;
 }
 
+/* a PSF from its real values P (CMCL order) on PSF dims d (d[0] = 1 with
+ * dim 2); the operator then applies exactly as for a computed one */
+or_psf *or_psf_from_real(int dim, const int64_t *d, const REAL *P) {
+    or_psf *p = (or_psf *)calloc(1, sizeof(or_psf));
+    p->dim = dim;
+    p->d[0] = d[0];
+    p->d[1] = d[1];
+    p->d[2] = d[2];
+    const int64_t n = d[0] * d[1] * d[2];
+    p->P = (REAL *)malloc(sizeof(REAL) * (size_t)n);
+    memcpy(p->P, P, sizeof(REAL) * (size_t)n);
+    psf_finish(p);
+    return p;
+}
+
 void or_psf_destroy(or_psf *p) {
     if (!p) return;
     free(p->P);
@@ -581,9 +596,25 @@ static double vdot(const REAL *a, const REAL *b, int64_t n) {
 }
 
 /* src/conjgrad.cpp:33-76 (unpreconditioned; the CPU pAp guard at :53-56) */
+/* or_cgsolver that also copies the iterate after iteration at[j] into
+ * xs + j n for j < nat (at ascending; an iterate the loop does not reach,
+ * because it stopped earlier, is the final x).  nat = 0: plain CG. */
+int or_cgsolver_trace(or_op_fn A, void *ctx, const REAL *y, const REAL *x0,
+                      int64_t n, int64_t max_iter, double tol, double xtol,
+                      REAL *x, or_report *rep, const int64_t *at, int64_t nat,
+                      REAL *xs);
+
 int or_cgsolver(or_op_fn A, void *ctx, const REAL *y, const REAL *x0,
                 int64_t n, int64_t max_iter, double tol, double xtol,
                 REAL *x, or_report *rep) {
+    return or_cgsolver_trace(A, ctx, y, x0, n, max_iter, tol, xtol, x, rep, NULL, 0, NULL);
+}
+
+int or_cgsolver_trace(or_op_fn A, void *ctx, const REAL *y, const REAL *x0,
+                      int64_t n, int64_t max_iter, double tol, double xtol,
+                      REAL *x, or_report *rep, const int64_t *at, int64_t nat,
+                      REAL *xs) {
+    int64_t next = 0;   /* the next snapshot */
     REAL *r = (REAL *)malloc(sizeof(REAL) * (size_t)n);
     REAL *p = (REAL *)malloc(sizeof(REAL) * (size_t)n);
     REAL *Ap = (REAL *)malloc(sizeof(REAL) * (size_t)n);
@@ -618,6 +649,10 @@ int or_cgsolver(or_op_fn A, void *ctx, const REAL *y, const REAL *x0,
         rp.iters = it + 1;
         rp.res = res;
         rp.dx = dx;
+        while (next < nat && at[next] <= it + 1) {
+            if (at[next] == it + 1) memcpy(xs + next * n, x, sizeof(REAL) * (size_t)n);
+            ++next;
+        }
         if (res < tol) {
             rp.stop = 1;
             break;
@@ -627,6 +662,8 @@ int or_cgsolver(or_op_fn A, void *ctx, const REAL *y, const REAL *x0,
             break;
         }
     }
+    for (; next < nat; ++next)   /* snapshots past the stop: the final x */
+        memcpy(xs + next * n, x, sizeof(REAL) * (size_t)n);
     if (rep) *rep = rp;
     free(r);
     free(p);

@@ -114,6 +114,8 @@ or_psf *or_psf3d_create(const REAL *theta, const REAL *gamma, const REAL *beta,
                         int64_t r1, int64_t r2, int64_t r3, int method,
                         double tol);
 void or_psf_destroy(or_psf *p);
+/* a PSF from given real values (CMCL order) on PSF dims d */
+or_psf *or_psf_from_real(int dim, const int64_t *d, const REAL *P);
 void or_psf_dims(const or_psf *p, int64_t *d3);   /* PSF dims (1,M,M) or 3-D */
 int or_psf_real(const or_psf *p, REAL *out);      /* P in CMCL order */
 int or_psf_kernel_hat(const or_psf *p, REAL *out);/* R2C half spectrum */
@@ -144,6 +146,12 @@ typedef struct {
 int or_cgsolver(or_op_fn A, void *ctx, const REAL *y, const REAL *x0,
                 int64_t n, int64_t max_iter, double tol, double xtol,
                 REAL *x_out, or_report *rep);
+/* or_cgsolver that also keeps the iterates after iterations at[0..nat)
+ * (ascending) in xs[j n ..]; an iterate past an early stop is the final x */
+int or_cgsolver_trace(or_op_fn A, void *ctx, const REAL *y, const REAL *x0,
+                      int64_t n, int64_t max_iter, double tol, double xtol,
+                      REAL *x_out, or_report *rep, const int64_t *at,
+                      int64_t nat, REAL *xs);
 int or_split_bregman(or_op_fn A, void *ctx, const REAL *y, const REAL *x0,
                      int64_t n1, int64_t n2, int64_t n3, double lambda,
                      double mu, int64_t outer, int64_t inner, double tol,

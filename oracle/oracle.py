@@ -90,6 +90,8 @@ class Oracle:
         L.or_psf3d_create.argtypes = [P, P, P, i64, i64, i64, i64, i64, i64,
                                       C.c_int, C.c_double]
         L.or_psf_destroy.argtypes = [C.c_void_p]
+        L.or_psf_from_real.restype = C.c_void_p
+        L.or_psf_from_real.argtypes = [C.c_int, C.POINTER(i64), P]
         L.or_psf_dims.argtypes = [C.c_void_p, C.POINTER(i64)]
         L.or_psf_real.argtypes = [C.c_void_p, P]
         L.or_psf_kernel_hat.argtypes = [C.c_void_p, P]
@@ -99,6 +101,9 @@ class Oracle:
         L.or_laplacian.argtypes = [P, i64, i64, i64, C.c_int, P]
         L.or_cgsolver.argtypes = [C.c_void_p, C.c_void_p, P, P, i64, i64,
                                   C.c_double, C.c_double, P, C.POINTER(Report)]
+        L.or_cgsolver_trace.argtypes = [C.c_void_p, C.c_void_p, P, P, i64, i64,
+                                        C.c_double, C.c_double, P, C.POINTER(Report),
+                                        C.POINTER(i64), i64, P]
         L.or_split_bregman.argtypes = [C.c_void_p, C.c_void_p, P, P, i64, i64,
                                        i64, C.c_double, C.c_double, i64, i64,
                                        C.c_double, C.c_double, C.c_int, C.c_int,
@@ -337,6 +342,17 @@ class Oracle:
         h = self.lib.or_psf2d_create(self._p(th), len(th), nrad, image_n, method, tol)
         return OraclePSF(self, h)
 
+    def psf_from_real(self, P):
+        """the Toeplitz operator of given real PSF values (CMCL order): a 2-D
+        PSF of shape (M, M) is applied slice by slice, a 3-D one of shape
+        (D1, D2, D3) to volumes"""
+        P = self._r(P)
+        dim = 2 if P.ndim == 2 else 3
+        if P.ndim == 2:
+            P = P[None]
+        d = (C.c_int64 * 3)(*P.shape)
+        return OraclePSF(self, self.lib.or_psf_from_real(dim, d, self._p(P)))
+
     def psf3d(self, theta, gamma, beta, nrows, ncols, recon_dims, method=0, tol=None):
         tol = self.default_tol if tol is None else tol
         th, ga, be = self._r(theta), self._r(gamma), self._r(beta)
@@ -395,6 +411,22 @@ class Oracle:
             self.lib.or_cgsolver(C.cast(fn, C.c_void_p), None, self._p(y), self._p(x0),
                                  y.size, max_iter, tol, xtol, self._p(out), C.byref(rep))
         return out, rep.as_dict()
+
+    def cgsolver_trace(self, A, y, x0, iters, tol, xtol):
+        """cgsolver on an OraclePSF up to max(iters) iterations, keeping the
+        iterate after each count in iters (ascending).  Returns (array of
+        shape (len(iters),) + y.shape, report of the whole run)."""
+        y = self._r(y)
+        x0 = self._r(x0)
+        at = (C.c_int64 * len(iters))(*[int(k) for k in iters])
+        xs = np.empty((len(iters),) + y.shape, self.dtype)
+        out = np.empty_like(y)
+        rep = Report()
+        ctx = _PsfOpCtx(A.handle, *_as3(y.shape))
+        self.lib.or_cgsolver_trace(self.psf_apply_op, C.cast(C.pointer(ctx), C.c_void_p),
+                                   self._p(y), self._p(x0), y.size, int(max(iters)), tol, xtol,
+                                   self._p(out), C.byref(rep), at, len(iters), self._p(xs))
+        return xs, rep.as_dict()
 
     # ---- Split-Bregman steps (src/bregman.cpp:58-84) ----
     def sb_rhs(self, y, d, b, mu, boundary=0):

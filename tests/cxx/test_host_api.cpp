@@ -8,6 +8,7 @@
 //                                   src/conjgrad.cpp:33-76)
 //   src/conjgrad.cpp:53-56          the |<p,Ap>| < 1e-10 guard stops a zero operator
 //   include/polar_grid2d.h:50-65    PolarGrid2D::npts, x, y
+//   include/config.h:284-343        broadcast_shifts, apply_alignment
 //   include/polar_grid.h:52-94      PolarGrid::npts, x, y, z, w against the full
 //                                   rotation R^T = (Rx(theta) Ry(beta) Rz(gamma))^T
 // Seeded inputs from a SplitMix64 stream; exits nonzero on the first failure.
@@ -170,6 +171,36 @@ int main() {
         std::printf("  PolarGrid max |point - R^T q| %.3e, masked %zu of %zu\n", md, masked, g.npts);
         CHECK("PolarGrid points and mask", g.npts == np * nr * nc && md < 1e-13 && masked == masked_ref &&
                                                masked > 0);
+    }
+
+    // ---- alignment override (include/config.h:284-343) ----
+    {
+        Dataset_t<double> ds;
+        ds.angles = {0.0, 1.0, 2.0};
+        ds.gamma = -0.5;
+        ds.shifts = {{0.25, -0.5}};   // a cor-offset: broadcast
+        AlignmentTable<double> t;
+        t.corrected_angles_deg = {0.0, 120.0, 240.0};
+        Dataset_t<double> a = ds;
+        apply_alignment(a, t);
+        bool ok = a.shifts.size() == 3 && a.shifts[2][0] == 0.25 && a.shifts[2][1] == -0.5 &&
+                  a.gamma == -0.5 && std::fabs(a.angles[1] - 120.0 * M_PI / 180) < 1e-15;
+        t.has_gamma = t.has_beta = t.has_shifts = true;
+        t.gamma_deg = 29.5;
+        t.beta_deg = -0.25;
+        t.shifts_px = {{1, 2}, {3, 4}, {5, 6}};
+        Dataset_t<double> b = ds;
+        apply_alignment(b, t);
+        ok = ok && b.gamma == -29.5 * M_PI / 180 && b.beta == -0.25 * M_PI / 180 && b.shifts[1][0] == 3;
+        t.corrected_angles_deg = {0.0, 90.0};
+        Dataset_t<double> c = ds;
+        bool threw = false;
+        try {
+            apply_alignment(c, t);
+        } catch (const std::runtime_error &) {
+            threw = true;
+        }
+        CHECK("apply_alignment", ok && threw);
     }
 
     std::printf("%s: %d failure(s)\n", g_fail ? "FAILED" : "OK", g_fail);

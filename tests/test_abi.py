@@ -3,6 +3,8 @@ declares (no compute calls: runs without a GPU)."""
 import os
 import re
 
+import numpy as np
+
 from laminocam_amd import _lib
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -99,3 +101,30 @@ def test_server_request_shapes_are_checked_before_the_library_reads_them():
     for case in bad:
         with pytest.raises(ValueError):
             lc._request_arrays(*case)
+
+
+def test_reference_memory_model_and_fit_rule():
+    # scripts/estimate_bregman_memory.py:18-62: Split-Bregman + CG high-water
+    # mark 22 N typical, 24 N conservative, 2 N inputs; the fit rule of
+    # src/gpu/mbir2d_multi.cu:459-470: 27 N (Split-Bregman) / 11 N (CG)
+    # elements below 0.8 of the free memory
+    import laminocam_amd as lc
+    from laminocam_amd.drivers import ReconParams, Regularizer
+
+    m = lc.estimate_memory((256, 256, 256), np.float32)
+    N = 256 ** 3
+    assert m["N"] == N
+    assert (m["typical_bytes"], m["conservative_bytes"], m["input_bytes"], m["fit_rule_bytes"]) == \
+        (22 * 4 * N, 24 * 4 * N, 2 * 4 * N, 27 * 4 * N)
+    m = lc.estimate_memory((21, 511, 511), np.float64, Regularizer.UNCONSTRAINED)
+    N = 21 * 511 * 511
+    assert m["typical_bytes"] == m["fit_rule_bytes"] == 11 * 8 * N
+    # 1 GiB volume in float32: SB needs 27 GiB < 0.8 free, CG 11 GiB
+    p = ReconParams(regularizer=Regularizer.SPLIT_BREGMAN, recon_dims=(1024, 512, 512))
+    need = 27 * 4 * 1024 * 512 * 512
+    edge = int(np.ceil(need / 0.8))
+    assert lc.volume_fits_one_gpu(p, np.float32, free_bytes=edge + 16)
+    assert not lc.volume_fits_one_gpu(p, np.float32, free_bytes=edge - 16)
+    assert not lc.volume_fits_one_gpu(p, np.float64, free_bytes=edge + 16)
+    p.regularizer = Regularizer.UNCONSTRAINED
+    assert lc.volume_fits_one_gpu(p, np.float64, free_bytes=int(np.ceil(11 * 8 * 1024 * 512 * 512 / 0.8)) + 16)

@@ -267,3 +267,24 @@ def test_server_admits_past_a_head_that_does_not_fit(lc):
           f"{[round(r[2][1] - r1[2][1], 3) for r in rs]}")
     assert all(r[2][1] < start2 for r in rs)
     assert np.array_equal(r1[0], r2[0])
+
+
+def test_request_bytes_is_what_admission_charges(lc, problems):
+    # lc_request_bytes: a budget of exactly that many bytes admits the
+    # request, one byte less refuses it; the reference's fit rule on the
+    # device's free memory (gpu::volume_fits_one_gpu)
+    announce("server request bytes")
+    from laminocam_amd.drivers import ReconParams, Regularizer
+    for r in (problems[0], problems[2], problems[3]):
+        kw = {k: v for k, v in r.items() if k in ("sino", "rhs")}
+        with lc.Server(hbm_budget=8 << 30, lanes=1) as srv:
+            b = srv.request_bytes(r["psf"], solver=r["solver"], **kw)
+        with lc.Server(hbm_budget=b, lanes=1) as srv:
+            x = srv.result(submit(srv, r), timeout=120.0)[0]
+            assert np.all(np.isfinite(x))
+        with lc.Server(hbm_budget=b - 1, lanes=1) as srv:
+            with pytest.raises(lc.LcError, match="above the HBM budget"):
+                submit(srv, r)
+    small = ReconParams(regularizer=Regularizer.SPLIT_BREGMAN, recon_dims=(21, 511, 511))
+    huge = ReconParams(regularizer=Regularizer.SPLIT_BREGMAN, recon_dims=(4096, 4096, 4096))
+    assert lc.volume_fits_one_gpu(small) and not lc.volume_fits_one_gpu(huge)

@@ -334,3 +334,38 @@ def test_split_bregman_single_slice_is_restarted_cg(orc64):
     for _ in range(4):
         x, _ = orc64.cgsolver(P, y, x, 1, 0.0, 0.0)
     assert rel_l2(xs, x) < 1e-12
+
+
+def test_cg_trace_equals_separate_solves(orc64):
+    # the traced iterates are the iterates of separate solves of that length
+    # (same arithmetic, bitwise), and a snapshot past an early stop is the
+    # final x
+    rng = np.random.default_rng(41)
+    N, th = 24, np.arange(17) * np.pi / 17
+    P = orc64.psf2d(th, N, N, method=0)
+    y = P.convolve(rng.random((1, N, N)))
+    ks = [1, 3, 4, 9]
+    xs, rep = orc64.cgsolver_trace(P, y, np.zeros_like(y), ks, 0.0, 0.0)
+    for j, k in enumerate(ks):
+        xk, _ = orc64.cgsolver(P, y, np.zeros_like(y), k, 0.0, 0.0)
+        assert np.array_equal(xs[j], xk), k
+    xe, re = orc64.cgsolver(P, y, np.zeros_like(y), 50, 1e-3, 0.0)
+    assert re["iters"] < 40
+    xt, _ = orc64.cgsolver_trace(P, y, np.zeros_like(y), [re["iters"], 40, 50], 1e-3, 0.0)
+    assert np.array_equal(xt[1], xe) and np.array_equal(xt[2], xe) and np.array_equal(xt[0], xe)
+
+
+def test_psf_from_real_applies_as_the_computed_psf(orc64):
+    # an operator rebuilt from the real PSF values applies exactly as the
+    # computed one (2-D and 3-D)
+    rng = np.random.default_rng(43)
+    th = np.arange(11) * np.pi / 11
+    P = orc64.psf2d(th, 20, 20, method=0)
+    Q = orc64.psf_from_real(P.real()[0])
+    x = rng.random((2, 20, 20))
+    assert np.array_equal(Q.convolve(x), P.convolve(x))
+    th3 = np.arange(7) * 2 * np.pi / 7
+    P3 = orc64.psf3d(th3, np.full(7, -0.5), np.zeros(7), 6, 12, (3, 12, 12), method=0)
+    Q3 = orc64.psf_from_real(P3.real())
+    x3 = rng.random((3, 12, 12))
+    assert np.array_equal(Q3.convolve(x3), P3.convolve(x3))
