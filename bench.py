@@ -83,23 +83,25 @@ def build_problem(lc, synth, rng, Ns, As):
     return geoms, rhs
 
 
-ERR_HORIZONS = (5, 10, 20, 30, 40)
+ERR_HORIZONS = (5, 8, 10, 15, 20, 30, 40)
 
 
 def rel_err_by_class(args, lc, synth, Ns, As, psfs, rhs, xb):
     """Relative L2 error against the FP64 oracle on the first slice of every
-    (N, angle count) class of the workload, at the horizons 5, 10, ... up to
-    the workload's iteration count.  Beside it, two FP64 realisations that
-    differ from the oracle in rounding only: the oracle rerun on a right-hand
-    side perturbed by 1e-15 relative, and the GPU's FP64 path.  Where they
-    move away from the oracle by more than 1e-3, rounding alone fixes the
-    result only to that floor; the horizon is then the last iteration count at
-    which both agree with the oracle to 1e-3, and the GPU's FP32 error is
-    reported there as well.  The FP32 restatement of the reference path
-    (oracle in float at the reference's float NUFFT tolerance) is reported at
-    the full horizon.  The (256, 180) class is the plain 2-D configuration
-    (256^2 ellipse phantom, 180 angles, 256 bins, CG 50 iterations) and is
-    also reported on its own, with its plain GPU and CPU solve times."""
+    (N, angle count) class of the workload, at the horizons 5, 8, 10, ... up to
+    the workload's iteration count, for the GPU's FP32 path, the GPU's FP64
+    path (an FP64 realisation whose roundings differ from the oracle's: other
+    FFT algorithm, other summation order) and the FP32 restatement of the
+    reference's own CPU path (oracle in float at the reference's float NUFFT
+    tolerance, full horizon only).  The oracle rerun on a right-hand side
+    perturbed by 1e-15 relative is reported too; it repeats the oracle's
+    roundings, so it stays closer than any independent realisation.  Per
+    class: the longest horizon at which the GPU's FP32 result is within 1e-3
+    of the oracle, and the longest at which the FP64 realisation is (past it,
+    rounding alone moves a result by more than 1e-3).  The (256, 180) class is
+    the plain 2-D configuration (256^2 ellipse phantom, 180 angles, 256 bins,
+    CG 50 iterations) and is also reported on its own, with its plain GPU and
+    CPU solve times."""
     from oracle.oracle import Oracle
     o64, o32 = Oracle(64), Oracle(32)
     H = [k for k in ERR_HORIZONS if k < args.iters] + [args.iters]
@@ -128,8 +130,9 @@ def rel_err_by_class(args, lc, synth, Ns, As, psfs, rhs, xb):
             e32.append(rel_l2(g32, xo[j]))
             e64.append(rel_l2(g64, xo[j]))
             fl.append(rel_l2(xp[j], xo[j]))
-        agree = [k for j, k in enumerate(H) if max(fl[j], e64[j]) <= 1e-3]
+        agree = [k for j, k in enumerate(H) if e64[j] <= 1e-3]
         hz = max(agree) if agree else None
+        ok32 = [k for j, k in enumerate(H) if e32[j] <= 1e-3]
         P32 = o32.psf2d(th, N, N, method=1)
         t0 = time.perf_counter()
         x32, _ = o32.cgsolver(P32, y, np.zeros_like(y), args.iters, args.tol, args.xtol)
@@ -137,7 +140,8 @@ def rel_err_by_class(args, lc, synth, Ns, As, psfs, rhs, xb):
         del P32
         row = {"N": N, "nang": a, "fp32": r3(e32[-1]), "fp64_gpu": r3(e64[-1]),
                "fp64_rerun": r3(fl[-1]), "fp32_port": r3(rel_l2(x32, xo[-1])),
-               "horizon": hz, "fp32_at_horizon": r3(e32[H.index(hz)]) if hz else None,
+               "fp32_horizon_1e-3": max(ok32) if ok32 else None,
+               "fp64_horizon_1e-3": hz, "fp32_at_fp64_horizon": r3(e32[H.index(hz)]) if hz else None,
                "oracle_iters": int(ro["iters"]),
                "by_iters": {"fp32": [r3(e) for e in e32], "fp64_gpu": [r3(e) for e in e64],
                             "fp64_rerun": [r3(e) for e in fl]}}
@@ -156,30 +160,33 @@ def rel_err_by_class(args, lc, synth, Ns, As, psfs, rhs, xb):
                     "gpu_plain_slices_per_s": round(1.0 / float(np.median(ts)), 2),
                     "cpu_port_s": round(t_port, 3), "cpu_threads": o32t,
                     "rel_err_vs_fp64": {k: row[k] for k in ("fp32", "fp64_gpu", "fp64_rerun",
-                                                            "fp32_port", "horizon",
-                                                            "fp32_at_horizon")},
+                                                            "fp32_port", "fp32_horizon_1e-3",
+                                                            "fp64_horizon_1e-3")},
                     "note": "gpu_plain: lc_cg2d, host arrays in and out, host-synchronised "
                             "iterations; cpu_port: the FP32 C restatement of the reference "
                             "path (PSF excluded)"}
         log(f"err class N={N} nang={a}: fp32 {e32[-1]:.2e} fp64 gpu {e64[-1]:.2e} rerun "
-            f"{fl[-1]:.2e} port {row['fp32_port']:.2e} horizon {hz} fp32@h {row['fp32_at_horizon']}")
-    ok = lambda r: r["fp32"] <= 1e-3 or (max(r["fp64_gpu"], r["fp64_rerun"]) > 1e-3 and
-                                         r["horizon"] is not None and r["fp32_at_horizon"] <= 1e-3)
+            f"{fl[-1]:.2e} port {row['fp32_port']:.2e}; within 1e-3 up to {row['fp32_horizon_1e-3']} "
+            f"it (FP32), {hz} it (FP64)")
+    h32 = [r["fp32_horizon_1e-3"] or 0 for r in rows]
+    h64 = [r["fp64_horizon_1e-3"] or 0 for r in rows]
     summary = {
-        "fp32_within_1e-3": sum(r["fp32"] <= 1e-3 for r in rows),
-        "fp64_floor_above_1e-3": sum(max(r["fp64_gpu"], r["fp64_rerun"]) > 1e-3 for r in rows),
-        "fp32_within_1e-3_at_horizon": sum(r["horizon"] is not None and r["fp32_at_horizon"] <= 1e-3
-                                           for r in rows),
-        "classes_not_accounted": [[r["N"], r["nang"]] for r in rows if not ok(r)],
+        "fp32_within_1e-3_at_full_horizon": sum(r["fp32"] <= 1e-3 for r in rows),
+        "fp64_gpu_within_1e-3_at_full_horizon": sum(r["fp64_gpu"] <= 1e-3 for r in rows),
+        "fp32_horizon_1e-3_min_max": [min(h32), max(h32)],
+        "fp64_horizon_1e-3_min_max": [min(h64), max(h64)],
+        "fp32_over_reference_float_path_median": r3(float(np.median(
+            [r["fp32"] / max(r["fp32_port"], 1e-30) for r in rows]))),
         "seconds": round(time.perf_counter() - t_start, 1)}
     return {"iters": args.iters, "horizons": H, "classes": rows, "summary": summary,
             "config1": cfg1,
             "note": "per class (first slice of each N x angles class): fp32 = GPU FP32 path "
                     "(the batched result), fp64_gpu = GPU FP64 path, fp64_rerun = FP64 oracle on "
-                    "a right-hand side perturbed by 1e-15, fp32_port = FP32 restatement of the "
-                    "reference CPU path, all against the FP64 oracle at the full horizon; "
-                    "horizon = last iteration count (of 'horizons') at which both FP64 "
-                    "realisations agree with the oracle to 1e-3; by_iters at every horizon"}
+                    "a right-hand side perturbed by 1e-15 (same roundings, so not a floor), "
+                    "fp32_port = FP32 restatement of the reference CPU path, all against the FP64 "
+                    "oracle at the full horizon; fp32_horizon_1e-3 / fp64_horizon_1e-3 = last "
+                    "iteration count of 'horizons' at which that path is within 1e-3 of the "
+                    "oracle; by_iters at every horizon"}
 
 
 def rel_l2(a, b):
@@ -237,8 +244,9 @@ def served_stream(args, lc, synth, ctx, frac3d=None, rate=None, sweep=None):
     Inputs come from a pool of seeded phantoms per geometry built before the
     clock.  frac3d / rate override --serve-3d-frac / --serve-rate; sweep is a
     list of arrival rates run for --serve-sweep-seconds each, whose highest
-    rate the server keeps up with (95% of the offered requests per second, p99
-    latency under 1 s) is reported as the saturating rate."""
+    rate the server keeps up with (90% of the offered requests per second over
+    the span from the first submission to the last result, p99 latency under
+    1 s) is reported as the saturating rate."""
     frac3d = args.serve_3d_frac if frac3d is None else frac3d
     rate = args.serve_rate if rate is None else rate
     import threading
@@ -328,7 +336,7 @@ def served_stream(args, lc, synth, ctx, frac3d=None, rate=None, sweep=None):
         for r in (sweep or []):
             m = max(1, int(round(r * args.serve_sweep_seconds)))
             rec = run(srv, make_requests(m, (5 << 16) + 1000 + int(r)), r)
-            rec["keeps_up"] = bool(rec["requests_per_s"] >= 0.95 * r and
+            rec["keeps_up"] = bool(rec["requests_per_s"] >= 0.9 * r and
                                    rec["latency_ms"]["p99"] < 1000.0)
             sweep_out.append(rec)
             log(f"served sweep {r:g} req/s: {rec['slices_per_s']} slices/s, p50 "
@@ -910,11 +918,12 @@ def main():
     err = None
     if not args.no_err:
         err = rel_err_by_class(args, lc, synth, Ns, As, psfs, rhs, xb)
-        log(f"rel err vs FP64 oracle ({args.iters} it), per class: GPU FP32 <= 1e-3 in "
-            f"{err['summary']['fp32_within_1e-3']} of {len(err['classes'])} classes; FP64 "
-            f"realisations disagree by > 1e-3 in {err['summary']['fp64_floor_above_1e-3']}; at the "
-            f"horizon where they agree the GPU FP32 error is <= 1e-3 in "
-            f"{err['summary']['fp32_within_1e-3_at_horizon']}")
+        sm = err["summary"]
+        log(f"rel err vs FP64 oracle, {len(err['classes'])} classes: within 1e-3 up to "
+            f"{sm['fp32_horizon_1e-3_min_max']} iterations (FP32), {sm['fp64_horizon_1e-3_min_max']} "
+            f"(FP64); at {args.iters} it FP32 within 1e-3 in {sm['fp32_within_1e-3_at_full_horizon']}, "
+            f"FP64 in {sm['fp64_gpu_within_1e-3_at_full_horizon']}; FP32 / reference float path "
+            f"error ratio (median) {sm['fp32_over_reference_float_path_median']}")
 
     # ---- served mixed stream ----
     served = None
