@@ -45,13 +45,17 @@ template <int NT> __device__ __forceinline__ double block_sum(double v, double *
     return t;   // valid in thread 0
 }
 
-// one wave per problem: fixed-order sum of partial[stride*i + comp] over [b, e)
-__device__ __forceinline__ double wave_sum(const double *partial, int b, int e, int stride,
-                                           int comp) {
+// threads per problem of the scalar kernels (k_cg_fin*, k_sb_fin)
+constexpr int FIN_NT = 256;
+
+// one block of FIN_NT threads per problem: fixed-order sum of
+// partial[stride*i + comp] over [b, e) (thread t takes i = b + t, b + t + FIN_NT,
+// ... in order, then the block tree); valid in thread 0; every thread calls it
+__device__ __forceinline__ double range_sum(const double *partial, int b, int e, int stride,
+                                            int comp, double *sh) {
     double v = 0.0;
-    for (int i = b + (int)threadIdx.x; i < e; i += 64) v += partial[(int64_t)i * stride + comp];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    return __shfl(v, 0, 64);
+    for (int i = b + (int)threadIdx.x; i < e; i += FIN_NT) v += partial[(int64_t)i * stride + comp];
+    return block_sum<FIN_NT>(v, sh);
 }
 
 template <typename R> __global__ void k_fill(R *p, R v, int64_t n) {
@@ -107,8 +111,9 @@ __global__ void k_cg_fin0(const ProbDev *__restrict__ pd, CgState *st, const dou
     const int p = blockIdx.x;
     CgState s = st[p];
     if (!s.active) return;
-    const double rr = wave_sum(cpart, pd[p].chunk0, pd[p].chunk0 + pd[p].nchunk, 4, 0);
-    const double yy = wave_sum(cpart, pd[p].chunk0, pd[p].chunk0 + pd[p].nchunk, 4, 1);
+    __shared__ double sh[FIN_NT / 64];
+    const double rr = range_sum(cpart, pd[p].chunk0, pd[p].chunk0 + pd[p].nchunk, 4, 0, sh);
+    const double yy = range_sum(cpart, pd[p].chunk0, pd[p].chunk0 + pd[p].nchunk, 4, 1, sh);
     if (threadIdx.x == 0) {
         s.rho = rr;
         s.ynorm = sqrt(yy);
@@ -123,9 +128,10 @@ __global__ void k_cg_fin1(const ProbDev *__restrict__ pd, CgState *st, const dou
     const int p = blockIdx.x;
     CgState s = st[p];
     if (!s.active) return;
+    __shared__ double sh[FIN_NT / 64];
     const double pap = use_chunks
-                           ? wave_sum(cpart, pd[p].chunk0, pd[p].chunk0 + pd[p].nchunk, 4, 3)
-                           : wave_sum(slot_part, pd[p].slot0, pd[p].slot0 + pd[p].nslot, 1, 0);
+                           ? range_sum(cpart, pd[p].chunk0, pd[p].chunk0 + pd[p].nchunk, 4, 3, sh)
+                           : range_sum(slot_part, pd[p].slot0, pd[p].slot0 + pd[p].nslot, 1, 0, sh);
     if (threadIdx.x == 0) {
         s.pAp = pap;
         if (fabs(pap) < 1.0e-10) {
@@ -226,9 +232,10 @@ __global__ void k_cg_fin2(const ProbDev *__restrict__ pd, CgState *st, const dou
     CgState s = st[p];
     if (!s.active) return;
     const int b = pd[p].chunk0, e = pd[p].chunk0 + pd[p].nchunk;
-    const double rr = wave_sum(cpart, b, e, 4, 0);
-    const double ss = wave_sum(cpart, b, e, 4, 1);
-    const double xx = wave_sum(cpart, b, e, 4, 2);
+    __shared__ double sh[FIN_NT / 64];
+    const double rr = range_sum(cpart, b, e, 4, 0, sh);
+    const double ss = range_sum(cpart, b, e, 4, 1, sh);
+    const double xx = range_sum(cpart, b, e, 4, 2, sh);
     if (threadIdx.x == 0) {
         s.beta = rr / s.rho;
         s.rho = rr;
@@ -446,8 +453,9 @@ __global__ void k_sb_fin(const ProbDev *__restrict__ pd, SbState *sb, const doub
     SbState s = sb[p];
     if (!s.active) return;
     const int b = pd[p].chunk0, e = pd[p].chunk0 + pd[p].nchunk;
-    const double dn = wave_sum(cpart, b, e, 4, 0);
-    const double on = wave_sum(cpart, b, e, 4, 1);
+    __shared__ double sh[FIN_NT / 64];
+    const double dn = range_sum(cpart, b, e, 4, 0, sh);
+    const double on = range_sum(cpart, b, e, 4, 1, sh);
     if (threadIdx.x == 0) {
         s.norm_diff = sqrt(dn) / (sqrt(on) + 1e-8);
         s.outer += 1;
@@ -911,7 +919,7 @@ void Batch2D::cg_setup(bool x0_zero, bool sb, const std::vector<int> &max_iter, 
                            P_.as<R>(), X_.as<R>(), cpartial_.as<double>(), x0_zero ? 1 : 0);
     });
     LC_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_cg_fin0, dim3(nprob()), dim3(64), 0, c_.stream(), dprob_.as<ProbDev>(),
+    hipLaunchKernelGGL(k_cg_fin0, dim3(nprob()), dim3(FIN_NT), 0, c_.stream(), dprob_.as<ProbDev>(),
                        st, cpartial_.as<double>());
     LC_HIP(hipGetLastError());
 }
@@ -939,14 +947,14 @@ template <typename R> void Batch2D::cg_iteration_t(bool host_sync, bool sb) {
                            cpartial_.as<double>(), 1);
         LC_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_cg_fin1, dim3(nprob()), dim3(64), 0, c_.stream(), dprob_.as<ProbDev>(),
+    hipLaunchKernelGGL(k_cg_fin1, dim3(nprob()), dim3(FIN_NT), 0, c_.stream(), dprob_.as<ProbDev>(),
                        st, partial_.as<double>(), cpartial_.as<double>(), sb ? 1 : 0);
     LC_HIP(hipGetLastError());
     hs();
     hipLaunchKernelGGL((k_cg_update<R, CG_UPD_NT>), dim3(nchunk_), dim3(CG_UPD_NT), 0, c_.stream(),
                        dchunk_.as<ChunkRef>(), st, X, Rv, P, AP, cpartial_.as<double>());
     LC_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_cg_fin2, dim3(nprob()), dim3(64), 0, c_.stream(), dprob_.as<ProbDev>(),
+    hipLaunchKernelGGL(k_cg_fin2, dim3(nprob()), dim3(FIN_NT), 0, c_.stream(), dprob_.as<ProbDev>(),
                        st, cpartial_.as<double>());
     LC_HIP(hipGetLastError());
     hs();
@@ -1010,7 +1018,7 @@ void Batch2D::sb_outer_t(bool hsync, double tol, double xtol, int inner, const i
                        dchunk_.as<ChunkRef>(), dprob_.as<ProbDev>(), sbs, X_.as<R>(), Dv_.as<R>(),
                        Bv_.as<R>(), XO_.as<R>(), cpartial_.as<double>(), total_,
                        sb_aniso_ ? 1 : 0);
-    hipLaunchKernelGGL(k_sb_fin, dim3(np), dim3(64), 0, c_.stream(), dprob_.as<ProbDev>(),
+    hipLaunchKernelGGL(k_sb_fin, dim3(np), dim3(FIN_NT), 0, c_.stream(), dprob_.as<ProbDev>(),
                        dsb_.as<SbState>(), cpartial_.as<double>());
     LC_HIP(hipGetLastError());
 }
