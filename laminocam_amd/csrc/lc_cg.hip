@@ -20,6 +20,7 @@
 // arithmetic) or double (the FP64 path of the reference's Array<double>
 // instantiations, src/conjgrad.cpp:82-85, src/bregman.cpp:101-104).
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 
@@ -537,10 +538,13 @@ void fd_launch(Ctx &c, int op, const double *in, double *out, int n1, int n2, in
 // ----------------------------------------------------------------------------
 // Batch2D
 // ----------------------------------------------------------------------------
-Batch2D::Batch2D(Ctx &c, const std::vector<ProbSpec> &probs, bool f64)
+Batch2D::Batch2D(Ctx &c, const std::vector<ProbSpec> &probs, bool f64, bool class_streams)
     : c_(c), f64_(f64), probs_(probs) {
     LC_REQUIRE(!probs.empty(), "batch: no problems");
     build();
+    const char *e = std::getenv("LC_CLASS_STREAMS");
+    class_streams_ = class_streams && !(e && e[0] == '0') && classes_.size() > 1;
+    if (class_streams_) c_.ensure_aux();
 }
 
 // call f(R{}) with R the batch's element type
@@ -699,14 +703,34 @@ void Batch2D::build() {
     c_.sync();
 }
 
+// The Toeplitz passes of the size classes touch disjoint parts of the batch
+// vectors, U and the slot partials, so the classes run on the context's main
+// and auxiliary streams side by side (class i on stream i mod 4), joined back
+// into the main stream before anything reads their results.
+template <typename F> void Batch2D::for_classes(F &&launch) {
+    const int nk = (int)classes_.size();
+    const int ns = class_streams_ ? std::min(nk, 1 + Ctx::kAux) : 1;
+    if (ns <= 1) {
+        for (auto &k : classes_) launch(*k);
+        return;
+    }
+    c_.fork(ns - 1);
+    for (int i = 0; i < nk; ++i) {
+        const int s = i % ns;
+        Ctx::StreamScope sc(c_, s == 0 ? c_.main_stream() : c_.aux(s - 1));
+        launch(*classes_[i]);
+    }
+    c_.join(ns - 1);
+}
+
 template <typename R> void Batch2D::apply_op_t(const R *src, R *dst, bool sb, bool dot) {
     CgState *st = dstate_.as<CgState>();
     cpx<R> *U = U_.as<cpx<R>>();
-    for (auto &k : classes_) {
-        toep_launch_fwd<R>(c_, *k, st, src, nullptr, nullptr, U, false);
-        toep_mid<R>(*k, st, U);
-        toep_launch_inv<R>(c_, *k, st, U, dst, src, partial_.as<double>(), dot && !sb);
-    }
+    for_classes([&](ToepClass &k) {
+        toep_launch_fwd<R>(c_, k, st, src, nullptr, nullptr, U, false);
+        toep_mid<R>(k, st, U);
+        toep_launch_inv<R>(c_, k, st, U, dst, src, partial_.as<double>(), dot && !sb);
+    });
     if (sb) {
         hipLaunchKernelGGL((k_sb_lapdot<R, 256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
                            dchunk_.as<ChunkRef>(), dprob_.as<ProbDev>(), st, src, dst,
@@ -936,11 +960,11 @@ template <typename R> void Batch2D::cg_iteration_t(bool host_sync, bool sb) {
             c_.sync();
         }
     };
-    for (auto &k : classes_) {
-        toep_launch_fwd<R>(c_, *k, st, P, Rv, P, U, true);
-        toep_mid<R>(*k, st, U);
-        toep_launch_inv<R>(c_, *k, st, U, AP, P, partial_.as<double>(), !sb);
-    }
+    for_classes([&](ToepClass &k) {
+        toep_launch_fwd<R>(c_, k, st, P, Rv, P, U, true);
+        toep_mid<R>(k, st, U);
+        toep_launch_inv<R>(c_, k, st, U, AP, P, partial_.as<double>(), !sb);
+    });
     if (sb) {
         hipLaunchKernelGGL((k_sb_lapdot<R, 256>), dim3(nchunk_), dim3(256), 0, c_.stream(),
                            dchunk_.as<ChunkRef>(), dprob_.as<ProbDev>(), st, P, AP,

@@ -278,7 +278,10 @@ void toep_launch_inv(Ctx &c, const ToepClass &k, const CgState *st, const cpx<R>
 // ---------------------------------------------------------------------------
 class Batch2D {
   public:
-    Batch2D(Ctx &c, const std::vector<ProbSpec> &probs, bool f64 = false);
+    // class_streams: run the size classes' Toeplitz passes on the context's
+    // auxiliary streams side by side (off where several contexts share the
+    // device's hardware queues, as the server's lanes do)
+    Batch2D(Ctx &c, const std::vector<ProbSpec> &probs, bool f64 = false, bool class_streams = true);
     bool f64() const { return f64_; }
     size_t esize() const { return f64_ ? sizeof(double) : sizeof(float); }
     int nprob() const { return (int)probs_.size(); }
@@ -338,6 +341,9 @@ class Batch2D {
                   double xtol);
     void apply_op(const void *src, void *dst, bool sb, bool dot);
     template <typename R> void apply_op_t(const R *src, R *dst, bool sb, bool dot);
+    template <typename F> void for_classes(F &&launch);
+    // size classes on parallel streams (LC_CLASS_STREAMS=0 in the environment: one stream)
+    bool class_streams_ = false;
     // middle pass of a class: 2-D column pass or the 3-D slice-coupled pass
     template <typename R> void toep_mid(const ToepClass &k, const CgState *st, cpx<R> *U);
     template <typename R> void sb_outer_t(bool hsync, double tol, double xtol, int inner,

@@ -11,14 +11,45 @@ Ctx::Ctx(int device) : device_(device) {
     // default-priority, non-blocking stream
     std::lock_guard<std::recursive_mutex> cl(capture_mutex());
     LC_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    main_ = stream_;
+}
+
+void Ctx::ensure_aux() {
+    if (fork_ev_) return;
+    std::lock_guard<std::recursive_mutex> cl(capture_mutex());
+    for (int i = 0; i < kAux; ++i) {
+        LC_HIP(hipStreamCreateWithFlags(&aux_[i], hipStreamNonBlocking));
+        LC_HIP(hipEventCreateWithFlags(&join_ev_[i], hipEventDisableTiming));
+    }
+    LC_HIP(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming));
+}
+
+void Ctx::fork(int n) {
+    LC_REQUIRE(fork_ev_ != nullptr && n <= kAux, "context: fork before ensure_aux");
+    LC_HIP(hipEventRecord(fork_ev_, stream_));
+    for (int i = 0; i < n; ++i) LC_HIP(hipStreamWaitEvent(aux_[i], fork_ev_, 0));
+}
+
+void Ctx::join(int n) {
+    for (int i = 0; i < n; ++i) {
+        LC_HIP(hipEventRecord(join_ev_[i], aux_[i]));
+        LC_HIP(hipStreamWaitEvent(stream_, join_ev_[i], 0));
+    }
 }
 
 Ctx::~Ctx() {
     (void)hipSetDevice(device_);
     if (stream_) (void)hipStreamSynchronize(stream_);
+    for (int i = 0; i < kAux; ++i)
+        if (aux_[i]) (void)hipStreamSynchronize(aux_[i]);
     tw_.clear();
     scratch_.clear();
     std::lock_guard<std::recursive_mutex> cl(capture_mutex());
+    for (int i = 0; i < kAux; ++i) {
+        if (join_ev_[i]) (void)hipEventDestroy(join_ev_[i]);
+        if (aux_[i]) (void)hipStreamDestroy(aux_[i]);
+    }
+    if (fork_ev_) (void)hipEventDestroy(fork_ev_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 

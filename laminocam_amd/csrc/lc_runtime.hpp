@@ -230,9 +230,35 @@ class Ctx {
 
     std::mutex mu;   // serialises API calls that share this context
 
+    // Auxiliary streams (default priority) on which independent work of one
+    // call may run beside the main stream: fork(n) makes the first n wait for
+    // the main stream's work so far, join(n) makes the main stream wait for
+    // theirs.  Inside a stream capture these become graph dependencies.
+    // They are created on first request (ensure_aux, outside any capture):
+    // a process gets few hardware queues (four by default) and streams beyond
+    // them share one, so a context that never forks holds one stream only.
+    static constexpr int kAux = 3;
+    void ensure_aux();
+    hipStream_t aux(int i) const { return aux_[i]; }
+    void fork(int n);
+    void join(int n);
+    // the stream launches go to (the main stream unless a StreamScope is active)
+    class StreamScope {
+      public:
+        StreamScope(Ctx &c, hipStream_t s) : c_(c), prev_(c.stream_) { c.stream_ = s; }
+        ~StreamScope() { c_.stream_ = prev_; }
+
+      private:
+        Ctx &c_;
+        hipStream_t prev_;
+    };
+    hipStream_t main_stream() const { return main_; }
+
   private:
     int device_;
-    hipStream_t stream_ = nullptr;
+    hipStream_t stream_ = nullptr, main_ = nullptr;
+    hipStream_t aux_[kAux] = {};
+    hipEvent_t fork_ev_ = nullptr, join_ev_[kAux] = {};
     std::map<std::pair<int, int>, DBuf> tw_;
     std::vector<DBuf> scratch_;
 };

@@ -353,7 +353,8 @@ std::vector<Server::OutCopy> Server::run_batch(Lane &lane, std::vector<ServerReq
     std::vector<ProbSpec> ps;
     for (ServerRequest *r : batch)
         ps.push_back(r->g3 ? ProbSpec{nullptr, (int)r->g3->n1, r->g3} : ProbSpec{r->g2, r->ns});
-    Batch2D b(c, ps);
+    // one stream per lane: the lanes already fill the hardware queues
+    Batch2D b(c, ps, false, false);
     const double t1 = now_s();
     // inputs: right-hand sides (or sinograms back-projected into y), initial
     // guesses.  Every host array of the batch is packed into the lane's pinned

@@ -250,13 +250,25 @@ def test_server_admits_past_a_head_that_does_not_fit(lc):
     # request runs, the second does not fit, and small requests behind it
     # start before it instead of waiting
     announce("server admission past the head")
-    pb, yb = _stack_problem(lc, 256, 32, 180, 6)
+    # the large requests run long (16 x 512^2, 1000 Split-Bregman outer
+    # iterations with the stopping tests off, so no early exit) and the small
+    # ones are submitted well inside the first
+    pb, yb = _stack_problem(lc, 512, 16, 180, 6)
     ps, ys = _stack_problem(lc, 64, 1, 90, 7)
-    big_bytes = 32 * (20 * 256 * 256 + 8 * (257 + 16) * 256) + (1 << 20)
+    with lc.Server(hbm_budget=8 << 30, lanes=1) as probe:
+        big_bytes = probe.request_bytes(pb, rhs=yb, solver="sb")
     with lc.Server(hbm_budget=int(1.5 * big_bytes), lanes=2) as srv:
-        t1 = srv.submit(pb, rhs=yb, solver="cg", max_iter=400, tol=0.0, xtol=0.0)
+        # warm both lanes (context, twiddles, graph capture) with two small
+        # requests submitted together that cannot share a batch
+        warm = [srv.submit(ps, rhs=ys, solver="cg", max_iter=2),
+                srv.submit(ps, rhs=ys, solver="sb", lam=0.05, mu=10.0, outer=1, inner=1)]
+        for t in warm:
+            srv.result(t, timeout=60.0)
+        t1 = srv.submit(pb, rhs=yb, solver="sb", lam=0.05, mu=10.0, outer=1000, inner=1,
+                        tol=0.0, xtol=0.0)
         time.sleep(0.01)
-        t2 = srv.submit(pb, rhs=yb, solver="cg", max_iter=400, tol=0.0, xtol=0.0)
+        t2 = srv.submit(pb, rhs=yb, solver="sb", lam=0.05, mu=10.0, outer=1000, inner=1,
+                        tol=0.0, xtol=0.0)
         ts = [srv.submit(ps, rhs=ys, solver="sb", lam=0.05, mu=10.0, outer=3, inner=1)
               for _ in range(4)]
         r1 = srv.result(t1, timeout=120.0)
@@ -266,7 +278,7 @@ def test_server_admits_past_a_head_that_does_not_fit(lc):
     print(f"big request 2 starts {start2 - r1[2][1]:.3f} s after big 1; smalls start "
           f"{[round(r[2][1] - r1[2][1], 3) for r in rs]}")
     assert all(r[2][1] < start2 for r in rs)
-    assert np.array_equal(r1[0], r2[0])
+    assert r1[1]["outer"] == r2[1]["outer"] == 1000 and np.array_equal(r1[0], r2[0])
 
 
 def test_request_bytes_is_what_admission_charges(lc, problems):
