@@ -419,6 +419,39 @@ int lc_autotune2d(lc_ctx ctx, lc_geom2d g, int32_t nslices, int32_t *variant, do
 int lc_set_tuning2d(lc_ctx ctx, int64_t image_n, const int32_t *variant);
 int lc_get_tuning2d(lc_ctx ctx, int64_t image_n, int32_t *variant);
 
+/* ---- file formats (host only; no context or device needed) ----
+ * TIFF stacks replace include/tiff.h:31-117 (tomocam::tiff::read / write over
+ * libtiff).  Samples are single 32-bit IEEE floats; pages are slices.
+ * lc_tiff_info: page count and the (first) page's height and width; every
+ * page must have that size.  Reads classic TIFF and BigTIFF, either byte
+ * order, strips (not tiles), compression none, LZW, deflate or PackBits, no
+ * predictor (LC_EUNSUP otherwise).
+ * lc_tiff_read: the first `pages` pages, each cropped to its top-left rows x
+ * cols (rows <= height, cols <= width), into out[pages][rows][cols].  The
+ * reference's reader makes even heights and widths odd by dropping the last
+ * row / column; its C++ and Python wrappers here pass those sizes.
+ * lc_tiff_write: data[pages][rows][cols] as little-endian pages of one
+ * uncompressed strip each, with the tags of the reference's writer
+ * (MinIsBlack, top-left orientation, contiguous planes, IEEE float samples);
+ * BigTIFF when the file reaches 4 GiB.
+ * HDF5 replaces include/hdfread.h:34-96 (tomocam::h5::read_images /
+ * read_angles over libhdf5), for numeric datasets: superblock versions 0-3,
+ * symbol-table and compact-link groups, integer and IEEE float elements of
+ * either byte order, contiguous / compact / chunked storage (v1 B-tree,
+ * single-chunk, implicit and unpaged fixed-array indexes), filters deflate,
+ * shuffle and Fletcher-32 (LC_EUNSUP otherwise).
+ * lc_h5_shape: rank (<= 32) and dims of the dataset at the absolute path
+ * `dataset` (for example "/images", "/coords/alpha").
+ * lc_h5_read: all count elements, converted to float32 (f64 = 0, as the
+ * reference's H5T_NATIVE_FLOAT reads) or float64 (f64 = 1), into out.
+ * Errors: a missing file or object, a malformed file and a size mismatch are
+ * LC_EINVAL. */
+int lc_tiff_info(const char *path, int64_t *pages, int64_t *height, int64_t *width);
+int lc_tiff_read(const char *path, float *out, int64_t pages, int64_t rows, int64_t cols);
+int lc_tiff_write(const char *path, const float *data, int64_t pages, int64_t rows, int64_t cols);
+int lc_h5_shape(const char *path, const char *dataset, int32_t *rank, int64_t *dims);
+int lc_h5_read(const char *path, const char *dataset, void *out, int64_t count, int32_t f64);
+
 #ifdef __cplusplus
 }
 #endif

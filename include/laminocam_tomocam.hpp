@@ -25,8 +25,8 @@
  * they compile against the reference's array.h when it is included first.
  * Defining LAMINOCAM_STANDALONE_ARRAY instead provides a minimal
  * tomocam::dims_t / tomocam::Array<T> with that same interface, and the
- * driver types Dataset_t, Regularizer and ReconParams of
- * include/recon_params.h:32-54 (otherwise include that header first).
+ * driver types Dataset_t, Regularizer, ReconParams and OutputParams of
+ * include/recon_params.h:32-93 (otherwise include that header first).
  *
  * Precision: Array<float> runs the FP32 path (FP64 reductions, PSF and
  * twiddles).  Array<double> runs the FP64 path of the Toeplitz operator,
@@ -54,6 +54,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <ostream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -827,6 +828,39 @@ struct ReconParams {
     float tol = 1e-5f;
     float xtol = 1e-5f;
     float PAD_FACTOR = 1.4142f;
+    // the reference's multi-device streaming fields (include/recon_params.h:56-60);
+    // read from configuration files, this library runs one device
+    size_t numGpus = 0;
+    size_t chunkSlices = 0;
+    bool streamed = false;
+
+    // include/recon_params.h:62-81
+    void print(std::ostream &os) const {
+        const bool sb = regularizer == Regularizer::SPLIT_BREGMAN;
+        os << "Reconstruction Parameters:\n";
+        os << "  max_outer_iters: " << maxIters << "\n";
+        os << "  recon_dims: [" << recon_dims[0] << ", " << recon_dims[1] << ", " << recon_dims[2] << "]\n";
+        os << "  tol: " << tol << "\n";
+        os << "  xtol: " << xtol << "\n";
+        os << "  regularizer: " << (sb ? "Split-Bregman" : "Unconstrained") << "\n";
+        if (sb) {
+            os << "    inner_iters: " << innerIters << "\n";
+            os << "    lambda: " << lambda << "\n";
+            os << "    mu: " << mu << "\n";
+        }
+        if (numGpus > 0) os << "  num_gpus: " << numGpus << "\n";
+        if (chunkSlices > 0) os << "  chunk_slices: " << chunkSlices << "\n";
+        if (streamed) os << "  streamed: true\n";
+    }
+};
+
+// include/recon_params.h:84-93
+struct OutputParams {
+    std::string filepath;
+    std::vector<std::string> formats;
+    bool has_format(const std::string &format) const {
+        return std::find(formats.begin(), formats.end(), format) != formats.end();
+    }
 };
 } // namespace tomocam
 #endif

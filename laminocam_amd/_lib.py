@@ -44,6 +44,7 @@ EXPORTS = [
     "lc_recon_params_default", "lc_prepare_sinogram", "lc_prepare_sinogram_f64", "lc_mbir2d",
     "lc_mbir2d_f64", "lc_mbir3d", "lc_mbir3d_f64", "lc_cg2d_noalloc",
     "lc_reference_memory_model", "lc_volume_fits", "lc_volume_fits_one_gpu", "lc_request_bytes",
+    "lc_tiff_info", "lc_tiff_read", "lc_tiff_write", "lc_h5_shape", "lc_h5_read",
 ]
 
 
@@ -184,6 +185,11 @@ def load():
         "lc_volume_fits": (C.c_int, [C.POINTER(LcReconParams), i32, C.c_uint64, C.POINTER(i32)]),
         "lc_volume_fits_one_gpu": (C.c_int, [vp, C.POINTER(LcReconParams), i32, C.POINTER(i32)]),
         "lc_request_bytes": (C.c_int, [C.POINTER(LcRequest), C.POINTER(C.c_uint64)]),
+        "lc_tiff_info": (C.c_int, [C.c_char_p, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)]),
+        "lc_tiff_read": (C.c_int, [C.c_char_p, vp, i64, i64, i64]),
+        "lc_tiff_write": (C.c_int, [C.c_char_p, vp, i64, i64, i64]),
+        "lc_h5_shape": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(i32), C.POINTER(i64)]),
+        "lc_h5_read": (C.c_int, [C.c_char_p, C.c_char_p, vp, i64, i32]),
         "lc_prepare_sinogram": (C.c_int, [vp, i32, C.POINTER(vp), C.POINTER(i64), i64, i64, vp, vp,
                                           u32]),
         "lc_prepare_sinogram_f64": (C.c_int, [vp, i32, C.POINTER(vp), C.POINTER(i64), i64, i64, vp,

@@ -1,7 +1,8 @@
 """Build recipe for the gfx950 HIP library (liblaminocam.so), in tree.
 
 Every .hip source under csrc/ is compiled by hipcc for gfx950 and linked into
-laminocam_amd/lib/liblaminocam.so.  Packed FP32 instructions are disabled
+laminocam_amd/lib/liblaminocam.so (with zlib, for the file formats' deflate
+streams).  Packed FP32 instructions are disabled
 (`-target-feature -packed-fp32-ops`).
 """
 from __future__ import annotations
@@ -60,7 +61,7 @@ def build(verbose=False, jobs=None):
                 print(err, file=sys.stderr)
     os.makedirs(LIBDIR, exist_ok=True)
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + [o for o, _ in results]
+    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + [o for o, _ in results] + ["-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
@@ -69,7 +70,7 @@ def build(verbose=False, jobs=None):
 
 
 CXX_DIR = os.path.join(HERE, "..", "tests", "cxx")
-CXX_TESTS = ("test_shim", "test_host_api")
+CXX_TESTS = ("test_shim", "test_host_api", "test_toml", "test_config")
 CXX_TEST_BIN = os.path.join(CXX_DIR, "_bin", "test_shim")
 
 
@@ -90,6 +91,31 @@ def build_cxx_test():
     return out
 
 
+CLI_DIR = os.path.join(HERE, "cli")
+BIN_DIR = os.path.join(HERE, "bin")
+CLI_PROGRAMS = ("recon_tomo", "recon_lamino", "forward_scalar")
+
+
+def build_cli():
+    """g++ build of the reconstruction programs (recon_tomo, recon_lamino,
+    forward_scalar) into laminocam_amd/bin/, linked to the in-tree library."""
+    os.makedirs(BIN_DIR, exist_ok=True)
+    cxx = os.environ.get("CXX") or shutil.which("g++") or "g++"
+    out = []
+    for name in CLI_PROGRAMS:
+        dst = os.path.join(BIN_DIR, name)
+        cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-I", os.path.join(HERE, "..", "include"),
+               os.path.join(CLI_DIR, name + ".cpp"), "-o", dst + ".tmp", "-L", LIBDIR, "-llaminocam",
+               "-Wl,-rpath,$ORIGIN/../lib"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"program build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        os.replace(dst + ".tmp", dst)
+        out.append(dst)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
     print(build_cxx_test())
+    print(build_cli())

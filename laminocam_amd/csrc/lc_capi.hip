@@ -8,28 +8,13 @@
 
 using namespace lc;
 
-namespace {
-thread_local std::string g_err;
-
-template <typename F> int guard(F &&f) {
-    try {
-        f();
-        g_err.clear();
-        return LC_OK;
-    } catch (const lc::Error &e) {
-        g_err = e.what();
-        return e.code;
-    } catch (const std::bad_alloc &) {
-        g_err = "host allocation failed";
-        return LC_ENOMEM;
-    } catch (const std::exception &e) {
-        g_err = e.what();
-        return LC_EINTERNAL;
-    } catch (...) {
-        g_err = "unknown error";
-        return LC_EINTERNAL;
-    }
+std::string &lc::last_error() {
+    thread_local std::string msg;
+    return msg;
 }
+
+namespace {
+template <typename F> int guard(F &&f) { return c_guard(std::forward<F>(f)); }
 } // namespace
 
 struct lc_ctx_s {
@@ -83,7 +68,7 @@ extern "C" {
 
 int lc_abi_version(void) { return LC_ABI_VERSION; }
 
-const char *lc_last_error(void) { return g_err.c_str(); }
+const char *lc_last_error(void) { return lc::last_error().c_str(); }
 
 int lc_ctx_create(int device, lc_ctx *out) {
     return guard([&] {

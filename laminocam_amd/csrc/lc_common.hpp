@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <stdexcept>
+#include <new>
 #include <string>
 
 #include "laminocam.h"
@@ -74,6 +75,30 @@ struct Error : public std::runtime_error {
     do {                                                                              \
         if (!(cond)) throw ::lc::Error(LC_EINVAL, std::string(msg));                 \
     } while (0)
+
+// the calling thread's message for lc_last_error
+std::string &last_error();
+
+// runs f at the C boundary: an exception becomes a status code and a message
+template <typename F> int c_guard(F &&f) {
+    try {
+        f();
+        last_error().clear();
+        return LC_OK;
+    } catch (const Error &e) {
+        last_error() = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        last_error() = "host allocation failed";
+        return LC_ENOMEM;
+    } catch (const std::exception &e) {
+        last_error() = e.what();
+        return LC_EINTERNAL;
+    } catch (...) {
+        last_error() = "unknown error";
+        return LC_EINTERNAL;
+    }
+}
 
 inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 

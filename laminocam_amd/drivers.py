@@ -51,6 +51,32 @@ class ReconParams:
     tol: float = 1e-5
     xtol: float = 1e-5
     PAD_FACTOR: float = 1.4142
+    # the reference's multi-device streaming fields (include/recon_params.h:56-60):
+    # read from configuration files; this library runs one device
+    numGpus: int = 0
+    chunkSlices: int = 0
+    streamed: bool = False
+
+    def print(self, file=None) -> None:
+        """ReconParams::print (include/recon_params.h:62-81)"""
+        import sys
+        f = file or sys.stdout
+        sb = self.regularizer == Regularizer.SPLIT_BREGMAN
+        g = lambda x: f"{float(np.float32(x)):g}"
+        lines = ["Reconstruction Parameters:", f"  max_outer_iters: {self.maxIters}",
+                 "  recon_dims: [{}, {}, {}]".format(*self.recon_dims), f"  tol: {g(self.tol)}",
+                 f"  xtol: {g(self.xtol)}",
+                 f"  regularizer: {'Split-Bregman' if sb else 'Unconstrained'}"]
+        if sb:
+            lines += [f"    inner_iters: {self.innerIters}", f"    lambda: {g(self.lam)}",
+                      f"    mu: {g(self.mu)}"]
+        if self.numGpus > 0:
+            lines.append(f"  num_gpus: {self.numGpus}")
+        if self.chunkSlices > 0:
+            lines.append(f"  chunk_slices: {self.chunkSlices}")
+        if self.streamed:
+            lines.append("  streamed: true")
+        print("\n".join(lines), file=f)
 
     def to_c(self) -> LcReconParams:
         p = LcReconParams()
