@@ -190,8 +190,8 @@ k_cg_update(const ChunkRef *__restrict__ ch, const CgState *__restrict__ st,
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int i = threadIdx.x + u * NT;
-            pv[u] = p4[i];
-            av[u] = a4[i];
+            pv[u] = ld_stream(p4 + i);
+            av[u] = ld_stream(a4 + i);
             xv[u] = x4[i];
             rv[u] = r4[i];
         }
@@ -199,8 +199,8 @@ k_cg_update(const ChunkRef *__restrict__ ch, const CgState *__restrict__ st,
         for (int u = 0; u < U; ++u) {
             upd4(pv[u], av[u], xv[u], rv[u]);
             const int i = threadIdx.x + u * NT;
-            x4[i] = xv[u];
-            r4[i] = rv[u];
+            st_stream(x4 + i, xv[u]);
+            st_stream(r4 + i, rv[u]);
         }
     } else {
         for (int64_t i = threadIdx.x; i < n4; i += NT) {
@@ -257,10 +257,17 @@ struct Idx3 {
     int s, a, b, n1, n2, n3;
 };
 
-template <typename R>
-__device__ __forceinline__ R at3(const R *u, const Idx3 &q, int s, int a, int b) {
-    return u[((int64_t)s * q.n2 + a) * q.n3 + b];
-}
+// Neighbours are read relative to the current element's own address: rel(u0,
+// ss, rs)(ds, da, db) is u0[ds*ss + da*rs + db] (ss = n2*n3, rs = n3), so the
+// offsets fold to constants and no element index is multiplied out per read.
+template <typename R> struct Rel {
+    const R *u0;
+    int64_t ss;
+    int rs;
+    __device__ __forceinline__ R operator()(int ds, int da, int db) const {
+        return u0[ds * ss + (int64_t)(da * rs + db)];
+    }
+};
 
 __device__ __forceinline__ bool interior(const Idx3 &q, int s, int a, int b) {
     return s >= 1 && s + 1 < q.n1 && a >= 1 && a + 1 < q.n2 && b >= 1 && b + 1 < q.n3;
@@ -270,72 +277,70 @@ __device__ __forceinline__ bool inside(const Idx3 &q, int s, int a, int b) {
     return s >= 0 && s < q.n1 && a >= 0 && a < q.n2 && b >= 0 && b < q.n3;
 }
 
-// gradient component c (0 slices, 1 rows, 2 cols) at (s,a,b)
-template <typename R>
-__device__ __forceinline__ R grad_c(const R *u, const Idx3 &q, int c, int s, int a, int b,
+// gradient component c (0 slices, 1 rows, 2 cols) at the element offset
+// (os, oa, ob) from q's; u reads relative to q's element
+template <typename R, typename A>
+__device__ __forceinline__ R grad_c(const A &u, const Idx3 &q, int c, int os, int oa, int ob,
                                     int boundary) {
     const int ds = c == 0, da = c == 1, db = c == 2;
+    const int s = q.s + os, a = q.a + oa, b = q.b + ob;
     if (boundary == 0) {
         if (!interior(q, s, a, b)) return (R)0;
-        return (at3(u, q, s + ds, a + da, b + db) - at3(u, q, s - ds, a - da, b - db)) / (R)2;
+        return (u(os + ds, oa + da, ob + db) - u(os - ds, oa - da, ob - db)) / (R)2;
     }
-    const R up = inside(q, s + ds, a + da, b + db) ? at3(u, q, s + ds, a + da, b + db) : (R)0;
-    const R um = inside(q, s - ds, a - da, b - db) ? at3(u, q, s - ds, a - da, b - db) : (R)0;
+    const R up = inside(q, s + ds, a + da, b + db) ? u(os + ds, oa + da, ob + db) : (R)0;
+    const R um = inside(q, s - ds, a - da, b - db) ? u(os - ds, oa - da, ob - db) : (R)0;
     return (R)0.5 * (up - um);
 }
 
-// divergence of a 3-component field at (s,a,b); G(c, s, a, b) reads
-// component c at an in-range point
+// divergence of a 3-component field at q's element; g(c, ds, da, db) reads
+// component c at an in-range offset from it
 template <typename R, typename G>
-__device__ __forceinline__ R div_gen(const G &g, const Idx3 &q, int s, int a, int b, int boundary) {
+__device__ __forceinline__ R div_gen(const G &g, const Idx3 &q, int boundary) {
+    const int s = q.s, a = q.a, b = q.b;
     if (boundary == 0) {
         if (!interior(q, s, a, b)) return (R)0;
-        return (g(0, s + 1, a, b) - g(0, s - 1, a, b)) / (R)2 +
-               (g(1, s, a + 1, b) - g(1, s, a - 1, b)) / (R)2 +
-               (g(2, s, a, b + 1) - g(2, s, a, b - 1)) / (R)2;
+        return (g(0, 1, 0, 0) - g(0, -1, 0, 0)) / (R)2 + (g(1, 0, 1, 0) - g(1, 0, -1, 0)) / (R)2 +
+               (g(2, 0, 0, 1) - g(2, 0, 0, -1)) / (R)2;
     }
-    const R g0p = inside(q, s + 1, a, b) ? g(0, s + 1, a, b) : (R)0;
-    const R g0m = inside(q, s - 1, a, b) ? g(0, s - 1, a, b) : (R)0;
-    const R g1p = inside(q, s, a + 1, b) ? g(1, s, a + 1, b) : (R)0;
-    const R g1m = inside(q, s, a - 1, b) ? g(1, s, a - 1, b) : (R)0;
-    const R g2p = inside(q, s, a, b + 1) ? g(2, s, a, b + 1) : (R)0;
-    const R g2m = inside(q, s, a, b - 1) ? g(2, s, a, b - 1) : (R)0;
+    const R g0p = inside(q, s + 1, a, b) ? g(0, 1, 0, 0) : (R)0;
+    const R g0m = inside(q, s - 1, a, b) ? g(0, -1, 0, 0) : (R)0;
+    const R g1p = inside(q, s, a + 1, b) ? g(1, 0, 1, 0) : (R)0;
+    const R g1m = inside(q, s, a - 1, b) ? g(1, 0, -1, 0) : (R)0;
+    const R g2p = inside(q, s, a, b + 1) ? g(2, 0, 0, 1) : (R)0;
+    const R g2m = inside(q, s, a, b - 1) ? g(2, 0, 0, -1) : (R)0;
     const R h = (R)0.5;
     return h * (g0p - g0m) + h * (g1p - g1m) + h * (g2p - g2m);
 }
 
-// divergence of a 3-component field g (component c at g + c*cstride) at (s,a,b)
-template <typename R>
-__device__ __forceinline__ R div_at(const R *g, int64_t cstride, const Idx3 &q, int s, int a, int b,
-                                    int boundary) {
-    auto get = [&](int c, int ss, int aa, int bb) { return at3(g + c * cstride, q, ss, aa, bb); };
-    return div_gen<R>(get, q, s, a, b, boundary);
-}
-
-// laplacian = div(grad u) evaluated directly from u
-template <typename R>
-__device__ __forceinline__ R lap_at(const R *u, const Idx3 &q, int s, int a, int b, int boundary) {
+// laplacian = div(grad u) at q's element, evaluated directly from u
+template <typename R, typename A>
+__device__ __forceinline__ R lap_at(const A &u, const Idx3 &q, int boundary) {
     R acc = (R)0;
     if (boundary == 0) {
-        if (!interior(q, s, a, b)) return (R)0;
+        if (!interior(q, q.s, q.a, q.b)) return (R)0;
         // component c: (G(i+e) - G(i-e))/2 with G zero off the interior
+#pragma unroll
         for (int c = 0; c < 3; ++c) {
             const int ds = c == 0, da = c == 1, db = c == 2;
-            const R gp = grad_c(u, q, c, s + ds, a + da, b + db, 0);
-            const R gm = grad_c(u, q, c, s - ds, a - da, b - db, 0);
+            const R gp = grad_c<R>(u, q, c, ds, da, db, 0);
+            const R gm = grad_c<R>(u, q, c, -ds, -da, -db, 0);
             acc += (gp - gm) / (R)2;
         }
         return acc;
     }
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
         const int ds = c == 0, da = c == 1, db = c == 2;
-        const R gp = inside(q, s + ds, a + da, b + db) ? grad_c(u, q, c, s + ds, a + da, b + db, 1) : (R)0;
-        const R gm = inside(q, s - ds, a - da, b - db) ? grad_c(u, q, c, s - ds, a - da, b - db, 1) : (R)0;
+        const R gp = inside(q, q.s + ds, q.a + da, q.b + db) ? grad_c<R>(u, q, c, ds, da, db, 1) : (R)0;
+        const R gm = inside(q, q.s - ds, q.a - da, q.b - db) ? grad_c<R>(u, q, c, -ds, -da, -db, 1) : (R)0;
         acc += (R)0.5 * (gp - gm);
     }
     return acc;
 }
 
+// (s, a, b) of the element at global offset i of a problem (one 64-bit
+// division pair per thread); step() advances it by NT elements
 __device__ __forceinline__ Idx3 idx_of(const ProbDev &d, int64_t i) {
     const int64_t loc = i - d.off;
     Idx3 q;
@@ -349,6 +354,16 @@ __device__ __forceinline__ Idx3 idx_of(const ProbDev &d, int64_t i) {
     q.b = (int)(rem - (int64_t)q.a * d.N);
     return q;
 }
+__device__ __forceinline__ void step(Idx3 &q, int n) {
+    q.b += n;
+    while (q.b >= q.n3) {
+        q.b -= q.n3;
+        if (++q.a >= q.n2) {
+            q.a = 0;
+            ++q.s;
+        }
+    }
+}
 
 // Ap -= mu * lap(p); partial <p, Ap> (Split-Bregman operator, src/bregman.cpp:55)
 template <typename R, int NT>
@@ -359,11 +374,11 @@ k_sb_lapdot(const ChunkRef *__restrict__ ch, const ProbDev *__restrict__ pd,
     const ChunkRef c = ch[blockIdx.x];
     if (st && !st[c.prob].active) return;
     const ProbDev d = pd[c.prob];
-    const R *pp = p + d.off;
+    const int64_t nn = (int64_t)d.N * d.N;
     double acc = 0.0;
-    for (int64_t i = c.begin + threadIdx.x; i < c.end; i += NT) {
-        const Idx3 q = idx_of(d, i);
-        const R l = lap_at(pp, q, q.s, q.a, q.b, d.boundary);
+    Idx3 q = idx_of(d, c.begin + threadIdx.x);
+    for (int64_t i = c.begin + threadIdx.x; i < c.end; i += NT, step(q, NT)) {
+        const R l = lap_at<R>(Rel<R>{p + i, nn, d.N}, q, d.boundary);
         const R v = ap[i] - l * (R)d.mu;
         ap[i] = v;
         acc += (double)p[i] * v;
@@ -386,13 +401,14 @@ k_sb_rhs(const ChunkRef *__restrict__ ch, const ProbDev *__restrict__ pd,
     const ChunkRef c = ch[blockIdx.x];
     if (!sb[c.prob].active) return;
     const ProbDev d = pd[c.prob];
-    const R *dp = D + d.off, *bp = Bv + d.off;
-    for (int64_t i = c.begin + threadIdx.x; i < c.end; i += NT) {
-        const Idx3 q = idx_of(d, i);
-        auto dmb = [&](int k, int ss, int aa, int bb) {
-            return at3(dp + k * total, q, ss, aa, bb) - at3(bp + k * total, q, ss, aa, bb);
+    const int64_t nn = (int64_t)d.N * d.N;
+    Idx3 q = idx_of(d, c.begin + threadIdx.x);
+    for (int64_t i = c.begin + threadIdx.x; i < c.end; i += NT, step(q, NT)) {
+        auto dmb = [&](int k, int ds, int da, int db) {
+            const int64_t o = k * total + i + ds * nn + (int64_t)(da * d.N + db);
+            return D[o] - Bv[o];
         };
-        const R dv = div_gen<R>(dmb, q, q.s, q.a, q.b, d.boundary);
+        const R dv = div_gen<R>(dmb, q, d.boundary);
         rhs[i] = y0[i] - dv * (R)d.mu;
     }
 }
@@ -409,14 +425,16 @@ k_sb_shrink(const ChunkRef *__restrict__ ch, const ProbDev *__restrict__ pd,
     const SbState S = sb[c.prob];
     if (!S.active) return;
     const ProbDev d = pd[c.prob];
-    const R *xp = x + d.off;
+    const int64_t nn = (int64_t)d.N * d.N;
     const R thr = (R)(S.lambda / S.mu);
     double dn = 0.0, on = 0.0;
-    for (int64_t i = c.begin + threadIdx.x; i < c.end; i += NT) {
-        const Idx3 q = idx_of(d, i);
+    Idx3 q = idx_of(d, c.begin + threadIdx.x);
+    for (int64_t i = c.begin + threadIdx.x; i < c.end; i += NT, step(q, NT)) {
+        const Rel<R> xr{x + i, nn, d.N};
         R g[3], v[3];
+#pragma unroll
         for (int k = 0; k < 3; ++k) {
-            g[k] = grad_c(xp, q, k, q.s, q.a, q.b, d.boundary);
+            g[k] = grad_c<R>(xr, q, k, 0, 0, 0, d.boundary);
             v[k] = g[k] + Bv[k * total + i];
         }
         if (!aniso) {
@@ -487,7 +505,8 @@ __global__ void k_fd_grad(const R *u, R *g, int n1, int n2, int n3, int boundary
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         const Idx3 q = idx_vol(i, n1, n2, n3);
-        for (int c = 0; c < 3; ++c) g[c * n + i] = grad_c(u, q, c, q.s, q.a, q.b, boundary);
+        const Rel<R> ur{u + i, (int64_t)n2 * n3, n3};
+        for (int c = 0; c < 3; ++c) g[c * n + i] = grad_c<R>(ur, q, c, 0, 0, 0, boundary);
     }
 }
 
@@ -497,7 +516,10 @@ __global__ void k_fd_div(const R *g, R *out, int n1, int n2, int n3, int boundar
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         const Idx3 q = idx_vol(i, n1, n2, n3);
-        out[i] = div_at(g, n, q, q.s, q.a, q.b, boundary);
+        auto gc = [&](int c, int ds, int da, int db) {
+            return g[c * n + i + ds * ((int64_t)n2 * n3) + (int64_t)(da * n3 + db)];
+        };
+        out[i] = div_gen<R>(gc, q, boundary);
     }
 }
 
@@ -507,7 +529,7 @@ __global__ void k_fd_lap(const R *u, R *out, int n1, int n2, int n3, int boundar
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         const Idx3 q = idx_vol(i, n1, n2, n3);
-        out[i] = lap_at(u, q, q.s, q.a, q.b, boundary);
+        out[i] = lap_at<R>(Rel<R>{u + i, (int64_t)n2 * n3, n3}, q, boundary);
     }
 }
 
