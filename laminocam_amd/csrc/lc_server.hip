@@ -50,8 +50,12 @@ size_t request_bytes(const ServerRequest &r) {
             int64_t nf[3];
             for (int a = 0; a < 3; ++a)
                 nf[a] = u.ready ? u.nf[a] : nufft3d_fine_n(a == 0 ? g.n1 : a == 1 ? g.n2 : g.n3);
-            bp = 8 * (g.nproj * g.nrows * g.ncols) * 2 +
-                 8 * (nf[0] * nf[1] * nf[2] + nf[0] * nf[1] * g.n3 + nf[0] * g.n2 * g.n3);
+            // detector spectra (and the staged projections), the fine grid,
+            // the line passes' intermediates and the spread's own buffers
+            const int nfi[3] = {(int)nf[0], (int)nf[1], (int)nf[2]};
+            const int64_t npts = g.nproj * g.nrows * g.ncols;
+            bp = 8 * npts * 2 + 8 * (nf[0] * nf[1] * nf[2] + nf[0] * nf[1] * g.n3 + nf[0] * g.n2 * g.n3) +
+                 (int64_t)nufft3d_spread_bytes(npts, nfi);
         }
     } else {
         const Geom2D &g = *r.g2;
