@@ -19,8 +19,9 @@
 //       x khat3[k2][k3][:], inverse, keep n1 -> W
 //   P2i every slice, column block: inverse FFT_M down the columns, keep N -> U
 //   P3  rows of every slice: the 2-D inverse row pass -> output
-// W is [nblk][M][n1][B] complex so that P2b reads one contiguous n1 x B tile
-// per k2.
+// W is [nblk][n1][M][B] complex: the column passes read and write one
+// contiguous M x B block per (column block, slice), and P2b reads, per slice,
+// kb2 x B consecutive values (its lines) as one contiguous piece.
 #include <cmath>
 
 #include <type_traits>
@@ -364,15 +365,19 @@ template void geom3d_kernel_hat<double>(Ctx &, Geom3D &, double *);
 // ---------------------------------------------------------------------------
 // FP32: column FFTs B * M <= 8192, slice-axis FFTs kb2 * B * M1 <= 4096.
 // FP64 (16-byte complex) keeps every block within 64 KiB of LDS: 4096 / 4096.
-constexpr int T3_NT = 256, T3_ME = 32;
-constexpr int T3B_NT = 256, T3B_ME = 16;
-template <typename R> constexpr int t3_me() { return sizeof(R) == 4 ? T3_ME : T3_ME / 2; }
+// Column blocks: FP32 512 threads x 16 elements (M = 1536, B = 4: 12 per
+// thread, 6 waves per SIMD within the 48 KiB of LDS per block), FP64 256 x 16.
+// Slice-axis blocks (4096 elements): FP32 512 threads x 8, FP64 256 x 16.
+template <typename R> constexpr int t3_nt() { return sizeof(R) == 4 ? 512 : 256; }
+template <typename R> constexpr int t3_me() { return 16; }
+template <typename R> constexpr int t3b_nt() { return sizeof(R) == 4 ? 512 : 256; }
+template <typename R> constexpr int t3b_me() { return sizeof(R) == 4 ? 8 : 16; }
 
 __device__ __forceinline__ bool prob_active3(const CgState *st, int prob) {
     return st == nullptr || st[prob].active != 0;
 }
 
-// P2f: U block (N x B, slice i1) -> FFT_M down columns -> W[kb][k2][i1][c].
+// P2f: U block (N x B, slice i1) -> FFT_M down columns -> W[kb][i1][k2][c].
 // LF > 0: M = LF at compile time (unrolled stages, B = 1 << g.lsh lines), with
 // the zero rows N.. of the input pruned from the first stage.
 template <typename R, int NT, int MAXE, int LF>
@@ -398,10 +403,10 @@ k_t3_cols_fwd(const SliceRef *__restrict__ sl, const CgState *__restrict__ st,
     __syncthreads();
     if constexpr (LF > 0) fft_lds_ct<R, LF, -1, NT, MAXE, true, -1, true, false>(buf, B, g.lsh, tw);
     else fft_lds<R, -1, NT, MAXE, true>(buf, B, 0, plan, tw);
-    C *w = W + (int64_t)kb * M * n1 * B;
+    C *w = W + ((int64_t)kb * n1 + i1) * M * B;
     for (int idx = threadIdx.x; idx < M * B; idx += NT) {
         const int k2 = idx / B, c = idx - k2 * B;
-        w[((int64_t)k2 * n1 + i1) * B + c] = buf[lds_col(c, k2, B)];
+        w[idx] = buf[lds_col(c, k2, B)];
     }
 }
 
@@ -420,12 +425,13 @@ k_t3_slices(const SliceRef *__restrict__ sl, const CgState *__restrict__ st, cpx
     const int nk2 = min(kb2, M - k20);
     const int nl = kb2 * B;
     C *buf = reinterpret_cast<C *>(lc_smem);
-    C *w = W + ((int64_t)kb * M + k20) * n1 * B;   // tile [j][i1][c], contiguous
+    // slice e's values of these lines: w[e M B + line] (j * B + c = line)
+    C *w = W + ((int64_t)kb * n1 * M + k20) * B;
     for (int idx = threadIdx.x; idx < nl * M1; idx += NT) {
         const int e = idx / nl, line = idx - e * nl;     // e: slice index, line = j*B + c
-        const int j = line / B, c = line - j * B;
+        const int j = line / B;
         C v = cmake<C>(0, 0);
-        if (j < nk2 && e < n1) v = w[((int64_t)j * n1 + e) * B + c];
+        if (j < nk2 && e < n1) v = w[(int64_t)e * M * B + line];
         buf[lds_col(line, e, nl)] = v;
     }
     __syncthreads();
@@ -442,8 +448,8 @@ k_t3_slices(const SliceRef *__restrict__ sl, const CgState *__restrict__ st, cpx
     fft_lds<R, 1, NT, MAXE, true>(buf, nl, 0, plan1, tw1);
     for (int idx = threadIdx.x; idx < nl * n1; idx += NT) {
         const int e = idx / nl, line = idx - e * nl;
-        const int j = line / B, c = line - j * B;
-        if (j < nk2) w[((int64_t)j * n1 + e) * B + c] = buf[lds_col(line, e, nl)];
+        const int j = line / B;
+        if (j < nk2) w[(int64_t)e * M * B + line] = buf[lds_col(line, e, nl)];
     }
 }
 
@@ -468,7 +474,9 @@ k_t3_slices_ct(const SliceRef *__restrict__ sl, const CgState *__restrict__ st, 
     const int kb = blockIdx.x, k20 = blockIdx.y * kb2;
     const int nk2 = min(kb2, M - k20);
     C *buf = reinterpret_cast<C *>(lc_smem);
-    C *w = W + ((int64_t)kb * M + k20) * n1 * B;   // tile [j][i1][c], contiguous
+    // slice e's values of these lines: w[e M B + line] (lines j << bsh | c
+    // are consecutive values)
+    C *w = W + ((int64_t)kb * n1 * M + k20) * B;
     // spectrum values of this thread's middle-stage butterflies (line, index)
     R kr[CM::MAXB][CM::R];
 #pragma unroll
@@ -484,21 +492,21 @@ k_t3_slices_ct(const SliceRef *__restrict__ sl, const CgState *__restrict__ st, 
     }
     for (int idx = threadIdx.x; idx < NL * HALF; idx += NT) {
         const int e = idx >> LSH, line = idx & (NL - 1);
-        const int j = line >> bsh, c = line & (B - 1);
+        const int j = line >> bsh;
         C v = cmake<C>(0, 0);
-        if (j < nk2 && e < n1) v = w[((int64_t)j * n1 + e) * B + c];
+        if (j < nk2 && e < n1) v = w[(int64_t)e * M * B + line];
         buf[swz((e << LSH) + line)] = v;
     }
     __syncthreads();
     fft_conv_ct<R, L1, NT, MAXE, LSH>(buf, NL, LSH, tw1, kr);
     for (int idx = threadIdx.x; idx < NL * n1; idx += NT) {
         const int e = idx >> LSH, line = idx & (NL - 1);
-        const int j = line >> bsh, c = line & (B - 1);
-        if (j < nk2) w[((int64_t)j * n1 + e) * B + c] = buf[swz((e << LSH) + line)];
+        const int j = line >> bsh;
+        if (j < nk2) w[(int64_t)e * M * B + line] = buf[swz((e << LSH) + line)];
     }
 }
 
-// P2i: W[kb][:][i1][:] -> inverse FFT_M down columns -> keep N rows -> U block
+// P2i: W[kb][i1][:][:] -> inverse FFT_M down columns -> keep N rows -> U block
 // (LF > 0: compiled length; the unused outputs N.. pruned from the last stage)
 template <typename R, int NT, int MAXE, int LF>
 __global__ void __launch_bounds__(NT)
@@ -513,10 +521,10 @@ k_t3_cols_inv(const SliceRef *__restrict__ sl, const CgState *__restrict__ st,
     if (!prob_active3(st, s.prob)) return;
     const int N = g.N, M = g.M, B = g.B;
     C *buf = reinterpret_cast<C *>(lc_smem);
-    const C *w = W + (int64_t)kb * M * n1 * B;
+    const C *w = W + ((int64_t)kb * n1 + i1) * M * B;
     for (int idx = threadIdx.x; idx < M * B; idx += NT) {
         const int k2 = idx / B, c = idx - k2 * B;
-        buf[lds_col(c, k2, B)] = w[((int64_t)k2 * n1 + i1) * B + c];
+        buf[lds_col(c, k2, B)] = w[idx];
     }
     __syncthreads();
     if constexpr (LF > 0) fft_lds_ct<R, LF, 1, NT, MAXE, true, -1, false, true>(buf, B, g.lsh, tw);
@@ -537,7 +545,7 @@ template <> const double *t3_khat<double>(const Geom3D &g) { return g.khat3_64.a
 
 template <typename R> void toep3d_mid(Ctx &c, const ToepClass &k, const CgState *st, cpx<R> *U) {
     using C = cpx<R>;
-    constexpr int ME = t3_me<R>();
+    constexpr int NT = t3_nt<R>(), ME = t3_me<R>(), NTB = t3b_nt<R>(), MEB = t3b_me<R>();
     const bool f64 = sizeof(R) == 8;
     LC_REQUIRE(k.B * k.M <= toep3d_col_cap(f64) && k.kb2 * k.B * k.M1 <= toep3d_slice_cap(f64),
                "toeplitz 3-D: block exceeds the LDS shape");
@@ -551,10 +559,10 @@ template <typename R> void toep3d_mid(Ctx &c, const ToepClass &k, const CgState 
         constexpr bool FWD = decltype(dir)::value;
 #define LC_T3_COLS(L)                                                                             \
     if constexpr (FWD)                                                                            \
-        hipLaunchKernelGGL((k_t3_cols_fwd<R, T3_NT, ME, L>), dim3(k.nblk, k.n1), dim3(T3_NT), shc, \
+        hipLaunchKernelGGL((k_t3_cols_fwd<R, NT, ME, L>), dim3(k.nblk, k.n1), dim3(NT), shc,       \
                            c.stream(), sl, st, U, W, k.g, k.n1, k.plan, tw);                      \
     else                                                                                          \
-        hipLaunchKernelGGL((k_t3_cols_inv<R, T3_NT, ME, L>), dim3(k.nblk, k.n1), dim3(T3_NT), shc, \
+        hipLaunchKernelGGL((k_t3_cols_inv<R, NT, ME, L>), dim3(k.nblk, k.n1), dim3(NT), shc,       \
                            c.stream(), sl, st, W, U, k.g, k.n1, k.plan, tw)
         switch (lf) {
         case 256: LC_T3_COLS(256); break;
@@ -574,17 +582,17 @@ template <typename R> void toep3d_mid(Ctx &c, const ToepClass &k, const CgState 
     const dim3 gb(k.nblk, ceil_div(k.M, k.kb2));
     // FP32 slice-axis lengths 16..128 with a full block of lines: compiled,
     // fused convolution; anything else: runtime plan
-    const bool ct1 = !f64 && k.kb2 * k.B * k.M1 == T3B_NT * T3B_ME &&
+    const bool ct1 = !f64 && k.kb2 * k.B * k.M1 == NTB * MEB &&
                      (k.M1 == 16 || k.M1 == 32 || k.M1 == 64 || k.M1 == 128);
 #define LC_T3_SL(L1)                                                                              \
-    hipLaunchKernelGGL((k_t3_slices_ct<R, T3B_NT, T3B_ME, L1>), gb, dim3(T3B_NT), shb, c.stream(),  \
+    hipLaunchKernelGGL((k_t3_slices_ct<R, NTB, MEB, L1>), gb, dim3(NTB), shb, c.stream(),          \
                        sl, st, W, t3_khat<R>(*k.g3), k.g, k.n1, k.kb2, t3_tw1<R>(k))
     if (ct1 && k.M1 == 16) LC_T3_SL(16);
     else if (ct1 && k.M1 == 32) LC_T3_SL(32);
     else if (ct1 && k.M1 == 64) LC_T3_SL(64);
     else if (ct1 && k.M1 == 128) LC_T3_SL(128);
     else
-        hipLaunchKernelGGL((k_t3_slices<R, T3B_NT, T3B_ME>), gb, dim3(T3B_NT), shb, c.stream(), sl, st, W,
+        hipLaunchKernelGGL((k_t3_slices<R, NTB, MEB>), gb, dim3(NTB), shb, c.stream(), sl, st, W,
                            t3_khat<R>(*k.g3), k.g, k.n1, k.M1, k.kb2, k.plan1, t3_tw1<R>(k));
 #undef LC_T3_SL
     LC_HIP(hipGetLastError());
@@ -594,10 +602,10 @@ template void toep3d_mid<float>(Ctx &, const ToepClass &, const CgState *, float
 template void toep3d_mid<double>(Ctx &, const ToepClass &, const CgState *, double2 *);
 
 // shape limits of the 3-D middle pass for the class set-up
-int toep3d_col_cap(bool f64) { return T3_NT * (f64 ? T3_ME / 2 : T3_ME); }
+int toep3d_col_cap(bool f64) { return f64 ? t3_nt<double>() * t3_me<double>() : t3_nt<float>() * t3_me<float>(); }
 int toep3d_slice_cap(bool f64) {
-    (void)f64;   // 4096 complex: 32 KiB (FP32) or 64 KiB (FP64) of LDS
-    return T3B_NT * T3B_ME;
+    // 4096 complex: 32 KiB (FP32) or 64 KiB (FP64) of LDS
+    return f64 ? t3b_nt<double>() * t3b_me<double>() : t3b_nt<float>() * t3b_me<float>();
 }
 
 } // namespace lc
