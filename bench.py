@@ -492,11 +492,45 @@ def lamino_leg(args, lc, synth, ctx):
                       f"{list(psf.dims())}), times {args.lamino_iters + 1} applies of a "
                       f"{args.lamino_iters}-iteration CG",
             "speedup_gpu_vs_cpu": round(t_cpu / t_cg, 1)}
+    out["mbir_driver"] = lamino_driver(args, lc, synth, ctx)
     out["workload"] = ("3-D laminography at the padded configuration shapes; CG 100 iterations "
                        "(1% Gaussian noise) and Split-Bregman TV 100 outer x 1 inner (Poisson "
                        "counts, lambda 0.1, mu 10, isotropic, interior-only differences); "
                        "synthetic phantoms, inputs resident")
     return out
+
+
+def lamino_driver(args, lc, synth, ctx):
+    """The whole laminography reconstruction through the MBIR driver
+    (lc_mbir3d, src/mbir.cpp:34-161), as recon_lamino runs it: projections
+    360 x 64 x 511 over [0, 360) at 30 degree tilt, recon_dims 21 x 511^2,
+    PAD_FACTOR 1.4142 (padded to 29 x 721^2 and 360 x 89 x 721), CG with
+    args.lamino_iters iterations (tol, xtol 0).  Host arrays in and out; the
+    driver builds the FP64 PSF, back-projects, solves and crops inside the
+    timed call.  The second of two calls is kept."""
+    from laminocam_amd import drivers
+    nproj, nrows, ncols, rdims = 360, 64, 511, (21, 511, 511)
+    th = np.arange(nproj) * (2 * np.pi / nproj)
+    grid = lc.PolarGrid(th, np.full(nproj, -np.pi / 6), np.zeros(nproj), nrows, ncols)
+    ph = synth.ellipsoid_phantom(rdims, 12, synth.Stream((2 << 16) + 21))
+    p = lc.forward(ph, grid, ctx=ctx)
+    p = synth.add_gaussian_noise(p, 0.01, synth.Stream((2 << 16) + 22))
+    ds = [drivers.Dataset(projs=p, angles=th, gamma=-np.pi / 6)]
+    par = drivers.ReconParams(recon_dims=rdims, maxIters=args.lamino_iters, tol=0.0, xtol=0.0)
+    ts, rep = [], {}
+    for _ in range(2):
+        rep = {}
+        t0 = time.perf_counter()
+        x = drivers.MBIR(ds, par, ctx=ctx, graph=True, report=rep)
+        ts.append(time.perf_counter() - t0)
+    c = float(np.sum(x * ph) / (np.linalg.norm(x) * np.linalg.norm(ph)))
+    return {"projections": [nproj, nrows, ncols], "recon_dims": list(rdims),
+            "padded_dims": [int(v) for v in rep.get("padded_dims", [])],
+            "iters": int(rep.get("iters", 0)), "seconds": round(ts[-1], 3),
+            "slices_per_s": round(rdims[0] / ts[-1], 2),
+            "cosine_to_phantom": round(c, 3),
+            "note": "end to end through lc_mbir3d: PSF build, back-projection, CG, crop; host "
+                    "arrays in and out"}
 
 
 def lamino_accuracy(args, lc, synth, ctx):

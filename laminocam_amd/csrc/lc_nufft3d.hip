@@ -653,27 +653,26 @@ void det_line_init(Ctx &c, DetLine &d, int n) {
         a[k] = std::complex<double>(std::cos(ang), std::sin(ang));
         ch[k] = make_float2((float)a[k].real(), (float)a[k].imag());
     }
-    // bhat for sign s: b_k = conj(a_k^s) = e^{-s i pi k^2/n} on k and L-k;
-    // bhat = FFT-(b) / L in FP64 (direct DFT), [sign -1 | sign +1]
+    // bhat for sign s: b_k = conj(a_k^s) = e^{-s i pi k^2/n} on k and L-k
+    // (L >= 2n - 1, so the two runs do not meet); bhat = FFT-(b) / L in FP64,
+    // [sign -1 | sign +1].  b_k = b_{L-k} pairs e^{-2 pi i k m / L} with its
+    // conjugate: bhat_m = (b_0 + sum_{0<k<n} b_k 2 cos(2 pi k m / L)) / L, a
+    // real combination of the b_k, so the sign +1 table is the conjugate of
+    // the sign -1 one.  cos(2 pi q / L) comes from a table over q = k m mod L.
+    std::vector<double> cq(L);
+    for (int64_t q = 0; q < L; ++q) cq[q] = std::cos(2.0 * M_PI * (double)q / (double)L);
     std::vector<float2> bh(2 * L);
-    for (int si = 0; si < 2; ++si) {
-        const double s = si == 0 ? -1.0 : 1.0;
-        std::vector<std::complex<double>> b(L, 0.0);
-        for (int k = 0; k < n; ++k) {
-            const std::complex<double> v = s < 0 ? a[k] : std::conj(a[k]);
-            b[k] = v;
-            if (k > 0) b[L - k] = v;
+    for (int64_t m = 0; m < L; ++m) {
+        std::complex<double> acc = a[0];
+        int64_t q = 0;
+        for (int k = 1; k < n; ++k) {
+            q += m;
+            if (q >= L) q -= L;
+            acc += a[k] * (2.0 * cq[q]);
         }
-        for (int64_t m = 0; m < L; ++m) {
-            std::complex<double> acc = 0.0;
-            for (int64_t k = 0; k < L; ++k) {
-                const int64_t q = (k * m) % L;
-                const double ang = -2.0 * M_PI * (double)q / (double)L;
-                acc += b[k] * std::complex<double>(std::cos(ang), std::sin(ang));
-            }
-            acc /= (double)L;
-            bh[si * L + m] = make_float2((float)acc.real(), (float)acc.imag());
-        }
+        acc /= (double)L;
+        bh[m] = make_float2((float)acc.real(), (float)acc.imag());
+        bh[L + m] = make_float2((float)acc.real(), (float)-acc.imag());
     }
     d.chirp.alloc(sizeof(float2) * n);
     d.bhat.alloc(sizeof(float2) * 2 * L);
