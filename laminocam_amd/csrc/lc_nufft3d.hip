@@ -12,7 +12,7 @@
 // Method as in 2-D: exponential-of-semicircle kernel, width 8, fine grid 2n
 // per axis (2^p 3^q, at least 16), centred so points map to cells without a
 // shift; the (-1)^m phase and 1/psi_hat sit in per-axis correction factors.
-//   type 1: deterministic tile-gather spreading -> fine grid, then one pruned
+//   type 1: deterministic sorted-bin spreading -> fine grid, then one pruned
 //           FFT pass per axis (x, y, z), deconvolution in each pass;
 //   type 2: the reverse passes, then 8x8x8 interpolation per point.
 // Detector transforms are line FFTs along x then y with the reference's shift
@@ -173,38 +173,49 @@ __global__ void k_phase_shift3d(float2 *__restrict__ C, const double *__restrict
 
 // ---------------------------------------------------------------------------
 // type-1 spreading by footprint-corner bins.  The fine grid is cut into
-// BB^3 bins of footprint corners (a point's corner is its first footprint
-// cell, ceil(g - W3/2) per axis, taken modulo nf).  k_bin_keys gives every
-// point its bin; a stable radix sort (rocPRIM) orders the points by bin,
-// keeping point order inside a bin; k_bin_ranges marks each bin's run.  A
-// workgroup then owns one bin: its waves (SB3_WAVES) take consecutive parts
-// of the bin's points in order and spread each one's whole 8 x 8 x 8
+// BZ x BY x BX bins of footprint corners (a point's corner is its first
+// footprint cell, ceil(g - W3/2) per axis, taken modulo nf).  k_bin_keys
+// gives every point its bin; a stable radix sort (rocPRIM) orders the points
+// by bin, keeping point order inside a bin; k_bin_ranges marks each bin's
+// run.  A workgroup then owns one bin: its waves (SB3_WAVES) take consecutive
+// parts of the bin's points in order and spread each one's whole 8 x 8 x 8
 // footprint (lanes over 8 x 8 in x, y; 8 steps in z) into the wave's private
-// LDS copy of the bin's padded PE^3 region (PE = BB + W3 - 1; 28.8 KiB, so a
-// CU holds five bins).  Every point is spread
-// once, with all 64 lanes inside its footprint, and no point is tested
-// against a bin it does not belong to.  The copies are summed in a fixed
-// order and added into Fp, the fine grid extended by W3 - 1 cells per axis
-// (not periodic).  Bins two apart along every axis have disjoint padded
-// regions, so the bins run in 8 phases (one parity per axis), each adding
+// LDS copy of the bin's padded region (bin + W3 - 1 cells per axis: 11 x 11 x
+// 15 cells, 15.5 KiB, so a CU holds ten bins).  Every point is spread once,
+// with all 64 lanes inside its footprint, and no point is tested against a
+// bin it does not belong to.  The copies are summed in a fixed order and
+// added into Fp, the fine grid extended by W3 - 1 cells per axis (not
+// periodic).  Bins P apart along an axis have disjoint padded regions when
+// P b >= b + W3 - 1 (P = 3 for b = 4, 2 for b = 8), so the bins run in
+// PZ PY PX = 18 phases (the bin's index modulo P along each axis), each adding
 // without conflicts, fullest bins first; k_fold3d then wraps the overhang
 // back onto the periodic grid.  Every sum runs in a fixed order: the result
 // is deterministic.  On the 32 x 511^2 back-projection from 720 x 128 x 511
-// points the spread takes about 16 ms, against 53 ms for a tile-gather spread
-// that searched every tile's candidate points.
+// points the spread takes about 14 ms (8^3 bins in 8 phases: 16 ms), against
+// 53 ms for a tile-gather spread that searched every tile's candidate points.
 // ---------------------------------------------------------------------------
-constexpr int BB = 8;                       // bin edge (corner cells)
-constexpr int PE = BB + W3 - 1;             // padded edge (15 cells)
+// bin edges (corner cells) along z, y, x; 8 x 8 x 8, 4 x 8 x 8, 8 x 4 x 4,
+// 4 x 4 x 4 and 2 x 4 x 8 measured 3-10% slower on the 32 x 511^2 back-projection
+constexpr int BZ = 4, BY = 4, BX = 8;
+constexpr int RZ = BZ + W3 - 1, RY = BY + W3 - 1, RX = BX + W3 - 1;   // padded region
 constexpr int PXS = 16;                     // padded row stride (cells)
+static_assert(RX <= PXS, "a region row must fit the padded row stride");
+// bins P apart along an axis have disjoint padded regions when P b >= b + W3 - 1
+constexpr int phase_period(int b) { return (b + W3 - 1 + b - 1) / b; }
+constexpr int PZ = phase_period(BZ), PY = phase_period(BY), PX = phase_period(BX);
+constexpr int NPH = PZ * PY * PX;           // launch phases
+// bits of the launch-order key: the phase above a 26-bit descending count
+constexpr int ORDER_BITS = 26 + (NPH > 16 ? 5 : NPH > 8 ? 4 : NPH > 4 ? 3 : NPH > 2 ? 2 : 1);
+static_assert(ORDER_BITS <= 32, "launch-order key too wide");
 constexpr int SB3_WAVES = 1;   // waves per bin (1, 2 and 4 measured: 1 is fastest)
 // padded cell (z, y, x) -> LDS index; x XOR-ed with 8 on every other row
 // pair so the 8 rows x 8 columns of one z step fall on distinct bank pairs
 __device__ __forceinline__ int ptile_idx(int z, int y, int x) {
-    return (z * PE + y) * PXS + (x ^ (((y >> 1) & 1) << 3));
+    return (z * RY + y) * PXS + (x ^ (((y >> 1) & 1) << 3));
 }
 
 struct Bins3 {
-    int nb[3];          // bins per axis, ceil(nf / BB)
+    int nb[3];          // bins per axis, ceil(nf / bin edge)
     int pf[3];          // padded grid dims nf + W3 - 1
 };
 
@@ -241,7 +252,7 @@ __global__ void k_bin_keys(const double *__restrict__ rot, Grid3 g, Bins3 bn, un
     const Corner3 c = corner3(rot, p, j, k, g);
     unsigned b = nbins;
     if (c.ok) {
-        const int bz = wrapi(c.lz, g.nf[0]) / BB, by = wrapi(c.ly, g.nf[1]) / BB, bx = wrapi(c.lx, g.nf[2]) / BB;
+        const int bz = wrapi(c.lz, g.nf[0]) / BZ, by = wrapi(c.ly, g.nf[1]) / BY, bx = wrapi(c.lx, g.nf[2]) / BX;
         b = (unsigned)(((int64_t)bz * bn.nb[1] + by) * bn.nb[2] + bx);
     }
     key[i] = b;
@@ -260,8 +271,8 @@ __global__ void k_bin_ranges(const unsigned *__restrict__ skey, int64_t n, unsig
     if (i == n - 1 || skey[i + 1] != b) end[b] = (unsigned)(i + 1);
 }
 
-// the bins in launch order: grouped by phase (the parity of the bin's
-// coordinates), fullest first inside a phase, so a phase's long-running
+// the bins in launch order: grouped by phase (the bin's indices modulo the
+// phase periods), fullest first inside a phase, so a phase's long-running
 // dense bins start first and its tail is short.  key = phase << 26 |
 // (2^26 - 1 - count), val = bin
 __global__ void k_bin_order_keys(const unsigned *__restrict__ bstart, const unsigned *__restrict__ bend,
@@ -270,7 +281,7 @@ __global__ void k_bin_order_keys(const unsigned *__restrict__ bstart, const unsi
     const unsigned b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbins) return;
     const unsigned bx = b % bn.nb[2], by = (b / bn.nb[2]) % bn.nb[1], bz = b / (bn.nb[2] * bn.nb[1]);
-    const unsigned ph = ((bz & 1) << 2) | ((by & 1) << 1) | (bx & 1);
+    const unsigned ph = ((bz % PZ) * PY + by % PY) * PX + bx % PX;
     const unsigned cnt = min(bend[b] - bstart[b], (1u << 26) - 1);
     key[b] = (ph << 26) | ((1u << 26) - 1 - cnt);
     val[b] = b;
@@ -280,7 +291,7 @@ __global__ void __launch_bounds__(64 * SB3_WAVES)
 k_spread3d_bin(const float2 *__restrict__ C, float2 *__restrict__ Fp, const double *__restrict__ rot,
                const unsigned *__restrict__ sval, const unsigned *__restrict__ bstart,
                const unsigned *__restrict__ bend, const unsigned *__restrict__ order, Grid3 g, Bins3 bn) {
-    constexpr int PC = PE * PE * PXS;
+    constexpr int PC = RZ * RY * PXS;
     // this phase's bins, fullest first
     const int bid = (int)order[blockIdx.x];
     const int b[3] = {bid / (bn.nb[1] * bn.nb[2]), (bid / bn.nb[2]) % bn.nb[1], bid % bn.nb[2]};
@@ -290,7 +301,7 @@ k_spread3d_bin(const float2 *__restrict__ C, float2 *__restrict__ Fp, const doub
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     float2 *my = acc + wave * PC;
     for (int i = lane; i < PC; i += 64) my[i] = make_float2(0.f, 0.f);
-    const int c0[3] = {BB * b[0], BB * b[1], BB * b[2]};
+    const int c0[3] = {BZ * b[0], BY * b[1], BX * b[2]};
     const float ihw = 1.0f / (0.5f * W3);
     const int dxl = lane & 7, dyl = lane >> 3;
     // wave w takes the w-th part of the bin's points, in order
@@ -308,7 +319,7 @@ k_spread3d_bin(const float2 *__restrict__ C, float2 *__restrict__ Fp, const doub
             const int k = (int)(pt - pj * (unsigned)g.ncols);
             const int p = (int)(pj / (unsigned)g.nrows), j = (int)(pj - (unsigned)p * (unsigned)g.nrows);
             const Corner3 c = corner3(rot, p, j, k, g);
-            // the corner taken modulo nf, relative to the bin: in [0, BB)
+            // the corner taken modulo nf, relative to the bin: in [0, bin edge)
             tz = wrapi(c.lz, g.nf[0]) - c0[0];
             ty = wrapi(c.ly, g.nf[1]) - c0[1];
             tx = wrapi(c.lx, g.nf[2]) - c0[2];
@@ -333,15 +344,15 @@ k_spread3d_bin(const float2 *__restrict__ C, float2 *__restrict__ Fp, const doub
 #pragma unroll
             for (int dz = 0; dz < W3; ++dz) {
                 const float wgt = wxy * rl_f32(wzl, dz);
-                float2 &a = my[base_idx + dz * PE * PXS];
+                float2 &a = my[base_idx + dz * RY * PXS];
                 a.x += wgt * vr;
                 a.y += wgt * vi;
             }
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < PE * PE * PE; i += 64 * SB3_WAVES) {
-        const int z = i / (PE * PE), y = (i / PE) % PE, x = i % PE;
+    for (int i = threadIdx.x; i < RZ * RY * RX; i += 64 * SB3_WAVES) {
+        const int z = i / (RY * RX), y = (i / RX) % RY, x = i % RX;
         const int zz = c0[0] + z, yy = c0[1] + y, xx = c0[2] + x;
         if (zz >= bn.pf[0] || yy >= bn.pf[1] || xx >= bn.pf[2]) continue;
         const int li = ptile_idx(z, y, x);
@@ -735,7 +746,8 @@ struct Work3 {
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-int bins_of(int nf) { return (nf + BB - 1) / BB; }
+int bins_of(int nf, int b) { return (nf + b - 1) / b; }
+unsigned nbins_of(const int nf[3]) { return (unsigned)bins_of(nf[0], BZ) * bins_of(nf[1], BY) * bins_of(nf[2], BX); }
 
 // bits of the sort key: bins 0 .. nbins - 1 and nbins for dropped points
 int key_bits(unsigned nbins) {
@@ -749,13 +761,13 @@ size_t sort_tmp_bytes(int64_t npts, unsigned nbins) {
     LC_HIP(rocprim::radix_sort_pairs(nullptr, tb, (unsigned *)nullptr, (unsigned *)nullptr, (unsigned *)nullptr,
                                      (unsigned *)nullptr, (size_t)npts, 0, key_bits(nbins), (hipStream_t)0));
     LC_HIP(rocprim::radix_sort_pairs(nullptr, to, (unsigned *)nullptr, (unsigned *)nullptr, (unsigned *)nullptr,
-                                     (unsigned *)nullptr, (size_t)nbins, 0, 29, (hipStream_t)0));
+                                     (unsigned *)nullptr, (size_t)nbins, 0, ORDER_BITS, (hipStream_t)0));
     return std::max(tb, to);
 }
 
 // bytes of the type-1 spread's buffers beyond the fine grid
 size_t spread_bytes(int64_t npts, const int nf[3]) {
-    const unsigned nbins = (unsigned)bins_of(nf[0]) * bins_of(nf[1]) * bins_of(nf[2]);
+    const unsigned nbins = nbins_of(nf);
     return align256(sizeof(float2) * (size_t)(nf[0] + W3 - 1) * (nf[1] + W3 - 1) * (nf[2] + W3 - 1)) +
            4 * align256(sizeof(unsigned) * (size_t)npts) + 6 * align256(sizeof(unsigned) * (size_t)nbins) +
            align256(sort_tmp_bytes(npts, nbins));
@@ -779,7 +791,7 @@ Work3 work3(Ctx &c, const Geom3D &g, bool spread = false) {
         return w;
     }
     LC_REQUIRE(npts < (int64_t)UINT32_MAX, "backproj: more than 2^32 detector points");
-    const unsigned nbins = (unsigned)bins_of(u.nf[0]) * bins_of(u.nf[1]) * bins_of(u.nf[2]);
+    const unsigned nbins = nbins_of(u.nf);
     const size_t bFp = align256(sizeof(float2) * (size_t)(u.nf[0] + W3 - 1) * (u.nf[1] + W3 - 1) * (u.nf[2] + W3 - 1));
     const size_t bK = align256(sizeof(unsigned) * (size_t)npts);
     const size_t bB = align256(sizeof(unsigned) * (size_t)nbins);
@@ -890,7 +902,7 @@ void type1_3d(Ctx &c, const Geom3D &g, const Work3 &w, float *vol, bool mask, co
     Bins3 bn;
     int64_t npad = 1;
     for (int a = 0; a < 3; ++a) {
-        bn.nb[a] = bins_of(u.nf[a]);
+        bn.nb[a] = bins_of(u.nf[a], a == 0 ? BZ : a == 1 ? BY : BX);
         bn.pf[a] = u.nf[a] + W3 - 1;
         npad *= bn.pf[a];
     }
@@ -911,18 +923,17 @@ void type1_3d(Ctx &c, const Geom3D &g, const Work3 &w, float *vol, bool mask, co
                        bn, w.okey_a, w.oval_a, nbins);
     LC_HIP(hipGetLastError());
     tb = w.sort_bytes;
-    LC_HIP(rocprim::radix_sort_pairs(w.sort_tmp, tb, w.okey_a, w.okey_b, w.oval_a, w.order, (size_t)nbins, 0, 29,
-                                     c.stream()));
+    LC_HIP(rocprim::radix_sort_pairs(w.sort_tmp, tb, w.okey_a, w.okey_b, w.oval_a, w.order, (size_t)nbins, 0,
+                                     ORDER_BITS, c.stream()));
     hipLaunchKernelGGL(k_zero2, dim3(2048), dim3(256), 0, c.stream(), w.Fp, npad);
     LC_HIP(hipGetLastError());
-    const size_t shs = sizeof(float2) * SB3_WAVES * PE * PE * PXS;
+    const size_t shs = sizeof(float2) * SB3_WAVES * RZ * RY * PXS;
     int64_t first = 0;   // the phase's first entry in the launch order
-    for (int ph = 0; ph < 8; ++ph) {
+    for (int ph = 0; ph < NPH; ++ph) {
+        // phase (rz, ry, rx): the bins whose index along each axis is r mod P
+        const int r[3] = {ph / (PY * PX), (ph / PX) % PY, ph % PX}, per[3] = {PZ, PY, PX};
         int nblk = 1;
-        for (int a = 0; a < 3; ++a) {
-            const int par = (ph >> (2 - a)) & 1;
-            nblk *= (bn.nb[a] - par + 1) / 2;
-        }
+        for (int a = 0; a < 3; ++a) nblk *= std::max(0, (bn.nb[a] - r[a] + per[a] - 1) / per[a]);
         if (nblk == 0) continue;
         hipLaunchKernelGGL(k_spread3d_bin, dim3(nblk), dim3(64 * SB3_WAVES), shs, c.stream(), w.C, w.Fp,
                            g.d_rot.as<double>(), w.val_b, w.bstart, w.bend, w.order + first, gg, bn);
