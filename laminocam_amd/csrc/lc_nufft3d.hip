@@ -34,6 +34,31 @@ namespace {
 constexpr int W3 = 8;                       // kernel width (cells)
 constexpr int LN_NT = 256, LN_ME = 32;      // line passes: NT * ME complex in LDS
 
+// Line lengths with compiled (unrolled) FFTs in the line passes; other
+// lengths take the runtime plan.  The compiled passes hold NT * LN_ME_CT
+// values per block (32 KiB), so a CU holds five blocks.
+constexpr int LN_ME_CT = 16;
+template <typename F> bool with_line_len(int L, F &&f) {
+    switch (L) {
+    case 16: f(std::integral_constant<int, 16>{}); return true;
+    case 32: f(std::integral_constant<int, 32>{}); return true;
+    case 64: f(std::integral_constant<int, 64>{}); return true;
+    case 96: f(std::integral_constant<int, 96>{}); return true;
+    case 128: f(std::integral_constant<int, 128>{}); return true;
+    case 192: f(std::integral_constant<int, 192>{}); return true;
+    case 256: f(std::integral_constant<int, 256>{}); return true;
+    case 384: f(std::integral_constant<int, 384>{}); return true;
+    case 512: f(std::integral_constant<int, 512>{}); return true;
+    case 768: f(std::integral_constant<int, 768>{}); return true;
+    case 1024: f(std::integral_constant<int, 1024>{}); return true;
+    case 1536: f(std::integral_constant<int, 1536>{}); return true;
+    case 1458: f(std::integral_constant<int, 1458>{}); return true;   // 2 * 721 (2 3^6)
+    case 2048: f(std::integral_constant<int, 2048>{}); return true;
+    default: return false;
+    }
+}
+
+
 // exp(beta (sqrt(1 - z^2) - 1)) for |z| < 1, else 0.  The hardware square
 // root (within 1 ulp; s lies in (0, 1]) instead of the correctly rounded
 // sequence: one instruction instead of about fifteen, a relative weight
@@ -416,17 +441,32 @@ __global__ void k_fold3d(const float2 *__restrict__ Fp, float2 *__restrict__ F, 
 // DIR 2 (type 2): in (outer, nm, inner) (* corr, real if RIN) placed at
 //                 m_a mod nf, zeros elsewhere -> FFT- -> out (outer, nf, inner)
 // ---------------------------------------------------------------------------
+// one FFT of nl lines in LDS: row layout (rowm) or nl = 1 << lsh interleaved
+// lines; LC > 0: length LC compiled (unrolled stages), else the runtime plan
+template <int SIGN, int NT, int MAXE, int LC>
+__device__ __forceinline__ void line_fft(float2 *buf, bool rowm, int nl, int lsh, int L, const FftPlan &plan,
+                                         const float2 *__restrict__ tw) {
+    if constexpr (LC > 0) {
+        if (rowm) fft_lds_ct<float, LC, SIGN, NT, MAXE, false>(buf, nl, 0, tw);
+        else fft_lds_ct<float, LC, SIGN, NT, MAXE, true>(buf, nl, lsh, tw);
+    } else {
+        if (rowm) fft_lds<float, SIGN, NT, MAXE, false>(buf, nl, L, plan, tw);
+        else fft_lds<float, SIGN, NT, MAXE, true>(buf, nl, 0, plan, tw);
+    }
+}
+
 __device__ __forceinline__ int m2f(int a, int nm, int nf) {
     const int m = a - nm / 2;
     return m < 0 ? m + nf : m;
 }
 
-template <int DIR, bool RIN, bool ROUT, int NT, int MAXE>
+template <int DIR, bool RIN, bool ROUT, int NT, int MAXE, int LC>
 __global__ void __launch_bounds__(NT)
 k_nu_axis(const void *__restrict__ vin, void *__restrict__ vout, int64_t outer, int64_t inner,
-          int nf, int nm, int nl, const float *__restrict__ corr, float scale, FftPlan plan,
+          int nf, int nm, int nl, int lsh, const float *__restrict__ corr, float scale, FftPlan plan,
           const float2 *__restrict__ tw) {
     float2 *buf = reinterpret_cast<float2 *>(lc_smem);
+    if constexpr (LC > 0) nf = LC;
     const int Lin = DIR == 1 ? nf : nm, Lout = DIR == 1 ? nm : nf;
     const bool rowm = inner == 1;
     int64_t o0, i0;
@@ -475,8 +515,7 @@ k_nu_axis(const void *__restrict__ vin, void *__restrict__ vout, int64_t outer, 
         }
     }
     __syncthreads();
-    if (rowm) fft_lds<float, DIR == 1 ? 1 : -1, NT, MAXE, false>(buf, nl, nf, plan, tw);
-    else fft_lds<float, DIR == 1 ? 1 : -1, NT, MAXE, true>(buf, nl, 0, plan, tw);
+    line_fft<DIR == 1 ? 1 : -1, NT, MAXE, LC>(buf, rowm, nl, lsh, nf, plan, tw);
     for (int idx = threadIdx.x; idx < nl * Lout; idx += NT) {
         int l, e;
         if (rowm) { l = idx / Lout; e = idx - l * Lout; }
@@ -501,14 +540,14 @@ k_nu_axis(const void *__restrict__ vin, void *__restrict__ vout, int64_t outer, 
 // X_m = a_m sum_k (x_k a_k) b_{m-k},  a_k = e^{s i pi k^2/n},  b_k = conj(a_k)
 // computed as a_m * FFT+(FFT-(x a) * bhat) with bhat = FFT-(b) / Lb.
 // ---------------------------------------------------------------------------
-template <bool RIN, bool ROUT, int NT, int MAXE>
+template <bool RIN, bool ROUT, int NT, int MAXE, int LC>
 __global__ void __launch_bounds__(NT)
 k_det_axis(const void *__restrict__ vin, void *__restrict__ vout, int64_t outer, int64_t inner,
-           int n, int nl, int din, int dout, int sign, float scale, int blue, FftPlan plan,
+           int n, int nl, int lsh, int din, int dout, int sign, float scale, int blue, FftPlan plan,
            const float2 *__restrict__ tw, const float2 *__restrict__ chirp,
            const float2 *__restrict__ bhat) {
     float2 *buf = reinterpret_cast<float2 *>(lc_smem);
-    const int L = blue ? plan.L : n;
+    const int L = LC > 0 ? LC : blue ? plan.L : n;
     const bool rowm = inner == 1;
     int64_t o0, i0;
     int nloc;
@@ -546,16 +585,10 @@ k_det_axis(const void *__restrict__ vin, void *__restrict__ vout, int64_t outer,
     }
     __syncthreads();
     if (!blue) {
-        if (sign < 0) {
-            if (rowm) fft_lds<float, -1, NT, MAXE, false>(buf, nl, L, plan, tw);
-            else fft_lds<float, -1, NT, MAXE, true>(buf, nl, 0, plan, tw);
-        } else {
-            if (rowm) fft_lds<float, 1, NT, MAXE, false>(buf, nl, L, plan, tw);
-            else fft_lds<float, 1, NT, MAXE, true>(buf, nl, 0, plan, tw);
-        }
+        if (sign < 0) line_fft<-1, NT, MAXE, LC>(buf, rowm, nl, lsh, L, plan, tw);
+        else line_fft<1, NT, MAXE, LC>(buf, rowm, nl, lsh, L, plan, tw);
     } else {
-        if (rowm) fft_lds<float, -1, NT, MAXE, false>(buf, nl, L, plan, tw);
-        else fft_lds<float, -1, NT, MAXE, true>(buf, nl, 0, plan, tw);
+        line_fft<-1, NT, MAXE, LC>(buf, rowm, nl, lsh, L, plan, tw);
         const float2 *bh = bhat + (sign < 0 ? 0 : L);
         for (int idx = threadIdx.x; idx < nl * L; idx += NT) {
             int l, e;
@@ -565,8 +598,7 @@ k_det_axis(const void *__restrict__ vin, void *__restrict__ vout, int64_t outer,
             *p = cmul(*p, bh[e]);
         }
         __syncthreads();
-        if (rowm) fft_lds<float, 1, NT, MAXE, false>(buf, nl, L, plan, tw);
-        else fft_lds<float, 1, NT, MAXE, true>(buf, nl, 0, plan, tw);
+        line_fft<1, NT, MAXE, LC>(buf, rowm, nl, lsh, L, plan, tw);
     }
     for (int idx = threadIdx.x; idx < nl * n; idx += NT) {
         int l, e;
@@ -650,8 +682,15 @@ void det_line_init(Ctx &c, DetLine &d, int n) {
         d.blue = false;
         return;
     }
+    // any L >= 2n - 1 gives the exact DFT: the smallest 2^p 3^q, or a length
+    // with a compiled line FFT when one is within 25% of it
     int64_t L = 2 * (int64_t)n - 1;
     while (!is_2_3_smooth(L)) ++L;
+    for (int cl = (int)(2 * (int64_t)n - 1); cl <= L + L / 4 && cl <= 2048; ++cl)
+        if (with_line_len(cl, [](auto) {})) {
+            L = cl;
+            break;
+        }
     LC_REQUIRE(L <= LN_NT * LN_ME, "detector transform: width too large");
     LC_REQUIRE(fft_make_plan((int)L, d.plan), "detector transform: no Bluestein length");
     d.blue = true;
@@ -821,26 +860,57 @@ Work3 work3(Ctx &c, const Geom3D &g, bool spread = false) {
     return w;
 }
 
+// lines per block and grid of a line pass over (outer, L, inner): row mode
+// (inner == 1) takes up to 64 whole rows; column mode up to 64 consecutive
+// inner indices per block (a power of two for the compiled passes)
+struct LineShape {
+    int nl, lsh;
+    dim3 grid;
+};
+LineShape line_shape(int64_t outer, int64_t inner, int L, int cap, bool pow2, const char *what) {
+    LineShape sh;
+    sh.lsh = 0;
+    const int most = std::max(1, std::min(cap / L, 64));
+    if (inner == 1) {
+        sh.nl = most;
+        sh.grid = dim3((unsigned)ceil_div(outer, sh.nl));
+    } else {
+        LC_REQUIRE(outer <= 65535, std::string(what) + ": too many planes");
+        if (pow2) {
+            while ((2 << sh.lsh) <= most && (1 << sh.lsh) < inner) ++sh.lsh;
+            sh.nl = 1 << sh.lsh;
+        } else {
+            sh.nl = (int)std::max<int64_t>(1, std::min<int64_t>(most, inner));
+        }
+        sh.grid = dim3((unsigned)ceil_div(inner, sh.nl), (unsigned)outer);
+    }
+    return sh;
+}
+
 // one axis pass of the fine-grid transform
 template <int DIR, bool RIN, bool ROUT>
 void nu_axis(Ctx &c, const void *in, void *out, int64_t outer, int64_t inner, int nf, int nm,
              const float *corr, float scale, const FftPlan &plan) {
     const float2 *tw = c.twiddles<float>(nf);
-    const int cap = LN_NT * LN_ME;
+    const bool ct = with_line_len(nf, [](auto) {});
+    // runtime plans: the smaller block (fewer registers) while a line fits it
+    const bool small = nf <= LN_NT * LN_ME_CT;
+    const int cap = LN_NT * (small ? LN_ME_CT : LN_ME);
     LC_REQUIRE(nf <= cap, "3-D NUFFT: line too long");
-    int nl;
-    dim3 grid;
-    if (inner == 1) {
-        nl = std::max(1, std::min(cap / nf, 64));
-        grid = dim3(ceil_div(outer, nl));
-    } else {
-        nl = (int)std::max<int64_t>(1, std::min<int64_t>(std::min(cap / nf, 64), inner));
-        LC_REQUIRE(outer <= 65535, "3-D NUFFT: too many planes");
-        grid = dim3(ceil_div(inner, nl), (unsigned)outer);
+    const LineShape ls = line_shape(outer, inner, nf, cap, ct, "3-D NUFFT");
+    const size_t sh = sizeof(float2) * lds_round(ls.nl * nf);
+    if (!with_line_len(nf, [&](auto len) {
+            hipLaunchKernelGGL((k_nu_axis<DIR, RIN, ROUT, LN_NT, LN_ME_CT, decltype(len)::value>), ls.grid,
+                               dim3(LN_NT), sh, c.stream(), in, out, outer, inner, nf, nm, ls.nl, ls.lsh, corr,
+                               scale, plan, tw);
+        })) {
+        if (small)
+            hipLaunchKernelGGL((k_nu_axis<DIR, RIN, ROUT, LN_NT, LN_ME_CT, 0>), ls.grid, dim3(LN_NT), sh,
+                               c.stream(), in, out, outer, inner, nf, nm, ls.nl, ls.lsh, corr, scale, plan, tw);
+        else
+            hipLaunchKernelGGL((k_nu_axis<DIR, RIN, ROUT, LN_NT, LN_ME, 0>), ls.grid, dim3(LN_NT), sh,
+                               c.stream(), in, out, outer, inner, nf, nm, ls.nl, ls.lsh, corr, scale, plan, tw);
     }
-    const size_t sh = sizeof(float2) * lds_round(nl * nf);
-    hipLaunchKernelGGL((k_nu_axis<DIR, RIN, ROUT, LN_NT, LN_ME>), grid, dim3(LN_NT), sh, c.stream(),
-                       in, out, outer, inner, nf, nm, nl, corr, scale, plan, tw);
     LC_HIP(hipGetLastError());
 }
 
@@ -850,21 +920,27 @@ void det_axis(Ctx &c, const DetLine &d, const void *in, void *out, int64_t outer
               int din, int dout, int sign, float scale) {
     const int L = d.blue ? d.plan.L : d.n;
     const float2 *tw = c.twiddles<float>(d.plan.L);
-    const int cap = LN_NT * LN_ME;
-    int nl;
-    dim3 grid;
-    if (inner == 1) {
-        nl = std::max(1, std::min(cap / L, 64));
-        grid = dim3(ceil_div(outer, nl));
-    } else {
-        nl = (int)std::max<int64_t>(1, std::min<int64_t>(std::min(cap / L, 64), inner));
-        LC_REQUIRE(outer <= 65535, "detector transform: too many projections");
-        grid = dim3(ceil_div(inner, nl), (unsigned)outer);
+    const bool ct = with_line_len(L, [](auto) {});
+    const bool small = L <= LN_NT * LN_ME_CT;
+    const int cap = LN_NT * (small ? LN_ME_CT : LN_ME);
+    LC_REQUIRE(L <= cap, "detector transform: line too long");
+    const LineShape ls = line_shape(outer, inner, L, cap, ct, "detector transform");
+    const size_t sh = sizeof(float2) * lds_round(ls.nl * L);
+    const float2 *ch = d.blue ? d.chirp.as<float2>() : nullptr, *bh = d.blue ? d.bhat.as<float2>() : nullptr;
+    if (!with_line_len(L, [&](auto len) {
+            hipLaunchKernelGGL((k_det_axis<RIN, ROUT, LN_NT, LN_ME_CT, decltype(len)::value>), ls.grid,
+                               dim3(LN_NT), sh, c.stream(), in, out, outer, inner, d.n, ls.nl, ls.lsh, din,
+                               dout, sign, scale, d.blue ? 1 : 0, d.plan, tw, ch, bh);
+        })) {
+        if (small)
+            hipLaunchKernelGGL((k_det_axis<RIN, ROUT, LN_NT, LN_ME_CT, 0>), ls.grid, dim3(LN_NT), sh, c.stream(),
+                               in, out, outer, inner, d.n, ls.nl, ls.lsh, din, dout, sign, scale, d.blue ? 1 : 0,
+                               d.plan, tw, ch, bh);
+        else
+            hipLaunchKernelGGL((k_det_axis<RIN, ROUT, LN_NT, LN_ME, 0>), ls.grid, dim3(LN_NT), sh, c.stream(), in,
+                               out, outer, inner, d.n, ls.nl, ls.lsh, din, dout, sign, scale, d.blue ? 1 : 0,
+                               d.plan, tw, ch, bh);
     }
-    const size_t sh = sizeof(float2) * lds_round(nl * L);
-    hipLaunchKernelGGL((k_det_axis<RIN, ROUT, LN_NT, LN_ME>), grid, dim3(LN_NT), sh, c.stream(), in,
-                       out, outer, inner, d.n, nl, din, dout, sign, scale, d.blue ? 1 : 0, d.plan, tw,
-                       d.blue ? d.chirp.as<float2>() : nullptr, d.blue ? d.bhat.as<float2>() : nullptr);
     LC_HIP(hipGetLastError());
 }
 
