@@ -17,6 +17,8 @@
 // Twiddles come from a table tw[k] = e^{-2 pi i k / L} generated in FP64.
 #pragma once
 
+#include <type_traits>
+
 #include "lc_common.hpp"
 
 namespace lc {
@@ -467,6 +469,29 @@ template <typename T, int L, int SIGN, int NT, int MAXE, bool LFAST, int LSH = -
 __device__ __forceinline__ void fft_lds_ct(cpx<T> *buf, int nl, int lsh,
                                            const cpx<T> *__restrict__ tw) {
     fft_stages_ct<T, L, 0, 1, SIGN, NT, MAXE, LFAST, LSH, PIN, POUT>(buf, nl, lsh, tw);
+}
+
+// Line lengths with compiled (unrolled) line-pass FFTs (fft_lds_ct) in the
+// NUFFT projectors: f(std::integral_constant<int, L>) for those, false for
+// any other length (the runtime plan).
+template <typename F> bool with_ct_len(int L, F &&f) {
+    switch (L) {
+    case 16: f(std::integral_constant<int, 16>{}); return true;
+    case 32: f(std::integral_constant<int, 32>{}); return true;
+    case 64: f(std::integral_constant<int, 64>{}); return true;
+    case 96: f(std::integral_constant<int, 96>{}); return true;
+    case 128: f(std::integral_constant<int, 128>{}); return true;
+    case 192: f(std::integral_constant<int, 192>{}); return true;
+    case 256: f(std::integral_constant<int, 256>{}); return true;
+    case 384: f(std::integral_constant<int, 384>{}); return true;
+    case 512: f(std::integral_constant<int, 512>{}); return true;
+    case 768: f(std::integral_constant<int, 768>{}); return true;
+    case 1024: f(std::integral_constant<int, 1024>{}); return true;
+    case 1458: f(std::integral_constant<int, 1458>{}); return true;   // 2 * 721 (2 3^6)
+    case 1536: f(std::integral_constant<int, 1536>{}); return true;
+    case 2048: f(std::integral_constant<int, 2048>{}); return true;
+    default: return false;
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -67,12 +67,22 @@ __device__ __forceinline__ float es_kernel(float z, float beta) {
 // forward:  y = Re(fftshift1(ifft1(ifftshift1(C)))) / n
 // fftshift: out[k] = in[(k + n/2) % n]; ifftshift: d = n/2 + (n odd)
 // ---------------------------------------------------------------------------
-template <int SIGN, int NT, int MAXE>
+// LC > 0 (every kernel of this file): line length LC compiled, with unrolled
+// stages (fft_lds_ct); 0: the runtime plan
+template <int SIGN, int NT, int MAXE, bool LFAST, int LC>
+__device__ __forceinline__ void nu_fft(float2 *buf, int nl, int L, int lsh, const FftPlan &plan,
+                                       const float2 *__restrict__ tw) {
+    if constexpr (LC > 0) fft_lds_ct<float, LC, SIGN, NT, MAXE, LFAST>(buf, nl, lsh, tw);
+    else fft_lds<float, SIGN, NT, MAXE, LFAST>(buf, nl, LFAST ? 0 : L, plan, tw);
+}
+
+template <int SIGN, int NT, int MAXE, int LC>
 __global__ void __launch_bounds__(NT)
 k_det_fft(const float *__restrict__ real_in, const float2 *__restrict__ cplx_in,
           float2 *__restrict__ cplx_out, float *__restrict__ real_out, int nrows, int n, int nl,
           FftPlan plan, const float2 *__restrict__ tw, float scale) {
     float2 *buf = reinterpret_cast<float2 *>(lc_smem);
+    if constexpr (LC > 0) n = LC;
     // blocks never straddle slices (blockIdx.y = slice, nrows rows each): a
     // row's place in its block, which its rounding depends on, is then the
     // same whether the slice is transformed alone or inside a stack
@@ -95,7 +105,7 @@ k_det_fft(const float *__restrict__ real_in, const float2 *__restrict__ cplx_in,
         buf[lds_row(l, k, n)] = v;
     }
     __syncthreads();
-    fft_lds<float, SIGN, NT, MAXE, false>(buf, nl, n, plan, tw);
+    nu_fft<SIGN, NT, MAXE, false, LC>(buf, nl, n, 0, plan, tw);
     for (int idx = threadIdx.x; idx < nloc * n; idx += NT) {
         const int l = idx / n, k = idx - l * n;
         int src = k + dout;
@@ -365,11 +375,12 @@ __device__ __forceinline__ int mode_to_fine(int a, int N, int nf) {
 }
 
 // type 1, pass A: FFT+ of every fine row, keep the N mode columns -> V
-template <int NT, int MAXE>
+template <int NT, int MAXE, int LC>
 __global__ void __launch_bounds__(NT)
 k_fine_rows_t1(const float2 *__restrict__ fine, float2 *__restrict__ V, int nf, int N, int nl,
                FftPlan plan, const float2 *__restrict__ tw) {
     float2 *buf = reinterpret_cast<float2 *>(lc_smem);
+    if constexpr (LC > 0) nf = LC;
     const int slice = blockIdx.y;
     const int row0 = blockIdx.x * nl;
     const int nloc = min(nl, nf - row0);
@@ -379,7 +390,7 @@ k_fine_rows_t1(const float2 *__restrict__ fine, float2 *__restrict__ V, int nf, 
         buf[lds_row(l, e, nf)] = l < nloc ? fs[(int64_t)(row0 + l) * nf + e] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    fft_lds<float, 1, NT, MAXE, false>(buf, nl, nf, plan, tw);
+    nu_fft<1, NT, MAXE, false, LC>(buf, nl, nf, 0, plan, tw);
     float2 *vs = V + (int64_t)slice * nf * N;
     for (int idx = threadIdx.x; idx < nloc * N; idx += NT) {
         const int l = idx / N, b = idx - l * N;
@@ -389,11 +400,13 @@ k_fine_rows_t1(const float2 *__restrict__ fine, float2 *__restrict__ V, int nf, 
 
 // type 1, pass B: FFT+ down each mode column, keep the N mode rows,
 // deconvolve, real part / nrad -> image (N x N real)
-template <int NT, int MAXE>
+// LC > 0: B = 1 << lsh
+template <int NT, int MAXE, int LC>
 __global__ void __launch_bounds__(NT)
 k_fine_cols_t1(const float2 *__restrict__ V, float *__restrict__ img, const float *__restrict__ corr,
-               int nf, int N, int B, float scale, FftPlan plan, const float2 *__restrict__ tw) {
+               int nf, int N, int B, int lsh, float scale, FftPlan plan, const float2 *__restrict__ tw) {
     float2 *buf = reinterpret_cast<float2 *>(lc_smem);
+    if constexpr (LC > 0) nf = LC;
     const int slice = blockIdx.y;
     const int col0 = blockIdx.x * B;
     const float2 *vs = V + (int64_t)slice * nf * N;
@@ -402,7 +415,7 @@ k_fine_cols_t1(const float2 *__restrict__ V, float *__restrict__ img, const floa
         buf[lds_col(c, i, B)] = col0 + c < N ? vs[(int64_t)i * N + col0 + c] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    fft_lds<float, 1, NT, MAXE, true>(buf, B, 0, plan, tw);
+    nu_fft<1, NT, MAXE, true, LC>(buf, B, nf, lsh, plan, tw);
     float *is = img + (int64_t)slice * N * N;
     for (int idx = threadIdx.x; idx < N * B; idx += NT) {
         const int a = idx / B, c = idx - a * B;
@@ -413,11 +426,12 @@ k_fine_cols_t1(const float2 *__restrict__ V, float *__restrict__ img, const floa
 
 // type 2, pass B: deconvolve the image, place the N mode rows on the fine
 // column, FFT- -> V (nf x N)
-template <int NT, int MAXE>
+template <int NT, int MAXE, int LC>
 __global__ void __launch_bounds__(NT)
 k_fine_cols_t2(const float *__restrict__ img, float2 *__restrict__ V, const float *__restrict__ corr,
-               int nf, int N, int B, FftPlan plan, const float2 *__restrict__ tw) {
+               int nf, int N, int B, int lsh, FftPlan plan, const float2 *__restrict__ tw) {
     float2 *buf = reinterpret_cast<float2 *>(lc_smem);
+    if constexpr (LC > 0) nf = LC;
     const int slice = blockIdx.y;
     const int col0 = blockIdx.x * B;
     const float *is = img + (int64_t)slice * N * N;
@@ -429,7 +443,7 @@ k_fine_cols_t2(const float *__restrict__ img, float2 *__restrict__ V, const floa
         if (b < N) buf[lds_col(c, mode_to_fine(a, N, nf), B)] = make_float2(is[(int64_t)a * N + b] * corr[a] * corr[b], 0.f);
     }
     __syncthreads();
-    fft_lds<float, -1, NT, MAXE, true>(buf, B, 0, plan, tw);
+    nu_fft<-1, NT, MAXE, true, LC>(buf, B, nf, lsh, plan, tw);
     float2 *vs = V + (int64_t)slice * nf * N;
     for (int idx = threadIdx.x; idx < nf * B; idx += NT) {
         const int i = idx / B, c = idx - i * B;
@@ -439,11 +453,12 @@ k_fine_cols_t2(const float *__restrict__ img, float2 *__restrict__ V, const floa
 
 // type 2, pass A: place the N mode columns of each V row on the fine row,
 // FFT- -> fine grid
-template <int NT, int MAXE>
+template <int NT, int MAXE, int LC>
 __global__ void __launch_bounds__(NT)
 k_fine_rows_t2(const float2 *__restrict__ V, float2 *__restrict__ fine, int nf, int N, int nl,
                FftPlan plan, const float2 *__restrict__ tw) {
     float2 *buf = reinterpret_cast<float2 *>(lc_smem);
+    if constexpr (LC > 0) nf = LC;
     const int slice = blockIdx.y;
     const int row0 = blockIdx.x * nl;
     const int nloc = min(nl, nf - row0);
@@ -455,7 +470,7 @@ k_fine_rows_t2(const float2 *__restrict__ V, float2 *__restrict__ fine, int nf, 
         buf[lds_row(l, mode_to_fine(b, N, nf), nf)] = vs[(int64_t)(row0 + l) * N + b];
     }
     __syncthreads();
-    fft_lds<float, -1, NT, MAXE, false>(buf, nl, nf, plan, tw);
+    nu_fft<-1, NT, MAXE, false, LC>(buf, nl, nf, 0, plan, tw);
     float2 *fs = fine + (int64_t)slice * nf * nf;
     for (int idx = threadIdx.x; idx < nloc * nf; idx += NT) {
         const int l = idx / nf, e = idx - l * nf;
@@ -557,6 +572,19 @@ void nufft2d_init(Ctx &c, Geom2D &g) {
 
 namespace {
 constexpr int NU_NT = 256, NU_ME = 16, NU_MEC = 32;   // rows / columns: NT*ME elements in LDS
+// compiled column passes: 512 threads x NU_ME, B = a power of two columns
+constexpr int NU_NT_CT = 512;
+struct ColShape {
+    int B, lsh;
+};
+ColShape col_shape(int nf, int N) {
+    ColShape cs{1, 0};
+    while (2 * cs.B * nf <= NU_NT_CT * NU_ME && cs.B < N) {
+        cs.B *= 2;
+        ++cs.lsh;
+    }
+    return cs;
+}
 
 SpreadGeom spread_geom(const Geom2D &g) {
     SpreadGeom s;
@@ -580,12 +608,16 @@ void det_transform(Ctx &c, const Geom2D &g, int sign, const float *real_in, cons
         const int nl = std::max(1, std::min(NU_NT * NU_ME / n, 64));
         const size_t sh = sizeof(float2) * lds_round(nl * n);
         const dim3 grid(ceil_div(nang, nl), ns);
-        if (sign < 0)
-            hipLaunchKernelGGL((k_det_fft<-1, NU_NT, NU_ME>), grid, dim3(NU_NT), sh, c.stream(), real_in,
-                               cplx_in, cplx_out, real_out, nang, n, nl, g.nu.plan_det, tw, scale);
-        else
-            hipLaunchKernelGGL((k_det_fft<1, NU_NT, NU_ME>), grid, dim3(NU_NT), sh, c.stream(), real_in,
-                               cplx_in, cplx_out, real_out, nang, n, nl, g.nu.plan_det, tw, scale);
+        auto launch = [&](auto len) {
+            constexpr int LC = decltype(len)::value;
+            if (sign < 0)
+                hipLaunchKernelGGL((k_det_fft<-1, NU_NT, NU_ME, LC>), grid, dim3(NU_NT), sh, c.stream(),
+                                   real_in, cplx_in, cplx_out, real_out, nang, n, nl, g.nu.plan_det, tw, scale);
+            else
+                hipLaunchKernelGGL((k_det_fft<1, NU_NT, NU_ME, LC>), grid, dim3(NU_NT), sh, c.stream(),
+                                   real_in, cplx_in, cplx_out, real_out, nang, n, nl, g.nu.plan_det, tw, scale);
+        };
+        if (!with_ct_len(n, launch)) launch(std::integral_constant<int, 0>{});
     } else {
         dim3 grid(nrows);
         if (sign < 0)
@@ -625,13 +657,25 @@ void type1(Ctx &c, const Geom2D &g, const NuWork &w, float *vol, int ns) {
     LC_HIP(hipGetLastError());
     const float2 *tw = c.twiddles<float>(nf);
     const int nl = std::min(NU_NT * NU_ME / nf, nf);
-    hipLaunchKernelGGL((k_fine_rows_t1<NU_NT, NU_ME>), dim3(ceil_div(nf, nl), ns), dim3(NU_NT),
-                       sizeof(float2) * lds_round(nl * nf), c.stream(), w.F, w.V, nf, N, nl, u.plan_nf, tw);
+    const size_t shr = sizeof(float2) * lds_round(nl * nf);
+    if (!with_ct_len(nf, [&](auto len) {
+            hipLaunchKernelGGL((k_fine_rows_t1<NU_NT, NU_ME, decltype(len)::value>), dim3(ceil_div(nf, nl), ns),
+                               dim3(NU_NT), shr, c.stream(), w.F, w.V, nf, N, nl, u.plan_nf, tw);
+        }))
+        hipLaunchKernelGGL((k_fine_rows_t1<NU_NT, NU_ME, 0>), dim3(ceil_div(nf, nl), ns), dim3(NU_NT), shr,
+                           c.stream(), w.F, w.V, nf, N, nl, u.plan_nf, tw);
     LC_HIP(hipGetLastError());
-    const int B = std::min(NU_NT * NU_MEC / nf, N);
-    hipLaunchKernelGGL((k_fine_cols_t1<NU_NT, NU_MEC>), dim3(ceil_div(N, B), ns), dim3(NU_NT),
-                       sizeof(float2) * lds_round(B * nf), c.stream(), w.V, vol, u.corr.as<float>(), nf, N,
-                       B, 1.0f / (float)g.nrad, u.plan_nf, tw);
+    if (!with_ct_len(nf, [&](auto len) {
+            const ColShape cs = col_shape(nf, N);
+            hipLaunchKernelGGL((k_fine_cols_t1<NU_NT_CT, NU_ME, decltype(len)::value>), dim3(ceil_div(N, cs.B), ns),
+                               dim3(NU_NT_CT), sizeof(float2) * lds_round(cs.B * nf), c.stream(), w.V, vol,
+                               u.corr.as<float>(), nf, N, cs.B, cs.lsh, 1.0f / (float)g.nrad, u.plan_nf, tw);
+        })) {
+        const int B = std::min(NU_NT * NU_MEC / nf, N);
+        hipLaunchKernelGGL((k_fine_cols_t1<NU_NT, NU_MEC, 0>), dim3(ceil_div(N, B), ns), dim3(NU_NT),
+                           sizeof(float2) * lds_round(B * nf), c.stream(), w.V, vol, u.corr.as<float>(), nf, N,
+                           B, 0, 1.0f / (float)g.nrad, u.plan_nf, tw);
+    }
     LC_HIP(hipGetLastError());
 }
 
@@ -641,14 +685,26 @@ void type2(Ctx &c, const Geom2D &g, const NuWork &w, const float *vol, int ns) {
     const int N = (int)g.N, nf = u.nf;
     const int64_t npts = g.nang * g.nrad;
     const float2 *tw = c.twiddles<float>(nf);
-    const int B = std::min(NU_NT * NU_MEC / nf, N);
-    hipLaunchKernelGGL((k_fine_cols_t2<NU_NT, NU_MEC>), dim3(ceil_div(N, B), ns), dim3(NU_NT),
-                       sizeof(float2) * lds_round(B * nf), c.stream(), vol, w.V, u.corr.as<float>(), nf,
-                       N, B, u.plan_nf, tw);
+    if (!with_ct_len(nf, [&](auto len) {
+            const ColShape cs = col_shape(nf, N);
+            hipLaunchKernelGGL((k_fine_cols_t2<NU_NT_CT, NU_ME, decltype(len)::value>), dim3(ceil_div(N, cs.B), ns),
+                               dim3(NU_NT_CT), sizeof(float2) * lds_round(cs.B * nf), c.stream(), vol, w.V,
+                               u.corr.as<float>(), nf, N, cs.B, cs.lsh, u.plan_nf, tw);
+        })) {
+        const int B = std::min(NU_NT * NU_MEC / nf, N);
+        hipLaunchKernelGGL((k_fine_cols_t2<NU_NT, NU_MEC, 0>), dim3(ceil_div(N, B), ns), dim3(NU_NT),
+                           sizeof(float2) * lds_round(B * nf), c.stream(), vol, w.V, u.corr.as<float>(), nf,
+                           N, B, 0, u.plan_nf, tw);
+    }
     LC_HIP(hipGetLastError());
     const int nl = std::min(NU_NT * NU_ME / nf, nf);
-    hipLaunchKernelGGL((k_fine_rows_t2<NU_NT, NU_ME>), dim3(ceil_div(nf, nl), ns), dim3(NU_NT),
-                       sizeof(float2) * lds_round(nl * nf), c.stream(), w.V, w.F, nf, N, nl, u.plan_nf, tw);
+    const size_t shr = sizeof(float2) * lds_round(nl * nf);
+    if (!with_ct_len(nf, [&](auto len) {
+            hipLaunchKernelGGL((k_fine_rows_t2<NU_NT, NU_ME, decltype(len)::value>), dim3(ceil_div(nf, nl), ns),
+                               dim3(NU_NT), shr, c.stream(), w.V, w.F, nf, N, nl, u.plan_nf, tw);
+        }))
+        hipLaunchKernelGGL((k_fine_rows_t2<NU_NT, NU_ME, 0>), dim3(ceil_div(nf, nl), ns), dim3(NU_NT), shr,
+                           c.stream(), w.V, w.F, nf, N, nl, u.plan_nf, tw);
     LC_HIP(hipGetLastError());
     const SpreadGeom sg = spread_geom(g);
     hipLaunchKernelGGL(k_interp2d, dim3(ceil_div(npts, 256), ns), dim3(256), 0, c.stream(), w.F, w.C,

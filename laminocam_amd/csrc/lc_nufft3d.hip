@@ -34,30 +34,10 @@ namespace {
 constexpr int W3 = 8;                       // kernel width (cells)
 constexpr int LN_NT = 256, LN_ME = 32;      // line passes: NT * ME complex in LDS
 
-// Line lengths with compiled (unrolled) FFTs in the line passes; other
-// lengths take the runtime plan.  The compiled passes hold NT * LN_ME_CT
-// values per block (32 KiB), so a CU holds five blocks.
+// The compiled line passes hold NT * LN_ME_CT values per block (32 KiB), so
+// a CU holds five blocks.
 constexpr int LN_ME_CT = 16;
-template <typename F> bool with_line_len(int L, F &&f) {
-    switch (L) {
-    case 16: f(std::integral_constant<int, 16>{}); return true;
-    case 32: f(std::integral_constant<int, 32>{}); return true;
-    case 64: f(std::integral_constant<int, 64>{}); return true;
-    case 96: f(std::integral_constant<int, 96>{}); return true;
-    case 128: f(std::integral_constant<int, 128>{}); return true;
-    case 192: f(std::integral_constant<int, 192>{}); return true;
-    case 256: f(std::integral_constant<int, 256>{}); return true;
-    case 384: f(std::integral_constant<int, 384>{}); return true;
-    case 512: f(std::integral_constant<int, 512>{}); return true;
-    case 768: f(std::integral_constant<int, 768>{}); return true;
-    case 1024: f(std::integral_constant<int, 1024>{}); return true;
-    case 1536: f(std::integral_constant<int, 1536>{}); return true;
-    case 1458: f(std::integral_constant<int, 1458>{}); return true;   // 2 * 721 (2 3^6)
-    case 2048: f(std::integral_constant<int, 2048>{}); return true;
-    default: return false;
-    }
-}
-
+template <typename F> bool with_line_len(int L, F &&f) { return with_ct_len(L, f); }
 
 // exp(beta (sqrt(1 - z^2) - 1)) for |z| < 1, else 0.  The hardware square
 // root (within 1 ulp; s lies in (0, 1]) instead of the correctly rounded
