@@ -20,6 +20,8 @@
 // copies are summed in a fixed order at the end.
 #include <cmath>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "lc_engine.hpp"
 
 
@@ -318,6 +320,208 @@ k_spread2d(const float2 *__restrict__ C, float2 *__restrict__ fine, const double
         const int cy = i / NU_T, cx = (i - cy * NU_T) ^ nu_swx(cy);
         const int gxx = x0 + cx, gyy = y0 + cy;
         if (gxx < g.nf && gyy < g.nf) fs[(int64_t)gyy * g.nf + gxx] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Type-1 spreading by sorted footprint-corner bins (the form of the 3-D
+// spread).  The fine grid of every slice is cut into B2 x B2 bins of
+// footprint corners (a point's corner is its first footprint cell,
+// ceil(g - 4) per axis, taken modulo nf).  k_bin2_keys gives every point of
+// the stack its (slice, bin) key; a stable radix sort orders the points by
+// key, keeping point order inside a bin; k_bin2_ranges marks each bin's run.
+// A wave then owns one bin: it takes the bin's points in order and spreads
+// each point's 8 x 8 footprint (one cell per lane) into its LDS copy of the
+// bin's padded region (B2 + 7 cells per edge), then adds the region into the
+// slice's fine grid, wrapping the overhang.  When nf is a multiple of 2 B2
+// the bins of one parity per axis have disjoint wrapped regions, so the bins
+// run in 4 phases, each adding without conflicts, fullest bins first.  Every
+// sum runs in a fixed order and a slice's points and bins do not depend on
+// the other slices of the stack: the result is deterministic and a slice
+// spreads the same alone as in a stack.
+// ---------------------------------------------------------------------------
+constexpr int B2 = 16;                 // bin edge (corner cells)
+constexpr int R2 = B2 + NU_W - 1;      // padded region edge (23 cells)
+constexpr int R2S = 24;                // region row stride: rows y, y + 1 on opposite bank halves
+constexpr int B2_ORDER_BITS = 28;      // launch-order key: phase (2 bits) above a 26-bit count
+
+struct Bins2 {
+    int nb;             // bins per fine-grid edge (nf / B2)
+    unsigned nbins;     // bins per slice
+    int npts;           // points per slice (nang * nrad)
+};
+
+// fine-grid footprint corner (unwrapped) and its offsets of point p of a slice
+__device__ __forceinline__ void corner2(const double2 *__restrict__ cs, const SpreadGeom &g, int p, int &lx0,
+                                        int &ly0, float &fx, float &fy) {
+    const int ang = p / g.nrad, k = p - ang * g.nrad;
+    const double2 t = cs[ang];
+    const double r = ((double)k + 0.5) * g.dr - M_PI;
+    const double gx = 0.5 * g.nf + g.sc * r * t.x, gy = 0.5 * g.nf + g.sc * r * t.y;
+    const double hw = 0.5 * NU_W;
+    lx0 = (int)ceil(gx - hw);
+    ly0 = (int)ceil(gy - hw);
+    fx = (float)((double)lx0 - gx);   // footprint start - point, in (-hw - 1, -hw]
+    fy = (float)((double)ly0 - gy);
+}
+
+__device__ __forceinline__ int wrap_nf(int a, int n) {
+    a %= n;
+    return a < 0 ? a + n : a;
+}
+
+// key[i] = slice * nbins + bin of point i of the stack, val[i] = i
+__global__ void k_bin2_keys(const double2 *__restrict__ cs, SpreadGeom g, Bins2 bn, unsigned *__restrict__ key,
+                            unsigned *__restrict__ val, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = (int)(i / bn.npts), p = (int)(i - (int64_t)s * bn.npts);
+    int lx0, ly0;
+    float fx, fy;
+    corner2(cs, g, p, lx0, ly0, fx, fy);
+    const int bx = wrap_nf(lx0, g.nf) / B2, by = wrap_nf(ly0, g.nf) / B2;
+    key[i] = (unsigned)s * bn.nbins + (unsigned)(by * bn.nb + bx);
+    val[i] = (unsigned)i;
+}
+
+// [start[b], end[b]) is bin b's run in the sorted keys (both cleared first)
+__global__ void k_bin2_ranges(const unsigned *__restrict__ skey, int64_t n, unsigned *__restrict__ start,
+                              unsigned *__restrict__ end) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned b = skey[i];
+    if (i == 0 || skey[i - 1] != b) start[b] = (unsigned)i;
+    if (i == n - 1 || skey[i + 1] != b) end[b] = (unsigned)(i + 1);
+}
+
+// launch order: phase (bin parity per axis), then fullest first
+__global__ void k_bin2_order_keys(const unsigned *__restrict__ bstart, const unsigned *__restrict__ bend, Bins2 bn,
+                                  unsigned *__restrict__ key, unsigned *__restrict__ val, unsigned total) {
+    const unsigned b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= total) return;
+    const unsigned local = b % bn.nbins;
+    const unsigned by = local / bn.nb, bx = local - by * bn.nb;
+    const unsigned ph = ((by & 1) << 1) | (bx & 1);
+    const unsigned cnt = min(bend[b] - bstart[b], (1u << 26) - 1);
+    key[b] = (ph << 26) | ((1u << 26) - 1 - cnt);
+    val[b] = b;
+}
+
+__global__ void k_zero_u32_2(unsigned *p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+__global__ void k_zero_f2(float2 *p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = make_float2(0.f, 0.f);
+}
+
+// Bins of more than B2_HEAVY points (near the k-space origin every angle
+// passes: ~8 samples per angle in the central bin) are spread by NW2_HEAVY
+// waves, each taking a consecutive part of the bin's points into its own copy
+// of the region; the copies are summed in wave order.  Whether a bin is heavy
+// depends only on its own count, so a slice's result is the same alone and in
+// a stack.
+constexpr unsigned B2_HEAVY = 512;
+constexpr int NW2_HEAVY = 12;   // 12 region copies: 53 KiB of LDS
+
+// spread bin bid's points with the block's NW waves into their region copies
+// (reg: NW x R2 x R2S) and add the copies' fixed-order sum into the fine grid
+template <int NW>
+__device__ __forceinline__ void spread2_bin(const float2 *__restrict__ C, float2 *__restrict__ fine,
+                                            const double2 *__restrict__ cs, const unsigned *__restrict__ sval,
+                                            unsigned bid, unsigned s0, unsigned s1, const SpreadGeom &g,
+                                            const Bins2 &bn, float2 *reg) {
+    const int slice = (int)(bid / bn.nbins);
+    const int local = (int)(bid - (unsigned)slice * bn.nbins);
+    const int by = local / bn.nb, bx = local - by * bn.nb;
+    const int c0y = by * B2, c0x = bx * B2;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int RC = R2 * R2S;   // cells of one region copy
+    static_assert(RC % 2 == 0, "the region clears as float4");
+    for (int i = threadIdx.x; i < NW * RC / 2; i += 64 * NW)
+        reinterpret_cast<float4 *>(reg)[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    __syncthreads();
+    float2 *my = reg + wave * RC;
+    const int dx = lane & 7, dy = lane >> 3;
+    const float inv_hw = 1.0f / (0.5f * NU_W);
+    const float2 *Cs = C + (int64_t)slice * bn.npts;
+    const unsigned pbase = (unsigned)slice * (unsigned)bn.npts;
+    // wave w takes the w-th part of the bin's points, in order
+    const unsigned per = (s1 - s0 + NW - 1) / NW;
+    const unsigned ws = s0 + wave * per, we = min(s1, ws + per);
+    for (unsigned base = ws; base < we; base += 64) {
+        const unsigned i = base + lane;
+        const bool ok = i < we;
+        int tx = 0, ty = 0;
+        float fx = 0.f, fy = 0.f;
+        float2 cv = make_float2(0.f, 0.f);
+        if (ok) {
+            const int p = (int)(sval[i] - pbase);
+            int lx0, ly0;
+            corner2(cs, g, p, lx0, ly0, fx, fy);
+            // the corner taken modulo nf, relative to the bin: in [0, B2)
+            tx = wrap_nf(lx0, g.nf) - c0x;
+            ty = wrap_nf(ly0, g.nf) - c0y;
+            cv = Cs[p];
+        }
+        unsigned long long pts = __ballot(ok);
+        while (pts) {
+            const int s2 = __ffsll((long long)pts) - 1;
+            pts &= pts - 1;
+            const int px0 = __builtin_amdgcn_readlane(tx, s2);
+            const int py0 = __builtin_amdgcn_readlane(ty, s2);
+            const float wgt = es_kernel(((float)dx + rl_f32(fx, s2)) * inv_hw, g.beta) *
+                              es_kernel(((float)dy + rl_f32(fy, s2)) * inv_hw, g.beta);
+            const float vr = rl_f32(cv.x, s2), vi = rl_f32(cv.y, s2);
+            float2 &a = my[(py0 + dy) * R2S + px0 + dx];
+            a.x += wgt * vr;
+            a.y += wgt * vi;
+        }
+    }
+    __syncthreads();
+    float2 *fs = fine + (int64_t)slice * g.nf * g.nf;
+    for (int i = threadIdx.x; i < R2 * R2; i += 64 * NW) {
+        const int y = i / R2, x = i - y * R2;
+        int gy = c0y + y, gx = c0x + x;
+        gy = gy < g.nf ? gy : gy - g.nf;
+        gx = gx < g.nf ? gx : gx - g.nf;
+        float2 v = reg[y * R2S + x];
+        for (int w = 1; w < NW; ++w) {
+            v.x += reg[w * RC + y * R2S + x].x;
+            v.y += reg[w * RC + y * R2S + x].y;
+        }
+        float2 &f = fs[(int64_t)gy * g.nf + gx];
+        f.x += v.x;
+        f.y += v.y;
+    }
+}
+
+// one wave per light bin of this phase (order: the phase's bins, fullest first)
+__global__ void __launch_bounds__(64)
+k_spread2d_bin(const float2 *__restrict__ C, float2 *__restrict__ fine, const double2 *__restrict__ cs,
+               const unsigned *__restrict__ sval, const unsigned *__restrict__ bstart,
+               const unsigned *__restrict__ bend, const unsigned *__restrict__ order, SpreadGeom g, Bins2 bn) {
+    const unsigned bid = order[blockIdx.x];
+    const unsigned s0 = bstart[bid], s1 = bend[bid];
+    if (s0 >= s1 || s1 - s0 > B2_HEAVY) return;   // empty, or a heavy bin
+    spread2_bin<1>(C, fine, cs, sval, bid, s0, s1, g, bn, reinterpret_cast<float2 *>(lc_smem));
+}
+
+// the phase's heavy bins (a prefix of its order): each block takes bins
+// j, j + gridDim.x, ... until the first light one
+__global__ void __launch_bounds__(64 * NW2_HEAVY)
+k_spread2d_bin_heavy(const float2 *__restrict__ C, float2 *__restrict__ fine, const double2 *__restrict__ cs,
+                     const unsigned *__restrict__ sval, const unsigned *__restrict__ bstart,
+                     const unsigned *__restrict__ bend, const unsigned *__restrict__ order, unsigned nphase,
+                     SpreadGeom g, Bins2 bn) {
+    for (unsigned j = blockIdx.x; j < nphase; j += gridDim.x) {
+        const unsigned bid = order[j];
+        const unsigned s0 = bstart[bid], s1 = bend[bid];
+        if (s1 - s0 <= B2_HEAVY) break;   // the order is fullest first: the rest are light
+        spread2_bin<NW2_HEAVY>(C, fine, cs, sval, bid, s0, s1, g, bn, reinterpret_cast<float2 *>(lc_smem));
+        __syncthreads();   // the copies are cleared for the next bin only after every wave has flushed
     }
 }
 
@@ -634,16 +838,122 @@ void det_transform(Ctx &c, const Geom2D &g, int sign, const float *real_in, cons
 namespace {
 struct NuWork {
     float2 *C, *F, *V;
+    // the binned spread's keys, indices, bin runs, launch order and sort
+    // scratch (null when the spread is not binned)
+    unsigned *key_a = nullptr, *key_b = nullptr, *val_a = nullptr, *val_b = nullptr;
+    unsigned *bstart = nullptr, *bend = nullptr, *okey_a = nullptr, *okey_b = nullptr, *oval_a = nullptr,
+             *order = nullptr;
+    void *sort_tmp = nullptr;
+    size_t sort_bytes = 0;
 };
 
-NuWork nu_work(Ctx &c, const Geom2D &g, int ns) {
+size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// bits of sort keys 0 .. total - 1
+int key_bits2(uint64_t total) {
+    int b = 1;
+    while (b < 32 && (1ull << b) < total) ++b;
+    return b;
+}
+
+// the binned spread needs nf a multiple of 2 B2 (conflict-free wrapped
+// phases) and 32-bit point indices and keys; otherwise the tile gather
+bool spread_binned(int64_t ns, int64_t npts, int nf) {
+    const uint64_t nbins = (uint64_t)(nf / B2) * (nf / B2);
+    return nf % (2 * B2) == 0 && (uint64_t)(ns * npts) < (1ull << 32) && ns * nbins < (1ull << 32);
+}
+
+// the binned spread's buffers beyond C, F and V: 4 x 32-bit per point, 6 x
+// 32-bit per bin, and the larger of the two sorts' scratch
+size_t spread2_extra_bytes(int64_t ns, int64_t npts, int nf, size_t *tmp_bytes = nullptr) {
+    const uint64_t n = (uint64_t)(ns * npts), total = (uint64_t)ns * (nf / B2) * (nf / B2);
+    size_t tb = 0, to = 0;
+    LC_HIP(rocprim::radix_sort_pairs(nullptr, tb, (unsigned *)nullptr, (unsigned *)nullptr, (unsigned *)nullptr,
+                                     (unsigned *)nullptr, (size_t)n, 0, key_bits2(total), (hipStream_t)0));
+    LC_HIP(rocprim::radix_sort_pairs(nullptr, to, (unsigned *)nullptr, (unsigned *)nullptr, (unsigned *)nullptr,
+                                     (unsigned *)nullptr, (size_t)total, 0, B2_ORDER_BITS, (hipStream_t)0));
+    const size_t t = al256(std::max(tb, to));
+    if (tmp_bytes) *tmp_bytes = t;
+    return 4 * al256(sizeof(unsigned) * n) + 6 * al256(sizeof(unsigned) * total) + t;
+}
+
+NuWork nu_work(Ctx &c, const Geom2D &g, int ns, bool spread = false) {
     const int N = (int)g.N, nf = g.nu.nf;
-    const size_t bC = sizeof(float2) * (size_t)ns * g.nang * g.nrad;
-    const size_t bF = sizeof(float2) * (size_t)ns * nf * nf;
-    const size_t bV = sizeof(float2) * (size_t)ns * nf * N;
-    char *ws = reinterpret_cast<char *>(c.scratch(3, bC + bF + bV));
-    return NuWork{reinterpret_cast<float2 *>(ws), reinterpret_cast<float2 *>(ws + bC),
-                  reinterpret_cast<float2 *>(ws + bC + bF)};
+    const int64_t npts = g.nang * g.nrad;
+    const size_t bC = al256(sizeof(float2) * (size_t)ns * npts);
+    const size_t bF = al256(sizeof(float2) * (size_t)ns * nf * nf);
+    const size_t bV = al256(sizeof(float2) * (size_t)ns * nf * N);
+    spread = spread && spread_binned(ns, npts, nf);
+    size_t tmp = 0;
+    const size_t extra = spread ? spread2_extra_bytes(ns, npts, nf, &tmp) : 0;
+    char *ws = reinterpret_cast<char *>(c.scratch(3, bC + bF + bV + extra));
+    NuWork w;
+    w.C = reinterpret_cast<float2 *>(ws);
+    w.F = reinterpret_cast<float2 *>(ws + bC);
+    w.V = reinterpret_cast<float2 *>(ws + bC + bF);
+    if (spread) {
+        const size_t bK = al256(sizeof(unsigned) * (size_t)(ns * npts));
+        const size_t bB = al256(sizeof(unsigned) * (size_t)ns * (nf / B2) * (nf / B2));
+        char *q = ws + bC + bF + bV;
+        w.key_a = reinterpret_cast<unsigned *>(q);
+        w.key_b = reinterpret_cast<unsigned *>(q + bK);
+        w.val_a = reinterpret_cast<unsigned *>(q + 2 * bK);
+        w.val_b = reinterpret_cast<unsigned *>(q + 3 * bK);
+        q += 4 * bK;
+        w.bstart = reinterpret_cast<unsigned *>(q);   // bstart and bend are adjacent
+        w.bend = reinterpret_cast<unsigned *>(q + bB);
+        w.okey_a = reinterpret_cast<unsigned *>(q + 2 * bB);
+        w.okey_b = reinterpret_cast<unsigned *>(q + 3 * bB);
+        w.oval_a = reinterpret_cast<unsigned *>(q + 4 * bB);
+        w.order = reinterpret_cast<unsigned *>(q + 5 * bB);
+        w.sort_tmp = q + 6 * bB;
+        w.sort_bytes = tmp;
+    }
+    return w;
+}
+
+// type-1 spread of the stack by sorted bins: C -> F (ns x nf x nf)
+void spread2_binned(Ctx &c, const Geom2D &g, const NuWork &w, int ns) {
+    const SpreadGeom sg = spread_geom(g);
+    Bins2 bn;
+    bn.nb = sg.nf / B2;
+    bn.nbins = (unsigned)bn.nb * bn.nb;
+    bn.npts = (int)(g.nang * g.nrad);
+    const int64_t n = (int64_t)ns * bn.npts;
+    const unsigned total = (unsigned)ns * bn.nbins;
+    const double2 *cs = g.nu.cs.as<double2>();
+    hipLaunchKernelGGL(k_bin2_keys, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, c.stream(), cs, sg, bn,
+                       w.key_a, w.val_a, n);
+    LC_HIP(hipGetLastError());
+    size_t tb = w.sort_bytes;
+    LC_HIP(rocprim::radix_sort_pairs(w.sort_tmp, tb, w.key_a, w.key_b, w.val_a, w.val_b, (size_t)n, 0,
+                                     key_bits2(total), c.stream()));
+    hipLaunchKernelGGL(k_zero_u32_2, dim3(1024), dim3(256), 0, c.stream(), w.bstart,
+                       (int64_t)(w.bend - w.bstart) + total);
+    hipLaunchKernelGGL(k_bin2_ranges, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, c.stream(), w.key_b, n,
+                       w.bstart, w.bend);
+    hipLaunchKernelGGL(k_bin2_order_keys, dim3(ceil_div(total, 256u)), dim3(256), 0, c.stream(), w.bstart,
+                       w.bend, bn, w.okey_a, w.oval_a, total);
+    LC_HIP(hipGetLastError());
+    tb = w.sort_bytes;
+    LC_HIP(rocprim::radix_sort_pairs(w.sort_tmp, tb, w.okey_a, w.okey_b, w.oval_a, w.order, (size_t)total, 0,
+                                     B2_ORDER_BITS, c.stream()));
+    hipLaunchKernelGGL(k_zero_f2, dim3(2048), dim3(256), 0, c.stream(), w.F, (int64_t)ns * sg.nf * sg.nf);
+    LC_HIP(hipGetLastError());
+    const size_t sh = sizeof(float2) * R2 * R2S;
+    int64_t first = 0;   // the phase's first entry in the launch order
+    for (int ph = 0; ph < 4; ++ph) {
+        const int py = ph >> 1, px = ph & 1;
+        const int64_t nblk = (int64_t)ns * ((bn.nb - py + 1) / 2) * ((bn.nb - px + 1) / 2);
+        if (nblk == 0) continue;
+        hipLaunchKernelGGL(k_spread2d_bin_heavy, dim3((unsigned)std::min<int64_t>(nblk, 256)), dim3(64 * NW2_HEAVY),
+                           sh * NW2_HEAVY, c.stream(), w.C, w.F, cs, w.val_b, w.bstart, w.bend, w.order + first,
+                           (unsigned)nblk, sg, bn);
+        hipLaunchKernelGGL(k_spread2d_bin, dim3((unsigned)nblk), dim3(64), sh, c.stream(), w.C, w.F, cs, w.val_b,
+                           w.bstart, w.bend, w.order + first, sg, bn);
+        LC_HIP(hipGetLastError());
+        first += nblk;
+    }
 }
 
 // type 1: C (ns x nang x nrad complex) -> vol = Re(T1 C) / nrad
@@ -651,10 +961,14 @@ void type1(Ctx &c, const Geom2D &g, const NuWork &w, float *vol, int ns) {
     const Nufft2D &u = g.nu;
     const int N = (int)g.N, nf = u.nf;
     const SpreadGeom sg = spread_geom(g);
-    const size_t shs = sizeof(float2) * NU_WAVES * NU_T * NU_T + sizeof(float) * NU_WAVES * 64 * 2 * NU_W;
-    hipLaunchKernelGGL(k_spread2d, dim3(sg.ntx * sg.ntx, ns), dim3(64 * NU_WAVES), shs, c.stream(),
-                       w.C, w.F, g.nu.cs.as<double2>(), sg);
-    LC_HIP(hipGetLastError());
+    if (w.key_a) {
+        spread2_binned(c, g, w, ns);
+    } else {
+        const size_t shs = sizeof(float2) * NU_WAVES * NU_T * NU_T + sizeof(float) * NU_WAVES * 64 * 2 * NU_W;
+        hipLaunchKernelGGL(k_spread2d, dim3(sg.ntx * sg.ntx, ns), dim3(64 * NU_WAVES), shs, c.stream(),
+                           w.C, w.F, g.nu.cs.as<double2>(), sg);
+        LC_HIP(hipGetLastError());
+    }
     const float2 *tw = c.twiddles<float>(nf);
     const int nl = std::min(NU_NT * NU_ME / nf, nf);
     const size_t shr = sizeof(float2) * lds_round(nl * nf);
@@ -722,7 +1036,7 @@ void nu_check(Ctx &c, Geom2D &g, int ns) {
 // backproj2d: sino (ns x nang x nrad real, device) -> vol (ns x N x N real, device)
 void backproj2d_dev(Ctx &c, Geom2D &g, const float *sino, float *vol, int ns) {
     nu_check(c, g, ns);
-    const NuWork w = nu_work(c, g, ns);
+    const NuWork w = nu_work(c, g, ns, true);
     det_transform(c, g, -1, sino, nullptr, w.C, nullptr, ns);
     type1(c, g, w, vol, ns);
 }
@@ -738,9 +1052,15 @@ void forward2d_dev(Ctx &c, Geom2D &g, const float *vol, float *sino, int ns) {
 // sysmat2d(x, grid) = Re(T1(T2 x)) / nrad (src/gradient.cpp:59-69)
 void sysmat2d_nufft_dev(Ctx &c, Geom2D &g, const float *x, float *y, int ns) {
     nu_check(c, g, ns);
-    const NuWork w = nu_work(c, g, ns);
+    const NuWork w = nu_work(c, g, ns, true);
     type2(c, g, w, x, ns);
     type1(c, g, w, y, ns);
+}
+
+// device bytes of a 2-D back-projection's binned spread beyond C, F and V
+// (0 when the stack spreads by tile gather)
+size_t nufft2d_spread_bytes(int64_t ns, int64_t npts, int nf) {
+    return spread_binned(ns, npts, nf) ? spread2_extra_bytes(ns, npts, nf) : 0;
 }
 
 } // namespace lc

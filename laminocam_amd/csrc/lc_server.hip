@@ -65,7 +65,8 @@ size_t request_bytes(const ServerRequest &r) {
         H = g.H;
         if (r.has_sino) {
             const int64_t nf = g.nu.ready ? g.nu.nf : nufft2d_fine_n(g.N);
-            bp = 4 * ns * g.nang * g.nrad + 8 * ns * (g.nang * g.nrad + nf * nf + nf * N);
+            bp = 4 * ns * g.nang * g.nrad + 8 * ns * (g.nang * g.nrad + nf * nf + nf * N) +
+                 (int64_t)nufft2d_spread_bytes(ns, g.nang * g.nrad, (int)nf);
         }
     }
     const int64_t n = ns * N * N;
