@@ -182,10 +182,11 @@ __global__ void k_phase_shift3d(float2 *__restrict__ C, const double *__restrict
 // footprint cell, ceil(g - W3/2) per axis, taken modulo nf).  k_bin_keys
 // gives every point its bin; a stable radix sort (rocPRIM) orders the points
 // by bin, keeping point order inside a bin; k_bin_ranges marks each bin's
-// run.  A workgroup then owns one bin: its waves (SB3_WAVES) take consecutive
-// parts of the bin's points in order and spread each one's whole 8 x 8 x 8
-// footprint (lanes over 8 x 8 in x, y; 8 steps in z) into the wave's private
-// LDS copy of the bin's padded region (bin + W3 - 1 cells per axis: 11 x 11 x
+// run.  A wave then owns one bin (a bin of more than B3_HEAVY points gets
+// NW3_HEAVY waves, each taking a consecutive part of the bin's points): it
+// takes the points in order and spreads each one's whole 8 x 8 x 8 footprint
+// (lanes over 8 x 8 in x, y; 8 steps in z) into the wave's private LDS copy
+// of the bin's padded region (bin + W3 - 1 cells per axis: 11 x 11 x
 // 15 cells, 15.5 KiB, so a CU holds ten bins).  Every point is spread once,
 // with all 64 lanes inside its footprint, and no point is tested against a
 // bin it does not belong to.  The copies are summed in a fixed order and
@@ -212,7 +213,6 @@ constexpr int NPH = PZ * PY * PX;           // launch phases
 // bits of the launch-order key: the phase above a 26-bit descending count
 constexpr int ORDER_BITS = 26 + (NPH > 16 ? 5 : NPH > 8 ? 4 : NPH > 4 ? 3 : NPH > 2 ? 2 : 1);
 static_assert(ORDER_BITS <= 32, "launch-order key too wide");
-constexpr int SB3_WAVES = 1;   // waves per bin (1, 2 and 4 measured: 1 is fastest)
 // padded cell (z, y, x) -> LDS index; x XOR-ed with 8 on every other row
 // pair so the 8 rows x 8 columns of one z step fall on distinct bank pairs
 __device__ __forceinline__ int ptile_idx(int z, int y, int x) {
@@ -292,17 +292,22 @@ __global__ void k_bin_order_keys(const unsigned *__restrict__ bstart, const unsi
     val[b] = b;
 }
 
-__global__ void __launch_bounds__(64 * SB3_WAVES)
-k_spread3d_bin(const float2 *__restrict__ C, float2 *__restrict__ Fp, const double *__restrict__ rot,
-               const unsigned *__restrict__ sval, const unsigned *__restrict__ bstart,
-               const unsigned *__restrict__ bend, const unsigned *__restrict__ order, Grid3 g, Bins3 bn) {
+// Bins of more than B3_HEAVY points are spread by NW3_HEAVY waves, each
+// taking a consecutive part of the bin's points into its own copy of the
+// region; the copies are summed in wave order.  Whether a bin is heavy
+// depends only on its own count.
+constexpr unsigned B3_HEAVY = 2048;
+constexpr int NW3_HEAVY = 4;   // 4 region copies: 62 KiB of LDS
+
+// spread bin bid's points with the block's NW waves into their region copies
+// (reg: NW x RZ x RY x PXS) and add the copies' fixed-order sum into Fp
+template <int NW>
+__device__ __forceinline__ void spread3_bin(const float2 *__restrict__ C, float2 *__restrict__ Fp,
+                                            const double *__restrict__ rot, const unsigned *__restrict__ sval,
+                                            int bid, unsigned s0, unsigned s1, const Grid3 &g, const Bins3 &bn,
+                                            float2 *acc) {
     constexpr int PC = RZ * RY * PXS;
-    // this phase's bins, fullest first
-    const int bid = (int)order[blockIdx.x];
     const int b[3] = {bid / (bn.nb[1] * bn.nb[2]), (bid / bn.nb[2]) % bn.nb[1], bid % bn.nb[2]};
-    const unsigned s0 = bstart[bid], s1 = bend[bid];
-    if (s0 >= s1) return;   // an empty bin adds nothing
-    float2 *acc = reinterpret_cast<float2 *>(lc_smem);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     float2 *my = acc + wave * PC;
     for (int i = lane; i < PC; i += 64) my[i] = make_float2(0.f, 0.f);
@@ -310,7 +315,7 @@ k_spread3d_bin(const float2 *__restrict__ C, float2 *__restrict__ Fp, const doub
     const float ihw = 1.0f / (0.5f * W3);
     const int dxl = lane & 7, dyl = lane >> 3;
     // wave w takes the w-th part of the bin's points, in order
-    const unsigned per = (s1 - s0 + SB3_WAVES - 1) / SB3_WAVES;
+    const unsigned per = (s1 - s0 + NW - 1) / NW;
     const unsigned ws = s0 + wave * per, we = min(s1, ws + per);
     for (unsigned base = ws; base < we; base += 64) {
         const unsigned i = base + lane;
@@ -356,19 +361,46 @@ k_spread3d_bin(const float2 *__restrict__ C, float2 *__restrict__ Fp, const doub
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < RZ * RY * RX; i += 64 * SB3_WAVES) {
+    for (int i = threadIdx.x; i < RZ * RY * RX; i += 64 * NW) {
         const int z = i / (RY * RX), y = (i / RX) % RY, x = i % RX;
         const int zz = c0[0] + z, yy = c0[1] + y, xx = c0[2] + x;
         if (zz >= bn.pf[0] || yy >= bn.pf[1] || xx >= bn.pf[2]) continue;
         const int li = ptile_idx(z, y, x);
         float2 s = acc[li];
-        for (int w = 1; w < SB3_WAVES; ++w) {
+        for (int w = 1; w < NW; ++w) {
             s.x += acc[w * PC + li].x;
             s.y += acc[w * PC + li].y;
         }
         float2 &f = Fp[((int64_t)zz * bn.pf[1] + yy) * bn.pf[2] + xx];
         f.x += s.x;
         f.y += s.y;
+    }
+}
+
+// one wave per light bin of this phase (order: the phase's bins, fullest first)
+__global__ void __launch_bounds__(64)
+k_spread3d_bin(const float2 *__restrict__ C, float2 *__restrict__ Fp, const double *__restrict__ rot,
+               const unsigned *__restrict__ sval, const unsigned *__restrict__ bstart,
+               const unsigned *__restrict__ bend, const unsigned *__restrict__ order, Grid3 g, Bins3 bn) {
+    const int bid = (int)order[blockIdx.x];
+    const unsigned s0 = bstart[bid], s1 = bend[bid];
+    if (s0 >= s1 || s1 - s0 > B3_HEAVY) return;   // empty, or a heavy bin
+    spread3_bin<1>(C, Fp, rot, sval, bid, s0, s1, g, bn, reinterpret_cast<float2 *>(lc_smem));
+}
+
+// the phase's heavy bins (a prefix of its order): each block takes bins
+// j, j + gridDim.x, ... until the first light one
+__global__ void __launch_bounds__(64 * NW3_HEAVY)
+k_spread3d_bin_heavy(const float2 *__restrict__ C, float2 *__restrict__ Fp, const double *__restrict__ rot,
+                     const unsigned *__restrict__ sval, const unsigned *__restrict__ bstart,
+                     const unsigned *__restrict__ bend, const unsigned *__restrict__ order, unsigned nphase,
+                     Grid3 g, Bins3 bn) {
+    for (unsigned j = blockIdx.x; j < nphase; j += gridDim.x) {
+        const int bid = (int)order[j];
+        const unsigned s0 = bstart[bid], s1 = bend[bid];
+        if (s1 - s0 <= B3_HEAVY) break;   // the order is fullest first: the rest are light
+        spread3_bin<NW3_HEAVY>(C, Fp, rot, sval, bid, s0, s1, g, bn, reinterpret_cast<float2 *>(lc_smem));
+        __syncthreads();   // the copies are cleared for the next bin only after every wave has flushed
     }
 }
 
@@ -983,7 +1015,7 @@ void type1_3d(Ctx &c, const Geom3D &g, const Work3 &w, float *vol, bool mask, co
                                      ORDER_BITS, c.stream()));
     hipLaunchKernelGGL(k_zero2, dim3(2048), dim3(256), 0, c.stream(), w.Fp, npad);
     LC_HIP(hipGetLastError());
-    const size_t shs = sizeof(float2) * SB3_WAVES * RZ * RY * PXS;
+    const size_t shs = sizeof(float2) * RZ * RY * PXS;
     int64_t first = 0;   // the phase's first entry in the launch order
     for (int ph = 0; ph < NPH; ++ph) {
         // phase (rz, ry, rx): the bins whose index along each axis is r mod P
@@ -991,7 +1023,10 @@ void type1_3d(Ctx &c, const Geom3D &g, const Work3 &w, float *vol, bool mask, co
         int nblk = 1;
         for (int a = 0; a < 3; ++a) nblk *= std::max(0, (bn.nb[a] - r[a] + per[a] - 1) / per[a]);
         if (nblk == 0) continue;
-        hipLaunchKernelGGL(k_spread3d_bin, dim3(nblk), dim3(64 * SB3_WAVES), shs, c.stream(), w.C, w.Fp,
+        hipLaunchKernelGGL(k_spread3d_bin_heavy, dim3(std::min(nblk, 256)), dim3(64 * NW3_HEAVY), shs * NW3_HEAVY,
+                           c.stream(), w.C, w.Fp, g.d_rot.as<double>(), w.val_b, w.bstart, w.bend, w.order + first,
+                           (unsigned)nblk, gg, bn);
+        hipLaunchKernelGGL(k_spread3d_bin, dim3(nblk), dim3(64), shs, c.stream(), w.C, w.Fp,
                            g.d_rot.as<double>(), w.val_b, w.bstart, w.bend, w.order + first, gg, bn);
         LC_HIP(hipGetLastError());
         first += nblk;
