@@ -273,12 +273,17 @@ __device__ __forceinline__ bool interior(const Idx3 &q, int s, int a, int b) {
     return s >= 1 && s + 1 < q.n1 && a >= 1 && a + 1 < q.n2 && b >= 1 && b + 1 < q.n3;
 }
 
-__device__ __forceinline__ bool inside(const Idx3 &q, int s, int a, int b) {
-    return s >= 0 && s < q.n1 && a >= 0 && a < q.n2 && b >= 0 && b < q.n3;
+// a point that differs from q's own element only along axis c (the element
+// itself is inside) is inside when its coordinate v along c is in [0, n_c)
+__device__ __forceinline__ bool inside_c(const Idx3 &q, int c, int v) {
+    const int n = c == 0 ? q.n1 : c == 1 ? q.n2 : q.n3;
+    return v >= 0 && v < n;
 }
+__device__ __forceinline__ int coord(const Idx3 &q, int c) { return c == 0 ? q.s : c == 1 ? q.a : q.b; }
 
 // gradient component c (0 slices, 1 rows, 2 cols) at the element offset
-// (os, oa, ob) from q's; u reads relative to q's element
+// (os, oa, ob) from q's, an offset along axis c or none; u reads relative to
+// q's element
 template <typename R, typename A>
 __device__ __forceinline__ R grad_c(const A &u, const Idx3 &q, int c, int os, int oa, int ob,
                                     int boundary) {
@@ -288,8 +293,10 @@ __device__ __forceinline__ R grad_c(const A &u, const Idx3 &q, int c, int os, in
         if (!interior(q, s, a, b)) return (R)0;
         return (u(os + ds, oa + da, ob + db) - u(os - ds, oa - da, ob - db)) / (R)2;
     }
-    const R up = inside(q, s + ds, a + da, b + db) ? u(os + ds, oa + da, ob + db) : (R)0;
-    const R um = inside(q, s - ds, a - da, b - db) ? u(os - ds, oa - da, ob - db) : (R)0;
+    // both neighbours differ from q's element along axis c only
+    const int v = coord(q, c) + (c == 0 ? os : c == 1 ? oa : ob);
+    const R up = inside_c(q, c, v + 1) ? u(os + ds, oa + da, ob + db) : (R)0;
+    const R um = inside_c(q, c, v - 1) ? u(os - ds, oa - da, ob - db) : (R)0;
     return (R)0.5 * (up - um);
 }
 
@@ -303,12 +310,12 @@ __device__ __forceinline__ R div_gen(const G &g, const Idx3 &q, int boundary) {
         return (g(0, 1, 0, 0) - g(0, -1, 0, 0)) / (R)2 + (g(1, 0, 1, 0) - g(1, 0, -1, 0)) / (R)2 +
                (g(2, 0, 0, 1) - g(2, 0, 0, -1)) / (R)2;
     }
-    const R g0p = inside(q, s + 1, a, b) ? g(0, 1, 0, 0) : (R)0;
-    const R g0m = inside(q, s - 1, a, b) ? g(0, -1, 0, 0) : (R)0;
-    const R g1p = inside(q, s, a + 1, b) ? g(1, 0, 1, 0) : (R)0;
-    const R g1m = inside(q, s, a - 1, b) ? g(1, 0, -1, 0) : (R)0;
-    const R g2p = inside(q, s, a, b + 1) ? g(2, 0, 0, 1) : (R)0;
-    const R g2m = inside(q, s, a, b - 1) ? g(2, 0, 0, -1) : (R)0;
+    const R g0p = inside_c(q, 0, s + 1) ? g(0, 1, 0, 0) : (R)0;
+    const R g0m = inside_c(q, 0, s - 1) ? g(0, -1, 0, 0) : (R)0;
+    const R g1p = inside_c(q, 1, a + 1) ? g(1, 0, 1, 0) : (R)0;
+    const R g1m = inside_c(q, 1, a - 1) ? g(1, 0, -1, 0) : (R)0;
+    const R g2p = inside_c(q, 2, b + 1) ? g(2, 0, 0, 1) : (R)0;
+    const R g2m = inside_c(q, 2, b - 1) ? g(2, 0, 0, -1) : (R)0;
     const R h = (R)0.5;
     return h * (g0p - g0m) + h * (g1p - g1m) + h * (g2p - g2m);
 }
@@ -332,8 +339,8 @@ __device__ __forceinline__ R lap_at(const A &u, const Idx3 &q, int boundary) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const int ds = c == 0, da = c == 1, db = c == 2;
-        const R gp = inside(q, q.s + ds, q.a + da, q.b + db) ? grad_c<R>(u, q, c, ds, da, db, 1) : (R)0;
-        const R gm = inside(q, q.s - ds, q.a - da, q.b - db) ? grad_c<R>(u, q, c, -ds, -da, -db, 1) : (R)0;
+        const R gp = inside_c(q, c, coord(q, c) + 1) ? grad_c<R>(u, q, c, ds, da, db, 1) : (R)0;
+        const R gm = inside_c(q, c, coord(q, c) - 1) ? grad_c<R>(u, q, c, -ds, -da, -db, 1) : (R)0;
         acc += (R)0.5 * (gp - gm);
     }
     return acc;
@@ -379,8 +386,8 @@ k_sb_lapdot(const ChunkRef *__restrict__ ch, const ProbDev *__restrict__ pd,
     Idx3 q = idx_of(d, c.begin + threadIdx.x);
     for (int64_t i = c.begin + threadIdx.x; i < c.end; i += NT, step(q, NT)) {
         const R l = lap_at<R>(Rel<R>{p + i, nn, d.N}, q, d.boundary);
-        const R v = ap[i] - l * (R)d.mu;
-        ap[i] = v;
+        const R v = __builtin_nontemporal_load(ap + i) - l * (R)d.mu;
+        __builtin_nontemporal_store(v, ap + i);
         acc += (double)p[i] * v;
     }
     if (dot) {
@@ -409,7 +416,7 @@ k_sb_rhs(const ChunkRef *__restrict__ ch, const ProbDev *__restrict__ pd,
             return D[o] - Bv[o];
         };
         const R dv = div_gen<R>(dmb, q, d.boundary);
-        rhs[i] = y0[i] - dv * (R)d.mu;
+        __builtin_nontemporal_store(__builtin_nontemporal_load(y0 + i) - dv * (R)d.mu, rhs + i);
     }
 }
 
@@ -431,32 +438,35 @@ k_sb_shrink(const ChunkRef *__restrict__ ch, const ProbDev *__restrict__ pd,
     Idx3 q = idx_of(d, c.begin + threadIdx.x);
     for (int64_t i = c.begin + threadIdx.x; i < c.end; i += NT, step(q, NT)) {
         const Rel<R> xr{x + i, nn, d.N};
-        R g[3], v[3];
+        // streaming (nontemporal) accesses for the fields only this
+        // element touches; x keeps ordinary loads (its neighbours are reread)
+        R g[3], v[3], bk[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             g[k] = grad_c<R>(xr, q, k, 0, 0, 0, d.boundary);
-            v[k] = g[k] + Bv[k * total + i];
+            bk[k] = __builtin_nontemporal_load(Bv + k * total + i);
+            v[k] = g[k] + bk[k];
         }
         if (!aniso) {
             const R sk = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
             const R sh = fmax((R)0, sk - thr);
             for (int k = 0; k < 3; ++k) {
                 const R dk = sh * (v[k] / (sk + (R)1e-8));
-                D[k * total + i] = dk;
-                Bv[k * total + i] += g[k] - dk;
+                __builtin_nontemporal_store(dk, D + k * total + i);
+                __builtin_nontemporal_store(bk[k] + (g[k] - dk), Bv + k * total + i);
             }
         } else {
             for (int k = 0; k < 3; ++k) {
                 const R sh = fmax((R)0, fabs(v[k]) - thr);
                 const R dk = copysign(sh, v[k]);
-                D[k * total + i] = dk;
-                Bv[k * total + i] += g[k] - dk;
+                __builtin_nontemporal_store(dk, D + k * total + i);
+                __builtin_nontemporal_store(bk[k] + (g[k] - dk), Bv + k * total + i);
             }
         }
-        const R xi = x[i], xo = xold[i];
+        const R xi = x[i], xo = __builtin_nontemporal_load(xold + i);
         dn += (double)(xi - xo) * (double)(xi - xo);
         on += (double)xo * xo;
-        xold[i] = xi;
+        __builtin_nontemporal_store(xi, xold + i);
     }
     __shared__ double sh[NT / 64];
     const double a = block_sum<NT>(dn, sh);
