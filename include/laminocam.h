@@ -419,6 +419,20 @@ int lc_autotune2d(lc_ctx ctx, lc_geom2d g, int32_t nslices, int32_t *variant, do
 int lc_set_tuning2d(lc_ctx ctx, int64_t image_n, const int32_t *variant);
 int lc_get_tuning2d(lc_ctx ctx, int64_t image_n, int32_t *variant);
 
+/* ---- 3-D NUFFT spread bin shapes per grid (no reference counterpart) ----
+ * The type-1 spread of lc_backproj3d sorts the points into bins of footprint
+ * corners; the library is compiled for several bin shapes (shape 0, 4 x 4 x 8
+ * corner cells along z, y, x, is the default; 1 is 8 x 4 x 4, 2 is 8 x 8 x 8).
+ * lc_autotune3d times a back-projection of g's grid with each shape,
+ * installs the fastest in the context for g's fine-grid shape (the volume's
+ * dims, oversampled) and returns it, with the time of one back-projection in
+ * ms.  lc_set_tuning3d installs a known choice; lc_get_tuning3d reads the
+ * current one.  Every choice gives the same result up to the order of the
+ * spread's sums (it is deterministic for a given choice). */
+int lc_autotune3d(lc_ctx ctx, lc_geom3d g, int32_t *shape, double *ms);
+int lc_set_tuning3d(lc_ctx ctx, lc_geom3d g, int32_t shape);
+int lc_get_tuning3d(lc_ctx ctx, lc_geom3d g, int32_t *shape);
+
 /* ---- file formats (host only; no context or device needed) ----
  * TIFF stacks replace include/tiff.h:31-117 (tomocam::tiff::read / write over
  * libtiff).  Samples are single 32-bit IEEE floats; pages are slices.

@@ -90,6 +90,24 @@ class Context:
         _lib.check(self.lib.lc_get_tuning2d(self.handle, int(image_n), v))
         return tuple(v)
 
+    # ---- 3-D NUFFT spread bin shape per grid ----
+    def autotune3d(self, psf: "PointSpreadFunction"):
+        """Time the 3-D spread's compiled bin shapes on psf's grid (its
+        back-projection), install the fastest for that grid's fine-grid
+        shape; returns (shape index, ms per back-projection)."""
+        sh = C.c_int32()
+        ms = C.c_double()
+        _lib.check(self.lib.lc_autotune3d(self.handle, psf.handle, C.byref(sh), C.byref(ms)))
+        return sh.value, ms.value
+
+    def set_tuning3d(self, psf: "PointSpreadFunction", shape: int):
+        _lib.check(self.lib.lc_set_tuning3d(self.handle, psf.handle, int(shape)))
+
+    def get_tuning3d(self, psf: "PointSpreadFunction") -> int:
+        sh = C.c_int32()
+        _lib.check(self.lib.lc_get_tuning3d(self.handle, psf.handle, C.byref(sh)))
+        return sh.value
+
     def load_tuning(self, path=None):
         """Install a tuning table {"N": [row, col, B], ...} (JSON)."""
         import json

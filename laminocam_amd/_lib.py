@@ -36,7 +36,7 @@ EXPORTS = [
     "lc_server_create", "lc_server_submit", "lc_server_poll", "lc_server_wait",
     "lc_server_result", "lc_server_stats", "lc_server_timeline", "lc_server_destroy",
     "lc_sysmat3d_psf", "lc_forward3d", "lc_backproj3d", "lc_sysmat3d_nufft", "lc_cg3d", "lc_split_bregman3d", "lc_batch_create", "lc_autotune2d", "lc_set_tuning2d",
-    "lc_get_tuning2d",
+    "lc_get_tuning2d", "lc_autotune3d", "lc_set_tuning3d", "lc_get_tuning3d",
     "lc_sysmat2d_psf_f64", "lc_cg2d_f64", "lc_split_bregman2d_f64", "lc_sysmat3d_psf_f64",
     "lc_cg3d_f64", "lc_split_bregman3d_f64", "lc_grad_f64", "lc_divergence_f64",
     "lc_laplacian_f64", "lc_batch_create_ex", "lc_batch2d_set_f64", "lc_batch2d_get_f64",
@@ -159,6 +159,9 @@ def load():
         "lc_autotune2d": (C.c_int, [vp, vp, i32, C.POINTER(i32), C.POINTER(C.c_double)]),
         "lc_set_tuning2d": (C.c_int, [vp, i64, C.POINTER(i32)]),
         "lc_get_tuning2d": (C.c_int, [vp, i64, C.POINTER(i32)]),
+        "lc_autotune3d": (C.c_int, [vp, vp, C.POINTER(i32), C.POINTER(C.c_double)]),
+        "lc_set_tuning3d": (C.c_int, [vp, vp, i32]),
+        "lc_get_tuning3d": (C.c_int, [vp, vp, C.POINTER(i32)]),
         "lc_sysmat2d_psf_f64": (C.c_int, [vp, vp, vp, i64, vp, u32]),
         "lc_cg2d_f64": (C.c_int, [vp, vp, vp, vp, i64, i64, C.c_double, C.c_double, u32,
                                   C.POINTER(LcReport)]),

@@ -810,6 +810,37 @@ int lc_get_tuning2d(lc_ctx ctx, int64_t image_n, int32_t *variant) {
     });
 }
 
+int lc_autotune3d(lc_ctx ctx, lc_geom3d g, int32_t *shape, double *ms) {
+    return guard([&] {
+        LC_REQUIRE(ctx && g, "lc_autotune3d: bad argument");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        ctx->c.set_current();
+        double t = 0.0;
+        const int i = nufft3d_autotune(ctx->c, g->g, &t);
+        if (shape) *shape = i;
+        if (ms) *ms = t;
+    });
+}
+
+int lc_set_tuning3d(lc_ctx ctx, lc_geom3d g, int32_t shape) {
+    return guard([&] {
+        LC_REQUIRE(ctx && g, "lc_set_tuning3d: bad argument");
+        LC_REQUIRE(shape >= 0 && shape < nufft3d_num_shapes(), "lc_set_tuning3d: shape out of range");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        const int64_t n[3] = {g->g.n1, g->g.n2, g->g.n3};
+        ctx->c.nu3_tuning[{nufft3d_fine_n(n[0]), nufft3d_fine_n(n[1]), nufft3d_fine_n(n[2])}] = shape;
+    });
+}
+
+int lc_get_tuning3d(lc_ctx ctx, lc_geom3d g, int32_t *shape) {
+    return guard([&] {
+        LC_REQUIRE(ctx && g && shape, "lc_get_tuning3d: bad argument");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        const int nf[3] = {nufft3d_fine_n(g->g.n1), nufft3d_fine_n(g->g.n2), nufft3d_fine_n(g->g.n3)};
+        *shape = ctx->c.nu3_shape(nf);
+    });
+}
+
 int lc_batch2d_profile(lc_batch2d b, int32_t reps, int32_t max_rows, double *rows,
                        int32_t *nrows) {
     return guard([&] {

@@ -233,6 +233,10 @@ ToepVariant toep_default_variant(int M);
 int nufft2d_fine_n(int64_t N);
 int nufft3d_fine_n(int64_t n);
 size_t nufft3d_spread_bytes(int64_t npts, const int nf[3]);
+// the 3-D spread's compiled bin shapes, and the autotuner that picks one per
+// fine-grid shape (returns its index; installs it in c.nu3_tuning)
+int nufft3d_num_shapes();
+int nufft3d_autotune(Ctx &c, Geom3D &g, double *best_ms);
 size_t nufft2d_spread_bytes(int64_t ns, int64_t npts, int nf);
 // An odd image size N needs column blocks of at least two columns (the block
 // copies move pairs of complex values).  A column shape too small for 2 M

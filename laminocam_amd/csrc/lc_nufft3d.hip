@@ -200,24 +200,44 @@ __global__ void k_phase_shift3d(float2 *__restrict__ C, const double *__restrict
 // points the spread takes about 14 ms (8^3 bins in 8 phases: 16 ms), against
 // 53 ms for a tile-gather spread that searched every tile's candidate points.
 // ---------------------------------------------------------------------------
-// bin edges (corner cells) along z, y, x; 8 x 8 x 8, 4 x 8 x 8, 8 x 4 x 4,
-// 4 x 4 x 4 and 2 x 4 x 8 measured 3-10% slower on the 32 x 511^2 back-projection
-constexpr int BZ = 4, BY = 4, BX = 8;
-constexpr int RZ = BZ + W3 - 1, RY = BY + W3 - 1, RX = BX + W3 - 1;   // padded region
 constexpr int PXS = 16;                     // padded row stride (cells)
-static_assert(RX <= PXS, "a region row must fit the padded row stride");
 // bins P apart along an axis have disjoint padded regions when P b >= b + W3 - 1
 constexpr int phase_period(int b) { return (b + W3 - 1 + b - 1) / b; }
-constexpr int PZ = phase_period(BZ), PY = phase_period(BY), PX = phase_period(BX);
-constexpr int NPH = PZ * PY * PX;           // launch phases
-// bits of the launch-order key: the phase above a 26-bit descending count
-constexpr int ORDER_BITS = 26 + (NPH > 16 ? 5 : NPH > 8 ? 4 : NPH > 4 ? 3 : NPH > 2 ? 2 : 1);
-static_assert(ORDER_BITS <= 32, "launch-order key too wide");
-// padded cell (z, y, x) -> LDS index; x XOR-ed with 8 on every other row
-// pair so the 8 rows x 8 columns of one z step fall on distinct bank pairs
-__device__ __forceinline__ int ptile_idx(int z, int y, int x) {
-    return (z * RY + y) * PXS + (x ^ (((y >> 1) & 1) << 3));
+
+// A bin shape (edges in corner cells along z, y, x) the spread is compiled
+// for: its padded region, phase periods and launch phases.  Region cell
+// (z, y, x) -> LDS index with x XOR-ed with 8 on every other row pair, so the
+// 8 rows x 8 columns of one z step fall on distinct bank pairs.
+template <int BZ_, int BY_, int BX_> struct Shape3 {
+    static constexpr int BZ = BZ_, BY = BY_, BX = BX_;
+    static constexpr int RZ = BZ + W3 - 1, RY = BY + W3 - 1, RX = BX + W3 - 1;
+    static constexpr int PZ = phase_period(BZ), PY = phase_period(BY), PX = phase_period(BX);
+    static constexpr int NPH = PZ * PY * PX;   // launch phases
+    // bits of the launch-order key: the phase above a 26-bit descending count
+    static constexpr int ORDER_BITS = 26 + (NPH > 16 ? 5 : NPH > 8 ? 4 : NPH > 4 ? 3 : NPH > 2 ? 2 : 1);
+    static constexpr int PC = RZ * RY * PXS;    // cells of one region copy
+    // waves of a heavy bin: as many region copies as fit 64 KiB
+    static constexpr int NWH = 65536 / (8 * PC) < 1 ? 1 : 65536 / (8 * PC);
+    static_assert(RX <= PXS, "a region row must fit the padded row stride");
+    static_assert(ORDER_BITS <= 32, "launch-order key too wide");
+    __device__ __forceinline__ static int tidx(int z, int y, int x) {
+        return (z * RY + y) * PXS + (x ^ (((y >> 1) & 1) << 3));
+    }
+};
+// The shapes the autotuner (spread3d_autotune) chooses from, per fine-grid
+// shape; shape 0 is the default.  On the 32 x 511^2 back-projection
+// (720 x 128 x 511 points) the whole call measured 30.9 ms with 4 x 4 x 8
+// bins, 31.2 with 8 x 4 x 4, 31.8 with 4 x 4 x 4, 32.8 with 2 x 4 x 8 and 33.0
+// with 8 x 8 x 8.
+constexpr int N_SHAPES3 = 3;
+template <typename F> void with_shape3(int i, F &&f) {
+    switch (i) {
+    case 1: f(Shape3<8, 4, 4>{}); break;
+    case 2: f(Shape3<8, 8, 8>{}); break;
+    default: f(Shape3<4, 4, 8>{}); break;
+    }
 }
+constexpr int MAX_ORDER_BITS = 31;
 
 struct Bins3 {
     int nb[3];          // bins per axis, ceil(nf / bin edge)
@@ -247,6 +267,7 @@ __device__ __forceinline__ Corner3 corner3(const double *rot, int p, int j, int 
 
 // key[i]: the bin of point i's wrapped corner, or nbins (sorted last) for a
 // point the mask drops; val[i] = i
+template <typename S>
 __global__ void k_bin_keys(const double *__restrict__ rot, Grid3 g, Bins3 bn, unsigned *__restrict__ key,
                            unsigned *__restrict__ val, int64_t npts, unsigned nbins) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -257,7 +278,8 @@ __global__ void k_bin_keys(const double *__restrict__ rot, Grid3 g, Bins3 bn, un
     const Corner3 c = corner3(rot, p, j, k, g);
     unsigned b = nbins;
     if (c.ok) {
-        const int bz = wrapi(c.lz, g.nf[0]) / BZ, by = wrapi(c.ly, g.nf[1]) / BY, bx = wrapi(c.lx, g.nf[2]) / BX;
+        const int bz = wrapi(c.lz, g.nf[0]) / S::BZ, by = wrapi(c.ly, g.nf[1]) / S::BY,
+                  bx = wrapi(c.lx, g.nf[2]) / S::BX;
         b = (unsigned)(((int64_t)bz * bn.nb[1] + by) * bn.nb[2] + bx);
     }
     key[i] = b;
@@ -280,38 +302,38 @@ __global__ void k_bin_ranges(const unsigned *__restrict__ skey, int64_t n, unsig
 // phase periods), fullest first inside a phase, so a phase's long-running
 // dense bins start first and its tail is short.  key = phase << 26 |
 // (2^26 - 1 - count), val = bin
+template <typename S>
 __global__ void k_bin_order_keys(const unsigned *__restrict__ bstart, const unsigned *__restrict__ bend,
                                  Bins3 bn, unsigned *__restrict__ key, unsigned *__restrict__ val,
                                  unsigned nbins) {
     const unsigned b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbins) return;
     const unsigned bx = b % bn.nb[2], by = (b / bn.nb[2]) % bn.nb[1], bz = b / (bn.nb[2] * bn.nb[1]);
-    const unsigned ph = ((bz % PZ) * PY + by % PY) * PX + bx % PX;
+    const unsigned ph = ((bz % S::PZ) * S::PY + by % S::PY) * S::PX + bx % S::PX;
     const unsigned cnt = min(bend[b] - bstart[b], (1u << 26) - 1);
     key[b] = (ph << 26) | ((1u << 26) - 1 - cnt);
     val[b] = b;
 }
 
-// Bins of more than B3_HEAVY points are spread by NW3_HEAVY waves, each
-// taking a consecutive part of the bin's points into its own copy of the
-// region; the copies are summed in wave order.  Whether a bin is heavy
-// depends only on its own count.
+// Bins of more than B3_HEAVY points are spread by S::NWH waves, each taking a
+// consecutive part of the bin's points into its own copy of the region; the
+// copies are summed in wave order.  Whether a bin is heavy depends only on
+// its own count.
 constexpr unsigned B3_HEAVY = 2048;
-constexpr int NW3_HEAVY = 4;   // 4 region copies: 62 KiB of LDS
 
 // spread bin bid's points with the block's NW waves into their region copies
 // (reg: NW x RZ x RY x PXS) and add the copies' fixed-order sum into Fp
-template <int NW>
+template <typename S, int NW>
 __device__ __forceinline__ void spread3_bin(const float2 *__restrict__ C, float2 *__restrict__ Fp,
                                             const double *__restrict__ rot, const unsigned *__restrict__ sval,
                                             int bid, unsigned s0, unsigned s1, const Grid3 &g, const Bins3 &bn,
                                             float2 *acc) {
-    constexpr int PC = RZ * RY * PXS;
+    constexpr int PC = S::PC, RY = S::RY, RZ = S::RZ, RX = S::RX;
     const int b[3] = {bid / (bn.nb[1] * bn.nb[2]), (bid / bn.nb[2]) % bn.nb[1], bid % bn.nb[2]};
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     float2 *my = acc + wave * PC;
     for (int i = lane; i < PC; i += 64) my[i] = make_float2(0.f, 0.f);
-    const int c0[3] = {BZ * b[0], BY * b[1], BX * b[2]};
+    const int c0[3] = {S::BZ * b[0], S::BY * b[1], S::BX * b[2]};
     const float ihw = 1.0f / (0.5f * W3);
     const int dxl = lane & 7, dyl = lane >> 3;
     // wave w takes the w-th part of the bin's points, in order
@@ -350,7 +372,7 @@ __device__ __forceinline__ void spread3_bin(const float2 *__restrict__ C, float2
                               es3(((float)dyl + rl_f32(fy, s2)) * ihw, g.beta);
             // lane dz holds wz[dz]
             const float wzl = es3(((float)dxl + rl_f32(fz, s2)) * ihw, g.beta);
-            const int base_idx = ptile_idx(pz0, py0 + dyl, px0 + dxl);
+            const int base_idx = S::tidx(pz0, py0 + dyl, px0 + dxl);
 #pragma unroll
             for (int dz = 0; dz < W3; ++dz) {
                 const float wgt = wxy * rl_f32(wzl, dz);
@@ -365,7 +387,7 @@ __device__ __forceinline__ void spread3_bin(const float2 *__restrict__ C, float2
         const int z = i / (RY * RX), y = (i / RX) % RY, x = i % RX;
         const int zz = c0[0] + z, yy = c0[1] + y, xx = c0[2] + x;
         if (zz >= bn.pf[0] || yy >= bn.pf[1] || xx >= bn.pf[2]) continue;
-        const int li = ptile_idx(z, y, x);
+        const int li = S::tidx(z, y, x);
         float2 s = acc[li];
         for (int w = 1; w < NW; ++w) {
             s.x += acc[w * PC + li].x;
@@ -378,6 +400,7 @@ __device__ __forceinline__ void spread3_bin(const float2 *__restrict__ C, float2
 }
 
 // one wave per light bin of this phase (order: the phase's bins, fullest first)
+template <typename S>
 __global__ void __launch_bounds__(64)
 k_spread3d_bin(const float2 *__restrict__ C, float2 *__restrict__ Fp, const double *__restrict__ rot,
                const unsigned *__restrict__ sval, const unsigned *__restrict__ bstart,
@@ -385,12 +408,13 @@ k_spread3d_bin(const float2 *__restrict__ C, float2 *__restrict__ Fp, const doub
     const int bid = (int)order[blockIdx.x];
     const unsigned s0 = bstart[bid], s1 = bend[bid];
     if (s0 >= s1 || s1 - s0 > B3_HEAVY) return;   // empty, or a heavy bin
-    spread3_bin<1>(C, Fp, rot, sval, bid, s0, s1, g, bn, reinterpret_cast<float2 *>(lc_smem));
+    spread3_bin<S, 1>(C, Fp, rot, sval, bid, s0, s1, g, bn, reinterpret_cast<float2 *>(lc_smem));
 }
 
 // the phase's heavy bins (a prefix of its order): each block takes bins
 // j, j + gridDim.x, ... until the first light one
-__global__ void __launch_bounds__(64 * NW3_HEAVY)
+template <typename S>
+__global__ void __launch_bounds__(64 * S::NWH)
 k_spread3d_bin_heavy(const float2 *__restrict__ C, float2 *__restrict__ Fp, const double *__restrict__ rot,
                      const unsigned *__restrict__ sval, const unsigned *__restrict__ bstart,
                      const unsigned *__restrict__ bend, const unsigned *__restrict__ order, unsigned nphase,
@@ -399,7 +423,7 @@ k_spread3d_bin_heavy(const float2 *__restrict__ C, float2 *__restrict__ Fp, cons
         const int bid = (int)order[j];
         const unsigned s0 = bstart[bid], s1 = bend[bid];
         if (s1 - s0 <= B3_HEAVY) break;   // the order is fullest first: the rest are light
-        spread3_bin<NW3_HEAVY>(C, Fp, rot, sval, bid, s0, s1, g, bn, reinterpret_cast<float2 *>(lc_smem));
+        spread3_bin<S, S::NWH>(C, Fp, rot, sval, bid, s0, s1, g, bn, reinterpret_cast<float2 *>(lc_smem));
         __syncthreads();   // the copies are cleared for the next bin only after every wave has flushed
     }
 }
@@ -798,7 +822,15 @@ struct Work3 {
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 int bins_of(int nf, int b) { return (nf + b - 1) / b; }
-unsigned nbins_of(const int nf[3]) { return (unsigned)bins_of(nf[0], BZ) * bins_of(nf[1], BY) * bins_of(nf[2], BX); }
+template <typename S> unsigned nbins_of(const int nf[3]) {
+    return (unsigned)bins_of(nf[0], S::BZ) * bins_of(nf[1], S::BY) * bins_of(nf[2], S::BX);
+}
+// the most bins any compiled shape makes of this grid (work buffers fit every shape)
+unsigned nbins_max(const int nf[3]) {
+    unsigned m = 0;
+    for (int i = 0; i < N_SHAPES3; ++i) with_shape3(i, [&](auto sh) { m = std::max(m, nbins_of<decltype(sh)>(nf)); });
+    return m;
+}
 
 // bits of the sort key: bins 0 .. nbins - 1 and nbins for dropped points
 int key_bits(unsigned nbins) {
@@ -812,13 +844,13 @@ size_t sort_tmp_bytes(int64_t npts, unsigned nbins) {
     LC_HIP(rocprim::radix_sort_pairs(nullptr, tb, (unsigned *)nullptr, (unsigned *)nullptr, (unsigned *)nullptr,
                                      (unsigned *)nullptr, (size_t)npts, 0, key_bits(nbins), (hipStream_t)0));
     LC_HIP(rocprim::radix_sort_pairs(nullptr, to, (unsigned *)nullptr, (unsigned *)nullptr, (unsigned *)nullptr,
-                                     (unsigned *)nullptr, (size_t)nbins, 0, ORDER_BITS, (hipStream_t)0));
+                                     (unsigned *)nullptr, (size_t)nbins, 0, MAX_ORDER_BITS, (hipStream_t)0));
     return std::max(tb, to);
 }
 
 // bytes of the type-1 spread's buffers beyond the fine grid
 size_t spread_bytes(int64_t npts, const int nf[3]) {
-    const unsigned nbins = nbins_of(nf);
+    const unsigned nbins = nbins_max(nf);
     return align256(sizeof(float2) * (size_t)(nf[0] + W3 - 1) * (nf[1] + W3 - 1) * (nf[2] + W3 - 1)) +
            4 * align256(sizeof(unsigned) * (size_t)npts) + 6 * align256(sizeof(unsigned) * (size_t)nbins) +
            align256(sort_tmp_bytes(npts, nbins));
@@ -842,7 +874,7 @@ Work3 work3(Ctx &c, const Geom3D &g, bool spread = false) {
         return w;
     }
     LC_REQUIRE(npts < (int64_t)UINT32_MAX, "backproj: more than 2^32 detector points");
-    const unsigned nbins = nbins_of(u.nf);
+    const unsigned nbins = nbins_max(u.nf);
     const size_t bFp = align256(sizeof(float2) * (size_t)(u.nf[0] + W3 - 1) * (u.nf[1] + W3 - 1) * (u.nf[2] + W3 - 1));
     const size_t bK = align256(sizeof(unsigned) * (size_t)npts);
     const size_t bB = align256(sizeof(unsigned) * (size_t)nbins);
@@ -975,6 +1007,57 @@ void type2_3d(Ctx &c, const Geom3D &g, const Work3 &w, const float *vol, bool ma
     LC_HIP(hipGetLastError());
 }
 
+// the sorted-bin spread with bin shape S: C -> Fp (padded, not yet folded)
+template <typename S> Bins3 spread3d(Ctx &c, const Geom3D &g, const Work3 &w, const Grid3 &gg) {
+    const Nufft3D &u = g.nu;
+    Bins3 bn;
+    int64_t npad = 1;
+    const int edge[3] = {S::BZ, S::BY, S::BX};
+    for (int a = 0; a < 3; ++a) {
+        bn.nb[a] = bins_of(u.nf[a], edge[a]);
+        bn.pf[a] = u.nf[a] + W3 - 1;
+        npad *= bn.pf[a];
+    }
+    const unsigned nbins = (unsigned)bn.nb[0] * bn.nb[1] * bn.nb[2];
+    const int64_t npts = (int64_t)g.nproj * g.nrows * g.ncols;
+    // points ordered by bin, point order kept inside a bin
+    hipLaunchKernelGGL(k_bin_keys<S>, dim3((unsigned)ceil_div(npts, 256)), dim3(256), 0, c.stream(),
+                       g.d_rot.as<double>(), gg, bn, w.key_a, w.val_a, npts, nbins);
+    LC_HIP(hipGetLastError());
+    size_t tb = w.sort_bytes;
+    LC_HIP(rocprim::radix_sort_pairs(w.sort_tmp, tb, w.key_a, w.key_b, w.val_a, w.val_b, (size_t)npts, 0,
+                                     key_bits(nbins), c.stream()));
+    hipLaunchKernelGGL(k_zero_u32, dim3(1024), dim3(256), 0, c.stream(), w.bstart,
+                       (int64_t)(w.bend - w.bstart) + nbins);
+    hipLaunchKernelGGL(k_bin_ranges, dim3((unsigned)ceil_div(npts, 256)), dim3(256), 0, c.stream(), w.key_b,
+                       npts, w.bstart, w.bend, nbins);
+    hipLaunchKernelGGL(k_bin_order_keys<S>, dim3(ceil_div(nbins, 256)), dim3(256), 0, c.stream(), w.bstart,
+                       w.bend, bn, w.okey_a, w.oval_a, nbins);
+    LC_HIP(hipGetLastError());
+    tb = w.sort_bytes;
+    LC_HIP(rocprim::radix_sort_pairs(w.sort_tmp, tb, w.okey_a, w.okey_b, w.oval_a, w.order, (size_t)nbins, 0,
+                                     S::ORDER_BITS, c.stream()));
+    hipLaunchKernelGGL(k_zero2, dim3(2048), dim3(256), 0, c.stream(), w.Fp, npad);
+    LC_HIP(hipGetLastError());
+    const size_t shs = sizeof(float2) * S::PC;
+    int64_t first = 0;   // the phase's first entry in the launch order
+    for (int ph = 0; ph < S::NPH; ++ph) {
+        // phase (rz, ry, rx): the bins whose index along each axis is r mod P
+        const int r[3] = {ph / (S::PY * S::PX), (ph / S::PX) % S::PY, ph % S::PX}, per[3] = {S::PZ, S::PY, S::PX};
+        int nblk = 1;
+        for (int a = 0; a < 3; ++a) nblk *= std::max(0, (bn.nb[a] - r[a] + per[a] - 1) / per[a]);
+        if (nblk == 0) continue;
+        hipLaunchKernelGGL(k_spread3d_bin_heavy<S>, dim3(std::min(nblk, 256)), dim3(64 * S::NWH), shs * S::NWH,
+                           c.stream(), w.C, w.Fp, g.d_rot.as<double>(), w.val_b, w.bstart, w.bend, w.order + first,
+                           (unsigned)nblk, gg, bn);
+        hipLaunchKernelGGL(k_spread3d_bin<S>, dim3(nblk), dim3(64), shs, c.stream(), w.C, w.Fp,
+                           g.d_rot.as<double>(), w.val_b, w.bstart, w.bend, w.order + first, gg, bn);
+        LC_HIP(hipGetLastError());
+        first += nblk;
+    }
+    return bn;
+}
+
 void type1_3d(Ctx &c, const Geom3D &g, const Work3 &w, float *vol, bool mask, const double *shifts,
               float scale) {
     const Nufft3D &u = g.nu;
@@ -988,49 +1071,7 @@ void type1_3d(Ctx &c, const Geom3D &g, const Work3 &w, float *vol, bool mask, co
     }
     LC_REQUIRE(w.Fp != nullptr, "type1_3d: no spread buffers in the work area");
     Bins3 bn;
-    int64_t npad = 1;
-    for (int a = 0; a < 3; ++a) {
-        bn.nb[a] = bins_of(u.nf[a], a == 0 ? BZ : a == 1 ? BY : BX);
-        bn.pf[a] = u.nf[a] + W3 - 1;
-        npad *= bn.pf[a];
-    }
-    const unsigned nbins = (unsigned)bn.nb[0] * bn.nb[1] * bn.nb[2];
-    const int64_t npts = (int64_t)g.nproj * g.nrows * g.ncols;
-    // points ordered by bin, point order kept inside a bin
-    hipLaunchKernelGGL(k_bin_keys, dim3((unsigned)ceil_div(npts, 256)), dim3(256), 0, c.stream(),
-                       g.d_rot.as<double>(), gg, bn, w.key_a, w.val_a, npts, nbins);
-    LC_HIP(hipGetLastError());
-    size_t tb = w.sort_bytes;
-    LC_HIP(rocprim::radix_sort_pairs(w.sort_tmp, tb, w.key_a, w.key_b, w.val_a, w.val_b, (size_t)npts, 0,
-                                     key_bits(nbins), c.stream()));
-    hipLaunchKernelGGL(k_zero_u32, dim3(1024), dim3(256), 0, c.stream(), w.bstart,
-                       (int64_t)(w.bend - w.bstart) + nbins);
-    hipLaunchKernelGGL(k_bin_ranges, dim3((unsigned)ceil_div(npts, 256)), dim3(256), 0, c.stream(), w.key_b,
-                       npts, w.bstart, w.bend, nbins);
-    hipLaunchKernelGGL(k_bin_order_keys, dim3(ceil_div(nbins, 256)), dim3(256), 0, c.stream(), w.bstart, w.bend,
-                       bn, w.okey_a, w.oval_a, nbins);
-    LC_HIP(hipGetLastError());
-    tb = w.sort_bytes;
-    LC_HIP(rocprim::radix_sort_pairs(w.sort_tmp, tb, w.okey_a, w.okey_b, w.oval_a, w.order, (size_t)nbins, 0,
-                                     ORDER_BITS, c.stream()));
-    hipLaunchKernelGGL(k_zero2, dim3(2048), dim3(256), 0, c.stream(), w.Fp, npad);
-    LC_HIP(hipGetLastError());
-    const size_t shs = sizeof(float2) * RZ * RY * PXS;
-    int64_t first = 0;   // the phase's first entry in the launch order
-    for (int ph = 0; ph < NPH; ++ph) {
-        // phase (rz, ry, rx): the bins whose index along each axis is r mod P
-        const int r[3] = {ph / (PY * PX), (ph / PX) % PY, ph % PX}, per[3] = {PZ, PY, PX};
-        int nblk = 1;
-        for (int a = 0; a < 3; ++a) nblk *= std::max(0, (bn.nb[a] - r[a] + per[a] - 1) / per[a]);
-        if (nblk == 0) continue;
-        hipLaunchKernelGGL(k_spread3d_bin_heavy, dim3(std::min(nblk, 256)), dim3(64 * NW3_HEAVY), shs * NW3_HEAVY,
-                           c.stream(), w.C, w.Fp, g.d_rot.as<double>(), w.val_b, w.bstart, w.bend, w.order + first,
-                           (unsigned)nblk, gg, bn);
-        hipLaunchKernelGGL(k_spread3d_bin, dim3(nblk), dim3(64), shs, c.stream(), w.C, w.Fp,
-                           g.d_rot.as<double>(), w.val_b, w.bstart, w.bend, w.order + first, gg, bn);
-        LC_HIP(hipGetLastError());
-        first += nblk;
-    }
+    with_shape3(c.nu3_shape(u.nf), [&](auto sh) { bn = spread3d<decltype(sh)>(c, g, w, gg); });
     const int64_t nfine = (int64_t)u.nf[0] * u.nf[1] * u.nf[2];
     hipLaunchKernelGGL(k_fold3d, dim3((unsigned)std::min<int64_t>(8192, ceil_div(nfine, 256))), dim3(256), 0,
                        c.stream(), w.Fp, w.F, gg, bn);
@@ -1087,5 +1128,52 @@ void sysmat3d_nufft_dev(Ctx &c, Geom3D &g, const float *x, float *y) {
 // device bytes of a 3-D back-projection's spread beyond the fine grid (the
 // padded grid, the sort of the points by bin and its scratch)
 size_t nufft3d_spread_bytes(int64_t npts, const int nf[3]) { return spread_bytes(npts, nf); }
+
+int nufft3d_num_shapes() { return N_SHAPES3; }
+
+// Times the back-projection of g's grid with every compiled bin shape of the
+// spread (zero projections: the spread's work depends on where the points
+// fall, not on their values), installs the fastest in c.nu3_tuning for g's
+// fine-grid shape and returns its index (best_ms: one back-projection).
+int nufft3d_autotune(Ctx &c, Geom3D &g, double *best_ms) {
+    nu3_check(c, g);
+    const int64_t np = g.nproj * g.nrows * g.ncols, nv = g.n1 * g.n2 * g.n3;
+    DBuf proj(sizeof(float) * (size_t)np), vol(sizeof(float) * (size_t)nv);
+    LC_HIP(hipMemsetAsync(proj.as(), 0, sizeof(float) * (size_t)np, c.stream()));
+    const std::array<int, 3> key{g.nu.nf[0], g.nu.nf[1], g.nu.nf[2]};
+    const auto saved = c.nu3_tuning;
+    hipEvent_t e0, e1;
+    LC_HIP(hipEventCreate(&e0));
+    LC_HIP(hipEventCreate(&e1));
+    int best = 0;
+    float tbest = 1e30f;
+    try {
+        for (int i = 0; i < N_SHAPES3; ++i) {
+            c.nu3_tuning[key] = i;
+            backproj3d_dev(c, g, proj.as<float>(), vol.as<float>(), nullptr);   // warm-up
+            LC_HIP(hipEventRecord(e0, c.stream()));
+            for (int r = 0; r < 2; ++r) backproj3d_dev(c, g, proj.as<float>(), vol.as<float>(), nullptr);
+            LC_HIP(hipEventRecord(e1, c.stream()));
+            LC_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            LC_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms / 2 < tbest) {
+                tbest = ms / 2;
+                best = i;
+            }
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        c.nu3_tuning = saved;
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c.nu3_tuning = saved;
+    c.nu3_tuning[key] = best;
+    if (best_ms) *best_ms = tbest;
+    return best;
+}
 
 } // namespace lc

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <algorithm>
+#include <array>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -214,6 +215,13 @@ class Ctx {
   public:
     // per image size N: kernel shapes chosen by the autotuner (or installed)
     std::map<int, ToepVariant> toep_tuning;
+    // per 3-D fine-grid shape (nf0, nf1, nf2): the type-1 spread's bin shape
+    // (an index into the compiled shapes; 0 when none was chosen)
+    std::map<std::array<int, 3>, int> nu3_tuning;
+    int nu3_shape(const int nf[3]) const {
+        auto it = nu3_tuning.find({nf[0], nf[1], nf[2]});
+        return it == nu3_tuning.end() ? 0 : it->second;
+    }
     explicit Ctx(int device);
     ~Ctx();
     int device() const { return device_; }

@@ -227,6 +227,42 @@ def test_backproj3d_mid_size_matches_oracle(lc, orc64, dims, nproj, nrows, ncols
     assert np.array_equal(lc.backproj(proj, g, dims, sh), x)
 
 
+def test_backproj3d_bin_shapes_and_autotune(lc, orc64):
+    # every compiled bin shape of the spread (0: 4x4x8, 1: 8x4x4, 2: 8x8x8
+    # corner cells) gives the oracle's back-projection, deterministically;
+    # the autotuner installs one of them for the grid's fine-grid shape
+    dims, nproj, nrows, ncols = (16, 96, 96), 90, 32, 96
+    announce(f"backproj3d bin shapes dims={dims}")
+    rng = np.random.default_rng(77)
+    th, ga, be = lamino_angles(nproj, -30.0)
+    proj = rng.standard_normal((nproj, nrows, ncols))
+    g = lc.PolarGrid(th, ga, be, nrows, ncols)
+    sh = rng.uniform(-2, 2, (nproj, 2))
+    xr = orc64.backproj3d(proj, th, ga, be, dims, shifts=sh, tol=1e-9)
+    ctx = lc.Context(0)
+    try:
+        psf = lc.PointSpreadFunction(g, dims, ctx=ctx)
+        xs = []
+        for shape in range(3):
+            ctx.set_tuning3d(psf, shape)
+            assert ctx.get_tuning3d(psf) == shape
+            x = lc.backproj(proj, g, dims, sh, ctx=ctx)
+            err = rel_l2(x, xr)
+            print(f"shape {shape}: rel_l2 = {err:.3e}")
+            assert err < PROJ_TOL
+            assert np.array_equal(lc.backproj(proj, g, dims, sh, ctx=ctx), x)
+            xs.append(x)
+        for x in xs[1:]:
+            assert rel_l2(x, xs[0]) < 1e-5
+        best, ms = ctx.autotune3d(psf)
+        print(f"autotune3d: shape {best}, {ms:.3f} ms")
+        assert best in (0, 1, 2) and ms > 0.0
+        assert ctx.get_tuning3d(psf) == best
+        assert np.array_equal(lc.backproj(proj, g, dims, sh, ctx=ctx), xs[best])
+    finally:
+        ctx.close()
+
+
 def test_backproj3d_shift_count_mismatch(lc):
     # src/projection.cpp:105-108: std::invalid_argument
     th, ga, be = lamino_angles(4)
