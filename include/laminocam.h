@@ -430,6 +430,13 @@ int lc_get_tuning2d(lc_ctx ctx, int64_t image_n, int32_t *variant);
  * current one.  Every choice gives the same result up to the order of the
  * spread's sums (it is deterministic for a given choice). */
 int lc_autotune3d(lc_ctx ctx, lc_geom3d g, int32_t *shape, double *ms);
+/* The 2-D spread of lc_backproj2d likewise (no reference counterpart): bins of
+ * 16 x 16 (shape 0, the default) or 32 x 32 (shape 1) corner cells per fine-
+ * grid size, timed on nslices slices of g.  Fine grids the sorted-bin spread
+ * does not serve keep shape 0 and report 0 ms. */
+int lc_autotune2d_nufft(lc_ctx ctx, lc_geom2d g, int32_t nslices, int32_t *shape, double *ms);
+int lc_set_tuning2d_nufft(lc_ctx ctx, lc_geom2d g, int32_t shape);
+int lc_get_tuning2d_nufft(lc_ctx ctx, lc_geom2d g, int32_t *shape);
 int lc_set_tuning3d(lc_ctx ctx, lc_geom3d g, int32_t shape);
 int lc_get_tuning3d(lc_ctx ctx, lc_geom3d g, int32_t *shape);
 

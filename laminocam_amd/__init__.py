@@ -90,6 +90,25 @@ class Context:
         _lib.check(self.lib.lc_get_tuning2d(self.handle, int(image_n), v))
         return tuple(v)
 
+    # ---- 2-D NUFFT spread bin shape per fine-grid size ----
+    def autotune2d_nufft(self, psf: "PointSpreadFunction2D", nslices: int = 8):
+        """Time the 2-D spread's compiled bin shapes on nslices slices of
+        psf's grid, install the fastest for its fine-grid size; returns
+        (shape index, ms per back-projection)."""
+        sh = C.c_int32()
+        ms = C.c_double()
+        _lib.check(self.lib.lc_autotune2d_nufft(self.handle, psf.handle, int(nslices), C.byref(sh),
+                                                C.byref(ms)))
+        return sh.value, ms.value
+
+    def set_tuning2d_nufft(self, psf: "PointSpreadFunction2D", shape: int):
+        _lib.check(self.lib.lc_set_tuning2d_nufft(self.handle, psf.handle, int(shape)))
+
+    def get_tuning2d_nufft(self, psf: "PointSpreadFunction2D") -> int:
+        sh = C.c_int32()
+        _lib.check(self.lib.lc_get_tuning2d_nufft(self.handle, psf.handle, C.byref(sh)))
+        return sh.value
+
     # ---- 3-D NUFFT spread bin shape per grid ----
     def autotune3d(self, psf: "PointSpreadFunction"):
         """Time the 3-D spread's compiled bin shapes on psf's grid (its

@@ -37,6 +37,7 @@ EXPORTS = [
     "lc_server_result", "lc_server_stats", "lc_server_timeline", "lc_server_destroy",
     "lc_sysmat3d_psf", "lc_forward3d", "lc_backproj3d", "lc_sysmat3d_nufft", "lc_cg3d", "lc_split_bregman3d", "lc_batch_create", "lc_autotune2d", "lc_set_tuning2d",
     "lc_get_tuning2d", "lc_autotune3d", "lc_set_tuning3d", "lc_get_tuning3d",
+    "lc_autotune2d_nufft", "lc_set_tuning2d_nufft", "lc_get_tuning2d_nufft",
     "lc_sysmat2d_psf_f64", "lc_cg2d_f64", "lc_split_bregman2d_f64", "lc_sysmat3d_psf_f64",
     "lc_cg3d_f64", "lc_split_bregman3d_f64", "lc_grad_f64", "lc_divergence_f64",
     "lc_laplacian_f64", "lc_batch_create_ex", "lc_batch2d_set_f64", "lc_batch2d_get_f64",
@@ -160,6 +161,9 @@ def load():
         "lc_set_tuning2d": (C.c_int, [vp, i64, C.POINTER(i32)]),
         "lc_get_tuning2d": (C.c_int, [vp, i64, C.POINTER(i32)]),
         "lc_autotune3d": (C.c_int, [vp, vp, C.POINTER(i32), C.POINTER(C.c_double)]),
+        "lc_autotune2d_nufft": (C.c_int, [vp, vp, i32, C.POINTER(i32), C.POINTER(C.c_double)]),
+        "lc_set_tuning2d_nufft": (C.c_int, [vp, vp, i32]),
+        "lc_get_tuning2d_nufft": (C.c_int, [vp, vp, C.POINTER(i32)]),
         "lc_set_tuning3d": (C.c_int, [vp, vp, i32]),
         "lc_get_tuning3d": (C.c_int, [vp, vp, C.POINTER(i32)]),
         "lc_sysmat2d_psf_f64": (C.c_int, [vp, vp, vp, i64, vp, u32]),

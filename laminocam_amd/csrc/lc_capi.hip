@@ -810,6 +810,35 @@ int lc_get_tuning2d(lc_ctx ctx, int64_t image_n, int32_t *variant) {
     });
 }
 
+int lc_autotune2d_nufft(lc_ctx ctx, lc_geom2d g, int32_t nslices, int32_t *shape, double *ms) {
+    return guard([&] {
+        LC_REQUIRE(ctx && g && nslices >= 1 && nslices <= 65535, "lc_autotune2d_nufft: bad argument");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        ctx->c.set_current();
+        double t = 0.0;
+        const int i = nufft2d_autotune(ctx->c, g->g, nslices, &t);
+        if (shape) *shape = i;
+        if (ms) *ms = t;
+    });
+}
+
+int lc_set_tuning2d_nufft(lc_ctx ctx, lc_geom2d g, int32_t shape) {
+    return guard([&] {
+        LC_REQUIRE(ctx && g, "lc_set_tuning2d_nufft: bad argument");
+        LC_REQUIRE(shape >= 0 && shape < nufft2d_num_shapes(), "lc_set_tuning2d_nufft: shape out of range");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        ctx->c.nu2_tuning[nufft2d_fine_n(g->g.N)] = shape;
+    });
+}
+
+int lc_get_tuning2d_nufft(lc_ctx ctx, lc_geom2d g, int32_t *shape) {
+    return guard([&] {
+        LC_REQUIRE(ctx && g && shape, "lc_get_tuning2d_nufft: bad argument");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        *shape = ctx->c.nu2_shape(nufft2d_fine_n(g->g.N));
+    });
+}
+
 int lc_autotune3d(lc_ctx ctx, lc_geom3d g, int32_t *shape, double *ms) {
     return guard([&] {
         LC_REQUIRE(ctx && g, "lc_autotune3d: bad argument");

@@ -237,6 +237,9 @@ size_t nufft3d_spread_bytes(int64_t npts, const int nf[3]);
 // fine-grid shape (returns its index; installs it in c.nu3_tuning)
 int nufft3d_num_shapes();
 int nufft3d_autotune(Ctx &c, Geom3D &g, double *best_ms);
+// the same for the 2-D spread, per fine-grid size, timed on nslices slices
+int nufft2d_num_shapes();
+int nufft2d_autotune(Ctx &c, Geom2D &g, int nslices, double *best_ms);
 size_t nufft2d_spread_bytes(int64_t ns, int64_t npts, int nf);
 // An odd image size N needs column blocks of at least two columns (the block
 // copies move pairs of complex values).  A column shape too small for 2 M

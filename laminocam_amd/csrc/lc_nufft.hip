@@ -340,13 +340,28 @@ k_spread2d(const float2 *__restrict__ C, float2 *__restrict__ fine, const double
 // the other slices of the stack: the result is deterministic and a slice
 // spreads the same alone as in a stack.
 // ---------------------------------------------------------------------------
-constexpr int B2 = 16;                 // bin edge (corner cells)
-constexpr int R2 = B2 + NU_W - 1;      // padded region edge (23 cells)
-constexpr int R2S = 24;                // region row stride: rows y, y + 1 on opposite bank halves
+// A bin shape the spread is compiled for: bin edge B corner cells, padded
+// region edge R = B + 7, region row stride RS = B + 8 (rows y and y + 1 fall
+// on opposite bank halves), heavy-bin threshold (points) and waves (region
+// copies within 64 KiB).  The autotuner picks one per fine-grid size
+// (spread2d_autotune); shape 0 (B = 16) is the default.
+template <int B_> struct Shape2 {
+    static constexpr int B = B_, R = B + NU_W - 1, RS = B + NU_W;
+    static constexpr int RC = R * RS;                 // cells of one region copy
+    static constexpr unsigned HEAVY = 2u * B * B;      // 512 points for B = 16
+    static constexpr int NWH = 65536 / (8 * RC) > 12 ? 12 : 65536 / (8 * RC);
+    static_assert(RC % 2 == 0, "the region clears as float4");
+};
+constexpr int N_SHAPES2 = 2;
+constexpr int shape2_edge(int i) { return i == 1 ? 32 : 16; }
+template <typename F> void with_shape2(int i, F &&f) {
+    if (i == 1) f(Shape2<32>{});
+    else f(Shape2<16>{});
+}
 constexpr int B2_ORDER_BITS = 28;      // launch-order key: phase (2 bits) above a 26-bit count
 
 struct Bins2 {
-    int nb;             // bins per fine-grid edge (nf / B2)
+    int nb;             // bins per fine-grid edge (nf / B)
     unsigned nbins;     // bins per slice
     int npts;           // points per slice (nang * nrad)
 };
@@ -371,6 +386,7 @@ __device__ __forceinline__ int wrap_nf(int a, int n) {
 }
 
 // key[i] = slice * nbins + bin of point i of the stack, val[i] = i
+template <typename S>
 __global__ void k_bin2_keys(const double2 *__restrict__ cs, SpreadGeom g, Bins2 bn, unsigned *__restrict__ key,
                             unsigned *__restrict__ val, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -379,7 +395,7 @@ __global__ void k_bin2_keys(const double2 *__restrict__ cs, SpreadGeom g, Bins2 
     int lx0, ly0;
     float fx, fy;
     corner2(cs, g, p, lx0, ly0, fx, fy);
-    const int bx = wrap_nf(lx0, g.nf) / B2, by = wrap_nf(ly0, g.nf) / B2;
+    const int bx = wrap_nf(lx0, g.nf) / S::B, by = wrap_nf(ly0, g.nf) / S::B;
     key[i] = (unsigned)s * bn.nbins + (unsigned)(by * bn.nb + bx);
     val[i] = (unsigned)i;
 }
@@ -417,18 +433,16 @@ __global__ void k_zero_f2(float2 *p, int64_t n) {
         p[i] = make_float2(0.f, 0.f);
 }
 
-// Bins of more than B2_HEAVY points (near the k-space origin every angle
-// passes: ~8 samples per angle in the central bin) are spread by NW2_HEAVY
-// waves, each taking a consecutive part of the bin's points into its own copy
-// of the region; the copies are summed in wave order.  Whether a bin is heavy
-// depends only on its own count, so a slice's result is the same alone and in
-// a stack.
-constexpr unsigned B2_HEAVY = 512;
-constexpr int NW2_HEAVY = 12;   // 12 region copies: 53 KiB of LDS
+// Bins of more than S::HEAVY points (near the k-space origin every angle
+// passes: ~8 samples per angle in the central 16 x 16 bin) are spread by
+// S::NWH waves, each taking a consecutive part of the bin's points into its
+// own copy of the region; the copies are summed in wave order.  Whether a bin
+// is heavy depends only on its own count, so a slice's result is the same
+// alone and in a stack.
 
 // spread bin bid's points with the block's NW waves into their region copies
 // (reg: NW x R2 x R2S) and add the copies' fixed-order sum into the fine grid
-template <int NW>
+template <typename S, int NW>
 __device__ __forceinline__ void spread2_bin(const float2 *__restrict__ C, float2 *__restrict__ fine,
                                             const double2 *__restrict__ cs, const unsigned *__restrict__ sval,
                                             unsigned bid, unsigned s0, unsigned s1, const SpreadGeom &g,
@@ -436,10 +450,10 @@ __device__ __forceinline__ void spread2_bin(const float2 *__restrict__ C, float2
     const int slice = (int)(bid / bn.nbins);
     const int local = (int)(bid - (unsigned)slice * bn.nbins);
     const int by = local / bn.nb, bx = local - by * bn.nb;
+    constexpr int B2 = S::B, R2 = S::R, R2S = S::RS;
     const int c0y = by * B2, c0x = bx * B2;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    constexpr int RC = R2 * R2S;   // cells of one region copy
-    static_assert(RC % 2 == 0, "the region clears as float4");
+    constexpr int RC = S::RC;
     for (int i = threadIdx.x; i < NW * RC / 2; i += 64 * NW)
         reinterpret_cast<float4 *>(reg)[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     __syncthreads();
@@ -499,19 +513,21 @@ __device__ __forceinline__ void spread2_bin(const float2 *__restrict__ C, float2
 }
 
 // one wave per light bin of this phase (order: the phase's bins, fullest first)
+template <typename S>
 __global__ void __launch_bounds__(64)
 k_spread2d_bin(const float2 *__restrict__ C, float2 *__restrict__ fine, const double2 *__restrict__ cs,
                const unsigned *__restrict__ sval, const unsigned *__restrict__ bstart,
                const unsigned *__restrict__ bend, const unsigned *__restrict__ order, SpreadGeom g, Bins2 bn) {
     const unsigned bid = order[blockIdx.x];
     const unsigned s0 = bstart[bid], s1 = bend[bid];
-    if (s0 >= s1 || s1 - s0 > B2_HEAVY) return;   // empty, or a heavy bin
-    spread2_bin<1>(C, fine, cs, sval, bid, s0, s1, g, bn, reinterpret_cast<float2 *>(lc_smem));
+    if (s0 >= s1 || s1 - s0 > S::HEAVY) return;   // empty, or a heavy bin
+    spread2_bin<S, 1>(C, fine, cs, sval, bid, s0, s1, g, bn, reinterpret_cast<float2 *>(lc_smem));
 }
 
 // the phase's heavy bins (a prefix of its order): each block takes bins
 // j, j + gridDim.x, ... until the first light one
-__global__ void __launch_bounds__(64 * NW2_HEAVY)
+template <typename S>
+__global__ void __launch_bounds__(64 * S::NWH)
 k_spread2d_bin_heavy(const float2 *__restrict__ C, float2 *__restrict__ fine, const double2 *__restrict__ cs,
                      const unsigned *__restrict__ sval, const unsigned *__restrict__ bstart,
                      const unsigned *__restrict__ bend, const unsigned *__restrict__ order, unsigned nphase,
@@ -519,8 +535,8 @@ k_spread2d_bin_heavy(const float2 *__restrict__ C, float2 *__restrict__ fine, co
     for (unsigned j = blockIdx.x; j < nphase; j += gridDim.x) {
         const unsigned bid = order[j];
         const unsigned s0 = bstart[bid], s1 = bend[bid];
-        if (s1 - s0 <= B2_HEAVY) break;   // the order is fullest first: the rest are light
-        spread2_bin<NW2_HEAVY>(C, fine, cs, sval, bid, s0, s1, g, bn, reinterpret_cast<float2 *>(lc_smem));
+        if (s1 - s0 <= S::HEAVY) break;   // the order is fullest first: the rest are light
+        spread2_bin<S, S::NWH>(C, fine, cs, sval, bid, s0, s1, g, bn, reinterpret_cast<float2 *>(lc_smem));
         __syncthreads();   // the copies are cleared for the next bin only after every wave has flushed
     }
 }
@@ -856,12 +872,16 @@ int key_bits2(uint64_t total) {
     return b;
 }
 
-// the binned spread needs nf a multiple of 2 B2 (conflict-free wrapped
-// phases) and 32-bit point indices and keys; otherwise the tile gather
-bool spread_binned(int64_t ns, int64_t npts, int nf) {
-    const uint64_t nbins = (uint64_t)(nf / B2) * (nf / B2);
-    return nf % (2 * B2) == 0 && (uint64_t)(ns * npts) < (1ull << 32) && ns * nbins < (1ull << 32);
+// the binned spread with bins of edge b needs nf a multiple of 2 b
+// (conflict-free wrapped phases) and 32-bit point indices and keys;
+// otherwise the tile gather.  Buffers are sized for 16-cell bins (the most
+// bins of any shape), so every shape that fits nf shares them.
+bool shape2_fits(int64_t ns, int64_t npts, int nf, int b) {
+    const uint64_t nbins = (uint64_t)(nf / b) * (nf / b);
+    return nf % (2 * b) == 0 && (uint64_t)(ns * npts) < (1ull << 32) && ns * nbins < (1ull << 32);
 }
+bool spread_binned(int64_t ns, int64_t npts, int nf) { return shape2_fits(ns, npts, nf, shape2_edge(0)); }
+constexpr int B2 = 16;   // bin edge the buffers are sized for
 
 // the binned spread's buffers beyond C, F and V: 4 x 32-bit per point, 6 x
 // 32-bit per bin, and the larger of the two sorts' scratch
@@ -912,17 +932,17 @@ NuWork nu_work(Ctx &c, const Geom2D &g, int ns, bool spread = false) {
     return w;
 }
 
-// type-1 spread of the stack by sorted bins: C -> F (ns x nf x nf)
-void spread2_binned(Ctx &c, const Geom2D &g, const NuWork &w, int ns) {
+// type-1 spread of the stack by sorted bins of shape S: C -> F (ns x nf x nf)
+template <typename S> void spread2_binned(Ctx &c, const Geom2D &g, const NuWork &w, int ns) {
     const SpreadGeom sg = spread_geom(g);
     Bins2 bn;
-    bn.nb = sg.nf / B2;
+    bn.nb = sg.nf / S::B;
     bn.nbins = (unsigned)bn.nb * bn.nb;
     bn.npts = (int)(g.nang * g.nrad);
     const int64_t n = (int64_t)ns * bn.npts;
     const unsigned total = (unsigned)ns * bn.nbins;
     const double2 *cs = g.nu.cs.as<double2>();
-    hipLaunchKernelGGL(k_bin2_keys, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, c.stream(), cs, sg, bn,
+    hipLaunchKernelGGL(k_bin2_keys<S>, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, c.stream(), cs, sg, bn,
                        w.key_a, w.val_a, n);
     LC_HIP(hipGetLastError());
     size_t tb = w.sort_bytes;
@@ -940,17 +960,17 @@ void spread2_binned(Ctx &c, const Geom2D &g, const NuWork &w, int ns) {
                                      B2_ORDER_BITS, c.stream()));
     hipLaunchKernelGGL(k_zero_f2, dim3(2048), dim3(256), 0, c.stream(), w.F, (int64_t)ns * sg.nf * sg.nf);
     LC_HIP(hipGetLastError());
-    const size_t sh = sizeof(float2) * R2 * R2S;
+    const size_t sh = sizeof(float2) * S::RC;
     int64_t first = 0;   // the phase's first entry in the launch order
     for (int ph = 0; ph < 4; ++ph) {
         const int py = ph >> 1, px = ph & 1;
         const int64_t nblk = (int64_t)ns * ((bn.nb - py + 1) / 2) * ((bn.nb - px + 1) / 2);
         if (nblk == 0) continue;
-        hipLaunchKernelGGL(k_spread2d_bin_heavy, dim3((unsigned)std::min<int64_t>(nblk, 256)), dim3(64 * NW2_HEAVY),
-                           sh * NW2_HEAVY, c.stream(), w.C, w.F, cs, w.val_b, w.bstart, w.bend, w.order + first,
-                           (unsigned)nblk, sg, bn);
-        hipLaunchKernelGGL(k_spread2d_bin, dim3((unsigned)nblk), dim3(64), sh, c.stream(), w.C, w.F, cs, w.val_b,
-                           w.bstart, w.bend, w.order + first, sg, bn);
+        hipLaunchKernelGGL(k_spread2d_bin_heavy<S>, dim3((unsigned)std::min<int64_t>(nblk, 256)),
+                           dim3(64 * S::NWH), sh * S::NWH, c.stream(), w.C, w.F, cs, w.val_b, w.bstart, w.bend,
+                           w.order + first, (unsigned)nblk, sg, bn);
+        hipLaunchKernelGGL(k_spread2d_bin<S>, dim3((unsigned)nblk), dim3(64), sh, c.stream(), w.C, w.F, cs,
+                           w.val_b, w.bstart, w.bend, w.order + first, sg, bn);
         LC_HIP(hipGetLastError());
         first += nblk;
     }
@@ -962,7 +982,10 @@ void type1(Ctx &c, const Geom2D &g, const NuWork &w, float *vol, int ns) {
     const int N = (int)g.N, nf = u.nf;
     const SpreadGeom sg = spread_geom(g);
     if (w.key_a) {
-        spread2_binned(c, g, w, ns);
+        // the context's bin shape for this fine-grid size, if it fits this stack
+        int shape = c.nu2_shape(nf);
+        if (!shape2_fits(ns, g.nang * g.nrad, nf, shape2_edge(shape))) shape = 0;
+        with_shape2(shape, [&](auto sh) { spread2_binned<decltype(sh)>(c, g, w, ns); });
     } else {
         const size_t shs = sizeof(float2) * NU_WAVES * NU_T * NU_T + sizeof(float) * NU_WAVES * 64 * 2 * NU_W;
         hipLaunchKernelGGL(k_spread2d, dim3(sg.ntx * sg.ntx, ns), dim3(64 * NU_WAVES), shs, c.stream(),
@@ -1056,6 +1079,57 @@ void sysmat2d_nufft_dev(Ctx &c, Geom2D &g, const float *x, float *y, int ns) {
     type2(c, g, w, x, ns);
     type1(c, g, w, y, ns);
 }
+
+// Times the back-projection of nslices zero sinograms on g's grid with every
+// compiled bin shape of the spread that fits its fine grid, installs the
+// fastest in c.nu2_tuning for that fine-grid size and returns its index
+// (best_ms: one back-projection).  Grids the binned spread does not serve
+// keep shape 0 (0 ms).
+int nufft2d_autotune(Ctx &c, Geom2D &g, int nslices, double *best_ms) {
+    nu_check(c, g, nslices);
+    const int nf = g.nu.nf;
+    const int64_t npts = g.nang * g.nrad;
+    if (best_ms) *best_ms = 0.0;
+    if (!spread_binned(nslices, npts, nf)) return 0;
+    DBuf sino(sizeof(float) * (size_t)(nslices * npts)), vol(sizeof(float) * (size_t)nslices * g.N * g.N);
+    LC_HIP(hipMemsetAsync(sino.as(), 0, sizeof(float) * (size_t)(nslices * npts), c.stream()));
+    const auto saved = c.nu2_tuning;
+    hipEvent_t e0, e1;
+    LC_HIP(hipEventCreate(&e0));
+    LC_HIP(hipEventCreate(&e1));
+    int best = 0;
+    float tbest = 1e30f;
+    try {
+        for (int i = 0; i < N_SHAPES2; ++i) {
+            if (!shape2_fits(nslices, npts, nf, shape2_edge(i))) continue;
+            c.nu2_tuning[nf] = i;
+            backproj2d_dev(c, g, sino.as<float>(), vol.as<float>(), nslices);   // warm-up
+            LC_HIP(hipEventRecord(e0, c.stream()));
+            for (int r = 0; r < 3; ++r) backproj2d_dev(c, g, sino.as<float>(), vol.as<float>(), nslices);
+            LC_HIP(hipEventRecord(e1, c.stream()));
+            LC_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            LC_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms / 3 < tbest) {
+                tbest = ms / 3;
+                best = i;
+            }
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        c.nu2_tuning = saved;
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c.nu2_tuning = saved;
+    c.nu2_tuning[nf] = best;
+    if (best_ms) *best_ms = tbest;
+    return best;
+}
+
+int nufft2d_num_shapes() { return N_SHAPES2; }
 
 // device bytes of a 2-D back-projection's binned spread beyond C, F and V
 // (0 when the stack spreads by tile gather)

@@ -222,6 +222,12 @@ class Ctx {
         auto it = nu3_tuning.find({nf[0], nf[1], nf[2]});
         return it == nu3_tuning.end() ? 0 : it->second;
     }
+    // per 2-D fine-grid size nf: the type-1 spread's bin shape
+    std::map<int, int> nu2_tuning;
+    int nu2_shape(int nf) const {
+        auto it = nu2_tuning.find(nf);
+        return it == nu2_tuning.end() ? 0 : it->second;
+    }
     explicit Ctx(int device);
     ~Ctx();
     int device() const { return device_; }

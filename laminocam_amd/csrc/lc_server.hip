@@ -89,6 +89,7 @@ Server::Server(Ctx &main, size_t budget, int nlanes, int max_batch)
         l->ctx = std::make_unique<Ctx>(main.device());
         l->ctx->toep_tuning = main.toep_tuning;
         l->ctx->nu3_tuning = main.nu3_tuning;
+        l->ctx->nu2_tuning = main.nu2_tuning;
         LC_HIP(hipEventCreate(&l->ev0));
         LC_HIP(hipEventCreate(&l->ev1));
         lanes_.push_back(std::move(l));

@@ -118,3 +118,36 @@ def test_sysmat_nufft_matches_oracle_and_toeplitz(lc, orc64, N, nang, ns):
     assert rel_l2(y1, orc64.sysmat2d_nufft(x, th, N, tol=1e-12)) < PROJ_TOL
     y2 = lc.sysmat2d(x, lc.PointSpreadFunction2D(g, N))
     assert rel_l2(y1, y2) < PROJ_TOL
+
+
+def test_backproj2d_bin_shapes_and_autotune(lc, orc64):
+    # both compiled bin sizes of the sorted-bin spread (16 and 32 corner
+    # cells) give the oracle's back-projection; a slice is the same alone and
+    # in a stack for each; the autotuner installs one of them
+    N, nang, ns = 128, 180, 3
+    announce(f"backproj2d bin shapes N={N}")
+    rng = np.random.default_rng(19)
+    th = uniform_theta(nang)
+    sino = rng.standard_normal((ns, nang, N)).astype(np.float32)
+    xr = orc64.backproj2d(sino, th, N, tol=1e-12)
+    ctx = lc.Context(0)
+    try:
+        g = lc.PolarGrid2D(th, N)
+        psf = lc.PointSpreadFunction2D(g, N, ctx=ctx)
+        xs = []
+        for shape in range(2):
+            ctx.set_tuning2d_nufft(psf, shape)
+            assert ctx.get_tuning2d_nufft(psf) == shape
+            x = lc.backproj2d(sino, g, (ns, N, N), ctx=ctx)
+            err = rel_l2(x, xr)
+            print(f"shape {shape}: rel_l2 = {err:.3e}")
+            assert err < PROJ_TOL
+            assert np.array_equal(lc.backproj2d(sino[1:2], g, (1, N, N), ctx=ctx), x[1:2])
+            xs.append(x)
+        assert rel_l2(xs[1], xs[0]) < 1e-5
+        best, ms = ctx.autotune2d_nufft(psf, 4)
+        print(f"autotune2d_nufft: shape {best}, {ms:.3f} ms")
+        assert best in (0, 1) and ms > 0.0
+        assert ctx.get_tuning2d_nufft(psf) == best
+    finally:
+        ctx.close()
