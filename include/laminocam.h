@@ -72,6 +72,9 @@ const char *lc_last_error(void);
 
 /* ---- context: one device, one stream ---- */
 int lc_ctx_create(int device, lc_ctx *out);
+/* drops the caller's handle; every geometry, batch and server created on the
+ * context holds a share of it, and the context is freed (after waiting for its
+ * stream) when the last of them is destroyed */
 int lc_ctx_destroy(lc_ctx ctx);
 int lc_ctx_sync(lc_ctx ctx);
 

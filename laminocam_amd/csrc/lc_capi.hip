@@ -1,4 +1,5 @@
 // extern "C" boundary of the library (include/laminocam.h).
+#include <atomic>
 #include <cstring>
 #include <string>
 
@@ -19,8 +20,31 @@ template <typename F> int guard(F &&f) { return c_guard(std::forward<F>(f)); }
 
 struct lc_ctx_s {
     Ctx c;
+    // the caller's handle plus one share per geometry, batch and server
+    // created on the context: it is freed when the last of them is destroyed
+    std::atomic<int> refs{1};
     explicit lc_ctx_s(int dev) : c(dev) {}
 };
+
+static lc_ctx_s *ctx_retain(lc_ctx_s *c) {
+    c->refs.fetch_add(1, std::memory_order_relaxed);
+    return c;
+}
+
+static void ctx_release(lc_ctx_s *c) {
+    if (c && c->refs.fetch_sub(1, std::memory_order_acq_rel) == 1) delete c;
+}
+
+// waits for the object's context, destroys the object, then drops its share
+// of the context
+template <typename H> static void destroy_on_ctx(H *h) {
+    if (!h) return;
+    lc_ctx_s *c = h->ctx;
+    c->c.set_current();
+    c->c.sync();
+    delete h;
+    ctx_release(c);
+}
 
 struct lc_geom2d_s {
     lc_ctx_s *ctx;
@@ -85,7 +109,7 @@ int lc_ctx_create(int device, lc_ctx *out) {
 }
 
 int lc_ctx_destroy(lc_ctx ctx) {
-    return guard([&] { delete ctx; });
+    return guard([&] { ctx_release(ctx); });
 }
 
 int lc_ctx_sync(lc_ctx ctx) {
@@ -112,18 +136,13 @@ int lc_geom2d_create(lc_ctx ctx, const double *theta, int64_t nang, int64_t nrad
             delete g;
             throw;
         }
+        ctx_retain(ctx);
         *out = g;
     });
 }
 
 int lc_geom2d_destroy(lc_geom2d g) {
-    return guard([&] {
-        if (g) {
-            g->ctx->c.set_current();
-            g->ctx->c.sync();
-        }
-        delete g;
-    });
+    return guard([&] { destroy_on_ctx(g); });
 }
 
 int lc_geom2d_info(lc_geom2d g, int64_t *info) {
@@ -375,18 +394,13 @@ int lc_geom3d_create(lc_ctx ctx, const double *theta, const double *gamma, const
             delete g;
             throw;
         }
+        ctx_retain(ctx);
         *out = g;
     });
 }
 
 int lc_geom3d_destroy(lc_geom3d g) {
-    return guard([&] {
-        if (g) {
-            g->ctx->c.set_current();
-            g->ctx->c.sync();
-        }
-        delete g;
-    });
+    return guard([&] { destroy_on_ctx(g); });
 }
 
 int lc_geom3d_info(lc_geom3d g, int64_t *info) {
@@ -641,18 +655,13 @@ int lc_batch_create_ex(lc_ctx ctx, int32_t nprob, const lc_geom2d *g2, const lc_
             throw;
         }
         b->reps.resize(nprob);
+        ctx_retain(ctx);
         *out = b;
     });
 }
 
 int lc_batch2d_destroy(lc_batch2d b) {
-    return guard([&] {
-        if (b) {
-            b->ctx->c.set_current();
-            b->ctx->c.sync();
-        }
-        delete b;
-    });
+    return guard([&] { destroy_on_ctx(b); });
 }
 
 } // extern "C"
@@ -918,6 +927,7 @@ int lc_server_create(lc_ctx ctx, uint64_t hbm_budget, int32_t lanes, int32_t max
             delete sv;
             throw;
         }
+        ctx_retain(ctx);
         *out = sv;
     });
 }
@@ -1103,7 +1113,12 @@ int lc_request_bytes(const lc_request *r, uint64_t *bytes) {
 }
 
 int lc_server_destroy(lc_server s) {
-    return guard([&] { delete s; });
+    return guard([&] {
+        if (!s) return;
+        lc_ctx_s *c = s->ctx;
+        delete s;
+        ctx_release(c);
+    });
 }
 
 } // extern "C"

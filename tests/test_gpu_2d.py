@@ -203,6 +203,25 @@ def test_batch_same_alone_and_in_mix(lc, orc64):
         assert np.array_equal(xs, batch.get(i, batch.X))
 
 
+def test_context_outlived_by_its_geometry(lc):
+    """A context whose handle is destroyed while a geometry made on it is alive
+    stays valid until that geometry is destroyed: the geometry's destroy
+    returns 0 and leaves no error behind for the next call."""
+    announce("context outlived by its geometry")
+    N = 32
+    th = ref_theta(16)
+    x = np.random.default_rng(3).random((1, N, N)).astype(np.float32)
+    grid = lc.PolarGrid2D(th, N)
+    ctx = lc.Context(0)
+    psf = lc.PointSpreadFunction2D(grid, N, ctx=ctx)
+    y = psf.convolve(x)
+    lib, h = psf.lib, psf.handle
+    ctx.close()
+    psf.handle = None
+    assert lib.lc_geom2d_destroy(h) == 0
+    assert np.array_equal(lc.PointSpreadFunction2D(grid, N).convolve(x), y)
+
+
 @pytest.mark.parametrize("N", [128, 384, 576, 721, 1024])
 def test_row_shapes_agree(lc, orc64, N):
     # both compiled row-pass shapes (256x8, 512x8) give the oracle's
