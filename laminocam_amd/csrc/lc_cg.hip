@@ -32,6 +32,9 @@ namespace lc {
 // loads of each vector per thread)
 constexpr int CG_UPD_NT = 512;
 
+// CG iterations per replay of a batch's cached iteration graph
+constexpr int CG_GRAPH_ITERS = 8;
+
 // ----------------------------------------------------------------------------
 // reductions
 // ----------------------------------------------------------------------------
@@ -1036,8 +1039,17 @@ void Batch2D::cg(const std::vector<int> &max_iter, double tol, double xtol, bool
     LC_HIP(hipGetLastError());
     cg_setup(x0_zero, false, max_iter, tol, xtol);
     if (graph && !host_sync) {
-        CapturedGraph gr(c_, [&] { cg_iteration(false, false); });
-        for (int it = 0; it < iters; ++it) gr.launch();
+        // CG_GRAPH_ITERS iterations per launch, then single ones; a problem
+        // that stops (tolerance or its own max_iter) leaves the later
+        // iterations' kernels with nothing to do
+        if (!cg_graph1_) cg_graph1_ = std::make_unique<CapturedGraph>(c_, [&] { cg_iteration(false, false); });
+        if (iters >= CG_GRAPH_ITERS && !cg_graphk_)
+            cg_graphk_ = std::make_unique<CapturedGraph>(c_, [&] {
+                for (int i = 0; i < CG_GRAPH_ITERS; ++i) cg_iteration(false, false);
+            });
+        int it = 0;
+        for (; it + CG_GRAPH_ITERS <= iters; it += CG_GRAPH_ITERS) cg_graphk_->launch();
+        for (; it < iters; ++it) cg_graph1_->launch();
     } else {
         for (int it = 0; it < iters; ++it) {
             cg_iteration(host_sync, false);

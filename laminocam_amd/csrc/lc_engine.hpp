@@ -372,6 +372,10 @@ class Batch2D {
     DBuf Dv_, Bv_, XO_, Y0_;
     int sb_boundary_ = 0;
     bool sb_aniso_ = false;
+    // executable graphs of one and of CG_GRAPH_ITERS CG iterations, captured
+    // by the first graph-mode solve and replayed by later ones (their kernels
+    // read every per-solve setting from device state); destroyed first
+    std::unique_ptr<CapturedGraph> cg_graph1_, cg_graphk_;
 };
 
 // finite differences on one volume: op 0 grad (3 components out), 1 div
