@@ -631,14 +631,12 @@ void Batch2D::build() {
         if (f64_) k->tw64 = c_.twiddles<double>(k->M);
         else k->tw = c_.twiddles<float>(k->M);
         if (g3) {
-            // row shape by M (512 x 8 above 1024: 1.7% faster per laminography
-            // CG iteration at M = 1536 than 256 x 8), column blocks B * M <= the 3-D cap
-            ToepVariant v;
-            v.row = k->M > 1024 ? 1 : 0;
-            v.col = f64_ ? 0 : 2;   // block-width bound only: 4096 (FP64) or 8192 elements
-            v.B = 0;
-            toep_configure(*k, v);
+            // column blocks B * M <= the 3-D cap, of the block width the PSF
+            // spectrum is stored in
+            toep_configure(*k, toep3d_variant(k->M, f64_));
             LC_REQUIRE(k->B * k->M <= toep3d_col_cap(f64_), "batch: 3-D column block too large");
+            LC_REQUIRE(k->B == (f64_ ? g3->kB64 : g3->kB),
+                       "batch: 3-D PSF spectrum stored in another column block width");
             k->g3 = g3;
             k->n1 = (int)g3->n1;
             k->M1 = (int)g3->M1;

@@ -103,8 +103,12 @@ struct Geom3D {
     int64_t Mr = 0;                   // the reference's in-plane PSF size next_fast_dim(2N - 1)
     std::vector<double> rot;          // per projection: u (3) then v (3)
     DBuf d_rot;                       // double[nproj][6]
-    DBuf khat3;                       // float[M][H][M1]: real spectrum of the centred PSF / (M1 M^2 N^2)
-    DBuf khat3_64;                    // double[M][H][M1]: the same for FP64 solves (on demand)
+    // real spectrum of the centred PSF / (M1 M^2 N^2), in the column blocks of
+    // the 3-D Toeplitz class: [ceil(H / kB)][M1][M][kB], value (k1, k2, k3) at
+    // ((k3 / kB) M1 + k1) M kB + k2 kB + k3 % kB, zero for k3 >= H
+    DBuf khat3;                       // float, block width kB
+    DBuf khat3_64;                    // double, block width kB64 (FP64 solves, on demand)
+    int kB = 0, kB64 = 0;
     bool psf_ready = false, psf64_ready = false;
     Nufft3D nu;
 };
@@ -229,6 +233,10 @@ template <> inline const double2 *toep_tw<double>(const ToepClass &k) { return k
 // block shapes for a class; U elements per slice (independent of the shapes)
 void toep_configure(ToepClass &k, ToepVariant v);
 ToepVariant toep_default_variant(int M);
+// shapes of a 3-D volume class, and the column block width they give (the
+// block width of the 3-D PSF spectrum's layout)
+ToepVariant toep3d_variant(int M, bool f64);
+int toep3d_block_width(int N, int M, bool f64);
 // fine-grid sizes of the 2-D and 3-D NUFFTs (per axis), as their init sets them
 int nufft2d_fine_n(int64_t N);
 int nufft3d_fine_n(int64_t n);

@@ -16,12 +16,14 @@
 //   P1  rows of every slice: the 2-D row pass (row pairs, FFT_M, half spectra) -> U
 //   P2f every slice, every column block: FFT_M down the columns -> W (all M rows)
 //   P2b every (k2, column block): FFT_M1 along the slices (zero padded),
-//       x khat3[k2][k3][:], inverse, keep n1 -> W
+//       x khat3 (k2, k3, :), inverse, keep n1 -> W
 //   P2i every slice, column block: inverse FFT_M down the columns, keep N -> U
 //   P3  rows of every slice: the 2-D inverse row pass -> output
 // W is [nblk][n1][M][B] complex: the column passes read and write one
 // contiguous M x B block per (column block, slice), and P2b reads, per slice,
-// kb2 x B consecutive values (its lines) as one contiguous piece.
+// kb2 x B consecutive values (its lines) as one contiguous piece.  khat3 is
+// blocked the same way ([nblk][M1][M][B] real), so P2b reads each slice
+// index's spectrum values of its lines as one contiguous piece too.
 #include <cmath>
 
 #include <type_traits>
@@ -217,11 +219,12 @@ k_psf3_cols(double2 *P, int M, int ncol, int B, FftPlan plan, const double2 *__r
     }
 }
 
-// FP64 FFT along z of columns (k2, k3 < H), real part x scale ->
-// khat3[k2][k3][k1] (OT: float or double).  grid (ceil(H / B), M)
+// FP64 FFT along z of columns (k2, k3 < H), real part x scale -> khat3 in
+// column blocks of width KB: value (z, k2, k3) at ((k3 / KB) M1 + z) M KB +
+// k2 KB + k3 % KB (OT: float or double).  grid (ceil(H / B), M)
 template <typename OT, int NT, int MAXE>
 __global__ void __launch_bounds__(NT)
-k_psf3_slices(const double2 *P, OT *khat3, int M1, int M, int H, int B, FftPlan plan,
+k_psf3_slices(const double2 *P, OT *khat3, int M1, int M, int H, int B, int KB, FftPlan plan,
               const double2 *__restrict__ tw, double scale) {
     double2 *buf = reinterpret_cast<double2 *>(lc_smem);
     const int k30 = blockIdx.x * B;
@@ -233,9 +236,11 @@ k_psf3_slices(const double2 *P, OT *khat3, int M1, int M, int H, int B, FftPlan 
     __syncthreads();
     fft_lds<double, -1, NT, MAXE, true>(buf, B, 0, plan, tw);
     for (int idx = threadIdx.x; idx < M1 * B; idx += NT) {
-        const int c = idx / M1, z = idx - c * M1;
-        if (k30 + c < H)
-            khat3[((size_t)k2 * H + k30 + c) * M1 + z] = (OT)(buf[lds_col(c, z, B)].x * scale);
+        const int z = idx / B, c = idx - z * B;
+        const int k3 = k30 + c;
+        if (k3 < H)
+            khat3[(((size_t)(k3 / KB) * M1 + z) * M + k2) * KB + k3 % KB] =
+                (OT)(buf[lds_col(c, z, B)].x * scale);
     }
 }
 
@@ -276,14 +281,20 @@ void geom3d_build_psf(Ctx &c, Geom3D &g, bool f64) {
         const double scale = 1.0 / ((double)M1 * M * M * (double)g.N * (double)g.N);
         const size_t sh = sizeof(double2) * lds_round(B * M1);
         const dim3 grid(ceil_div(H, B), M);
+        // the 3-D Toeplitz class's column block width; k3 in [H, nblk KB) stays zero
+        const int KB = toep3d_block_width((int)g.N, M, f64);
+        const size_t nk = (size_t)ceil_div(H, KB) * KB * M * M1;
+        DBuf &kh = f64 ? g.khat3_64 : g.khat3;
+        kh.alloc((f64 ? sizeof(double) : sizeof(float)) * nk);
+        LC_HIP(hipMemsetAsync(kh.as(), 0, (f64 ? sizeof(double) : sizeof(float)) * nk, c.stream()));
         if (f64) {
-            g.khat3_64.alloc(sizeof(double) * (size_t)M * H * M1);
+            g.kB64 = KB;
             hipLaunchKernelGGL((k_psf3_slices<double, NT, MAXE>), grid, dim3(NT), sh, c.stream(), P,
-                               g.khat3_64.as<double>(), M1, M, H, B, plan1, tw1, scale);
+                               g.khat3_64.as<double>(), M1, M, H, B, KB, plan1, tw1, scale);
         } else {
-            g.khat3.alloc(sizeof(float) * (size_t)M * H * M1);
+            g.kB = KB;
             hipLaunchKernelGGL((k_psf3_slices<float, NT, MAXE>), grid, dim3(NT), sh, c.stream(), P,
-                               g.khat3.as<float>(), M1, M, H, B, plan1, tw1, scale);
+                               g.khat3.as<float>(), M1, M, H, B, KB, plan1, tw1, scale);
         }
         LC_HIP(hipGetLastError());
     }
@@ -418,7 +429,7 @@ k_t3_cols_fwd(const SliceRef *__restrict__ sl, const CgState *__restrict__ st,
 }
 
 // P2b: for kb2 consecutive k2 of column block kb: FFT_M1 along the slices
-// (n1 values, zero padded), x khat3[k2][k3][k1], inverse, keep n1.
+// (n1 values, zero padded), x khat3 (k2, k3, k1), inverse, keep n1.
 // Lines (j, c) -> line j * B + c.  grid (nblk, ceil(M / kb2))
 template <typename R, int NT, int MAXE>
 __global__ void __launch_bounds__(NT)
@@ -427,7 +438,7 @@ k_t3_slices(const SliceRef *__restrict__ sl, const CgState *__restrict__ st, cpx
             const cpx<R> *__restrict__ tw1) {
     using C = cpx<R>;
     if (!prob_active3(st, sl[0].prob)) return;
-    const int M = g.M, H = g.H, B = g.B;
+    const int M = g.M, B = g.B;
     const int kb = blockIdx.x, k20 = blockIdx.y * kb2;
     const int nk2 = min(kb2, M - k20);
     const int nl = kb2 * B;
@@ -443,11 +454,13 @@ k_t3_slices(const SliceRef *__restrict__ sl, const CgState *__restrict__ st, cpx
     }
     __syncthreads();
     fft_lds<R, -1, NT, MAXE, true>(buf, nl, 0, plan1, tw1);
+    // the spectrum's column block kb holds these lines contiguously per slice
+    // index e: kh[e M B + line]
+    const R *kh = khat3 + ((int64_t)kb * M1 * M + k20) * B;
     for (int idx = threadIdx.x; idx < nl * M1; idx += NT) {
-        const int line = idx / M1, e = idx - line * M1;   // e fastest: contiguous khat reads
-        const int j = line / B, c = line - j * B;
-        const int k2 = k20 + j, k3 = kb * B + c;
-        const R kv = (j < nk2 && k3 < H) ? khat3[((int64_t)k2 * H + k3) * M1 + e] : (R)0;
+        const int e = idx / nl, line = idx - e * nl;
+        const int j = line / B;
+        const R kv = j < nk2 ? kh[(int64_t)e * M * B + line] : (R)0;
         C *p = buf + lds_col(line, e, nl);
         *p = cscale(*p, kv);
     }
@@ -477,25 +490,25 @@ k_t3_slices_ct(const SliceRef *__restrict__ sl, const CgState *__restrict__ st, 
     static_assert((1 << LSH) == NL, "lines per block must be a power of two up to 256");
     constexpr int HALF = (L1 + 1) / 2;
     if (!prob_active3(st, sl[0].prob)) return;
-    const int M = g.M, H = g.H, B = g.B, bsh = g.lsh;
+    const int M = g.M, B = g.B, bsh = g.lsh;
     const int kb = blockIdx.x, k20 = blockIdx.y * kb2;
     const int nk2 = min(kb2, M - k20);
     C *buf = reinterpret_cast<C *>(lc_smem);
     // slice e's values of these lines: w[e M B + line] (lines j << bsh | c
-    // are consecutive values)
+    // are consecutive values); the spectrum's column block kb holds them at
+    // kh[e M B + line] (neighbouring lines of a wave read neighbouring values)
     C *w = W + ((int64_t)kb * n1 * M + k20) * B;
+    const R *kh = khat3 + ((int64_t)kb * L1 * M + k20) * B;
     // spectrum values of this thread's middle-stage butterflies (line, index)
     R kr[CM::MAXB][CM::R];
 #pragma unroll
     for (int t = 0; t < CM::MAXB; ++t) {
         int line = 0, jm = 0;
         const bool has = conv_mid_index<R, L1, NT, MAXE, LSH>(t, NL, LSH, line, jm);
-        const int j = line >> bsh, c = line & (B - 1);
-        const int k3 = kb * B + c;
-        const bool ok = has && j < nk2 && k3 < H;
-        const R *kc = khat3 + ((int64_t)(ok ? k20 + j : 0) * H + (ok ? k3 : 0)) * L1;
+        const bool ok = has && (line >> bsh) < nk2;
 #pragma unroll
-        for (int r = 0; r < CM::R; ++r) kr[t][r] = ok ? kc[jm + r * CM::LR] : (R)0;
+        for (int r = 0; r < CM::R; ++r)
+            kr[t][r] = ok ? kh[(int64_t)(jm + r * CM::LR) * M * B + line] : (R)0;
     }
     for (int idx = threadIdx.x; idx < NL * HALF; idx += NT) {
         const int e = idx >> LSH, line = idx & (NL - 1);
