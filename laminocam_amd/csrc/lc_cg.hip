@@ -24,6 +24,8 @@
 #include <cstring>
 #include <map>
 
+#include <type_traits>
+
 #include "lc_engine.hpp"
 
 namespace lc {
@@ -387,12 +389,38 @@ k_sb_lapdot(const ChunkRef *__restrict__ ch, const ProbDev *__restrict__ pd,
     const int64_t nn = (int64_t)d.N * d.N;
     double acc = 0.0;
     Idx3 q = idx_of(d, c.begin + threadIdx.x);
-    for (int64_t i = c.begin + threadIdx.x; i < c.end; i += NT, step(q, NT)) {
-        const R l = lap_at<R>(Rel<R>{p + i, nn, d.N}, q, d.boundary);
-        const R v = __builtin_nontemporal_load(ap + i) - l * (R)d.mu;
-        __builtin_nontemporal_store(v, ap + i);
-        acc += (double)p[i] * v;
-    }
+    // UE elements per thread per step (i, i + NT, ...): the stencil loads of
+    // all of them are issued before any result is used, so a wave keeps UE
+    // elements' loads in flight instead of one; acc sums in the same order
+    constexpr int UE = 4;
+    auto run = [&](auto full) {
+        constexpr bool FULL = decltype(full)::value;
+        for (int64_t i0 = c.begin + threadIdx.x; i0 < c.end; i0 += UE * NT) {
+            R l[UE], a[UE], pv[UE];
+#pragma unroll
+            for (int u = 0; u < UE; ++u) {
+                const int64_t i = i0 + u * NT;
+                if (FULL || i < c.end) {
+                    l[u] = lap_at<R>(Rel<R>{p + i, nn, d.N}, q, d.boundary);
+                    a[u] = __builtin_nontemporal_load(ap + i);
+                    pv[u] = p[i];
+                }
+                step(q, NT);
+            }
+#pragma unroll
+            for (int u = 0; u < UE; ++u) {
+                const int64_t i = i0 + u * NT;
+                if (FULL || i < c.end) {
+                    const R v = a[u] - l[u] * (R)d.mu;
+                    __builtin_nontemporal_store(v, ap + i);
+                    acc += (double)pv[u] * v;
+                }
+            }
+        }
+    };
+    static_assert(CHUNK % (UE * NT) == 0, "a whole chunk is a whole number of steps");
+    if (c.end - c.begin == CHUNK) run(std::true_type{});
+    else run(std::false_type{});
     if (dot) {
         __shared__ double sh[NT / 64];
         const double t = block_sum<NT>(acc, sh);
