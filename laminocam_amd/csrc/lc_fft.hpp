@@ -107,6 +107,27 @@ template <int SIGN, typename C> __host__ __device__ __forceinline__ void dft3(C 
     v[2] = t2 - jd;
 }
 
+// 6 = 2 x 3: DFT3 of the even and of the odd samples, then one radix-2
+// layer with the twiddles w^1 = (1/2, SIGN sqrt(3)/2), w^2 = (-1/2, SIGN sqrt(3)/2)
+template <int SIGN, typename C> __host__ __device__ __forceinline__ void dft6(C *v) {
+    using R = decltype(C{}.x);
+    const R h = (R)0.5, s = (R)SIGN * (R)0.8660254037844386467637231707529362;
+    C e[3] = {v[0], v[2], v[4]}, o[3] = {v[1], v[3], v[5]};
+    dft3<SIGN>(e);
+    dft3<SIGN>(o);
+    C t1, t2;
+    t1.x = h * o[1].x - s * o[1].y;
+    t1.y = h * o[1].y + s * o[1].x;
+    t2.x = -h * o[2].x - s * o[2].y;
+    t2.y = -h * o[2].y + s * o[2].x;
+    v[0] = e[0] + o[0];
+    v[3] = e[0] - o[0];
+    v[1] = e[1] + t1;
+    v[4] = e[1] - t1;
+    v[2] = e[2] + t2;
+    v[5] = e[2] - t2;
+}
+
 template <int SIGN, typename C> __host__ __device__ __forceinline__ void dft8(C *v) {
     using R = decltype(C{}.x);
     const R s = (R)0.7071067811865475244008443621048490;
@@ -171,6 +192,7 @@ template <int R, int SIGN, typename C> __host__ __device__ __forceinline__ void 
     if constexpr (R == 2) dft2<SIGN>(v);
     else if constexpr (R == 3) dft3<SIGN>(v);
     else if constexpr (R == 4) dft4<SIGN>(v);
+    else if constexpr (R == 6) dft6<SIGN>(v);
     else if constexpr (R == 8) dft8<SIGN>(v);
     else dft16<SIGN>(v);
 }
@@ -284,10 +306,17 @@ constexpr int ct_threes(int L) { int t = 0; while (L % 3 == 0) { L /= 3; ++t; } 
 //   one RM, remainder, 3s, the other RMs    (L = 2048: 16 8 16; 768: 8 4 3 8)
 constexpr int ct_lg(int RM) { return RM == 16 ? 4 : 3; }
 constexpr bool ct_big_last(int L) { return L == 2048 || L == 768; }
+// radix 16 with a remainder of one 2 and a single 3 (L = 2^(4a+1) 3, e.g.
+// 1536 = 16 16 6): the 2 and the 3 merge into one radix-6 stage
+constexpr bool ct_six(int L, int RM) {
+    return RM == 16 && ct_threes(L) == 1 && ct_twos(L) % 4 == 1;
+}
 constexpr int ct_nst(int L, int RM = 8) {
-    return ct_twos(L) / ct_lg(RM) + (ct_twos(L) % ct_lg(RM) ? 1 : 0) + ct_threes(L);
+    return ct_six(L, RM) ? ct_twos(L) / 4 + 1
+                         : ct_twos(L) / ct_lg(RM) + (ct_twos(L) % ct_lg(RM) ? 1 : 0) + ct_threes(L);
 }
 constexpr int ct_radix(int L, int s, int RM = 8) {
+    if (ct_six(L, RM)) return s < ct_twos(L) / 4 ? 16 : 6;
     const int nb = ct_twos(L) / ct_lg(RM), rem = ct_twos(L) % ct_lg(RM);
     const int nrem = rem ? 1 : 0, th = ct_threes(L);
     const int lead = ct_big_last(L) ? (nb > 0 ? 1 : 0) : nb;   // RMs before the remainder
@@ -339,7 +368,7 @@ __device__ __forceinline__ void fft_stage_ct(cpx<T> *buf, int nl, int lsh,
             const int j = LFAST ? (b >> ls) : (b % LR);
             const int k = j % NS;
             wa[t] = tw[k * TWS];
-            if constexpr (R >= 8) wb[t] = tw[4 * k * TWS];
+            if constexpr (R >= 5) wb[t] = tw[4 * k * TWS];
             if constexpr (R == 16) wc[t] = tw[8 * k * TWS];
         }
     }
@@ -384,6 +413,11 @@ __device__ __forceinline__ void fft_stage_ct(cpx<T> *buf, int nl, int lsh,
                 if (SIGN > 0) w[1].y = -w[1].y;
                 if constexpr (R >= 3) w[2] = cmul(w[1], w[1]);
                 if constexpr (R >= 4) w[3] = cmul(w[2], w[1]);
+                if constexpr (R == 6) {
+                    w[4] = wb[t];
+                    if (SIGN > 0) w[4].y = -w[4].y;
+                    w[5] = cmul(w[4], w[1]);
+                }
                 if constexpr (R >= 8) {
                     w[4] = wb[t];
                     if (SIGN > 0) w[4].y = -w[4].y;
@@ -439,7 +473,7 @@ __device__ __forceinline__ void fft_stage_ct(cpx<T> *buf, int nl, int lsh,
 // fewer than radix 8.  Elsewhere radix 8: the stage count is the same or
 // the larger radix measured slower (lower occupancy).
 template <typename T, int MAXE, int L> constexpr int ct_rmax() {
-    return (sizeof(T) == 4 && MAXE >= 16 && L >= 1024 && ct_threes(L) == 0) ? 16 : 8;
+    return (sizeof(T) == 4 && MAXE >= 16 && L >= 1024 && (ct_threes(L) == 0 || ct_six(L, 16))) ? 16 : 8;
 }
 
 // Stages S .. SEND - 1 of the plan (REV: the radices in reverse order).  PIN
@@ -542,7 +576,7 @@ __device__ __forceinline__ void fft_conv_mid(cpx<T> *buf, int nl, int lsh,
         int line, j;
         if (!conv_mid_index<T, L, NT, MAXE, LSH>(t, nl, lsh, line, j)) j = 0;
         wa[t] = tw[j];
-        if constexpr (R >= 8) wb[t] = tw[4 * j];
+        if constexpr (R >= 5) wb[t] = tw[4 * j];
         if constexpr (R == 16) wc[t] = tw[8 * j];
     }
 #pragma unroll
@@ -563,6 +597,10 @@ __device__ __forceinline__ void fft_conv_mid(cpx<T> *buf, int nl, int lsh,
             w[1] = wa[t];
             if constexpr (R >= 3) w[2] = cmul(w[1], w[1]);
             if constexpr (R >= 4) w[3] = cmul(w[2], w[1]);
+            if constexpr (R == 6) {
+                w[4] = wb[t];
+                w[5] = cmul(w[4], w[1]);
+            }
             if constexpr (R >= 8) {
                 w[4] = wb[t];
                 w[5] = cmul(w[4], w[1]);
