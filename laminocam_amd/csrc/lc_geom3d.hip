@@ -150,8 +150,8 @@ k_psf3d_gen(double2 *P, const double *rot, int nproj, int nrows, int ncols, int 
                 // D_ncols(t1) D_nrows(t2) = (a1i a2i) / (b1i b2i) away from
                 // the removable zeros.  The quotient takes the hardware
                 // reciprocal refined by one Newton step (|b1i b2i| > 1e-6; the
-                // step squares the estimate's relative error, at most 2^-23,
-                // to 2^-46 or less) instead of the full division sequence.
+                // step squares the estimate's relative error) instead of the
+                // full division sequence.
                 const bool f1 = fabs(b1i) > 1e-3, f2 = fabs(b2i) > 1e-3;
                 if (f1 && f2) {
                     const double den = b1i * b2i;
