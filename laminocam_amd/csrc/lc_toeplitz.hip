@@ -296,9 +296,9 @@ template void geom2d_kernel_hat<double>(Ctx &, Geom2D &, double *);
 namespace {
 struct RowShape { int nt, maxe; };
 struct ColShape { int nt, maxe; };
-constexpr RowShape ROW_SHAPES[] = {{256, 8}, {512, 8}};
+constexpr RowShape ROW_SHAPES[] = {{256, 8}, {512, 8}, {256, 16}};
 constexpr ColShape COL_SHAPES[] = {{256, 16}, {256, 8}, {512, 16}};
-constexpr int N_ROW_SHAPES = 2, N_COL_SHAPES = 3;
+constexpr int N_ROW_SHAPES = 3, N_COL_SHAPES = 3;
 } // namespace
 
 #define LC_DECL_L(L)                                                                              \
