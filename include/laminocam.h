@@ -415,8 +415,9 @@ int lc_request_bytes(const lc_request *r, uint64_t *bytes);
 /* ---- Toeplitz kernel shapes per image size (no reference counterpart) ----
  * variant[0] = row-pass shape (0: 256 threads x 8 elements, 1: 512 x 8,
  * 2: 256 x 16), variant[1] = column-pass shape (0: 256 x 16, 1: 256 x 8,
- * 2: 512 x 16, FP32 only), variant[2] = columns per column-pass block (a
- * power of two, at most 16; 0 = the widest that fits).  lc_autotune2d times the
+ * 2: 512 x 16, FP32 only), variant[2] = columns per column-pass block (at
+ * most 16, rounded down to a power of two; 0 = the widest that fits).
+ * lc_autotune2d times the
  * candidate shapes on nslices slices of geometry g, installs the fastest for
  * g's image size in the context and returns it; batches created afterwards
  * use it.  lc_set_tuning2d installs a known choice. */
